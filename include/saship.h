@@ -1,0 +1,114 @@
+/*
+ * saship.h -- C ABI of libsaship.so, the MI355X (gfx950) I(q) engine.
+ *
+ * The library evaluates the sasmodels polydispersity-mesh kernel for five
+ * form factors on one GPU.  Callers pass the same buffers the reference's
+ * compiled kernels take:
+ *
+ *   void <model>_{Iq|Iqxy|Imagnetic}(int32 nq, int32 pd_start, int32 pd_stop,
+ *        const ProblemDetails *details, const double *values, const double *q,
+ *        double *result, double cutoff, int32 radius_effective_mode)
+ *   (sasmodels/kernel_iq.c:317-328; ctypes argtypes sasmodels/kerneldll.py:369)
+ *
+ * split into handles: the model (and its quadrature) once, the q vector once
+ * per kernel (sasmodels/kernel.py:19-24, make_kernel(q_vectors)), then one
+ * call per evaluation (sasmodels/kernel.py:198-205, Kernel._call_kernel).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative code on error;
+ *     saship_last_error() then gives a message for the calling thread.
+ *   - Buffers are host memory owned by the caller; device memory is owned by
+ *     the handles.  Calls are synchronous.
+ *   - One HIP stream per kernel/batch handle.  Calls on different handles
+ *     may run from different threads; calls on one handle are serialised.
+ *   - result keeps the reference layout and accumulate semantics
+ *     (kernel_iq.c:392-423, :826-847): for the 1D kernel result[2i] = sum
+ *     w*F^2, result[2i+1] = sum w*F; for 2D kernels result[i] = sum w*F^2;
+ *     then result[nout*nq + 0..3] = sum w, sum w*V_form, sum w*V_shell,
+ *     sum w*R_eff.  pd_start == 0 starts fresh; pd_start > 0 adds to the
+ *     caller's result, so a mesh may be split into chunks.
+ */
+#ifndef SASHIP_H
+#define SASHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* model ids (parameter tables: sasmodels/models/<name>.py) */
+#define SASHIP_SPHERE 0
+#define SASHIP_CYLINDER 1
+#define SASHIP_ELLIPSOID 2
+#define SASHIP_CORE_SHELL_CYLINDER 3
+#define SASHIP_CORE_SHELL_SPHERE 4
+
+/* kernel kinds: the three kernels generated per model (generate.py:1144-1180) */
+#define SASHIP_KIND_IQ 0        /* <model>_Iq: 1D, <F> and <F^2> */
+#define SASHIP_KIND_IQXY 1      /* <model>_Iqxy: 2D */
+#define SASHIP_KIND_IMAGNETIC 2 /* <model>_Imagnetic: 2D with spin cross sections */
+
+/* error codes */
+#define SASHIP_OK 0
+#define SASHIP_EINVAL -1
+#define SASHIP_EHIP -2
+#define SASHIP_ENOMEM -3
+
+typedef struct saship_model saship_model;
+typedef struct saship_kernel saship_kernel;
+typedef struct saship_batch saship_batch;
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *saship_last_error(void);
+
+/* Number of visible HIP devices.  Replaces the platform probe of
+ * sasmodels/core.py:374-450 (parse_dtype) for the "hip" platform. */
+int saship_device_count(int32_t *count);
+
+/* Select the device used by handles created afterwards on this thread.
+ * Replaces the context set-up of sasmodels/kernelcl.py:246-313. */
+int saship_init(int32_t device);
+
+/* Model handle: form factor plus orientation quadrature.  quad_n == 0 is
+ * the reference default, adaptive 20/76/500/5000 points by q*r_max
+ * (models/lib/adaptive.c:11276-11298); quad_n > 0 fixes an n-point
+ * Gauss-Legendre rule (generate.py:291-320, set_integration_size).
+ * Replaces sasmodels/core.py:293-341 (build_model) for platform "hip". */
+int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out);
+int saship_model_release(saship_model *model);
+
+/* Kernel handle: uploads q once.  1D: q[nq]; 2D (is2d != 0): q[2*nq]
+ * interleaved (qx, qy), the layout of sasmodels/kernelpy.py:84-94.
+ * Replaces KernelModel.make_kernel (sasmodels/kernel.py:19-24). */
+int saship_kernel_create(saship_model *model, int32_t nq, const double *q,
+                         int32_t is2d, saship_kernel **out);
+
+/* One evaluation over mesh steps [pd_start, pd_stop).  details is the
+ * CallDetails int32 buffer (4*max_pd + 4 entries, sasmodels/details.py:73-88),
+ * values the packed parameter vector (sasmodels/details.py:225-254).
+ * result has nout*nq + 4 doubles, nout = 2 for SASHIP_KIND_IQ, else 1.
+ * Replaces Kernel._call_kernel (sasmodels/kernel.py:198-205) as the DLL,
+ * OpenCL and CUDA backends implement it (kerneldll.py:449-475,
+ * kernelcl.py:599-654, kernelcuda.py:471-518). */
+int saship_kernel_call(saship_kernel *kernel, int32_t kind,
+                       const int32_t *details, int32_t n_details,
+                       const double *values, int32_t n_values,
+                       double cutoff, int32_t radius_effective_mode,
+                       int32_t pd_start, int32_t pd_stop, double *result);
+
+/* Kernel timing: when enabled, the device time of the evaluation kernel of
+ * every call is summed (HIP events on the handle's stream).  No reference
+ * counterpart; used by the benchmark. */
+int saship_kernel_set_timing(saship_kernel *kernel, int32_t enable);
+int saship_kernel_get_timing(saship_kernel *kernel, double *eval_ms, int32_t *ncalls);
+
+/* Releases the device buffers and stream.  Replaces Kernel.release
+ * (sasmodels/kernel.py:190-196). */
+int saship_kernel_release(saship_kernel *kernel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SASHIP_H */

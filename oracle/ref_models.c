@@ -1,0 +1,110 @@
+/*
+ * ref_models.c -- build glue for oracle/_ref (TEST INFRASTRUCTURE ONLY).
+ *
+ * Compiles the reference's own form-factor sources, unmodified, from where
+ * they lie under the reference tree (include path set by oracle/Makefile):
+ * the portability header, the special-function library and one model file,
+ * in the order the model's `source` list gives (e.g. cylinder.py:145).
+ * FLOAT_SIZE=8 selects the double-precision branches, as the reference's
+ * generator does for double kernels (generate.py:472).  Nothing here
+ * restates reference code; the functions below only forward a parameter
+ * vector to the reference's static functions so that ctypes can call them.
+ *
+ * One shared object is built per (model, quadrature) pair:
+ *   -DREF_MODEL=<1..5>  sphere, cylinder, ellipsoid, core_shell_cylinder,
+ *                       core_shell_sphere
+ *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
+ *                       lib/adaptive.c (the set_integration_size(info, 76) case)
+ * kernel_iq.c is not built: it needs generator-emitted macros.
+ */
+#define FLOAT_SIZE 8
+#include "kernel_header.c"
+
+#if REF_MODEL == 1
+#  include "lib/sas_3j1x_x.c"
+#  include "sphere.c"
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2])
+#  define NPAR_VOLUME(p) form_volume(p[2])
+#  define NPAR_REFF(m, p) radius_effective(m, p[2])
+#elif REF_MODEL == 2 || REF_MODEL == 4
+#  include "lib/polevl.c"
+#  include "lib/sas_J1.c"
+#  ifdef REF_FIXED76
+#    include "lib/gauss76.c"
+#    include "lib/nonadaptive.c"
+#  else
+#    include "lib/adaptive.c"
+#  endif
+#  if REF_MODEL == 2
+#    include "cylinder.c"
+#    define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3])
+#    define NPAR_IQAC(a, c, p) Iqac(a, c, p[0], p[1], p[2], p[3])
+#    define NPAR_VOLUME(p) form_volume(p[2], p[3])
+#    define NPAR_REFF(m, p) radius_effective(m, p[2], p[3])
+#  else
+#    include "core_shell_cylinder.c"
+#    define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4], p[5])
+#    define NPAR_IQAC(a, c, p) Iqac(a, c, p[0], p[1], p[2], p[3], p[4], p[5])
+#    define NPAR_VOLUME(p) form_volume(p[3], p[4], p[5])
+#    define NPAR_REFF(m, p) radius_effective(m, p[3], p[4], p[5])
+#  endif
+#elif REF_MODEL == 3
+#  include "lib/sas_3j1x_x.c"
+#  ifdef REF_FIXED76
+#    include "lib/gauss76.c"
+#    include "lib/nonadaptive.c"
+#  else
+#    include "lib/adaptive.c"
+#  endif
+#  include "ellipsoid.c"
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3])
+#  define NPAR_IQAC(a, c, p) Iqac(a, c, p[0], p[1], p[2], p[3])
+#  define NPAR_VOLUME(p) form_volume(p[2], p[3])
+#  define NPAR_REFF(m, p) radius_effective(m, p[2], p[3])
+#elif REF_MODEL == 5
+#  include "lib/sas_3j1x_x.c"
+#  include "lib/core_shell.c"
+#  include "core_shell_sphere.c"
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4])
+#  define NPAR_VOLUME(p) form_volume(p[0], p[1])
+#  define NPAR_REFF(m, p) radius_effective(m, p[0], p[1])
+#else
+#  error "REF_MODEL must be 1..5"
+#endif
+
+void ref_fq(double q, const double *p, double *f1, double *f2) { NPAR_CALL_FQ(q, f1, f2, p); }
+double ref_form_volume(const double *p) { return NPAR_VOLUME(p); }
+double ref_reff(int mode, const double *p) { return NPAR_REFF(mode, p); }
+#ifdef NPAR_IQAC
+double ref_iqac(double qab, double qc, const double *p) { return NPAR_IQAC(qab, qc, p); }
+#endif
+
+#if REF_MODEL == 1 || REF_MODEL == 3
+double ref_sph_j1c(double x) { return sas_3j1x_x(x); }
+double ref_sinx_x(double x) { return sas_sinx_x(x); }
+#endif
+#if REF_MODEL == 2
+double ref_j1(double x) { return cephes_j1(x); }
+double ref_2j1x_x(double x) { return sas_2J1x_x(x); }
+#  ifdef REF_FIXED76
+int ref_gauss(int n, double *z, double *w)
+{
+    if (n != GAUSS_N) return -1;
+    for (int i = 0; i < GAUSS_N; i++) { z[i] = GAUSS_Z[i]; w[i] = GAUSS_W[i]; }
+    return GAUSS_N;
+}
+#  else
+int ref_gauss(int n, double *z, double *w)
+{
+    constant double *zz, *ww;
+    int got;
+    if (n == 20) got = gauss_weights(1.0, 1, &ww, &zz);
+    else if (n == 76) got = gauss_weights(50.0, 1, &ww, &zz);
+    else if (n == 500) got = gauss_weights(500.0, 1, &ww, &zz);
+    else if (n == 5000) got = gauss_weights(5000.0, 1, &ww, &zz);
+    else return -1;
+    for (int i = 0; i < got; i++) { z[i] = zz[i]; w[i] = ww[i]; }
+    return got;
+}
+#  endif
+#endif

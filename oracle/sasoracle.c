@@ -1,0 +1,717 @@
+/*
+ * sasoracle.c -- FP64 CPU oracle for the sasmodels I(q) evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product
+ * path is compared against.  Only tests/, the smoke check and the CPU-baseline
+ * leg of bench.py may load it.  The shipped library (libsaship.so) never
+ * links, calls or falls back to it.
+ *
+ * What it restates (reference file:line, sasmodels v1.1.0):
+ *   - the polydispersity-mesh integration kernel  sasmodels/kernel_iq.c:317-858
+ *     (mesh walk with restart :642-664/:682-823, weight product :654,
+ *      strict "weight > cutoff" :726, VALID exclusion :721, projection
+ *      weight |cos dtheta| :621-626, view angles at theta_par+2 :612-613,
+ *      accumulator slots result[nout*nq + 0..3] :392-423/:826-847)
+ *   - view/jitter rotation qac_rotation/qac_apply  kernel_iq.c:197-242
+ *   - magnetism set_spin_weights/mag_sld/xs loop  kernel_iq.c:84-175, :750-779
+ *   - Cephes J1 (double)  models/lib/sas_J1.c:107-153, 2J1(x)/x :241-245
+ *   - Horner evaluation  models/lib/polevl.c:52-85
+ *   - 3 j1(x)/x  models/lib/sas_3j1x_x.c:40-57 (double cutoff 0.1)
+ *   - sin(x)/x  kernel_header.c:177
+ *   - quadrature selection  models/lib/adaptive.c:11276-11298 and
+ *     models/lib/nonadaptive.c:9-13; Gauss-Legendre nodes as made by
+ *     gengauss.py:10-46 (numpy leggauss), recomputed here by Newton's method
+ *   - models: sphere.c:1-20, cylinder.c:1-102, ellipsoid.c:1-105,
+ *     core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30 with
+ *     lib/core_shell.c:8-30
+ *
+ * Floating-point expressions keep the reference's operand order so that,
+ * compiled like the reference DLL (gcc -std=c99 -O2, no FMA contraction),
+ * results agree to the last bits; the summation over q runs q-outer with
+ * the per-q accumulation still in mesh order, so OpenMP threads do not
+ * change any result.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* constants with the values the reference's C uses (kernel_header.c:144-169) */
+#define O_PI 3.14159265358979323846
+#define O_PI_4 0.78539816339744830962
+#define O_PI_180 0.017453292519943295
+#define O_4PI_3 4.18879020478639
+#define O_SQRT1_PI 0.56418958354775628
+
+/* ------------------------------------------------------------------ */
+/* Gauss-Legendre nodes: Newton iteration on P_n in long double.       */
+/* ------------------------------------------------------------------ */
+#define GL_MAX_TABLES 16
+static int gl_n[GL_MAX_TABLES];
+static double *gl_z[GL_MAX_TABLES], *gl_w[GL_MAX_TABLES];
+static int gl_count = 0;
+
+static void gl_compute(int n, double *z, double *w)
+{
+    for (int i = 0; i < (n + 1) / 2; i++) {
+        /* root i counted from the top (x near +1) */
+        long double x = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+        long double dp = 0.0L;
+        for (int it = 0; it < 100; it++) {
+            long double p0 = 1.0L, p1 = x;
+            for (int k = 2; k <= n; k++) {
+                long double p2 = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (x * p1 - p0) / (x * x - 1.0L);
+            long double dx = p1 / dp;
+            x -= dx;
+            if (fabsl(dx) < 1e-19L) {
+                /* one more pass for the derivative at the converged root */
+                p0 = 1.0L; p1 = x;
+                for (int k = 2; k <= n; k++) {
+                    long double p2 = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / k;
+                    p0 = p1; p1 = p2;
+                }
+                dp = n * (x * p1 - p0) / (x * x - 1.0L);
+                break;
+            }
+        }
+        long double wt = 2.0L / ((1.0L - x * x) * dp * dp);
+        /* ascending order, as numpy.polynomial.legendre.leggauss returns */
+        z[n - 1 - i] = (double)x;
+        z[i] = (double)(-x);
+        w[n - 1 - i] = (double)wt;
+        w[i] = (double)wt;
+    }
+    if (n % 2) z[n / 2] = 0.0;
+}
+
+/* Returns the table for n points, computing it on first use.
+ * Not thread safe on first use: callers prime it before parallel regions. */
+static int gl_table(int n, const double **z, const double **w)
+{
+    for (int t = 0; t < gl_count; t++) {
+        if (gl_n[t] == n) { *z = gl_z[t]; *w = gl_w[t]; return n; }
+    }
+    if (gl_count >= GL_MAX_TABLES || n < 1) return -1;
+    double *zz = (double *)malloc(sizeof(double) * n);
+    double *ww = (double *)malloc(sizeof(double) * n);
+    gl_compute(n, zz, ww);
+    gl_n[gl_count] = n; gl_z[gl_count] = zz; gl_w[gl_count] = ww; gl_count++;
+    *z = zz; *w = ww;
+    return n;
+}
+
+/* quad_n == 0: adaptive choice by q*r_max (adaptive.c:11290-11297);
+ * quad_n > 0: fixed table (nonadaptive.c:9-13 after set_integration_size). */
+static int quad_select(int quad_n, double qr, const double **z, const double **w)
+{
+    int n;
+    if (quad_n > 0) n = quad_n;
+    else if (qr < 10) n = 20;
+    else if (qr < 100) n = 76;
+    else if (qr < 750) n = 500;
+    else n = 5000;
+    return gl_table(n, z, w);
+}
+
+static void quad_prime(int quad_n)
+{
+    const double *z, *w;
+    if (quad_n > 0) { gl_table(quad_n, &z, &w); return; }
+    gl_table(20, &z, &w); gl_table(76, &z, &w);
+    gl_table(500, &z, &w); gl_table(5000, &z, &w);
+}
+
+/* ------------------------------------------------------------------ */
+/* Special functions                                                   */
+/* ------------------------------------------------------------------ */
+/* c[0]*x^n + c[1]*x^(n-1) + ... + c[n]   (polevl.c:52-66) */
+static double horner(double x, const double *c, int n)
+{
+    double acc = c[0];
+    for (int i = 1; i <= n; i++) acc = acc * x + c[i];
+    return acc;
+}
+
+/* x^n + c[0]*x^(n-1) + ... + c[n-1]   (polevl.c:73-85) */
+static double horner_monic(double x, const double *c, int n)
+{
+    double acc = x + c[0];
+    for (int i = 1; i < n; i++) acc = acc * x + c[i];
+    return acc;
+}
+
+/* Cephes j1.c rational approximations (S. L. Moshier, Cephes 2.8), the
+ * published coefficient sets RP/RQ on |x|<=5 and PP/PQ/QP/QQ beyond. */
+static const double J1_RP[4] = {
+    -8.99971225705559398224E8, 4.52228297998194034323E11,
+    -7.27494245221818276015E13, 3.68295732863852883286E15};
+static const double J1_RQ[8] = {
+    6.20836478118054335476E2, 2.56987256757748830383E5,
+    8.35146791431949253037E7, 2.21511595479792499675E10,
+    4.74914122079991414898E12, 7.84369607876235854894E14,
+    8.95222336184627338078E16, 5.32278620332680085395E18};
+static const double J1_PP[7] = {
+    7.62125616208173112003E-4, 7.31397056940917570436E-2,
+    1.12719608129684925192E0, 5.11207951146807644818E0,
+    8.42404590141772420927E0, 5.21451598682361504063E0,
+    1.00000000000000000254E0};
+static const double J1_PQ[7] = {
+    5.71323128072548699714E-4, 6.88455908754495404082E-2,
+    1.10514232634061696926E0, 5.07386386128601488557E0,
+    8.39985554327604159757E0, 5.20982848682361821619E0,
+    9.99999999999999997461E-1};
+static const double J1_QP[8] = {
+    5.10862594750176621635E-2, 4.98213872951233449420E0,
+    7.58238284132545283818E1, 3.66779609360150777800E2,
+    7.10856304998926107277E2, 5.97489612400613639965E2,
+    2.11688757100572135698E2, 2.52070205858023719784E1};
+static const double J1_QQ[7] = {
+    7.42373277035675149943E1, 1.05644886038262816351E3,
+    4.98641058337653607651E3, 9.56231892404756170795E3,
+    7.99704160447350683650E3, 2.82619278517639096600E3,
+    3.36093607810698293419E2};
+
+/* J1(x), odd extension to x<0 (sas_J1.c:107-153) */
+static double o_j1(double x)
+{
+    const double Z1 = 1.46819706421238932572E1;  /* first zero squared */
+    const double Z2 = 4.92184563216946036703E1;  /* second zero squared */
+    const double ax = x < 0 ? -x : x;
+    const double sgn = x < 0 ? -1.0 : 1.0;
+    if (ax <= 5.0) {
+        const double z = ax * ax;
+        double r = horner(z, J1_RP, 3) / horner_monic(z, J1_RQ, 8);
+        r = r * ax * (z - Z1) * (z - Z2);
+        return sgn * r;
+    }
+    const double u = 5.0 / ax;
+    const double z = u * u;
+    double p = horner(z, J1_PP, 6) / horner(z, J1_PQ, 6);
+    const double qq = horner(z, J1_QP, 7) / horner_monic(z, J1_QQ, 7);
+    const double s = sin(ax), c = cos(ax);
+    /* p*cos(x-3pi/4) - u*q*sin(x-3pi/4) expanded in sin(x), cos(x) */
+    p = p * (s - c) + u * qq * (s + c);
+    return sgn * p * O_SQRT1_PI / sqrt(ax);
+}
+
+static double o_2j1x_x(double x) { return (x != 0.0) ? 2.0 * o_j1(x) / x : 1.0; }
+
+static double o_sinx_x(double x) { return x == 0 ? 1.0 : sin(x) / x; }
+
+/* 3 j1(x)/x: 4-term Taylor series below 0.1, closed form above */
+static double o_sph_j1c(double x)
+{
+    if (fabs(x) < 0.1) {
+        const double x2 = x * x;
+        return (1.0 + x2 * (-3. / 30. + x2 * (3. / 840. + x2 * (-3. / 45360.))));
+    }
+    const double s = sin(x), c = cos(x);
+    return 3.0 * (s / x - c) / (x * x);
+}
+
+static double sq(double x) { return x * x; }
+static double cb(double x) { return x * x * x; }
+
+/* ------------------------------------------------------------------ */
+/* Models.  p[] is the kernel parameter vector (no scale/background).  */
+/* ------------------------------------------------------------------ */
+
+/* ---- sphere (sphere.c:1-20): p = sld, sld_solvent, radius ---- */
+static double sph_vol(const double *p) { return O_4PI_3 * cb(p[2]); }
+static double sph_reff(int mode, const double *p) { (void)mode; return p[2]; }
+static void sph_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    (void)quad_n;
+    const double bes = o_sph_j1c(q * p[2]);
+    const double contrast = (p[0] - p[1]);
+    const double form = contrast * sph_vol(p) * bes;
+    *f1 = 1.0e-2 * form;
+    *f2 = 1.0e-4 * form * form;
+}
+
+/* ---- cylinder (cylinder.c:1-102): p = sld, sld_solvent, radius, length, theta, phi ---- */
+static double cyl_volume(double r, double l) { return O_PI * r * r * l; }
+static double cyl_vol(const double *p) { return cyl_volume(p[2], p[3]); }
+static int cyl_valid(const double *p) { return p[2] >= 0.0 && p[3] >= 0.0; }
+static double cyl_reff(int mode, const double *p)
+{
+    const double r = p[2], l = p[3];
+    switch (mode) {
+    default:
+    case 1: return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (O_PI * r + l)));
+    case 2: return cbrt(O_PI * r * r * l / O_4PI_3);
+    case 3: return r;
+    case 4: return 0.5 * l;
+    case 5: return (r < 0.5 * l ? r : 0.5 * l);
+    case 6: return (r > 0.5 * l ? r : 0.5 * l);
+    case 7: return sqrt(r * r + 0.25 * l * l);
+    }
+}
+static double cyl_amp(double qab, double qc, double r, double l)
+{
+    return o_2j1x_x(qab * r) * o_sinx_x(qc * 0.5 * l);
+}
+static void cyl_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    const double r = p[2], l = p[3];
+    const double *z, *w;
+    const int n = quad_select(quad_n, q * fmax(r, 0.5 * l), &z, &w);
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double th = z[i] * O_PI_4 + O_PI_4;
+        const double s = sin(th), c = cos(th);
+        const double form = cyl_amp(q * s, q * c, r, l);
+        t1 += w[i] * form * s;
+        t2 += w[i] * form * form * s;
+    }
+    t1 *= O_PI_4;
+    t2 *= O_PI_4;
+    const double sv = (p[0] - p[1]) * cyl_volume(r, l);
+    *f1 = 1e-2 * sv * t1;
+    *f2 = 1e-4 * sv * sv * t2;
+}
+static double cyl_iqac(double qab, double qc, const double *p)
+{
+    const double form = cyl_amp(qab, qc, p[2], p[3]);
+    const double sv = (p[0] - p[1]) * cyl_volume(p[2], p[3]);
+    return 1.0e-4 * sq(sv * form);
+}
+
+/* ---- ellipsoid (ellipsoid.c:1-105): p = sld, sld_solvent, rp, re, theta, phi ---- */
+static double ell_volume(double rp, double re) { return O_4PI_3 * rp * re * re; }
+static double ell_vol(const double *p) { return ell_volume(p[2], p[3]); }
+static double ell_reff(int mode, const double *p)
+{
+    const double rp = p[2], re = p[3];
+    switch (mode) {
+    default:
+    case 1: {
+        if (rp == re) return rp;
+        if (rp * re == 0.) return 0.;
+        /* Isihara (1950) eq. 26 mean curvature radius */
+        const double ratio = (rp < re ? rp / re : re / rp);
+        const double e1 = sqrt(1.0 - ratio * ratio);
+        const double b1 = 1.0 + asin(e1) / (e1 * ratio);
+        const double bL = (1.0 + e1) / (1.0 - e1);
+        const double b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
+        const double delta = 0.75 * b1 * b2;
+        const double ddd = 2.0 * (delta + 1.0) * rp * re * re;
+        return 0.5 * cbrt(ddd);
+    }
+    case 2: return cbrt(rp * re * re);
+    case 3: return (rp < re ? rp : re);
+    case 4: return (rp > re ? rp : re);
+    }
+}
+static void ell_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    const double rp = p[2], re = p[3];
+    const double vm1 = sq(rp / re) - 1.0;
+    const double *z, *w;
+    const int n = quad_select(quad_n, q * fmax(rp, re), &z, &w);
+    double t2 = 0.0, t1 = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double u = z[i] * 0.5 + 0.5;
+        const double r = re * sqrt(1.0 + u * u * vm1);
+        const double f = o_sph_j1c(q * r);
+        t2 += w[i] * f * f;
+        t1 += w[i] * f;
+    }
+    t1 *= 0.5;
+    t2 *= 0.5;
+    const double sv = (p[0] - p[1]) * ell_volume(rp, re);
+    *f1 = 1e-2 * sv * t1;
+    *f2 = 1e-4 * sv * sv * t2;
+}
+static double ell_iqac(double qab, double qc, const double *p)
+{
+    const double rp = p[2], re = p[3];
+    const double qr = sqrt(sq(re * qab) + sq(rp * qc));
+    const double f = o_sph_j1c(qr);
+    const double sv = (p[0] - p[1]) * ell_volume(rp, re);
+    return 1.0e-4 * sq(f * sv);
+}
+
+/* ---- core_shell_cylinder (core_shell_cylinder.c:1-123):
+ *      p = sld_core, sld_shell, sld_solvent, radius, thickness, length, theta, phi ---- */
+static double csc_volume(double r, double t, double l) { return O_PI * sq(r + t) * (l + 2.0 * t); }
+static double csc_vol(const double *p) { return csc_volume(p[3], p[4], p[5]); }
+static double csc_reff(int mode, const double *p)
+{
+    const double r = p[3], t = p[4], l = p[5];
+    switch (mode) {
+    default:
+    case 1: {
+        const double rt = r + t, lt = l + 2.0 * t;
+        return 0.5 * cbrt(0.75 * rt * (2.0 * rt * lt + (rt + lt) * (O_PI * rt + lt)));
+    }
+    case 2: return cbrt(csc_volume(r, t, l) / O_4PI_3);
+    case 3: return r + t;
+    case 4: return 0.5 * l + t;
+    case 5: return (r < 0.5 * l ? r + t : 0.5 * l + t);
+    case 6: return (r > 0.5 * l ? r + t : 0.5 * l + t);
+    case 7: {
+        const double ro = r + t, lo = l + 2.0 * t;
+        return sqrt(ro * ro + 0.25 * lo * lo);
+    }
+    }
+}
+static double csc_part(double vd, double besarg, double siarg)
+{
+    return vd * o_sinx_x(siarg) * o_2j1x_x(besarg);
+}
+static double csc_amp(double qab, double qc, const double *p)
+{
+    const double r = p[3], t = p[4], l = p[5];
+    const double core_r = r;
+    const double core_h = 0.5 * l;
+    const double core_vd = csc_volume(r, 0, l) * (p[0] - p[1]);
+    const double shell_r = (r + t);
+    const double shell_h = (0.5 * l + t);
+    const double shell_vd = csc_volume(r, t, l) * (p[1] - p[2]);
+    return csc_part(core_vd, core_r * qab, core_h * qc)
+         + csc_part(shell_vd, shell_r * qab, shell_h * qc);
+}
+static void csc_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    const double r = p[3], t = p[4], l = p[5];
+    const double *z, *w;
+    const int n = quad_select(quad_n, q * fmax(r + t, 0.5 * l + t), &z, &w);
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double th = z[i] * O_PI_4 + O_PI_4;
+        const double s = sin(th), c = cos(th);
+        const double fq = csc_amp(q * s, q * c, p);
+        t1 += w[i] * fq * s;
+        t2 += w[i] * fq * fq * s;
+    }
+    *f1 = 1.0e-2 * t1 * O_PI_4;
+    *f2 = 1.0e-4 * t2 * O_PI_4;
+}
+static double csc_iqac(double qab, double qc, const double *p)
+{
+    const double fq = csc_amp(qab, qc, p);
+    return 1.0e-4 * fq * fq;
+}
+
+/* ---- core_shell_sphere (core_shell_sphere.c:1-30, lib/core_shell.c:8-30):
+ *      p = radius, thickness, sld_core, sld_shell, sld_solvent ---- */
+static double css_vol(const double *p) { return O_4PI_3 * cb(p[0] + p[1]); }
+static double css_reff(int mode, const double *p)
+{
+    switch (mode) {
+    default:
+    case 1: return p[0] + p[1];
+    case 2: return p[0];
+    }
+}
+static void css_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    (void)quad_n;
+    const double r = p[0], t = p[1];
+    const double core_qr = q * r;
+    const double core_contrast = p[2] - p[3];
+    const double core_bes = o_sph_j1c(core_qr);
+    const double core_volume = O_4PI_3 * cb(r);
+    double f = core_volume * core_bes * core_contrast;
+    const double shell_qr = q * (r + t);
+    const double shell_contrast = p[3] - p[4];
+    const double shell_bes = o_sph_j1c(shell_qr);
+    const double shell_volume = O_4PI_3 * cb(r + t);
+    f += shell_volume * shell_bes * shell_contrast;
+    *f1 = 1.0e-2 * f;
+    *f2 = 1.0e-4 * f * f;
+}
+
+/* ------------------------------------------------------------------ */
+/* Model table                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int npars, nmag, max_pd, oriented;
+    int sld_idx[3];
+    int (*valid)(const double *p);
+    double (*vol)(const double *p);
+    double (*reff)(int mode, const double *p);
+    void (*fq)(double q, const double *p, int quad_n, double *f1, double *f2);
+    double (*iqac)(double qab, double qc, const double *p);
+} OModel;
+
+static int always_valid(const double *p) { (void)p; return 1; }
+
+/* model ids: 0 sphere, 1 cylinder, 2 ellipsoid, 3 core_shell_cylinder,
+ * 4 core_shell_sphere (the same ids libsaship uses) */
+static const OModel MODELS[5] = {
+    {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0},
+    {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac},
+    {6, 2, 4, 1, {0, 1, -1}, always_valid, ell_vol, ell_reff, ell_fq, ell_iqac},
+    {8, 3, 5, 1, {0, 1, 2}, always_valid, csc_vol, csc_reff, csc_fq, csc_iqac},
+    {5, 3, 2, 0, {2, 3, 4}, always_valid, css_vol, css_reff, css_fq, 0},
+};
+
+/* ------------------------------------------------------------------ */
+/* Rotation and magnetism helpers                                      */
+/* ------------------------------------------------------------------ */
+/* Third row of (jitter^-1 * view^-1); maps (qx, qy) to qc (kernel_iq.c:197-230) */
+static void view_jitter_row(double theta, double phi, double dtheta, double dphi,
+                            double *r31, double *r32)
+{
+    const double st = sin(theta * O_PI_180), ct = cos(theta * O_PI_180);
+    const double sp = sin(phi * O_PI_180), cp = cos(phi * O_PI_180);
+    const double v11 = cp * ct, v12 = sp * ct;
+    const double v21 = -sp, v22 = cp;
+    const double v31 = st * cp, v32 = sp * st;
+    const double sdt = sin(dtheta * O_PI_180), cdt = cos(dtheta * O_PI_180);
+    const double sdp = sin(dphi * O_PI_180), cdp = cos(dphi * O_PI_180);
+    const double j31 = sdt, j32 = -sdp * cdt, j33 = cdp * cdt;
+    *r31 = j31 * v11 + j32 * v21 + j33 * v31;
+    *r32 = j31 * v12 + j32 * v22 + j33 * v32;
+}
+
+/* spin cross-section weights dd, du.re, ud.re, uu, du.im, ud.im (kernel_iq.c:84-124) */
+static void spin_weights(double in_spin, double out_spin, double wt[6])
+{
+    in_spin = in_spin < 0.0 ? 0.0 : in_spin > 1.0 ? 1.0 : in_spin;
+    out_spin = out_spin < 0.0 ? 0.0 : out_spin > 1.0 ? 1.0 : out_spin;
+    const double norm = out_spin < 0.5 ? 1 - out_spin : out_spin;
+    wt[0] = (1.0 - in_spin) * (1.0 - out_spin) / norm;
+    wt[1] = (1.0 - in_spin) * out_spin / norm;
+    wt[2] = in_spin * (1.0 - out_spin) / norm;
+    wt[3] = in_spin * out_spin / norm;
+    wt[4] = wt[1];
+    wt[5] = wt[2];
+}
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* effective sld for cross section xs (kernel_iq.c:127-175) */
+static double magnetic_sld(int xs, double qx, double qy, double cmt, double smt,
+                           double cmp, double smp, double sld, double mx, double my, double mz)
+{
+    const double qn = sqrt(qx * qx + qy * qy);
+    const double qhat[3] = {qx / qn, qy / qn, 0};
+    const double m[3] = {mx, my, mz};
+    const double pol[3] = {smt * cmp, smt * smp, cmt};
+    const double py[3] = {-smp, cmp, 0};
+    const double pz[3] = {-cmt * cmp, -cmt * smp, smt};
+    const double k = dot3(m, qhat) / dot3(qhat, qhat);
+    const double mp[3] = {m[0] - k * qhat[0], m[1] - k * qhat[1], m[2] - k * qhat[2]};
+    switch (xs) {
+    case 0: return sld - dot3(pol, mp);
+    case 1: case 2: return dot3(py, mp);
+    case 3: return sld + dot3(pol, mp);
+    case 4: return -dot3(pz, mp);
+    default: return dot3(pz, mp);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Kernel                                                              */
+/* ------------------------------------------------------------------ */
+static int n_threads = 1;
+
+void sasoracle_set_threads(int n) { n_threads = n > 0 ? n : 1; }
+
+/*
+ * Reference-shaped kernel call (kernel_iq.c:317-328, argtypes kerneldll.py:369)
+ * with the model, the kernel kind (0: <model>_Iq, 1: _Iqxy, 2: _Imagnetic) and
+ * the quadrature (0 adaptive, else fixed Gauss-n) named explicitly.
+ * details: int32[4*max_pd+4]; values: f64; q: nq (1D) or 2*nq interleaved (2D);
+ * result: nout*nq+4 with nout = 2 for the 1D kernel (all five models have Fq).
+ * Returns 0, or -1 for a bad model/kind, -2 for a bad quadrature size.
+ */
+int sasoracle_kernel(int model_id, int kind, int quad_n,
+                     int32_t nq, int32_t pd_start, int32_t pd_stop,
+                     const int32_t *details, const double *values,
+                     const double *q, double *result,
+                     double cutoff, int32_t reff_mode)
+{
+    if (model_id < 0 || model_id > 4 || kind < 0 || kind > 2) return -1;
+    if (quad_n < 0) return -2;
+    const OModel *m = &MODELS[model_id];
+    const int np = m->npars, maxpd = m->max_pd;
+    const int32_t *pd_par = details, *pd_len = details + maxpd;
+    const int32_t *pd_off = details + 2 * maxpd, *pd_stride = details + 3 * maxpd;
+    const int32_t num_weights = details[4 * maxpd + 1];
+    const int32_t theta_par = details[4 * maxpd + 3];
+    const double *pd_value = values + (2 + np + (m->nmag ? 4 + 3 * m->nmag : 0));
+    const double *pd_weight = pd_value + num_weights;
+    const int nout = (kind == 0) ? 2 : 1;
+    const int oriented2d = (kind != 0) && m->oriented;
+    quad_prime(quad_n);
+
+    double acc[4];
+    for (int k = 0; k < 4; k++) acc[k] = pd_start == 0 ? 0.0 : result[nout * nq + k];
+    if (pd_start == 0) memset(result, 0, sizeof(double) * nout * nq);
+
+    double base[8];
+    for (int i = 0; i < np; i++) base[i] = values[2 + i];
+    double view_theta = 0, view_phi = 0;
+    if (oriented2d) {
+        view_theta = values[theta_par + 2];
+        view_phi = values[theta_par + 3];
+        base[theta_par] = 0.;
+        base[theta_par + 1] = 0.;
+    }
+
+    double xsw[6] = {0, 0, 0, 0, 0, 0}, smt = 0, cmt = 1, smp = 0, cmp = 1;
+    if (kind == 2) {
+        spin_weights(values[np + 2], values[np + 3], xsw);
+        smt = sin(values[np + 4] * O_PI_180); cmt = cos(values[np + 4] * O_PI_180);
+        smp = sin(values[np + 5] * O_PI_180); cmp = cos(values[np + 5] * O_PI_180);
+    }
+
+    /* Pass 1: walk the mesh in order, keep the accepted points. */
+    const int npts = pd_stop > pd_start ? pd_stop - pd_start : 0;
+    double *pts = (double *)malloc(sizeof(double) * (size_t)(npts ? npts : 1) * (np + 3));
+    int nacc = 0;
+    for (int step = pd_start; step < pd_stop; step++) {
+        double p[8];
+        memcpy(p, base, sizeof(double) * np);
+        double w0 = 1.0;
+        for (int k = maxpd - 1; k >= 0; k--) {
+            const int idx = (step / pd_stride[k]) % pd_len[k];
+            p[pd_par[k]] = pd_value[pd_off[k] + idx];
+            w0 = pd_weight[pd_off[k] + idx] * w0;
+        }
+        if (!m->valid(p)) continue;
+        double weight = w0;
+        double r31 = 0, r32 = 0;
+        if (oriented2d) weight = fabs(cos(p[theta_par] * O_PI_180)) * w0;
+        if (!(weight > cutoff)) continue;
+        const double form = m->vol(p);
+        acc[0] += weight;
+        acc[1] += weight * form;
+        acc[2] += weight * form;
+        if (reff_mode != 0) acc[3] += weight * m->reff(reff_mode, p);
+        if (oriented2d)
+            view_jitter_row(view_theta, view_phi, p[theta_par], p[theta_par + 1], &r31, &r32);
+        double *rec = pts + (size_t)nacc * (np + 3);
+        memcpy(rec, p, sizeof(double) * np);
+        rec[np] = weight;
+        rec[np + 1] = r31;
+        rec[np + 2] = r32;
+        nacc++;
+    }
+
+    /* Pass 2: every q accumulates the accepted points in mesh order. */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads)
+#endif
+    for (int iq = 0; iq < nq; iq++) {
+        for (int a = 0; a < nacc; a++) {
+            const double *rec = pts + (size_t)a * (np + 3);
+            double p[8];
+            memcpy(p, rec, sizeof(double) * np);
+            const double weight = rec[np];
+            if (kind == 0) {
+                double f1, f2;
+                m->fq(q[iq], p, quad_n, &f1, &f2);
+                result[2 * iq + 0] += weight * f2;
+                result[2 * iq + 1] += weight * f1;
+                continue;
+            }
+            const double qx = q[2 * iq], qy = q[2 * iq + 1];
+            double qab = 0, qc = 0;
+            if (oriented2d) {
+                const double dqc = rec[np + 1] * qx + rec[np + 2] * qy;
+                const double dqab_sq = -dqc * dqc + qx * qx + qy * qy;
+                qab = dqab_sq > 0.0 ? sqrt(dqab_sq) : 0.0;
+                qc = dqc;
+            }
+            double f2 = 0.0;
+            if (kind == 1) {
+                if (oriented2d) f2 = m->iqac(qab, qc, p);
+                else { double f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &f2); }
+            } else {
+                const double qsq = qx * qx + qy * qy;
+                if (qsq > 1.e-16) {
+                    for (int xs = 0; xs < 6; xs++) {
+                        if (!(xsw[xs] > 1.e-8)) continue;
+                        for (int sk = 0; sk < m->nmag; sk++) {
+                            const int mi = np + 6 + 3 * sk;
+                            const int si = m->sld_idx[sk];
+                            p[si] = magnetic_sld(xs, qx, qy, cmt, smt, cmp, smp, values[si + 2],
+                                                 values[mi], values[mi + 1], values[mi + 2]);
+                        }
+                        double fx;
+                        if (oriented2d) fx = m->iqac(qab, qc, p);
+                        else { double f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &fx); }
+                        f2 += xsw[xs] * fx;
+                    }
+                }
+            }
+            result[iq] += weight * f2;
+        }
+    }
+    free(pts);
+    for (int k = 0; k < 4; k++) result[nout * nq + k] = acc[k];
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Per-function entry points used by the tests                         */
+/* ------------------------------------------------------------------ */
+double sasoracle_j1(double x) { return o_j1(x); }
+double sasoracle_2j1x_x(double x) { return o_2j1x_x(x); }
+double sasoracle_sph_j1c(double x) { return o_sph_j1c(x); }
+double sasoracle_sinx_x(double x) { return o_sinx_x(x); }
+
+int sasoracle_gauss(int n, double *z, double *w)
+{
+    const double *zz, *ww;
+    if (gl_table(n, &zz, &ww) != n) return -1;
+    memcpy(z, zz, sizeof(double) * n);
+    memcpy(w, ww, sizeof(double) * n);
+    return n;
+}
+
+int sasoracle_fq(int model_id, int quad_n, double q, const double *p, double *f1, double *f2)
+{
+    if (model_id < 0 || model_id > 4) return -1;
+    quad_prime(quad_n);
+    MODELS[model_id].fq(q, p, quad_n, f1, f2);
+    return 0;
+}
+
+double sasoracle_iqac(int model_id, double qab, double qc, const double *p)
+{
+    if (model_id < 0 || model_id > 4 || !MODELS[model_id].iqac) return NAN;
+    return MODELS[model_id].iqac(qab, qc, p);
+}
+
+double sasoracle_form_volume(int model_id, const double *p)
+{
+    if (model_id < 0 || model_id > 4) return NAN;
+    return MODELS[model_id].vol(p);
+}
+
+double sasoracle_radius_effective(int model_id, int mode, const double *p)
+{
+    if (model_id < 0 || model_id > 4) return NAN;
+    return MODELS[model_id].reff(mode, p);
+}
+
+void sasoracle_view_jitter_row(double theta, double phi, double dtheta, double dphi,
+                               double *r31, double *r32)
+{
+    view_jitter_row(theta, phi, dtheta, dphi, r31, r32);
+}
+
+double sasoracle_magnetic_sld(int xs, double qx, double qy, double mtheta_deg, double mphi_deg,
+                              double sld, double mx, double my, double mz)
+{
+    return magnetic_sld(xs, qx, qy, cos(mtheta_deg * O_PI_180), sin(mtheta_deg * O_PI_180),
+                        cos(mphi_deg * O_PI_180), sin(mphi_deg * O_PI_180), sld, mx, my, mz);
+}
+
+void sasoracle_spin_weights(double in_spin, double out_spin, double *wt)
+{
+    spin_weights(in_spin, out_spin, wt);
+}

@@ -1,0 +1,374 @@
+// models.h -- the five form factors as device functors.
+//
+// Each model supplies, from its FP64 kernel-parameter vector p[]
+// (modelinfo order, no scale/background):
+//   valid(p), volume(p), reff(mode, p)       FP64, as the pd loop needs them
+//   rec1d(p, w, r)    FP32 record for the 1D <F>,<F^2> kernel, with the mesh
+//                     weight and every per-point constant folded in
+//   eval1d(q, r, quad, f1, f2)   weighted F and F^2 contributions at q
+//   rec2d(p, w, r)    geometry + per-layer volumes + w*1e-4 for 2D kernels
+//   geom2d(qab, qc, q, r, g)     amplitude basis g[k]; F = sum_k con[k]*g[k]
+//   contrasts(sld, con)          con[k] from the SLD slot values (FP64)
+// The amplitude of every model is linear in its SLDs, which lets the
+// magnetic kernel evaluate the geometry once and reuse it for each spin
+// cross section.
+//
+// Reference formulas (sasmodels/models): sphere.c:1-20, cylinder.c:1-102,
+// ellipsoid.c:1-105, core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30
+// with lib/core_shell.c:8-30.  Parameter order: sphere.py:66-72,
+// cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
+// core_shell_sphere.py:78-82.
+#pragma once
+#include "sasmath.h"
+
+namespace sas {
+
+// pi and 4*pi/3 with the values the reference's C uses (kernel_header.c:144-169)
+constexpr double PI_D = 3.14159265358979323846;
+constexpr double PI_4_D = 0.78539816339744830962;
+constexpr double FOUR_PI_3_D = 4.18879020478639;
+
+// Quadrature tables on the device.  For the cylinder family the nodes are
+// theta = z*pi/4 + pi/4 and the tables hold sin, cos and w*sin; for the
+// ellipsoid u = z/2 + 1/2 and the tables hold u^2 and w.  Adaptive
+// selection (adaptive.c:11290-11297) picks one of four tables per lane.
+struct Quad {
+    const float *a;     // sin(theta) | u^2
+    const float *b;     // cos(theta) | unused
+    const float *w;     // w*sin(theta) | w
+    int n;              // fixed size, or 0 for adaptive
+    int off[4];         // adaptive: offsets of the 20/76/500/5000 tables
+};
+
+SAS_HD int adaptive_pick(float qr)
+{
+    return qr < 10.0f ? 0 : qr < 100.0f ? 1 : qr < 750.0f ? 2 : 3;
+}
+
+template <bool FIXED>
+__device__ __forceinline__ void quad_range(const Quad &t, float qr, int &off, int &n)
+{
+    if constexpr (FIXED) {
+        off = 0;
+        n = t.n;
+    } else {
+        const int k = adaptive_pick(qr);
+        off = k == 0 ? t.off[0] : k == 1 ? t.off[1] : k == 2 ? t.off[2] : t.off[3];
+        n = k == 0 ? 20 : k == 1 ? 76 : k == 2 ? 500 : 5000;
+    }
+}
+
+// ---------------------------------------------------------------- sphere
+struct Sphere {
+    static constexpr int ID = 0, NPARS = 3, NMAG = 2, MAX_PD = 1, NCON = 1;
+    static constexpr bool ORIENTED = false;
+    static constexpr int REC1D = 4, REC2D = 4;
+    static constexpr int A2D = 2;   // slot of w*1e-4 in the 2D record
+    SAS_HD static int sld_param(int k) { return k; }            // sld, sld_solvent
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[2]); }
+    __device__ static double reff(int, const double *p) { return p[2]; }
+    __device__ static double qr_scale(const double *p) { return p[2]; }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double s = (p[0] - p[1]) * volume(p);
+        r[0] = (float)p[2];
+        r[1] = (float)(w * 1e-2 * s);
+        r[2] = (float)(w * 1e-4 * s * s);
+        r[3] = 0.0f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &, float &f1, float &f2)
+    {
+        const float f = sph_j1c_f(q * r[0]);
+        f1 = r[1] * f;
+        f2 = r[2] * f * f;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        r[0] = (float)p[2];
+        r[1] = (float)volume(p);
+        r[2] = (float)(w * 1e-4);
+        r[3] = 0.0f;
+    }
+    __device__ static void geom2d(float, float, float q, const float *r, float *g)
+    {
+        g[0] = r[1] * sph_j1c_f(q * r[0]);
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- cylinder
+struct Cylinder {
+    static constexpr int ID = 1, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
+    static constexpr bool ORIENTED = true;
+    static constexpr int REC1D = 4, REC2D = 4;
+    static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
+    SAS_HD static int sld_param(int k) { return k; }
+    __device__ static bool valid(const double *p) { return p[2] >= 0.0 && p[3] >= 0.0; }
+    __device__ static double volume(const double *p) { return PI_D * p[2] * p[2] * p[3]; }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double r = p[2], l = p[3];
+        switch (mode) {
+        default:
+        case 1: return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (PI_D * r + l)));
+        case 2: return cbrt(PI_D * r * r * l / FOUR_PI_3_D);
+        case 3: return r;
+        case 4: return 0.5 * l;
+        case 5: return r < 0.5 * l ? r : 0.5 * l;
+        case 6: return r > 0.5 * l ? r : 0.5 * l;
+        case 7: return sqrt(r * r + 0.25 * l * l);
+        }
+    }
+    __device__ static double qr_scale(const double *p) { return fmax(p[2], 0.5 * p[3]); }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double s = (p[0] - p[1]) * volume(p);
+        r[0] = (float)p[2];
+        r[1] = (float)(0.5 * p[3]);
+        r[2] = (float)(w * 1e-2 * s * PI_4_D);
+        r[3] = (float)(w * 1e-4 * s * s * PI_4_D);
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int off, n;
+        quad_range<FIXED>(t, q * fmaxf(r[0], r[1]), off, n);
+        const float qr = q * r[0], qh = q * r[1];
+        float a1 = 0.0f, a2 = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float f = j1c_f(qr * t.a[off + i]) * sinc_prod(qh, t.b[off + i]);
+            const float wf = t.w[off + i] * f;
+            a1 += wf;
+            a2 = fmaf(wf, f, a2);
+        }
+        f1 = r[2] * a1;
+        f2 = r[3] * a2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        r[0] = (float)p[2];
+        r[1] = (float)(0.5 * p[3]);
+        r[2] = (float)volume(p);
+        r[3] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(float qab, float qc, float, const float *r, float *g)
+    {
+        g[0] = r[2] * j1c_f(qab * r[0]) * sinc_prod(qc, r[1]);
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- ellipsoid
+struct Ellipsoid {
+    static constexpr int ID = 2, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
+    static constexpr bool ORIENTED = true;
+    static constexpr int REC1D = 4, REC2D = 4;
+    static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
+    SAS_HD static int sld_param(int k) { return k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * p[2] * p[3] * p[3]; }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double rp = p[2], re = p[3];
+        switch (mode) {
+        default:
+        case 1: {
+            if (rp == re) return rp;
+            if (rp * re == 0.) return 0.;
+            // mean curvature radius, Isihara, J. Chem. Phys. 18 (1950) 1446, eq. 26
+            const double ratio = rp < re ? rp / re : re / rp;
+            const double e1 = sqrt(1.0 - ratio * ratio);
+            const double b1 = 1.0 + asin(e1) / (e1 * ratio);
+            const double bL = (1.0 + e1) / (1.0 - e1);
+            const double b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
+            const double delta = 0.75 * b1 * b2;
+            return 0.5 * cbrt(2.0 * (delta + 1.0) * rp * re * re);
+        }
+        case 2: return cbrt(rp * re * re);
+        case 3: return rp < re ? rp : re;
+        case 4: return rp > re ? rp : re;
+        }
+    }
+    __device__ static double qr_scale(const double *p) { return fmax(p[2], p[3]); }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double s = (p[0] - p[1]) * volume(p);
+        const double ratio = p[2] / p[3];
+        r[0] = (float)p[3];
+        r[1] = (float)(ratio * ratio - 1.0);
+        r[2] = (float)(w * 1e-2 * s * 0.5);
+        r[3] = (float)(w * 1e-4 * s * s * 0.5);
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int off, n;
+        // r_max = max(rp, re) = re * max(1, sqrt(1 + v))
+        const float rmax = r[0] * sqrt_f(fmaxf(1.0f, 1.0f + r[1]));
+        quad_range<FIXED>(t, q * rmax, off, n);
+        const float qre = q * r[0];
+        float a1 = 0.0f, a2 = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float x = qre * sqrt_f(fmaf(t.a[off + i], r[1], 1.0f));
+            const float f = sph_j1c_f(x);
+            const float wf = t.w[off + i] * f;
+            a1 += wf;
+            a2 = fmaf(wf, f, a2);
+        }
+        f1 = r[2] * a1;
+        f2 = r[3] * a2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        r[0] = (float)(p[3] * p[3]);
+        r[1] = (float)(p[2] * p[2]);
+        r[2] = (float)volume(p);
+        r[3] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(float qab, float qc, float, const float *r, float *g)
+    {
+        const float qr = sqrt_f(fmaf(r[0], qab * qab, r[1] * (qc * qc)));
+        g[0] = r[2] * sph_j1c_f(qr);
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- core_shell_cylinder
+struct CoreShellCylinder {
+    static constexpr int ID = 3, NPARS = 8, NMAG = 3, MAX_PD = 5, NCON = 2;
+    static constexpr bool ORIENTED = true;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 6;   // slot of w*1e-4 in the 2D record
+    SAS_HD static int sld_param(int k) { return k; }            // core, shell, solvent
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double vol3(double r, double t, double l) { return PI_D * (r + t) * (r + t) * (l + 2.0 * t); }
+    __device__ static double volume(const double *p) { return vol3(p[3], p[4], p[5]); }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double r = p[3], t = p[4], l = p[5];
+        switch (mode) {
+        default:
+        case 1: {
+            const double rt = r + t, lt = l + 2.0 * t;
+            return 0.5 * cbrt(0.75 * rt * (2.0 * rt * lt + (rt + lt) * (PI_D * rt + lt)));
+        }
+        case 2: return cbrt(vol3(r, t, l) / FOUR_PI_3_D);
+        case 3: return r + t;
+        case 4: return 0.5 * l + t;
+        case 5: return r < 0.5 * l ? r + t : 0.5 * l + t;
+        case 6: return r > 0.5 * l ? r + t : 0.5 * l + t;
+        case 7: {
+            const double ro = r + t, lo = l + 2.0 * t;
+            return sqrt(ro * ro + 0.25 * lo * lo);
+        }
+        }
+    }
+    __device__ static double qr_scale(const double *p) { return fmax(p[3] + p[4], 0.5 * p[5] + p[4]); }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double rr = p[3], t = p[4], l = p[5];
+        r[0] = (float)rr;
+        r[1] = (float)(0.5 * l);
+        r[2] = (float)(rr + t);
+        r[3] = (float)(0.5 * l + t);
+        r[4] = (float)(vol3(rr, 0.0, l) * (p[0] - p[1]));
+        r[5] = (float)(vol3(rr, t, l) * (p[1] - p[2]));
+        r[6] = (float)(w * 1e-2 * PI_4_D);
+        r[7] = (float)(w * 1e-4 * PI_4_D);
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int off, n;
+        quad_range<FIXED>(t, q * fmaxf(r[2], r[3]), off, n);
+        const float qrc = q * r[0], qhc = q * r[1], qrs = q * r[2], qhs = q * r[3];
+        float a1 = 0.0f, a2 = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float s = t.a[off + i], c = t.b[off + i];
+            const float f = r[4] * sinc_prod(qhc, c) * j1c_f(qrc * s)
+                          + r[5] * sinc_prod(qhs, c) * j1c_f(qrs * s);
+            const float wf = t.w[off + i] * f;
+            a1 += wf;
+            a2 = fmaf(wf, f, a2);
+        }
+        f1 = r[6] * a1;
+        f2 = r[7] * a2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        const double rr = p[3], t = p[4], l = p[5];
+        r[0] = (float)rr;
+        r[1] = (float)(0.5 * l);
+        r[2] = (float)(rr + t);
+        r[3] = (float)(0.5 * l + t);
+        r[4] = (float)vol3(rr, 0.0, l);
+        r[5] = (float)vol3(rr, t, l);
+        r[6] = (float)(w * 1e-4);
+        r[7] = 0.0f;
+    }
+    __device__ static void geom2d(float qab, float qc, float, const float *r, float *g)
+    {
+        g[0] = r[4] * sinc_prod(qc, r[1]) * j1c_f(qab * r[0]);
+        g[1] = r[5] * sinc_prod(qc, r[3]) * j1c_f(qab * r[2]);
+    }
+    SAS_HD static void contrasts(const double *sld, float *con)
+    {
+        con[0] = (float)(sld[0] - sld[1]);
+        con[1] = (float)(sld[1] - sld[2]);
+    }
+};
+
+// ---------------------------------------------------------------- core_shell_sphere
+struct CoreShellSphere {
+    static constexpr int ID = 4, NPARS = 5, NMAG = 3, MAX_PD = 2, NCON = 2;
+    static constexpr bool ORIENTED = false;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 4;   // slot of w*1e-4 in the 2D record
+    SAS_HD static int sld_param(int k) { return 2 + k; }        // core, shell, solvent
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[0] + p[1]); }
+    __device__ static double reff(int mode, const double *p) { return mode == 2 ? p[0] : p[0] + p[1]; }
+    __device__ static double qr_scale(const double *p) { return p[0] + p[1]; }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double rc = p[0], rs = p[0] + p[1];
+        r[0] = (float)rc;
+        r[1] = (float)rs;
+        r[2] = (float)(FOUR_PI_3_D * cube_d(rc) * (p[2] - p[3]));
+        r[3] = (float)(FOUR_PI_3_D * cube_d(rs) * (p[3] - p[4]));
+        r[4] = (float)(w * 1e-2);
+        r[5] = (float)(w * 1e-4);
+        r[6] = r[7] = 0.0f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &, float &f1, float &f2)
+    {
+        const float f = r[2] * sph_j1c_f(q * r[0]) + r[3] * sph_j1c_f(q * r[1]);
+        f1 = r[4] * f;
+        f2 = r[5] * f * f;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        const double rc = p[0], rs = p[0] + p[1];
+        r[0] = (float)rc;
+        r[1] = (float)rs;
+        r[2] = (float)(FOUR_PI_3_D * cube_d(rc));
+        r[3] = (float)(FOUR_PI_3_D * cube_d(rs));
+        r[4] = (float)(w * 1e-4);
+        r[5] = r[6] = r[7] = 0.0f;
+    }
+    __device__ static void geom2d(float, float, float q, const float *r, float *g)
+    {
+        g[0] = r[2] * sph_j1c_f(q * r[0]);
+        g[1] = r[3] * sph_j1c_f(q * r[1]);
+    }
+    SAS_HD static void contrasts(const double *sld, float *con)
+    {
+        con[0] = (float)(sld[0] - sld[1]);
+        con[1] = (float)(sld[1] - sld[2]);
+    }
+};
+
+}  // namespace sas

@@ -1,0 +1,266 @@
+"""
+Parameter tables and model metadata for the models the hip backend runs.
+
+Restates the parts of the reference's parameter handling that the I(q)
+path depends on (sasmodels/modelinfo.py):
+
+* common parameters scale/background (:49-56) and the magnetic parameters
+  appended to the call list (:634-672);
+* the values-vector layout ``[scale, background, kernel pars...,
+  up_frac_i, up_frac_f, up_theta, up_phi, (M0, mtheta, mphi) per sld]``
+  and ``nvalues`` (:454-459);
+* ``polydisperse`` for volume and orientation parameters and
+  ``relative_pd`` for volume parameters (:176-177);
+* ``theta_offset``, ``max_pd``, ``pd_1d``/``pd_2d`` (:470-495).
+
+The five parameter tables are the reference's own (sphere.py:66-72,
+cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
+core_shell_sphere.py:78-82), with the attributes the kernels use
+(``have_Fq``, ``valid``, ``radius_effective_modes``, the orientation mode).
+"""
+
+import numpy as np
+from numpy import inf
+
+#: Maximum number of simultaneously polydisperse parameters (modelinfo.py:38)
+MAX_PD = 5
+NUM_COMMON_PARS = 2
+NUM_MAGFIELD_PARS = 4
+NUM_MAGNETIC_PARS = 3
+DEFAULT_BACKGROUND = 1e-3
+
+COMMON_PARAMETERS = [
+    ("scale", "", 1, (0.0, np.inf), "", "Scale factor or Volume fraction"),
+    ("background", "1/cm", DEFAULT_BACKGROUND, (-np.inf, np.inf), "", "Source background"),
+]
+
+
+class Parameter:
+    """
+    One model parameter: *name*, *units*, *default*, *limits*, *type*
+    ('volume', 'orientation', 'sld', 'magnetic' or '') and *description*.
+    """
+    def __init__(self, name, units="", default=None, limits=(-np.inf, np.inf),
+                 type="", description=""):
+        self.id = name
+        self.name = name
+        self.units = units
+        self.default = default
+        self.limits = tuple(limits)
+        self.type = type
+        self.description = description
+        self.length = 1
+        self.length_control = None
+        self.is_control = False
+        self.polydisperse = type in ("volume", "orientation")
+        self.relative_pd = type == "volume"
+        self.choices = []
+
+    def __repr__(self):
+        return "P<%s>" % self.name
+
+
+def parse_parameter(name, units, default, user_limits, ptype, description):
+    """
+    Build a :class:`Parameter` from a table row, checking limits and type
+    (the subset of modelinfo.py:80-181 the five models exercise).
+    """
+    if ptype not in ("volume", "orientation", "sld", "magnetic", ""):
+        raise ValueError("unexpected type %r for %s" % (ptype, name))
+    low, high = user_limits
+    limits = (float(low), float(high))
+    if default < limits[0] or default > limits[1]:
+        raise ValueError("default value %r not in range for %s" % (default, name))
+    if ptype == "" and (name.startswith("sld") or name.endswith("sld")):
+        ptype = "sld"
+    return Parameter(name, units, float(default), limits, ptype, description)
+
+
+class ParameterTable:
+    """
+    Kernel and call parameter lists for one model (modelinfo.py:390-779).
+    """
+    def __init__(self, parameters, allow_magnetic=True):
+        self.common_parameters = [Parameter(*p) for p in COMMON_PARAMETERS]
+        self.kernel_parameters = parameters
+        self.npars = sum(p.length for p in parameters)
+        self.nmagnetic = (sum(p.length for p in parameters if p.type == "sld")
+                          if allow_magnetic else 0)
+        self.nvalues = NUM_COMMON_PARS + self.npars
+        if self.nmagnetic:
+            self.nvalues += NUM_MAGFIELD_PARS + NUM_MAGNETIC_PARS*self.nmagnetic
+        self.call_parameters = self._get_call_parameters()
+        self.defaults = {p.id: p.default for p in self.call_parameters}
+        self.iq_parameters = [p for p in parameters
+                              if p.type not in ("orientation", "magnetic")]
+        self.orientation_parameters = [p for p in parameters if p.type == "orientation"]
+        self.form_volume_parameters = [p for p in parameters if p.type == "volume"]
+        offset = 0
+        self.theta_offset = -1
+        for p in parameters:
+            if p.name == "theta":
+                self.theta_offset = offset
+                break
+            offset += p.length
+        num_pd = sum(p.length for p in parameters if p.polydisperse)
+        self.max_pd = min(num_pd, MAX_PD)
+        self.has_2d = any(p.type in ("orientation", "magnetic") for p in parameters)
+        self.is_asymmetric = any(p.name == "psi" for p in parameters)
+        self.magnetism_index = [k for k, p in enumerate(self.call_parameters)
+                                if p.id.endswith("_M0")]
+        self.pd_1d = set(p.name for p in self.call_parameters
+                         if p.polydisperse and p.type not in ("orientation", "magnetic"))
+        self.pd_2d = set(p.name for p in self.call_parameters if p.polydisperse)
+
+    def _get_call_parameters(self):
+        full = self.common_parameters[:] + list(self.kernel_parameters)
+        if self.nmagnetic > 0:
+            full.extend([
+                Parameter("up_frac_i", "", 0., [0., 1.], "magnetic",
+                          "fraction of spin up incident"),
+                Parameter("up_frac_f", "", 0., [0., 1.], "magnetic",
+                          "fraction of spin up final"),
+                Parameter("up_theta", "degrees", 90., [-180., 180.], "magnetic",
+                          "polarization axis rotation angle"),
+                Parameter("up_phi", "degrees", 0., [-360., 360.], "magnetic",
+                          "polarization axis inclination angle"),
+            ])
+            for p in [p for p in full if p.type == "sld"]:
+                full.extend([
+                    Parameter(p.id + "_M0", "1e-6/Ang^2", 0., [-np.inf, np.inf],
+                              "magnetic", "magnetic amplitude for " + p.description),
+                    Parameter(p.id + "_mtheta", "degrees", 0., [-180., 180.],
+                              "magnetic", "magnetic latitude for " + p.description),
+                    Parameter(p.id + "_mphi", "degrees", 0., [-360., 360.],
+                              "magnetic", "magnetic longitude for " + p.description),
+                ])
+        return full
+
+    def __getitem__(self, key):
+        for par in self.call_parameters:
+            if par.name == key:
+                return par
+        raise KeyError("unknown parameter %r" % key)
+
+    def __contains__(self, key):
+        return any(par.name == key for par in self.call_parameters)
+
+
+def make_parameter_table(pars, allow_magnetic=True):
+    """Build a :class:`ParameterTable` from 6-element table rows."""
+    processed = []
+    for p in pars:
+        if not isinstance(p, (list, tuple)) or len(p) != 6:
+            raise ValueError("Parameter should be [name, units, default, limits, type, desc], but got %r" % (p,))
+        processed.append(parse_parameter(*p))
+    return ParameterTable(processed, allow_magnetic=allow_magnetic)
+
+
+class ModelInfo:
+    """
+    Model metadata used by the kernels (the fields of modelinfo.py:1033-1228
+    that the I(q) path reads).
+    """
+    id = None
+    name = None
+    title = None
+    parameters = None
+    base = None
+    composition = None
+    compiled = True
+    have_Fq = False
+    valid = ""
+    radius_effective_modes = None
+    structure_factor = False
+    xy_mode = "qa"
+    #: id of the model in the C ABI (include/saship.h)
+    kernel_id = None
+    #: Gauss-Legendre size for the orientation average; 0 is the reference's
+    #: adaptive choice (models/lib/adaptive.c:11276-11298)
+    integration_size = 0
+    #: models whose 1D kernel integrates over orientation
+    has_quadrature = False
+
+    def get_hidden_parameters(self, control):
+        """No hidden parameters for these models."""
+        return set()
+
+
+# name -> (kernel id, parameter rows, extra attributes)
+_MODEL_TABLE = {
+    "sphere": (0, [
+        ["sld", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Layer scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 6, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["radius", "Ang", 50, [0, inf], "volume", "Sphere radius"],
+    ], dict(radius_effective_modes=["radius"], xy_mode="qa", has_quadrature=False)),
+    "cylinder": (1, [
+        ["sld", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Cylinder scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["radius", "Ang", 20, [0, inf], "volume", "Cylinder radius"],
+        ["length", "Ang", 400, [0, inf], "volume", "Cylinder length"],
+        ["theta", "degrees", 60, [-180, 180], "orientation", "cylinder axis to beam angle"],
+        ["phi", "degrees", 60, [-360, 360], "orientation", "rotation about beam"],
+    ], dict(valid="radius >= 0.0 && length >= 0.0",
+            radius_effective_modes=[
+                "excluded volume", "equivalent volume sphere", "radius",
+                "half length", "half min dimension", "half max dimension",
+                "half diagonal"],
+            xy_mode="qac", has_quadrature=True)),
+    "ellipsoid": (2, [
+        ["sld", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Ellipsoid scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["radius_polar", "Ang", 20, [0, inf], "volume", "Polar radius"],
+        ["radius_equatorial", "Ang", 400, [0, inf], "volume", "Equatorial radius"],
+        ["theta", "degrees", 60, [-180, 180], "orientation", "ellipsoid axis to beam angle"],
+        ["phi", "degrees", 60, [-360, 360], "orientation", "rotation about beam"],
+    ], dict(radius_effective_modes=[
+                "average curvature", "equivalent volume sphere", "min radius",
+                "max radius"],
+            xy_mode="qac", has_quadrature=True)),
+    "core_shell_cylinder": (3, [
+        ["sld_core", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Cylinder core scattering length density"],
+        ["sld_shell", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Cylinder shell scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["radius", "Ang", 20, [0, inf], "volume", "Cylinder core radius"],
+        ["thickness", "Ang", 20, [0, inf], "volume", "Cylinder shell thickness"],
+        ["length", "Ang", 400, [0, inf], "volume", "Cylinder length"],
+        ["theta", "degrees", 60, [-180, 180], "orientation", "cylinder axis to beam angle"],
+        ["phi", "degrees", 60, [-360, 360], "orientation", "rotation about beam"],
+    ], dict(radius_effective_modes=[
+                "excluded volume", "equivalent volume sphere", "outer radius",
+                "half outer length", "half min outer dimension",
+                "half max outer dimension", "half outer diagonal"],
+            xy_mode="qac", has_quadrature=True)),
+    "core_shell_sphere": (4, [
+        ["radius", "Ang", 60.0, [0, inf], "volume", "Sphere core radius"],
+        ["thickness", "Ang", 10.0, [0, inf], "volume", "Sphere shell thickness"],
+        ["sld_core", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "core scattering length density"],
+        ["sld_shell", "1e-6/Ang^2", 2.0, [-inf, inf], "sld", "shell scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 3.0, [-inf, inf], "sld", "Solvent scattering length density"],
+    ], dict(radius_effective_modes=["outer radius", "core radius"], xy_mode="qa",
+            has_quadrature=False)),
+}
+
+MODELS = tuple(_MODEL_TABLE)
+
+
+def make_model_info(name):
+    """
+    Return a fresh :class:`ModelInfo` for model *name* (a new object each
+    call, so :func:`generate.set_integration_size` on it affects only it).
+    """
+    if name not in _MODEL_TABLE:
+        raise ValueError("model %r is not available on the hip platform; "
+                         "available models: %s" % (name, ", ".join(MODELS)))
+    kernel_id, rows, extra = _MODEL_TABLE[name]
+    info = ModelInfo()
+    info.id = name
+    info.name = " ".join(w.capitalize() for w in name.split("_"))
+    info.title = info.name + " model"
+    info.kernel_id = kernel_id
+    info.parameters = info.base = make_parameter_table(rows)
+    info.have_Fq = True
+    info.integration_size = 0
+    for key, value in extra.items():
+        setattr(info, key, value)
+    return info

@@ -1,0 +1,64 @@
+"""Shared pieces of the GPU parity tests: run HIP and the oracle on the same buffers."""
+
+import numpy as np
+
+from oracle.oracle import OracleKernel
+from sasmodels_amd import direct_model, generate, modelinfo
+from sasmodels_amd.details import make_kernel_args
+from sasmodels_amd.kernelhip import HipModel
+
+#: relative tolerance of the FP32 GPU path against FP64 (model_test.py:256-260)
+RTOL = 5e-5
+#: mask: compare I where I_fp64 > MASK * max I_fp64
+MASK = 1e-6
+
+_models = {}
+
+
+def hip_model(name, quad=0):
+    key = (name, quad)
+    if key not in _models:
+        info = modelinfo.make_model_info(name)
+        generate.set_integration_size(info, quad)
+        _models[key] = HipModel(info)
+    return _models[key]
+
+
+def buffers(model, q_vectors, pars, quad=0):
+    """(hip kernel, oracle kernel, call_details, values, is_magnetic) for *pars*."""
+    hm = hip_model(model, quad)
+    hk = hm.make_kernel(q_vectors)
+    ok = OracleKernel(hm.info, q_vectors)
+    mesh = direct_model.get_mesh(hm.info, dict(pars), dim=hk.dim)
+    details, values, magnetic = make_kernel_args(hk, mesh)
+    return hk, ok, details, values, magnetic
+
+
+def masked_relerr(actual, target, mask=MASK):
+    """max |a - t|/|t| over points where |t| > mask * max |t|."""
+    target = np.asarray(target)
+    actual = np.asarray(actual)
+    sel = np.abs(target) > mask*np.abs(target).max()
+    if not sel.any():
+        return 0.0
+    return float(np.max(np.abs(actual[sel] - target[sel])/np.abs(target[sel])))
+
+
+def compare_raw(hip_res, ora_res, nq, nout, rtol=RTOL):
+    """
+    Compare raw accumulators: F^2 sums within *rtol* on the masked points,
+    F sums within *rtol* of the RMS amplitude sqrt(sum w * sum w F^2), and
+    the four FP64 weight sums within 1e-12.
+    """
+    f2h, f2o = hip_res[0:nout*nq:nout], ora_res[0:nout*nq:nout]
+    err2 = masked_relerr(f2h, f2o)
+    assert err2 <= rtol, "F^2 rel err %.3g > %.3g" % (err2, rtol)
+    if nout == 2:
+        f1h, f1o = hip_res[1:2*nq:2], ora_res[1:2*nq:2]
+        scale = np.sqrt(np.abs(f2o)*abs(ora_res[2*nq]))
+        sel = f2o > MASK*np.abs(f2o).max()
+        if sel.any():
+            err1 = float(np.max(np.abs(f1h[sel] - f1o[sel])/scale[sel]))
+            assert err1 <= rtol, "F rel err %.3g > %.3g" % (err1, rtol)
+    np.testing.assert_allclose(hip_res[nout*nq:], ora_res[nout*nq:], rtol=1e-12, atol=1e-300)
+    return err2

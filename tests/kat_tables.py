@@ -1,0 +1,140 @@
+"""
+Known-answer tests of the reference, as data.
+
+Each entry is copied from the ``tests`` list of the reference model file
+named beside it (inputs and expected outputs only) or from
+sasmodels/direct_model.py:562-603 (``test_simple_interface``).  Entries have
+the reference's format (sasmodels/model_test.py:1-44):
+
+    (pars, q, I)                       q a float, list, (qx, qy) or list of tuples
+    (pars, q, F, F^2, R_eff, V, V_r)   Fq checks; F None when not given
+
+The reference evaluates them with its default adaptive quadrature, and
+compares to 5 significant digits (model_test.py:443-471) or, for the
+direct_model entries, to rtol 1e-6.
+"""
+
+from math import cos, pi, sin
+
+import numpy as np
+
+# sphere.py:87-121
+SPHERE = [
+    ({}, 0.2, 0.726362),
+    ({"scale": 1., "background": 0., "sld": 6., "sld_solvent": 1., "radius": 120.},
+     [0.01, 0.1, 0.2], [1.34836265e+04, 6.20114062e+00, 1.04733914e-01]),
+    ({"scale": 1., "background": 0., "sld": 6., "sld_solvent": 1., "radius": 120.,
+      "radius_pd": 0.2, "radius_pd_n": 45},
+     [0.01, 0.1, 0.2], [1.74395295e+04, 3.68016987e+00, 2.28843099e-01]),
+    ({"scale": 1., "background": 0., "sld": 6., "sld_solvent": 1., "radius": 120.,
+      "radius_pd": 0.2, "radius_pd_n": 45},
+     0.01, 335839.88055473, 1.41045057e+11, 120.0, 8087664.122641933, 1.0),
+    ({"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45},
+     0.1, 482.93824329, 29763977.79867414, 120.0, 8087664.122641933, 1.0),
+    ({"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45},
+     0.2, 1.23330406, 1850806.1197361, 120.0, 8087664.122641933, 1.0),
+    ({"scale": 1., "background": 0., "sld": 6., "sld_solvent": 1., "radius": 120.},
+     [(0.006, 0.008), (0.06, 0.08), (0.12, 0.16)],
+     [1.34836265e+04, 6.20114062e+00, 1.04733914e-01]),
+    # magnetic (the reference marks these values "not validated")
+    (dict(radius=120, sld_M0=4, sld_mphi=20, sld_mtheta=60,
+          up_frac_i=0.05, up_frac_f=0.1, up_theta=-15, up_phi=10),
+     [(0.0, 0.01), (0.0, 0.1), (0.0, 0.2)],
+     [20247.206006297125, 9.312720770235483, 0.15826993186001856]),
+]
+
+# cylinder.py:171-248 (structure-factor products omitted)
+_cyl_qx, _cyl_qy = 0.2*np.cos(2.5), 0.2*np.sin(2.5)
+_cyl_theta, _cyl_phi = 80.1534480601659, 10.1510817110481
+
+
+def _cylinder_reff_tests(radius=20., length=400.):
+    V = pi*radius**2*length
+    Fsq = (0.042761386790780453 - 0.001)*V
+    r_effs = [
+        0.,
+        0.5*(0.75*radius*(2.0*radius*length + (radius + length)*(pi*radius + length)))**(1./3.),
+        (0.75*radius**2*length)**(1./3.),
+        radius,
+        length/2.,
+        min(radius, length/2.),
+        max(radius, length/2.),
+        np.sqrt(4*radius**2 + length**2)/2.,
+    ]
+    return [({"radius_effective_mode": k}, 0.2, None, Fsq, r_effs[k], V, 1.0)
+            for k in range(8)]
+
+
+CYLINDER = [
+    ({}, 0.2, 0.042761386790780453),
+    ({}, [0.2], [0.042761386790780453]),
+    ({"scale": 1., "background": 0.}, [0.01, 0.05, 0.2],
+     [3.01823887e+02, 5.36661653e+01, 4.17613868e-02]),
+    ({"scale": 1., "background": 0.}, 0.05, 2214.9614083, 26975556.88749548,
+     73.34013315261608, 502654.8245743669, 1.0),
+    ({"theta": _cyl_theta, "phi": _cyl_phi}, (_cyl_qx, _cyl_qy), 0.03514647218513852),
+    ({"theta": _cyl_theta, "phi": _cyl_phi}, [(_cyl_qx, _cyl_qy)], [0.03514647218513852]),
+] + _cylinder_reff_tests()
+
+# ellipsoid.py:174-180 (structure-factor products omitted)
+_ell_q = 0.1
+ELLIPSOID = [
+    ({}, 0.05, 54.8525847025),
+    ({"theta": 80., "phi": 10.}, (_ell_q*cos(pi/6.0), _ell_q*sin(pi/6.0)), 1.74134670026),
+]
+
+# core_shell_cylinder.py:158-164
+_csc_q = 0.1
+CORE_SHELL_CYLINDER = [
+    ({}, 0.075, 10.8552692237),
+    ({}, (_csc_q*cos(pi/6.0), _csc_q*sin(pi/6.0)), 0.444618752741),
+]
+
+# core_shell_sphere.py:102-107
+CORE_SHELL_SPHERE = [
+    ({"radius": 20.0, "thickness": 10.0}, 0.1, None, None, 30.0, 4.*pi/3*30**3, 1.0),
+    ({"radius": 60.0, "thickness": 10.0, "sld_core": 1.0, "sld_shell": 2.0,
+      "sld_solvent": 3.0, "background": 0.0}, 0.4, 0.000698838),
+]
+
+MODEL_KATS = {
+    "sphere": SPHERE,
+    "cylinder": CYLINDER,
+    "ellipsoid": ELLIPSOID,
+    "core_shell_cylinder": CORE_SHELL_CYLINDER,
+    "core_shell_sphere": CORE_SHELL_SPHERE,
+}
+
+# direct_model.py:562-603, rtol 1e-6: (model, pars, q, I)
+DIRECT_MODEL = [
+    ("sphere", dict(radius=200, background=0), [0.1], [0.6190146273894904]),
+    ("sphere", dict(radius=200, background=0), [(0.1, 0.1)], [1.1771532874802199]),
+    ("sphere", dict(radius=200, sld_M0=3, sld_mtheta=30), [(0.1, 0.1)], [1.5577852226925908]),
+    ("sphere", dict(radius=200, radius_pd=0.1, radius_pd_n=15, radius_pd_nsigma=2.5,
+                    radius_pd_type="uniform"), [0.1], [2.703169824954617]),
+    ("sphere", dict(radius=200, background=1e-4, scale=0.1), [0.1],
+     [0.6190146273894904*0.1 + 1e-4]),
+]
+
+
+def is_near(target, actual, digits=5):
+    """*actual* within *digits* significant digits of *target* (model_test.py:443-471)."""
+    if target is None:
+        return not np.isnan(actual)
+    if target == 0.:
+        return actual == 0.
+    if np.isfinite(target):
+        shift = np.ceil(np.log10(abs(target)))
+        return abs(target - actual) < 1.5*10**(shift - digits)
+    if target == actual:
+        return True
+    return np.isnan(target) == np.isnan(actual)
+
+
+def q_vectors(x):
+    """[q] or [qx, qy] arrays from a KAT q entry."""
+    xs = x if isinstance(x, list) else [x]
+    if isinstance(xs[0], tuple):
+        qx, qy = zip(*xs)
+        return [np.array(qx), np.array(qy)], xs
+    return [np.array(xs, "d")], xs
