@@ -107,6 +107,52 @@ int saship_kernel_get_timing(saship_kernel *kernel, double *eval_ms, int32_t *nc
  * (sasmodels/kernel.py:190-196). */
 int saship_kernel_release(saship_kernel *kernel);
 
+/*
+ * Batched evaluation: many parameter sets, each with its own q vector and
+ * mesh, in one persistent launch.  No reference counterpart: the reference
+ * evaluates a fit population (a bumps DREAM generation, a batch fit) one
+ * Kernel call per set (sasmodels/bumps_model.py:250-285,
+ * sasmodels/direct_model.py:319-351).  For callers with many independent
+ * curves per step; each set's result equals its single-call result.
+ */
+
+/* nq[s] q points per set; q concatenated (1D) or (qx, qy) interleaved (2D). */
+int saship_batch_create(saship_model *model, int32_t nsets, const int32_t *nq,
+                        const double *q, int32_t is2d, saship_batch **out);
+
+/* Whole-mesh evaluation of every set.  details: nsets * n_details int32
+ * (one CallDetails buffer per set); values: concatenated value vectors with
+ * set s at values[values_offset[s] .. values_offset[s+1]).  Outputs (either
+ * may be NULL): result, the raw accumulators of every set back to back in
+ * the layout of saship_kernel_call (nout*nq[s] + 4 each); iq, I(q) of every
+ * set back to back (sum of nq[s]), normalised as Kernel.Iq does
+ * (sasmodels/kernel.py:80-188). */
+int saship_batch_call(saship_batch *batch, int32_t kind,
+                      const int32_t *details, int32_t n_details,
+                      const double *values, const int64_t *values_offset,
+                      double cutoff, int32_t radius_effective_mode,
+                      double *result, double *iq);
+
+/* Launch shape: use_graph replays a captured hipGraph for repeated calls of
+ * the same shape; pts_per_item is the mesh chunk of one work item;
+ * waves_per_cu sizes the persistent grid. */
+int saship_batch_set_options(saship_batch *batch, int32_t use_graph,
+                             int32_t pts_per_item, int32_t waves_per_cu);
+int saship_batch_get_options(saship_batch *batch, int32_t *use_graph,
+                             int32_t *pts_per_item, int32_t *waves_per_cu);
+
+/* Times the evaluation kernel over a grid of (pts_per_item, waves_per_cu)
+ * on these inputs, keeps the fastest and returns it. */
+int saship_batch_autotune(saship_batch *batch, int32_t kind,
+                          const int32_t *details, int32_t n_details,
+                          const double *values, const int64_t *values_offset,
+                          double cutoff, int32_t *best_pts_per_item,
+                          int32_t *best_waves_per_cu);
+
+int saship_batch_set_timing(saship_batch *batch, int32_t enable);
+int saship_batch_get_timing(saship_batch *batch, double *eval_ms, int32_t *ncalls);
+int saship_batch_release(saship_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,17 @@ SIGNATURES = {
     "saship_kernel_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_kernel_get_timing": (ct.c_int, [ct.c_void_p, c_double_p, c_int32_p]),
     "saship_kernel_release": (ct.c_int, [ct.c_void_p]),
+    "saship_batch_create": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_void_p, ct.c_int32,
+                                       ct.POINTER(ct.c_void_p)]),
+    "saship_batch_call": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32, ct.c_void_p,
+                                     ct.c_void_p, ct.c_double, ct.c_int32, ct.c_void_p, ct.c_void_p]),
+    "saship_batch_set_options": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_int32]),
+    "saship_batch_get_options": (ct.c_int, [ct.c_void_p, c_int32_p, c_int32_p, c_int32_p]),
+    "saship_batch_autotune": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32, ct.c_void_p,
+                                         ct.c_void_p, ct.c_double, c_int32_p, c_int32_p]),
+    "saship_batch_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
+    "saship_batch_get_timing": (ct.c_int, [ct.c_void_p, c_double_p, c_int32_p]),
+    "saship_batch_release": (ct.c_int, [ct.c_void_p]),
 }
 
 

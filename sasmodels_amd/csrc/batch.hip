@@ -1,0 +1,632 @@
+// batch.hip -- many parameter sets, each with its own q vector and mesh, in
+// one persistent launch.  No reference counterpart: the reference evaluates
+// a fit population one Kernel call at a time (bumps_model.py:250-285).
+//
+// One call runs on the batch's stream:
+//   H2D staging (values, details, work list, per-set tables, all sets)
+//   k_expand        one workgroup per set (kernels.h); resets the work queue
+//   k_batch_eval_*  persistent: CUs * waves_per_cu / 4 workgroups; every
+//                   wave64 takes (set, 64-q tile, mesh chunk) items from an
+//                   atomic counter -- longest items first -- and reads the
+//                   item's records with wave-uniform (scalar) loads
+//   k_batch_final   one workgroup per set: per output, the chunk partials in
+//                   chunk order (FP64), the weight sums, and I(q) with the
+//                   normalisation of Kernel.Iq (sasmodels/kernel.py:80-188)
+//   D2H             raw result and/or I(q)
+// The whole sequence can be captured once in a hipGraph and replayed for
+// every later call with the same shapes (a fit step).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace saship_host;
+
+namespace sas {
+
+constexpr int QTILE = 64;   // q points per work item: one per lane
+
+struct BatchArgs {
+    const float *q32;            // 1D: q; 2D: interleaved (qx, qy)
+    const double *q64;           // 2D: interleaved (qx, qy) in FP64
+    const int64_t *qoff;         // [nsets+1] q offsets
+    const double *values;
+    const int64_t *voff;         // [nsets]
+    const float4 *recs;
+    const int64_t *roff;         // [nsets] record offsets
+    const int64_t *poff;         // [nsets] partial offsets (doubles)
+    const int *npts;             // [nsets] accepted points (from k_expand)
+    const int4 *items;           // (set, tile, chunk, 0)
+    const int *n_items;
+    int *queue;
+    int pts_per_item;
+    Quad quad;
+    double *partial;
+};
+
+__device__ __forceinline__ int next_item(int *queue)
+{
+    int item = 0;
+    if ((threadIdx.x & 63) == 0) item = atomicAdd(queue, 1);
+    return __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+}
+
+template <class M, bool FIXED>
+__global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a)
+{
+    constexpr int RV = rv1<M>();
+    const int lane = threadIdx.x & 63;
+    const int n_items = *a.n_items;
+    for (;;) {
+        const int item = next_item(a.queue);
+        if (item >= n_items) break;
+        const int4 it = a.items[item];
+        const int s = it.x, tile = it.y, chunk = it.z;
+        const int64_t q0 = a.qoff[s];
+        const int nq = (int)(a.qoff[s + 1] - q0);
+        const int iq = tile * QTILE + lane;
+        const float q = iq < nq ? a.q32[q0 + iq] : 0.0f;
+        const int npts = a.npts[s];
+        const int c0 = min(npts, chunk * a.pts_per_item);
+        const int c1 = min(npts, c0 + a.pts_per_item);
+        const float4 *rec = a.recs + a.roff[s] * RV;
+        KSum a1, a2;
+        for (int j = c0; j < c1; j++) {
+            float r[4 * RV];
+            load_rec(rec + (size_t)j * RV, r, RV);
+            float f1, f2;
+            M::template eval1d<FIXED>(q, r, a.quad, f1, f2);
+            a1.add(f1);
+            a2.add(f2);
+        }
+        if (iq < nq) {
+            double *dst = a.partial + a.poff[s] + (size_t)chunk * 2 * nq;
+            dst[iq] = a2.value();
+            dst[nq + iq] = a1.value();
+        }
+    }
+}
+
+template <class M, bool MAG>
+__global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a)
+{
+    constexpr int RV = rv2<M>();
+    const int lane = threadIdx.x & 63;
+    const int n_items = *a.n_items;
+    for (;;) {
+        const int item = next_item(a.queue);
+        if (item >= n_items) break;
+        const int4 it = a.items[item];
+        const int s = it.x, tile = it.y, chunk = it.z;
+        const int64_t q0 = a.qoff[s];
+        const int nq = (int)(a.qoff[s + 1] - q0);
+        const int iq = tile * QTILE + lane;
+        const bool ok = iq < nq;
+        const float qx = ok ? a.q32[2 * (q0 + iq)] : 0.0f;
+        const float qy = ok ? a.q32[2 * (q0 + iq) + 1] : 0.0f;
+        const float qmag = sqrt_f(fmaf(qx, qx, qy * qy));
+        Contrast<M, MAG> con;
+        con.setup(a.values + a.voff[s], ok ? a.q64[2 * (q0 + iq)] : 0.0, ok ? a.q64[2 * (q0 + iq) + 1] : 0.0);
+        const int npts = a.npts[s];
+        const int c0 = min(npts, chunk * a.pts_per_item);
+        const int c1 = min(npts, c0 + a.pts_per_item);
+        const float4 *rec = a.recs + a.roff[s] * RV;
+        KSum acc;
+        for (int j = c0; j < c1; j++) {
+            float r[4 * RV];
+            load_rec(rec + (size_t)j * RV, r, RV);
+            float g[M::NCON];
+            geom_at<M>(r, qx, qy, qmag, g);
+            acc.add(r[M::A2D] * con.f2(g));
+        }
+        if (ok) a.partial[a.poff[s] + (size_t)chunk * nq + iq] = acc.value();
+    }
+}
+
+// Per set: sum the chunk partials of every output in chunk order, add the
+// weight sums, and form I(q) = scale * <F^2>/<V> + background with the
+// guards of Kernel.Fq (total weight 0 -> 1, shell volume 0 -> 1).
+__global__ __launch_bounds__(256) void k_batch_final(
+    int nout, const int64_t *__restrict__ qoff, const double *__restrict__ values,
+    const int64_t *__restrict__ voff, const int64_t *__restrict__ poff,
+    const int64_t *__restrict__ resoff, const int *__restrict__ nchunk,
+    const double *__restrict__ sums, const double *__restrict__ partial,
+    double *__restrict__ result, double *__restrict__ iq_out)
+{
+    const int s = blockIdx.x;
+    const int64_t q0 = qoff[s];
+    const int nq = (int)(qoff[s + 1] - q0);
+    const int nc = nchunk[s];
+    const double *part = partial + poff[s];
+    const double *sm = sums + 4 * s;
+    double total = sm[0];
+    if (total == 0.0) total = 1.0;
+    double shell = sm[2] / total;
+    if (shell == 0.0) shell = 1.0;
+    const double scale = values[voff[s]], background = values[voff[s] + 1];
+    for (int e = threadIdx.x; e < nq; e += 256) {
+        double f2 = 0.0, f1 = 0.0;
+        for (int c = 0; c < nc; c++) {
+            f2 += part[(size_t)c * nout * nq + e];
+            if (nout == 2) f1 += part[(size_t)c * nout * nq + nq + e];
+        }
+        if (result) {
+            result[resoff[s] + (size_t)e * nout] = f2;
+            if (nout == 2) result[resoff[s] + (size_t)e * nout + 1] = f1;
+        }
+        if (iq_out) iq_out[q0 + e] = scale / shell * (f2 / total) + background;
+    }
+    if (result && threadIdx.x < 4) result[resoff[s] + (size_t)nout * nq + threadIdx.x] = sm[threadIdx.x];
+}
+
+}  // namespace sas
+
+struct saship_batch {
+    saship_model *m = nullptr;
+    int nsets = 0, is2d = 0, dev = 0, cus = 256;
+    hipStream_t st = nullptr;
+    std::vector<int64_t> qoff;
+    float *q32 = nullptr;
+    double *q64 = nullptr;
+    int64_t *d_qoff = nullptr;
+    // per-call buffers
+    char *d_args = nullptr;
+    size_t args_cap = 0;
+    char *h_stage = nullptr;
+    size_t stage_cap = 0;
+    float4 *d_recs = nullptr;
+    size_t recs_cap = 0;
+    int *d_npts = nullptr;
+    double *d_sums = nullptr;
+    int *d_queue = nullptr;
+    double *d_partial = nullptr;
+    size_t partial_cap = 0;
+    double *d_out = nullptr;          // raw results then I(q)
+    size_t out_cap = 0;
+    char *h_out = nullptr;
+    size_t hout_cap = 0;
+    // launch shape
+    int pts_per_item = 16;
+    int waves_per_cu = 32;
+    // graph replay
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<int64_t> gkey;
+    // timing of the evaluation kernel
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing = false;
+    double t_ms = 0.0;
+    int ncalls = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+void drop_graph(saship_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->gkey.clear();
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Host-side plan of one call: the staging layout and the work list.
+struct Plan {
+    int kind = 0, nout = 2, ndet = 0, rv = 1;
+    int64_t n_values = 0, rec_total = 0, partial_total = 0, result_total = 0, nq_total = 0;
+    int n_items = 0;
+    size_t off_values = 0, off_voff = 0, off_det = 0, off_roff = 0, off_poff = 0, off_resoff = 0;
+    size_t off_nchunk = 0, off_items = 0, off_nitems = 0, bytes = 0;
+};
+
+template <class M>
+int rv_of(int kind) { return kind == 0 ? sas::rv1<M>() : sas::rv2<M>(); }
+
+int record_width(int id, int kind)
+{
+    switch (id) {
+    case 0: return rv_of<sas::Sphere>(kind);
+    case 1: return rv_of<sas::Cylinder>(kind);
+    case 2: return rv_of<sas::Ellipsoid>(kind);
+    case 3: return rv_of<sas::CoreShellCylinder>(kind);
+    default: return rv_of<sas::CoreShellSphere>(kind);
+    }
+}
+
+// Validate every set, lay out the staging buffer and fill it (host pinned).
+int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_details,
+                   const double *values, const int64_t *voff_in, Plan &p)
+{
+    const int id = b->m->id, ns = b->nsets;
+    p.kind = kind;
+    p.nout = kind == 0 ? 2 : 1;
+    p.ndet = n_details;
+    p.rv = record_width(id, kind);
+    p.n_values = voff_in[ns];
+    if (voff_in[0] != 0) return fail(SASHIP_EINVAL, "values_offset[0] must be 0");
+    std::vector<int64_t> roff(ns), poff(ns), resoff(ns);
+    std::vector<int32_t> nchunk(ns);
+    int64_t items = 0;
+    const int P = b->pts_per_item;
+    for (int s = 0; s < ns; s++) {
+        const int64_t nv = voff_in[s + 1] - voff_in[s];
+        const int32_t *det = details + (size_t)s * n_details;
+        const std::string where = "set " + std::to_string(s) + ": ";
+        if (nv < 0) return fail(SASHIP_EINVAL, where + "values_offset must be non-decreasing");
+        if (int e = validate_set(id, kind, det, n_details, nv, 0, -1, where)) return e;
+        const int64_t num_eval = det[n_details - 4];
+        const int nq = (int)(b->qoff[s + 1] - b->qoff[s]);
+        roff[s] = p.rec_total;
+        p.rec_total += num_eval;
+        nchunk[s] = (int32_t)std::max<int64_t>(1, (num_eval + P - 1) / P);
+        poff[s] = p.partial_total;
+        p.partial_total += (int64_t)nchunk[s] * p.nout * nq;
+        resoff[s] = p.result_total;
+        p.result_total += (int64_t)p.nout * nq + 4;
+        items += (int64_t)nchunk[s] * ((nq + sas::QTILE - 1) / sas::QTILE);
+    }
+    if (items > 0x7fffffff) return fail(SASHIP_EINVAL, "too many work items in one batch");
+    p.n_items = (int)items;
+    p.nq_total = b->qoff[ns];
+    size_t off = 0;
+    p.off_values = off; off = align256(off + sizeof(double) * p.n_values);
+    p.off_voff = off; off = align256(off + sizeof(int64_t) * ns);
+    p.off_det = off; off = align256(off + sizeof(int32_t) * (size_t)ns * n_details);
+    p.off_roff = off; off = align256(off + sizeof(int64_t) * ns);
+    p.off_poff = off; off = align256(off + sizeof(int64_t) * ns);
+    p.off_resoff = off; off = align256(off + sizeof(int64_t) * ns);
+    p.off_nchunk = off; off = align256(off + sizeof(int32_t) * ns);
+    p.off_items = off; off = align256(off + sizeof(int4) * (size_t)std::max(1, p.n_items));
+    p.off_nitems = off; off = align256(off + sizeof(int32_t));
+    p.bytes = off;
+    if (int e = ensure_host(b->h_stage, b->stage_cap, p.bytes)) return e;
+    char *h = b->h_stage;
+    memcpy(h + p.off_values, values, sizeof(double) * p.n_values);
+    memcpy(h + p.off_voff, voff_in, sizeof(int64_t) * ns);
+    memcpy(h + p.off_det, details, sizeof(int32_t) * (size_t)ns * n_details);
+    memcpy(h + p.off_roff, roff.data(), sizeof(int64_t) * ns);
+    memcpy(h + p.off_poff, poff.data(), sizeof(int64_t) * ns);
+    memcpy(h + p.off_resoff, resoff.data(), sizeof(int64_t) * ns);
+    memcpy(h + p.off_nchunk, nchunk.data(), sizeof(int32_t) * ns);
+    // work list, longest items first: items are bucketed by the number of
+    // mesh points their chunk can hold (an upper bound before pruning)
+    int4 *it = (int4 *)(h + p.off_items);
+    int64_t k = 0;
+    for (int fill = P; fill >= 1; fill--) {
+        for (int s = 0; s < ns; s++) {
+            const int64_t num_eval = details[(size_t)s * n_details + n_details - 4];
+            const int nq = (int)(b->qoff[s + 1] - b->qoff[s]);
+            const int ntile = (nq + sas::QTILE - 1) / sas::QTILE;
+            for (int c = 0; c < nchunk[s]; c++) {
+                const int64_t here = std::min<int64_t>(P, num_eval - (int64_t)c * P);
+                if (std::max<int64_t>(1, here) != fill) continue;
+                for (int t = 0; t < ntile; t++) it[k++] = make_int4(s, t, c, 0);
+            }
+        }
+    }
+    *(int32_t *)(h + p.off_nitems) = p.n_items;
+    return 0;
+}
+
+template <class M>
+int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq)
+{
+    using namespace sas;
+    const char *d = b->d_args;
+    const double *d_values = (const double *)(d + p.off_values);
+    const int64_t *d_voff = (const int64_t *)(d + p.off_voff);
+    const int32_t *d_det = (const int32_t *)(d + p.off_det);
+    const int64_t *d_roff = (const int64_t *)(d + p.off_roff);
+    const int64_t *d_poff = (const int64_t *)(d + p.off_poff);
+    const int64_t *d_resoff = (const int64_t *)(d + p.off_resoff);
+    const int32_t *d_nchunk = (const int32_t *)(d + p.off_nchunk);
+    const int4 *d_items = (const int4 *)(d + p.off_items);
+    const int *d_nitems = (const int *)(d + p.off_nitems);
+    double *d_result = b->d_out;
+    double *d_iq = b->d_out + p.result_total;
+
+    HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
+    ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
+                  b->d_recs, b->d_npts, b->d_sums, b->d_queue};
+    switch (p.kind) {
+    case 0: k_expand<M, 0><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+    case 1: k_expand<M, 1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+    default: k_expand<M, 2><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+    }
+    HIP_TRY(hipGetLastError());
+
+    BatchArgs ba{b->q32, b->q64, b->d_qoff, d_values, d_voff, b->d_recs, d_roff, d_poff, b->d_npts,
+                 d_items, d_nitems, b->d_queue, b->pts_per_item, b->m->quad, b->d_partial};
+    const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
+    if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
+    if (p.kind == 0) {
+        if (b->m->quad_n > 0) k_batch_eval_1d<M, true><<<grid, 256, 0, b->st>>>(ba);
+        else k_batch_eval_1d<M, false><<<grid, 256, 0, b->st>>>(ba);
+    } else if (p.kind == 1) {
+        k_batch_eval_2d<M, false><<<grid, 256, 0, b->st>>>(ba);
+    } else {
+        k_batch_eval_2d<M, true><<<grid, 256, 0, b->st>>>(ba);
+    }
+    HIP_TRY(hipGetLastError());
+    if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
+
+    k_batch_final<<<b->nsets, 256, 0, b->st>>>(p.nout, b->d_qoff, d_values, d_voff, d_poff, d_resoff,
+                                                d_nchunk, b->d_sums, b->d_partial,
+                                                want_result ? d_result : nullptr, want_iq ? d_iq : nullptr);
+    HIP_TRY(hipGetLastError());
+    if (want_result)
+        HIP_TRY(hipMemcpyAsync(b->h_out, d_result, sizeof(double) * p.result_total, hipMemcpyDeviceToHost, b->st));
+    if (want_iq)
+        HIP_TRY(hipMemcpyAsync(b->h_out + sizeof(double) * p.result_total, d_iq, sizeof(double) * p.nq_total,
+                               hipMemcpyDeviceToHost, b->st));
+    return 0;
+}
+
+int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq)
+{
+    switch (b->m->id) {
+    case 0: return enqueue<sas::Sphere>(b, p, cutoff, reff_mode, want_result, want_iq);
+    case 1: return enqueue<sas::Cylinder>(b, p, cutoff, reff_mode, want_result, want_iq);
+    case 2: return enqueue<sas::Ellipsoid>(b, p, cutoff, reff_mode, want_result, want_iq);
+    case 3: return enqueue<sas::CoreShellCylinder>(b, p, cutoff, reff_mode, want_result, want_iq);
+    default: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq);
+    }
+}
+
+// Make every device buffer large enough for plan p (drops a stale graph).
+int size_buffers(saship_batch *b, const Plan &p)
+{
+    bool grew = false;
+    auto grow = [&](auto &ptr, size_t &cap, size_t bytes) -> int {
+        if (bytes <= cap && ptr) return 0;
+        grew = true;
+        return ensure(ptr, cap, bytes);
+    };
+    if (int e = grow(b->d_args, b->args_cap, p.bytes)) return e;
+    if (int e = grow(b->d_recs, b->recs_cap, sizeof(float4) * (size_t)std::max<int64_t>(1, p.rec_total) * p.rv)) return e;
+    if (int e = grow(b->d_partial, b->partial_cap, sizeof(double) * (size_t)std::max<int64_t>(1, p.partial_total))) return e;
+    if (int e = grow(b->d_out, b->out_cap, sizeof(double) * (size_t)(p.result_total + p.nq_total))) return e;
+    const size_t hb = sizeof(double) * (size_t)(p.result_total + p.nq_total);
+    if (hb > b->hout_cap || !b->h_out) {
+        grew = true;
+        if (int e = ensure_host(b->h_out, b->hout_cap, hb)) return e;
+    }
+    if (grew) drop_graph(b);
+    return 0;
+}
+
+int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, const double *values,
+        const int64_t *voff, double cutoff, int reff_mode, double *result, double *iq)
+{
+    Plan p;
+    if (int e = plan_and_stage(b, kind, details, n_details, values, voff, p)) return e;
+    if (int e = size_buffers(b, p)) return e;
+    const bool want_result = result != nullptr, want_iq = iq != nullptr;
+    if (b->use_graph && !b->timing) {
+        std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
+                                    reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu};
+        int64_t cbits;
+        memcpy(&cbits, &cutoff, sizeof cbits);
+        key.push_back(cbits);
+        if (!b->gexec || key != b->gkey) {
+            drop_graph(b);
+            HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(b->st, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ce != hipSuccess) return fail(SASHIP_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            b->graph = g;
+            HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+            b->gkey = key;
+        }
+        HIP_TRY(hipGraphLaunch(b->gexec, b->st));
+    } else {
+        if (int e = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq)) return e;
+    }
+    HIP_TRY(hipStreamSynchronize(b->st));
+    if (want_result) memcpy(result, b->h_out, sizeof(double) * p.result_total);
+    if (want_iq) memcpy(iq, b->h_out + sizeof(double) * p.result_total, sizeof(double) * p.nq_total);
+    if (b->timing) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->t_ms += ms;
+        b->ncalls++;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const double *q, int32_t is2d,
+                        saship_batch **out)
+{
+    if (!out) return fail(SASHIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!m) return fail(SASHIP_EINVAL, "model is NULL");
+    if (nsets < 1 || !nq) return fail(SASHIP_EINVAL, "need nsets >= 1 and a q-count array");
+    std::vector<int64_t> qoff(nsets + 1, 0);
+    for (int s = 0; s < nsets; s++) {
+        if (nq[s] < 0) return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": nq < 0");
+        qoff[s + 1] = qoff[s] + nq[s];
+    }
+    const int64_t total = qoff[nsets];
+    if (total > 0 && !q) return fail(SASHIP_EINVAL, "q is NULL");
+    HIP_TRY(hipSetDevice(m->dev));
+    saship_batch *b = new (std::nothrow) saship_batch();
+    if (!b) return fail(SASHIP_ENOMEM, "out of host memory");
+    b->m = m;
+    model_retain(m);
+    b->nsets = nsets;
+    b->is2d = is2d ? 1 : 0;
+    b->dev = m->dev;
+    b->cus = cu_count(m->dev);
+    b->qoff = qoff;
+    const size_t nqv = (size_t)total * (b->is2d ? 2 : 1);
+    std::vector<float> q32(std::max<size_t>(nqv, 1));
+    for (size_t i = 0; i < nqv; i++) q32[i] = (float)q[i];
+    hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->q32, sizeof(float) * std::max<size_t>(nqv, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->q64, sizeof(double) * std::max<size_t>(nqv, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_qoff, sizeof(int64_t) * (nsets + 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_npts, sizeof(int) * nsets);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_sums, sizeof(double) * 4 * nsets);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_queue, 256);
+    if (e == hipSuccess && nqv) e = hipMemcpy(b->q32, q32.data(), sizeof(float) * nqv, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nqv) e = hipMemcpy(b->q64, q, sizeof(double) * nqv, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_qoff, qoff.data(), sizeof(int64_t) * (nsets + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        std::string msg = std::string("batch create: ") + hipGetErrorString(e);
+        saship_batch_release(b);
+        return fail(SASHIP_EHIP, msg);
+    }
+    *out = b;
+    return 0;
+}
+
+int saship_batch_set_options(saship_batch *b, int32_t use_graph, int32_t pts_per_item, int32_t waves_per_cu)
+{
+    if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
+    if (pts_per_item < 1 || pts_per_item > 4096) return fail(SASHIP_EINVAL, "pts_per_item must be 1..4096");
+    if (waves_per_cu < 4 || waves_per_cu > 128) return fail(SASHIP_EINVAL, "waves_per_cu must be 4..128");
+    std::lock_guard<std::mutex> lock(b->mu);
+    b->use_graph = use_graph != 0;
+    b->pts_per_item = pts_per_item;
+    b->waves_per_cu = waves_per_cu;
+    drop_graph(b);
+    return 0;
+}
+
+int saship_batch_get_options(saship_batch *b, int32_t *use_graph, int32_t *pts_per_item, int32_t *waves_per_cu)
+{
+    if (!b || !use_graph || !pts_per_item || !waves_per_cu) return fail(SASHIP_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lock(b->mu);
+    *use_graph = b->use_graph;
+    *pts_per_item = b->pts_per_item;
+    *waves_per_cu = b->waves_per_cu;
+    return 0;
+}
+
+int saship_batch_call(saship_batch *b, int32_t kind, const int32_t *details, int32_t n_details,
+                      const double *values, const int64_t *values_offset, double cutoff,
+                      int32_t radius_effective_mode, double *result, double *iq)
+{
+    if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
+    if (!details || !values || !values_offset) return fail(SASHIP_EINVAL, "details, values and values_offset must not be NULL");
+    if (kind < 0 || kind > 2) return fail(SASHIP_EINVAL, "kind must be 0 (Iq), 1 (Iqxy) or 2 (Imagnetic)");
+    if ((kind == 0) == (b->is2d != 0))
+        return fail(SASHIP_EINVAL, kind == 0 ? "Iq kernel needs 1D q" : "2D kernels need (qx, qy) input");
+    std::lock_guard<std::mutex> lock(b->mu);
+    HIP_TRY(hipSetDevice(b->dev));
+    return run(b, kind, details, n_details, values, values_offset, cutoff, radius_effective_mode, result, iq);
+}
+
+int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details, int32_t n_details,
+                          const double *values, const int64_t *values_offset, double cutoff,
+                          int32_t *best_pts_per_item, int32_t *best_waves_per_cu)
+{
+    if (!b || !best_pts_per_item || !best_waves_per_cu) return fail(SASHIP_EINVAL, "NULL argument");
+    if (!details || !values || !values_offset) return fail(SASHIP_EINVAL, "details, values and values_offset must not be NULL");
+    if ((kind == 0) == (b->is2d != 0) || kind < 0 || kind > 2) return fail(SASHIP_EINVAL, "kind does not match the q input");
+    std::lock_guard<std::mutex> lock(b->mu);
+    HIP_TRY(hipSetDevice(b->dev));
+    const bool graph = b->use_graph, timing = b->timing;
+    const double t_keep = b->t_ms;
+    const int n_keep = b->ncalls;
+    b->use_graph = false;
+    b->timing = true;
+    int best_p = b->pts_per_item, best_w = b->waves_per_cu;
+    double best_t = 1e300;
+    const int pts[] = {4, 8, 16, 32, 64, 128};
+    const int waves[] = {8, 16, 32};
+    int rc = 0;
+    for (int P : pts) {
+        for (int W : waves) {
+            b->pts_per_item = P;
+            b->waves_per_cu = W;
+            double t = 1e300;
+            for (int rep = 0; rep < 2 && rc == 0; rep++) {      // first run warms up
+                b->t_ms = 0.0;
+                b->ncalls = 0;
+                rc = run(b, kind, details, n_details, values, values_offset, cutoff, 0, nullptr, nullptr);
+                if (rc == 0) t = b->t_ms;
+            }
+            if (rc) break;
+            if (t < best_t) {
+                best_t = t;
+                best_p = P;
+                best_w = W;
+            }
+        }
+        if (rc) break;
+    }
+    b->pts_per_item = best_p;
+    b->waves_per_cu = best_w;
+    b->use_graph = graph;
+    b->timing = timing;
+    b->t_ms = t_keep;
+    b->ncalls = n_keep;
+    drop_graph(b);
+    if (rc) return rc;
+    *best_pts_per_item = best_p;
+    *best_waves_per_cu = best_w;
+    return 0;
+}
+
+int saship_batch_set_timing(saship_batch *b, int32_t enable)
+{
+    if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
+    std::lock_guard<std::mutex> lock(b->mu);
+    b->timing = enable != 0;
+    b->t_ms = 0.0;
+    b->ncalls = 0;
+    return 0;
+}
+
+int saship_batch_get_timing(saship_batch *b, double *eval_ms, int32_t *ncalls)
+{
+    if (!b || !eval_ms || !ncalls) return fail(SASHIP_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lock(b->mu);
+    *eval_ms = b->t_ms;
+    *ncalls = b->ncalls;
+    return 0;
+}
+
+int saship_batch_release(saship_batch *b)
+{
+    if (!b) return 0;
+    (void)hipSetDevice(b->dev);
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    drop_graph(b);
+    void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums,
+                        b->d_queue, b->d_partial, b->d_out};
+    for (void *ptr : dev_ptrs)
+        if (ptr) (void)hipFree(ptr);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    if (b->m) model_drop(b->m);
+    delete b;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+// common.h -- host-side helpers shared by saship.hip and batch.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/saship.h"
+#include "models.h"
+
+namespace saship_host {
+
+// thread-local error message and current device (defined in saship.hip)
+std::string &last_error();
+int &current_device();
+
+inline int fail(int code, const std::string &msg)
+{
+    last_error() = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return saship_host::fail(SASHIP_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Gauss-Legendre nodes and weights on [-1, 1], ascending, by Newton's method
+// on the three-term Legendre recurrence (long double).
+inline void gauss_legendre(int n, std::vector<double> &z, std::vector<double> &w)
+{
+    z.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < (n + 1) / 2; i++) {
+        long double x = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+        for (int it = 0; it < 100; it++) {
+            long double p0 = 1.0L, p1 = x;
+            for (int k = 2; k <= n; k++) {
+                const long double p2 = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            const long double dp = n * (x * p1 - p0) / (x * x - 1.0L);
+            const long double dx = p1 / dp;
+            x -= dx;
+            if (fabsl(dx) < 1e-19L) break;
+        }
+        long double p0 = 1.0L, p1 = x;
+        for (int k = 2; k <= n; k++) {
+            const long double p2 = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        const long double dp = n * (x * p1 - p0) / (x * x - 1.0L);
+        const long double wt = 2.0L / ((1.0L - x * x) * dp * dp);
+        z[n - 1 - i] = (double)x;
+        z[i] = (double)-x;
+        w[n - 1 - i] = w[i] = (double)wt;
+    }
+    if (n % 2) z[n / 2] = 0.0;
+}
+
+inline int cu_count(int dev)
+{
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    return n;
+}
+
+inline int model_npars(int id)
+{
+    static const int np[5] = {3, 6, 6, 8, 5};
+    return np[id];
+}
+inline int model_maxpd(int id)
+{
+    static const int mp[5] = {1, 4, 4, 5, 2};
+    return mp[id];
+}
+inline int model_nvalues(int id)
+{
+    static const int nv[5] = {15, 18, 18, 23, 20};
+    return nv[id];
+}
+inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3; }
+
+// Checks one set's details/values against the model before any device work:
+// every index the kernels derive from them stays inside the buffers.
+inline int validate_set(int id, int kind, const int32_t *details, int64_t n_details,
+                        int64_t n_values, int pd_start, int pd_stop, const std::string &where)
+{
+    const int mp = model_maxpd(id);
+    if (n_details != 4 * mp + 4)
+        return fail(SASHIP_EINVAL, where + "details must have 4*max_pd+4 = " + std::to_string(4 * mp + 4) + " entries");
+    const int32_t num_eval = details[4 * mp], num_weights = details[4 * mp + 1];
+    if (num_eval < 0) return fail(SASHIP_EINVAL, where + "num_eval < 0");
+    if (pd_start < 0 || (pd_stop >= 0 && pd_stop < pd_start))
+        return fail(SASHIP_EINVAL, where + "need 0 <= pd_start <= pd_stop");
+    if (pd_stop > num_eval) return fail(SASHIP_EINVAL, where + "pd_stop exceeds num_eval");
+    if (num_weights < 0 || n_values < model_nvalues(id) + 2 * (int64_t)num_weights)
+        return fail(SASHIP_EINVAL, where + "values shorter than nvalues + 2*num_weights");
+    for (int i = 0; i < mp; i++) {
+        const int par = details[i], len = details[mp + i], off = details[2 * mp + i], stride = details[3 * mp + i];
+        if (par < 0 || par >= model_npars(id) || len < 1 || off < 0 || off + len > num_weights || stride < 1)
+            return fail(SASHIP_EINVAL, where + "inconsistent details buffer at pd level " + std::to_string(i));
+    }
+    if (kind != 0 && model_oriented(id)) {
+        const int tp = details[4 * mp + 3];
+        if (tp < 0 || tp + 1 >= model_npars(id)) return fail(SASHIP_EINVAL, where + "theta_par out of range");
+    }
+    return 0;
+}
+
+// Grow a device buffer to at least `bytes` (contents are not kept).
+template <class T>
+int ensure(T *&ptr, size_t &cap, size_t bytes)
+{
+    if (bytes <= cap && ptr) return 0;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes, 256);
+    HIP_TRY(hipMalloc((void **)&ptr, want));
+    cap = want;
+    return 0;
+}
+
+// Grow a pinned host buffer to at least `bytes` (contents are not kept).
+inline int ensure_host(char *&ptr, size_t &cap, size_t bytes)
+{
+    if (bytes <= cap && ptr) return 0;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes, 4096);
+    HIP_TRY(hipHostMalloc((void **)&ptr, want, hipHostMallocDefault));
+    cap = want;
+    return 0;
+}
+
+}  // namespace saship_host
+
+struct saship_model {
+    int id = 0, quad_n = 0, dev = 0;
+    float *tab = nullptr;     // 3 * ntab floats: a | b | w
+    int ntab = 0;
+    sas::Quad quad{};
+    // kernels and batches made from the model hold a reference; the tables
+    // are freed once the model is released and the last holder is gone
+    std::atomic<int> refs{0};
+    std::atomic<bool> released{false};
+};
+
+namespace saship_host {
+inline void model_free(saship_model *m)
+{
+    (void)hipSetDevice(m->dev);
+    if (m->tab) (void)hipFree(m->tab);
+    delete m;
+}
+inline void model_retain(saship_model *m) { m->refs.fetch_add(1); }
+inline void model_drop(saship_model *m)
+{
+    if (m->refs.fetch_sub(1) == 1 && m->released.load()) model_free(m);
+}
+}  // namespace saship_host

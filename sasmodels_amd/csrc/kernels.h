@@ -1,0 +1,420 @@
+// kernels.h -- device kernels shared by the single-call and batched paths.
+//
+//   k_expand   one workgroup per parameter set walks mesh steps
+//              [pd_start, pd_stop) in the reference order (kernel_iq.c:642-664,
+//              :682-823: level 0 innermost, index_k = (step/stride_k) % length_k),
+//              applies VALID, the projection weight |cos dtheta| of oriented 2D
+//              kernels (kernel_iq.c:621-626) and the strict "weight > cutoff" test
+//              (:726), and compacts accepted points into FP32 records, in mesh
+//              order, with every per-point constant folded in.  Sum w, sum w*V and
+//              sum w*R_eff are reduced in FP64 with a fixed tree.
+//   k_fq1d     1D <F>, <F^2>: 256 q per workgroup (four wave64s), mesh split over
+//              blockIdx.y; records staged in LDS, every lane sums w*F^2 and w*F in
+//              FP32 with TwoSum compensation.
+//   k_iq2d     2D |F|^2, plain or summed over spin cross sections, same tiling.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "models.h"
+
+namespace sas {
+
+constexpr int EXPAND_THREADS = 512;
+constexpr int EVAL_THREADS = 256;
+constexpr int TILE_PTS = 64;
+constexpr double DEG = 0.017453292519943295;   // pi/180
+
+// Record widths in float4 units.
+template <class M> constexpr int rv1() { return (M::REC1D + 3) / 4; }
+template <class M> constexpr int rv2() { return (M::REC2D + (M::ORIENTED ? 8 : 0) + 3) / 4; }
+template <class M, int KIND> constexpr int rvk() { return KIND == 0 ? rv1<M>() : rv2<M>(); }
+
+template <class M> constexpr int nvalues()
+{
+    return 2 + M::NPARS + (M::NMAG ? 4 + 3 * M::NMAG : 0);
+}
+
+// Compensated sum (Knuth TwoSum): s + c carries the exact running total.
+struct KSum {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float x)
+    {
+        const float t = s + x;
+        const float z = t - s;
+        c += (s - (t - z)) + (x - z);
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
+// Where one workgroup of k_expand finds its set and leaves its results.
+struct ExpandArgs {
+    const int32_t *det;         // details of set 0; set s at det + s*ndet
+    const double *values;       // values of set 0; set s at values + voff[s]
+    const int64_t *voff;        // per-set value offsets (NULL: single set)
+    const int64_t *roff;        // per-set record offsets in records (NULL: 0)
+    int ndet;
+    int pd_start, pd_stop;      // pd_stop < 0: the set's whole mesh (num_eval)
+    double cutoff;
+    int reff_mode;
+    float4 *recs;
+    int *npts;                  // [nsets] accepted points
+    double *sums;               // [4*nsets] sum w, sum w*V_form, sum w*V_shell, sum w*R_eff
+    int *queue;                 // reset to 0 by workgroup 0 when not NULL
+};
+
+template <class M, int KIND>
+__global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
+{
+    constexpr int NP = M::NPARS, MP = M::MAX_PD;
+    constexpr bool OR2D = (KIND != 0) && M::ORIENTED;
+    constexpr int RV = rvk<M, KIND>();
+    const int set = blockIdx.x;
+    const int32_t *det = a.det + (size_t)set * a.ndet;
+    const double *values = a.values + (a.voff ? a.voff[set] : 0);
+    float4 *recs = a.recs + (a.roff ? a.roff[set] : 0) * RV;
+    if (a.queue && set == 0 && threadIdx.x == 0) a.queue[0] = 0;
+
+    const int32_t *pd_par = det, *pd_len = det + MP, *pd_off = det + 2 * MP, *pd_stride = det + 3 * MP;
+    const int num_eval = det[4 * MP];
+    const int num_weights = det[4 * MP + 1];
+    const int theta_par = det[4 * MP + 3];
+    const int pd_start = a.pd_start;
+    const int pd_stop = a.pd_stop < 0 ? num_eval : a.pd_stop;
+    const double *pd_value = values + nvalues<M>();
+    const double *pd_weight = pd_value + num_weights;
+
+    double base[NP];
+#pragma unroll
+    for (int i = 0; i < NP; i++) base[i] = values[2 + i];
+    double view_theta = 0.0, view_phi = 0.0;
+    if (OR2D) {
+        view_theta = values[theta_par + 2];
+        view_phi = values[theta_par + 3];
+#pragma unroll
+        for (int i = 0; i < NP; i++)
+            if (i == theta_par || i == theta_par + 1) base[i] = 0.0;
+    }
+    double sv, cv, sp, cp;
+    sincos(view_theta * DEG, &sv, &cv);
+    sincos(view_phi * DEG, &sp, &cp);
+
+    __shared__ int wave_cnt[EXPAND_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    double acc_w = 0.0, acc_v = 0.0, acc_r = 0.0;
+    int running = 0;
+    for (int b0 = pd_start; b0 < pd_stop; b0 += EXPAND_THREADS) {
+        const int step = b0 + tid;
+        bool keep = false;
+        float rec[4 * RV];
+        if (step < pd_stop) {
+            double p[NP];
+#pragma unroll
+            for (int i = 0; i < NP; i++) p[i] = base[i];
+            double w0 = 1.0;
+            for (int k = MP - 1; k >= 0; k--) {
+                const int idx = (step / pd_stride[k]) % pd_len[k];
+                const double v = pd_value[pd_off[k] + idx];
+                const int par = pd_par[k];
+#pragma unroll
+                for (int i = 0; i < NP; i++)
+                    if (i == par) p[i] = v;
+                w0 = pd_weight[pd_off[k] + idx] * w0;
+            }
+            double dth = 0.0, dph = 0.0;
+            if (OR2D) {
+#pragma unroll
+                for (int i = 0; i < NP; i++) {
+                    if (i == theta_par) dth = p[i];
+                    if (i == theta_par + 1) dph = p[i];
+                }
+            }
+            const double weight = OR2D ? fabs(cos(dth * DEG)) * w0 : w0;
+            if (M::valid(p) && weight > a.cutoff) {
+                keep = true;
+                const double vol = M::volume(p);
+                acc_w += weight;
+                acc_v += weight * vol;
+                if (a.reff_mode != 0) acc_r += weight * M::reff(a.reff_mode, p);
+#pragma unroll
+                for (int i = 0; i < 4 * RV; i++) rec[i] = 0.0f;
+                if (KIND == 0) {
+                    M::rec1d(p, weight, rec);
+                } else {
+                    M::rec2d(p, weight, rec);
+                    if (OR2D) {
+                        // rows of (jitter^-1 * view^-1) acting on (qx, qy, 0);
+                        // row 3 is the reference's qac_rotation (kernel_iq.c:197-230),
+                        // rows 1-2 give qab directly instead of sqrt(q^2 - qc^2).
+                        double sdt, cdt, sdp, cdp;
+                        sincos(dth * DEG, &sdt, &cdt);
+                        sincos(dph * DEG, &sdp, &cdp);
+                        const double v11 = cp * cv, v12 = sp * cv, v21 = -sp, v22 = cp;
+                        const double v31 = sv * cp, v32 = sp * sv;
+                        const double j11 = cdt, j12 = sdp * sdt, j13 = -sdt * cdp;
+                        const double j22 = cdp, j23 = sdp;
+                        const double j31 = sdt, j32 = -sdp * cdt, j33 = cdp * cdt;
+                        float *rot = rec + M::REC2D;
+                        rot[0] = (float)(j11 * v11 + j12 * v21 + j13 * v31);
+                        rot[1] = (float)(j11 * v12 + j12 * v22 + j13 * v32);
+                        rot[2] = (float)(j22 * v21 + j23 * v31);
+                        rot[3] = (float)(j22 * v22 + j23 * v32);
+                        rot[4] = (float)(j31 * v11 + j32 * v21 + j33 * v31);
+                        rot[5] = (float)(j31 * v12 + j32 * v22 + j33 * v32);
+                    }
+                }
+            }
+        }
+        const uint64_t mask = __ballot(keep);
+        if (lane == 0) wave_cnt[wid] = __popcll(mask);
+        __syncthreads();
+        int woff = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < EXPAND_THREADS / 64; w++) {
+            const int c = wave_cnt[w];
+            woff += (w < wid) ? c : 0;
+            total += c;
+        }
+        if (keep) {
+            const int pos = running + woff + __popcll(mask & lt);
+            float4 *dst = recs + (size_t)pos * RV;
+#pragma unroll
+            for (int j = 0; j < RV; j++)
+                dst[j] = make_float4(rec[4 * j], rec[4 * j + 1], rec[4 * j + 2], rec[4 * j + 3]);
+        }
+        running += total;
+        __syncthreads();
+    }
+
+    // fixed-order FP64 tree reduction of the normalisation sums
+    __shared__ double red[3][EXPAND_THREADS];
+    red[0][tid] = acc_w;
+    red[1][tid] = acc_v;
+    red[2][tid] = acc_r;
+    __syncthreads();
+    for (int s = EXPAND_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red[0][tid] += red[0][tid + s];
+            red[1][tid] += red[1][tid + s];
+            red[2][tid] += red[2][tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        a.npts[set] = running;
+        a.sums[4 * set + 0] = red[0][0];
+        a.sums[4 * set + 1] = red[1][0];
+        a.sums[4 * set + 2] = red[1][0];   // shell volume == form volume for these solids
+        a.sums[4 * set + 3] = red[2][0];
+    }
+}
+
+// Range of compacted points handled by split `split` of `nsplit`.
+__device__ __forceinline__ void split_range(int npts, int nsplit, int split, int &s0, int &s1)
+{
+    const int per = (npts + nsplit - 1) / nsplit;
+    s0 = min(npts, split * per);
+    s1 = min(npts, s0 + per);
+}
+
+__device__ __forceinline__ void load_rec(const float4 *src, float *r, int rv)
+{
+    for (int v = 0; v < rv; v++) {
+        const float4 x = src[v];
+        r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
+    }
+}
+
+// ------------------------------------------------------------------ 1D <F>, <F^2>
+template <class M, bool FIXED>
+__global__ __launch_bounds__(EVAL_THREADS) void k_fq1d(
+    int nq, const float *__restrict__ q32, const float4 *__restrict__ recs,
+    const int *__restrict__ npts_p, int nsplit, Quad quad, double *__restrict__ partial)
+{
+    constexpr int RV = rv1<M>();
+    __shared__ float4 tile[TILE_PTS * RV];
+    const int iq = blockIdx.x * EVAL_THREADS + threadIdx.x;
+    const int split = blockIdx.y;
+    int s0, s1;
+    split_range(*npts_p, nsplit, split, s0, s1);
+    const float q = iq < nq ? q32[iq] : 0.0f;
+    KSum a1, a2;
+    for (int t0 = s0; t0 < s1; t0 += TILE_PTS) {
+        const int cnt = min(TILE_PTS, s1 - t0);
+        for (int i = threadIdx.x; i < cnt * RV; i += EVAL_THREADS) tile[i] = recs[(size_t)t0 * RV + i];
+        __syncthreads();
+        if (iq < nq) {
+            for (int j = 0; j < cnt; j++) {
+                float r[4 * RV];
+#pragma unroll
+                for (int v = 0; v < RV; v++) {
+                    const float4 x = tile[j * RV + v];
+                    r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
+                }
+                float f1, f2;
+                M::template eval1d<FIXED>(q, r, quad, f1, f2);
+                a1.add(f1);
+                a2.add(f2);
+            }
+        }
+        __syncthreads();
+    }
+    if (iq < nq) {
+        partial[((size_t)0 * nq + iq) * nsplit + split] = a2.value();
+        partial[((size_t)1 * nq + iq) * nsplit + split] = a1.value();
+    }
+}
+
+// ------------------------------------------------------------------ 2D, plain and magnetic
+// Effective SLD for spin cross section xs (kernel_iq.c:127-175): the
+// magnetisation component perpendicular to q, projected on the polarisation
+// (non-spin-flip) or on the two axes normal to it (spin-flip real/imag).
+__device__ __forceinline__ double mag_sld(int xs, double qx, double qy, double cmt, double smt,
+                                          double cmp, double smp, double sld,
+                                          double mx, double my, double mz)
+{
+    const double qn = sqrt(qx * qx + qy * qy);
+    const double ux = qx / qn, uy = qy / qn;
+    const double k = (mx * ux + my * uy) / (ux * ux + uy * uy);
+    const double px = mx - k * ux, py = my - k * uy, pz = mz;
+    switch (xs) {
+    case 0: return sld - (smt * cmp * px + smt * smp * py + cmt * pz);
+    case 1:
+    case 2: return -smp * px + cmp * py;
+    case 3: return sld + (smt * cmp * px + smt * smp * py + cmt * pz);
+    case 4: return -(-cmt * cmp * px - cmt * smp * py + smt * pz);
+    default: return -cmt * cmp * px - cmt * smp * py + smt * pz;
+    }
+}
+
+// Spin cross-section weights dd, du.re, ud.re, uu, du.im, ud.im (kernel_iq.c:84-124)
+__device__ __forceinline__ void spin_weights(double in_spin, double out_spin, double *w)
+{
+    in_spin = fmin(fmax(in_spin, 0.0), 1.0);
+    out_spin = fmin(fmax(out_spin, 0.0), 1.0);
+    const double norm = out_spin < 0.5 ? 1.0 - out_spin : out_spin;
+    w[0] = (1.0 - in_spin) * (1.0 - out_spin) / norm;
+    w[1] = (1.0 - in_spin) * out_spin / norm;
+    w[2] = in_spin * (1.0 - out_spin) / norm;
+    w[3] = in_spin * out_spin / norm;
+    w[4] = w[1];
+    w[5] = w[2];
+}
+
+// Per-lane contrast terms for each spin cross section, from the set's values.
+template <class M, bool MAG>
+struct Contrast {
+    static constexpr int NX = MAG ? 6 : 1;
+    float con[NX][M::NCON];
+    float xsw[NX];
+    bool active[NX];
+
+    __device__ __forceinline__ void setup(const double *values, double qx, double qy)
+    {
+        constexpr int NP = M::NPARS;
+        double sld[M::NMAG];
+#pragma unroll
+        for (int k = 0; k < M::NMAG; k++) sld[k] = values[2 + M::sld_param(k)];
+        if constexpr (!MAG) {
+            M::contrasts(sld, con[0]);
+            xsw[0] = 1.0f;
+            active[0] = true;
+        } else {
+            double w[6];
+            spin_weights(values[2 + NP], values[3 + NP], w);
+            double smt, cmt, smp, cmp;
+            sincos(values[4 + NP] * DEG, &smt, &cmt);
+            sincos(values[5 + NP] * DEG, &smp, &cmp);
+            const bool q_ok = qx * qx + qy * qy > 1.e-16;   // kernel_iq.c:754
+#pragma unroll
+            for (int x = 0; x < NX; x++) {
+                active[x] = w[x] > 1.e-8;                    // kernel_iq.c:761
+                xsw[x] = (active[x] && q_ok) ? (float)w[x] : 0.0f;
+                double eff[M::NMAG];
+#pragma unroll
+                for (int k = 0; k < M::NMAG; k++) {
+                    const double *mv = values + 2 + NP + 4 + 3 * k;
+                    eff[k] = q_ok ? mag_sld(x, qx, qy, cmt, smt, cmp, smp, sld[k], mv[0], mv[1], mv[2]) : 0.0;
+                }
+                M::contrasts(eff, con[x]);
+            }
+        }
+    }
+
+    // sum over cross sections of w_xs * F_xs^2 for amplitude basis g
+    __device__ __forceinline__ float f2(const float *g) const
+    {
+        float s = 0.0f;
+#pragma unroll
+        for (int x = 0; x < NX; x++) {
+            if (active[x]) {
+                float F = 0.0f;
+#pragma unroll
+                for (int k = 0; k < M::NCON; k++) F = fmaf(con[x][k], g[k], F);
+                s = fmaf(xsw[x] * F, F, s);
+            }
+        }
+        return s;
+    }
+};
+
+// |F|^2 contribution of one 2D record at (qx, qy).
+template <class M>
+__device__ __forceinline__ void geom_at(const float *r, float qx, float qy, float qmag, float *g)
+{
+    float qab = qmag, qc = 0.0f;
+    if constexpr (M::ORIENTED) {
+        const float *rot = r + M::REC2D;
+        const float qa = fmaf(rot[0], qx, rot[1] * qy);
+        const float qb = fmaf(rot[2], qx, rot[3] * qy);
+        qc = fmaf(rot[4], qx, rot[5] * qy);
+        qab = sqrt_f(fmaf(qa, qa, qb * qb));
+    }
+    M::geom2d(qab, qc, qmag, r, g);
+}
+
+template <class M, bool MAG>
+__global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
+    int nq, const float2 *__restrict__ q32, const double *__restrict__ q64,
+    const double *__restrict__ values, const float4 *__restrict__ recs,
+    const int *__restrict__ npts_p, int nsplit, double *__restrict__ partial)
+{
+    constexpr int RV = rv2<M>();
+    __shared__ float4 tile[TILE_PTS * RV];
+    const int iq = blockIdx.x * EVAL_THREADS + threadIdx.x;
+    const int split = blockIdx.y;
+    int s0, s1;
+    split_range(*npts_p, nsplit, split, s0, s1);
+    const float2 qv = iq < nq ? q32[iq] : make_float2(0.0f, 0.0f);
+    const float qx = qv.x, qy = qv.y;
+    const float qmag = sqrt_f(fmaf(qx, qx, qy * qy));
+    Contrast<M, MAG> con;
+    con.setup(values, iq < nq ? q64[2 * iq] : 0.0, iq < nq ? q64[2 * iq + 1] : 0.0);
+
+    KSum acc;
+    for (int t0 = s0; t0 < s1; t0 += TILE_PTS) {
+        const int cnt = min(TILE_PTS, s1 - t0);
+        for (int i = threadIdx.x; i < cnt * RV; i += EVAL_THREADS) tile[i] = recs[(size_t)t0 * RV + i];
+        __syncthreads();
+        if (iq < nq) {
+            for (int j = 0; j < cnt; j++) {
+                float r[4 * RV];
+#pragma unroll
+                for (int v = 0; v < RV; v++) {
+                    const float4 x = tile[j * RV + v];
+                    r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
+                }
+                float g[M::NCON];
+                geom_at<M>(r, qx, qy, qmag, g);
+                acc.add(r[M::A2D] * con.f2(g));
+            }
+        }
+        __syncthreads();
+    }
+    if (iq < nq) partial[(size_t)iq * nsplit + split] = acc.value();
+}
+
+}  // namespace sas

@@ -54,27 +54,28 @@ class HipModel(KernelModel):
             raise ValueError("model %s has no hip kernel" % model_info.id)
         self.info = model_info
         self.dtype = np.dtype("d")
-        self._handle = None
-        self._quad = None
+        # one device handle per quadrature size; kernels keep using the
+        # handle they were made with, so none is freed before release()
+        self._handles = {}
 
     def _model_handle(self):
         quad = int(self.info.integration_size or 0)
-        if self._handle is None or quad != self._quad:
+        if quad not in self._handles:
             _ensure_device()
-            self.release()
             h = ct.c_void_p()
             _lib.check(_lib.lib().saship_model_create(self.info.kernel_id, quad, ct.byref(h)))
-            self._handle, self._quad = h, quad
-        return self._handle
+            self._handles[quad] = h
+        return self._handles[quad]
 
     def make_kernel(self, q_vectors):
         """Bind the model to *q_vectors* ([q] or [qx, qy])."""
         return HipKernel(self, q_vectors)
 
     def release(self):
-        if self._handle is not None:
-            _lib.lib().saship_model_release(self._handle)
-            self._handle = None
+        """Free the device tables (kernels made from this model must be released first)."""
+        for h in self._handles.values():
+            _lib.lib().saship_model_release(h)
+        self._handles = {}
 
     def __del__(self):
         try:
@@ -145,6 +146,128 @@ class HipKernel(Kernel):
     def release(self):
         if getattr(self, "_handle", None) is not None:
             _lib.lib().saship_kernel_release(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class HipBatch:
+    """
+    Many parameter sets, each with its own q points, evaluated in one
+    persistent launch (see saship_batch_call in include/saship.h).
+
+    *q_list* holds one ``[q]`` or ``[qx, qy]`` per set.  Calling the batch
+    with a list of ``(call_details, values)`` pairs, one per set, returns
+    the list of I(q); :meth:`Fq` returns the per-set Fq tuples.  Each set's
+    result equals what ``HipKernel`` gives for that set alone.  With
+    *use_graph* the device work of a call is captured once as a hipGraph and
+    replayed while the shapes stay the same (a fit loop).
+    """
+
+    def __init__(self, model, q_list, use_graph=False):
+        if not q_list:
+            raise ValueError("empty batch")
+        self._model = model
+        self.info = model.info
+        self.dim = "2d" if len(q_list[0]) == 2 else "1d"
+        if any((len(qv) == 2) != (self.dim == "2d") for qv in q_list):
+            raise ValueError("all sets of a batch must be 1D or all 2D")
+        inputs = [HipInput(qv) for qv in q_list]
+        self.nq = np.array([inp.nq for inp in inputs], "i4")
+        self.qoff = np.concatenate([[0], np.cumsum(self.nq)]).astype(np.int64)
+        q = np.concatenate([inp.q.ravel() for inp in inputs]) if self.qoff[-1] else np.zeros(1)
+        self._q = np.ascontiguousarray(q, "d")
+        mh = model._model_handle()
+        h = ct.c_void_p()
+        _lib.check(_lib.lib().saship_batch_create(
+            mh, len(q_list), self.nq.ctypes.data, self._q.ctypes.data,
+            1 if self.dim == "2d" else 0, ct.byref(h)))
+        self._handle = h
+        if use_graph:
+            self.set_options(use_graph=True)
+
+    @property
+    def nsets(self):
+        return len(self.nq)
+
+    def set_options(self, use_graph=None, pts_per_item=None, waves_per_cu=None):
+        """Change the launch shape; unspecified options keep their values."""
+        g, p, w = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        _lib.check(_lib.lib().saship_batch_get_options(self._handle, ct.byref(g), ct.byref(p), ct.byref(w)))
+        _lib.check(_lib.lib().saship_batch_set_options(
+            self._handle, int(g.value if use_graph is None else bool(use_graph)),
+            int(p.value if pts_per_item is None else pts_per_item),
+            int(w.value if waves_per_cu is None else waves_per_cu)))
+
+    def _pack(self, calls):
+        if len(calls) != self.nsets:
+            raise ValueError("expected %d (call_details, values) pairs, got %d" % (self.nsets, len(calls)))
+        det = np.concatenate([np.asarray(getattr(cd, "buffer", cd), "i4") for cd, _ in calls])
+        vals = [np.asarray(v, "d") for _, v in calls]
+        voff = np.concatenate([[0], np.cumsum([v.size for v in vals])]).astype(np.int64)
+        ndet = det.size//self.nsets
+        return np.ascontiguousarray(det), np.ascontiguousarray(np.concatenate(vals)), voff, ndet
+
+    def _kind(self, magnetic):
+        return KIND_IQ if self.dim == "1d" else (KIND_IMAGNETIC if magnetic else KIND_IQXY)
+
+    def raw(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0, want_result=True, want_iq=True):
+        """(raw accumulators of all sets back to back, I(q) of all sets back to back)."""
+        det, vals, voff, ndet = self._pack(calls)
+        nout = 2 if self.dim == "1d" else 1
+        result = np.empty(int(np.sum(nout*self.nq + 4)), "d") if want_result else None
+        iq = np.empty(int(self.qoff[-1]), "d") if want_iq else None
+        _lib.check(_lib.lib().saship_batch_call(
+            self._handle, self._kind(magnetic), det.ctypes.data, ndet, vals.ctypes.data, voff.ctypes.data,
+            float(cutoff), int(radius_effective_mode),
+            result.ctypes.data if want_result else None, iq.ctypes.data if want_iq else None))
+        return result, iq
+
+    def __call__(self, calls, cutoff=0.0, magnetic=False):
+        """List of I(q), one per set."""
+        _, iq = self.raw(calls, cutoff, magnetic, 0, want_result=False, want_iq=True)
+        return np.split(iq, self.qoff[1:-1])
+
+    def Fq(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0):
+        """List of (F1, F2, R_eff, V_shell, V_form/V_shell), one per set."""
+        result, _ = self.raw(calls, cutoff, magnetic, radius_effective_mode, want_result=True, want_iq=False)
+        nout = 2 if self.dim == "1d" else 1
+        out = []
+        off = 0
+        for nq in self.nq:
+            r = result[off:off + nout*nq + 4]
+            off += nout*nq + 4
+            total = r[nout*nq] or 1.0
+            shell = r[nout*nq + 2]/total or 1.0
+            F1 = r[1:nout*nq:nout]/total if nout == 2 else None
+            out.append((F1, r[0:nout*nq:nout]/total, r[nout*nq + 3]/total, shell,
+                        (r[nout*nq + 1]/total)/shell))
+        return out
+
+    def autotune(self, calls, cutoff=0.0, magnetic=False):
+        """Pick the fastest (pts_per_item, waves_per_cu) on these inputs."""
+        det, vals, voff, ndet = self._pack(calls)
+        p, w = ct.c_int32(), ct.c_int32()
+        _lib.check(_lib.lib().saship_batch_autotune(
+            self._handle, self._kind(magnetic), det.ctypes.data, ndet, vals.ctypes.data, voff.ctypes.data,
+            float(cutoff), ct.byref(p), ct.byref(w)))
+        return p.value, w.value
+
+    def set_timing(self, enable=True):
+        _lib.check(_lib.lib().saship_batch_set_timing(self._handle, 1 if enable else 0))
+
+    def get_timing(self):
+        ms, n = ct.c_double(), ct.c_int32()
+        _lib.check(_lib.lib().saship_batch_get_timing(self._handle, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+    def release(self):
+        if getattr(self, "_handle", None) is not None:
+            _lib.lib().saship_batch_release(self._handle)
             self._handle = None
 
     def __del__(self):

@@ -1,0 +1,160 @@
+"""
+GPU tests of the batched persistent path (saship_batch_*): every set of a
+batch must match the oracle to the GPU tolerance and the single-call path to
+FP32 summation order; graph replays must match eager calls.
+"""
+
+import numpy as np
+import pytest
+
+from gpu_helpers import RTOL, compare_raw, hip_model, masked_relerr
+from oracle.oracle import OracleKernel
+from sasmodels_amd import direct_model
+from sasmodels_amd.details import make_kernel_args
+from sasmodels_amd.kernelhip import HipBatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _pack(model, kernel_like, pars, dim):
+    mesh = direct_model.get_mesh(model.info, dict(pars), dim=dim)
+    det, val, mag = make_kernel_args(kernel_like, mesh)
+    return det, val, mag
+
+
+def _cs_cyl_sets(n, seed):
+    r = np.random.default_rng(seed)
+    sets = []
+    for _ in range(n):
+        nq = int(r.integers(64, 400))
+        q = np.logspace(-3, np.log10(0.5), nq)
+        pd = float(r.uniform(0, 0.2))
+        pars = dict(radius=r.uniform(10, 100), thickness=r.uniform(5, 30), length=r.uniform(50, 1000),
+                    sld_core=r.uniform(-1, 7), sld_shell=r.uniform(-1, 7), sld_solvent=r.uniform(-1, 7),
+                    radius_pd=pd, radius_pd_n=int(r.integers(1, 36)))
+        sets.append(([q], pars))
+    return sets
+
+
+@pytest.mark.parametrize("pts_per_item", [4, 16, 64])
+def test_batch_1d_matches_single_and_oracle(pts_per_item):
+    model = hip_model("core_shell_cylinder", 76)
+    sets = _cs_cyl_sets(24, 5 + pts_per_item)
+    batch = HipBatch(model, [qv for qv, _ in sets])
+    batch.set_options(pts_per_item=pts_per_item)
+    calls = []
+    for qv, pars in sets:
+        k = model.make_kernel(qv)
+        det, val, mag = _pack(model, k, pars, "1d")
+        calls.append((det, val))
+        k.release()
+    raw, iq = batch.raw(calls, 0.0, False, 1)
+    off = 0
+    for s, ((qv, pars), (det, val)) in enumerate(zip(sets, calls)):
+        nq = qv[0].size
+        r = raw[off:off + 2*nq + 4]
+        off += 2*nq + 4
+        k = model.make_kernel(qv)
+        k._call_kernel(det, val, 0.0, False, 1)
+        compare_raw(r, k.result, nq, 2, rtol=2e-6)          # same FP32 terms, other order
+        ora = OracleKernel(model.info, qv).raw(det, val, 0.0, False, 1)
+        compare_raw(r, ora, nq, 2)
+        want = OracleKernel(model.info, qv)(det, val, 0.0, False)
+        got = iq[batch.qoff[s]:batch.qoff[s + 1]]
+        assert masked_relerr(got, want) <= RTOL
+        k.release()
+    batch.release()
+
+
+def test_batch_2d_magnetic_sets():
+    model = hip_model("core_shell_sphere")
+    r = np.random.default_rng(17)
+    x = np.linspace(-0.3, 0.3, 33)
+    qx, qy = [a.ravel() for a in np.meshgrid(x, x)]
+    sets = []
+    for _ in range(6):
+        pars = dict(radius=r.uniform(20, 80), thickness=r.uniform(5, 25),
+                    sld_core_M0=r.uniform(0, 3), sld_shell_M0=r.uniform(0, 3), sld_solvent_M0=r.uniform(0, 3),
+                    sld_core_mtheta=r.uniform(0, 90), sld_core_mphi=r.uniform(0, 360),
+                    sld_shell_mtheta=r.uniform(0, 90), sld_shell_mphi=r.uniform(0, 360),
+                    up_frac_i=r.uniform(0, 1), up_frac_f=r.uniform(0, 1), up_theta=r.uniform(0, 90),
+                    radius_pd=0.1, radius_pd_n=int(r.integers(1, 8)))
+        sets.append(pars)
+    batch = HipBatch(model, [[qx, qy]]*len(sets))
+    calls = []
+    k = model.make_kernel([qx, qy])
+    for pars in sets:
+        det, val, mag = _pack(model, k, pars, "2d")
+        assert mag
+        calls.append((det, val))
+    raw, iq = batch.raw(calls, 0.0, True, 0)
+    nq = qx.size
+    for s, (det, val) in enumerate(calls):
+        r_s = raw[s*(nq + 4):(s + 1)*(nq + 4)]
+        ora = OracleKernel(model.info, [qx, qy]).raw(det, val, 0.0, True, 0)
+        compare_raw(r_s, ora, nq, 1)
+    k.release()
+    batch.release()
+
+
+def test_graph_replay_matches_eager():
+    """Replays of one captured graph, with other GPU work and synchronisation
+    in between, each against the eager result for its own inputs."""
+    model = hip_model("cylinder", 76)
+    sets = [([np.logspace(-3, 0, 200 + 37*i)], dict(radius=20 + 3*i, length=300 + 20*i, radius_pd=0.1,
+                                                     radius_pd_n=7, length_pd=0.1, length_pd_n=5))
+            for i in range(5)]
+    qlist = [qv for qv, _ in sets]
+    eager = HipBatch(model, qlist)
+    graph = HipBatch(model, qlist, use_graph=True)
+    k = model.make_kernel(qlist[0])
+    for step in range(4):
+        calls = []
+        for qv, pars in sets:
+            p = dict(pars, radius=pars["radius"]*(1 + 0.05*step))
+            det, val, _ = _pack(model, k, p, "1d")
+            calls.append((det, val))
+        want = eager(calls)
+        got = graph(calls)
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b)
+        k._call_kernel(calls[0][0], calls[0][1], 0.0, False, 0)   # eager single-call work between replays
+    k.release()
+    eager.release()
+    graph.release()
+
+
+def test_autotune_and_timing():
+    model = hip_model("cylinder", 76)
+    sets = [([np.logspace(-3, 0, 128)], dict(radius=20, length=400, radius_pd=0.1, radius_pd_n=9))]*8
+    batch = HipBatch(model, [qv for qv, _ in sets])
+    k = model.make_kernel(sets[0][0])
+    calls = [_pack(model, k, pars, "1d")[:2] for _, pars in sets]
+    before = batch(calls)
+    p, w = batch.autotune(calls)
+    assert p >= 1 and w >= 4
+    after = batch(calls)
+    for a, b in zip(after, before):
+        np.testing.assert_allclose(a, b, rtol=2e-6)
+    batch.set_timing(True)
+    batch(calls)
+    ms, n = batch.get_timing()
+    assert n == 1 and ms > 0
+    k.release()
+    batch.release()
+
+
+def test_batch_rejects_bad_inputs():
+    model = hip_model("sphere")
+    batch = HipBatch(model, [[np.array([0.1, 0.2])]])
+    k = model.make_kernel([np.array([0.1, 0.2])])
+    det, val, _ = _pack(model, k, dict(radius_pd=0.1, radius_pd_n=5), "1d")
+    bad = det.buffer.copy()
+    bad[1] = 99                          # pd_length beyond num_weights
+    from sasmodels_amd._lib import HipError
+    with pytest.raises(HipError):
+        batch.raw([(bad, val)])
+    with pytest.raises(ValueError):
+        batch.raw([(det, val), (det, val)])
+    k.release()
+    batch.release()

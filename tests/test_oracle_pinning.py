@@ -1,0 +1,234 @@
+"""
+Pins the FP64 oracle (oracle/sasoracle.c) before it is trusted as the checker.
+
+1. Against the reference's own model and special-function sources compiled
+   unmodified into oracle/_ref (skipped where the reference tree was absent
+   at build time): special functions bit for bit, form factors to the
+   precision of the reference's printed Gauss tables.
+2. Against the reference's known answers (model files and
+   direct_model.test_simple_interface), fed through the host packing.
+3. Against mathematics that does not share the oracle's formulas: scipy
+   Bessel functions, numpy's Gauss-Legendre rule, closed forms.
+"""
+
+import ctypes as ct
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+from kat_tables import DIRECT_MODEL, MODEL_KATS, is_near, q_vectors
+from oracle import oracle
+from sasmodels_amd import direct_model, generate, modelinfo
+
+P = ct.POINTER(ct.c_double)
+NPARS = {"sphere": 3, "cylinder": 6, "ellipsoid": 6, "core_shell_cylinder": 8, "core_shell_sphere": 5}
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
+def _need_ref(name):
+    lib = oracle.ref_lib(name)
+    if lib is None:
+        pytest.skip("oracle/_ref/%s.so not built (reference tree absent)" % name)
+    return lib
+
+
+# ---------------------------------------------------------------- 1. reference sources
+@pytest.mark.parametrize("fn,refname,lib", [
+    ("sasoracle_j1", "ref_j1", "ref_cylinder"),
+    ("sasoracle_2j1x_x", "ref_2j1x_x", "ref_cylinder"),
+    ("sasoracle_sph_j1c", "ref_sph_j1c", "ref_sphere"),
+    ("sasoracle_sinx_x", "ref_sinx_x", "ref_sphere"),
+])
+def test_special_functions_bit_exact(fn, refname, lib):
+    ref = _need_ref(lib)
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-12, 12, 4000), 10**rng.uniform(-4, 4, 4000),
+                        [0.0, 0.1, -0.1, 5.0, -5.0, 1e-300]])
+    f_o, f_r = getattr(oracle.lib(), fn), getattr(ref, refname)
+    mismatches = [v for v in x if f_o(v) != f_r(v)]
+    assert not mismatches, mismatches[:5]
+
+
+def _rand_pars(model, rng):
+    p = rng.uniform(1, 400, NPARS[model])
+    if model in ("sphere", "cylinder", "ellipsoid"):
+        p[0:2] = rng.uniform(-2, 8, 2)
+    elif model == "core_shell_cylinder":
+        p[0:3] = rng.uniform(-2, 8, 3)
+    else:
+        p[2:5] = rng.uniform(-2, 8, 3)
+    return np.ascontiguousarray(p)
+
+
+@pytest.mark.parametrize("model,refname,quad", [
+    ("sphere", "ref_sphere", 0),
+    ("cylinder", "ref_cylinder", 0), ("cylinder", "ref_cylinder_g76", 76),
+    ("ellipsoid", "ref_ellipsoid", 0), ("ellipsoid", "ref_ellipsoid_g76", 76),
+    ("core_shell_cylinder", "ref_core_shell_cylinder", 0),
+    ("core_shell_cylinder", "ref_core_shell_cylinder_g76", 76),
+    ("core_shell_sphere", "ref_core_shell_sphere", 0),
+])
+def test_form_factors_match_reference_sources(model, refname, quad):
+    ref = _need_ref(refname)
+    L = oracle.lib()
+    mid = oracle.MODEL_IDS[model]
+    rng = np.random.default_rng(11 + quad)
+    worst = 0.0
+    for _ in range(400):
+        p = _rand_pars(model, rng)
+        q = 10**rng.uniform(-3, 0)
+        a1, a2, b1, b2 = (ct.c_double() for _ in range(4))
+        L.sasoracle_fq(mid, quad, q, _ptr(p), ct.byref(a1), ct.byref(a2))
+        ref.ref_fq(q, _ptr(p), ct.byref(b1), ct.byref(b2))
+        # F^2 relative; F relative to the RMS amplitude sqrt(F^2)
+        worst = max(worst, abs(a2.value - b2.value)/abs(b2.value),
+                    abs(a1.value - b1.value)/math.sqrt(abs(b2.value))*1e-2)
+        assert L.sasoracle_form_volume(mid, _ptr(p)) == ref.ref_form_volume(_ptr(p))
+        for mode in range(1, 8):
+            assert L.sasoracle_radius_effective(mid, mode, _ptr(p)) == ref.ref_reff(mode, _ptr(p))
+        if hasattr(ref, "ref_iqac"):
+            qab, qc = rng.uniform(0, 0.5, 2)
+            u, v = L.sasoracle_iqac(mid, qab, qc, _ptr(p)), ref.ref_iqac(qab, qc, _ptr(p))
+            worst = max(worst, abs(u - v)/abs(v) if v else abs(u))
+    # bit-exact without quadrature; the reference's printed Gauss tables
+    # (15-16 digits, 500/5000-point weights to ~4e-10) bound the rest
+    limit = 0.0 if model in ("sphere", "core_shell_sphere") else 1e-9
+    assert worst <= limit, worst
+
+
+@pytest.mark.parametrize("n", [20, 76])
+def test_gauss_tables_match_reference(n):
+    ref = _need_ref("ref_cylinder")
+    z, w, zr, wr = (np.zeros(n) for _ in range(4))
+    oracle.lib().sasoracle_gauss(n, _ptr(z), _ptr(w))
+    assert ref.ref_gauss(n, _ptr(zr), _ptr(wr)) == n
+    np.testing.assert_allclose(z, zr, atol=1e-15)
+    np.testing.assert_allclose(w, wr, rtol=5e-15)
+
+
+# ---------------------------------------------------------------- 2. reference known answers
+def _kat_cases():
+    for model, kats in MODEL_KATS.items():
+        for k, t in enumerate(kats):
+            yield pytest.param(model, t, id="%s-%d" % (model, k))
+
+
+@pytest.mark.parametrize("model,test", list(_kat_cases()))
+def test_reference_kats(model, test):
+    info = modelinfo.make_model_info(model)
+    qv, xs = q_vectors(test[1])
+    kernel = oracle.OracleKernel(info, qv)
+    pars = dict(test[0])
+    if len(test) == 3:
+        actual = direct_model.call_kernel(kernel, pars)
+        ys = test[2] if isinstance(test[2], list) else [test[2]]
+        for y, a in zip(ys, actual):
+            assert is_near(y, a, 5), (y, a)
+    else:
+        F1, F2, R, V, Vr = direct_model.call_Fq(kernel, pars)
+        if test[2] is not None:
+            assert is_near(test[2], F1[0], 5)
+        if test[3] is not None:
+            assert is_near(test[3], F2[0], 5)
+        assert is_near(test[4], R, 5) and is_near(test[5], V, 5) and is_near(test[6], Vr, 5)
+
+
+@pytest.mark.parametrize("case", range(len(DIRECT_MODEL)))
+def test_direct_model_kats_full_precision(case):
+    """The reference prints these to full double precision; sphere has no
+    quadrature, so the oracle reproduces them to the last bit or two."""
+    model, pars, x, y = DIRECT_MODEL[case]
+    info = modelinfo.make_model_info(model)
+    qv, _ = q_vectors(x)
+    actual = direct_model.call_kernel(oracle.OracleKernel(info, qv), dict(pars))
+    np.testing.assert_allclose(actual, y, rtol=4e-16, atol=0)
+
+
+def test_magnetic_kat_full_precision():
+    """All six spin cross sections (sphere.py:115-121) to the last bit."""
+    pars, x, y = MODEL_KATS["sphere"][7]
+    info = modelinfo.make_model_info("sphere")
+    qv, _ = q_vectors(x)
+    actual = direct_model.call_kernel(oracle.OracleKernel(info, qv), dict(pars))
+    np.testing.assert_allclose(actual, y, rtol=4e-16, atol=0)
+
+
+# ---------------------------------------------------------------- 3. mathematics
+def test_j1_against_scipy():
+    x = np.concatenate([np.linspace(-30, 30, 3001), 10**np.linspace(-6, 3, 500)])
+    got = np.array([oracle.lib().sasoracle_j1(v) for v in x])
+    np.testing.assert_allclose(got, special.j1(x), rtol=0, atol=2e-15)
+    got2 = np.array([oracle.lib().sasoracle_2j1x_x(v) for v in x[x != 0]])
+    np.testing.assert_allclose(got2, 2*special.j1(x[x != 0])/x[x != 0], rtol=1e-12, atol=1e-15)
+
+
+def test_sph_j1c_against_scipy():
+    x = np.concatenate([np.linspace(1e-4, 50, 5000), [0.05, 0.0999, 0.1001]])
+    got = np.array([oracle.lib().sasoracle_sph_j1c(v) for v in x])
+    want = 3*special.spherical_jn(1, x)/x
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-14)
+
+
+@pytest.mark.parametrize("n", [20, 76, 150, 500])
+def test_gauss_against_leggauss(n):
+    z, w = np.zeros(n), np.zeros(n)
+    assert oracle.lib().sasoracle_gauss(n, _ptr(z), _ptr(w)) == n
+    zl, wl = np.polynomial.legendre.leggauss(n)
+    np.testing.assert_allclose(z, zl, atol=2e-16)
+    np.testing.assert_allclose(w, wl, rtol=0, atol=3e-14)   # leggauss end weights carry ~1e-14 abs error
+    # exact on polynomials of degree <= 2n-1
+    for deg in (0, 2, 10, 2*n - 2):
+        np.testing.assert_allclose(np.sum(w*z**deg), 2.0/(deg + 1), rtol=1e-12)
+
+
+def test_sphere_closed_form_and_forward_limit():
+    """Monodisperse sphere: I(q) = scale*V*drho^2*1e-4 * [3 j1(qR)/(qR)]^2 + bkg."""
+    info = modelinfo.make_model_info("sphere")
+    q = np.logspace(-5, 0, 50)
+    pars = dict(radius=75, sld=3.5, sld_solvent=1.25, scale=0.3, background=0.01)
+    I = direct_model.call_kernel(oracle.OracleKernel(info, [q]), pars)
+    R = 75.0
+    V = 4*np.pi/3*R**3
+    x = q*R
+    want = 0.3*V*(3.5 - 1.25)**2*1e-4*(3*special.spherical_jn(1, x)/x)**2 + 0.01
+    np.testing.assert_allclose(I, want, rtol=1e-12)
+
+
+def test_cylinder_orientation_average_against_scipy_quad():
+    """<F^2> of a cylinder by adaptive integration over the polar angle."""
+    from scipy.integrate import quad
+    info = modelinfo.make_model_info("cylinder")
+    generate.set_integration_size(info, 150)
+    q = np.array([0.003, 0.02, 0.07])
+    R, L, drho = 25.0, 180.0, 2.5
+    F1, F2, _, V, _ = direct_model.call_Fq(oracle.OracleKernel(info, [q]),
+                                           dict(radius=R, length=L, sld=3.5, sld_solvent=1.0))
+    for qi, f2 in zip(q, F2):
+        def amp(a):
+            x, y = qi*R*np.sin(a), qi*L/2*np.cos(a)
+            jc = 2*special.j1(x)/x if x else 1.0
+            return jc*np.sinc(y/np.pi)
+        num, _ = quad(lambda a: amp(a)**2*np.sin(a), 0, np.pi/2, epsabs=0, epsrel=1e-13, limit=200)
+        want = 1e-4*(drho*V)**2*num
+        assert abs(f2/want - 1) < 1e-10
+
+
+def test_ellipsoid_orientation_average_against_scipy_quad():
+    from scipy.integrate import quad
+    info = modelinfo.make_model_info("ellipsoid")
+    generate.set_integration_size(info, 150)
+    q = np.array([0.004, 0.03])
+    rp, re = 30.0, 90.0
+    _, F2, _, V, _ = direct_model.call_Fq(oracle.OracleKernel(info, [q]),
+                                          dict(radius_polar=rp, radius_equatorial=re, sld=2, sld_solvent=1))
+    for qi, f2 in zip(q, F2):
+        def f(u):
+            x = qi*np.sqrt((re*np.sqrt(1 - u*u))**2 + (rp*u)**2)
+            return (3*special.spherical_jn(1, x)/x)**2
+        num, _ = quad(f, 0, 1, epsabs=0, epsrel=1e-13)
+        assert abs(f2/(1e-4*V**2*num) - 1) < 1e-10
