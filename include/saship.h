@@ -153,6 +153,36 @@ int saship_batch_set_timing(saship_batch *batch, int32_t enable);
 int saship_batch_get_timing(saship_batch *batch, double *eval_ms, int32_t *ncalls);
 int saship_batch_release(saship_batch *batch);
 
+/*
+ * Learned surrogate: MLP 6 -> 4x256 SiLU -> 1, trained and served with BF16
+ * MFMA (FP32 accumulate).  No reference counterpart: for callers that need
+ * many approximate curves of the polydisperse cylinder family quickly, with
+ * the plain kernels as fallback and judge.  All pointers are device memory;
+ * `stream` is a hipStream_t (NULL: the default stream).
+ *
+ * theta: the flat FP32 parameter vector (n_params floats: W1[6][256], b1,
+ * W2[256][256], b2, W3, b3, W4, b4, w5[256], b5); pack: n_pack BF16 values
+ * refreshed by saship_mlp_pack from theta; work: work_bytes for `batch` rows.
+ */
+int saship_mlp_sizes(int32_t *n_params, int32_t *n_pack, int64_t *work_bytes, int32_t batch);
+int saship_mlp_pack(const float *theta, void *pack, void *stream);
+/* y[batch] = network(X[batch][6]) */
+int saship_mlp_forward(const float *theta, const void *pack, const float *X, int32_t batch,
+                       void *work, float *y, void *stream);
+/* Forward, mean squared error against target, and the gradient of the loss
+ * with respect to theta into grad (n_params floats); loss_parts receives
+ * ceil(batch/256) partial sums of squared residuals (may be NULL). */
+int saship_mlp_train_step(const float *theta, const void *pack, const float *X, const float *target,
+                          int32_t batch, void *work, float *grad, float *loss_parts, void *stream);
+/* Fused inference over ncurves x nq rows: curves[ncurves][5] = radius,
+ * length, radius_pd, length_pd, contrast; logq[nq] = log10 q; norm[12] =
+ * feature means then standard deviations.  out_iq = 0 writes the network
+ * output; 1 writes I = 1e-4 contrast^2 pi R^2 L exp(y).  grid <= 0 picks the
+ * persistent grid size. */
+int saship_mlp_infer(const float *theta, const void *pack, const float *curves, int32_t ncurves,
+                     const float *logq, int32_t nq, const float *norm, int32_t out_iq, float *y,
+                     int32_t grid, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,15 @@ SIGNATURES = {
     "saship_batch_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_batch_get_timing": (ct.c_int, [ct.c_void_p, c_double_p, c_int32_p]),
     "saship_batch_release": (ct.c_int, [ct.c_void_p]),
+    "saship_mlp_sizes": (ct.c_int, [c_int32_p, c_int32_p, ct.POINTER(ct.c_int64), ct.c_int32]),
+    "saship_mlp_pack": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "saship_mlp_forward": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_void_p,
+                                      ct.c_void_p, ct.c_void_p]),
+    "saship_mlp_train_step": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int32,
+                                         ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "saship_mlp_infer": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_void_p,
+                                    ct.c_int32, ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
+                                    ct.c_void_p]),
 }
 
 

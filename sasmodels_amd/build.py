@@ -10,7 +10,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = [os.path.join(HERE, "csrc", name) for name in ("saship.hip", "batch.hip")]
+SOURCES = [os.path.join(HERE, "csrc", name) for name in ("saship.hip", "batch.hip", "mlp.hip")]
 OUTPUT = os.path.join(HERE, "libsaship.so")
 ARCH = os.environ.get("SASHIP_ARCH", "gfx950")
 

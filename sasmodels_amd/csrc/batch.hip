@@ -108,11 +108,11 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a)
         const int nq = (int)(a.qoff[s + 1] - q0);
         const int iq = tile * QTILE + lane;
         const bool ok = iq < nq;
-        const float qx = ok ? a.q32[2 * (q0 + iq)] : 0.0f;
-        const float qy = ok ? a.q32[2 * (q0 + iq) + 1] : 0.0f;
-        const float qmag = sqrt_f(fmaf(qx, qx, qy * qy));
+        const double qx = ok ? a.q64[2 * (q0 + iq)] : 0.0;
+        const double qy = ok ? a.q64[2 * (q0 + iq) + 1] : 0.0;
+        const double qmag = sqrt(qx * qx + qy * qy);
         Contrast<M, MAG> con;
-        con.setup(a.values + a.voff[s], ok ? a.q64[2 * (q0 + iq)] : 0.0, ok ? a.q64[2 * (q0 + iq) + 1] : 0.0);
+        con.setup(a.values + a.voff[s], qx, qy);
         const int npts = a.npts[s];
         const int c0 = min(npts, chunk * a.pts_per_item);
         const int c1 = min(npts, c0 + a.pts_per_item);

@@ -28,7 +28,7 @@ constexpr double DEG = 0.017453292519943295;   // pi/180
 
 // Record widths in float4 units.
 template <class M> constexpr int rv1() { return (M::REC1D + 3) / 4; }
-template <class M> constexpr int rv2() { return (M::REC2D + (M::ORIENTED ? 8 : 0) + 3) / 4; }
+template <class M> constexpr int rv2() { return (M::REC2D + (M::ORIENTED ? 12 : 0) + 3) / 4; }
 template <class M, int KIND> constexpr int rvk() { return KIND == 0 ? rv1<M>() : rv2<M>(); }
 
 template <class M> constexpr int nvalues()
@@ -157,13 +157,13 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
                         const double j11 = cdt, j12 = sdp * sdt, j13 = -sdt * cdp;
                         const double j22 = cdp, j23 = sdp;
                         const double j31 = sdt, j32 = -sdp * cdt, j33 = cdp * cdt;
-                        float *rot = rec + M::REC2D;
-                        rot[0] = (float)(j11 * v11 + j12 * v21 + j13 * v31);
-                        rot[1] = (float)(j11 * v12 + j12 * v22 + j13 * v32);
-                        rot[2] = (float)(j22 * v21 + j23 * v31);
-                        rot[3] = (float)(j22 * v22 + j23 * v32);
-                        rot[4] = (float)(j31 * v11 + j32 * v21 + j33 * v31);
-                        rot[5] = (float)(j31 * v12 + j32 * v22 + j33 * v32);
+                        float *rot = rec + M::REC2D;      // six FP64 entries
+                        put_d(rot, 0, j11 * v11 + j12 * v21 + j13 * v31);
+                        put_d(rot, 2, j11 * v12 + j12 * v22 + j13 * v32);
+                        put_d(rot, 4, j22 * v21 + j23 * v31);
+                        put_d(rot, 6, j22 * v22 + j23 * v32);
+                        put_d(rot, 8, j31 * v11 + j32 * v21 + j33 * v31);
+                        put_d(rot, 10, j31 * v12 + j32 * v22 + j33 * v32);
                     }
                 }
             }
@@ -361,24 +361,24 @@ struct Contrast {
     }
 };
 
-// |F|^2 contribution of one 2D record at (qx, qy).
+// Amplitude basis of one 2D record at (qx, qy), with q rotated into the
+// particle frame in FP64.
 template <class M>
-__device__ __forceinline__ void geom_at(const float *r, float qx, float qy, float qmag, float *g)
+__device__ __forceinline__ void geom_at(const float *r, double qx, double qy, double qmag, float *g)
 {
-    float qab = qmag, qc = 0.0f;
+    QRot q{qmag, 0.0, 0.0, qmag};
     if constexpr (M::ORIENTED) {
         const float *rot = r + M::REC2D;
-        const float qa = fmaf(rot[0], qx, rot[1] * qy);
-        const float qb = fmaf(rot[2], qx, rot[3] * qy);
-        qc = fmaf(rot[4], qx, rot[5] * qy);
-        qab = sqrt_f(fmaf(qa, qa, qb * qb));
+        q.qa = fma(get_d(rot, 0), qx, get_d(rot, 2) * qy);
+        q.qb = fma(get_d(rot, 4), qx, get_d(rot, 6) * qy);
+        q.qc = fma(get_d(rot, 8), qx, get_d(rot, 10) * qy);
     }
-    M::geom2d(qab, qc, qmag, r, g);
+    M::geom2d(q, r, g);
 }
 
 template <class M, bool MAG>
 __global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
-    int nq, const float2 *__restrict__ q32, const double *__restrict__ q64,
+    int nq, const double *__restrict__ q64,
     const double *__restrict__ values, const float4 *__restrict__ recs,
     const int *__restrict__ npts_p, int nsplit, double *__restrict__ partial)
 {
@@ -388,11 +388,10 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
     const int split = blockIdx.y;
     int s0, s1;
     split_range(*npts_p, nsplit, split, s0, s1);
-    const float2 qv = iq < nq ? q32[iq] : make_float2(0.0f, 0.0f);
-    const float qx = qv.x, qy = qv.y;
-    const float qmag = sqrt_f(fmaf(qx, qx, qy * qy));
+    const double qx = iq < nq ? q64[2 * iq] : 0.0, qy = iq < nq ? q64[2 * iq + 1] : 0.0;
+    const double qmag = sqrt(qx * qx + qy * qy);
     Contrast<M, MAG> con;
-    con.setup(values, iq < nq ? q64[2 * iq] : 0.0, iq < nq ? q64[2 * iq + 1] : 0.0);
+    con.setup(values, qx, qy);
 
     KSum acc;
     for (int t0 = s0; t0 < s1; t0 += TILE_PTS) {

@@ -6,8 +6,9 @@
 //   rec1d(p, w, r)    FP32 record for the 1D <F>,<F^2> kernel, with the mesh
 //                     weight and every per-point constant folded in
 //   eval1d(q, r, quad, f1, f2)   weighted F and F^2 contributions at q
-//   rec2d(p, w, r)    geometry + per-layer volumes + w*1e-4 for 2D kernels
-//   geom2d(qab, qc, q, r, g)     amplitude basis g[k]; F = sum_k con[k]*g[k]
+//   rec2d(p, w, r)    2D record: lengths that enter a phase in FP64 (two
+//                     words each), per-layer volumes and w*1e-4 in FP32
+//   geom2d(qr, r, g)  amplitude basis g[k] at the rotated q (FP64); F = sum_k con[k]*g[k]
 //   contrasts(sld, con)          con[k] from the SLD slot values (FP64)
 // The amplitude of every model is linear in its SLDs, which lets the
 // magnetic kernel evaluate the geometry once and reuse it for each spin
@@ -32,6 +33,11 @@ constexpr double FOUR_PI_3_D = 4.18879020478639;
 // theta = z*pi/4 + pi/4 and the tables hold sin, cos and w*sin; for the
 // ellipsoid u = z/2 + 1/2 and the tables hold u^2 and w.  Adaptive
 // selection (adaptive.c:11290-11297) picks one of four tables per lane.
+// q in the particle frame (FP64): qa, qb across the axis, qc along it; q = |q|.
+struct QRot {
+    double qa, qb, qc, q;
+};
+
 struct Quad {
     const float *a;     // sin(theta) | u^2
     const float *b;     // cos(theta) | unused
@@ -63,7 +69,7 @@ struct Sphere {
     static constexpr int ID = 0, NPARS = 3, NMAG = 2, MAX_PD = 1, NCON = 1;
     static constexpr bool ORIENTED = false;
     static constexpr int REC1D = 4, REC2D = 4;
-    static constexpr int A2D = 2;   // slot of w*1e-4 in the 2D record
+    static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // sld, sld_solvent
     __device__ static bool valid(const double *) { return true; }
     __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[2]); }
@@ -86,14 +92,13 @@ struct Sphere {
     }
     __device__ static void rec2d(const double *p, double w, float *r)
     {
-        r[0] = (float)p[2];
-        r[1] = (float)volume(p);
-        r[2] = (float)(w * 1e-4);
-        r[3] = 0.0f;
+        put_d(r, 0, p[2]);
+        r[2] = (float)volume(p);
+        r[3] = (float)(w * 1e-4);
     }
-    __device__ static void geom2d(float, float, float q, const float *r, float *g)
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
     {
-        g[0] = r[1] * sph_j1c_f(q * r[0]);
+        g[0] = r[2] * sph_j1c_d(q.q * get_d(r, 0));
     }
     SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
@@ -102,8 +107,8 @@ struct Sphere {
 struct Cylinder {
     static constexpr int ID = 1, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
     static constexpr bool ORIENTED = true;
-    static constexpr int REC1D = 4, REC2D = 4;
-    static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
+    static constexpr int REC1D = 4, REC2D = 8;
+    static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
     __device__ static bool valid(const double *p) { return p[2] >= 0.0 && p[3] >= 0.0; }
     __device__ static double volume(const double *p) { return PI_D * p[2] * p[2] * p[3]; }
@@ -148,14 +153,15 @@ struct Cylinder {
     }
     __device__ static void rec2d(const double *p, double w, float *r)
     {
-        r[0] = (float)p[2];
-        r[1] = (float)(0.5 * p[3]);
-        r[2] = (float)volume(p);
-        r[3] = (float)(w * 1e-4);
+        put_d(r, 0, p[2]);
+        put_d(r, 2, 0.5 * p[3]);
+        r[4] = (float)volume(p);
+        r[5] = (float)(w * 1e-4);
     }
-    __device__ static void geom2d(float qab, float qc, float, const float *r, float *g)
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
     {
-        g[0] = r[2] * j1c_f(qab * r[0]) * sinc_prod(qc, r[1]);
+        const double qab = sqrt(q.qa * q.qa + q.qb * q.qb);
+        g[0] = r[4] * j1c_d(qab * get_d(r, 0)) * sinc_d(q.qc * get_d(r, 2));
     }
     SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
@@ -164,8 +170,8 @@ struct Cylinder {
 struct Ellipsoid {
     static constexpr int ID = 2, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
     static constexpr bool ORIENTED = true;
-    static constexpr int REC1D = 4, REC2D = 4;
-    static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
+    static constexpr int REC1D = 4, REC2D = 8;
+    static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
     __device__ static bool valid(const double *) { return true; }
     __device__ static double volume(const double *p) { return FOUR_PI_3_D * p[2] * p[3] * p[3]; }
@@ -222,15 +228,15 @@ struct Ellipsoid {
     }
     __device__ static void rec2d(const double *p, double w, float *r)
     {
-        r[0] = (float)(p[3] * p[3]);
-        r[1] = (float)(p[2] * p[2]);
-        r[2] = (float)volume(p);
-        r[3] = (float)(w * 1e-4);
+        put_d(r, 0, p[3] * p[3]);
+        put_d(r, 2, p[2] * p[2]);
+        r[4] = (float)volume(p);
+        r[5] = (float)(w * 1e-4);
     }
-    __device__ static void geom2d(float qab, float qc, float, const float *r, float *g)
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
     {
-        const float qr = sqrt_f(fmaf(r[0], qab * qab, r[1] * (qc * qc)));
-        g[0] = r[2] * sph_j1c_f(qr);
+        const double qr = sqrt(get_d(r, 0) * (q.qa * q.qa + q.qb * q.qb) + get_d(r, 2) * (q.qc * q.qc));
+        g[0] = r[4] * sph_j1c_d(qr);
     }
     SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
@@ -239,8 +245,8 @@ struct Ellipsoid {
 struct CoreShellCylinder {
     static constexpr int ID = 3, NPARS = 8, NMAG = 3, MAX_PD = 5, NCON = 2;
     static constexpr bool ORIENTED = true;
-    static constexpr int REC1D = 8, REC2D = 8;
-    static constexpr int A2D = 6;   // slot of w*1e-4 in the 2D record
+    static constexpr int REC1D = 8, REC2D = 12;
+    static constexpr int A2D = 10;  // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // core, shell, solvent
     __device__ static bool valid(const double *) { return true; }
     __device__ static double vol3(double r, double t, double l) { return PI_D * (r + t) * (r + t) * (l + 2.0 * t); }
@@ -299,19 +305,20 @@ struct CoreShellCylinder {
     __device__ static void rec2d(const double *p, double w, float *r)
     {
         const double rr = p[3], t = p[4], l = p[5];
-        r[0] = (float)rr;
-        r[1] = (float)(0.5 * l);
-        r[2] = (float)(rr + t);
-        r[3] = (float)(0.5 * l + t);
-        r[4] = (float)vol3(rr, 0.0, l);
-        r[5] = (float)vol3(rr, t, l);
-        r[6] = (float)(w * 1e-4);
-        r[7] = 0.0f;
+        put_d(r, 0, rr);
+        put_d(r, 2, 0.5 * l);
+        put_d(r, 4, rr + t);
+        put_d(r, 6, 0.5 * l + t);
+        r[8] = (float)vol3(rr, 0.0, l);
+        r[9] = (float)vol3(rr, t, l);
+        r[10] = (float)(w * 1e-4);
+        r[11] = 0.0f;
     }
-    __device__ static void geom2d(float qab, float qc, float, const float *r, float *g)
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
     {
-        g[0] = r[4] * sinc_prod(qc, r[1]) * j1c_f(qab * r[0]);
-        g[1] = r[5] * sinc_prod(qc, r[3]) * j1c_f(qab * r[2]);
+        const double qab = sqrt(q.qa * q.qa + q.qb * q.qb);
+        g[0] = r[8] * sinc_d(q.qc * get_d(r, 2)) * j1c_d(qab * get_d(r, 0));
+        g[1] = r[9] * sinc_d(q.qc * get_d(r, 6)) * j1c_d(qab * get_d(r, 4));
     }
     SAS_HD static void contrasts(const double *sld, float *con)
     {
@@ -325,7 +332,7 @@ struct CoreShellSphere {
     static constexpr int ID = 4, NPARS = 5, NMAG = 3, MAX_PD = 2, NCON = 2;
     static constexpr bool ORIENTED = false;
     static constexpr int REC1D = 8, REC2D = 8;
-    static constexpr int A2D = 4;   // slot of w*1e-4 in the 2D record
+    static constexpr int A2D = 6;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return 2 + k; }        // core, shell, solvent
     __device__ static bool valid(const double *) { return true; }
     __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[0] + p[1]); }
@@ -352,17 +359,17 @@ struct CoreShellSphere {
     __device__ static void rec2d(const double *p, double w, float *r)
     {
         const double rc = p[0], rs = p[0] + p[1];
-        r[0] = (float)rc;
-        r[1] = (float)rs;
-        r[2] = (float)(FOUR_PI_3_D * cube_d(rc));
-        r[3] = (float)(FOUR_PI_3_D * cube_d(rs));
-        r[4] = (float)(w * 1e-4);
-        r[5] = r[6] = r[7] = 0.0f;
+        put_d(r, 0, rc);
+        put_d(r, 2, rs);
+        r[4] = (float)(FOUR_PI_3_D * cube_d(rc));
+        r[5] = (float)(FOUR_PI_3_D * cube_d(rs));
+        r[6] = (float)(w * 1e-4);
+        r[7] = 0.0f;
     }
-    __device__ static void geom2d(float, float, float q, const float *r, float *g)
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
     {
-        g[0] = r[2] * sph_j1c_f(q * r[0]);
-        g[1] = r[3] * sph_j1c_f(q * r[1]);
+        g[0] = r[4] * sph_j1c_d(q.q * get_d(r, 0));
+        g[1] = r[5] * sph_j1c_d(q.q * get_d(r, 2));
     }
     SAS_HD static void contrasts(const double *sld, float *con)
     {

@@ -125,9 +125,9 @@ int launch(saship_kernel *k, int kind, const int32_t *d_det, const double *d_val
         else
             k_fq1d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q32, k->d_recs, npts, nsplit, k->m->quad, k->d_partial);
     } else if (kind == 1) {
-        k_iq2d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, (const float2 *)k->q32, k->q64, d_val, k->d_recs, npts, nsplit, k->d_partial);
+        k_iq2d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q64, d_val, k->d_recs, npts, nsplit, k->d_partial);
     } else {
-        k_iq2d<M, true><<<grid, EVAL_THREADS, 0, k->st>>>(nq, (const float2 *)k->q32, k->q64, d_val, k->d_recs, npts, nsplit, k->d_partial);
+        k_iq2d<M, true><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q64, d_val, k->d_recs, npts, nsplit, k->d_partial);
     }
     HIP_TRY(hipGetLastError());
     if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));

@@ -14,6 +14,10 @@
 //   sinc_f        sin(x)/x, 1 at x == 0 (sas_sinx_x, kernel_header.c:177).
 //   sph_j1c_f     3 j1(x)/x: Taylor series below 1, direct form above
 //                 (the same two-branch scheme as sas_3j1x_x.c:46-57).
+//   *_d           the same functions for arguments known in FP64: the
+//                 reduction by pi/2 runs in FP64, the polynomials in FP32, so
+//                 the phase error does not grow with x (2D kernels, where a
+//                 single orientation puts pixels next to form-factor zeros).
 //   j1c_f         2 J1(x)/x from the Cephes single-precision j1f
 //                 approximations (S. L. Moshier, Cephes Math Library 2.8):
 //                 a polynomial on |x| <= 2 and the modulus/phase asymptotic
@@ -194,7 +198,79 @@ SAS_HD float j1c_f(float x)
     return 2.0f * j1 * q;   // even function: sign of x cancels
 }
 
-// FP64 versions used where the kernels need double (volumes, weights).
+// ---- arguments known in FP64 (the 2D kernels): FP64 reduction by pi/2,
+// FP32 polynomials and FP32 for every factor that does not carry the phase.
+SAS_HD void sincos_d(double x, float &s, float &c)
+{
+    const double kd = rint(x * 0x1.45f306dc9c883p-1);
+    double rd = fma(-kd, 0x1.921fb54442d18p+0, x);
+    rd = fma(-kd, 0x1.1a62633145c07p-54, rd);
+    float sr, cr;
+    sincos_poly((float)rd, sr, cr);
+    quadrant((int)(long long)kd & 3, sr, cr, s, c);
+}
+
+SAS_HD float sinc_d(double x)
+{
+    if (x == 0.0) return 1.0f;
+    float s, c;
+    sincos_d(x, s, c);
+    return s * rcp_f((float)x);
+}
+
+SAS_HD float sph_j1c_d(double x)
+{
+    const float xf = (float)x;
+    if (fabsf(xf) < 1.0f) return sph_j1c_f(xf);
+    float s, c;
+    sincos_d(x, s, c);
+    const float inv = rcp_f(xf);
+    return 3.0f * (s * inv - c) * inv * inv;
+}
+
+SAS_HD float j1c_d(double x)
+{
+    const double axd = fabs(x);
+    const float ax = (float)axd;
+    if (ax <= 2.0f) return j1c_f(ax);
+    const float q = rcp_f(ax);
+    const float w = sqrt_f(q);
+    float m = fmaf(q, 6.913942741265801e-2f, -2.284801500053359e-1f);
+    m = fmaf(q, m, 3.138238455499697e-1f);
+    m = fmaf(q, m, -2.102302420403875e-1f);
+    m = fmaf(q, m, 5.435364690523026e-3f);
+    m = fmaf(q, m, 1.493389585089498e-1f);
+    m = fmaf(q, m, 4.976029650847191e-6f);
+    m = fmaf(q, m, 7.978845453073848e-1f);
+    const float amp = w * m;
+    const float q2 = q * q;
+    float ph = fmaf(q2, -4.497014141919556e+1f, 5.073465654089319e+1f);
+    ph = fmaf(q2, ph, -2.485774108720340e+1f);
+    ph = fmaf(q2, ph, 7.222973196770240e+0f);
+    ph = fmaf(q2, ph, -1.544842782180211e+0f);
+    ph = fmaf(q2, ph, 3.503787691653334e-1f);
+    ph = fmaf(q2, ph, -1.637986776941202e-1f);
+    ph = fmaf(q2, ph, 3.749989509080821e-1f);
+    const float xn = q * ph;
+    const float xn2 = xn * xn;
+    const float sxn = xn * fmaf(xn2, fmaf(xn2, 8.33333333e-3f, -1.66666667e-1f), 1.0f);
+    const float cxn = fmaf(xn2, fmaf(xn2, fmaf(xn2, -1.38888889e-3f, 4.16666667e-2f), -0.5f), 1.0f);
+    float sx, cx;
+    sincos_d(axd, sx, cx);
+    const float j1 = amp * 0.70710678118654752f * (sxn * (sx + cx) + cxn * (sx - cx));
+    return 2.0f * j1 * q;
+}
+
+// FP64 helpers used where the kernels need double (volumes, weights).
 SAS_HD double cube_d(double x) { return x * x * x; }
+
+// A double stored in two consecutive words of an FP32 record.
+SAS_HD double get_d(const float *r, int i)
+{
+    double d;
+    __builtin_memcpy(&d, r + i, sizeof d);
+    return d;
+}
+SAS_HD void put_d(float *r, int i, double v) { __builtin_memcpy(r + i, &v, sizeof v); }
 
 }  // namespace sas

@@ -1,0 +1,103 @@
+"""
+GPU tests of the surrogate: the BF16 MFMA forward and backward kernels
+against the FP64 MLP oracle, inference against the training forward pass,
+training on plain-kernel data, and the plain-kernel gate.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sur():
+    from sasmodels_amd.surrogate import CylinderSurrogate
+    return CylinderSurrogate(seed=5)
+
+
+def _batch(rng, B):
+    return rng.normal(size=(B, 6)).astype(np.float32), rng.normal(size=B).astype(np.float32)
+
+
+def test_forward_matches_oracle(sur):
+    torch = sur.torch
+    rng = np.random.default_rng(0)
+    X, _ = _batch(rng, 300)
+    y = sur.forward(torch.as_tensor(X, device="cuda")).cpu().numpy()
+    theta = sur.theta.cpu().numpy()
+    y_emul, _ = omlp.forward(theta, X, emulate_bf16=True)
+    y_exact, _ = omlp.forward(theta, X, emulate_bf16=False)
+    scale = np.abs(y_exact).max()
+    assert np.max(np.abs(y - y_emul)) <= 2e-3*scale        # same roundings, FP32 sums
+    assert np.max(np.abs(y - y_exact)) <= 5e-2*scale       # BF16 against exact FP64
+
+
+def test_gradient_matches_oracle(sur):
+    """Per parameter block: relative L2 error against the BF16-emulating
+    FP64 oracle, and direction (cosine) against the exact FP64 gradient."""
+    torch = sur.torch
+    rng = np.random.default_rng(1)
+    X, t = _batch(rng, 512)
+    loss, grad = sur.loss_and_grad(torch.as_tensor(X, device="cuda"), torch.as_tensor(t, device="cuda"))
+    grad = grad.cpu().numpy().astype(np.float64)
+    theta = sur.theta.cpu().numpy()
+    loss_e, g_emul = omlp.backward(theta, X, t, emulate_bf16=True)
+    _, g_exact = omlp.backward(theta, X, t, emulate_bf16=False)
+    assert abs(float(loss) - loss_e) <= 1e-3*loss_e
+    off = 0
+    for name, shape in omlp.SIZES:
+        n = int(np.prod(shape))
+        a, b, c = grad[off:off + n], g_emul[off:off + n], g_exact[off:off + n]
+        off += n
+        rel = np.linalg.norm(a - b)/max(np.linalg.norm(b), 1e-30)
+        cos = np.dot(a, c)/max(np.linalg.norm(a)*np.linalg.norm(c), 1e-30)
+        print(name, "rel %.2e cos %.6f" % (rel, cos))
+        assert rel <= 2e-2, (name, rel)
+        assert cos >= 0.999, (name, cos)
+
+
+def test_inference_kernel_equals_training_forward(sur):
+    torch = sur.torch
+    rng = np.random.default_rng(2)
+    from sasmodels_amd.surrogate import features, sample_box
+    curves = sample_box(37, rng)
+    lq = np.linspace(-3, 0, 61)
+    raw = features(curves, lq)
+    sur.set_normalisation(raw)
+    X = torch.as_tensor(sur.standardise(raw), device="cuda")
+    y_train = sur.forward(X).cpu().numpy()
+    y_inf = sur.predict(curves, lq, out_iq=False).cpu().numpy()
+    np.testing.assert_allclose(y_inf, y_train, rtol=1e-4, atol=1e-4*np.abs(y_train).max())
+    I = sur.predict(curves, lq, out_iq=True).cpu().numpy().reshape(37, 61)
+    v0 = np.pi*curves[:, 0]**2*curves[:, 1]
+    np.testing.assert_allclose(I, 1e-4*curves[:, 4, None]**2*v0[:, None]*np.exp(y_inf.reshape(37, 61)),
+                               rtol=2e-5)
+
+
+def test_training_fits_plain_kernel_data_and_gate_falls_back(sur):
+    """A short training on plain-kernel curves must cut the loss by orders of
+    magnitude; the gate must return plain-kernel curves where the
+    surrogate is off or the input is outside the box."""
+    torch = sur.torch
+    from sasmodels_amd.surrogate import features, target
+    curves, lq, Iq = sur.generate(ncurves=256, nq=128, seed=3)
+    raw = features(curves, lq)
+    sur.init_params(7)
+    sur.set_normalisation(raw)
+    X = torch.as_tensor(sur.standardise(raw), device="cuda")
+    t = torch.as_tensor(target(curves, Iq).ravel().astype(np.float32), device="cuda")
+    hist = sur.train(X, t, epochs=30, batch=2048, lr=2e-3)
+    print("loss", hist[0], hist[-1])
+    assert hist[-1] < 0.02*hist[0]
+    # gate: one curve outside the box, threshold 0 forces every curve to the plain kernel
+    test = np.vstack([curves[:3], [[500.0, 100.0, 0.1, 0.1, 1.0]]])
+    I, bad, err = sur.evaluate(test, lq, threshold=1e-3)
+    assert bad[3]
+    plain = np.array(sur.plain.iq(test, [10.0**lq]*4))
+    np.testing.assert_allclose(I[bad], plain[bad], rtol=1e-12)
+    I0, bad0, _ = sur.evaluate(test, lq, threshold=0.0)
+    assert bad0.all()
+    np.testing.assert_allclose(I0, plain, rtol=1e-12)
