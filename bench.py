@@ -206,6 +206,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a kernel of known traffic for "
+                         "scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -215,6 +218,15 @@ def main():
     from oracle import oracle
     from oracle.oracle import OracleKernel
     from sasmodels_amd import core, direct_model, generate
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy*(1 << 20)//4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     threads = min(16, len(os.sched_getaffinity(0)))
     oracle.set_threads(threads)
