@@ -97,6 +97,13 @@ int saship_kernel_call(saship_kernel *kernel, int32_t kind,
                        double cutoff, int32_t radius_effective_mode,
                        int32_t pd_start, int32_t pd_stop, double *result);
 
+/* Evaluation strategy: with separable != 0 (the default) a cylinder 1D call
+ * with a fixed Gauss rule and a mesh over radius and length only is
+ * evaluated as bilinear forms of the radius and length factors (same sums,
+ * far fewer special-function evaluations); 0 forces the generic mesh kernel.
+ * No reference counterpart. */
+int saship_kernel_set_path(saship_kernel *kernel, int32_t separable);
+
 /* Kernel timing: when enabled, the device time of the evaluation kernel of
  * every call is summed (HIP events on the handle's stream).  No reference
  * counterpart; used by the benchmark. */

@@ -31,6 +31,7 @@ SIGNATURES = {
     "saship_kernel_call": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
                                       ct.c_void_p, ct.c_int32, ct.c_double, ct.c_int32,
                                       ct.c_int32, ct.c_int32, ct.c_void_p]),
+    "saship_kernel_set_path": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_kernel_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_kernel_get_timing": (ct.c_int, [ct.c_void_p, c_double_p, c_int32_p]),
     "saship_kernel_release": (ct.c_int, [ct.c_void_p]),
@@ -61,6 +62,29 @@ class HipError(RuntimeError):
     """An error reported by libsaship."""
 
 
+def _share_hip_runtime():
+    """
+    Keep one HIP runtime per process.  PyTorch-ROCm wheels carry their own
+    libamdhip64.so (same soname, libamdhip64.so.7, as the system one this
+    library is linked against), and torch loads it by file name, so
+    loading libsaship.so first and torch second would put two HIP (and
+    HSA) runtimes in the process; torch then sees no device and stream
+    handles cannot cross between the two.  When torch is installed, load
+    its runtime first (by path, RTLD_GLOBAL): libsaship.so then binds to
+    it by soname, and torch's own later load finds the same file.
+    """
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ct.CDLL(path, mode=ct.RTLD_GLOBAL)
+
+
 def lib():
     """Load libsaship.so once (raises OSError if it is missing)."""
     global _lib
@@ -69,6 +93,7 @@ def lib():
             if not os.path.exists(LIBPATH):
                 raise OSError("%s not built; run __graft_entry__.build() or "
                               "python -m sasmodels_amd.build" % LIBPATH)
+            _share_hip_runtime()
             handle = ct.CDLL(LIBPATH)
             for name, (restype, argtypes) in SIGNATURES.items():
                 fn = getattr(handle, name)

@@ -39,6 +39,7 @@ def build_library(verbose=False):
     log = "\n".join(line for line in (proc.stdout + proc.stderr).splitlines()
                     if "packed-fp32-ops" not in line)
     if proc.returncode != 0:
+        sys.stderr.write(log + "\n")
         raise subprocess.CalledProcessError(proc.returncode, cmd, output=log)
     if verbose and log.strip():
         print(log)

@@ -75,11 +75,13 @@ struct saship_kernel {
     double *d_partial = nullptr;
     size_t partial_cap = 0;
     double *d_result = nullptr;
+    char *d_sep = nullptr;          // separable cylinder tables (R, H, A, B, dims)
     char *h_stage = nullptr;
     size_t stage_cap = 0;
     double *h_result = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing = false;
+    bool separable = true;           // use the separable cylinder path when it applies
     double t_ms = 0.0;
     int ncalls = 0;
     std::mutex mu;
@@ -136,6 +138,76 @@ int launch(saship_kernel *k, int kind, const int32_t *d_det, const double *d_val
     k_finalize<<<fblocks, 256, 0, k->st>>>(nq, nout, nsplit, k->d_partial, sums, accumulate, k->d_result);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// Table layout of the separable path inside one allocation (one set).
+sas::sep::Tables sep_tables(char *base, int nlp, size_t set_stride = sas::sep::NRMAX)
+{
+    using sas::sep::NRMAX;
+    sas::sep::Tables t;
+    (void)set_stride;
+    t.R = (float *)base;
+    t.H = t.R + NRMAX;
+    t.A = t.H + NRMAX;
+    t.B = t.A + (size_t)NRMAX * nlp;
+    t.dims = (int *)(t.B + (size_t)NRMAX * nlp);
+    return t;
+}
+
+size_t sep_bytes(int nlp) { return sizeof(float) * (2 * sas::sep::NRMAX + 2 * (size_t)sas::sep::NRMAX * nlp) + 64; }
+
+template <int NLP>
+int launch_sep_nlp(saship_kernel *k, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
+                   int pd_stop, double cutoff, int reff_mode, int accumulate, int kr, int kl)
+{
+    using namespace sas;
+    const int nq = k->nq, nnode = k->m->quad.n;
+    if (!k->d_sep) HIP_TRY(hipMalloc((void **)&k->d_sep, sep_bytes(64)));
+    sep::Tables t = sep_tables(k->d_sep, NLP);
+    double *sums = (double *)(k->d_meta + 64);
+    ExpandArgs ea{d_det, d_val, nullptr, nullptr, n_details, pd_start, pd_stop, cutoff, reff_mode,
+                  nullptr, (int *)k->d_meta, sums, nullptr};
+    sep::k_sep_expand<0><<<1, EXPAND_THREADS, 0, k->st>>>(ea, t, NLP, kr, kl, sep::NRMAX);
+    HIP_TRY(hipGetLastError());
+    if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * 2 * (size_t)nq * nnode)) return e;
+    const dim3 grid(std::max(1, (nq + 63) / 64), (nnode + 3) / 4);
+    if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
+    sep::k_fq1d_sep<NLP><<<grid, 256, 0, k->st>>>(nq, k->q32, t, k->m->quad, k->d_partial);
+    HIP_TRY(hipGetLastError());
+    if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
+    const int fblocks = std::max(1, (2 * nq + 3) / 4);
+    k_finalize<<<fblocks, 256, 0, k->st>>>(nq, 2, nnode, k->d_partial, sums, accumulate, k->d_result);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Mesh levels of a cylinder details buffer that hold radius and length, if
+// every other level has a single point (the separable case).
+bool cylinder_separable(const int32_t *details, int &kr, int &kl, int &nr, int &nl)
+{
+    const int mp = model_maxpd(1);
+    kr = kl = -1;
+    for (int k = 0; k < mp; k++) {
+        const int par = details[k], len = details[mp + k];
+        if (par == 2) kr = k;
+        else if (par == 3) kl = k;
+        else if (len != 1) return false;
+    }
+    if (kr < 0 || kl < 0) return false;
+    nr = details[mp + kr];
+    nl = details[mp + kl];
+    return nr >= 1 && nr <= sas::sep::NRMAX && nl >= 1 && nl <= 64;
+}
+
+int launch_sep(saship_kernel *k, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
+               int pd_stop, double cutoff, int reff_mode, int accumulate, int kr, int kl, int nl)
+{
+#define SEP_CASE(N) \
+    if (nl <= N) return launch_sep_nlp<N>(k, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, kr, kl);
+    SEP_CASE(4) SEP_CASE(8) SEP_CASE(12) SEP_CASE(16) SEP_CASE(20) SEP_CASE(24) SEP_CASE(28)
+    SEP_CASE(32) SEP_CASE(36) SEP_CASE(40) SEP_CASE(48) SEP_CASE(64)
+#undef SEP_CASE
+    return fail(SASHIP_EINVAL, "length grid too large for the separable path");
 }
 
 }  // namespace
@@ -306,6 +378,22 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     const int32_t *d_det = (const int32_t *)(k->d_args + doff);
     const double *d_val = (const double *)k->d_args;
     int rc;
+    int kr, kl, nr, nl;
+    if (id == 1 && kind == 0 && k->m->quad_n > 0 && k->separable &&
+        cylinder_separable(details, kr, kl, nr, nl)) {
+        rc = launch_sep(k, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, kr, kl, nl);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(k->h_result, k->d_result, rbytes, hipMemcpyDeviceToHost, k->st));
+        HIP_TRY(hipStreamSynchronize(k->st));
+        memcpy(result, k->h_result, rbytes);
+        if (k->timing) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, k->ev0, k->ev1));
+            k->t_ms += ms;
+            k->ncalls++;
+        }
+        return 0;
+    }
     switch (id) {
     case 0: rc = launch<sas::Sphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     case 1: rc = launch<sas::Cylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
@@ -323,6 +411,14 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
         k->t_ms += ms;
         k->ncalls++;
     }
+    return 0;
+}
+
+int saship_kernel_set_path(saship_kernel *k, int32_t separable)
+{
+    if (!k) return fail(SASHIP_EINVAL, "kernel is NULL");
+    std::lock_guard<std::mutex> lock(k->mu);
+    k->separable = separable != 0;
     return 0;
 }
 
@@ -357,6 +453,7 @@ int saship_kernel_release(saship_kernel *k)
     if (k->d_meta) (void)hipFree(k->d_meta);
     if (k->d_partial) (void)hipFree(k->d_partial);
     if (k->d_result) (void)hipFree(k->d_result);
+    if (k->d_sep) (void)hipFree(k->d_sep);
     if (k->h_stage) (void)hipHostFree(k->h_stage);
     if (k->h_result) (void)hipHostFree(k->h_result);
     if (k->ev0) (void)hipEventDestroy(k->ev0);

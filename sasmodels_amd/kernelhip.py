@@ -133,6 +133,10 @@ class HipKernel(Kernel):
             vals.ctypes.data, vals.size, float(cutoff), int(radius_effective_mode),
             int(pd_start), stop, self.result.ctypes.data))
 
+    def set_separable(self, enable=True):
+        """Allow (default) or forbid the separable cylinder evaluation."""
+        _lib.check(_lib.lib().saship_kernel_set_path(self._handle, 1 if enable else 0))
+
     def set_timing(self, enable=True):
         """Sum the device time of the evaluation kernel over later calls."""
         _lib.check(_lib.lib().saship_kernel_set_timing(self._handle, 1 if enable else 0))

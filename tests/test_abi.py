@@ -42,3 +42,21 @@ def test_error_reporting_without_device():
         _lib.check(lib.saship_kernel_release(None) - 1)
     n = _lib.device_count()
     assert n >= 0
+
+
+def test_one_hip_runtime_per_process():
+    """Loading libsaship.so and then torch leaves a single HIP runtime mapped
+    (two runtimes would hide the device from torch and break stream sharing)."""
+    import subprocess
+    import sys
+    code = (
+        "import re\n"
+        "from sasmodels_amd import _lib\n"
+        "_lib.lib()\n"
+        "import torch\n"
+        "maps = open('/proc/self/maps').read()\n"
+        "print(len(set(re.findall(r'\\S*libamdhip64\\S*', maps))))\n"
+    )
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "1"
