@@ -5,19 +5,24 @@ Headline workload (the cylinder configuration the metric is quoted on):
 cylinder 1D, 1000 log-spaced q in [1e-3, 1] 1/A, radius 20 A and length
 400 A each with lognormal polydispersity 0.1 (35 x 35 mesh points, 3
 sigma), sld 4 / solvent 1, orientation average by fixed Gauss-76, FP32
-kernels with compensated sums and FP64 weight sums.
+kernels with compensated sums and FP64 weight sums, cutoff 0.
 
 Paths, one JSON line:
-  batched    (value)  a step evaluates BATCH curves of the workload in one
-                      persistent launch; every curve has its own radius and
-                      length (jittered by step and curve), all packed
-                      before the timed region
-  plain               one curve per reference-shaped call
-  surrogate           the trained MLP on 4096 curves x 512 q drawn from its
-                      box, against the plain kernel on the same curves
-roofline: reference-formula FLOPs of the batched evaluation kernel over its
-device time (HIP events on the batch's stream).  cpu_baseline: the FP64
-oracle on the host cores, on a bounded sample of the workload.
+  batched          (value)  a step evaluates BATCH curves of the workload in
+                            one persistent launch (separable cylinder path);
+                            every curve has its own radius and length
+                            (jittered by step and curve), all packed before
+                            the timed region
+  batched_cutoff            the same at cutoff 1e-5 (pruned mesh corners)
+  batched_generic           the same on the generic mesh kernel
+  plain, plain_generic      one curve per reference-shaped call
+  surrogate                 the trained MLP on 4096 curves x 512 q drawn from
+                            its box, against the plain kernel on those curves
+roofline: FLOPs of the separable evaluation (counted with the reference's
+formulas for J1 and sin, see separable_flops_cylinder) over the device time
+of k_batch_sep (HIP events on the batch's stream); traffic from the counter
+summary in profiles/ when present.  cpu_baseline: the FP64 oracle on the
+host cores, on a bounded sample of the workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
 """
@@ -66,6 +71,27 @@ def reference_flops_cylinder_fq(q, radii):
     return total
 
 
+def separable_flops_cylinder(q, radii, nl):
+    """
+    FLOPs of the separable evaluation of one curve with the same convention
+    (add, sub, mul, div, sin = 1, FMA = 2; J1 with the reference's formula,
+    29 for |x| <= 5, 67 beyond), prefix mode with every row complete:
+      per (q, node): 6 + 8 per length point + per radius point (10 + J1)
+    (q cos/sin products and weighted sums; per length point the sinc and
+    two prefix sums; per radius point the J1 argument, 2J1/x and two
+    products into F and F^2).  *radii* holds the distinct radii.
+    """
+    z, _ = np.polynomial.legendre.leggauss(QUAD)
+    s = np.sin(z*np.pi/4 + np.pi/4)
+    R = np.asarray(radii, "d")
+    total = 0.0
+    for qi in q:
+        x = qi*R[:, None]*s[None, :]
+        small = np.count_nonzero(np.abs(x) <= 5.0)
+        total += s.size*(6 + 8*nl) + small*(10 + 29) + (x.size - small)*(10 + 67)
+    return total
+
+
 def _kernel_like(info):
     return type("K", (), {"info": info, "dtype": np.dtype("d")})()
 
@@ -88,28 +114,53 @@ def masked_relerr(a, b):
     return np.abs(a[sel] - b[sel])/b[sel]
 
 
-def cpu_baseline(details, values, threads, budget_s=15.0):
-    """Oracle (FP64 C, OpenMP over q) on a q subsample sized to ~budget_s."""
+def err_stats(rel):
+    rel = np.concatenate([np.ravel(r) for r in rel])
+    return {"max": float(rel.max()), "median": float(np.median(rel)),
+            "p98": float(np.percentile(rel, 98)), "rms": float(np.sqrt(np.mean(rel**2)))}
+
+
+def accuracy(out, refs):
+    """Relative error of GPU curves against the FP64 oracle curves (pairs in order)."""
+    return err_stats([masked_relerr(np.asarray(o), r) for o, r in zip(out, refs)])
+
+
+def cpu_baseline(info, jobs, threads, budget_s=12.0):
+    """
+    The CPU leg: the FP64 oracle (C, OpenMP over q) on whole workload curves
+    -- *jobs* is a list of (details, values, cutoff) -- in order until
+    budget_s of host time is spent, and on one thread on a q subsample.
+    Returns the baseline record and the oracle's I(q) of the curves it ran,
+    which are the FP64 references for the accuracy figures.
+    """
     from oracle import oracle
+    from oracle.oracle import OracleKernel
     oracle.set_threads(threads)
-    probe = C2_Q[::100]
-    t0 = time.perf_counter()
-    oracle.raw_call("cylinder", 0, QUAD, probe, details.buffer, values)
-    per_q = max(time.perf_counter() - t0, 1e-3)/probe.size
-    nq = int(min(C2_Q.size, max(10, budget_s/per_q)))
-    sample = C2_Q[np.linspace(0, C2_Q.size - 1, nq).astype(int)]
-    t0 = time.perf_counter()
-    oracle.raw_call("cylinder", 0, QUAD, sample, details.buffer, values)
+    ok = OracleKernel(info, [C2_Q])
+    refs, t0 = [], time.perf_counter()
+    for det, val, cutoff in jobs:
+        if time.perf_counter() - t0 >= budget_s:
+            break
+        refs.append(ok(det, val, cutoff, False))
     dt = time.perf_counter() - t0
-    curve_s = dt*C2_Q.size/sample.size
-    return {"value": 1.0/curve_s, "unit": "curves/s", "cores": threads, "kind": "port",
-            "sample": "oracle/sasoracle.c FP64 on %d of the 1000 q points of one workload curve "
-                      "(full 35x35 mesh, Gauss-76), %d OpenMP threads, %.1f s, scaled to a whole curve"
-                      % (sample.size, threads, dt)}
+    oracle.set_threads(1)
+    det, val, _ = jobs[0]
+    sample = C2_Q[::25]
+    t1 = time.perf_counter()
+    oracle.raw_call("cylinder", 0, QUAD, sample, det.buffer, val)
+    dt1 = (time.perf_counter() - t1)*C2_Q.size/sample.size
+    oracle.set_threads(threads)
+    rec = {"value": len(refs)/dt, "unit": "curves/s", "cores": threads, "kind": "port",
+           "sample": "oracle/sasoracle.c FP64, %d whole workload curves (1000 q, 35x35 mesh, Gauss-76) "
+                     "in %.1f s on %d OpenMP threads" % (len(refs), dt, threads),
+           "single_thread_curves_per_s": 1.0/dt1,
+           "single_thread_sample": "1 thread, 40 of the 1000 q of one curve, scaled to a whole curve"}
+    return rec, refs
 
 
-def bench_plain(model, info, steps, warmup):
+def bench_plain(model, info, steps, warmup, separable=True):
     kernel = model.make_kernel([C2_Q])
+    kernel.set_separable(separable)
     calls = c2_calls(info, steps + warmup, offset=1000)
     for det, val in calls[:warmup]:
         kernel(det, val, 0.0, False)
@@ -123,26 +174,28 @@ def bench_plain(model, info, steps, warmup):
     return {"curves_per_s": steps/dt, "ms_per_curve": dt/steps*1e3, "eval_kernel_ms": ms/max(n, 1)}
 
 
-def bench_batched(model, info, steps, warmup, batch_size):
+def bench_batched(model, info, steps, warmup, batch_size, separable=True, cutoff=0.0):
     from sasmodels_amd.kernelhip import HipBatch
     batch = HipBatch(model, [[C2_Q]]*batch_size, use_graph=True)
+    batch.set_separable(separable)
     groups = [c2_calls(info, batch_size, offset=g*batch_size) for g in range(steps + warmup)]
-    pts, waves = batch.autotune(groups[0])
+    pts, waves = batch.autotune(groups[0], cutoff)
     for calls in groups[:warmup]:
-        batch(calls)
+        batch(calls, cutoff)
     t0 = time.perf_counter()
     for calls in groups[warmup:]:
-        out = batch(calls)
+        out = batch(calls, cutoff)
     dt = time.perf_counter() - t0
     # device time of the evaluation kernel (eager launches with events)
     batch.set_timing(True)
     for calls in groups[warmup:warmup + min(steps, 10)]:
-        batch(calls)
+        batch(calls, cutoff)
     ms, n = batch.get_timing()
     batch.set_timing(False)
-    first = batch(groups[0])
+    first = batch(groups[0], cutoff)
     batch.release()
     return ({"curves_per_s": steps*batch_size/dt, "ms_per_step": dt/steps*1e3, "batch": batch_size,
+             "cutoff": cutoff, "separable": separable,
              "pts_per_item": pts, "waves_per_cu": waves, "graph": True,
              "eval_kernel_ms": ms/max(n, 1)}, groups[0], first)
 
@@ -215,8 +268,6 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    from oracle import oracle
-    from oracle.oracle import OracleKernel
     from sasmodels_amd import core, direct_model, generate
 
     if args.calibrate_copy:
@@ -229,24 +280,41 @@ def main():
         del src, dst
 
     threads = min(16, len(os.sched_getaffinity(0)))
-    oracle.set_threads(threads)
     info = core.load_model_info("cylinder")
     generate.set_integration_size(info, QUAD)
     model = core.build_model(info, platform="hip")
 
     batched, calls0, out0 = bench_batched(model, info, args.steps, args.warmup, args.batch)
+    short = max(2, min(args.steps, 10))
+    cut, ccalls, cout = bench_batched(model, info, args.steps, args.warmup, args.batch, cutoff=1e-5)
+    generic, gcalls, gout = bench_batched(model, info, short, 1, args.batch, separable=False)
     plain = bench_plain(model, info, args.steps, args.warmup)
-
-    # accuracy of the timed workload against the FP64 oracle (first curve of the first batch)
-    det0, val0 = calls0[0]
-    I_ref = OracleKernel(info, [C2_Q])(det0, val0, 0.0, False)
-    rel = masked_relerr(np.asarray(out0[0]), I_ref)
+    plain_generic = bench_plain(model, info, short, 1, separable=False)
 
     mesh = direct_model.get_mesh(info, C2_PARS, dim="1d")
-    radii = np.repeat(mesh[4][1], len(mesh[5][1]))
-    flops_curve = reference_flops_cylinder_fq(C2_Q, radii)
+    radii, lengths = mesh[4][1], mesh[5][1]
+    npts = radii.size*lengths.size
+    flops_curve = separable_flops_cylinder(C2_Q, radii, lengths.size)
     flops = flops_curve*args.batch
     achieved = flops/(batched["eval_kernel_ms"]*1e-3)/1e12
+    ref_flops = reference_flops_cylinder_fq(C2_Q, np.repeat(radii, lengths.size))*args.batch
+    generic["roofline_tflops_reference_formula"] = ref_flops/(generic["eval_kernel_ms"]*1e-3)/1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_batch_sep_cylinder_c2_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # CPU leg: oracle timing on whole curves; its curves are the FP64 references
+    baseline, acc = None, {"max": None, "median": None}
+    if not args.no_cpu:
+        jobs = [(d, v, 1e-5) for d, v in ccalls[:4]] + [(d, v, 0.0) for d, v in gcalls[:4]] + \
+               [(d, v, 0.0) for d, v in calls0]
+        baseline, refs = cpu_baseline(info, jobs, threads)
+        cut["accuracy_vs_fp64"] = accuracy(cout[:4], refs[:4])
+        generic["accuracy_vs_fp64"] = accuracy(gout[:4], refs[4:8])
+        batched["accuracy_vs_fp64"] = acc = accuracy(out0, refs[8:])
+        batched["accuracy_curves"] = len(refs) - 8
 
     line = {
         "metric": METRIC,
@@ -262,21 +330,25 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "cylinder 1D: 1000 log q in [1e-3,1], radius 20 / length 400 lognormal pd 0.1 "
-                               "(35x35 mesh), sld 4/1, Gauss-76; %d curves per batched launch" % args.batch,
-                   "mesh_points": int(radii.size), "nq": int(C2_Q.size), "quadrature": QUAD},
-        "max_rel_err_vs_fp64": float(rel.max()),
-        "median_rel_err_vs_fp64": float(np.median(rel)),
-        "roofline": {"bound": "valu", "kernel": "k_batch_eval_1d<Cylinder,fixed>", "achieved": achieved,
+                               "(35x35 mesh), sld 4/1, Gauss-76, cutoff 0; %d curves per batched launch"
+                               % args.batch,
+                   "mesh_points": int(npts), "nq": int(C2_Q.size), "quadrature": QUAD},
+        "max_rel_err_vs_fp64": acc["max"],
+        "median_rel_err_vs_fp64": acc["median"],
+        "qpd_evals_per_s": batched["curves_per_s"]*C2_Q.size*npts,
+        "roofline": {"bound": "valu", "kernel": "k_batch_sep", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved/PEAK_FP32_TFLOPS,
                      "flops_per_launch": flops, "flops_per_curve": flops_curve,
-                     "avg_kernel_ms": batched["eval_kernel_ms"], "traffic": None},
-        "paths": {"batched": batched, "plain": plain},
+                     "avg_kernel_ms": batched["eval_kernel_ms"], "traffic": traffic,
+                     "reference_formula_tflops": ref_flops/(batched["eval_kernel_ms"]*1e-3)/1e12},
+        "paths": {"batched": batched, "batched_cutoff": cut, "batched_generic": generic,
+                  "plain": plain, "plain_generic": plain_generic},
     }
     if not args.no_surrogate:
         line["paths"]["surrogate"] = bench_surrogate()
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(det0, val0, threads)
-        line["speedup_vs_cpu"] = line["value"]/line["cpu_baseline"]["value"]
+    if baseline:
+        line["cpu_baseline"] = baseline
+        line["speedup_vs_cpu"] = line["value"]/baseline["value"]
     model.release()
     print(json.dumps(line))
 

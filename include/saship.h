@@ -141,12 +141,18 @@ int saship_batch_call(saship_batch *batch, int32_t kind,
                       double *result, double *iq);
 
 /* Launch shape: use_graph replays a captured hipGraph for repeated calls of
- * the same shape; pts_per_item is the mesh chunk of one work item;
- * waves_per_cu sizes the persistent grid. */
+ * the same shape; pts_per_item is the mesh chunk of one work item (on the
+ * separable cylinder path: Gauss nodes per item); waves_per_cu sizes the
+ * persistent grid. */
 int saship_batch_set_options(saship_batch *batch, int32_t use_graph,
                              int32_t pts_per_item, int32_t waves_per_cu);
 int saship_batch_get_options(saship_batch *batch, int32_t *use_graph,
                              int32_t *pts_per_item, int32_t *waves_per_cu);
+
+/* Evaluation strategy of the batch, as saship_kernel_set_path: the
+ * separable cylinder path is taken when every set of a call allows it.
+ * No reference counterpart. */
+int saship_batch_set_path(saship_batch *batch, int32_t separable);
 
 /* Times the evaluation kernel over a grid of (pts_per_item, waves_per_cu)
  * on these inputs, keeps the fastest and returns it. */

@@ -17,6 +17,7 @@
 // every later call with the same shapes (a fit step).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -129,6 +130,53 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a)
     }
 }
 
+// Separable cylinder sets (radius x length product meshes, fixed Gauss
+// rule): every wave takes (set, 64-q tile, block of Gauss nodes) items from
+// the queue, sums its nodes (compensated FP32) from the set's tables and
+// writes partial[poff[set] + (block*2 + o)*nq + iq], o = 0 for F^2, 1 for F.
+struct SepArgs {
+    const float *q32;
+    const int64_t *qoff;
+    sep::Tables t;
+    Quad quad;
+    const int4 *items;           // (set, tile, node block, 0)
+    const int *n_items;
+    int *queue;
+    int nodes_per_item;
+    const int64_t *poff;
+    double *partial;
+};
+
+__global__ __launch_bounds__(256) void k_batch_sep(SepArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int n_items = *a.n_items;
+    for (;;) {
+        const int item = next_item(a.queue);
+        if (item >= n_items) break;
+        const int4 it = a.items[item];
+        const int s = it.x, tile = it.y, blk = it.z;
+        const int64_t q0 = a.qoff[s];
+        const int nq = (int)(a.qoff[s + 1] - q0);
+        const int iq = tile * QTILE + lane;
+        const float q = iq < nq ? a.q32[q0 + iq] : 0.0f;
+        const int n0 = blk * a.nodes_per_item, n1 = min(a.quad.n, n0 + a.nodes_per_item);
+        KSum a1, a2;
+        for (int node = n0; node < n1; node++) {
+            float f1, f2;
+            sep::node_sums(a.t, s, q, a.quad.a[node], a.quad.b[node], f1, f2);
+            const float w = a.quad.w[node];
+            a1.add(w * f1);
+            a2.add(w * f2);
+        }
+        if (iq < nq) {
+            double *dst = a.partial + a.poff[s] + (size_t)blk * 2 * nq;
+            dst[iq] = a2.value();
+            dst[nq + iq] = a1.value();
+        }
+    }
+}
+
 // Per set: sum the chunk partials of every output in chunk order, add the
 // weight sums, and form I(q) = scale * <F^2>/<V> + background with the
 // guards of Kernel.Fq (total weight 0 -> 1, shell volume 0 -> 1).
@@ -189,6 +237,9 @@ struct saship_batch {
     size_t partial_cap = 0;
     double *d_out = nullptr;          // raw results then I(q)
     size_t out_cap = 0;
+    char *d_sep = nullptr;            // separable cylinder tables
+    size_t sep_cap = 0;
+    bool separable = true;            // use the separable path when every set allows it
     char *h_out = nullptr;
     size_t hout_cap = 0;
     // launch shape
@@ -225,6 +276,8 @@ struct Plan {
     int kind = 0, nout = 2, ndet = 0, rv = 1;
     int64_t n_values = 0, rec_total = 0, partial_total = 0, result_total = 0, nq_total = 0;
     int n_items = 0;
+    int sep_nlp = 0;                  // separable path: bilinear table columns (0: generic path)
+    int nodes_per_item = 0;           // separable path: Gauss nodes per work item
     size_t off_values = 0, off_voff = 0, off_det = 0, off_roff = 0, off_poff = 0, off_resoff = 0;
     size_t off_nchunk = 0, off_items = 0, off_nitems = 0, bytes = 0;
 };
@@ -258,6 +311,18 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     std::vector<int32_t> nchunk(ns);
     int64_t items = 0;
     const int P = b->pts_per_item;
+    // separable cylinder path when every set's mesh allows it
+    if (id == 1 && kind == 0 && b->m->quad_n > 0 && b->separable) {
+        int nlp = 0;
+        for (int s = 0; s < ns && (s == 0 || nlp); s++) {
+            const int here = (int)n_details == model_maxpd(1) * 4 + 4 ? cylinder_separable(details + (size_t)s * n_details) : 0;
+            nlp = here ? std::max(nlp, here) : 0;
+        }
+        p.sep_nlp = nlp;
+    }
+    const int nnode = b->m->quad.n;
+    if (p.sep_nlp) p.nodes_per_item = std::min(P, nnode);
+    const int nblk = p.sep_nlp ? (nnode + p.nodes_per_item - 1) / p.nodes_per_item : 0;
     for (int s = 0; s < ns; s++) {
         const int64_t nv = voff_in[s + 1] - voff_in[s];
         const int32_t *det = details + (size_t)s * n_details;
@@ -267,8 +332,8 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         const int64_t num_eval = det[n_details - 4];
         const int nq = (int)(b->qoff[s + 1] - b->qoff[s]);
         roff[s] = p.rec_total;
-        p.rec_total += num_eval;
-        nchunk[s] = (int32_t)std::max<int64_t>(1, (num_eval + P - 1) / P);
+        if (!p.sep_nlp) p.rec_total += num_eval;
+        nchunk[s] = p.sep_nlp ? nblk : (int32_t)std::max<int64_t>(1, (num_eval + P - 1) / P);
         poff[s] = p.partial_total;
         p.partial_total += (int64_t)nchunk[s] * p.nout * nq;
         resoff[s] = p.result_total;
@@ -302,7 +367,20 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     // mesh points their chunk can hold (an upper bound before pruning)
     int4 *it = (int4 *)(h + p.off_items);
     int64_t k = 0;
-    for (int fill = P; fill >= 1; fill--) {
+    if (p.sep_nlp) {
+        // costliest sets first (mesh size as the proxy), then tiles, then node blocks
+        std::vector<int> order(ns);
+        for (int s = 0; s < ns; s++) order[s] = s;
+        auto cost = [&](int s) { return details[(size_t)s * n_details + n_details - 4]; };
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+        for (int s : order) {
+            const int nq = (int)(b->qoff[s + 1] - b->qoff[s]);
+            const int ntile = (nq + sas::QTILE - 1) / sas::QTILE;
+            for (int t = 0; t < ntile; t++)
+                for (int c = 0; c < nblk; c++) it[k++] = make_int4(s, t, c, 0);
+        }
+    }
+    for (int fill = p.sep_nlp ? 0 : P; fill >= 1; fill--) {
         for (int s = 0; s < ns; s++) {
             const int64_t num_eval = details[(size_t)s * n_details + n_details - 4];
             const int nq = (int)(b->qoff[s + 1] - b->qoff[s]);
@@ -315,6 +393,24 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         }
     }
     *(int32_t *)(h + p.off_nitems) = p.n_items;
+    return 0;
+}
+
+// Separable cylinder path: tables per set, then the node-block evaluation.
+int enqueue_sep(saship_batch *b, const Plan &p, const sas::ExpandArgs &ea, const int4 *d_items,
+                const int *d_nitems, const int64_t *d_poff)
+{
+    using namespace sas;
+    const sep::Tables t = sep::carve(b->d_sep, b->nsets, p.sep_nlp);
+    sep::k_sep_expand<1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea, t);
+    HIP_TRY(hipGetLastError());
+    const SepArgs sa{b->q32, b->d_qoff, t, b->m->quad, d_items, d_nitems, b->d_queue,
+                     p.nodes_per_item, d_poff, b->d_partial};
+    const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
+    if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
+    k_batch_sep<<<grid, 256, 0, b->st>>>(sa);
+    HIP_TRY(hipGetLastError());
+    if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
     return 0;
 }
 
@@ -338,6 +434,9 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
     ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
                   b->d_recs, b->d_npts, b->d_sums, b->d_queue};
+    if (p.sep_nlp) {
+        if (int e = enqueue_sep(b, p, ea, d_items, d_nitems, d_poff)) return e;
+    } else {
     switch (p.kind) {
     case 0: k_expand<M, 0><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
     case 1: k_expand<M, 1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
@@ -359,6 +458,7 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     }
     HIP_TRY(hipGetLastError());
     if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
+    }
 
     k_batch_final<<<b->nsets, 256, 0, b->st>>>(p.nout, b->d_qoff, d_values, d_voff, d_poff, d_resoff,
                                                 d_nchunk, b->d_sums, b->d_partial,
@@ -396,6 +496,8 @@ int size_buffers(saship_batch *b, const Plan &p)
     if (int e = grow(b->d_recs, b->recs_cap, sizeof(float4) * (size_t)std::max<int64_t>(1, p.rec_total) * p.rv)) return e;
     if (int e = grow(b->d_partial, b->partial_cap, sizeof(double) * (size_t)std::max<int64_t>(1, p.partial_total))) return e;
     if (int e = grow(b->d_out, b->out_cap, sizeof(double) * (size_t)(p.result_total + p.nq_total))) return e;
+    if (p.sep_nlp)
+        if (int e = grow(b->d_sep, b->sep_cap, sas::sep::tables_bytes(b->nsets, p.sep_nlp))) return e;
     const size_t hb = sizeof(double) * (size_t)(p.result_total + p.nq_total);
     if (hb > b->hout_cap || !b->h_out) {
         grew = true;
@@ -414,7 +516,8 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     const bool want_result = result != nullptr, want_iq = iq != nullptr;
     if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
-                                    reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu};
+                                    reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
+                                    p.sep_nlp};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
@@ -510,6 +613,15 @@ int saship_batch_set_options(saship_batch *b, int32_t use_graph, int32_t pts_per
     b->use_graph = use_graph != 0;
     b->pts_per_item = pts_per_item;
     b->waves_per_cu = waves_per_cu;
+    drop_graph(b);
+    return 0;
+}
+
+int saship_batch_set_path(saship_batch *b, int32_t separable)
+{
+    if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
+    std::lock_guard<std::mutex> lock(b->mu);
+    b->separable = separable != 0;
     drop_graph(b);
     return 0;
 }
@@ -616,7 +728,7 @@ int saship_batch_release(saship_batch *b)
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
     void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums,
-                        b->d_queue, b->d_partial, b->d_out};
+                        b->d_queue, b->d_partial, b->d_out, b->d_sep};
     for (void *ptr : dev_ptrs)
         if (ptr) (void)hipFree(ptr);
     if (b->h_stage) (void)hipHostFree(b->h_stage);

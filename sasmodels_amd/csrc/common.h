@@ -90,6 +90,25 @@ inline int model_nvalues(int id)
 }
 inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3; }
 
+// Separable cylinder 1D meshes: only the radius (parameter 2) and length
+// (parameter 3) levels have more than one point, each at most SEP_MAX.
+// Returns the length count rounded up to a multiple of 4 (the column
+// count of the bilinear tables), or 0 when the mesh is not separable.
+constexpr int SEP_MAX = 64;
+inline int cylinder_separable(const int32_t *details)
+{
+    const int mp = model_maxpd(1);
+    int nr = 0, nl = 0;
+    for (int k = 0; k < mp; k++) {
+        const int par = details[k], len = details[mp + k];
+        if (par == 2) nr = len;
+        else if (par == 3) nl = len;
+        else if (len != 1) return 0;
+    }
+    if (nr < 1 || nr > SEP_MAX || nl < 1 || nl > SEP_MAX) return 0;
+    return (nl + 3) / 4 * 4;
+}
+
 // Checks one set's details/values against the model before any device work:
 // every index the kernels derive from them stays inside the buffers.
 inline int validate_set(int id, int kind, const int32_t *details, int64_t n_details,

@@ -420,42 +420,80 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
 
 // ===================================================================== separable cylinder
 // For the cylinder 1D kernel the amplitude factorises,
-//   f(q, theta; R, L) = 2J1(q R sin theta)/(q R sin theta) * sinc(q L/2 cos theta),
-// and a mesh that varies only radius and length is the product of a radius
-// grid and a length grid.  With a fixed Gauss rule the sums
-//   sum_ij w_ij s_ij f_ij   and   sum_ij w_ij s_ij^2 f_ij^2     (per node)
-// become bilinear forms J^T A S and (J.J)^T B (S.S), where J and S are the
-// radius and length factors at this q and node: 35 + 35 special-function
-// evaluations and 2 x 35 x 35 FMAs per (q, node) instead of 1225 of each.
-// Same terms as the generic kernel (weights, cutoff and VALID exclusion fold
-// into A and B as zeros), summed in another order.
+//   f(q, a; R, L) = J(q R sin a) S(q L/2 cos a),  J = 2J1(x)/x, S = sin(x)/x,
+// and a mesh that varies only radius and length is a product grid
+// (R_i, L_j) with weights w_ij = wr_i wl_j wo and s_ij = drho pi R_i^2 L_j,
+// so every accepted term splits into a radius factor and a length factor:
+//   F  term = (wr_i R_i^2 J_i) (wl_j L_j S_j) * c1,
+//   F2 term = (wr_i R_i^4 J_i^2) (wl_j L_j^2 S_j^2) * c2.
+// Acceptance (VALID and w_ij > cutoff) is monotone in wl_j for a fixed row,
+// so with the columns sorted by weight (descending) every row accepts a
+// prefix of them, k_i columns.  With the rows sorted by k_i the per-node
+// sums are one pass over the sorted columns with running prefix sums
+//   P1(k) = sum_{m<k} v1_m S_m,   P2(k) = sum_{m<k} v2_m S_m^2,
+// adding u1_i J_i P1(k_i) and u2_i J_i^2 P2(k_i) as each row's prefix is
+// complete: NL sinc and NR J1 evaluations per (q, node) instead of NR NL of
+// each ("prefix" mode; a mesh with no pruning is the case k_i = NL).
+// When the accepted points are not row prefixes (a pd_start/pd_stop piece
+// of the mesh, negative weights) the sums are bilinear forms J^T A S over
+// tables with zeros for rejected points ("bilinear" mode), evaluated eight
+// rows at a time.  Same terms as the generic kernel, summed in another
+// order.  Tables are read with wave-uniform loads.
 namespace sas {
 namespace sep {
 
-constexpr int NRMAX = 64;
+constexpr int NRMAX = 64;    // radius and length points per set (either axis)
+constexpr int RCHUNK = 8;    // bilinear mode: rows per pass
 
+// Per-set tables, set s at R + s*NRMAX, H + s*NRMAX, U + s*4*NRMAX,
+// K + s*NRMAX, AT/BT + s*nlp*NRMAX, dims + 4*s.
 struct Tables {
-    float *R;        // [NRMAX] radii
-    float *H;        // [NRMAX] half lengths
-    float *A;        // [NRMAX][nlp] w * 1e-2 * drho * V * pi/4
-    float *B;        // [NRMAX][nlp] w * 1e-4 * (drho * V)^2 * pi/4
-    int *dims;       // NR, NL
+    float *R;        // radii (prefix mode: rows by ascending k; bilinear: mesh order)
+    float *H;        // half lengths (prefix mode: columns by descending weight; bilinear: mesh order)
+    float *U;        // prefix mode [4][NRMAX]: u1, u2 per row, v1, v2 per column
+    int *K;          // prefix mode: accepted columns of each row (ascending)
+    float *AT;       // bilinear mode [nlp][NRMAX]: w * 1e-2 * drho * V * pi/4 at (row, column)
+    float *BT;       // bilinear mode [nlp][NRMAX]: w * 1e-4 * (drho * V)^2 * pi/4
+    int *dims;       // NR, NL, mode (1 prefix, 0 bilinear), 0
+    int nlp;         // columns per set in AT/BT (>= every set's NL)
 };
 
-// One workgroup per set: zero the tables, walk the mesh steps of the set,
-// place each accepted point at (radius index, length index), reduce the
-// FP64 normalisation sums like k_expand.  (Templated so that both
+inline size_t tables_bytes(int nsets, int nlp)
+{
+    return (size_t)nsets * (sizeof(float) * NRMAX * (7 + 2 * (size_t)nlp) + 4 * sizeof(int)) + 256;
+}
+
+// Carve the tables of nsets sets out of one buffer of tables_bytes(nsets, nlp).
+inline Tables carve(void *base, int nsets, int nlp)
+{
+    Tables t;
+    const size_t n = (size_t)nsets * NRMAX;
+    t.R = (float *)base;
+    t.H = t.R + n;
+    t.U = t.H + n;
+    t.K = (int *)(t.U + 4 * n);
+    t.AT = (float *)(t.K + n);
+    t.BT = t.AT + n * nlp;
+    t.dims = (int *)(t.BT + n * nlp);
+    t.nlp = nlp;
+    return t;
+}
+
+// One workgroup per set: find the radius and length levels, walk the set's
+// mesh steps (FP64 normalisation sums as k_expand, acceptance mask and the
+// bilinear tables), then test the prefix structure and, when it holds,
+// write the sorted prefix-mode tables.  (Templated so that both
 // translation units can instantiate it.)
 template <int VARIANT = 0>
-__global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tables t, int nlp, int kr, int kl,
-                                                                size_t set_stride)
+__global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tables t)
 {
     constexpr int MP = Cylinder::MAX_PD, NP = Cylinder::NPARS;
-    const int set = blockIdx.x;
+    const int set = blockIdx.x, nlp = t.nlp;
     const int32_t *det = a.det + (size_t)set * a.ndet;
     const double *values = a.values + (a.voff ? a.voff[set] : 0);
-    float *R = t.R + set * set_stride, *H = t.H + set * set_stride;
-    float *A = t.A + set * set_stride * nlp, *B = t.B + set * set_stride * nlp;
+    float *R = t.R + (size_t)set * NRMAX, *H = t.H + (size_t)set * NRMAX, *U = t.U + (size_t)set * 4 * NRMAX;
+    int *K = t.K + (size_t)set * NRMAX;
+    float *AT = t.AT + (size_t)set * NRMAX * nlp, *BT = t.BT + (size_t)set * NRMAX * nlp;
     const int32_t *pd_par = det, *pd_len = det + MP, *pd_off = det + 2 * MP, *pd_stride = det + 3 * MP;
     const int num_eval = det[4 * MP];
     const int num_weights = det[4 * MP + 1];
@@ -463,21 +501,24 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     const int pd_stop = a.pd_stop < 0 ? num_eval : a.pd_stop;
     const double *pd_value = values + nvalues<Cylinder>();
     const double *pd_weight = pd_value + num_weights;
+    int kr = 0, kl = 0;
+    for (int k = 0; k < MP; k++) {
+        if (pd_par[k] == 2) kr = k;
+        if (pd_par[k] == 3) kl = k;
+    }
     const int NR = pd_len[kr], NL = pd_len[kl];
+    const double *rv = pd_value + pd_off[kr], *rw = pd_weight + pd_off[kr];
+    const double *lv = pd_value + pd_off[kl], *lw = pd_weight + pd_off[kl];
     const int tid = threadIdx.x;
+    __shared__ unsigned char acc[NRMAX * NRMAX];
+    __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad;
     if (a.queue && set == 0 && tid == 0) a.queue[0] = 0;
     for (int i = tid; i < NRMAX * nlp; i += EXPAND_THREADS) {
-        A[i] = 0.0f;
-        B[i] = 0.0f;
+        AT[i] = 0.0f;
+        BT[i] = 0.0f;
     }
-    for (int i = tid; i < NRMAX; i += EXPAND_THREADS) {
-        R[i] = i < NR ? (float)pd_value[pd_off[kr] + i] : 0.0f;
-        H[i] = i < NL ? (float)(0.5 * pd_value[pd_off[kl] + i]) : 0.0f;
-    }
-    if (tid == 0) {
-        t.dims[2 * set] = NR;
-        t.dims[2 * set + 1] = NL;
-    }
+    for (int i = tid; i < NRMAX * NRMAX; i += EXPAND_THREADS) acc[i] = 0;
+    if (tid == 0) bad = 0;
     __syncthreads();
     double base[NP];
 #pragma unroll
@@ -506,14 +547,25 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
             acc_v += w0 * vol;
             if (a.reff_mode != 0) acc_r += w0 * Cylinder::reff(a.reff_mode, p);
             const double s = (p[0] - p[1]) * vol;
-            A[ir * nlp + il] = (float)(w0 * 1e-2 * s * PI_4_D);
-            B[ir * nlp + il] = (float)(w0 * 1e-4 * s * s * PI_4_D);
+            AT[il * NRMAX + ir] = (float)(w0 * 1e-2 * s * PI_4_D);
+            BT[il * NRMAX + ir] = (float)(w0 * 1e-4 * s * s * PI_4_D);
+            acc[ir * NRMAX + il] = 1;
         }
     }
     __shared__ double red[3][EXPAND_THREADS];
     red[0][tid] = acc_w;
     red[1][tid] = acc_v;
     red[2][tid] = acc_r;
+    // columns by descending weight (invalid lengths last), ties by index
+    if (tid < NL) {
+        const double key = lv[tid] >= 0.0 ? lw[tid] : -__builtin_inf();
+        int rank = 0;
+        for (int j = 0; j < NL; j++) {
+            const double kj = lv[j] >= 0.0 ? lw[j] : -__builtin_inf();
+            rank += (kj > key || (kj == key && j < tid)) ? 1 : 0;
+        }
+        colorder[rank] = tid;
+    }
     __syncthreads();
     for (int s = EXPAND_THREADS / 2; s > 0; s >>= 1) {
         if (tid < s) {
@@ -523,78 +575,148 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         }
         __syncthreads();
     }
+    // accepted columns of each row, and whether they are a prefix of colorder
+    if (tid < NR) {
+        int k = 0;
+        for (int j = 0; j < NL; j++) k += acc[tid * NRMAX + j];
+        rowk[tid] = k;
+        for (int m = 0; m < NL; m++)
+            if ((acc[tid * NRMAX + colorder[m]] != 0) != (m < k)) bad = 1;
+    }
+    __syncthreads();
+    if (tid < NR) {
+        const int k = rowk[tid];
+        int rank = 0;
+        for (int i = 0; i < NR; i++) rank += (rowk[i] < k || (rowk[i] == k && i < tid)) ? 1 : 0;
+        roworder[rank] = tid;
+    }
+    __syncthreads();
+    const bool prefix = bad == 0;
     if (tid == 0) {
         a.sums[4 * set + 0] = red[0][0];
         a.sums[4 * set + 1] = red[1][0];
         a.sums[4 * set + 2] = red[1][0];
         a.sums[4 * set + 3] = red[2][0];
+        int *d = t.dims + 4 * set;
+        d[0] = NR;
+        d[1] = NL;
+        d[2] = prefix ? 1 : 0;
+        d[3] = 0;
     }
-}
-
-// F and F^2 contributions of one Gauss node at q (sin/cos of the node, its
-// weight w*sin folded in by the caller).  R, H, A, B may live in LDS or in
-// global memory; every lane reads the same addresses.
-template <int NLP>
-__device__ __forceinline__ void node_sums(float q, float sn, float cs, int NR, int NL,
-                                          const float *R, const float *H, const float *A, const float *B,
-                                          float &f1, float &f2)
-{
-    float S[NLP], S2[NLP];
-    const float qc = q * cs, qs = q * sn;
-#pragma unroll
-    for (int j = 0; j < NLP; j++) {
-        S[j] = j < NL ? sinc_prod(qc, H[j]) : 0.0f;
-        S2[j] = S[j] * S[j];
-    }
-    float a1 = 0.0f, a2 = 0.0f;
-    for (int i = 0; i < NR; i++) {
-        const float J = j1c_f(qs * R[i]);
-        const float4 *ra = (const float4 *)(A + i * NLP);
-        const float4 *rb = (const float4 *)(B + i * NLP);
-        float t1 = 0.0f, t2 = 0.0f;
-#pragma unroll
-        for (int j = 0; j < NLP / 4; j++) {
-            const float4 x = ra[j], y = rb[j];
-            t1 = fmaf(x.x, S[4 * j], t1);
-            t1 = fmaf(x.y, S[4 * j + 1], t1);
-            t1 = fmaf(x.z, S[4 * j + 2], t1);
-            t1 = fmaf(x.w, S[4 * j + 3], t1);
-            t2 = fmaf(y.x, S2[4 * j], t2);
-            t2 = fmaf(y.y, S2[4 * j + 1], t2);
-            t2 = fmaf(y.z, S2[4 * j + 2], t2);
-            t2 = fmaf(y.w, S2[4 * j + 3], t2);
+    if (tid < NRMAX) {
+        const int m = tid;
+        float r = 0.0f, h = 0.0f, u1 = 0.0f, u2 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+        int k = 0;
+        if (prefix) {
+            // single-point levels multiply in, in mesh order like w0
+            double wo = 1.0;
+            for (int lev = MP - 1; lev >= 0; lev--)
+                if (lev != kr && lev != kl) wo = pd_weight[pd_off[lev]] * wo;
+            const double drho = base[0] - base[1];
+            const double c1 = wo * 1e-2 * drho * PI_D * PI_4_D;
+            const double c2 = wo * 1e-4 * drho * drho * PI_D * PI_D * PI_4_D;
+            if (m < NR) {
+                const int i = roworder[m];
+                const double x = rv[i];
+                r = (float)x;
+                u1 = (float)(rw[i] * x * x);
+                u2 = (float)(rw[i] * x * x * x * x);
+                k = rowk[i];
+            }
+            if (m < NL) {
+                const int j = colorder[m];
+                const double x = lv[j];
+                h = (float)(0.5 * x);
+                v1 = (float)(lw[j] * x * c1);
+                v2 = (float)(lw[j] * x * x * c2);
+            }
+        } else {
+            if (m < NR) r = (float)rv[m];
+            if (m < NL) h = (float)(0.5 * lv[m]);
         }
-        a1 = fmaf(J, t1, a1);
-        a2 = fmaf(J * J, t2, a2);
+        R[m] = r;
+        H[m] = h;
+        K[m] = k;
+        U[m] = u1;
+        U[NRMAX + m] = u2;
+        U[2 * NRMAX + m] = v1;
+        U[3 * NRMAX + m] = v2;
     }
-    f1 = a1;
-    f2 = a2;
 }
 
-// Single set: workgroup = 4 waves = 4 Gauss nodes over one 64-q tile; tables
-// staged in LDS; partial[(o*nq + iq)*nnode + node] for k_finalize.
-template <int NLP>
+// F and F^2 sums of one set at (q, Gauss node) -- sn, cs: sine and cosine of
+// the node; the node weight is applied by the caller.  Every lane of the
+// wave reads the same table entries.
+__device__ __forceinline__ void node_sums(const Tables &t, int set, float q, float sn, float cs, float &f1,
+                                          float &f2)
+{
+    const int *dims = t.dims + 4 * set;
+    const int NR = dims[0], NL = dims[1];
+    const float *__restrict__ R = t.R + (size_t)set * NRMAX;
+    const float *__restrict__ H = t.H + (size_t)set * NRMAX;
+    const float qc = q * cs, qs = q * sn;
+    float F1 = 0.0f, F2 = 0.0f;
+    if (dims[2]) {
+        const float *__restrict__ U = t.U + (size_t)set * 4 * NRMAX;
+        const int *__restrict__ K = t.K + (size_t)set * NRMAX;
+        float P1 = 0.0f, P2 = 0.0f;
+        int r = 0;
+        while (r < NR && K[r] == 0) r++;
+        for (int m = 0; m < NL && r < NR; m++) {
+            const float S = sinc_prod(qc, H[m]);
+            P1 = fmaf(U[2 * NRMAX + m], S, P1);
+            P2 = fmaf(U[3 * NRMAX + m], S * S, P2);
+            while (r < NR && K[r] == m + 1) {
+                const float J = j1c_f(qs * R[r]);
+                F1 = fmaf(U[r] * J, P1, F1);
+                F2 = fmaf(U[NRMAX + r] * (J * J), P2, F2);
+                r++;
+            }
+        }
+    } else {
+        const float *__restrict__ AT = t.AT + (size_t)set * NRMAX * t.nlp;
+        const float *__restrict__ BT = t.BT + (size_t)set * NRMAX * t.nlp;
+        for (int i0 = 0; i0 < NR; i0 += RCHUNK) {
+            float J[RCHUNK], t1[RCHUNK], t2[RCHUNK];
+#pragma unroll
+            for (int c = 0; c < RCHUNK; c++) {
+                J[c] = i0 + c < NR ? j1c_f(qs * R[i0 + c]) : 0.0f;
+                t1[c] = 0.0f;
+                t2[c] = 0.0f;
+            }
+            for (int j = 0; j < NL; j++) {
+                const float S = sinc_prod(qc, H[j]), S2 = S * S;
+                const float *a = AT + j * NRMAX + i0, *b = BT + j * NRMAX + i0;
+#pragma unroll
+                for (int c = 0; c < RCHUNK; c++) {
+                    t1[c] = fmaf(a[c], S, t1[c]);
+                    t2[c] = fmaf(b[c], S2, t2[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < RCHUNK; c++) {
+                F1 = fmaf(J[c], t1[c], F1);
+                F2 = fmaf(J[c] * J[c], t2[c], F2);
+            }
+        }
+    }
+    f1 = F1;
+    f2 = F2;
+}
+
+// Single set: one wave per (64-q tile, Gauss node), four nodes per
+// workgroup; partial[(o*nq + iq)*nnode + node] for k_finalize.
+template <int VARIANT = 0>
 __global__ __launch_bounds__(256) void k_fq1d_sep(int nq, const float *__restrict__ q32, Tables t, Quad quad,
                                                    double *__restrict__ partial)
 {
-    __shared__ __align__(16) float sA[NRMAX * NLP];
-    __shared__ __align__(16) float sB[NRMAX * NLP];
-    __shared__ float sR[NRMAX], sH[NLP];
-    const int NR = t.dims[0], NL = t.dims[1];
-    for (int i = threadIdx.x; i < NR * NLP; i += 256) {
-        sA[i] = t.A[i];
-        sB[i] = t.B[i];
-    }
-    for (int i = threadIdx.x; i < NRMAX; i += 256) sR[i] = t.R[i];
-    for (int i = threadIdx.x; i < NLP; i += 256) sH[i] = t.H[i];
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int iq = blockIdx.x * 64 + lane;
     const int node = blockIdx.y * 4 + wave;
     if (node >= quad.n) return;
     const float q = iq < nq ? q32[iq] : 0.0f;
     float f1, f2;
-    node_sums<NLP>(q, quad.a[node], quad.b[node], NR, NL, sR, sH, sA, sB, f1, f2);
+    node_sums(t, 0, q, quad.a[node], quad.b[node], f1, f2);
     if (iq < nq) {
         const float w = quad.w[node];
         partial[((size_t)0 * nq + iq) * quad.n + node] = (double)(w * f2);

@@ -140,39 +140,23 @@ int launch(saship_kernel *k, int kind, const int32_t *d_det, const double *d_val
     return 0;
 }
 
-// Table layout of the separable path inside one allocation (one set).
-sas::sep::Tables sep_tables(char *base, int nlp, size_t set_stride = sas::sep::NRMAX)
-{
-    using sas::sep::NRMAX;
-    sas::sep::Tables t;
-    (void)set_stride;
-    t.R = (float *)base;
-    t.H = t.R + NRMAX;
-    t.A = t.H + NRMAX;
-    t.B = t.A + (size_t)NRMAX * nlp;
-    t.dims = (int *)(t.B + (size_t)NRMAX * nlp);
-    return t;
-}
-
-size_t sep_bytes(int nlp) { return sizeof(float) * (2 * sas::sep::NRMAX + 2 * (size_t)sas::sep::NRMAX * nlp) + 64; }
-
-template <int NLP>
-int launch_sep_nlp(saship_kernel *k, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
-                   int pd_stop, double cutoff, int reff_mode, int accumulate, int kr, int kl)
+// Separable cylinder path: tables, one wave per (q tile, node), FP64 node sums.
+int launch_sep(saship_kernel *k, int nlp, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
+               int pd_stop, double cutoff, int reff_mode, int accumulate)
 {
     using namespace sas;
     const int nq = k->nq, nnode = k->m->quad.n;
-    if (!k->d_sep) HIP_TRY(hipMalloc((void **)&k->d_sep, sep_bytes(64)));
-    sep::Tables t = sep_tables(k->d_sep, NLP);
+    if (!k->d_sep) HIP_TRY(hipMalloc((void **)&k->d_sep, sep::tables_bytes(1, SEP_MAX)));
+    const sep::Tables t = sep::carve(k->d_sep, 1, nlp);
     double *sums = (double *)(k->d_meta + 64);
     ExpandArgs ea{d_det, d_val, nullptr, nullptr, n_details, pd_start, pd_stop, cutoff, reff_mode,
                   nullptr, (int *)k->d_meta, sums, nullptr};
-    sep::k_sep_expand<0><<<1, EXPAND_THREADS, 0, k->st>>>(ea, t, NLP, kr, kl, sep::NRMAX);
+    sep::k_sep_expand<0><<<1, EXPAND_THREADS, 0, k->st>>>(ea, t);
     HIP_TRY(hipGetLastError());
     if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * 2 * (size_t)nq * nnode)) return e;
     const dim3 grid(std::max(1, (nq + 63) / 64), (nnode + 3) / 4);
     if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
-    sep::k_fq1d_sep<NLP><<<grid, 256, 0, k->st>>>(nq, k->q32, t, k->m->quad, k->d_partial);
+    sep::k_fq1d_sep<0><<<grid, 256, 0, k->st>>>(nq, k->q32, t, k->m->quad, k->d_partial);
     HIP_TRY(hipGetLastError());
     if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
     const int fblocks = std::max(1, (2 * nq + 3) / 4);
@@ -181,34 +165,7 @@ int launch_sep_nlp(saship_kernel *k, const int32_t *d_det, const double *d_val, 
     return 0;
 }
 
-// Mesh levels of a cylinder details buffer that hold radius and length, if
-// every other level has a single point (the separable case).
-bool cylinder_separable(const int32_t *details, int &kr, int &kl, int &nr, int &nl)
-{
-    const int mp = model_maxpd(1);
-    kr = kl = -1;
-    for (int k = 0; k < mp; k++) {
-        const int par = details[k], len = details[mp + k];
-        if (par == 2) kr = k;
-        else if (par == 3) kl = k;
-        else if (len != 1) return false;
-    }
-    if (kr < 0 || kl < 0) return false;
-    nr = details[mp + kr];
-    nl = details[mp + kl];
-    return nr >= 1 && nr <= sas::sep::NRMAX && nl >= 1 && nl <= 64;
-}
-
-int launch_sep(saship_kernel *k, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
-               int pd_stop, double cutoff, int reff_mode, int accumulate, int kr, int kl, int nl)
-{
-#define SEP_CASE(N) \
-    if (nl <= N) return launch_sep_nlp<N>(k, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, kr, kl);
-    SEP_CASE(4) SEP_CASE(8) SEP_CASE(12) SEP_CASE(16) SEP_CASE(20) SEP_CASE(24) SEP_CASE(28)
-    SEP_CASE(32) SEP_CASE(36) SEP_CASE(40) SEP_CASE(48) SEP_CASE(64)
-#undef SEP_CASE
-    return fail(SASHIP_EINVAL, "length grid too large for the separable path");
-}
+static_assert(SEP_MAX == sas::sep::NRMAX, "separable table size");
 
 }  // namespace
 
@@ -378,10 +335,9 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     const int32_t *d_det = (const int32_t *)(k->d_args + doff);
     const double *d_val = (const double *)k->d_args;
     int rc;
-    int kr, kl, nr, nl;
-    if (id == 1 && kind == 0 && k->m->quad_n > 0 && k->separable &&
-        cylinder_separable(details, kr, kl, nr, nl)) {
-        rc = launch_sep(k, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, kr, kl, nl);
+    const int nlp = id == 1 && kind == 0 && k->m->quad_n > 0 && k->separable ? cylinder_separable(details) : 0;
+    if (nlp) {
+        rc = launch_sep(k, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(k->h_result, k->d_result, rbytes, hipMemcpyDeviceToHost, k->st));
         HIP_TRY(hipStreamSynchronize(k->st));

@@ -207,6 +207,10 @@ class HipBatch:
             int(p.value if pts_per_item is None else pts_per_item),
             int(w.value if waves_per_cu is None else waves_per_cu)))
 
+    def set_separable(self, enable=True):
+        """Allow (default) or forbid the separable cylinder evaluation."""
+        _lib.check(_lib.lib().saship_batch_set_path(self._handle, int(bool(enable))))
+
     def _pack(self, calls):
         if len(calls) != self.nsets:
             raise ValueError("expected %d (call_details, values) pairs, got %d" % (self.nsets, len(calls)))

@@ -69,3 +69,115 @@ def test_separable_cutoff_chunks_and_truncation():
         hk._call_kernel(det, val, 0.0, mag, 2, pd_start=start, pd_stop=min(n, start + 37))
     compare_raw(hk.result, whole, Q.size, 2, rtol=1e-6)
 
+
+
+def _custom_mesh(hm, nq_dim, radius, rw, length, lw):
+    """Mesh with explicit radius/length points and weights (array dispersion)."""
+    from sasmodels_amd import direct_model
+    mesh = direct_model.get_mesh(hm.info, dict(sld=2.0, sld_solvent=-1.0), dim=nq_dim)
+    names = [p.name for p in hm.info.parameters.call_parameters]
+    mesh[names.index("radius")] = (float(radius[0]), np.asarray(radius, "d"), np.asarray(rw, "d"))
+    mesh[names.index("length")] = (float(length[0]), np.asarray(length, "d"), np.asarray(lw, "d"))
+    return mesh
+
+
+def _mixed_sign_mesh(hm, seed):
+    """Radius and length weights of both signs: the accepted points of a row
+    are then not a prefix of the weight order (bilinear mode)."""
+    r = np.random.default_rng(seed)
+    nr, nl = int(r.integers(3, 20)), int(r.integers(3, 20))
+    radius = np.sort(r.uniform(5, 80, nr))
+    length = np.sort(r.uniform(40, 600, nl))
+    rw = r.uniform(0.2, 1.0, nr)*np.where(np.arange(nr) % 3 == 1, -1, 1)
+    lw = r.uniform(0.2, 1.0, nl)*np.where(np.arange(nl) % 4 == 2, -1, 1)
+    return _custom_mesh(hm, "1d", radius, rw, length, lw)
+
+
+def test_separable_bilinear_mode():
+    from gpu_helpers import hip_model
+    from sasmodels_amd.details import make_kernel_args
+    for seed in (1, 2, 3):
+        hk, ok, _, _, _ = buffers("cylinder", [Q], {}, 76)
+        mesh = _mixed_sign_mesh(hip_model("cylinder", 76), seed)
+        det, val, mag = make_kernel_args(hk, mesh)
+        print("bilinear mode seed", seed, flush=True)
+        ora = ok.raw(det, val, 0.0, mag, 0)
+        hk.set_separable(True)
+        hk._call_kernel(det, val, 0.0, mag, 0)
+        compare_raw(hk.result, ora, Q.size, 2)
+
+
+def _cylinder_sets(hm, n, seed, with_mixed=True):
+    from sasmodels_amd import direct_model
+    from sasmodels_amd.details import make_kernel_args
+    r = np.random.default_rng(seed)
+    kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
+    q_list, calls = [], []
+    for s in range(n):
+        nq = int(r.integers(1, 700)) if s % 7 else 0 if s == 7 else 64
+        q = np.logspace(-3, 0, nq)
+        if with_mixed and s == 3:
+            mesh = _mixed_sign_mesh(hm, seed + s)
+        else:
+            pars = dict(radius=r.uniform(5, 150), length=r.uniform(20, 1500),
+                        sld=r.uniform(-1, 7), sld_solvent=r.uniform(-1, 7),
+                        radius_pd=r.uniform(0, 0.3), radius_pd_n=int(r.integers(1, 65)),
+                        radius_pd_type=["gaussian", "lognormal", "schulz"][s % 3],
+                        length_pd=r.uniform(0, 0.3), length_pd_n=int(r.integers(1, 65)),
+                        length_pd_type=["lognormal", "gaussian", "uniform"][s % 3])
+            mesh = direct_model.get_mesh(hm.info, pars, dim="1d")
+        q_list.append([q])
+        calls.append(make_kernel_args(kl, mesh)[:2])
+    return q_list, calls
+
+
+@pytest.mark.parametrize("cutoff", [0.0, 1e-5, 1e-3])
+def test_batched_separable(cutoff):
+    """Batched separable path against the oracle and the generic batched path."""
+    from gpu_helpers import hip_model
+    from oracle.oracle import OracleKernel
+    from sasmodels_amd.kernelhip import HipBatch
+    hm = hip_model("cylinder", 76)
+    q_list, calls = _cylinder_sets(hm, 24, 1234)
+    batch = HipBatch(hm, q_list)
+    print("batched separable cutoff", cutoff, flush=True)
+    sep, _ = batch.raw(calls, cutoff)
+    batch.set_separable(False)
+    gen, _ = batch.raw(calls, cutoff)
+    batch.release()
+    off = 0
+    for qv, (det, val) in zip(q_list, calls):
+        nq = qv[0].size
+        n = 2*nq + 4
+        ref = OracleKernel(hm.info, qv).raw(det, val, cutoff, False, 0)
+        compare_raw(sep[off:off + n], ref, nq, 2)
+        compare_raw(gen[off:off + n], ref, nq, 2)
+        off += n
+
+
+def test_batched_separable_graph_replay_and_options():
+    """Graph replays and launch shapes change nothing: every replay equals
+    the eager call bit for bit, and other node blocks agree to rounding."""
+    from gpu_helpers import hip_model
+    from sasmodels_amd.kernelhip import HipBatch
+    hm = hip_model("cylinder", 76)
+    q_list, calls = _cylinder_sets(hm, 16, 99)
+    batch = HipBatch(hm, q_list)
+    eager, _ = batch.raw(calls, 1e-5)
+    batch.set_options(use_graph=True)
+    for rep in range(3):
+        print("replay", rep, flush=True)
+        again, _ = batch.raw(calls, 1e-5)
+        assert np.array_equal(again, eager)
+        hk, _, det, val, mag = buffers("cylinder", [Q], _pars(5, 7), 76)
+        hk._call_kernel(det, val, 0.0, mag, 0)          # eager work between replays
+    batch.set_options(use_graph=False)
+    for nodes, waves in ((1, 8), (7, 16), (76, 32), (128, 4)):
+        batch.set_options(pts_per_item=nodes, waves_per_cu=waves)
+        other, _ = batch.raw(calls, 1e-5)
+        off = 0
+        for qv in q_list:
+            nq = qv[0].size
+            compare_raw(other[off:off + 2*nq + 4], eager[off:off + 2*nq + 4], nq, 2, rtol=1e-6)
+            off += 2*nq + 4
+    batch.release()
