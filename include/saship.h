@@ -154,8 +154,9 @@ int saship_batch_get_options(saship_batch *batch, int32_t *use_graph,
  * No reference counterpart. */
 int saship_batch_set_path(saship_batch *batch, int32_t separable);
 
-/* Times the evaluation kernel over a grid of (pts_per_item, waves_per_cu)
- * on these inputs, keeps the fastest and returns it. */
+/* Times the device work of a call (staging copy, expansion, evaluation,
+ * final sums) over a grid of (pts_per_item, waves_per_cu) on these inputs,
+ * keeps the fastest and returns it. */
 int saship_batch_autotune(saship_batch *batch, int32_t kind,
                           const int32_t *details, int32_t n_details,
                           const double *values, const int64_t *values_offset,
@@ -189,9 +190,11 @@ int saship_mlp_train_step(const float *theta, const void *pack, const float *X, 
                           int32_t batch, void *work, float *grad, float *loss_parts, void *stream);
 /* Fused inference over ncurves x nq rows: curves[ncurves][5] = radius,
  * length, radius_pd, length_pd, contrast; logq[nq] = log10 q; norm[12] =
- * feature means then standard deviations.  out_iq = 0 writes the network
- * output; 1 writes I = 1e-4 contrast^2 pi R^2 L exp(y).  grid <= 0 picks the
- * persistent grid size. */
+ * feature means then standard deviations.  Features of a row: log qR,
+ * log qL, radius_pd, length_pd, bL = log(1 + (qL/pi)^2)/2,
+ * bR = 3/2 log(1 + (qR)^2/4).  out_iq = 0 writes the network output r;
+ * 1 writes I = 1e-4 contrast^2 pi R^2 L exp(r - bL - bR).  grid <= 0 picks
+ * the persistent grid size. */
 int saship_mlp_infer(const float *theta, const void *pack, const float *curves, int32_t ncurves,
                      const float *logq, int32_t nq, const float *norm, int32_t out_iq, float *y,
                      int32_t grid, void *stream);

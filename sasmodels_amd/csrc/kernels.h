@@ -220,6 +220,28 @@ __device__ __forceinline__ void split_range(int npts, int nsplit, int split, int
     s1 = min(npts, s0 + per);
 }
 
+// Read-only view of device memory that no kernel running with it writes:
+// loads through the constant address space go through the scalar cache when
+// the address is wave-uniform, whatever else the kernel stores.
+#define SAS_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const SAS_CONST T *as_const(const T *p)
+{
+    return (const SAS_CONST T *)p;
+}
+
+// Element loads through such a view (HIP's vector types cannot bind to it).
+__device__ __forceinline__ float4 ld4(const SAS_CONST float4 *p)
+{
+    const SAS_CONST float *f = (const SAS_CONST float *)p;
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+__device__ __forceinline__ int2 ld2(const SAS_CONST int2 *p)
+{
+    const SAS_CONST int *v = (const SAS_CONST int *)p;
+    return make_int2(v[0], v[1]);
+}
+
 __device__ __forceinline__ void load_rec(const float4 *src, float *r, int rv)
 {
     for (int v = 0; v < rv; v++) {
@@ -445,22 +467,26 @@ namespace sep {
 constexpr int NRMAX = 64;    // radius and length points per set (either axis)
 constexpr int RCHUNK = 8;    // bilinear mode: rows per pass
 
-// Per-set tables, set s at R + s*NRMAX, H + s*NRMAX, U + s*4*NRMAX,
-// K + s*NRMAX, AT/BT + s*nlp*NRMAX, dims + 4*s.
+// Per-set tables, set s at ROW/COL/GRP + s*NRMAX, AT/BT + s*nlp*NRMAX,
+// dims + 4*s.  Prefix mode: ROW[r] = (R, u1, u2, 0) for the rows that accept
+// at least one column, by ascending accepted count; COL[m] = (L/2, v1, v2, 0)
+// by descending weight; GRP[g] = (columns, rows) at the end of the g-th run
+// of rows with equal accepted count.  Bilinear mode: ROW[i].x = R_i,
+// COL[j].x = L_j/2 in mesh order.
 struct Tables {
-    float *R;        // radii (prefix mode: rows by ascending k; bilinear: mesh order)
-    float *H;        // half lengths (prefix mode: columns by descending weight; bilinear: mesh order)
-    float *U;        // prefix mode [4][NRMAX]: u1, u2 per row, v1, v2 per column
-    int *K;          // prefix mode: accepted columns of each row (ascending)
+    float4 *ROW;
+    float4 *COL;
+    int2 *GRP;
     float *AT;       // bilinear mode [nlp][NRMAX]: w * 1e-2 * drho * V * pi/4 at (row, column)
     float *BT;       // bilinear mode [nlp][NRMAX]: w * 1e-4 * (drho * V)^2 * pi/4
-    int *dims;       // NR, NL, mode (1 prefix, 0 bilinear), 0
+    int *dims;       // NR, NL, mode (1 prefix, 0 bilinear), number of row groups
     int nlp;         // columns per set in AT/BT (>= every set's NL)
 };
 
 inline size_t tables_bytes(int nsets, int nlp)
 {
-    return (size_t)nsets * (sizeof(float) * NRMAX * (7 + 2 * (size_t)nlp) + 4 * sizeof(int)) + 256;
+    return (size_t)nsets * (NRMAX * (2 * sizeof(float4) + sizeof(int2) + 2 * sizeof(float) * (size_t)nlp) +
+                            4 * sizeof(int)) + 256;
 }
 
 // Carve the tables of nsets sets out of one buffer of tables_bytes(nsets, nlp).
@@ -468,11 +494,10 @@ inline Tables carve(void *base, int nsets, int nlp)
 {
     Tables t;
     const size_t n = (size_t)nsets * NRMAX;
-    t.R = (float *)base;
-    t.H = t.R + n;
-    t.U = t.H + n;
-    t.K = (int *)(t.U + 4 * n);
-    t.AT = (float *)(t.K + n);
+    t.ROW = (float4 *)base;
+    t.COL = t.ROW + n;
+    t.GRP = (int2 *)(t.COL + n);
+    t.AT = (float *)(t.GRP + n);
     t.BT = t.AT + n * nlp;
     t.dims = (int *)(t.BT + n * nlp);
     t.nlp = nlp;
@@ -491,8 +516,8 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     const int set = blockIdx.x, nlp = t.nlp;
     const int32_t *det = a.det + (size_t)set * a.ndet;
     const double *values = a.values + (a.voff ? a.voff[set] : 0);
-    float *R = t.R + (size_t)set * NRMAX, *H = t.H + (size_t)set * NRMAX, *U = t.U + (size_t)set * 4 * NRMAX;
-    int *K = t.K + (size_t)set * NRMAX;
+    float4 *ROW = t.ROW + (size_t)set * NRMAX, *COL = t.COL + (size_t)set * NRMAX;
+    int2 *GRP = t.GRP + (size_t)set * NRMAX;
     float *AT = t.AT + (size_t)set * NRMAX * nlp, *BT = t.BT + (size_t)set * NRMAX * nlp;
     const int32_t *pd_par = det, *pd_len = det + MP, *pd_off = det + 2 * MP, *pd_stride = det + 3 * MP;
     const int num_eval = det[4 * MP];
@@ -511,7 +536,7 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     const double *lv = pd_value + pd_off[kl], *lw = pd_weight + pd_off[kl];
     const int tid = threadIdx.x;
     __shared__ unsigned char acc[NRMAX * NRMAX];
-    __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad;
+    __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad, nzero, ngroups;
     if (a.queue && set == 0 && tid == 0) a.queue[0] = 0;
     for (int i = tid; i < NRMAX * nlp; i += EXPAND_THREADS) {
         AT[i] = 0.0f;
@@ -593,6 +618,21 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     __syncthreads();
     const bool prefix = bad == 0;
     if (tid == 0) {
+        // rows accepting nothing drop out; runs of equal accepted count form groups
+        int z = 0;
+        while (z < NR && rowk[roworder[z]] == 0) z++;
+        int g = 0;
+        if (prefix) {
+            for (int m = z; m < NR; m++) {
+                const int k = rowk[roworder[m]];
+                if (m + 1 == NR || rowk[roworder[m + 1]] != k) GRP[g++] = make_int2(k, m + 1 - z);
+            }
+        }
+        nzero = z;
+        ngroups = g;
+    }
+    __syncthreads();
+    if (tid == 0) {
         a.sums[4 * set + 0] = red[0][0];
         a.sums[4 * set + 1] = red[1][0];
         a.sums[4 * set + 2] = red[1][0];
@@ -601,12 +641,11 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         d[0] = NR;
         d[1] = NL;
         d[2] = prefix ? 1 : 0;
-        d[3] = 0;
+        d[3] = ngroups;
     }
     if (tid < NRMAX) {
         const int m = tid;
-        float r = 0.0f, h = 0.0f, u1 = 0.0f, u2 = 0.0f, v1 = 0.0f, v2 = 0.0f;
-        int k = 0;
+        float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f), col = row;
         if (prefix) {
             // single-point levels multiply in, in mesh order like w0
             double wo = 1.0;
@@ -615,78 +654,77 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
             const double drho = base[0] - base[1];
             const double c1 = wo * 1e-2 * drho * PI_D * PI_4_D;
             const double c2 = wo * 1e-4 * drho * drho * PI_D * PI_D * PI_4_D;
-            if (m < NR) {
-                const int i = roworder[m];
+            if (m < NR - nzero) {
+                const int i = roworder[m + nzero];
                 const double x = rv[i];
-                r = (float)x;
-                u1 = (float)(rw[i] * x * x);
-                u2 = (float)(rw[i] * x * x * x * x);
-                k = rowk[i];
+                row = make_float4((float)x, (float)(rw[i] * x * x), (float)(rw[i] * x * x * x * x), 0.0f);
             }
             if (m < NL) {
                 const int j = colorder[m];
                 const double x = lv[j];
-                h = (float)(0.5 * x);
-                v1 = (float)(lw[j] * x * c1);
-                v2 = (float)(lw[j] * x * x * c2);
+                col = make_float4((float)(0.5 * x), (float)(lw[j] * x * c1), (float)(lw[j] * x * x * c2), 0.0f);
             }
         } else {
-            if (m < NR) r = (float)rv[m];
-            if (m < NL) h = (float)(0.5 * lv[m]);
+            if (m < NR) row.x = (float)rv[m];
+            if (m < NL) col.x = (float)(0.5 * lv[m]);
         }
-        R[m] = r;
-        H[m] = h;
-        K[m] = k;
-        U[m] = u1;
-        U[NRMAX + m] = u2;
-        U[2 * NRMAX + m] = v1;
-        U[3 * NRMAX + m] = v2;
+        ROW[m] = row;
+        COL[m] = col;
     }
 }
 
 // F and F^2 sums of one set at (q, Gauss node) -- sn, cs: sine and cosine of
 // the node; the node weight is applied by the caller.  Every lane of the
-// wave reads the same table entries.
+// wave reads the same table entries (scalar loads: the tables are written
+// by k_sep_expand, an earlier launch, and only read here).
 __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, float sn, float cs, float &f1,
                                           float &f2)
 {
-    const int *dims = t.dims + 4 * set;
+    const SAS_CONST int *dims = as_const(t.dims) + 4 * set;
     const int NR = dims[0], NL = dims[1];
-    const float *__restrict__ R = t.R + (size_t)set * NRMAX;
-    const float *__restrict__ H = t.H + (size_t)set * NRMAX;
+    const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
+    const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
     const float qc = q * cs, qs = q * sn;
     float F1 = 0.0f, F2 = 0.0f;
     if (dims[2]) {
-        const float *__restrict__ U = t.U + (size_t)set * 4 * NRMAX;
-        const int *__restrict__ K = t.K + (size_t)set * NRMAX;
+        const SAS_CONST int2 *GRP = as_const(t.GRP) + (size_t)set * NRMAX;
+        const int G = dims[3];
         float P1 = 0.0f, P2 = 0.0f;
-        int r = 0;
-        while (r < NR && K[r] == 0) r++;
-        for (int m = 0; m < NL && r < NR; m++) {
-            const float S = sinc_prod(qc, H[m]);
-            P1 = fmaf(U[2 * NRMAX + m], S, P1);
-            P2 = fmaf(U[3 * NRMAX + m], S * S, P2);
-            while (r < NR && K[r] == m + 1) {
-                const float J = j1c_f(qs * R[r]);
-                F1 = fmaf(U[r] * J, P1, F1);
-                F2 = fmaf(U[NRMAX + r] * (J * J), P2, F2);
-                r++;
+        int m = 0, r = 0;
+        // next table entries are loaded one iteration ahead (entry NRMAX of
+        // a set is the next set's first entry or the following table: in bounds)
+        float4 cn = ld4(COL), wn = ld4(ROW);
+        for (int g = 0; g < G; g++) {
+            const int2 e = ld2(GRP + g);
+            for (; m < e.x; m++) {
+                const float4 c = cn;
+                cn = ld4(COL + m + 1);
+                const float S = sinc_prod(qc, c.x);
+                P1 = fmaf(c.y, S, P1);
+                P2 = fmaf(c.z, S * S, P2);
+            }
+            for (; r < e.y; r++) {
+                const float4 w = wn;
+                wn = ld4(ROW + r + 1);
+                const float J = j1c_f(qs * w.x);
+                F1 = fmaf(w.y * J, P1, F1);
+                F2 = fmaf(w.z * (J * J), P2, F2);
             }
         }
     } else {
-        const float *__restrict__ AT = t.AT + (size_t)set * NRMAX * t.nlp;
-        const float *__restrict__ BT = t.BT + (size_t)set * NRMAX * t.nlp;
+        const SAS_CONST float *AT = as_const(t.AT) + (size_t)set * NRMAX * t.nlp;
+        const SAS_CONST float *BT = as_const(t.BT) + (size_t)set * NRMAX * t.nlp;
         for (int i0 = 0; i0 < NR; i0 += RCHUNK) {
             float J[RCHUNK], t1[RCHUNK], t2[RCHUNK];
 #pragma unroll
             for (int c = 0; c < RCHUNK; c++) {
-                J[c] = i0 + c < NR ? j1c_f(qs * R[i0 + c]) : 0.0f;
+                J[c] = i0 + c < NR ? j1c_f(qs * ld4(ROW + i0 + c).x) : 0.0f;
                 t1[c] = 0.0f;
                 t2[c] = 0.0f;
             }
             for (int j = 0; j < NL; j++) {
-                const float S = sinc_prod(qc, H[j]), S2 = S * S;
-                const float *a = AT + j * NRMAX + i0, *b = BT + j * NRMAX + i0;
+                const float S = sinc_prod(qc, ld4(COL + j).x), S2 = S * S;
+                const SAS_CONST float *a = AT + j * NRMAX + i0, *b = BT + j * NRMAX + i0;
 #pragma unroll
                 for (int c = 0; c < RCHUNK; c++) {
                     t1[c] = fmaf(a[c], S, t1[c]);

@@ -322,6 +322,30 @@ __global__ void k_sum_splits(const float *__restrict__ P, int splits, int n, flo
 //   x = ((log R, log L, sR, sL, log drho, log10 q) - mean) / std
 constexpr int BMI = 64;
 
+// Features of one (curve, q) row and the analytic baseline of log I.  The
+// polydisperse, orientation-averaged cylinder curve scaled by
+// 1e-4 contrast^2 pi R^2 L depends on q, R, L only through qR and qL, so the
+// features are (log qR, log qL, radius_pd, length_pd, bL, bR) with the
+// smooth envelope terms bL = log(1 + (qL/pi)^2)/2 (thin rod, pi/(qL) at large
+// qL) and bR = 3/2 log(1 + (qR)^2/4) (disc cross-section, (qR)^-3); the
+// network predicts the residual r = log(I/(1e-4 contrast^2 pi R^2 L)) + bL + bR,
+// a quantity of order one.
+__device__ __forceinline__ float features(const float *cp, float lq, float *x)
+{
+    const float ln10 = 2.302585093f;
+    const float lqr = fmaf(lq, ln10, logf(cp[0]));
+    const float lql = fmaf(lq, ln10, logf(cp[1]));
+    const float qr = expf(lqr), ql = expf(lql);
+    const float u = ql * 0.318309886f;
+    x[0] = lqr;
+    x[1] = lql;
+    x[2] = cp[2];
+    x[3] = cp[3];
+    x[4] = 0.5f * log1pf(u * u);
+    x[5] = 1.5f * log1pf(0.25f * qr * qr);
+    return -(x[4] + x[5]);
+}
+
 struct InferArgs {
     const float *theta;
     const __bf16 *pack;
@@ -330,7 +354,7 @@ struct InferArgs {
     const float *logq;        // [nq] log10 q
     int nq;
     const float *norm;        // [12]: mean[6], std[6]
-    int out_iq;               // 0: network output y; 1: I = 1e-4 drho^2 pi R^2 L exp(y)
+    int out_iq;               // 0: network output r; 1: I = 1e-4 drho^2 pi R^2 L exp(r + baseline)
     float *y;                 // [ncurves*nq]
 };
 
@@ -354,13 +378,7 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
                 float x[D] = {0, 0, 0, 0, 0, 0};
                 if (row < rows) {
                     const int c = (int)(row / a.nq), iq = (int)(row % a.nq);
-                    const float *cp = a.curves + (size_t)c * 5;
-                    x[0] = __logf(cp[0]);
-                    x[1] = __logf(cp[1]);
-                    x[2] = cp[2];
-                    x[3] = cp[3];
-                    x[4] = __logf(fabsf(cp[4]));
-                    x[5] = a.logq[iq];
+                    (void)features(a.curves + (size_t)c * 5, a.logq[iq], x);
                 }
                 float z = b1;
 #pragma unroll
@@ -436,8 +454,10 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
                 const float yv = partial[0][tid] + partial[1][tid] + partial[2][tid] + partial[3][tid] + a.theta[OFF_B5];
                 if (a.out_iq) {
                     const float *cp = a.curves + (size_t)(row / a.nq) * 5;
+                    float x[D];
+                    const float base = features(cp, a.logq[row % a.nq], x);
                     const float v0 = 3.14159265f * cp[0] * cp[0] * cp[1];
-                    a.y[row] = 1e-4f * cp[4] * cp[4] * v0 * __expf(yv);
+                    a.y[row] = 1e-4f * cp[4] * cp[4] * v0 * expf(yv + base);
                 } else {
                     a.y[row] = yv;
                 }

@@ -214,11 +214,12 @@ class HipBatch:
     def _pack(self, calls):
         if len(calls) != self.nsets:
             raise ValueError("expected %d (call_details, values) pairs, got %d" % (self.nsets, len(calls)))
-        det = np.concatenate([np.asarray(getattr(cd, "buffer", cd), "i4") for cd, _ in calls])
-        vals = [np.asarray(v, "d") for _, v in calls]
-        voff = np.concatenate([[0], np.cumsum([v.size for v in vals])]).astype(np.int64)
-        ndet = det.size//self.nsets
-        return np.ascontiguousarray(det), np.ascontiguousarray(np.concatenate(vals)), voff, ndet
+        det = np.concatenate([getattr(cd, "buffer", cd) for cd, _ in calls]).astype(np.int32, copy=False)
+        vals = [v for _, v in calls]
+        voff = np.zeros(len(vals) + 1, np.int64)
+        np.cumsum(np.fromiter((len(v) for v in vals), np.int64, len(vals)), out=voff[1:])
+        vals = np.concatenate(vals).astype(np.float64, copy=False)
+        return np.ascontiguousarray(det), np.ascontiguousarray(vals), voff, det.size//self.nsets
 
     def _kind(self, magnetic):
         return KIND_IQ if self.dim == "1d" else (KIND_IMAGNETIC if magnetic else KIND_IQXY)

@@ -1,18 +1,23 @@
 """
 Learned surrogate for the polydisperse, orientation-averaged cylinder I(q).
 
-An MLP (6 -> 4x256 SiLU -> 1) maps the features
+The curve scaled by its nominal forward intensity, I / (1e-4 contrast^2
+pi R^2 L), depends on q, R and L only through qR and qL (the lognormal
+meshes scale with R and L).  An MLP (6 -> 4x256 SiLU -> 1) maps the features
 
-    (log radius, log length, radius_pd, length_pd, log |contrast|, log10 q)
+    (log qR, log qL, radius_pd, length_pd, bL, bR),
+    bL = log(1 + (qL/pi)^2)/2,   bR = 3/2 log(1 + (qR)^2/4)
 
-(standardised) to the network output y, with
+(standardised) to a residual r, with
 
-    I(q) = 1e-4 * contrast^2 * pi R^2 L * exp(y)      (scale 1, background 0)
+    I(q) = 1e-4 * contrast^2 * pi R^2 L * exp(r - bL - bR)   (scale 1, background 0)
 
-so y is the log of the mesh-averaged <V^2 f^2>/<V> over the nominal volume:
-of order 0 at low q, falling with q.  Polydispersity is lognormal with 35
-points over 3 sigma on radius and length (none when the width is 0), and the
-orientation average is Gauss-76 -- the plain kernel's definition.
+bL and bR are smooth envelopes of the thin-rod and disc-cross-section
+factors (pi/(qL) and (qR)^-3 at large argument), so r stays of order one
+over the whole box and the BF16 layers carry only the correction.
+Polydispersity is lognormal with 35 points over 3 sigma on radius and
+length (none when the width is 0), and the orientation average is
+Gauss-76 -- the plain kernel's definition.
 
 Everything that computes runs in libsaship: the training data come from the
 batched plain kernel (HipBatch), the forward and backward passes are the BF16
@@ -68,25 +73,42 @@ def curve_pars(c):
                 length_pd=sL, length_pd_n=PD_N, length_pd_type="lognormal", length_pd_nsigma=3)
 
 
+def _raw(curves, log10q):
+    """(features [n, nq, 6], baseline [n, nq]) in float64."""
+    curves = np.asarray(curves, np.float64)
+    lq = np.asarray(log10q, np.float64)[None, :]*np.log(10.0)
+    lqr = lq + np.log(curves[:, 0])[:, None]
+    lql = lq + np.log(curves[:, 1])[:, None]
+    f = np.empty(lqr.shape + (6,))
+    f[..., 0] = lqr
+    f[..., 1] = lql
+    f[..., 2] = curves[:, 2][:, None]
+    f[..., 3] = curves[:, 3][:, None]
+    f[..., 4] = 0.5*np.log1p((np.exp(lql)/np.pi)**2)
+    f[..., 5] = 1.5*np.log1p(0.25*np.exp(lqr)**2)
+    return f, -(f[..., 4] + f[..., 5])
+
+
 def features(curves, log10q):
     """Raw (unstandardised) features for every (curve, q) row, float64 [n*nq, 6]."""
-    curves = np.asarray(curves, np.float64)
-    n, nq = len(curves), len(log10q)
-    f = np.empty((n, nq, 6))
-    f[:, :, 0] = np.log(curves[:, 0])[:, None]
-    f[:, :, 1] = np.log(curves[:, 1])[:, None]
-    f[:, :, 2] = curves[:, 2][:, None]
-    f[:, :, 3] = curves[:, 3][:, None]
-    f[:, :, 4] = np.log(np.abs(curves[:, 4]))[:, None]
-    f[:, :, 5] = np.asarray(log10q)[None, :]
-    return f.reshape(n*nq, 6)
+    f, _ = _raw(curves, log10q)
+    return f.reshape(-1, 6)
 
 
-def target(curves, Iq):
-    """Network target y = log(I / (1e-4 contrast^2 pi R^2 L)) for I[n, nq]."""
+def target(curves, log10q, Iq):
+    """Network target r = log(I / (1e-4 contrast^2 pi R^2 L)) + bL + bR for I[n, nq]."""
     curves = np.asarray(curves, np.float64)
     v0 = np.pi*curves[:, 0]**2*curves[:, 1]
-    return np.log(np.asarray(Iq)/(1e-4*curves[:, 4, None]**2*v0[:, None]))
+    _, base = _raw(curves, log10q)
+    return np.log(np.asarray(Iq)/(1e-4*curves[:, 4, None]**2*v0[:, None])) - base
+
+
+def iq_from_output(curves, log10q, r):
+    """I(q) [n, nq] from network outputs r [n, nq]."""
+    curves = np.asarray(curves, np.float64)
+    v0 = np.pi*curves[:, 0]**2*curves[:, 1]
+    _, base = _raw(curves, log10q)
+    return 1e-4*curves[:, 4, None]**2*v0[:, None]*np.exp(np.asarray(r, np.float64) + base)
 
 
 def sample_box(n, rng):

@@ -53,7 +53,7 @@ def main(argv=None):
     raw = features(curves, lq)
     sur.set_normalisation(raw)
     X = torch.as_tensor(sur.standardise(raw), device="cuda")
-    t = torch.as_tensor(target(curves, Iq).ravel().astype(np.float32), device="cuda")
+    t = torch.as_tensor(target(curves, lq, Iq).ravel().astype(np.float32), device="cuda")
     t0 = time.time()
     hist = sur.train(X, t, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed,
                      log=lambda m: print(m, flush=True))

@@ -70,8 +70,8 @@ def test_inference_kernel_equals_training_forward(sur):
     X = torch.as_tensor(sur.standardise(raw), device="cuda")
     y_train = sur.forward(X).cpu().numpy()
     y_inf = sur.predict(curves, lq, out_iq=False).cpu().numpy()
-    # The fused kernel builds the features on the device (logf in FP32) and the
-    # training path on the host (FP64 log, then FP32): one-ulp differences in a
+    # The fused kernel builds the features on the device (FP32 log/exp) and the
+    # training path on the host (FP64, then FP32): one-ulp differences in a
     # feature can flip a BF16 rounding of a layer input.  Almost every row
     # agrees to FP32 summation order; the rest differ by a few BF16 ulps.
     diff = np.abs(y_inf - y_train)
@@ -79,9 +79,8 @@ def test_inference_kernel_equals_training_forward(sur):
     assert np.mean(diff <= 1e-5*scale) >= 0.95
     assert diff.max() <= 1e-2*scale
     I = sur.predict(curves, lq, out_iq=True).cpu().numpy().reshape(37, 61)
-    v0 = np.pi*curves[:, 0]**2*curves[:, 1]
-    np.testing.assert_allclose(I, 1e-4*curves[:, 4, None]**2*v0[:, None]*np.exp(y_inf.reshape(37, 61)),
-                               rtol=2e-5)
+    from sasmodels_amd.surrogate import iq_from_output
+    np.testing.assert_allclose(I, iq_from_output(curves, lq, y_inf.reshape(37, 61)), rtol=5e-5)
 
 
 def test_training_fits_plain_kernel_data_and_gate_falls_back(sur):
@@ -95,7 +94,7 @@ def test_training_fits_plain_kernel_data_and_gate_falls_back(sur):
     sur.init_params(7)
     sur.set_normalisation(raw)
     X = torch.as_tensor(sur.standardise(raw), device="cuda")
-    t = torch.as_tensor(target(curves, Iq).ravel().astype(np.float32), device="cuda")
+    t = torch.as_tensor(target(curves, lq, Iq).ravel().astype(np.float32), device="cuda")
     hist = sur.train(X, t, epochs=30, batch=2048, lr=2e-3)
     print("loss", hist[0], hist[-1])
     assert hist[-1] < 0.02*hist[0]
