@@ -20,8 +20,10 @@ Paths, one JSON line:
                             its box, against the plain kernel on those curves
 roofline: FLOPs of the separable evaluation (counted with the reference's
 formulas for J1 and sin, see separable_flops_cylinder) over the device time
-of k_batch_sep (HIP events on the batch's stream); traffic from the counter
-summary in profiles/ when present.  cpu_baseline: the FP64 oracle on the
+of k_batch_sep (HIP events on the batch's stream), against the published
+FP32 peak and against the FMA rate measured on the device (saship_probe_fma,
+non-packed instructions); traffic and VALU instruction counts of the same
+launch from the counter summary in profiles/ when present.  cpu_baseline: the FP64 oracle on the
 host cores, on a bounded sample of the workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
@@ -299,11 +301,24 @@ def main():
     achieved = flops/(batched["eval_kernel_ms"]*1e-3)/1e12
     ref_flops = reference_flops_cylinder_fq(C2_Q, np.repeat(radii, lengths.size))*args.batch
     generic["roofline_tflops_reference_formula"] = ref_flops/(generic["eval_kernel_ms"]*1e-3)/1e12
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_batch_sep_cylinder_c2_traffic.json")
+    # counter figures of the same launch from the committed profile summary
+    traffic = valu_per_launch = None
+    tfile = os.path.join(ROOT, "profiles", "k_batch_sep_cylinder_c2_counters.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        valu_per_launch = prof.get("valu_wave_instructions_per_launch")
+    from sasmodels_amd.kernelhip import probe_fma
+    fma_tflops = probe_fma()
+    valu_issue = None
+    if valu_per_launch:
+        # wave64 FMA instructions/s of the probe = TFLOP/s * 1e12 / (2 * 64)
+        rate = valu_per_launch/(batched["eval_kernel_ms"]*1e-3)
+        valu_issue = {"valu_wave_instructions_per_launch": valu_per_launch,
+                      "valu_wave_instructions_per_s": rate,
+                      "probe_fma_wave_instructions_per_s": fma_tflops*1e12/128.0,
+                      "issue_frac": rate/(fma_tflops*1e12/128.0)}
 
     # CPU leg: oracle timing on whole curves; its curves are the FP64 references
     baseline, acc = None, {"max": None, "median": None}
@@ -340,7 +355,9 @@ def main():
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved/PEAK_FP32_TFLOPS,
                      "flops_per_launch": flops, "flops_per_curve": flops_curve,
                      "avg_kernel_ms": batched["eval_kernel_ms"], "traffic": traffic,
-                     "reference_formula_tflops": ref_flops/(batched["eval_kernel_ms"]*1e-3)/1e12},
+                     "reference_formula_tflops": ref_flops/(batched["eval_kernel_ms"]*1e-3)/1e12,
+                     "measured_fp32_fma_peak_tflops": fma_tflops, "frac_of_measured_peak": achieved/fma_tflops,
+                     "valu_issue": valu_issue},
         "paths": {"batched": batched, "batched_cutoff": cut, "batched_generic": generic,
                   "plain": plain, "plain_generic": plain_generic},
     }

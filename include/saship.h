@@ -70,6 +70,12 @@ int saship_device_count(int32_t *count);
  * Replaces the context set-up of sasmodels/kernelcl.py:246-313. */
 int saship_init(int32_t device);
 
+/* Measured FP32 FMA rate of the current device in TFLOP/s (non-packed
+ * instructions, eight independent chains per lane, 16 workgroups of 256
+ * per CU, `iters` x 64 FMAs per lane): the issue ceiling the FP32
+ * evaluation kernels run against.  No reference counterpart. */
+int saship_probe_fma(int32_t iters, double *tflops);
+
 /* Model handle: form factor plus orientation quadrature.  quad_n == 0 is
  * the reference default, adaptive 20/76/500/5000 points by q*r_max
  * (models/lib/adaptive.c:11276-11298); quad_n > 0 fixes an n-point

@@ -31,6 +31,7 @@ SIGNATURES = {
     "saship_kernel_call": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
                                       ct.c_void_p, ct.c_int32, ct.c_double, ct.c_int32,
                                       ct.c_int32, ct.c_int32, ct.c_void_p]),
+    "saship_probe_fma": (ct.c_int, [ct.c_int32, ct.POINTER(ct.c_double)]),
     "saship_kernel_set_path": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_batch_set_path": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_kernel_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),

@@ -167,6 +167,26 @@ int launch_sep(saship_kernel *k, int nlp, const int32_t *d_det, const double *d_
 
 static_assert(SEP_MAX == sas::sep::NRMAX, "separable table size");
 
+// FP32 FMA issue-rate probe: eight independent chains per lane, no memory
+// traffic besides one store per thread (the ceiling the evaluation kernels
+// are compared with; the compiler flags forbid packed FP32 instructions).
+__global__ __launch_bounds__(256) void k_fma_probe(float *out, int iters, float a, float b)
+{
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = (float)(threadIdx.x + k);
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = fmaf(x[k], a, b);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += x[k];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -190,6 +210,39 @@ int saship_init(int32_t device)
         return fail(SASHIP_EINVAL, "device " + std::to_string(device) + " out of range (" + std::to_string(n) + " devices)");
     HIP_TRY(hipSetDevice(device));
     current_device() = device;
+    return 0;
+}
+
+int saship_probe_fma(int32_t iters, double *tflops)
+{
+    if (!tflops) return fail(SASHIP_EINVAL, "tflops is NULL");
+    if (iters < 1 || iters > (1 << 20)) return fail(SASHIP_EINVAL, "iters must be 1..2^20");
+    HIP_TRY(hipSetDevice(current_device()));
+    const int blocks = cu_count(current_device()) * 16;
+    float *out = nullptr;
+    HIP_TRY(hipMalloc((void **)&out, sizeof(float) * 256 * (size_t)blocks));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) {
+        k_fma_probe<<<blocks, 256>>>(out, 16, 0.999f, 0.001f);     // warm-up
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) {
+        k_fma_probe<<<blocks, 256>>>(out, iters, 0.999f, 0.001f);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (e != hipSuccess) return fail(SASHIP_EHIP, std::string("fma probe: ") + hipGetErrorString(e));
+    const double flops = 2.0 * 64.0 * iters * 256.0 * blocks;
+    *tflops = ms > 0.0f ? flops / (ms * 1e-3) / 1e12 : 0.0;
     return 0;
 }
 

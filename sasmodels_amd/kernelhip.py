@@ -46,6 +46,14 @@ def _ensure_device():
         _device_ready = True
 
 
+def probe_fma(iters=4096):
+    """Measured non-packed FP32 FMA rate of the device in TFLOP/s (saship_probe_fma)."""
+    _ensure_device()
+    out = ct.c_double()
+    _lib.check(_lib.lib().saship_probe_fma(int(iters), ct.byref(out)))
+    return out.value
+
+
 class HipModel(KernelModel):
     """A model compiled for the GPU (one per model and quadrature)."""
 
