@@ -9,10 +9,12 @@ kernels with compensated sums and FP64 weight sums, cutoff 0.
 
 Paths, one JSON line:
   batched          (value)  a step evaluates BATCH curves of the workload in
-                            one persistent launch (separable cylinder path);
-                            every curve has its own radius and length
-                            (jittered by step and curve), all packed before
-                            the timed region
+                            one persistent launch (separable cylinder path)
+                            with its inputs resident in HBM (device work
+                            only, replayed back to back); the host-inclusive
+                            rate -- a full call per step with new jittered
+                            inputs: packing, upload, graph replay, download
+                            -- is reported beside it
   batched_cutoff            the same at cutoff 1e-5 (pruned mesh corners)
   batched_generic           the same on the generic mesh kernel
   plain, plain_generic      one curve per reference-shaped call
@@ -26,7 +28,12 @@ non-packed instructions); traffic and VALU instruction counts of the same
 launch from the counter summary in profiles/ when present.  cpu_baseline: the FP64 oracle on the
 host cores, on a bounded sample of the workload.
 
+other_configs: the sphere 1D, ellipsoid 2D, batched core-shell cylinder and
+batched magnetic core-shell sphere configurations, each with device
+throughput, FP64 accuracy on a subset and the oracle's rate on the host.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+                    [--no-surrogate] [--no-configs]
 """
 
 import argparse
@@ -177,6 +184,14 @@ def bench_plain(model, info, steps, warmup, separable=True):
 
 
 def bench_batched(model, info, steps, warmup, batch_size, separable=True, cutoff=0.0):
+    """
+    Two timings of a batch of the workload: (1) device-resident, the value:
+    `steps` back-to-back runs of the device work (expand, evaluate, final
+    sums) with inputs and outputs in HBM, wall clock bracketed by
+    synchronisation; (2) host-inclusive: `steps` full calls with different
+    jittered inputs each (host packing, staging upload, hipGraph replay,
+    result download).
+    """
     from sasmodels_amd.kernelhip import HipBatch
     batch = HipBatch(model, [[C2_Q]]*batch_size, use_graph=True)
     batch.set_separable(separable)
@@ -187,7 +202,7 @@ def bench_batched(model, info, steps, warmup, batch_size, separable=True, cutoff
     t0 = time.perf_counter()
     for calls in groups[warmup:]:
         out = batch(calls, cutoff)
-    dt = time.perf_counter() - t0
+    dt_host = time.perf_counter() - t0
     # device time of the evaluation kernel (eager launches with events)
     batch.set_timing(True)
     for calls in groups[warmup:warmup + min(steps, 10)]:
@@ -195,8 +210,15 @@ def bench_batched(model, info, steps, warmup, batch_size, separable=True, cutoff
     ms, n = batch.get_timing()
     batch.set_timing(False)
     first = batch(groups[0], cutoff)
+    batch.replay(max(1, warmup))
+    t0 = time.perf_counter()
+    dev_ms = batch.replay(steps)
+    dt = time.perf_counter() - t0
     batch.release()
-    return ({"curves_per_s": steps*batch_size/dt, "ms_per_step": dt/steps*1e3, "batch": batch_size,
+    return ({"curves_per_s": steps*batch_size/dt, "ms_per_step": dt/steps*1e3,
+             "device_ms_per_step": dev_ms/steps, "batch": batch_size,
+             "curves_per_s_host_inclusive": steps*batch_size/dt_host,
+             "ms_per_step_host_inclusive": dt_host/steps*1e3,
              "cutoff": cutoff, "separable": separable,
              "pts_per_item": pts, "waves_per_cu": waves, "graph": True,
              "eval_kernel_ms": ms/max(n, 1)}, groups[0], first)
@@ -253,6 +275,148 @@ def bench_surrogate(steps=20):
             "speedup_vs_plain": (ncurves/(ms*1e-3))/(nsub/t_plain)}
 
 
+def _timed(fn, reps):
+    fn()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    return (time.perf_counter() - t0)/reps, out
+
+
+def _oracle_leg(info, jobs, threads):
+    """CPU leg of a secondary configuration: the oracle on `jobs` =
+    [(q_vectors, details, values, magnetic)], timed; returns (seconds, I list)."""
+    from oracle import oracle
+    from oracle.oracle import OracleKernel
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    refs = [OracleKernel(info, qv)(det, val, 0.0, mag) for qv, det, val, mag in jobs]
+    return time.perf_counter() - t0, refs
+
+
+def bench_configs(threads, cpu=True):
+    """
+    The other benchmark configurations, one entry each: device throughput
+    (graph replay for batches), the FP64 accuracy and the oracle's time on a
+    bounded subset (CPU leg, scaled to the same unit).
+    """
+    from sasmodels_amd import core, direct_model, generate
+    from sasmodels_amd.details import make_kernel_args
+    from sasmodels_amd.kernelhip import HipBatch
+    rng = np.random.default_rng(SEED)
+    out = {}
+
+    # C1: sphere 1D, 200 q, gaussian radius pd 0.1 x 35; 4096 jittered curves per launch
+    info = core.load_model_info("sphere")
+    model = core.build_model(info, platform="hip")
+    q = np.logspace(-3, np.log10(0.5), 200)
+    kl = _kernel_like(info)
+    calls = [make_kernel_args(kl, direct_model.get_mesh(info, dict(
+        radius=50*(1 + 1e-3*(k % 101)), radius_pd=0.1, radius_pd_n=35, radius_pd_nsigma=3, sld=4, sld_solvent=1,
+        scale=1, background=1e-3), dim="1d"))[:2] for k in range(4096)]
+    b = HipBatch(model, [[q]]*len(calls), use_graph=True)
+    b.autotune(calls)
+    dt, iq = _timed(lambda: b(calls), 10)
+    b.release()
+    ent = {"workload": "sphere 1D, 200 q, radius 50 gaussian pd 0.1 (35 points), 4096 curves per launch",
+           "curves_per_s": len(calls)/dt, "ms_per_launch": dt*1e3, "qpd_evals_per_s": len(calls)*200*35/dt}
+    if cpu:
+        t, refs = _oracle_leg(info, [([q], d, v, False) for d, v in calls[:64]], threads)
+        ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), r).max() for a, r in zip(iq, refs)))
+        ent["cpu_curves_per_s"] = 64/t
+    out["sphere_1d_batched"] = ent
+    model.release()
+
+    # C3: ellipsoid 2D on 256 x 256, theta/phi/radius_equatorial dispersion 20 x 20 x 20
+    info = core.load_model_info("ellipsoid")
+    model = core.build_model(info, platform="hip")
+    x = np.linspace(-0.3, 0.3, 256)
+    qx, qy = [a.ravel() for a in np.meshgrid(x, x)]
+    pars = dict(radius_polar=20, radius_equatorial=400, theta=60, phi=60, theta_pd=10, theta_pd_n=20,
+                phi_pd=20, phi_pd_n=20, radius_equatorial_pd=0.1, radius_equatorial_pd_n=20)
+    k = model.make_kernel([qx, qy])
+    mesh = direct_model.get_mesh(info, pars, dim="2d")
+    det, val, mag = make_kernel_args(k, mesh)
+    k.set_timing(True)
+    dt, img = _timed(lambda: k(det, val, 0.0, mag), 5)
+    ms, n = k.get_timing()
+    k.release()
+    ent = {"workload": "ellipsoid 2D, 256 x 256 detector, 20 x 20 x 20 mesh (theta, phi, radius_equatorial)",
+           "images_per_s": 1.0/dt, "eval_kernel_ms": ms/max(n, 1),
+           "qpd_evals_per_s": qx.size*int(det.num_eval)/(ms/max(n, 1)*1e-3)}
+    if cpu:
+        sub = rng.choice(qx.size, 1024, replace=False)
+        t, refs = _oracle_leg(info, [([qx[sub], qy[sub]], det, val, mag)], threads)
+        ent["max_rel_err_vs_fp64"] = float(masked_relerr(np.asarray(img)[sub], refs[0]).max())
+        ent["max_rel_err_pixels"] = 1024
+        ent["cpu_images_per_s"] = 1024/(t*qx.size)
+    out["ellipsoid_2d"] = ent
+    model.release()
+
+    # C4a: 4096 core-shell cylinder sets, uneven q counts and radius meshes, Gauss-76
+    info = core.load_model_info("core_shell_cylinder")
+    generate.set_integration_size(info, QUAD)
+    model = core.build_model(info, platform="hip")
+    kl = _kernel_like(info)
+    q_list, calls = [], []
+    for _ in range(4096):
+        nq = int(rng.integers(64, 2049))
+        q_list.append([np.logspace(-3, np.log10(0.5), nq)])
+        p = dict(radius=rng.uniform(10, 100), thickness=rng.uniform(5, 30), length=rng.uniform(50, 1000),
+                 sld_core=rng.uniform(-1, 7), sld_shell=rng.uniform(-1, 7), sld_solvent=rng.uniform(-1, 7),
+                 radius_pd=rng.uniform(0, 0.2), radius_pd_n=int(rng.integers(1, 36)))
+        calls.append(make_kernel_args(kl, direct_model.get_mesh(info, p, dim="1d"))[:2])
+    b = HipBatch(model, q_list, use_graph=True)
+    b.autotune(calls)
+    dt, iq = _timed(lambda: b(calls), 3)
+    b.set_timing(True)
+    b(calls)
+    ms, n = b.get_timing()
+    b.release()
+    evals = sum(qv[0].size*int(d.num_eval) for qv, (d, _) in zip(q_list, calls))
+    ent = {"workload": "4096 core_shell_cylinder sets, nq U{64..2048}, radius pd U[0,0.2] x U{1..35} points, "
+                       "Gauss-76, one persistent launch", "sets_per_s": 4096/dt, "ms_per_launch": dt*1e3,
+           "eval_kernel_ms": ms/max(n, 1), "qpd_evals_per_s": evals/(ms/max(n, 1)*1e-3)}
+    if cpu:
+        t, refs = _oracle_leg(info, [(qv, d, v, False) for qv, (d, v) in zip(q_list[:16], calls[:16])], threads)
+        ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), r).max() for a, r in zip(iq, refs)))
+        e16 = sum(qv[0].size*int(d.num_eval) for qv, (d, _) in zip(q_list[:16], calls[:16]))
+        ent["cpu_sets_per_s"] = 4096/(t*evals/e16)
+    out["core_shell_cylinder_batched"] = ent
+    model.release()
+
+    # C4b: 256 magnetic core-shell sphere sets on a 128 x 128 detector, four spin states
+    info = core.load_model_info("core_shell_sphere")
+    model = core.build_model(info, platform="hip")
+    kl = _kernel_like(info)
+    x = np.linspace(-0.3, 0.3, 128)
+    qx, qy = [a.ravel() for a in np.meshgrid(x, x)]
+    calls = []
+    for _ in range(256):
+        p = dict(sld_core_M0=rng.uniform(0, 3), sld_shell_M0=rng.uniform(0, 3), sld_solvent_M0=rng.uniform(0, 3),
+                 sld_core_mtheta=rng.uniform(0, 90), sld_core_mphi=rng.uniform(0, 360),
+                 sld_shell_mtheta=rng.uniform(0, 90), sld_shell_mphi=rng.uniform(0, 360),
+                 sld_solvent_mtheta=rng.uniform(0, 90), sld_solvent_mphi=rng.uniform(0, 360),
+                 up_frac_i=rng.uniform(0, 1), up_frac_f=rng.uniform(0, 1), up_theta=rng.uniform(0, 90))
+        calls.append(make_kernel_args(kl, direct_model.get_mesh(info, p, dim="2d"))[:2])
+    b = HipBatch(model, [[qx, qy]]*len(calls), use_graph=True)
+    b.autotune(calls, magnetic=True)
+    dt, img = _timed(lambda: b(calls, magnetic=True), 3)
+    b.release()
+    ent = {"workload": "256 magnetic core_shell_sphere sets, 128 x 128 detector, 4 spin states",
+           "sets_per_s": 256/dt, "ms_per_launch": dt*1e3}
+    if cpu:
+        t, refs = _oracle_leg(info, [([qx, qy], d, v, True) for d, v in calls[:2]], threads)
+        ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), r).max() for a, r in zip(img, refs)))
+        ent["cpu_sets_per_s"] = 2/t
+    out["magnetic_core_shell_sphere_batched"] = ent
+    model.release()
+    if cpu:
+        for e in out.values():
+            e["cpu_threads"] = threads
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +425,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other benchmark configurations")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a kernel of known traffic for "
                          "scaling hardware byte counters)")
@@ -363,6 +528,8 @@ def main():
     }
     if not args.no_surrogate:
         line["paths"]["surrogate"] = bench_surrogate()
+    if not args.no_configs:
+        line["other_configs"] = bench_configs(threads, cpu=not args.no_cpu)
     if baseline:
         line["cpu_baseline"] = baseline
         line["speedup_vs_cpu"] = line["value"]/baseline["value"]

@@ -146,6 +146,13 @@ int saship_batch_call(saship_batch *batch, int32_t kind,
                       double cutoff, int32_t radius_effective_mode,
                       double *result, double *iq);
 
+/* Runs the device work of the last successful saship_batch_call again
+ * nsteps times back to back with its inputs already in device memory and
+ * its outputs left there (no staging upload, no result download), and
+ * returns the elapsed device time of all nsteps in device_ms: the
+ * device-resident throughput of the call.  No reference counterpart. */
+int saship_batch_replay(saship_batch *batch, int32_t nsteps, double *device_ms);
+
 /* Launch shape: use_graph replays a captured hipGraph for repeated calls of
  * the same shape; pts_per_item is the mesh chunk of one work item (on the
  * separable cylinder path: Gauss nodes per item); waves_per_cu sizes the

@@ -230,6 +230,23 @@ __global__ __launch_bounds__(256) void k_batch_final(
 
 }  // namespace sas
 
+namespace batch_host {
+
+// Host-side plan of one call: the staging layout and the work list.
+struct Plan {
+    int kind = 0, nout = 2, ndet = 0, rv = 1;
+    int64_t n_values = 0, rec_total = 0, partial_total = 0, result_total = 0, nq_total = 0;
+    int n_items = 0;
+    int sep_nlp = 0;                  // separable path: bilinear table columns (0: generic path)
+    int nodes_per_item = 0;           // separable path: Gauss nodes per work item
+    size_t off_values = 0, off_voff = 0, off_det = 0, off_roff = 0, off_poff = 0, off_resoff = 0;
+    size_t off_nchunk = 0, off_items = 0, off_nitems = 0, off_order = 0, off_ipre = 0, bytes = 0;
+    int nblk = 0;                     // separable path: node blocks per (set, tile)
+    int max_nq = 0;                   // largest q count of a set
+};
+
+}  // namespace batch_host
+
 struct saship_batch {
     saship_model *m = nullptr;
     int nsets = 0, is2d = 0, dev = 0, cus = 256;
@@ -265,6 +282,12 @@ struct saship_batch {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<int64_t> gkey;
+    // the last successful call, for device-resident replays
+    bool have_last = false;
+    double last_cutoff = 0.0;
+    int last_reff = 0;
+    bool last_result = false, last_iq = false;
+    batch_host::Plan last;
     // timing of the evaluation kernel
     hipEvent_t ev0 = nullptr, ev1 = nullptr;     // around the evaluation kernel
     hipEvent_t ev2 = nullptr, ev3 = nullptr;     // around the whole device sequence
@@ -275,6 +298,8 @@ struct saship_batch {
 };
 
 namespace {
+
+using batch_host::Plan;
 
 void drop_graph(saship_batch *b)
 {
@@ -287,18 +312,6 @@ void drop_graph(saship_batch *b)
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Host-side plan of one call: the staging layout and the work list.
-struct Plan {
-    int kind = 0, nout = 2, ndet = 0, rv = 1;
-    int64_t n_values = 0, rec_total = 0, partial_total = 0, result_total = 0, nq_total = 0;
-    int n_items = 0;
-    int sep_nlp = 0;                  // separable path: bilinear table columns (0: generic path)
-    int nodes_per_item = 0;           // separable path: Gauss nodes per work item
-    size_t off_values = 0, off_voff = 0, off_det = 0, off_roff = 0, off_poff = 0, off_resoff = 0;
-    size_t off_nchunk = 0, off_items = 0, off_nitems = 0, off_order = 0, off_ipre = 0, bytes = 0;
-    int nblk = 0;                     // separable path: node blocks per (set, tile)
-    int max_nq = 0;                   // largest q count of a set
-};
 
 template <class M>
 int rv_of(int kind) { return kind == 0 ? sas::rv1<M>() : sas::rv2<M>(); }
@@ -437,8 +450,11 @@ int enqueue_sep(saship_batch *b, const Plan &p, const sas::ExpandArgs &ea, const
     return 0;
 }
 
+// The device work of one call; with `copies` the staging upload before it
+// and the result downloads after it (without: inputs and outputs stay in HBM).
 template <class M>
-int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq)
+int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq,
+            bool copies)
 {
     using namespace sas;
     const char *d = b->d_args;
@@ -454,8 +470,8 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     double *d_result = b->d_out;
     double *d_iq = b->d_out + p.result_total;
 
+    if (copies) HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
     if (b->timing) HIP_TRY(hipEventRecord(b->ev2, b->st));
-    HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
     ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
                   b->d_recs, b->d_npts, b->d_sums, b->d_queue};
     if (p.sep_nlp) {
@@ -489,6 +505,7 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
                                                 want_result ? d_result : nullptr, want_iq ? d_iq : nullptr);
     HIP_TRY(hipGetLastError());
     if (b->timing) HIP_TRY(hipEventRecord(b->ev3, b->st));
+    if (!copies) return 0;
     if (want_result)
         HIP_TRY(hipMemcpyAsync(b->h_out, d_result, sizeof(double) * p.result_total, hipMemcpyDeviceToHost, b->st));
     if (want_iq)
@@ -497,14 +514,15 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     return 0;
 }
 
-int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq)
+int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq,
+                  bool copies = true)
 {
     switch (b->m->id) {
-    case 0: return enqueue<sas::Sphere>(b, p, cutoff, reff_mode, want_result, want_iq);
-    case 1: return enqueue<sas::Cylinder>(b, p, cutoff, reff_mode, want_result, want_iq);
-    case 2: return enqueue<sas::Ellipsoid>(b, p, cutoff, reff_mode, want_result, want_iq);
-    case 3: return enqueue<sas::CoreShellCylinder>(b, p, cutoff, reff_mode, want_result, want_iq);
-    default: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq);
+    case 0: return enqueue<sas::Sphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 1: return enqueue<sas::Cylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 2: return enqueue<sas::Ellipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 3: return enqueue<sas::CoreShellCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    default: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }
 
@@ -536,6 +554,7 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
         const int64_t *voff, double cutoff, int reff_mode, double *result, double *iq)
 {
     Plan p;
+    b->have_last = false;
     if (int e = plan_and_stage(b, kind, details, n_details, values, voff, p)) return e;
     if (int e = size_buffers(b, p)) return e;
     const bool want_result = result != nullptr, want_iq = iq != nullptr;
@@ -566,6 +585,12 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
         if (int e = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq)) return e;
     }
     HIP_TRY(hipStreamSynchronize(b->st));
+    b->last = p;
+    b->last_cutoff = cutoff;
+    b->last_reff = reff_mode;
+    b->last_result = want_result;
+    b->last_iq = want_iq;
+    b->have_last = true;
     if (want_result) memcpy(result, b->h_out, sizeof(double) * p.result_total);
     if (want_iq) memcpy(iq, b->h_out + sizeof(double) * p.result_total, sizeof(double) * p.nq_total);
     if (b->timing) {
@@ -677,6 +702,35 @@ int saship_batch_call(saship_batch *b, int32_t kind, const int32_t *details, int
     std::lock_guard<std::mutex> lock(b->mu);
     HIP_TRY(hipSetDevice(b->dev));
     return run(b, kind, details, n_details, values, values_offset, cutoff, radius_effective_mode, result, iq);
+}
+
+int saship_batch_replay(saship_batch *b, int32_t nsteps, double *device_ms)
+{
+    if (!b || !device_ms) return fail(SASHIP_EINVAL, "NULL argument");
+    if (nsteps < 1) return fail(SASHIP_EINVAL, "nsteps must be >= 1");
+    std::lock_guard<std::mutex> lock(b->mu);
+    if (!b->have_last) return fail(SASHIP_EINVAL, "no successful saship_batch_call to replay");
+    HIP_TRY(hipSetDevice(b->dev));
+    const bool timing = b->timing;
+    b->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, b->st);
+    int rc = 0;
+    for (int i = 0; i < nsteps && e == hipSuccess && rc == 0; i++)
+        rc = enqueue_model(b, b->last, b->last_cutoff, b->last_reff, b->last_result, b->last_iq, false);
+    if (e == hipSuccess && rc == 0) e = hipEventRecord(e1, b->st);
+    if (e == hipSuccess && rc == 0) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess && rc == 0) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    b->timing = timing;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SASHIP_EHIP, std::string("replay: ") + hipGetErrorString(e));
+    *device_ms = ms;
+    return 0;
 }
 
 int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details, int32_t n_details,

@@ -274,6 +274,13 @@ class HipBatch:
             float(cutoff), ct.byref(p), ct.byref(w)))
         return p.value, w.value
 
+    def replay(self, nsteps):
+        """Device time in ms of nsteps back-to-back runs of the last call's
+        device work, inputs and outputs resident in device memory."""
+        ms = ct.c_double()
+        _lib.check(_lib.lib().saship_batch_replay(self._handle, int(nsteps), ct.byref(ms)))
+        return ms.value
+
     def set_timing(self, enable=True):
         _lib.check(_lib.lib().saship_batch_set_timing(self._handle, 1 if enable else 0))
 

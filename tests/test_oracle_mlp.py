@@ -46,3 +46,25 @@ def test_forward_shapes_and_exact_mode():
     y2, _ = mlp.forward(theta, X, emulate_bf16=True)
     assert np.max(np.abs(y2 - y)) < 0.05*np.max(np.abs(y)) + 1e-3
     assert mlp.NPARAM == 6*256 + 256 + 3*(256*256 + 256) + 256 + 1
+
+
+def test_forward_matches_torch_float64():
+    """The exact (FP64) forward against torch.nn.functional in float64 with
+    the documented layout: W1[6][256], b1, W2..W4[256][256], b2..b4, w5[256], b5."""
+    torch = pytest.importorskip("torch")
+    F = torch.nn.functional
+    rng = np.random.default_rng(4)
+    theta = mlp.init(1).astype(np.float64) + rng.normal(0, 0.01, mlp.NPARAM)
+    X = rng.normal(size=(9, mlp.D))
+    y, _ = mlp.forward(theta, X, emulate_bf16=False)
+    parts, off = [], 0
+    for _, shape in mlp.SIZES:
+        n = int(np.prod(shape))
+        parts.append(torch.from_numpy(theta[off:off + n].reshape(shape)))
+        off += n
+    h = torch.from_numpy(X)
+    for k in range(4):
+        W, b = parts[2*k], parts[2*k + 1]
+        h = F.silu(F.linear(h, W.T, b))
+    ref = F.linear(h, parts[8].reshape(1, -1), parts[9]).reshape(-1).numpy()
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
