@@ -468,7 +468,8 @@ constexpr int NRMAX = 64;    // radius and length points per set (either axis)
 constexpr int RCHUNK = 8;    // bilinear mode: rows per pass
 
 // Per-set tables, set s at ROW/COL/GRP + s*NRMAX, AT/BT + s*nlp*NRMAX,
-// dims + 4*s.  Prefix mode: ROW[r] = (R, u1, u2, 0) for the rows that accept
+// dims + 4*s.  Rotation mode: ROW/COL in grid order with the grid step in
+// .w, no groups.  Prefix mode: ROW[r] = (R, u1, u2, 0) for the rows that accept
 // at least one column, by ascending accepted count; COL[m] = (L/2, v1, v2, 0)
 // by descending weight; GRP[g] = (columns, rows) at the end of the g-th run
 // of rows with equal accepted count.  Bilinear mode: ROW[i].x = R_i,
@@ -479,7 +480,7 @@ struct Tables {
     int2 *GRP;
     float *AT;       // bilinear mode [nlp][NRMAX]: w * 1e-2 * drho * V * pi/4 at (row, column)
     float *BT;       // bilinear mode [nlp][NRMAX]: w * 1e-4 * (drho * V)^2 * pi/4
-    int *dims;       // NR, NL, mode (1 prefix, 0 bilinear), number of row groups
+    int *dims;       // NR, NL, mode (2 rotation, 1 prefix, 0 bilinear), number of row groups
     int nlp;         // columns per set in AT/BT (>= every set's NL)
 };
 
@@ -536,7 +537,8 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     const double *lv = pd_value + pd_off[kl], *lw = pd_weight + pd_off[kl];
     const int tid = threadIdx.x;
     __shared__ unsigned char acc[NRMAX * NRMAX];
-    __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad, nzero, ngroups;
+    __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad, nzero, ngroups, mode;
+    __shared__ float steps[2];
     if (a.queue && set == 0 && tid == 0) a.queue[0] = 0;
     for (int i = tid; i < NRMAX * nlp; i += EXPAND_THREADS) {
         AT[i] = 0.0f;
@@ -630,6 +632,21 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         }
         nzero = z;
         ngroups = g;
+        // every point accepted and both grids equally spaced: rotation mode
+        int md = prefix ? 1 : 0;
+        if (prefix && z == 0 && g == 1 && rowk[roworder[0]] == NL) {
+            const double dr = NR > 1 ? (rv[NR - 1] - rv[0]) / (NR - 1) : 0.0;
+            const double dl = NL > 1 ? (lv[NL - 1] - lv[0]) / (NL - 1) : 0.0;
+            bool ap = true;
+            for (int i = 0; i < NR; i++) ap = ap && fabs(rv[i] - (rv[0] + i * dr)) <= 1e-9 * fabs(rv[NR - 1]);
+            for (int j = 0; j < NL; j++) ap = ap && fabs(lv[j] - (lv[0] + j * dl)) <= 1e-9 * fabs(lv[NL - 1]);
+            if (ap) {
+                md = 2;
+                steps[0] = (float)dr;
+                steps[1] = (float)(0.5 * dl);
+            }
+        }
+        mode = md;
     }
     __syncthreads();
     if (tid == 0) {
@@ -640,7 +657,7 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         int *d = t.dims + 4 * set;
         d[0] = NR;
         d[1] = NL;
-        d[2] = prefix ? 1 : 0;
+        d[2] = mode;
         d[3] = ngroups;
     }
     if (tid < NRMAX) {
@@ -654,15 +671,19 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
             const double drho = base[0] - base[1];
             const double c1 = wo * 1e-2 * drho * PI_D * PI_4_D;
             const double c2 = wo * 1e-4 * drho * drho * PI_D * PI_D * PI_4_D;
+            // rotation mode: grid order, grid steps in .w
+            const bool rot = mode == 2;
             if (m < NR - nzero) {
-                const int i = roworder[m + nzero];
+                const int i = rot ? m : roworder[m + nzero];
                 const double x = rv[i];
-                row = make_float4((float)x, (float)(rw[i] * x * x), (float)(rw[i] * x * x * x * x), 0.0f);
+                row = make_float4((float)x, (float)(rw[i] * x * x), (float)(rw[i] * x * x * x * x),
+                                  rot ? steps[0] : 0.0f);
             }
             if (m < NL) {
-                const int j = colorder[m];
+                const int j = rot ? m : colorder[m];
                 const double x = lv[j];
-                col = make_float4((float)(0.5 * x), (float)(lw[j] * x * c1), (float)(lw[j] * x * x * c2), 0.0f);
+                col = make_float4((float)(0.5 * x), (float)(lw[j] * x * c1), (float)(lw[j] * x * x * c2),
+                                  rot ? steps[1] : 0.0f);
             }
         } else {
             if (m < NR) row.x = (float)rv[m];
@@ -671,6 +692,124 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         ROW[m] = row;
         COL[m] = col;
     }
+}
+
+// Loads for rank_one_sums from host or device tables.
+SAS_HD float4 ldv(const float4 *p) { return *p; }
+__device__ __forceinline__ float4 ldv(const SAS_CONST float4 *p) { return ld4(p); }
+
+constexpr int RESTART = 8;   // rotation mode: direct sin/cos every RESTART points
+
+// Rotation mode (every mesh point accepted, both grids equally spaced, in
+// grid order, grid step in .w): F = (sum_i u1_i J_i)(sum_j v1_j S_j) and
+// F^2 = (sum_i u2_i J_i^2)(sum_j v2_j S_j^2), with sin and cos along each
+// grid advanced by rotating through the step angle (the exact product of
+// q and the step, reduced like any argument) and recomputed directly every
+// RESTART points; S = sin(y)/y by its Taylor series below |y| = 2.
+// Wave-uniform vote (every lane of the wave) on the device, the value itself on the host.
+SAS_HD bool wave_all(bool p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __all(p);
+#else
+    return p;
+#endif
+}
+
+// Length factors T1 = sum v1 S, T2 = sum v2 S^2 (grid order, step COL[0].w).
+template <class P4>
+SAS_HD void length_sums(P4 COL, int NL, float qc, float &T1, float &T2)
+{
+    const float ya = qc * ldv(COL).x, yb = qc * ldv(COL + NL - 1).x;   // grid ends (either order)
+    const float ymin = fminf(ya, yb), ymax = fmaxf(ya, yb);
+    T1 = 0.0f;
+    T2 = 0.0f;
+    if (wave_all(ymax < 2.0f)) {
+        for (int m = 0; m < NL; m++) {
+            const float4 e = ldv(COL + m);
+            const float S = sinc_taylor(qc * e.x);
+            T1 = fmaf(e.y, S, T1);
+            T2 = fmaf(e.z, S * S, T2);
+        }
+        return;
+    }
+    const bool mixed = !wave_all(ymin >= 2.0f);
+    float sd, cd;
+    sincos_prod(qc, ldv(COL).w, sd, cd);
+    for (int m0 = 0; m0 < NL; m0 += RESTART) {
+        float s, c;
+        sincos_prod(qc, ldv(COL + m0).x, s, c);
+        const int m1 = m0 + RESTART < NL ? m0 + RESTART : NL;
+        for (int m = m0; m < m1; m++) {
+            const float4 e = ldv(COL + m);
+            const float y = qc * e.x;
+            float S = s * rcp_f(y);
+            if (mixed && y < 2.0f) S = sinc_taylor(y);
+            T1 = fmaf(e.y, S, T1);
+            T2 = fmaf(e.z, S * S, T2);
+            const float s2 = fmaf(s, cd, c * sd);
+            c = fmaf(c, cd, -s * sd);
+            s = s2;
+        }
+    }
+}
+
+// Radius factors U1 = sum u1 J, U2 = sum u2 J^2 (grid order, step ROW[0].w).
+template <class P4>
+SAS_HD void radius_sums(P4 ROW, int NR, float qs, float &U1, float &U2)
+{
+    const float xa = qs * ldv(ROW).x, xb = qs * ldv(ROW + NR - 1).x;   // grid ends (either order)
+    const float xmin = fminf(xa, xb), xmax = fmaxf(xa, xb);
+    U1 = 0.0f;
+    U2 = 0.0f;
+    if (wave_all(xmax <= 2.0f)) {
+        for (int i = 0; i < NR; i++) {
+            const float4 e = ldv(ROW + i);
+            const float J = j1c_small(qs * e.x);
+            U1 = fmaf(e.y, J, U1);
+            U2 = fmaf(e.z, J * J, U2);
+        }
+        return;
+    }
+    const bool mixed = !wave_all(xmin > 2.0f);
+    float sd, cd;
+    sincos_prod(qs, ldv(ROW).w, sd, cd);
+    for (int i0 = 0; i0 < NR; i0 += RESTART) {
+        float s, c;
+        sincos_prod(qs, ldv(ROW + i0).x, s, c);
+        const int i1 = i0 + RESTART < NR ? i0 + RESTART : NR;
+        for (int i = i0; i < i1; i++) {
+            const float4 e = ldv(ROW + i);
+            const float x = qs * e.x;
+            float J;
+            if (mixed && x <= 2.0f) J = j1c_small(x);
+            else J = j1c_big(x, s, c);
+            U1 = fmaf(e.y, J, U1);
+            U2 = fmaf(e.z, J * J, U2);
+            const float s2 = fmaf(s, cd, c * sd);
+            c = fmaf(c, cd, -s * sd);
+            s = s2;
+        }
+    }
+}
+
+// Rotation mode (every mesh point accepted, both grids equally spaced, in
+// grid order, grid step in .w): F = (sum_i u1_i J_i)(sum_j v1_j S_j) and
+// F^2 = (sum_i u2_i J_i^2)(sum_j v2_j S_j^2), with sin and cos along each
+// grid advanced by rotating through the step angle (the exact product of
+// q and the step, reduced like any argument) and recomputed directly every
+// RESTART points; S = sin(y)/y by its Taylor series below |y| = 2, J by its
+// polynomial below 2.  Loops are specialised on wave-uniform votes: all
+// arguments of the wave small (series only), all large (rotation only),
+// or mixed.
+template <class P4>
+SAS_HD void rank_one_sums(P4 ROW, P4 COL, int NR, int NL, float q, float sn, float cs, float &f1, float &f2)
+{
+    float T1, T2, U1, U2;
+    length_sums(COL, NL, fabsf(q * cs), T1, T2);
+    radius_sums(ROW, NR, fabsf(q * sn), U1, U2);
+    f1 = U1 * T1;
+    f2 = U2 * T2;
 }
 
 // F and F^2 sums of one set at (q, Gauss node) -- sn, cs: sine and cosine of
@@ -684,6 +823,10 @@ __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, flo
     const int NR = dims[0], NL = dims[1];
     const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
     const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
+    if (dims[2] == 2) {
+        rank_one_sums(ROW, COL, NR, NL, q, sn, cs, f1, f2);
+        return;
+    }
     const float qc = q * cs, qs = q * sn;
     float F1 = 0.0f, F2 = 0.0f;
     if (dims[2]) {

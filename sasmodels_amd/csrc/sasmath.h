@@ -21,7 +21,10 @@
 //   j1c_f         2 J1(x)/x from the Cephes single-precision j1f
 //                 approximations (S. L. Moshier, Cephes Math Library 2.8):
 //                 a polynomial on |x| <= 2 and the modulus/phase asymptotic
-//                 form beyond, as sas_J1.c:190-245 uses for float kernels.
+//                 form beyond, as sas_J1.c:190-245 uses for float kernels;
+//                 j1c_small / j1c_big are the two branches (j1c_big takes
+//                 sin and cos of x from the caller).
+//   sinc_taylor   sin(x)/x on |x| <= 2 by its Taylor series.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -154,18 +157,35 @@ SAS_HD float sph_j1c_f(float x)
     return 3.0f * (s * inv - c) * inv * inv;
 }
 
-// 2 J1(x)/x
-SAS_HD float j1c_f(float x)
+// sin(x)/x for |x| <= 2 by its Taylor series to x^14 (truncation below
+// 2e-10 at |x| = 2)
+SAS_HD float sinc_taylor(float x)
 {
-    const float ax = fabsf(x);
-    if (ax <= 2.0f) {
-        const float z = x * x;
-        float p = fmaf(z, -4.878788132172128e-9f, 6.009061827883699e-7f);
-        p = fmaf(z, p, -4.541343896997497e-5f);
-        p = fmaf(z, p, 1.937383947804541e-3f);
-        p = fmaf(z, p, -3.405537384615824e-2f);
-        return 2.0f * (z - 14.681970596f) * p;
-    }
+    const float z = x * x;
+    float p = fmaf(z, -7.64716373e-13f, 1.60590438e-10f);   // -1/15!, 1/13!
+    p = fmaf(z, p, -2.50521084e-8f);                          // -1/11!
+    p = fmaf(z, p, 2.75573192e-6f);                           // 1/9!
+    p = fmaf(z, p, -1.98412698e-4f);                          // -1/7!
+    p = fmaf(z, p, 8.33333333e-3f);                           // 1/5!
+    p = fmaf(z, p, -1.66666667e-1f);                          // -1/3!
+    return fmaf(z, p, 1.0f);
+}
+
+// 2 J1(x)/x on |x| <= 2 (Cephes j1f polynomial)
+SAS_HD float j1c_small(float x)
+{
+    const float z = x * x;
+    float p = fmaf(z, -4.878788132172128e-9f, 6.009061827883699e-7f);
+    p = fmaf(z, p, -4.541343896997497e-5f);
+    p = fmaf(z, p, 1.937383947804541e-3f);
+    p = fmaf(z, p, -3.405537384615824e-2f);
+    return 2.0f * (z - 14.681970596f) * p;
+}
+
+// 2 J1(x)/x for x = ax > 2 given sx = sin(ax), cx = cos(ax) (Cephes j1f
+// modulus/phase form)
+SAS_HD float j1c_big(float ax, float sx, float cx)
+{
     const float q = rcp_f(ax);
     const float w = sqrt_f(q);
     // modulus sqrt(1/x) * M(1/x)
@@ -191,11 +211,19 @@ SAS_HD float j1c_f(float x)
     const float xn2 = xn * xn;
     const float sxn = xn * fmaf(xn2, fmaf(xn2, 8.33333333e-3f, -1.66666667e-1f), 1.0f);
     const float cxn = fmaf(xn2, fmaf(xn2, fmaf(xn2, -1.38888889e-3f, 4.16666667e-2f), -0.5f), 1.0f);
-    float sx, cx;
-    sincos_f(ax, sx, cx);
     // cos(x + xn - 3pi/4) = [sin xn (sin x + cos x) + cos xn (sin x - cos x)] / sqrt(2)
     const float j1 = amp * 0.70710678118654752f * (sxn * (sx + cx) + cxn * (sx - cx));
     return 2.0f * j1 * q;   // even function: sign of x cancels
+}
+
+// 2 J1(x)/x
+SAS_HD float j1c_f(float x)
+{
+    const float ax = fabsf(x);
+    if (ax <= 2.0f) return j1c_small(x);
+    float sx, cx;
+    sincos_f(ax, sx, cx);
+    return j1c_big(ax, sx, cx);
 }
 
 // ---- arguments known in FP64 (the 2D kernels): FP64 reduction by pi/2,

@@ -158,3 +158,22 @@ def test_batch_rejects_bad_inputs():
         batch.raw([(det, val), (det, val)])
     k.release()
     batch.release()
+
+
+def test_replay_resident():
+    """saship_batch_replay needs a staged call, reruns its device work with
+    inputs in HBM, and leaves later calls' results unchanged."""
+    from sasmodels_amd._lib import HipError
+    model = hip_model("core_shell_cylinder", 76)
+    sets = _cs_cyl_sets(12, 77)
+    kl = type("K", (), {"info": model.info, "dtype": np.dtype("d")})()
+    calls = [_pack(model, kl, pars, "1d")[:2] for _, pars in sets]
+    batch = HipBatch(model, [qv for qv, _ in sets])
+    with pytest.raises(HipError):
+        batch.replay(1)
+    first, _ = batch.raw(calls)
+    ms = batch.replay(3)
+    assert ms > 0.0
+    again, _ = batch.raw(calls)
+    assert np.array_equal(first, again)
+    batch.release()
