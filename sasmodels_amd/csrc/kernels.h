@@ -480,7 +480,7 @@ struct Tables {
     int2 *GRP;
     float *AT;       // bilinear mode [nlp][NRMAX]: w * 1e-2 * drho * V * pi/4 at (row, column)
     float *BT;       // bilinear mode [nlp][NRMAX]: w * 1e-4 * (drho * V)^2 * pi/4
-    int *dims;       // NR, NL, mode (2 rotation, 1 prefix, 0 bilinear), number of row groups
+    int *dims;       // NR, NL, mode (3 interval, 2 rotation, 1 prefix, 0 bilinear), number of row groups
     int nlp;         // columns per set in AT/BT (>= every set's NL)
 };
 
@@ -632,18 +632,66 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         }
         nzero = z;
         ngroups = g;
-        // every point accepted and both grids equally spaced: rotation mode
+        // equally spaced grids (every distribution of the reference is a linspace)
         int md = prefix ? 1 : 0;
-        if (prefix && z == 0 && g == 1 && rowk[roworder[0]] == NL) {
-            const double dr = NR > 1 ? (rv[NR - 1] - rv[0]) / (NR - 1) : 0.0;
-            const double dl = NL > 1 ? (lv[NL - 1] - lv[0]) / (NL - 1) : 0.0;
-            bool ap = true;
-            for (int i = 0; i < NR; i++) ap = ap && fabs(rv[i] - (rv[0] + i * dr)) <= 1e-9 * fabs(rv[NR - 1]);
-            for (int j = 0; j < NL; j++) ap = ap && fabs(lv[j] - (lv[0] + j * dl)) <= 1e-9 * fabs(lv[NL - 1]);
-            if (ap) {
-                md = 2;
-                steps[0] = (float)dr;
-                steps[1] = (float)(0.5 * dl);
+        bool ap = prefix;
+        const double dr = NR > 1 ? (rv[NR - 1] - rv[0]) / (NR - 1) : 0.0;
+        const double dl = NL > 1 ? (lv[NL - 1] - lv[0]) / (NL - 1) : 0.0;
+        for (int i = 0; i < NR && ap; i++) ap = fabs(rv[i] - (rv[0] + i * dr)) <= 1e-9 * fabs(rv[NR - 1]);
+        for (int j = 0; j < NL && ap; j++) ap = fabs(lv[j] - (lv[0] + j * dl)) <= 1e-9 * fabs(lv[NL - 1]);
+        if (ap) {
+            steps[0] = (float)dr;
+            steps[1] = (float)(0.5 * dl);
+        }
+        if (ap && z == 0 && g == 1 && rowk[roworder[0]] == NL) {
+            md = 2;     // every point accepted: rotation mode
+        } else if (ap && g >= 1 && g < NRMAX) {
+            // interval mode: the first k columns of the weight order are a
+            // grid interval growing outward from colorder[0], and the rows of
+            // each group (and the rows accepting nothing, first) are taken
+            // inward from the two ends of the row grid
+            const int kmax = GRP[g - 1].x, c0 = colorder[0];
+            bool iv = true;
+            int lo = c0, hi = c0;
+            for (int m = 1; m < kmax && iv; m++) {
+                const int j = colorder[m];
+                if (j == lo - 1) lo = j;
+                else if (j == hi + 1) hi = j;
+                else iv = false;
+            }
+            int rl = 0, rr = NR - 1, rl0 = 0, rr0 = NR - 1, kprev = 0, t = 0;
+            for (int gi = -1; gi < g && iv; gi++) {
+                const int end = gi < 0 ? z : z + GRP[gi].y;      // rows of this group: roworder[t..end)
+                int nrl = 0;
+                while (t < end && roworder[t] == rl) {
+                    rl++;
+                    nrl++;
+                    t++;
+                }
+                const int nrr = end - t;
+                for (int u = 0; u < nrr && iv; u++) iv = roworder[t + u] == rr - nrr + 1 + u;
+                rr -= nrr;
+                t = end;
+                if (gi < 0) {
+                    rl0 = rl;
+                    rr0 = rr;
+                    continue;
+                }
+                const int k = GRP[gi].x;
+                int ncl = 0, ncr = 0;
+                for (int m = kprev; m < k; m++) (colorder[m] < c0 ? ncl : ncr)++;
+                kprev = k;
+                GRP[gi] = make_int2(ncl | (ncr << 16), nrl | (nrr << 16));
+            }
+            if (iv) {
+                GRP[g] = make_int2(c0, rl0 | (rr0 << 16));
+                md = 3;
+            } else {
+                // restore the prefix-mode groups overwritten above
+                for (int m = z, gi = 0; m < NR; m++) {
+                    const int k = rowk[roworder[m]];
+                    if (m + 1 == NR || rowk[roworder[m + 1]] != k) GRP[gi++] = make_int2(k, m + 1 - z);
+                }
             }
         }
         mode = md;
@@ -671,9 +719,9 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
             const double drho = base[0] - base[1];
             const double c1 = wo * 1e-2 * drho * PI_D * PI_4_D;
             const double c2 = wo * 1e-4 * drho * drho * PI_D * PI_D * PI_4_D;
-            // rotation mode: grid order, grid steps in .w
-            const bool rot = mode == 2;
-            if (m < NR - nzero) {
+            // rotation and interval modes: grid order, grid steps in .w
+            const bool rot = mode >= 2;
+            if (m < (rot ? NR : NR - nzero)) {
                 const int i = rot ? m : roworder[m + nzero];
                 const double x = rv[i];
                 row = make_float4((float)x, (float)(rw[i] * x * x), (float)(rw[i] * x * x * x * x),
@@ -812,6 +860,120 @@ SAS_HD void rank_one_sums(P4 ROW, P4 COL, int NR, int NL, float q, float sn, flo
     f2 = U2 * T2;
 }
 
+SAS_HD int2 ldv2(const int2 *p) { return *p; }
+__device__ __forceinline__ int2 ldv2(const SAS_CONST int2 *p) { return ld2(p); }
+
+// One element of a rotation stream along a grid: sin/cos at the element
+// (recomputed directly every RESTART elements of the stream, else carried
+// by rotation), then advanced by the step in direction dir (+1 or -1).
+struct Stream {
+    float s = 0.0f, c = 1.0f;
+    int n = 0;
+    SAS_HD void at(float qa, float x, float sd, float cd, int dir, float &so, float &co)
+    {
+        if ((n & (RESTART - 1)) == 0) sincos_prod(qa, x, s, c);
+        so = s;
+        co = c;
+        const float sdd = dir > 0 ? sd : -sd;
+        const float s2 = fmaf(s, cd, c * sdd);
+        c = fmaf(c, cd, -s * sdd);
+        s = s2;
+        n++;
+    }
+};
+
+// Interval mode (pruned mesh, equally spaced grids, prefixes of the weight
+// order are grid intervals): groups in ascending accepted count; group g
+// extends the column interval by ncl columns to the left and ncr to the
+// right of the current interval (prefix sums P1, P2 grow), then adds its
+// rows, nrl taken inward from the left end and nrr from the right end,
+// with J_i P(k_g).  GRP[g] = (ncl | ncr << 16, nrl | nrr << 16), GRP[G] =
+// (first column, first left row | first right row << 16).  Four rotation
+// streams; series below argument 2 as in rotation mode.
+// Argument regimes of a grid at one (q, node), voted over the wave:
+// every argument small (series only), every argument large (rotation
+// only), or mixed (both, per lane).
+enum { ARG_SMALL = 0, ARG_LARGE = 1, ARG_MIXED = 2 };
+
+SAS_HD int regime(float a, float b, float bound)
+{
+    if (wave_all(fmaxf(a, b) <= bound)) return ARG_SMALL;
+    if (wave_all(fminf(a, b) > bound)) return ARG_LARGE;
+    return ARG_MIXED;
+}
+
+template <int CM, int RM, class P4, class PG>
+SAS_HD void interval_sums_r(P4 ROW, P4 COL, PG GRP, int NR, int NL, int G, float qc, float qs, float &f1,
+                            float &f2)
+{
+    float sdc = 0.0f, cdc = 1.0f, sdr = 0.0f, cdr = 1.0f;
+    if (CM != ARG_SMALL) sincos_prod(qc, ldv(COL).w, sdc, cdc);
+    if (RM != ARG_SMALL) sincos_prod(qs, ldv(ROW).w, sdr, cdr);
+    const int2 st = ldv2(GRP + G);
+    int cl = st.x - 1, cr = st.x, rl = st.y & 0xffff, rr = st.y >> 16;
+    Stream CL, CR, RL, RR;
+    float P1 = 0.0f, P2 = 0.0f, F1 = 0.0f, F2 = 0.0f;
+    auto column = [&](Stream &str, int m, int dir) {
+        const float4 e = ldv(COL + m);
+        const float y = qc * e.x;
+        float S;
+        if (CM == ARG_SMALL) {
+            S = sinc_taylor(y);
+        } else {
+            float sy, cy;
+            str.at(qc, e.x, sdc, cdc, dir, sy, cy);
+            S = sy * rcp_f(y);
+            if (CM == ARG_MIXED && y <= 2.0f) S = sinc_taylor(y);
+        }
+        P1 = fmaf(e.y, S, P1);
+        P2 = fmaf(e.z, S * S, P2);
+    };
+    auto row = [&](Stream &str, int i, int dir) {
+        const float4 e = ldv(ROW + i);
+        const float x = qs * e.x;
+        float J;
+        if (RM == ARG_SMALL) {
+            J = j1c_small(x);
+        } else {
+            float sx, cx;
+            str.at(qs, e.x, sdr, cdr, dir, sx, cx);
+            if (RM == ARG_MIXED && x <= 2.0f) J = j1c_small(x);
+            else J = j1c_big(x, sx, cx);
+        }
+        F1 = fmaf(e.y * J, P1, F1);
+        F2 = fmaf(e.z * (J * J), P2, F2);
+    };
+    for (int g = 0; g < G; g++) {
+        const int2 e = ldv2(GRP + g);
+        const int ncl = min(e.x & 0xffff, cl + 1), ncr = min(e.x >> 16, NL - cr);
+        const int nrl = min(e.y & 0xffff, NR - rl), nrr = min(e.y >> 16, rr + 1);
+        for (int t = 0; t < ncl; t++) column(CL, cl - t, -1);
+        for (int t = 0; t < ncr; t++) column(CR, cr + t, +1);
+        cl -= ncl;
+        cr += ncr;
+        for (int t = 0; t < nrl; t++) row(RL, rl + t, +1);
+        for (int t = 0; t < nrr; t++) row(RR, rr - t, -1);
+        rl += nrl;
+        rr -= nrr;
+    }
+    f1 = F1;
+    f2 = F2;
+}
+
+template <class P4, class PG>
+SAS_HD void interval_sums(P4 ROW, P4 COL, PG GRP, int NR, int NL, int G, float q, float sn, float cs, float &f1,
+                          float &f2)
+{
+    const float qc = fabsf(q * cs), qs = fabsf(q * sn);
+    const int cm = regime(qc * ldv(COL).x, qc * ldv(COL + NL - 1).x, 2.0f);
+    const int rm = regime(qs * ldv(ROW).x, qs * ldv(ROW + NR - 1).x, 2.0f);
+#define SAS_IV(C, R) \
+    if (cm == C && rm == R) return interval_sums_r<C, R>(ROW, COL, GRP, NR, NL, G, qc, qs, f1, f2);
+    SAS_IV(0, 0) SAS_IV(0, 1) SAS_IV(0, 2) SAS_IV(1, 0) SAS_IV(1, 1) SAS_IV(1, 2) SAS_IV(2, 0) SAS_IV(2, 1)
+#undef SAS_IV
+    interval_sums_r<2, 2>(ROW, COL, GRP, NR, NL, G, qc, qs, f1, f2);
+}
+
 // F and F^2 sums of one set at (q, Gauss node) -- sn, cs: sine and cosine of
 // the node; the node weight is applied by the caller.  Every lane of the
 // wave reads the same table entries (scalar loads: the tables are written
@@ -825,6 +987,10 @@ __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, flo
     const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
     if (dims[2] == 2) {
         rank_one_sums(ROW, COL, NR, NL, q, sn, cs, f1, f2);
+        return;
+    }
+    if (dims[2] == 3) {
+        interval_sums(ROW, COL, as_const(t.GRP) + (size_t)set * NRMAX, NR, NL, dims[3], q, sn, cs, f1, f2);
         return;
     }
     const float qc = q * cs, qs = q * sn;
