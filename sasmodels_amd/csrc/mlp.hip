@@ -41,10 +41,12 @@ constexpr int NPACK = 3 * PACK_PER_LAYER;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid through the hardware reciprocal (1 ulp; exp(-z) = inf gives 0)
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float dsilu(float z)
 {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -362,10 +364,25 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
 {
     __shared__ __align__(16) __bf16 act[BMI][H + 8];
     __shared__ float partial[4][BMI];
+    __shared__ float feat[BMI][D + 1];            // standardised features, then the baseline
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t rows = (int64_t)a.ncurves * a.nq;
     const int wn = wave * 64;                     // this wave's 64 output columns
     for (int64_t r0 = (int64_t)blockIdx.x * BMI; r0 < rows; r0 += (int64_t)gridDim.x * BMI) {
+        // ---- features once per row (first wave)
+        if (tid < BMI) {
+            const int64_t row = r0 + tid;
+            float x[D] = {0, 0, 0, 0, 0, 0};
+            float base = 0.0f;
+            if (row < rows) {
+                const int c = (int)(row / a.nq), iq = (int)(row % a.nq);
+                base = features(a.curves + (size_t)c * 5, a.logq[iq], x);
+            }
+#pragma unroll
+            for (int k = 0; k < D; k++) feat[tid][k] = (x[k] - a.norm[k]) / a.norm[6 + k];
+            feat[tid][D] = base;
+        }
+        __syncthreads();
         // ---- layer 1: thread handles column n = tid for all BMI rows
         {
             const int n = tid;
@@ -374,15 +391,9 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
             for (int k = 0; k < D; k++) w1[k] = a.theta[OFF_W1 + k * H + n];
             const float b1 = a.theta[OFF_B1 + n];
             for (int m = 0; m < BMI; m++) {
-                const int64_t row = r0 + m;
-                float x[D] = {0, 0, 0, 0, 0, 0};
-                if (row < rows) {
-                    const int c = (int)(row / a.nq), iq = (int)(row % a.nq);
-                    (void)features(a.curves + (size_t)c * 5, a.logq[iq], x);
-                }
                 float z = b1;
 #pragma unroll
-                for (int k = 0; k < D; k++) z = fmaf((x[k] - a.norm[k]) / a.norm[6 + k], w1[k], z);
+                for (int k = 0; k < D; k++) z = fmaf(feat[m][k], w1[k], z);
                 act[m][n] = (__bf16)silu(z);
             }
         }
@@ -454,10 +465,8 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
                 const float yv = partial[0][tid] + partial[1][tid] + partial[2][tid] + partial[3][tid] + a.theta[OFF_B5];
                 if (a.out_iq) {
                     const float *cp = a.curves + (size_t)(row / a.nq) * 5;
-                    float x[D];
-                    const float base = features(cp, a.logq[row % a.nq], x);
                     const float v0 = 3.14159265f * cp[0] * cp[0] * cp[1];
-                    a.y[row] = 1e-4f * cp[4] * cp[4] * v0 * expf(yv + base);
+                    a.y[row] = 1e-4f * cp[4] * cp[4] * v0 * expf(yv + feat[tid][D]);
                 } else {
                     a.y[row] = yv;
                 }
