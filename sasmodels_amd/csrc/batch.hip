@@ -81,7 +81,15 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
         const int c1 = min(npts, c0 + a.pts_per_item);
         const float4 *rec = a.recs + a.roff[s] * RV;
         KSum a1, a2;
-        for (int j = c0; j < c1; j++) {
+        bool shared = false;
+        if constexpr (FIXED && M::SHARED_1D) {
+            // every point of the chunk has the same lengths (a radius-only
+            // mesh): length factors once per node for the whole chunk
+            const float *rf = (const float *)rec;
+            shared = c1 - c0 > 1 && M::shares_lengths(rf, 4 * RV, c0, c1);
+            if (shared) M::eval1d_shared(q, rf, 4 * RV, c0, c1, a.quad, a1, a2);
+        }
+        for (int j = c0; j < c1 && !shared; j++) {
             float r[4 * RV];
             load_rec(rec + (size_t)j * RV, r, RV);
             float f1, f2;

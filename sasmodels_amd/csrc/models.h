@@ -67,6 +67,7 @@ __device__ __forceinline__ void quad_range(const Quad &t, float qr, int &off, in
 // ---------------------------------------------------------------- sphere
 struct Sphere {
     static constexpr int ID = 0, NPARS = 3, NMAG = 2, MAX_PD = 1, NCON = 1;
+    static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = false;
     static constexpr int REC1D = 4, REC2D = 4;
     static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
@@ -106,6 +107,7 @@ struct Sphere {
 // ---------------------------------------------------------------- cylinder
 struct Cylinder {
     static constexpr int ID = 1, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
+    static constexpr bool SHARED_1D = true;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
@@ -151,6 +153,35 @@ struct Cylinder {
         f1 = r[2] * a1;
         f2 = r[3] * a2;
     }
+    // Fixed rule, records r0..r1 of a chunk with the same half length r[1]:
+    // nodes outer, the length factor once per node.  Adds sum_pts r[2] w f
+    // and sum_pts r[3] w f^2 of every node to a1, a2.
+    template <class ACC>
+    __device__ static void eval1d_shared(float q, const float *rec, int rv, int n0, int n1, const Quad &t, ACC &a1,
+                                         ACC &a2)
+    {
+        const float qh = q * rec[n0 * rv + 1];
+        for (int i = 0; i < t.n; i++) {
+            const float s = t.a[i], w = t.w[i];
+            const float S = sinc_prod(qh, t.b[i]);
+            float t1 = 0.0f, t2 = 0.0f;
+            for (int j = n0; j < n1; j++) {
+                const float *r = rec + j * rv;
+                const float f = j1c_f((q * r[0]) * s) * S;
+                const float wf = w * f;
+                t1 = fmaf(r[2], wf, t1);
+                t2 = fmaf(r[3], wf * f, t2);
+            }
+            a1.add(t1);
+            a2.add(t2);
+        }
+    }
+    __device__ static bool shares_lengths(const float *rec, int rv, int n0, int n1)
+    {
+        bool same = true;
+        for (int j = n0 + 1; j < n1; j++) same = same && rec[j * rv + 1] == rec[n0 * rv + 1];
+        return same;
+    }
     __device__ static void rec2d(const double *p, double w, float *r)
     {
         put_d(r, 0, p[2]);
@@ -169,6 +200,7 @@ struct Cylinder {
 // ---------------------------------------------------------------- ellipsoid
 struct Ellipsoid {
     static constexpr int ID = 2, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
+    static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
@@ -244,6 +276,7 @@ struct Ellipsoid {
 // ---------------------------------------------------------------- core_shell_cylinder
 struct CoreShellCylinder {
     static constexpr int ID = 3, NPARS = 8, NMAG = 3, MAX_PD = 5, NCON = 2;
+    static constexpr bool SHARED_1D = true;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
     static constexpr int REC1D = 8, REC2D = 12;
     static constexpr int A2D = 10;  // slot of w*1e-4 in the 2D record
@@ -302,6 +335,36 @@ struct CoreShellCylinder {
         f1 = r[6] * a1;
         f2 = r[7] * a2;
     }
+    // Fixed rule, records n0..n1 of a chunk with the same half lengths r[1]
+    // and r[3] (only the radius varies): nodes outer, the two length factors
+    // once per node.  Adds sum_pts r[6] w f and sum_pts r[7] w f^2 per node.
+    template <class ACC>
+    __device__ static void eval1d_shared(float q, const float *rec, int rv, int n0, int n1, const Quad &t, ACC &a1,
+                                         ACC &a2)
+    {
+        const float qhc = q * rec[n0 * rv + 1], qhs = q * rec[n0 * rv + 3];
+        for (int i = 0; i < t.n; i++) {
+            const float s = t.a[i], c = t.b[i], w = t.w[i];
+            const float S1 = sinc_prod(qhc, c), S2 = sinc_prod(qhs, c);
+            float t1 = 0.0f, t2 = 0.0f;
+            for (int j = n0; j < n1; j++) {
+                const float *r = rec + j * rv;
+                const float f = r[4] * S1 * j1c_f((q * r[0]) * s) + r[5] * S2 * j1c_f((q * r[2]) * s);
+                const float wf = w * f;
+                t1 = fmaf(r[6], wf, t1);
+                t2 = fmaf(r[7], wf * f, t2);
+            }
+            a1.add(t1);
+            a2.add(t2);
+        }
+    }
+    __device__ static bool shares_lengths(const float *rec, int rv, int n0, int n1)
+    {
+        bool same = true;
+        for (int j = n0 + 1; j < n1; j++)
+            same = same && rec[j * rv + 1] == rec[n0 * rv + 1] && rec[j * rv + 3] == rec[n0 * rv + 3];
+        return same;
+    }
     __device__ static void rec2d(const double *p, double w, float *r)
     {
         const double rr = p[3], t = p[4], l = p[5];
@@ -330,6 +393,7 @@ struct CoreShellCylinder {
 // ---------------------------------------------------------------- core_shell_sphere
 struct CoreShellSphere {
     static constexpr int ID = 4, NPARS = 5, NMAG = 3, MAX_PD = 2, NCON = 2;
+    static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = false;
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 6;   // slot of w*1e-4 in the 2D record
