@@ -236,6 +236,24 @@ __global__ __launch_bounds__(256) void k_batch_final(
         result[resoff[s] + (size_t)nout * nq + threadIdx.x] = sm[threadIdx.x];
 }
 
+// Resolution smearing of the batch's I(q): one thread per smeared point,
+// out[o] = sum_k w[off[o] + k] * iq[from[o] + k] for k < len[o] (FP64; the
+// banded weight matrix of each set, rows indexed in the batch's q order).
+__global__ __launch_bounds__(256) void k_batch_smear(int64_t n_out, const int64_t *__restrict__ from,
+                                                     const int32_t *__restrict__ len,
+                                                     const int64_t *__restrict__ off,
+                                                     const double *__restrict__ w,
+                                                     const double *__restrict__ iq, double *__restrict__ out)
+{
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= n_out) return;
+    const double *src = iq + from[o];
+    const double *wt = w + off[o];
+    double acc = 0.0;
+    for (int k = 0; k < len[o]; k++) acc = fma(wt[k], src[k], acc);
+    out[o] = acc;
+}
+
 }  // namespace sas
 
 namespace batch_host {
@@ -279,6 +297,11 @@ struct saship_batch {
     size_t out_cap = 0;
     char *d_sep = nullptr;            // separable cylinder tables
     size_t sep_cap = 0;
+    // resolution smearing of I(q) (saship_batch_set_resolution)
+    int64_t n_smear = 0;              // smeared points, all sets (0: no smearing)
+    int64_t *d_sfrom = nullptr, *d_soff = nullptr;
+    int32_t *d_slen = nullptr;
+    double *d_sw = nullptr;
     bool separable = true;            // use the separable path when every set allows it
     char *h_out = nullptr;
     size_t hout_cap = 0;
@@ -512,12 +535,20 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
                                                 d_nchunk, b->d_sums, b->d_partial,
                                                 want_result ? d_result : nullptr, want_iq ? d_iq : nullptr);
     HIP_TRY(hipGetLastError());
+    double *d_siq = d_iq + p.nq_total;
+    if (want_iq && b->n_smear > 0) {
+        const int64_t blocks = (b->n_smear + 255) / 256;
+        k_batch_smear<<<(unsigned)blocks, 256, 0, b->st>>>(b->n_smear, b->d_sfrom, b->d_slen, b->d_soff, b->d_sw, d_iq,
+                                                          d_siq);
+        HIP_TRY(hipGetLastError());
+    }
     if (b->timing) HIP_TRY(hipEventRecord(b->ev3, b->st));
     if (!copies) return 0;
     if (want_result)
         HIP_TRY(hipMemcpyAsync(b->h_out, d_result, sizeof(double) * p.result_total, hipMemcpyDeviceToHost, b->st));
     if (want_iq)
-        HIP_TRY(hipMemcpyAsync(b->h_out + sizeof(double) * p.result_total, d_iq, sizeof(double) * p.nq_total,
+        HIP_TRY(hipMemcpyAsync(b->h_out + sizeof(double) * p.result_total, b->n_smear > 0 ? d_siq : d_iq,
+                               sizeof(double) * (b->n_smear > 0 ? b->n_smear : p.nq_total),
                                hipMemcpyDeviceToHost, b->st));
     return 0;
 }
@@ -546,10 +577,10 @@ int size_buffers(saship_batch *b, const Plan &p)
     if (int e = grow(b->d_args, b->args_cap, p.bytes)) return e;
     if (int e = grow(b->d_recs, b->recs_cap, sizeof(float4) * (size_t)std::max<int64_t>(1, p.rec_total) * p.rv)) return e;
     if (int e = grow(b->d_partial, b->partial_cap, sizeof(double) * (size_t)std::max<int64_t>(1, p.partial_total))) return e;
-    if (int e = grow(b->d_out, b->out_cap, sizeof(double) * (size_t)(p.result_total + p.nq_total))) return e;
+    if (int e = grow(b->d_out, b->out_cap, sizeof(double) * (size_t)(p.result_total + p.nq_total + b->n_smear))) return e;
     if (p.sep_nlp)
         if (int e = grow(b->d_sep, b->sep_cap, sas::sep::tables_bytes(b->nsets, p.sep_nlp))) return e;
-    const size_t hb = sizeof(double) * (size_t)(p.result_total + p.nq_total);
+    const size_t hb = sizeof(double) * (size_t)(p.result_total + std::max<int64_t>(p.nq_total, b->n_smear));
     if (hb > b->hout_cap || !b->h_out) {
         grew = true;
         if (int e = ensure_host(b->h_out, b->hout_cap, hb)) return e;
@@ -569,7 +600,7 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
-                                    p.sep_nlp};
+                                    p.sep_nlp, b->n_smear};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
@@ -600,7 +631,9 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     b->last_iq = want_iq;
     b->have_last = true;
     if (want_result) memcpy(result, b->h_out, sizeof(double) * p.result_total);
-    if (want_iq) memcpy(iq, b->h_out + sizeof(double) * p.result_total, sizeof(double) * p.nq_total);
+    if (want_iq)
+        memcpy(iq, b->h_out + sizeof(double) * p.result_total,
+               sizeof(double) * (b->n_smear > 0 ? b->n_smear : p.nq_total));
     if (b->timing) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -685,6 +718,55 @@ int saship_batch_set_path(saship_batch *b, int32_t separable)
     std::lock_guard<std::mutex> lock(b->mu);
     b->separable = separable != 0;
     drop_graph(b);
+    return 0;
+}
+
+int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int32_t *start, const int32_t *length,
+                                const int64_t *offset, const double *weights, int64_t n_weights)
+{
+    if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
+    std::lock_guard<std::mutex> lock(b->mu);
+    HIP_TRY(hipSetDevice(b->dev));
+    drop_graph(b);
+    void *old[] = {b->d_sfrom, b->d_soff, b->d_slen, b->d_sw};
+    for (void *ptr : old)
+        if (ptr) (void)hipFree(ptr);
+    b->d_sfrom = b->d_soff = nullptr;
+    b->d_slen = nullptr;
+    b->d_sw = nullptr;
+    b->n_smear = 0;
+    if (!n_out) return 0;                          // smearing off
+    if (!start || !length || !offset || (n_weights > 0 && !weights) || n_weights < 0)
+        return fail(SASHIP_EINVAL, "start, length, offset and weights must not be NULL");
+    int64_t total = 0;
+    for (int s = 0; s < b->nsets; s++) {
+        if (n_out[s] < 0) return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": n_out < 0");
+        total += n_out[s];
+    }
+    std::vector<int64_t> from(std::max<int64_t>(total, 1));
+    for (int s = 0, o = 0; s < b->nsets; s++) {
+        const int64_t nq = b->qoff[s + 1] - b->qoff[s];
+        for (int k = 0; k < n_out[s]; k++, o++) {
+            if (start[o] < 0 || length[o] < 0 || start[o] + (int64_t)length[o] > nq)
+                return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": band of point " + std::to_string(k) +
+                                               " outside the set's q points");
+            if (offset[o] < 0 || offset[o] + length[o] > n_weights)
+                return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": weights of point " + std::to_string(k) +
+                                               " outside the weight array");
+            from[o] = b->qoff[s] + start[o];
+        }
+    }
+    if (total == 0) return 0;
+    hipError_t e = hipMalloc((void **)&b->d_sfrom, sizeof(int64_t) * total);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_soff, sizeof(int64_t) * total);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_slen, sizeof(int32_t) * total);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_sw, sizeof(double) * std::max<int64_t>(n_weights, 1));
+    if (e == hipSuccess) e = hipMemcpy(b->d_sfrom, from.data(), sizeof(int64_t) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_soff, offset, sizeof(int64_t) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_slen, length, sizeof(int32_t) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_weights > 0) e = hipMemcpy(b->d_sw, weights, sizeof(double) * n_weights, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(SASHIP_EHIP, std::string("set resolution: ") + hipGetErrorString(e));
+    b->n_smear = total;
     return 0;
 }
 
@@ -822,7 +904,7 @@ int saship_batch_release(saship_batch *b)
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
     void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums,
-                        b->d_queue, b->d_partial, b->d_out, b->d_sep};
+                        b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_slen, b->d_sw};
     for (void *ptr : dev_ptrs)
         if (ptr) (void)hipFree(ptr);
     if (b->h_stage) (void)hipHostFree(b->h_stage);

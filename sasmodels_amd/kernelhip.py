@@ -199,6 +199,7 @@ class HipBatch:
             mh, len(q_list), self.nq.ctypes.data, self._q.ctypes.data,
             1 if self.dim == "2d" else 0, ct.byref(h)))
         self._handle = h
+        self._n_out = None          # smeared points per set (set_resolution)
         if use_graph:
             self.set_options(use_graph=True)
 
@@ -219,6 +220,47 @@ class HipBatch:
         """Allow (default) or forbid the separable cylinder evaluation."""
         _lib.check(_lib.lib().saship_batch_set_path(self._handle, int(bool(enable))))
 
+    def set_resolution(self, resolutions):
+        """
+        Smear every set's I(q) on the device: *resolutions* holds one
+        :class:`~sasmodels_amd.resolution.Resolution` per set whose q_calc is
+        that set's q (None or Perfect1D: no smearing for that set), or is
+        None to switch smearing off.  Calls then return I at each
+        resolution's measured q.
+        """
+        if resolutions is None:
+            _lib.check(_lib.lib().saship_batch_set_resolution(self._handle, None, None, None, None, None, 0))
+            self._n_out = None
+            return
+        from .resolution import band
+        if len(resolutions) != self.nsets:
+            raise ValueError("expected %d resolutions, got %d" % (self.nsets, len(resolutions)))
+        n_out, starts, lengths, offsets, weights = [], [], [], [], []
+        total = 0
+        for s, res in enumerate(resolutions):
+            W = getattr(res, "weight_matrix", None)
+            if W is None:                       # identity on the set's own q
+                W = np.eye(int(self.nq[s]))
+            if W.shape[0] != self.nq[s]:
+                raise ValueError("set %d: weight matrix has %d rows for %d q points" % (s, W.shape[0], self.nq[s]))
+            st, ln, off, w = band(W)
+            n_out.append(W.shape[1])
+            starts.append(st)
+            lengths.append(ln)
+            offsets.append(off + total)
+            weights.append(w)
+            total += w.size
+        self._res_keep = [np.ascontiguousarray(np.asarray(n_out, np.int32)),
+                          np.ascontiguousarray(np.concatenate(starts).astype(np.int32)),
+                          np.ascontiguousarray(np.concatenate(lengths).astype(np.int32)),
+                          np.ascontiguousarray(np.concatenate(offsets).astype(np.int64)),
+                          np.ascontiguousarray(np.concatenate(weights) if total else np.zeros(1))]
+        k = self._res_keep
+        _lib.check(_lib.lib().saship_batch_set_resolution(
+            self._handle, k[0].ctypes.data, k[1].ctypes.data, k[2].ctypes.data, k[3].ctypes.data,
+            k[4].ctypes.data, int(total)))
+        self._n_out = k[0].astype(np.int64)
+
     def _pack(self, calls):
         if len(calls) != self.nsets:
             raise ValueError("expected %d (call_details, values) pairs, got %d" % (self.nsets, len(calls)))
@@ -237,7 +279,8 @@ class HipBatch:
         det, vals, voff, ndet = self._pack(calls)
         nout = 2 if self.dim == "1d" else 1
         result = np.empty(int(np.sum(nout*self.nq + 4)), "d") if want_result else None
-        iq = np.empty(int(self.qoff[-1]), "d") if want_iq else None
+        n_iq = int(self.qoff[-1]) if self._n_out is None else int(self._n_out.sum())
+        iq = np.empty(n_iq, "d") if want_iq else None
         _lib.check(_lib.lib().saship_batch_call(
             self._handle, self._kind(magnetic), det.ctypes.data, ndet, vals.ctypes.data, voff.ctypes.data,
             float(cutoff), int(radius_effective_mode),
@@ -247,6 +290,8 @@ class HipBatch:
     def __call__(self, calls, cutoff=0.0, magnetic=False):
         """List of I(q), one per set."""
         _, iq = self.raw(calls, cutoff, magnetic, 0, want_result=False, want_iq=True)
+        if self._n_out is not None:
+            return np.split(iq, np.cumsum(self._n_out)[:-1])
         return np.split(iq, self.qoff[1:-1])
 
     def Fq(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0):
