@@ -30,9 +30,10 @@ non-packed instructions); traffic and VALU instruction counts of the same
 launch from the counter summary in profiles/ when present.  cpu_baseline: the FP64 oracle on the
 host cores, on a bounded sample of the workload.
 
-other_configs: the sphere 1D, ellipsoid 2D, batched core-shell cylinder and
-batched magnetic core-shell sphere configurations, each with device
-throughput, FP64 accuracy on a subset and the oracle's rate on the host.
+other_configs: the sphere 1D, pinhole-smeared cylinder, ellipsoid 2D,
+batched core-shell cylinder and batched magnetic core-shell sphere
+configurations, each with device throughput, FP64 accuracy on a subset and
+the oracle's rate on the host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
                     [--no-surrogate] [--no-configs]
@@ -332,6 +333,39 @@ def bench_configs(threads, cpu=True):
         ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), r).max() for a, r in zip(iq, refs)))
         ent["cpu_curves_per_s"] = 64/t
     out["sphere_1d_batched"] = ent
+    model.release()
+
+    # cylinder headline curves measured through a pinhole (dq/q = 5 %): the
+    # batch evaluates q_calc and smears on the device
+    from sasmodels_amd import resolution
+    info = core.load_model_info("cylinder")
+    generate.set_integration_size(info, QUAD)
+    model = core.build_model(info, platform="hip")
+    res = resolution.Pinhole1D(C2_Q, 0.05*C2_Q)
+    calls = c2_calls(info, 64, offset=9000)
+    b = HipBatch(model, [[res.q_calc]]*len(calls), use_graph=True)
+    b.autotune(calls)
+    b.set_resolution([res]*len(calls))
+    dt, iq = _timed(lambda: b(calls), 10)
+    b.replay(2)
+    with_smear = b.replay(20)*1e-3/20
+    b.set_resolution(None)
+    b(calls)
+    b.replay(2)
+    without = b.replay(20)*1e-3/20
+    b.release()
+    _, length, _, _ = resolution.band(res.weight_matrix)
+    smear_bytes = 64*(16*int(length.sum()) + 8*C2_Q.size)
+    ent = {"workload": "cylinder headline curves (64 per launch) with pinhole resolution dq/q = 0.05: "
+                       "%d q_calc points smeared to 1000 q on the device" % res.q_calc.size,
+           "curves_per_s": 64/with_smear, "curves_per_s_host_inclusive": 64/dt,
+           "ms_per_launch": with_smear*1e3, "smear_ms": (with_smear - without)*1e3,
+           "smear_gbytes_per_s": smear_bytes/max(with_smear - without, 1e-9)/1e9}
+    if cpu:
+        t, refs = _oracle_leg(info, [([res.q_calc], d, v, False) for d, v in calls[:4]], threads)
+        ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), res.apply(r)).max()
+                                                for a, r in zip(iq, refs)))
+    out["cylinder_batched_pinhole"] = ent
     model.release()
 
     # C3: ellipsoid 2D on 256 x 256, theta/phi/radius_equatorial dispersion 20 x 20 x 20
