@@ -147,16 +147,18 @@ int saship_batch_call(saship_batch *batch, int32_t kind,
                       double *result, double *iq);
 
 /* Resolution smearing of the batch's I(q) (Pinhole1D / Slit1D weight
- * matrices, sasmodels/resolution.py:172-221): per set s, n_out[s] smeared
- * points; for the o-th smeared point of all sets (set order), the weights
+ * matrices, sasmodels/resolution.py:172-221; Pinhole2D bin averages,
+ * resolution2d.py:91-175): per set s, n_out[s] smeared points; for the
+ * o-th smeared point of all sets (set order), the weights
  * weights[offset[o] .. offset[o] + length[o]) apply to the set's q points
- * start[o] .. start[o] + length[o] (the set's q are its q_calc).  Later
- * saship_batch_call results return sum(n_out) smeared I values in `iq`
- * (the raw result stays unsmeared).  n_out == NULL switches smearing off.
- * Bands are validated against the sets' q counts and n_weights. */
+ * start[o] + k*stride[o], k < length[o] (the set's q are its q_calc;
+ * stride == NULL means 1).  Later saship_batch_call results return
+ * sum(n_out) smeared I values in `iq` (the raw result stays unsmeared).
+ * n_out == NULL switches smearing off.  Every band is validated against
+ * the set's q count and n_weights. */
 int saship_batch_set_resolution(saship_batch *batch, const int32_t *n_out, const int32_t *start,
-                                const int32_t *length, const int64_t *offset, const double *weights,
-                                int64_t n_weights);
+                                const int32_t *length, const int64_t *stride, const int64_t *offset,
+                                const double *weights, int64_t n_weights);
 
 /* Runs the device work of the last successful saship_batch_call again
  * nsteps times back to back with its inputs already in device memory and

@@ -237,10 +237,12 @@ __global__ __launch_bounds__(256) void k_batch_final(
 }
 
 // Resolution smearing of the batch's I(q): one thread per smeared point,
-// out[o] = sum_k w[off[o] + k] * iq[from[o] + k] for k < len[o] (FP64; the
-// banded weight matrix of each set, rows indexed in the batch's q order).
+// out[o] = sum_k w[off[o] + k] * iq[from[o] + k*step[o]] for k < len[o]
+// (FP64): contiguous bands of a 1D weight matrix (step 1) or the strided
+// polar bins of a 2D pinhole (step = pixels of the set).
 __global__ __launch_bounds__(256) void k_batch_smear(int64_t n_out, const int64_t *__restrict__ from,
                                                      const int32_t *__restrict__ len,
+                                                     const int64_t *__restrict__ step,
                                                      const int64_t *__restrict__ off,
                                                      const double *__restrict__ w,
                                                      const double *__restrict__ iq, double *__restrict__ out)
@@ -249,8 +251,9 @@ __global__ __launch_bounds__(256) void k_batch_smear(int64_t n_out, const int64_
     if (o >= n_out) return;
     const double *src = iq + from[o];
     const double *wt = w + off[o];
+    const int64_t st = step[o];
     double acc = 0.0;
-    for (int k = 0; k < len[o]; k++) acc = fma(wt[k], src[k], acc);
+    for (int k = 0; k < len[o]; k++) acc = fma(wt[k], src[k * st], acc);
     out[o] = acc;
 }
 
@@ -299,7 +302,7 @@ struct saship_batch {
     size_t sep_cap = 0;
     // resolution smearing of I(q) (saship_batch_set_resolution)
     int64_t n_smear = 0;              // smeared points, all sets (0: no smearing)
-    int64_t *d_sfrom = nullptr, *d_soff = nullptr;
+    int64_t *d_sfrom = nullptr, *d_soff = nullptr, *d_sstep = nullptr;
     int32_t *d_slen = nullptr;
     double *d_sw = nullptr;
     bool separable = true;            // use the separable path when every set allows it
@@ -538,8 +541,8 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     double *d_siq = d_iq + p.nq_total;
     if (want_iq && b->n_smear > 0) {
         const int64_t blocks = (b->n_smear + 255) / 256;
-        k_batch_smear<<<(unsigned)blocks, 256, 0, b->st>>>(b->n_smear, b->d_sfrom, b->d_slen, b->d_soff, b->d_sw, d_iq,
-                                                          d_siq);
+        k_batch_smear<<<(unsigned)blocks, 256, 0, b->st>>>(b->n_smear, b->d_sfrom, b->d_slen, b->d_sstep, b->d_soff,
+                                                          b->d_sw, d_iq, d_siq);
         HIP_TRY(hipGetLastError());
     }
     if (b->timing) HIP_TRY(hipEventRecord(b->ev3, b->st));
@@ -722,16 +725,16 @@ int saship_batch_set_path(saship_batch *b, int32_t separable)
 }
 
 int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int32_t *start, const int32_t *length,
-                                const int64_t *offset, const double *weights, int64_t n_weights)
+                                const int64_t *stride, const int64_t *offset, const double *weights, int64_t n_weights)
 {
     if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
     std::lock_guard<std::mutex> lock(b->mu);
     HIP_TRY(hipSetDevice(b->dev));
     drop_graph(b);
-    void *old[] = {b->d_sfrom, b->d_soff, b->d_slen, b->d_sw};
+    void *old[] = {b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw};
     for (void *ptr : old)
         if (ptr) (void)hipFree(ptr);
-    b->d_sfrom = b->d_soff = nullptr;
+    b->d_sfrom = b->d_soff = b->d_sstep = nullptr;
     b->d_slen = nullptr;
     b->d_sw = nullptr;
     b->n_smear = 0;
@@ -743,11 +746,14 @@ int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int
         if (n_out[s] < 0) return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": n_out < 0");
         total += n_out[s];
     }
-    std::vector<int64_t> from(std::max<int64_t>(total, 1));
+    std::vector<int64_t> from(std::max<int64_t>(total, 1)), steps(std::max<int64_t>(total, 1), 1);
     for (int s = 0, o = 0; s < b->nsets; s++) {
         const int64_t nq = b->qoff[s + 1] - b->qoff[s];
         for (int k = 0; k < n_out[s]; k++, o++) {
-            if (start[o] < 0 || length[o] < 0 || start[o] + (int64_t)length[o] > nq)
+            const int64_t st = stride ? stride[o] : 1;
+            steps[o] = st;
+            if (start[o] < 0 || length[o] < 0 || st < 1 ||
+                (length[o] > 0 && start[o] + (int64_t)(length[o] - 1) * st >= nq))
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": band of point " + std::to_string(k) +
                                                " outside the set's q points");
             if (offset[o] < 0 || offset[o] + length[o] > n_weights)
@@ -759,6 +765,8 @@ int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int
     if (total == 0) return 0;
     hipError_t e = hipMalloc((void **)&b->d_sfrom, sizeof(int64_t) * total);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_soff, sizeof(int64_t) * total);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_sstep, sizeof(int64_t) * total);
+    if (e == hipSuccess) e = hipMemcpy(b->d_sstep, steps.data(), sizeof(int64_t) * total, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_slen, sizeof(int32_t) * total);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_sw, sizeof(double) * std::max<int64_t>(n_weights, 1));
     if (e == hipSuccess) e = hipMemcpy(b->d_sfrom, from.data(), sizeof(int64_t) * total, hipMemcpyHostToDevice);
@@ -904,7 +912,7 @@ int saship_batch_release(saship_batch *b)
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
     void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums,
-                        b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_slen, b->d_sw};
+                        b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw};
     for (void *ptr : dev_ptrs)
         if (ptr) (void)hipFree(ptr);
     if (b->h_stage) (void)hipHostFree(b->h_stage);

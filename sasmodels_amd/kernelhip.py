@@ -223,43 +223,33 @@ class HipBatch:
     def set_resolution(self, resolutions):
         """
         Smear every set's I(q) on the device: *resolutions* holds one
-        :class:`~sasmodels_amd.resolution.Resolution` per set whose q_calc is
-        that set's q (None or Perfect1D: no smearing for that set), or is
-        None to switch smearing off.  Calls then return I at each
-        resolution's measured q.
+        resolution object per set whose q_calc is that set's q (Pinhole1D,
+        Slit1D, Pinhole2D; None or Perfect1D: no smearing for that set), or
+        is None to switch smearing off.  Calls then return I at each
+        resolution's measured q (pixels for Pinhole2D).
         """
         if resolutions is None:
-            _lib.check(_lib.lib().saship_batch_set_resolution(self._handle, None, None, None, None, None, 0))
+            _lib.check(_lib.lib().saship_batch_set_resolution(self._handle, None, None, None, None, None, None, 0))
             self._n_out = None
             return
-        from .resolution import band
+        from .resolution import smear_bands
         if len(resolutions) != self.nsets:
             raise ValueError("expected %d resolutions, got %d" % (self.nsets, len(resolutions)))
-        n_out, starts, lengths, offsets, weights = [], [], [], [], []
-        total = 0
-        for s, res in enumerate(resolutions):
-            W = getattr(res, "weight_matrix", None)
-            if W is None:                       # identity on the set's own q
-                W = np.eye(int(self.nq[s]))
-            if W.shape[0] != self.nq[s]:
-                raise ValueError("set %d: weight matrix has %d rows for %d q points" % (s, W.shape[0], self.nq[s]))
-            st, ln, off, w = band(W)
-            n_out.append(W.shape[1])
-            starts.append(st)
-            lengths.append(ln)
-            offsets.append(off + total)
-            weights.append(w)
-            total += w.size
-        self._res_keep = [np.ascontiguousarray(np.asarray(n_out, np.int32)),
-                          np.ascontiguousarray(np.concatenate(starts).astype(np.int32)),
-                          np.ascontiguousarray(np.concatenate(lengths).astype(np.int32)),
-                          np.ascontiguousarray(np.concatenate(offsets).astype(np.int64)),
-                          np.ascontiguousarray(np.concatenate(weights) if total else np.zeros(1))]
-        k = self._res_keep
+        parts = [smear_bands(res, int(self.nq[s])) for s, res in enumerate(resolutions)]
+        offsets, total = [], 0
+        for p in parts:
+            offsets.append(p[4] + total)
+            total += p[5].size
+        keep = [np.ascontiguousarray(np.array([p[0] for p in parts], np.int32)),
+                np.ascontiguousarray(np.concatenate([p[1] for p in parts]).astype(np.int32)),
+                np.ascontiguousarray(np.concatenate([p[2] for p in parts]).astype(np.int32)),
+                np.ascontiguousarray(np.concatenate([p[3] for p in parts]).astype(np.int64)),
+                np.ascontiguousarray(np.concatenate(offsets).astype(np.int64)),
+                np.ascontiguousarray(np.concatenate([p[5] for p in parts]) if total else np.zeros(1))]
         _lib.check(_lib.lib().saship_batch_set_resolution(
-            self._handle, k[0].ctypes.data, k[1].ctypes.data, k[2].ctypes.data, k[3].ctypes.data,
-            k[4].ctypes.data, int(total)))
-        self._n_out = k[0].astype(np.int64)
+            self._handle, *[k.ctypes.data for k in keep], int(total)))
+        self._res_keep = keep
+        self._n_out = keep[0].astype(np.int64)
 
     def _pack(self, calls):
         if len(calls) != self.nsets:

@@ -20,15 +20,20 @@ Construction follows the reference's definitions:
   (resolution.py:223-396);
 * q_calc extension: linear steps below/above the data for pinhole,
   geometric steps for slit; q_calc below 0.02 min(q) is dropped
-  (resolution.py:80-107, 131-166, 398-544).
+  (resolution.py:80-107, 131-166, 398-544);
+* 2D pinhole: every pixel sampled at nr x nphi polar offsets (radii at bin
+  centres of [0, nsigma] in units of (dq_parallel, dq_perp), angles
+  2 pi k/nphi, rotated to the pixel's direction), averaged with the
+  Gaussian mass of each radial bin (resolution2d.py:29-175).
 """
 
 import numpy as np
 from scipy.special import erf
 
-__all__ = ["Resolution", "Perfect1D", "Pinhole1D", "Slit1D", "apply_resolution_matrix",
+__all__ = ["Resolution", "Perfect1D", "Pinhole1D", "Slit1D", "Pinhole2D", "apply_resolution_matrix",
            "pinhole_resolution", "slit_resolution", "pinhole_extend_q", "slit_extend_q",
-           "bin_edges", "interpolate", "linear_extrapolation", "geometric_extrapolation", "band"]
+           "bin_edges", "interpolate", "linear_extrapolation", "geometric_extrapolation", "band",
+           "smear_bands"]
 
 MINIMUM_RESOLUTION = 1e-8
 MINIMUM_ABSOLUTE_Q = 0.02          # relative to the smallest measured q
@@ -85,6 +90,71 @@ class Slit1D(Resolution):
 
     def apply(self, theory):
         return apply_resolution_matrix(self.weight_matrix, theory)
+
+
+SIGMA_ZERO = 1.0e-10
+NR_2D = {"xhigh": 10, "high": 5, "med": 5, "low": 3}
+NPHI_2D = {"xhigh": 20, "high": 12, "med": 6, "low": 4}
+
+
+class Pinhole2D(Resolution):
+    """
+    Gaussian smearing of 2D data: *data* holds qx_data, qy_data, q_data and
+    optionally dqx_data (radial) and dqy_data (azimuthal); *index* selects
+    the pixels.  q_calc = [qx, qy] of nbins x npix points, bin-major.
+    """
+
+    def __init__(self, data=None, index=None, nsigma=3.0, accuracy="Low", coords="polar"):
+        self.nr = NR_2D[accuracy.lower()]
+        self.nphi = NPHI_2D[accuracy.lower()]
+        self.nsigma = nsigma
+        self.coords = coords
+        self.data = data
+        self.index = index if index is not None else slice(None)
+        self.qx_data = data.qx_data[self.index]
+        self.qy_data = data.qy_data[self.index]
+        self.q_data = data.q_data[self.index]
+        dqx, dqy = getattr(data, "dqx_data", None), getattr(data, "dqy_data", None)
+        if dqx is not None and dqy is not None:
+            self.dqx_data = np.maximum(dqx[self.index], SIGMA_ZERO)
+            self.dqy_data = np.maximum(dqy[self.index], SIGMA_ZERO)
+            qx, qy, w = self._sample()
+            self.q_calc = [qx, qy]
+            self.q_calc_weights = w
+        else:
+            self.dqx_data = self.dqy_data = None
+            self.q_calc = [self.qx_data, self.qy_data]
+            self.q_calc_weights = None
+
+    def _sample(self):
+        nr, nphi, npix = self.nr, self.nphi, len(self.qx_data)
+        step = self.nsigma/nr
+        r = step*(np.arange(nr) + 0.5)                       # radial bin centres
+        ang = 2.0*np.pi*np.arange(nphi)/nphi
+        # bins ordered angle-major, radius-minor; each bin holds all pixels
+        rb = np.tile(r, nphi)
+        ab = np.repeat(ang, nr)
+        wb = np.tile(np.exp(-0.5*(r - 0.5*step)**2) - np.exp(-0.5*(r + 0.5*step)**2), nphi)
+        dx = rb[:, None]*self.dqx_data[None, :]
+        dy = rb[:, None]*self.dqy_data[None, :]
+        qx, qy = self.qx_data[None, :], self.qy_data[None, :]
+        if self.coords == "polar":
+            phi = np.arctan(qy/qx)
+            qr = np.sqrt(qx*qx + qy*qy)
+            rad = dx*np.cos(ab)[:, None] + qr
+            tan = dy*np.sin(ab)[:, None]
+            qx_s = rad*np.cos(phi) - tan*np.sin(phi)
+            qy_s = rad*np.sin(phi) + tan*np.cos(phi)
+        else:
+            qx_s = qx + dx*np.cos(ab)[:, None]
+            qy_s = qy + dy*np.sin(ab)[:, None]
+        return qx_s.ravel(), qy_s.ravel(), wb
+
+    def apply(self, theory):
+        if self.q_calc_weights is None:
+            return theory
+        t = np.reshape(theory, (len(self.q_calc_weights), len(self.qx_data)))
+        return np.average(t, axis=0, weights=self.q_calc_weights)
 
 
 def apply_resolution_matrix(weight_matrix, theory):
@@ -235,3 +305,28 @@ def band(weight_matrix):
     np.cumsum(length, out=offset[1:])
     weights = np.concatenate(parts) if parts else np.zeros(0)
     return start, length, offset[:-1], weights
+
+
+def smear_bands(res, n_calc):
+    """
+    Device form of a resolution for a set with *n_calc* q_calc points:
+    (n_out, start, length, stride, offset, weights).  1D weight matrices
+    give contiguous bands; Pinhole2D gives one strided band per pixel over
+    its polar bins, all pixels sharing one normalised weight vector; None
+    or a resolution without weights is the identity on the set's points.
+    """
+    W = getattr(res, "weight_matrix", None)
+    wb = getattr(res, "q_calc_weights", None)
+    if isinstance(res, Pinhole2D) and wb is not None:
+        npix = len(res.qx_data)
+        if len(wb)*npix != n_calc:
+            raise ValueError("Pinhole2D has %d q_calc points, the set %d" % (len(wb)*npix, n_calc))
+        return (npix, np.arange(npix, dtype=np.int32), np.full(npix, len(wb), np.int32),
+                np.full(npix, npix, np.int64), np.zeros(npix, np.int64), np.asarray(wb, float)/np.sum(wb))
+    if W is None:
+        return (n_calc, np.arange(n_calc, dtype=np.int32), np.ones(n_calc, np.int32),
+                np.ones(n_calc, np.int64), np.arange(n_calc, dtype=np.int64), np.ones(n_calc))
+    if W.shape[0] != n_calc:
+        raise ValueError("weight matrix has %d rows for %d q_calc points" % (W.shape[0], n_calc))
+    start, length, offset, weights = band(W)
+    return W.shape[1], start, length, np.ones(W.shape[1], np.int64), offset, weights

@@ -76,7 +76,14 @@ def test_smearing_rejects_bad_bands():
     offset, w = np.array([0], np.int64), np.ones(6)
     with pytest.raises(HipError):
         _lib.check(_lib.lib().saship_batch_set_resolution(batch._handle, n_out.ctypes.data, start.ctypes.data,
-                                                          length.ctypes.data, offset.ctypes.data, w.ctypes.data, 6))
+                                                          length.ctypes.data, None, offset.ctypes.data,
+                                                          w.ctypes.data, 6))
+    stride = np.array([8], np.int64)                                    # rows 0, 8, .., 40 of 40
+    start, length = np.array([0], np.int32), np.array([6], np.int32)
+    with pytest.raises(HipError):
+        _lib.check(_lib.lib().saship_batch_set_resolution(batch._handle, n_out.ctypes.data, start.ctypes.data,
+                                                          length.ctypes.data, stride.ctypes.data,
+                                                          offset.ctypes.data, w.ctypes.data, 6))
     W = np.eye(40)
     W[39, 0] = 0.5                        # band of point 0 spans all rows: still inside
     ok = type("Res", (), {"weight_matrix": W})()
@@ -91,3 +98,35 @@ def test_smearing_rejects_bad_bands():
 def make_calls(hm, q):
     kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
     return [make_kernel_args(kl, direct_model.get_mesh(hm.info, dict(radius=30), dim="1d"))[:2]]
+
+
+class _Det:
+    def __init__(self, n, span, shift):
+        x = np.linspace(-span, span, n)
+        qx, qy = np.meshgrid(x + shift, x - 0.5*shift)
+        self.qx_data, self.qy_data = qx.ravel(), qy.ravel()
+        self.q_data = np.hypot(self.qx_data, self.qy_data)
+        self.dqx_data = 0.06*self.q_data + 0.002
+        self.dqy_data = 0.03*self.q_data + 0.001
+
+
+def test_batch_pinhole2d():
+    hm = hip_model("cylinder")
+    rng = np.random.default_rng(5)
+    kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
+    res, calls = [], []
+    for s, acc in enumerate(("low", "med", "low")):
+        res.append(R.Pinhole2D(_Det(12 + 2*s, 0.2, 0.003*(s + 1)), accuracy=acc))
+        pars = dict(radius=rng.uniform(20, 60), length=rng.uniform(100, 400), theta=rng.uniform(0, 90),
+                    phi=rng.uniform(0, 90), theta_pd=5, theta_pd_n=3)
+        calls.append(make_kernel_args(kl, direct_model.get_mesh(hm.info, pars, dim="2d"))[:2])
+    batch = HipBatch(hm, [list(r.q_calc) for r in res])
+    plain = batch(calls)
+    batch.set_resolution(res)
+    smeared = batch(calls)
+    for r, p, sm, (det, val) in zip(res, plain, smeared, calls):
+        print("pinhole2d", len(r.qx_data), flush=True)
+        np.testing.assert_allclose(sm, r.apply(p), rtol=1e-12, atol=0)
+        ref = r.apply(OracleKernel(hm.info, list(r.q_calc))(det, val, 0.0, False))
+        assert masked_relerr(sm, ref) <= RTOL
+    batch.release()

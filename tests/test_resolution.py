@@ -162,3 +162,65 @@ def test_igor_sphere_pinhole():
     out = res.apply(_sphere_oracle(res.q_calc, pars))
     err = np.abs(out - answer)/answer
     assert err.max() < 2e-3 and np.median(err) < 5e-4
+
+
+class _Data2D:
+    def __init__(self, n=12, span=0.08, dq=True):
+        x = np.linspace(-span, span, n)
+        qx, qy = np.meshgrid(x + 0.0031, x + 0.0017)
+        self.qx_data, self.qy_data = qx.ravel(), qy.ravel()
+        self.q_data = np.hypot(self.qx_data, self.qy_data)
+        if dq:
+            self.dqx_data = 0.05*self.q_data + 0.002      # radial
+            self.dqy_data = 0.02*self.q_data + 0.001      # azimuthal
+
+
+def test_pinhole2d_identity_and_constants():
+    res = R.Pinhole2D(_Data2D(dq=False))
+    np.testing.assert_array_equal(res.apply(res.qx_data), res.qx_data)
+    res = R.Pinhole2D(_Data2D(), accuracy="med")
+    assert len(res.q_calc[0]) == 30*len(res.qx_data)
+    np.testing.assert_allclose(res.apply(np.full(len(res.q_calc[0]), 2.5)), 2.5, rtol=1e-14)
+    sub = R.Pinhole2D(_Data2D(), index=np.arange(144) % 3 == 0)
+    assert len(sub.qx_data) == 48
+
+
+def test_pinhole2d_matches_gaussian_average():
+    """xhigh sampling (10 x 20 polar bins) against a fine quadrature of the
+    2D Gaussian (radial dq_parallel, azimuthal dq_perp) truncated at 3 sigma,
+    for a smooth centrosymmetric function."""
+    data = _Data2D(n=5)
+    res = R.Pinhole2D(data, accuracy="xhigh")
+    f = lambda qx, qy: np.exp(-(qx*qx + 2*qy*qy)/0.003) + 0.5*np.cos(25*qx)*np.cos(15*qy)
+    out = res.apply(f(*res.q_calc))
+    # reference average by a fine polar grid
+    nr, nt = 400, 720
+    r = (np.arange(nr) + 0.5)*3.0/nr
+    t = 2*np.pi*np.arange(nt)/nt
+    wr = r*np.exp(-0.5*r*r)
+    want = []
+    for qx, qy, sr, st in zip(data.qx_data, data.qy_data, data.dqx_data, data.dqy_data):
+        phi = np.arctan2(qy, qx)
+        er, et = np.array([np.cos(phi), np.sin(phi)]), np.array([-np.sin(phi), np.cos(phi)])
+        dx = (r[:, None]*np.cos(t)[None, :])*sr
+        dy = (r[:, None]*np.sin(t)[None, :])*st
+        px = qx + dx*er[0] + dy*et[0]
+        py = qy + dx*er[1] + dy*et[1]
+        want.append(np.sum(wr[:, None]*f(px, py))/np.sum(wr[:, None]*np.ones_like(px)))
+    np.testing.assert_allclose(out, want, rtol=5e-3)
+
+
+def test_smear_bands_forms():
+    q = np.logspace(-3, -1, 30)
+    res = R.Pinhole1D(q, 0.1*q)
+    n_out, start, length, stride, offset, w = R.smear_bands(res, len(res.q_calc))
+    assert n_out == 30 and (stride == 1).all()
+    p2 = R.Pinhole2D(_Data2D())
+    n_out, start, length, stride, offset, w = R.smear_bands(p2, len(p2.q_calc[0]))
+    assert n_out == 144 and (stride == 144).all() and (length == 12).all() and (offset == 0).all()
+    np.testing.assert_allclose(w.sum(), 1.0, rtol=1e-15)
+    theory = np.random.default_rng(0).random(len(p2.q_calc[0]))
+    dev = np.array([np.dot(w, theory[s::144][:12]) for s in start])
+    np.testing.assert_allclose(dev, p2.apply(theory), rtol=1e-13)
+    with pytest.raises(ValueError):
+        R.smear_bands(p2, 100)
