@@ -43,6 +43,11 @@ extern "C" {
 #define SASHIP_ELLIPSOID 2
 #define SASHIP_CORE_SHELL_CYLINDER 3
 #define SASHIP_CORE_SHELL_SPHERE 4
+/* structure factor (sasmodels/models/hardsphere.py): parameters
+ * radius_effective, volfraction; nothing polydisperse, so details has the
+ * four trailing entries only (max_pd 0); Iq and Iqxy (q = |(qx, qy)|) give
+ * result[i] = S(q_i) and sums (1, 1, 1, 0); evaluated in FP64 */
+#define SASHIP_HARDSPHERE 5
 
 /* kernel kinds: the three kernels generated per model (generate.py:1144-1180) */
 #define SASHIP_KIND_IQ 0        /* <model>_Iq: 1D, <F> and <F^2> */
@@ -159,6 +164,23 @@ int saship_batch_call(saship_batch *batch, int32_t kind,
 int saship_batch_set_resolution(saship_batch *batch, const int32_t *n_out, const int32_t *start,
                                 const int32_t *length, const int64_t *stride, const int64_t *offset,
                                 const double *weights, int64_t n_weights);
+
+/* P@S products on the device: the batch's form factor times structure
+ * factor s_model (SASHIP_HARDSPHERE; 0 switches it off), combined in the
+ * final kernel as ProductKernel.Iq does (sasmodels/product.py:425-528).
+ * pars holds per set (radius_effective, volfraction,
+ * structure_factor_mode, radius_effective_mode).  Later calls return in
+ * `iq`
+ *     scale*volfraction/V_shell * (<F^2> S)                 mode 0
+ *     scale*volfraction/V_shell * (<F^2> + <F>^2 (S - 1))   mode > 0 (beta, 1D only)
+ *   + background,
+ * with S at the form's mean effective radius (radius_effective_mode > 0;
+ * it must equal the call's radius_effective_mode) or at radius_effective
+ * (mode 0), and at volume fraction volfraction * V_form/V_shell.  scale
+ * and background are the set's values[0..1]; the raw result stays P's.
+ * The five form factors have no volfraction parameter of their own, so
+ * volfraction always scales the number density. */
+int saship_batch_set_structure(saship_batch *batch, int32_t s_model, const double *pars, int32_t nsets);
 
 /* Runs the device work of the last successful saship_batch_call again
  * nsteps times back to back with its inputs already in device memory and

@@ -58,6 +58,8 @@ def lib():
         L.sasoracle_iqac.argtypes = [ct.c_int, _D, _D, ct.c_void_p]
         L.sasoracle_form_volume.restype = _D
         L.sasoracle_form_volume.argtypes = [ct.c_int, ct.c_void_p]
+        L.sasoracle_hardsphere.restype = _D
+        L.sasoracle_hardsphere.argtypes = [_D, _D, _D]
         L.sasoracle_radius_effective.restype = _D
         L.sasoracle_radius_effective.argtypes = [ct.c_int, ct.c_int, ct.c_void_p]
         L.sasoracle_view_jitter_row.argtypes = [_D, _D, _D, _D, _P, _P]
@@ -153,12 +155,40 @@ class OracleKernel:
     __call__ = Iq
 
 
+def hardsphere(q, radius_effective, volfraction):
+    """Hard-sphere S(q) (sasmodels/models/hardsphere.c), q scalar or array."""
+    f = lib().sasoracle_hardsphere
+    q = np.asarray(q, "d")
+    return np.array([f(float(v), float(radius_effective), float(volfraction)) for v in q.ravel()]
+                    ).reshape(q.shape)
+
+
+def product(q, fq, scale, background, radius_effective, volfraction, beta_mode, er_mode):
+    """
+    P@S intensity from P's normalised (F, F^2, R_eff, V_shell, V_form/V_shell)
+    (sasmodels/product.py:425-528, P without its own volfraction): S at
+    P's R_eff (er_mode > 0) or *radius_effective*, volume fraction
+    volfraction * V_form/V_shell; I = scale*volfraction/V_shell * PS +
+    background with PS = F^2 S, or F^2 + F1^2 (S - 1) with beta.  *q* is the
+    1D q or the |q| of 2D points.  Returns (I, S).
+    """
+    F1, F2, reff, shell, ratio = fq
+    radius = reff if er_mode > 0 else radius_effective
+    S = hardsphere(q, radius, volfraction*ratio)
+    PS = F2 + F1**2*(S - 1.0) if beta_mode else F2*S
+    return scale*volfraction/shell*PS + background, S
+
+
 def ref_lib(name):
     """ctypes handle of oracle/_ref/<name>.so (reference sources), or None."""
     path = os.path.join(REF_DIR, name + ".so")
     if not os.path.exists(path):
         return None
     L = ct.CDLL(path)
+    if hasattr(L, "ref_sq"):
+        L.ref_sq.restype = _D
+        L.ref_sq.argtypes = [_D, _D, _D]
+        return L
     L.ref_fq.argtypes = [_D, ct.c_void_p, _P, _P]
     L.ref_form_volume.restype = _D
     L.ref_form_volume.argtypes = [ct.c_void_p]

@@ -11,8 +11,8 @@
  * vector to the reference's static functions so that ctypes can call them.
  *
  * One shared object is built per (model, quadrature) pair:
- *   -DREF_MODEL=<1..5>  sphere, cylinder, ellipsoid, core_shell_cylinder,
- *                       core_shell_sphere
+ *   -DREF_MODEL=<1..6>  sphere, cylinder, ellipsoid, core_shell_cylinder,
+ *                       core_shell_sphere, hardsphere (structure factor)
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -68,13 +68,18 @@
 #  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4])
 #  define NPAR_VOLUME(p) form_volume(p[0], p[1])
 #  define NPAR_REFF(m, p) radius_effective(m, p[0], p[1])
+#elif REF_MODEL == 6
+#  include "hardsphere.c"
+double ref_sq(double q, double radius_effective, double volfraction) { return Iq(q, radius_effective, volfraction); }
 #else
-#  error "REF_MODEL must be 1..5"
+#  error "REF_MODEL must be 1..6"
 #endif
 
+#ifdef NPAR_CALL_FQ
 void ref_fq(double q, const double *p, double *f1, double *f2) { NPAR_CALL_FQ(q, f1, f2, p); }
 double ref_form_volume(const double *p) { return NPAR_VOLUME(p); }
 double ref_reff(int mode, const double *p) { return NPAR_REFF(mode, p); }
+#endif
 #ifdef NPAR_IQAC
 double ref_iqac(double qab, double qc, const double *p) { return NPAR_IQAC(qab, qc, p); }
 #endif

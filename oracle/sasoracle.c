@@ -715,3 +715,38 @@ void sasoracle_spin_weights(double in_spin, double out_spin, double *wt)
 {
     spin_weights(in_spin, out_spin, wt);
 }
+
+/*
+ * Hard-sphere structure factor, Percus-Yevick closure, as
+ * sasmodels/models/hardsphere.c:1-90 computes it in double precision:
+ *   |R| < 1e-12            -> 1
+ *   x = |2 q R| < 5e-6     -> 1/A, the q = 0 limit
+ *   x < 0.05               -> 1/(1 + phi FF), FF its Taylor series to x^4
+ *   otherwise              -> 1/(1 + 24 phi FF/x^2), FF from sin and cos
+ * with A = ((1 + 2 phi)/(1 - phi)^2)^2, B = -6 phi ((1 + phi/2)/(1 - phi)^2)^2,
+ * G = phi A / 2 (the Percus-Yevick direct-correlation coefficients).
+ */
+double sasoracle_hardsphere(double q, double radius, double phi)
+{
+    if (fabs(radius) < 1e-12) return 1.0;
+    const double inv = 1.0 / (1.0 - phi);
+    const double d = inv * inv;
+    const double a = sq((1.0 + 2.0 * phi) * d);
+    const double x = fabs(2.0 * q * radius);
+    if (x < 5e-6) return 1.0 / a;
+    const double b = -6.0 * phi * sq((1.0 + 0.5 * phi) * d);
+    const double g = 0.5 * phi * a;
+    const double x2 = x * x;
+    if (x < 0.05) {
+        const double c0 = 8.0 * a + 6.0 * b + 4.0 * g;
+        const double c2 = -0.8 * a - b / 1.5 - 0.5 * g;
+        const double c4 = a / 35.0 + 0.0125 * b + 0.02 * g;
+        return 1.0 / (1.0 + phi * (c0 + (c2 + c4 * x2) * x2));
+    }
+    const double s = sin(x), c = cos(x), x4 = x2 * x2;
+    const double ta = a * (s - x * c);
+    const double tb = b * (2.0 * x * s - (x2 - 2.0) * c - 2.0);
+    const double tg = g * ((4.0 * x2 - 24.0) * x * s - (x4 - 12.0 * x2 + 24.0) * c + 24.0);
+    const double ff = ((tg / x2 + tb) / x + ta) / x;
+    return 1.0 / (1.0 + 24.0 * phi * ff / x2);
+}

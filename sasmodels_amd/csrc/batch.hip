@@ -11,7 +11,9 @@
 //                   item's records with wave-uniform (scalar) loads
 //   k_batch_final   one workgroup per set: per output, the chunk partials in
 //                   chunk order (FP64), the weight sums, and I(q) with the
-//                   normalisation of Kernel.Iq (sasmodels/kernel.py:80-188)
+//                   normalisation of Kernel.Iq (sasmodels/kernel.py:80-188),
+//                   or of a P@S product with the hard-sphere S(q) evaluated
+//                   there in FP64 (saship_batch_set_structure)
 //   D2H             raw result and/or I(q)
 // The whole sequence can be captured once in a hipGraph and replayed for
 // every later call with the same shapes (a fit step).
@@ -26,6 +28,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "structure.h"
 
 using namespace saship_host;
 
@@ -207,6 +210,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
     const int64_t *__restrict__ voff, const int64_t *__restrict__ poff,
     const int64_t *__restrict__ resoff, const int *__restrict__ nchunk,
     const double *__restrict__ sums, const double *__restrict__ partial,
+    const double *__restrict__ spars, const double *__restrict__ q64, int is2d,
     double *__restrict__ result, double *__restrict__ iq_out)
 {
     const int s = blockIdx.x;
@@ -220,6 +224,19 @@ __global__ __launch_bounds__(256) void k_batch_final(
     double shell = sm[2] / total;
     if (shell == 0.0) shell = 1.0;
     const double scale = values[voff[s]], background = values[voff[s] + 1];
+    double cscale = scale / shell;
+    // P@S (sasmodels/product.py:425-528): S at the effective radius of the
+    // form (mode > 0) or the given one, and at volfraction * form:shell
+    // volume ratio; the number density is volfraction / shell volume
+    const double *sp = spars ? spars + 4 * s : nullptr;
+    double reff = 0.0, phi = 0.0;
+    bool beta = false;
+    if (sp) {
+        reff = sp[3] > 0.0 ? sm[3] / total : sp[0];
+        phi = sp[1] * ((sm[1] / total) / shell);
+        beta = sp[2] > 0.0;
+        cscale *= sp[1];
+    }
     for (int e = blockIdx.y * 256 + threadIdx.x; e < nq; e += 256 * gridDim.y) {
         double f2 = 0.0, f1 = 0.0;
         for (int c = 0; c < nc; c++) {
@@ -230,7 +247,16 @@ __global__ __launch_bounds__(256) void k_batch_final(
             result[resoff[s] + (size_t)e * nout] = f2;
             if (nout == 2) result[resoff[s] + (size_t)e * nout + 1] = f1;
         }
-        if (iq_out) iq_out[q0 + e] = scale / shell * (f2 / total) + background;
+        if (!iq_out) continue;
+        double v = f2 / total;
+        if (sp) {
+            const int64_t g = q0 + e;
+            const double q = is2d ? sqrt(q64[2 * g] * q64[2 * g] + q64[2 * g + 1] * q64[2 * g + 1]) : q64[g];
+            const double S = hardsphere_sq(q, reff, phi);
+            const double F1 = f1 / total;
+            v = beta ? v + F1 * F1 * (S - 1.0) : v * S;
+        }
+        iq_out[q0 + e] = cscale * v + background;
     }
     if (result && blockIdx.y == 0 && threadIdx.x < 4)
         result[resoff[s] + (size_t)nout * nq + threadIdx.x] = sm[threadIdx.x];
@@ -269,7 +295,8 @@ struct Plan {
     int sep_nlp = 0;                  // separable path: bilinear table columns (0: generic path)
     int nodes_per_item = 0;           // separable path: Gauss nodes per work item
     size_t off_values = 0, off_voff = 0, off_det = 0, off_roff = 0, off_poff = 0, off_resoff = 0;
-    size_t off_nchunk = 0, off_items = 0, off_nitems = 0, off_order = 0, off_ipre = 0, bytes = 0;
+    size_t off_nchunk = 0, off_items = 0, off_nitems = 0, off_order = 0, off_ipre = 0, off_spars = 0, bytes = 0;
+    bool structure = false;           // P@S: per-set structure-factor parameters staged at off_spars
     int nblk = 0;                     // separable path: node blocks per (set, tile)
     int max_nq = 0;                   // largest q count of a set
 };
@@ -306,6 +333,10 @@ struct saship_batch {
     int32_t *d_slen = nullptr;
     double *d_sw = nullptr;
     bool separable = true;            // use the separable path when every set allows it
+    // P@S (saship_batch_set_structure): structure-factor model (0: none) and
+    // per set (radius_effective, volfraction, structure_factor_mode, radius_effective_mode)
+    int s_model = 0;
+    std::vector<double> spars;
     char *h_out = nullptr;
     size_t hout_cap = 0;
     // launch shape
@@ -422,6 +453,14 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     p.off_nitems = off; off = align256(off + sizeof(int32_t));
     p.off_order = off; off = align256(off + sizeof(int32_t) * ns);
     p.off_ipre = off; off = align256(off + sizeof(int32_t) * (ns + 1));
+    p.structure = b->s_model != 0;
+    if (p.structure) {
+        for (int s = 0; s < ns; s++)
+            if (b->spars[4 * s + 2] > 0.0 && kind != 0)
+                return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": the beta approximation needs 1D q");
+        p.off_spars = off;
+        off = align256(off + sizeof(double) * 4 * ns);
+    }
     p.bytes = off;
     if (int e = ensure_host(b->h_stage, b->stage_cap, p.bytes)) return e;
     char *h = b->h_stage;
@@ -432,6 +471,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     memcpy(h + p.off_poff, poff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_resoff, resoff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_nchunk, nchunk.data(), sizeof(int32_t) * ns);
+    if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * 4 * ns);
     // work list, longest items first: items are bucketed by the number of
     // mesh points their chunk can hold (an upper bound before pruning)
     int4 *it = (int4 *)(h + p.off_items);
@@ -536,6 +576,8 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
 
     k_batch_final<<<dim3(b->nsets, std::max(1, (p.max_nq + 255) / 256)), 256, 0, b->st>>>(p.nout, b->d_qoff, d_values, d_voff, d_poff, d_resoff,
                                                 d_nchunk, b->d_sums, b->d_partial,
+                                                p.structure ? (const double *)(d + p.off_spars) : nullptr,
+                                                b->q64, b->is2d,
                                                 want_result ? d_result : nullptr, want_iq ? d_iq : nullptr);
     HIP_TRY(hipGetLastError());
     double *d_siq = d_iq + p.nq_total;
@@ -603,7 +645,7 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
-                                    p.sep_nlp, b->n_smear};
+                                    p.sep_nlp, b->n_smear, b->s_model};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
@@ -659,6 +701,8 @@ int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const
     *out = nullptr;
     if (!m) return fail(SASHIP_EINVAL, "model is NULL");
     if (nsets < 1 || !nq) return fail(SASHIP_EINVAL, "need nsets >= 1 and a q-count array");
+    if (model_structure(m->id))
+        return fail(SASHIP_EINVAL, "a structure factor is batched through saship_batch_set_structure of its form factor");
     std::vector<int64_t> qoff(nsets + 1, 0);
     for (int s = 0; s < nsets; s++) {
         if (nq[s] < 0) return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": nq < 0");
@@ -798,8 +842,42 @@ int saship_batch_call(saship_batch *b, int32_t kind, const int32_t *details, int
     if ((kind == 0) == (b->is2d != 0))
         return fail(SASHIP_EINVAL, kind == 0 ? "Iq kernel needs 1D q" : "2D kernels need (qx, qy) input");
     std::lock_guard<std::mutex> lock(b->mu);
+    if (b->s_model)
+        for (int s = 0; s < b->nsets; s++) {
+            const double er = b->spars[4 * s + 3];
+            if (er != 0.0 && er != (double)radius_effective_mode)
+                return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": structure radius_effective_mode " +
+                                               std::to_string((int)er) + " differs from the call's " +
+                                               std::to_string(radius_effective_mode));
+        }
     HIP_TRY(hipSetDevice(b->dev));
     return run(b, kind, details, n_details, values, values_offset, cutoff, radius_effective_mode, result, iq);
+}
+
+int saship_batch_set_structure(saship_batch *b, int32_t s_model, const double *pars, int32_t nsets)
+{
+    if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
+    std::lock_guard<std::mutex> lock(b->mu);
+    if (s_model == 0) {
+        b->s_model = 0;
+        b->spars.clear();
+        return 0;
+    }
+    if (!model_structure(s_model)) return fail(SASHIP_EINVAL, "model " + std::to_string(s_model) + " is not a structure factor");
+    if (model_structure(b->m->id)) return fail(SASHIP_EINVAL, "the batch model is itself a structure factor");
+    if (!pars || nsets != b->nsets) return fail(SASHIP_EINVAL, "need 4 structure parameters for each of the batch's sets");
+    for (int s = 0; s < nsets; s++) {
+        const double *sp = pars + 4 * s;
+        const std::string where = "set " + std::to_string(s) + ": ";
+        for (int k = 0; k < 4; k++)
+            if (!std::isfinite(sp[k])) return fail(SASHIP_EINVAL, where + "structure parameters must be finite");
+        if (sp[3] < 0.0 || sp[3] != std::floor(sp[3]) || sp[3] > 100.0)
+            return fail(SASHIP_EINVAL, where + "radius_effective_mode must be a small non-negative integer");
+    }
+    b->spars.assign(pars, pars + 4 * (size_t)nsets);
+    b->s_model = s_model;
+    b->have_last = false;
+    return 0;
 }
 
 int saship_batch_replay(saship_batch *b, int32_t nsteps, double *device_ms)

@@ -73,22 +73,26 @@ inline int cu_count(int dev)
     return n;
 }
 
+// Model ids 0-4 are the form factors, 5 the hard-sphere structure factor
+// (two parameters, nothing polydisperse, no sld: no magnetic values).
+constexpr int MODEL_HARDSPHERE = 5;
 inline int model_npars(int id)
 {
-    static const int np[5] = {3, 6, 6, 8, 5};
+    static const int np[6] = {3, 6, 6, 8, 5, 2};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[5] = {1, 4, 4, 5, 2};
+    static const int mp[6] = {1, 4, 4, 5, 2, 0};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[5] = {15, 18, 18, 23, 20};
+    static const int nv[6] = {15, 18, 18, 23, 20, 4};
     return nv[id];
 }
 inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3; }
+inline bool model_structure(int id) { return id == MODEL_HARDSPHERE; }
 
 // Separable cylinder 1D meshes: only the radius (parameter 2) and length
 // (parameter 3) levels have more than one point, each at most SEP_MAX.
@@ -129,6 +133,7 @@ inline int validate_set(int id, int kind, const int32_t *details, int64_t n_deta
         if (par < 0 || par >= model_npars(id) || len < 1 || off < 0 || off + len > num_weights || stride < 1)
             return fail(SASHIP_EINVAL, where + "inconsistent details buffer at pd level " + std::to_string(i));
     }
+    if (kind == 2 && model_structure(id)) return fail(SASHIP_EINVAL, where + "structure factors have no magnetic kernel");
     if (kind != 0 && model_oriented(id)) {
         const int tp = details[4 * mp + 3];
         if (tp < 0 || tp + 1 >= model_npars(id)) return fail(SASHIP_EINVAL, where + "theta_par out of range");

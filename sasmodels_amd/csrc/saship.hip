@@ -16,6 +16,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "structure.h"
 
 using namespace saship_host;
 
@@ -56,6 +57,27 @@ __global__ __launch_bounds__(256) void k_finalize(
         const size_t idx = (size_t)nout * nq + threadIdx.x;
         const double v = sums[threadIdx.x];
         result[idx] = accumulate ? result[idx] + v : v;
+    }
+}
+
+// Structure-factor evaluation (hard spheres, FP64): nothing is polydisperse,
+// so the mesh is one point of weight 1 whatever pd_start/pd_stop are
+// (kernel_iq.c with MAX_PD 0), counted when 1 > cutoff.  One thread per q,
+// q = |(qx, qy)| for 2D input; the weight sums are (1, 1, 1, 0): volume 1,
+// no effective radius.  Adds to the result when accumulate is set.
+__global__ __launch_bounds__(256) void k_hardsphere(int nq, int is2d, const double *__restrict__ q64,
+                                                    const double *__restrict__ values, int include,
+                                                    int accumulate, double *__restrict__ result)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nq) {
+        const double q = is2d ? sqrt(q64[2 * i] * q64[2 * i] + q64[2 * i + 1] * q64[2 * i + 1]) : q64[i];
+        const double s = include ? hardsphere_sq(q, values[2], values[3]) : 0.0;
+        result[i] = accumulate ? result[i] + s : s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        const double v = include && threadIdx.x < 3 ? 1.0 : 0.0;
+        result[nq + threadIdx.x] = accumulate ? result[nq + threadIdx.x] + v : v;
     }
 }
 
@@ -189,6 +211,23 @@ __global__ __launch_bounds__(256) void k_fma_probe(float *out, int iters, float 
 
 }  // namespace
 
+namespace {
+// Download the result of the evaluation just enqueued and account its time.
+int finish_call(saship_kernel *k, size_t rbytes, double *result)
+{
+    HIP_TRY(hipMemcpyAsync(k->h_result, k->d_result, rbytes, hipMemcpyDeviceToHost, k->st));
+    HIP_TRY(hipStreamSynchronize(k->st));
+    memcpy(result, k->h_result, rbytes);
+    if (k->timing) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, k->ev0, k->ev1));
+        k->t_ms += ms;
+        k->ncalls++;
+    }
+    return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char *saship_last_error(void) { return last_error().c_str(); }
@@ -250,7 +289,8 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
 {
     if (!out) return fail(SASHIP_EINVAL, "out is NULL");
     *out = nullptr;
-    if (model_id < 0 || model_id > 4) return fail(SASHIP_EINVAL, "unknown model id " + std::to_string(model_id));
+    if (model_id < 0 || model_id > MODEL_HARDSPHERE)
+        return fail(SASHIP_EINVAL, "unknown model id " + std::to_string(model_id));
     if (quad_n < 0 || quad_n > 100000) return fail(SASHIP_EINVAL, "quadrature size must be 0 (adaptive) or 1..100000");
     HIP_TRY(hipSetDevice(current_device()));
     saship_model *m = new (std::nothrow) saship_model();
@@ -370,7 +410,7 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, "")) return e;
     std::lock_guard<std::mutex> lock(k->mu);
     HIP_TRY(hipSetDevice(k->dev));
-    const int nq = k->nq, nout = kind == 0 ? 2 : 1;
+    const int nq = k->nq, nout = kind == 0 && !model_structure(id) ? 2 : 1;
     const size_t vbytes = sizeof(double) * (size_t)n_values;
     const size_t dbytes = sizeof(int32_t) * (size_t)n_details;
     const size_t doff = (vbytes + 255) & ~(size_t)255;
@@ -388,20 +428,20 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     const int32_t *d_det = (const int32_t *)(k->d_args + doff);
     const double *d_val = (const double *)k->d_args;
     int rc;
+    if (model_structure(id)) {
+        if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
+        const int include = 1.0 > cutoff ? 1 : 0;
+        sas::k_hardsphere<<<std::max(1, (nq + 255) / 256), 256, 0, k->st>>>(nq, k->is2d, k->q64, d_val, include,
+                                                                            accumulate, k->d_result);
+        HIP_TRY(hipGetLastError());
+        if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
+        return finish_call(k, rbytes, result);
+    }
     const int nlp = id == 1 && kind == 0 && k->m->quad_n > 0 && k->separable ? cylinder_separable(details) : 0;
     if (nlp) {
         rc = launch_sep(k, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
         if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(k->h_result, k->d_result, rbytes, hipMemcpyDeviceToHost, k->st));
-        HIP_TRY(hipStreamSynchronize(k->st));
-        memcpy(result, k->h_result, rbytes);
-        if (k->timing) {
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, k->ev0, k->ev1));
-            k->t_ms += ms;
-            k->ncalls++;
-        }
-        return 0;
+        return finish_call(k, rbytes, result);
     }
     switch (id) {
     case 0: rc = launch<sas::Sphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
@@ -411,16 +451,7 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     default: rc = launch<sas::CoreShellSphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     }
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(k->h_result, k->d_result, rbytes, hipMemcpyDeviceToHost, k->st));
-    HIP_TRY(hipStreamSynchronize(k->st));
-    memcpy(result, k->h_result, rbytes);
-    if (k->timing) {
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, k->ev0, k->ev1));
-        k->t_ms += ms;
-        k->ncalls++;
-    }
-    return 0;
+    return finish_call(k, rbytes, result);
 }
 
 int saship_kernel_set_path(saship_kernel *k, int32_t separable)

@@ -119,7 +119,7 @@ class HipKernel(Kernel):
         self.dtype = np.dtype("d")
         self.q_input = HipInput(q_vectors)
         self.dim = "2d" if self.q_input.is_2d else "1d"
-        nout = 2 if self.dim == "1d" else 1
+        nout = 2 if self.dim == "1d" and self.info.have_Fq else 1
         self.result = np.empty(self.q_input.nq*nout + 4, "d")
         mh = model._model_handle()
         h = ct.c_void_p()
@@ -251,6 +251,21 @@ class HipBatch:
         self._res_keep = keep
         self._n_out = keep[0].astype(np.int64)
 
+    def set_structure(self, pars, s_model=5):
+        """
+        Multiply every set's form factor by a structure factor on the device
+        (saship_batch_set_structure): *pars* is an (nsets, 4) array of
+        (radius_effective, volfraction, structure_factor_mode,
+        radius_effective_mode) per set; None switches the product off.
+        Calls then return the P@S intensity of each set.
+        """
+        if pars is None:
+            _lib.check(_lib.lib().saship_batch_set_structure(self._handle, 0, None, 0))
+            return
+        pars = np.ascontiguousarray(pars, "d").reshape(-1, 4)
+        _lib.check(_lib.lib().saship_batch_set_structure(
+            self._handle, int(s_model), pars.ctypes.data, pars.shape[0]))
+
     def _pack(self, calls):
         if len(calls) != self.nsets:
             raise ValueError("expected %d (call_details, values) pairs, got %d" % (self.nsets, len(calls)))
@@ -277,9 +292,9 @@ class HipBatch:
             result.ctypes.data if want_result else None, iq.ctypes.data if want_iq else None))
         return result, iq
 
-    def __call__(self, calls, cutoff=0.0, magnetic=False):
+    def __call__(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0):
         """List of I(q), one per set."""
-        _, iq = self.raw(calls, cutoff, magnetic, 0, want_result=False, want_iq=True)
+        _, iq = self.raw(calls, cutoff, magnetic, radius_effective_mode, want_result=False, want_iq=True)
         if self._n_out is not None:
             return np.split(iq, np.cumsum(self._n_out)[:-1])
         return np.split(iq, self.qoff[1:-1])

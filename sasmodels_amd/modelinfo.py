@@ -11,12 +11,14 @@ path depends on (sasmodels/modelinfo.py):
   and ``nvalues`` (:454-459);
 * ``polydisperse`` for volume and orientation parameters and
   ``relative_pd`` for volume parameters (:176-177);
-* ``theta_offset``, ``max_pd``, ``pd_1d``/``pd_2d`` (:470-495).
+* ``theta_offset``, ``max_pd``, ``pd_1d``/``pd_2d`` (:470-495);
+* choice-list limits ``[[name, ...]]`` -> ``(0, len - 1)`` (:95-108).
 
-The five parameter tables are the reference's own (sphere.py:66-72,
+The parameter tables are the reference's own (sphere.py:66-72,
 cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
-core_shell_sphere.py:78-82), with the attributes the kernels use
-(``have_Fq``, ``valid``, ``radius_effective_modes``, the orientation mode).
+core_shell_sphere.py:78-82, hardsphere.py:84-88), with the attributes the
+kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
+orientation mode, ``structure_factor``).
 """
 
 import numpy as np
@@ -67,13 +69,24 @@ def parse_parameter(name, units, default, user_limits, ptype, description):
     """
     if ptype not in ("volume", "orientation", "sld", "magnetic", ""):
         raise ValueError("unexpected type %r for %s" % (ptype, name))
-    low, high = user_limits
-    limits = (float(low), float(high))
+    choices = []
+    if isinstance(user_limits[0], (tuple, list)):
+        choices = list(user_limits[0])
+        if not all(isinstance(k, str) for k in choices):
+            raise ValueError("choices must be strings for %s" % name)
+        limits = (0., len(choices) - 1.)
+    else:
+        low, high = user_limits
+        limits = (float(low), float(high))
+        if limits[0] >= limits[1]:
+            raise ValueError("require lower limit < upper limit")
     if default < limits[0] or default > limits[1]:
         raise ValueError("default value %r not in range for %s" % (default, name))
     if ptype == "" and (name.startswith("sld") or name.endswith("sld")):
         ptype = "sld"
-    return Parameter(name, units, float(default), limits, ptype, description)
+    par = Parameter(name, units, float(default), limits, ptype, description)
+    par.choices = choices
+    return par
 
 
 class ParameterTable:
@@ -239,6 +252,12 @@ _MODEL_TABLE = {
         ["sld_solvent", "1e-6/Ang^2", 3.0, [-inf, inf], "sld", "Solvent scattering length density"],
     ], dict(radius_effective_modes=["outer radius", "core radius"], xy_mode="qa",
             has_quadrature=False)),
+    # structure factor: Iq only (no F), no volume, nothing polydisperse;
+    # evaluated in FP64 (the reference marks it double precision only)
+    "hardsphere": (5, [
+        ["radius_effective", "Ang", 50.0, [0, inf], "", "effective radius of hard sphere"],
+        ["volfraction", "", 0.2, [0, 0.74], "", "volume fraction of hard spheres"],
+    ], dict(structure_factor=True, have_Fq=False, xy_mode="qa", has_quadrature=False)),
 }
 
 MODELS = tuple(_MODEL_TABLE)
@@ -260,6 +279,7 @@ def make_model_info(name):
     info.kernel_id = kernel_id
     info.parameters = info.base = make_parameter_table(rows)
     info.have_Fq = True
+    info.structure_factor = False
     info.integration_size = 0
     for key, value in extra.items():
         setattr(info, key, value)

@@ -105,6 +105,70 @@ MODEL_KATS = {
     "core_shell_sphere": CORE_SHELL_SPHERE,
 }
 
+# P@S products with the hardsphere structure factor (the "@S" entries of
+# the model files): (form factor, pars, q, I[, intermediates])
+PRODUCT_KATS = [
+    # sphere.py:142-176
+    ("sphere", {"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45, "volfraction": 0.2,
+                "structure_factor_mode": 1, "radius_effective_mode": 0},
+     [0.01, 0.1, 0.2], [1.32473756e+03, 7.36633631e-01, 4.67686201e-02]),
+    ("sphere", {"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45, "volfraction": 0.2,
+                "radius_effective": 45.0, "structure_factor_mode": 1, "radius_effective_mode": 0},
+     [0.01], [1316.2990966463444]),
+    ("sphere", {"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45, "volfraction": 0.2,
+                "radius_effective": 120.0, "structure_factor_mode": 1, "radius_effective_mode": 0},
+     [0.01, 0.1, 0.2], [1.57928589e+03, 7.37067923e-01, 4.67686197e-02]),
+    ("sphere", {"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45, "volfraction": 0.2,
+                "structure_factor_mode": 0, "radius_effective_mode": 1},
+     [0.01, 0.1, 0.2], [1.10112335e+03, 7.41366536e-01, 4.66630207e-02]),
+    ("sphere", {"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45, "volfraction": 0.2,
+                "structure_factor_mode": 0, "radius_effective_mode": 0},
+     [0.01, 0.1, 0.2], [7.82803598e+02, 6.85943611e-01, 4.71586457e-02]),
+    # sphere.py:182-197, with the intermediate results (beta(Q) not given)
+    ("sphere", {"radius": 120., "radius_pd": 0.2, "radius_pd_n": 45, "volfraction": 0.2,
+                "radius_effective": 120.0, "structure_factor_mode": 1, "radius_effective_mode": 0},
+     [0.01, 0.1, 0.2], [1.57928589e+03, 7.37067923e-01, 4.67686197e-02],
+     {"P(Q)": [3487.905895219423, 0.7360339734027279, 0.04576861975646704],
+      "S(Q)": [0.31569726516764035, 1.005886362143737, 0.9976927625183415],
+      "S_eff(Q)": [0.4527888487743462, 1.0000461252997597, 0.9999999981038543],
+      "volume": 8087664.122641933, "volume_ratio": 1.0, "radius_effective": 0.0}),
+    # cylinder.py:181-206
+    ("cylinder", {"scale": 5., "background": 0., "volfraction": 0.2,
+                  "structure_factor_mode": 0, "radius_effective_mode": 1},
+     [0.01, 0.05, 0.2], [7.35571916e+01, 5.78147797e+01, 4.15623248e-2]),
+    ("cylinder", {"scale": 5., "background": 0., "volfraction": 0.2,
+                  "structure_factor_mode": 1, "radius_effective_mode": 1},
+     [0.01, 0.05, 0.2], [8.29729770e+01, 5.44206752e+01, 4.17598382e-2]),
+    ("cylinder", {"scale": 5., "background": 0., "volfraction": 0.2,
+                  "radius_pd": 0.2, "radius_pd_n": 15, "length_pd": 0.1,
+                  "structure_factor_mode": 0, "radius_effective_mode": 1},
+     [0.01, 0.05, 0.2], [87.50350037, 63.95202427, 0.27889988]),
+    ("cylinder", {"scale": 5., "background": 0., "volfraction": 0.2,
+                  "radius_pd": 0.2, "radius_pd_n": 15, "length_pd": 0.1,
+                  "structure_factor_mode": 1, "radius_effective_mode": 1},
+     [0.01, 0.05, 0.2], [132.2101165, 59.89948174, 0.28048784]),
+    # ellipsoid.py:185-232
+    ("ellipsoid", {"scale": 1., "background": 0., "volfraction": 0.2,
+                   "structure_factor_mode": 1, "radius_effective_mode": 2},
+     [0.006, 0.05, 0.1], [330.0082676127404, 10.96932155837644, 1.35347369429977]),
+    ("ellipsoid", {"scale": 1., "background": 0., "volfraction": 0.2,
+                   "structure_factor_mode": 1, "radius_effective_mode": 3},
+     [0.006, 0.05, 0.1], [273.64522316236287, 10.93682961898512, 1.352864896244188]),
+    ("ellipsoid", {"scale": 1., "background": 0., "volfraction": 0.2,
+                   "structure_factor_mode": 1, "radius_effective_mode": 4},
+     [0.006, 0.05, 0.1], [1062.3690121068357, 10.970147034298845, 1.3534794742102454]),
+    ("ellipsoid", {"scale": 1., "background": 0., "volfraction": 0.2,
+                   "structure_factor_mode": 1, "radius_effective_mode": 1},
+     [0.006, 0.01, 0.05, 0.1],
+     [529.0109355849872, 419.2280055002956, 10.970118278295908, 1.3534762023877278]),
+]
+
+# hardsphere.py:107-111
+HARDSPHERE = [
+    ({"scale": 1.0, "background": 0.0, "radius_effective": 50.0, "volfraction": 0.2},
+     [0.001, 0.1], [0.209128, 0.930587]),
+]
+
 # direct_model.py:562-603, rtol 1e-6: (model, pars, q, I)
 DIRECT_MODEL = [
     ("sphere", dict(radius=200, background=0), [0.1], [0.6190146273894904]),

@@ -1,0 +1,243 @@
+"""
+P@S products on the GPU: the hard-sphere structure-factor kernel, the
+ProductKernel (P and S kernels on the device, combined as
+sasmodels/product.py:425-528 does) and the batched product with S and the
+combination fused into the batch's final kernel.  Checked against the
+reference's "@S" known answers and the FP64 oracle.
+"""
+
+import numpy as np
+import pytest
+
+from gpu_helpers import RTOL, masked_relerr
+from kat_tables import HARDSPHERE, PRODUCT_KATS, is_near
+from oracle import oracle
+from sasmodels_amd import core, direct_model, generate, modelinfo, resolution as R
+from sasmodels_amd.details import make_details, make_kernel_args
+from sasmodels_amd.kernelhip import HipBatch, HipModel
+from sasmodels_amd.product import ProductBatch
+from sasmodels_amd import _lib
+from test_product import oracle_product
+
+pytestmark = pytest.mark.gpu
+
+#: the device S(q) against the oracle (the reference's formula): the
+#: reference's closed form loses up to 1.7e-6 to cancellation just above its
+#: series switch at 2qR = 0.05; from 2qR = 0.3 on both agree to 1e-10
+S_RTOL, S_RTOL_FAR = 5e-6, 1e-10
+
+
+def _hs_call(info, radius, phi):
+    det = make_details(info, np.array([1, 1]), np.array([0, 1]), 2)
+    values = np.zeros(32)
+    values[:4] = [1.0, 0.0, radius, phi]
+    values[4:6] = [radius, phi]
+    values[6:8] = [1.0, 1.0]
+    return det, values
+
+
+@pytest.mark.parametrize("dim", ["1d", "2d"])
+def test_hardsphere_kernel(dim):
+    hm = HipModel(modelinfo.make_model_info("hardsphere"))
+    q = np.logspace(-5, 0.3, 400)
+    if dim == "1d":
+        qv = [q]
+        qmag = q
+    else:
+        ang = np.linspace(0, 2*np.pi, q.size)
+        qv = [q*np.cos(ang), q*np.sin(ang)]
+        qmag = np.hypot(*qv)
+    kern = hm.make_kernel(qv)
+    for radius in (0.0, 3.0, 50.0, 700.0):
+        for phi in (0.0, 0.2, 0.55):
+            print("hardsphere", dim, radius, phi, flush=True)
+            det, values = _hs_call(hm.info, radius, phi)
+            got = kern(det, values, 0.0, False)
+            want = oracle.hardsphere(qmag, radius, phi)
+            np.testing.assert_allclose(got, want, rtol=S_RTOL)
+            far = 2*qmag*radius >= 0.3
+            np.testing.assert_allclose(got[far], want[far], rtol=S_RTOL_FAR)
+            r = kern.result
+            assert list(r[-4:]) == [1.0, 1.0, 1.0, 0.0]
+    # weight 1 is dropped at cutoff >= 1; pd_start > 0 accumulates
+    det, values = _hs_call(hm.info, 50.0, 0.2)
+    kern._call_kernel(det, values, 1.0, False, 0)
+    assert not kern.result.any()
+    kern._call_kernel(det, values, 0.0, False, 0)
+    first = kern.result.copy()
+    kern._call_kernel(det, values, 0.0, False, 0, pd_start=1, pd_stop=1)
+    np.testing.assert_array_equal(kern.result, 2*first)
+    kern.release()
+    hm.release()
+
+
+def test_hardsphere_kat_gpu():
+    model = core.load_model("hardsphere")
+    pars, q, want = HARDSPHERE[0]
+    kern = model.make_kernel([np.asarray(q, "d")])
+    got = direct_model.call_kernel(kern, pars)
+    assert all(is_near(w, g) for w, g in zip(want, got)), got
+    kern.release()
+    # a structure factor has no magnetic kernel
+    kern2 = model.make_kernel([np.array([0.01, 0.02]), np.array([0.02, 0.0])])
+    det, values = _hs_call(model.info, 50.0, 0.2)
+    with pytest.raises(_lib.HipError):
+        kern2._call_kernel(det, values, 0.0, True, 0)
+    kern2.release()
+
+
+@pytest.mark.parametrize("case", PRODUCT_KATS, ids=lambda c: "%s-%s" % (c[0], sorted(c[1].items())))
+def test_product_kernel_kats(case):
+    p_name, pars, q, want = case[:4]
+    print("product", p_name, pars, flush=True)
+    model = core.load_model(p_name + "@hardsphere")
+    kern = model.make_kernel([np.asarray(q, "d")])
+    got = direct_model.call_kernel(kern, pars)
+    assert all(is_near(w, g) for w, g in zip(want, got)), (want, got)
+    ref = oracle_product(p_name, q, pars)[0]
+    assert masked_relerr(got, ref) <= RTOL
+    if len(case) > 4:
+        parts = kern.results()
+        for key in ("P(Q)", "S(Q)", "S_eff(Q)"):
+            assert all(is_near(w, g) for w, g in zip(case[4][key], parts[key][1])), key
+        for key in ("volume", "volume_ratio", "radius_effective"):
+            assert is_near(case[4][key], parts[key]), key
+    kern.release()
+
+
+def _pars(rng, beta, er_mode, pd=True):
+    pars = dict(radius=rng.uniform(10, 60), length=rng.uniform(60, 500), sld=4.0, sld_solvent=1.0,
+                volfraction=rng.uniform(0.05, 0.4), radius_effective=rng.uniform(20, 200),
+                structure_factor_mode=beta, radius_effective_mode=er_mode,
+                scale=rng.uniform(0.5, 2), background=rng.uniform(0, 0.01))
+    if pd:
+        pars.update(radius_pd=0.1, radius_pd_n=9, length_pd=0.15, length_pd_n=7)
+    return pars
+
+
+def test_product_kernel_2d_and_magnetic():
+    rng = np.random.default_rng(4)
+    model = core.load_model("cylinder@hardsphere")
+    qx, qy = np.meshgrid(np.linspace(-0.2, 0.2, 17), np.linspace(-0.15, 0.2, 13))
+    qv = [qx.ravel(), qy.ravel()]
+    kern = model.make_kernel(qv)
+    p_info = model.info.composition[1][0]
+    for er_mode in (0, 1, 3):
+        pars = _pars(rng, 0, er_mode)
+        pars.update(theta=30.0, phi=20.0, theta_pd=5.0, theta_pd_n=5)
+        for magnetic in (False, True):
+            if magnetic:
+                pars.update(sld_M0=2.0, sld_mtheta=30.0, up_frac_i=0.2, up_frac_f=0.7)
+            print("product 2d", er_mode, magnetic, flush=True)
+            got = direct_model.call_kernel(kern, pars)
+            # the oracle: P's Fq on the same P call, S at |q|
+            p_pars = {k: v for k, v in pars.items()
+                      if k not in ("volfraction", "radius_effective", "structure_factor_mode",
+                                   "radius_effective_mode", "scale", "background")}
+            kl = type("K", (), {"info": p_info, "dtype": np.dtype("d")})()
+            det, val, mag = make_kernel_args(kl, direct_model.get_mesh(p_info, p_pars, dim="2d"))
+            fq = oracle.OracleKernel(p_info, qv).Fq(det, val, 0.0, mag, er_mode)
+            want, _ = oracle.product(np.hypot(*qv), fq, pars["scale"], pars["background"],
+                                     pars["radius_effective"], pars["volfraction"], False, er_mode)
+            assert masked_relerr(got, want) <= 2*RTOL
+    pars = _pars(rng, 1, 1)
+    with pytest.raises(NotImplementedError):
+        direct_model.call_kernel(kern, pars)
+    kern.release()
+
+
+def _calls(info, sets):
+    kl = type("K", (), {"info": info, "dtype": np.dtype("d")})()
+    return [make_kernel_args(kl, direct_model.get_mesh(info, p, dim=d))[:2] for p, d in sets]
+
+
+@pytest.mark.parametrize("quad,graph", [(0, False), (76, False), (76, True)])
+def test_product_batch_matches_product_kernel(quad, graph):
+    rng = np.random.default_rng(11 + quad)
+    info = core.load_model_info("cylinder@hardsphere")
+    generate.set_integration_size(info.composition[1][0], quad)
+    model = core.build_model(info)
+    q_list = [np.logspace(-3, np.log10(0.4), int(rng.integers(30, 200))) for _ in range(10)]
+    pars = [_pars(rng, s % 2, 2 if s % 3 else 0, pd=s % 4 != 3) for s in range(10)]
+    batch = ProductBatch(model, [[q] for q in q_list], use_graph=graph)
+    calls = _calls(info, [(p, "1d") for p in pars])
+    for rep in range(3 if graph else 1):
+        print("product batch quad", quad, "graph", graph, "rep", rep, flush=True)
+        out = batch(calls)
+        for q, p, got in zip(q_list, pars, out):
+            kern = model.make_kernel([q])
+            single = direct_model.call_kernel(kern, p)
+            kern.release()
+            assert masked_relerr(got, single) <= RTOL
+            if quad == 0:
+                assert masked_relerr(got, oracle_product("cylinder", q, p)[0]) <= RTOL
+    # sets of one call must share the effective-radius mode
+    bad = list(pars)
+    bad[0] = dict(bad[0], radius_effective_mode=1)
+    bad[1] = dict(bad[1], radius_effective_mode=2)
+    with pytest.raises(ValueError):
+        batch(_calls(info, [(p, "1d") for p in bad]))
+    batch.release()
+
+
+def test_product_batch_structure_checks():
+    rng = np.random.default_rng(5)
+    hm = HipModel(modelinfo.make_model_info("sphere"))
+    q = np.logspace(-3, -1, 50)
+    batch = HipBatch(hm, [[q], [q]])
+    with pytest.raises(_lib.HipError):
+        batch.set_structure(np.zeros((3, 4)))
+    with pytest.raises(_lib.HipError):
+        batch.set_structure(np.array([[50, 0.2, 0, 1.5], [50, 0.2, 0, 0]]))
+    with pytest.raises(_lib.HipError):
+        batch.set_structure(np.array([[50, 0.2, 0, 0], [50, 0.2, 0, 0]]), s_model=1)
+    batch.set_structure(np.array([[50, 0.2, 0, 1], [50, 0.2, 0, 0]]))
+    kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
+    calls = [make_kernel_args(kl, direct_model.get_mesh(hm.info, dict(radius=40.0), dim="1d"))[:2]] * 2
+    with pytest.raises(_lib.HipError):
+        batch(calls, radius_effective_mode=0)      # set 0 asks for mode 1
+    out = batch(calls, radius_effective_mode=1)
+    # S at the sphere radius for set 0, at 50 for set 1
+    base = HipBatch(hm, [[q], [q]])
+    plain = base(calls)
+    for s, radius in enumerate((40.0, 50.0)):
+        bg = 1e-3
+        want = (plain[s] - bg)*0.2*oracle.hardsphere(q, radius, 0.2) + bg
+        np.testing.assert_allclose(out[s], want, rtol=1e-6)
+    batch.set_structure(None)
+    np.testing.assert_allclose(batch(calls)[0], plain[0], rtol=1e-13)
+    batch.release()
+    base.release()
+    with pytest.raises(_lib.HipError):
+        HipBatch(HipModel(modelinfo.make_model_info("hardsphere")), [[q]])
+
+
+def test_product_batch_2d_and_smeared():
+    rng = np.random.default_rng(21)
+    info = core.load_model_info("cylinder@hardsphere")
+    model = core.build_model(info)
+    qx, qy = np.meshgrid(np.linspace(-0.2, 0.2, 11), np.linspace(-0.2, 0.2, 9))
+    qv = [qx.ravel(), qy.ravel()]
+    pars = [dict(_pars(rng, 0, 1), theta=40.0, phi=10.0) for _ in range(3)]
+    batch = ProductBatch(model, [qv]*3)
+    out = batch(_calls(info, [(p, "2d") for p in pars]))
+    kern = model.make_kernel(qv)
+    for p, got in zip(pars, out):
+        assert masked_relerr(got, direct_model.call_kernel(kern, p)) <= RTOL
+    kern.release()
+    with pytest.raises(NotImplementedError):
+        batch(_calls(info, [(dict(p, structure_factor_mode=1), "2d") for p in pars]))
+    batch.release()
+    # pinhole smearing of the product on the device
+    q = np.logspace(-3, np.log10(0.3), 80)
+    res = [R.Pinhole1D(q, 0.08*q), R.Slit1D(q, q_length=0.02, q_width=0.0)]
+    pars = [_pars(rng, 1, 1), _pars(rng, 0, 0)]
+    batch = ProductBatch(model, [[r.q_calc] for r in res])
+    batch.set_resolution(res)
+    out = batch(_calls(info, [(p, "1d") for p in pars]))
+    for r, p, got in zip(res, pars, out):
+        kern = model.make_kernel([r.q_calc])
+        want = r.apply(direct_model.call_kernel(kern, p))
+        kern.release()
+        assert masked_relerr(got, want) <= RTOL
+    batch.release()
