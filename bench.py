@@ -30,9 +30,9 @@ non-packed instructions); traffic and VALU instruction counts of the same
 launch from the counter summary in profiles/ when present.  cpu_baseline: the FP64 oracle on the
 host cores, on a bounded sample of the workload.
 
-other_configs: the sphere 1D, pinhole-smeared cylinder, ellipsoid 2D,
-batched core-shell cylinder and batched magnetic core-shell sphere
-configurations, each with device throughput, FP64 accuracy on a subset and
+other_configs: the sphere 1D, pinhole-smeared cylinder, cylinder@hardsphere
+(beta), ellipsoid 2D, batched core-shell cylinder and batched magnetic
+core-shell sphere configurations, each with device throughput, FP64 accuracy on a subset and
 the oracle's rate on the host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
@@ -108,15 +108,15 @@ def _kernel_like(info):
     return type("K", (), {"info": info, "dtype": np.dtype("d")})()
 
 
-def c2_calls(info, n, offset=0):
-    """(details, values) for n jittered copies of the workload."""
+def c2_calls(info, n, offset=0, extra=None):
+    """(details, values) for n jittered copies of the workload (plus *extra* parameters)."""
     from sasmodels_amd import direct_model
     from sasmodels_amd.details import make_kernel_args
     kl = _kernel_like(info)
     out = []
     for k in range(offset, offset + n):
         f = 1 + 1e-4*(k % 97)
-        pars = dict(C2_PARS, radius=C2_PARS["radius"]*f, length=C2_PARS["length"]*(2 - f))
+        pars = dict(C2_PARS, radius=C2_PARS["radius"]*f, length=C2_PARS["length"]*(2 - f), **(extra or {}))
         out.append(make_kernel_args(kl, direct_model.get_mesh(info, pars, dim="1d"))[:2])
     return out
 
@@ -366,6 +366,46 @@ def bench_configs(threads, cpu=True):
         ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), res.apply(r)).max()
                                                 for a, r in zip(iq, refs)))
     out["cylinder_batched_pinhole"] = ent
+    model.release()
+
+    # the cylinder headline curves as cylinder@hardsphere, beta approximation,
+    # excluded-volume effective radius: S(q) and the combination fused into
+    # the batch's final kernel (FP64)
+    from sasmodels_amd.product import ProductBatch
+    info = core.load_model_info("cylinder@hardsphere")
+    p_info = info.composition[1][0]
+    generate.set_integration_size(p_info, QUAD)
+    model = core.build_model(info, platform="hip")
+    sfac = dict(volfraction=0.2, structure_factor_mode=1, radius_effective_mode=1)
+    calls = c2_calls(info, 64, offset=12000, extra=sfac)
+    pb = ProductBatch(model, [[C2_Q]]*len(calls), use_graph=True)
+    pb.autotune(calls)
+    dt, iq = _timed(lambda: pb(calls), 10)
+    pb.replay(2)
+    with_s = pb.replay(20)*1e-3/20
+    p_calls = pb._split(calls)[0]
+    pb.batch.set_structure(None)
+    pb.batch(p_calls, radius_effective_mode=1)
+    pb.replay(2)
+    without = pb.replay(20)*1e-3/20
+    pb.release()
+    ent = {"workload": "cylinder headline curves (64 per launch) times the hard-sphere S(q), volfraction 0.2, "
+                       "beta approximation, excluded-volume effective radius",
+           "curves_per_s": 64/with_s, "curves_per_s_host_inclusive": 64/dt, "ms_per_launch": with_s*1e3,
+           "structure_factor_ms": (with_s - without)*1e3}
+    if cpu:
+        from oracle import oracle
+        from oracle.oracle import OracleKernel
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        refs = []
+        for det, val in c2_calls(p_info, 4, offset=12000):
+            fq = OracleKernel(p_info, [C2_Q]).Fq(det, val, 0.0, False, 1)
+            refs.append(oracle.product(C2_Q, fq, 1.0, 1e-3, 50.0, 0.2, True, 1)[0])
+        t = time.perf_counter() - t0
+        ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), r).max() for a, r in zip(iq, refs)))
+        ent["cpu_curves_per_s"] = 4/t
+    out["cylinder_hardsphere_beta_batched"] = ent
     model.release()
 
     # C3: ellipsoid 2D on 256 x 256, theta/phi/radius_equatorial dispersion 20 x 20 x 20
