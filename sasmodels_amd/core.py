@@ -4,11 +4,13 @@ Model loading: :func:`load_model_info`, :func:`build_model`, :func:`load_model`.
 Mirrors the entry points of sasmodels/core.py:117-341.  The only platform
 here is ``"hip"`` (libsaship.so on an AMD GPU); asking for another platform
 raises, and so does a missing library or GPU -- there is no CPU fallback.
-Model strings are a model name or a product ``P@S`` of a form factor and a
-structure factor (core.py:151-153); mixtures (``+``, ``*``) are not provided.
+Model strings are a model name, a product ``P@S`` of a form factor and a
+structure factor, or a mixture of those with ``+`` (sum) and ``*``
+(product of intensities), parsed as the reference does (core.py:130-167):
+split on every ``+`` first, then on ``*``, then on ``@``.
 """
 
-from . import modelinfo, product
+from . import mixture, modelinfo, product
 from .kernelhip import HipModel
 
 PLATFORMS = ("hip",)
@@ -20,10 +22,11 @@ def list_models():
 
 
 def load_model_info(model_string):
-    """ModelInfo for a model name or a ``P@S`` product."""
-    if any(op in model_string for op in "+*"):
-        raise ValueError("model mixtures (%s) are not provided by the hip backend"
-                         % model_string)
+    """ModelInfo for a model name, a ``P@S`` product or a ``+``/``*`` mixture."""
+    for op in "+*":
+        if op in model_string:
+            parts = [load_model_info(part) for part in model_string.split(op)]
+            return mixture.make_mixture_info(parts, operation=op)
     if "@" in model_string:
         parts = model_string.split("@")
         if len(parts) != 2:
@@ -46,10 +49,14 @@ def build_model(model_info, dtype=None, platform="hip"):
     if dtype is not None and str(dtype).rstrip("!") not in (
             "single", "double", "fast", "float32", "float64", "default"):
         raise ValueError("unsupported dtype %r" % (dtype,))
-    if model_info.composition is not None and model_info.composition[0] == "product":
-        P, S = [build_model(part, dtype=dtype, platform=platform)
-                for part in model_info.composition[1]]
-        return product.ProductModel(model_info, P, S)
+    if model_info.composition is not None:
+        kind, parts = model_info.composition
+        models = [build_model(part, dtype=dtype, platform=platform) for part in parts]
+        if kind == "product":
+            return product.ProductModel(model_info, *models)
+        if kind == "mixture":
+            return mixture.MixtureModel(model_info, models)
+        raise ValueError("unknown composition %r" % kind)
     return HipModel(model_info, dtype=dtype)
 
 

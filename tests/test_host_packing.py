@@ -173,7 +173,9 @@ def test_backend_selection_errors():
     with pytest.raises(ValueError):
         core.build_model(core.load_model_info("sphere"), platform="dll")
     with pytest.raises(ValueError):
-        core.load_model_info("sphere*hardsphere")
+        core.load_model_info("sphere@cylinder")
+    with pytest.raises(ValueError):
+        core.load_model_info("barbell+sphere")
     with pytest.raises(ValueError):
         modelinfo.make_model_info("barbell")
     info = core.load_model_info("cylinder")

@@ -98,7 +98,7 @@ def test_product_info_errors():
     with pytest.raises(ValueError):
         core.load_model_info("hardsphere@hardsphere")
     with pytest.raises(ValueError):
-        core.load_model_info("sphere+cylinder")
+        core.load_model_info("sphere@hardsphere@hardsphere")
     s_bad = modelinfo.make_model_info("sphere")
     with pytest.raises(TypeError):
         product.make_product_info(modelinfo.make_model_info("cylinder"), s_bad)
