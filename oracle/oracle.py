@@ -23,7 +23,8 @@ REF_DIR = os.path.join(HERE, "_ref")
 
 KIND_IQ, KIND_IQXY, KIND_IMAGNETIC = 0, 1, 2
 MODEL_IDS = {"sphere": 0, "cylinder": 1, "ellipsoid": 2,
-             "core_shell_cylinder": 3, "core_shell_sphere": 4}
+             "core_shell_cylinder": 3, "core_shell_sphere": 4,
+             "parallelepiped": 6, "triaxial_ellipsoid": 7}
 
 _lib = None
 _D = ct.c_double
@@ -58,6 +59,9 @@ def lib():
         L.sasoracle_iqac.argtypes = [ct.c_int, _D, _D, ct.c_void_p]
         L.sasoracle_form_volume.restype = _D
         L.sasoracle_form_volume.argtypes = [ct.c_int, ct.c_void_p]
+        L.sasoracle_iqabc.restype = _D
+        L.sasoracle_iqabc.argtypes = [ct.c_int, _D, _D, _D, ct.c_void_p]
+        L.sasoracle_view_jitter_abc.argtypes = [_D]*6 + [ct.c_void_p]
         L.sasoracle_hardsphere.restype = _D
         L.sasoracle_hardsphere.argtypes = [_D, _D, _D]
         L.sasoracle_radius_effective.restype = _D
@@ -197,6 +201,9 @@ def ref_lib(name):
     if hasattr(L, "ref_iqac"):
         L.ref_iqac.restype = _D
         L.ref_iqac.argtypes = [_D, _D, ct.c_void_p]
+    if hasattr(L, "ref_iqabc"):
+        L.ref_iqabc.restype = _D
+        L.ref_iqabc.argtypes = [_D, _D, _D, ct.c_void_p]
     for name in ("ref_j1", "ref_2j1x_x", "ref_sph_j1c", "ref_sinx_x"):
         if hasattr(L, name):
             getattr(L, name).restype = _D

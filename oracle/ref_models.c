@@ -11,8 +11,9 @@
  * vector to the reference's static functions so that ctypes can call them.
  *
  * One shared object is built per (model, quadrature) pair:
- *   -DREF_MODEL=<1..6>  sphere, cylinder, ellipsoid, core_shell_cylinder,
- *                       core_shell_sphere, hardsphere (structure factor)
+ *   -DREF_MODEL=<1..8>  sphere, cylinder, ellipsoid, core_shell_cylinder,
+ *                       core_shell_sphere, hardsphere (structure factor),
+ *                       parallelepiped, triaxial_ellipsoid
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -71,8 +72,27 @@
 #elif REF_MODEL == 6
 #  include "hardsphere.c"
 double ref_sq(double q, double radius_effective, double volfraction) { return Iq(q, radius_effective, volfraction); }
+#elif REF_MODEL == 7 || REF_MODEL == 8
+#  if REF_MODEL == 8
+#    include "lib/sas_3j1x_x.c"
+#  endif
+#  ifdef REF_FIXED76
+#    include "lib/gauss76.c"
+#    include "lib/nonadaptive.c"
+#  else
+#    include "lib/adaptive.c"
+#  endif
+#  if REF_MODEL == 7
+#    include "parallelepiped.c"
+#  else
+#    include "triaxial_ellipsoid.c"
+#  endif
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4])
+#  define NPAR_IQABC(a, b, c, p) Iqabc(a, b, c, p[0], p[1], p[2], p[3], p[4])
+#  define NPAR_VOLUME(p) form_volume(p[2], p[3], p[4])
+#  define NPAR_REFF(m, p) radius_effective(m, p[2], p[3], p[4])
 #else
-#  error "REF_MODEL must be 1..6"
+#  error "REF_MODEL must be 1..8"
 #endif
 
 #ifdef NPAR_CALL_FQ
@@ -82,6 +102,9 @@ double ref_reff(int mode, const double *p) { return NPAR_REFF(mode, p); }
 #endif
 #ifdef NPAR_IQAC
 double ref_iqac(double qab, double qc, const double *p) { return NPAR_IQAC(qab, qc, p); }
+#endif
+#ifdef NPAR_IQABC
+double ref_iqabc(double qa, double qb, double qc, const double *p) { return NPAR_IQABC(qa, qb, qc, p); }
 #endif
 
 #if REF_MODEL == 1 || REF_MODEL == 3

@@ -23,7 +23,11 @@
  *     gengauss.py:10-46 (numpy leggauss), recomputed here by Newton's method
  *   - models: sphere.c:1-20, cylinder.c:1-102, ellipsoid.c:1-105,
  *     core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30 with
- *     lib/core_shell.c:8-30
+ *     lib/core_shell.c:8-30, parallelepiped.c:1-150,
+ *     triaxial_ellipsoid.c:1-160 (nested rules adaptive.c:11243-11298);
+ *     view/jitter rotation with psi qabc_rotation/qabc_apply
+ *     kernel_iq.c:251-310
+ *   - hard-sphere structure factor hardsphere.c:1-90
  *
  * Floating-point expressions keep the reference's operand order so that,
  * compiled like the reference DLL (gcc -std=c99 -O2, no FMA contraction),
@@ -116,6 +120,21 @@ static int quad_select(int quad_n, double qr, const double **z, const double **w
     else if (qr < 10) n = 20;
     else if (qr < 100) n = 76;
     else if (qr < 750) n = 500;
+    else n = 5000;
+    return gl_table(n, z, w);
+}
+
+/* Nested rules (adaptive.c:11243-11298): the outer call passes
+ * outer_n = ADAPTIVE_MAX_OUTER, the inner call the outer rule's size. */
+#define O_ADAPTIVE_MAX_N 100000
+#define O_ADAPTIVE_MAX_OUTER (O_ADAPTIVE_MAX_N / 500)
+static int quad_select_nested(int quad_n, double qr, int outer_n, const double **z, const double **w)
+{
+    int n;
+    if (quad_n > 0) n = quad_n;
+    else if (qr < 10 || outer_n > O_ADAPTIVE_MAX_N / 76) n = 20;
+    else if (qr < 100 || outer_n > O_ADAPTIVE_MAX_N / 500) n = 76;
+    else if (qr < 750 || outer_n > O_ADAPTIVE_MAX_N / 5000) n = 500;
     else n = 5000;
     return gl_table(n, z, w);
 }
@@ -430,6 +449,157 @@ static void css_fq(double q, const double *p, int quad_n, double *f1, double *f2
     *f2 = 1.0e-4 * f * f;
 }
 
+/* ---- parallelepiped (parallelepiped.c:1-150):
+ *      p = sld, sld_solvent, length_a, length_b, length_c, theta, phi, psi ---- */
+static double ppd_vol(const double *p) { return p[2] * p[3] * p[4]; }
+static double ppd_excluded(double a, double b, double c)
+{
+    const double len[3] = {a, b, c};
+    const double lmax = fmax(len[0], fmax(len[1], len[2]));
+    double l1 = lmax, l2 = lmax;
+    const double l3 = lmax;
+    for (int i = 0; i < 3; i++) {
+        if (len[i] < l1) { l2 = l1; l1 = len[i]; }
+        else if (len[i] < l2) l2 = len[i];
+    }
+    double r, l;
+    if (l2 - l1 > l3 - l2) { r = sqrt(l2 * l3 / O_PI); l = l1; }
+    else { r = sqrt(l1 * l2 / O_PI); l = l3; }
+    return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (O_PI * r + l)));
+}
+static double ppd_reff(int mode, const double *p)
+{
+    const double a = p[2], b = p[3], c = p[4];
+    switch (mode) {
+    default:
+    case 1: return ppd_excluded(a, b, c);
+    case 2: return cbrt(a * b * c / O_4PI_3);
+    case 3: return 0.5 * a;
+    case 4: return 0.5 * b;
+    case 5: return 0.5 * c;
+    case 6: return sqrt(a * b / O_PI);
+    case 7: return 0.5 * sqrt(a * a + b * b);
+    case 8: return 0.5 * sqrt(a * a + b * b + c * c);
+    }
+}
+static void ppd_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    const double a = p[2], b = p[3], c = p[4];
+    const double mu = 0.5 * q * b;
+    const double as = a / b, cs = c / b;
+    const double *zo, *wo;
+    const int no = quad_select_nested(quad_n, q * fmax(sqrt(a * a + b * b), c) / 2, O_ADAPTIVE_MAX_OUTER, &zo, &wo);
+    double o1 = 0.0, o2 = 0.0;
+    for (int i = 0; i < no; i++) {
+        const double sigma = 0.5 * (zo[i] + 1.0);
+        const double mp = mu * sqrt(1.0 - sigma * sigma);
+        const double *zi, *wi;
+        const int ni = quad_select_nested(quad_n, mp * fmax(as, 1.0), no, &zi, &wi);
+        double i1 = 0.0, i2 = 0.0;
+        for (int j = 0; j < ni; j++) {
+            const double uu = 0.5 * (zi[j] + 1.0);
+            const double su = sin(O_PI / 2 * uu), cu = cos(O_PI / 2 * uu);
+            const double fq = o_sinx_x(mp * su * as) * o_sinx_x(mp * cu);
+            i1 += wi[j] * fq;
+            i2 += wi[j] * fq * fq;
+        }
+        i1 *= 0.5;
+        i2 *= 0.5;
+        const double si = o_sinx_x(mu * cs * sigma);
+        o1 += wo[i] * i1 * si;
+        o2 += wo[i] * i2 * si * si;
+    }
+    o1 *= 0.5;
+    o2 *= 0.5;
+    const double s = (p[0] - p[1]) * ppd_vol(p);
+    *f1 = 1.0e-2 * s * o1;
+    *f2 = 1.0e-4 * s * s * o2;
+}
+static double ppd_iqabc(double qa, double qb, double qc, const double *p)
+{
+    const double f = ppd_vol(p) * (p[0] - p[1]) * o_sinx_x(0.5 * p[2] * qa) * o_sinx_x(0.5 * p[3] * qb)
+                     * o_sinx_x(0.5 * p[4] * qc);
+    return 1.0e-4 * f * f;
+}
+
+/* ---- triaxial ellipsoid (triaxial_ellipsoid.c:1-160): p = sld, sld_solvent,
+ *      radius_equat_minor, radius_equat_major, radius_polar, theta, phi, psi ---- */
+static double tri_vol(const double *p) { return O_4PI_3 * p[2] * p[3] * p[4]; }
+static double tri_curvature(double ra, double rb, double rc)
+{
+    /* the reference compares the 0/1 result of ra == rb with rc */
+    if ((ra == rb) == rc) return rc;
+    if (ra * rb * rc == 0.) return 0.;
+    const double rad[3] = {ra, rb, rc};
+    const double rmax = fmax(rad[0], fmax(rad[1], rad[2]));
+    double r1 = rmax, r2 = rmax, r3 = rmax;
+    for (int i = 0; i < 3; i++) {
+        if (rad[i] < r1) { r3 = r2; r2 = r1; r1 = rad[i]; }
+        else if (rad[i] < r2) r2 = rad[i];
+    }
+    double re, rp;
+    if (r2 - r1 > r3 - r2) { re = sqrt(r2 * r3); rp = r1; }
+    else { re = sqrt(r1 * r2); rp = r3; }
+    const double ratio = (rp < re ? rp / re : re / rp);
+    const double e1 = sqrt(1.0 - ratio * ratio);
+    const double b1 = 1.0 + asin(e1) / (e1 * ratio);
+    const double bL = (1.0 + e1) / (1.0 - e1);
+    const double b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
+    const double delta = 0.75 * b1 * b2;
+    const double ddd = 2.0 * (delta + 1.0) * rp * re * re;
+    return 0.5 * cbrt(ddd);
+}
+static double tri_reff(int mode, const double *p)
+{
+    const double ra = p[2], rb = p[3], rc = p[4];
+    switch (mode) {
+    default:
+    case 1: return tri_curvature(ra, rb, rc);
+    case 2: return cbrt(ra * rb * rc);
+    case 3: { const double m = (ra < rb ? ra : rb); return (m < rc ? m : rc); }
+    case 4: { const double m = (ra < rb ? rb : ra); return (m > rc ? m : rc); }
+    }
+}
+static void tri_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    const double lo = fmin(p[2], p[3]), hi = fmax(p[2], p[3]);
+    const double mid = fmax(lo, p[4]);
+    const double minor = fmin(lo, p[4]), major = fmin(hi, mid), polar = fmax(hi, mid);
+    const double pa = sq(minor / major) - 1.0;
+    const double pc = sq(polar / major) - 1.0;
+    const double qr_inner = fmax(q * minor, q * major);
+    const double qr_max = fmax(qr_inner, q * polar);
+    const double *zo, *wo, *zi, *wi;
+    const int no = quad_select_nested(quad_n, qr_max, O_ADAPTIVE_MAX_OUTER, &zo, &wo);
+    const int ni = quad_select_nested(quad_n, qr_inner, no, &zi, &wi);
+    double o1 = 0.0, o2 = 0.0;
+    for (int i = 0; i < no; i++) {
+        const double phi = zo[i] * O_PI_4 + O_PI_4;
+        const double pas = pa * sq(sin(phi));
+        double i1 = 0.0, i2 = 0.0;
+        for (int j = 0; j < ni; j++) {
+            const double usq = sq(zi[j] * 0.5 + 0.5);
+            const double r = major * sqrt(pas * (1.0 - usq) + 1.0 + pc * usq);
+            const double fq = o_sph_j1c(q * r);
+            i1 += wi[j] * fq;
+            i2 += wi[j] * fq * fq;
+        }
+        o1 += wo[i] * i1;
+        o2 += wo[i] * i2;
+    }
+    o1 *= 0.25;
+    o2 *= 0.25;
+    const double s = (p[0] - p[1]) * O_4PI_3 * minor * major * polar;
+    *f1 = 1.0e-2 * s * o1;
+    *f2 = 1.0e-4 * s * s * o2;
+}
+static double tri_iqabc(double qa, double qb, double qc, const double *p)
+{
+    const double qr = sqrt(sq(p[2] * qa) + sq(p[3] * qb) + sq(p[4] * qc));
+    const double f = o_sph_j1c(qr);
+    return 1.0e-4 * sq(tri_vol(p) * (p[0] - p[1]) * f);
+}
+
 /* ------------------------------------------------------------------ */
 /* Model table                                                         */
 /* ------------------------------------------------------------------ */
@@ -441,19 +611,26 @@ typedef struct {
     double (*reff)(int mode, const double *p);
     void (*fq)(double q, const double *p, int quad_n, double *f1, double *f2);
     double (*iqac)(double qab, double qc, const double *p);
+    double (*iqabc)(double qa, double qb, double qc, const double *p);
 } OModel;
 
 static int always_valid(const double *p) { (void)p; return 1; }
 
 /* model ids: 0 sphere, 1 cylinder, 2 ellipsoid, 3 core_shell_cylinder,
- * 4 core_shell_sphere (the same ids libsaship uses) */
-static const OModel MODELS[5] = {
-    {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0},
-    {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac},
-    {6, 2, 4, 1, {0, 1, -1}, always_valid, ell_vol, ell_reff, ell_fq, ell_iqac},
-    {8, 3, 5, 1, {0, 1, 2}, always_valid, csc_vol, csc_reff, csc_fq, csc_iqac},
-    {5, 3, 2, 0, {2, 3, 4}, always_valid, css_vol, css_reff, css_fq, 0},
+ * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid (the ids
+ * libsaship uses; 5, the hard-sphere structure factor, has its own entry) */
+#define N_MODELS 8
+static const OModel MODELS[N_MODELS] = {
+    {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0},
+    {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0},
+    {6, 2, 4, 1, {0, 1, -1}, always_valid, ell_vol, ell_reff, ell_fq, ell_iqac, 0},
+    {8, 3, 5, 1, {0, 1, 2}, always_valid, csc_vol, csc_reff, csc_fq, csc_iqac, 0},
+    {5, 3, 2, 0, {2, 3, 4}, always_valid, css_vol, css_reff, css_fq, 0, 0},
+    {0, 0, 0, 0, {-1, -1, -1}, 0, 0, 0, 0, 0, 0},
+    {8, 2, 5, 1, {0, 1, -1}, always_valid, ppd_vol, ppd_reff, ppd_fq, 0, ppd_iqabc},
+    {8, 2, 5, 1, {0, 1, -1}, always_valid, tri_vol, tri_reff, tri_fq, 0, tri_iqabc},
 };
+static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npars > 0; }
 
 /* ------------------------------------------------------------------ */
 /* Rotation and magnetism helpers                                      */
@@ -472,6 +649,40 @@ static void view_jitter_row(double theta, double phi, double dtheta, double dphi
     const double j31 = sdt, j32 = -sdp * cdt, j33 = cdp * cdt;
     *r31 = j31 * v11 + j32 * v21 + j33 * v31;
     *r32 = j31 * v12 + j32 * v22 + j33 * v32;
+}
+
+/* First two columns of (jitter^-1 * view^-1) with psi: (qx, qy) -> (qa, qb, qc)
+ * (kernel_iq.c:251-296), r[0..5] = R11, R12, R21, R22, R31, R32 */
+static void view_jitter_abc(double theta, double phi, double psi,
+                            double dtheta, double dphi, double dpsi, double r[6])
+{
+    double st = sin(theta * O_PI_180), ct = cos(theta * O_PI_180);
+    double sp = sin(phi * O_PI_180), cp = cos(phi * O_PI_180);
+    double ss = sin(psi * O_PI_180), cs = cos(psi * O_PI_180);
+    const double v11 = -sp * ss + cp * cs * ct;
+    const double v12 = sp * cs * ct + ss * cp;
+    const double v21 = -sp * cs - ss * cp * ct;
+    const double v22 = -sp * ss * ct + cp * cs;
+    const double v31 = st * cp;
+    const double v32 = sp * st;
+    st = sin(dtheta * O_PI_180); ct = cos(dtheta * O_PI_180);
+    sp = sin(dphi * O_PI_180); cp = cos(dphi * O_PI_180);
+    ss = sin(dpsi * O_PI_180); cs = cos(dpsi * O_PI_180);
+    const double j11 = cs * ct;
+    const double j12 = sp * st * cs + ss * cp;
+    const double j13 = sp * ss - st * cp * cs;
+    const double j21 = -ss * ct;
+    const double j22 = -sp * ss * st + cp * cs;
+    const double j23 = sp * cs + ss * st * cp;
+    const double j31 = st;
+    const double j32 = -sp * ct;
+    const double j33 = cp * ct;
+    r[0] = j11 * v11 + j12 * v21 + j13 * v31;
+    r[1] = j11 * v12 + j12 * v22 + j13 * v32;
+    r[2] = j21 * v11 + j22 * v21 + j23 * v31;
+    r[3] = j21 * v12 + j22 * v22 + j23 * v32;
+    r[4] = j31 * v11 + j32 * v21 + j33 * v31;
+    r[5] = j31 * v12 + j32 * v22 + j33 * v32;
 }
 
 /* spin cross-section weights dd, du.re, ud.re, uu, du.im, ud.im (kernel_iq.c:84-124) */
@@ -532,7 +743,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
                      const double *q, double *result,
                      double cutoff, int32_t reff_mode)
 {
-    if (model_id < 0 || model_id > 4 || kind < 0 || kind > 2) return -1;
+    if (!model_ok(model_id) || kind < 0 || kind > 2) return -1;
     if (quad_n < 0) return -2;
     const OModel *m = &MODELS[model_id];
     const int np = m->npars, maxpd = m->max_pd;
@@ -544,6 +755,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
     const double *pd_weight = pd_value + num_weights;
     const int nout = (kind == 0) ? 2 : 1;
     const int oriented2d = (kind != 0) && m->oriented;
+    const int abc = m->iqabc != 0;
     quad_prime(quad_n);
 
     double acc[4];
@@ -552,12 +764,17 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
 
     double base[8];
     for (int i = 0; i < np; i++) base[i] = values[2 + i];
-    double view_theta = 0, view_phi = 0;
+    double view_theta = 0, view_phi = 0, view_psi = 0;
     if (oriented2d) {
         view_theta = values[theta_par + 2];
         view_phi = values[theta_par + 3];
         base[theta_par] = 0.;
         base[theta_par + 1] = 0.;
+        if (abc) {
+            /* psi: kernel_iq.c:541-543 */
+            view_psi = values[theta_par + 4];
+            base[theta_par + 2] = 0.;
+        }
     }
 
     double xsw[6] = {0, 0, 0, 0, 0, 0}, smt = 0, cmt = 1, smp = 0, cmp = 1;
@@ -569,7 +786,8 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
 
     /* Pass 1: walk the mesh in order, keep the accepted points. */
     const int npts = pd_stop > pd_start ? pd_stop - pd_start : 0;
-    double *pts = (double *)malloc(sizeof(double) * (size_t)(npts ? npts : 1) * (np + 3));
+    const int rw = np + 7;   /* parameters, weight, rotation */
+    double *pts = (double *)malloc(sizeof(double) * (size_t)(npts ? npts : 1) * rw);
     int nacc = 0;
     for (int step = pd_start; step < pd_stop; step++) {
         double p[8];
@@ -590,13 +808,17 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
         acc[1] += weight * form;
         acc[2] += weight * form;
         if (reff_mode != 0) acc[3] += weight * m->reff(reff_mode, p);
-        if (oriented2d)
-            view_jitter_row(view_theta, view_phi, p[theta_par], p[theta_par + 1], &r31, &r32);
-        double *rec = pts + (size_t)nacc * (np + 3);
+        double *rec = pts + (size_t)nacc * rw;
         memcpy(rec, p, sizeof(double) * np);
         rec[np] = weight;
-        rec[np + 1] = r31;
-        rec[np + 2] = r32;
+        if (oriented2d && abc) {
+            view_jitter_abc(view_theta, view_phi, view_psi, p[theta_par], p[theta_par + 1],
+                            p[theta_par + 2], rec + np + 1);
+        } else if (oriented2d) {
+            view_jitter_row(view_theta, view_phi, p[theta_par], p[theta_par + 1], &r31, &r32);
+            rec[np + 1] = r31;
+            rec[np + 2] = r32;
+        }
         nacc++;
     }
 
@@ -606,7 +828,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
 #endif
     for (int iq = 0; iq < nq; iq++) {
         for (int a = 0; a < nacc; a++) {
-            const double *rec = pts + (size_t)a * (np + 3);
+            const double *rec = pts + (size_t)a * rw;
             double p[8];
             memcpy(p, rec, sizeof(double) * np);
             const double weight = rec[np];
@@ -618,8 +840,13 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
                 continue;
             }
             const double qx = q[2 * iq], qy = q[2 * iq + 1];
-            double qab = 0, qc = 0;
-            if (oriented2d) {
+            double qab = 0, qc = 0, qa = 0, qb = 0;
+            if (oriented2d && abc) {
+                /* qabc_apply (kernel_iq.c:298-310) */
+                qa = rec[np + 1] * qx + rec[np + 2] * qy;
+                qb = rec[np + 3] * qx + rec[np + 4] * qy;
+                qc = rec[np + 5] * qx + rec[np + 6] * qy;
+            } else if (oriented2d) {
                 const double dqc = rec[np + 1] * qx + rec[np + 2] * qy;
                 const double dqab_sq = -dqc * dqc + qx * qx + qy * qy;
                 qab = dqab_sq > 0.0 ? sqrt(dqab_sq) : 0.0;
@@ -627,7 +854,8 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
             }
             double f2 = 0.0;
             if (kind == 1) {
-                if (oriented2d) f2 = m->iqac(qab, qc, p);
+                if (oriented2d && abc) f2 = m->iqabc(qa, qb, qc, p);
+                else if (oriented2d) f2 = m->iqac(qab, qc, p);
                 else { double f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &f2); }
             } else {
                 const double qsq = qx * qx + qy * qy;
@@ -641,7 +869,8 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
                                                  values[mi], values[mi + 1], values[mi + 2]);
                         }
                         double fx;
-                        if (oriented2d) fx = m->iqac(qab, qc, p);
+                        if (oriented2d && abc) fx = m->iqabc(qa, qb, qc, p);
+                        else if (oriented2d) fx = m->iqac(qab, qc, p);
                         else { double f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &fx); }
                         f2 += xsw[xs] * fx;
                     }
@@ -674,7 +903,7 @@ int sasoracle_gauss(int n, double *z, double *w)
 
 int sasoracle_fq(int model_id, int quad_n, double q, const double *p, double *f1, double *f2)
 {
-    if (model_id < 0 || model_id > 4) return -1;
+    if (!model_ok(model_id)) return -1;
     quad_prime(quad_n);
     MODELS[model_id].fq(q, p, quad_n, f1, f2);
     return 0;
@@ -682,19 +911,31 @@ int sasoracle_fq(int model_id, int quad_n, double q, const double *p, double *f1
 
 double sasoracle_iqac(int model_id, double qab, double qc, const double *p)
 {
-    if (model_id < 0 || model_id > 4 || !MODELS[model_id].iqac) return NAN;
+    if (!model_ok(model_id) || !MODELS[model_id].iqac) return NAN;
     return MODELS[model_id].iqac(qab, qc, p);
+}
+
+double sasoracle_iqabc(int model_id, double qa, double qb, double qc, const double *p)
+{
+    if (!model_ok(model_id) || !MODELS[model_id].iqabc) return NAN;
+    return MODELS[model_id].iqabc(qa, qb, qc, p);
+}
+
+void sasoracle_view_jitter_abc(double theta, double phi, double psi, double dtheta, double dphi,
+                               double dpsi, double *r)
+{
+    view_jitter_abc(theta, phi, psi, dtheta, dphi, dpsi, r);
 }
 
 double sasoracle_form_volume(int model_id, const double *p)
 {
-    if (model_id < 0 || model_id > 4) return NAN;
+    if (!model_ok(model_id)) return NAN;
     return MODELS[model_id].vol(p);
 }
 
 double sasoracle_radius_effective(int model_id, int mode, const double *p)
 {
-    if (model_id < 0 || model_id > 4) return NAN;
+    if (!model_ok(model_id)) return NAN;
     return MODELS[model_id].reff(mode, p);
 }
 

@@ -388,7 +388,9 @@ int record_width(int id, int kind)
     case 1: return rv_of<sas::Cylinder>(kind);
     case 2: return rv_of<sas::Ellipsoid>(kind);
     case 3: return rv_of<sas::CoreShellCylinder>(kind);
-    default: return rv_of<sas::CoreShellSphere>(kind);
+    case 4: return rv_of<sas::CoreShellSphere>(kind);
+    case 6: return rv_of<sas::Parallelepiped>(kind);
+    default: return rv_of<sas::TriaxialEllipsoid>(kind);
     }
 }
 
@@ -606,7 +608,9 @@ int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, 
     case 1: return enqueue<sas::Cylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 2: return enqueue<sas::Ellipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 3: return enqueue<sas::CoreShellCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    default: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 4: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 6: return enqueue<sas::Parallelepiped>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    default: return enqueue<sas::TriaxialEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }
 

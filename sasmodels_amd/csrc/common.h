@@ -73,25 +73,30 @@ inline int cu_count(int dev)
     return n;
 }
 
-// Model ids 0-4 are the form factors, 5 the hard-sphere structure factor
-// (two parameters, nothing polydisperse, no sld: no magnetic values).
+// Model ids 0-4 and 6-7 are form factors, 5 the hard-sphere structure
+// factor (two parameters, nothing polydisperse, no sld: no magnetic values).
 constexpr int MODEL_HARDSPHERE = 5;
+constexpr int MODEL_LAST = 7;
 inline int model_npars(int id)
 {
-    static const int np[6] = {3, 6, 6, 8, 5, 2};
+    static const int np[8] = {3, 6, 6, 8, 5, 2, 8, 8};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[6] = {1, 4, 4, 5, 2, 0};
+    static const int mp[8] = {1, 4, 4, 5, 2, 0, 5, 5};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[6] = {15, 18, 18, 23, 20, 4};
+    static const int nv[8] = {15, 18, 18, 23, 20, 4, 20, 20};
     return nv[id];
 }
-inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3; }
+inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || id == 6 || id == 7; }
+// orientation (theta, phi, psi): view and jitter about three axes
+inline bool model_abc(int id) { return id == 6 || id == 7; }
+// 1D kernels integrating over orientation by nested (two-level) rules
+inline bool model_nested(int id) { return id == 6 || id == 7; }
 inline bool model_structure(int id) { return id == MODEL_HARDSPHERE; }
 
 // Separable cylinder 1D meshes: only the radius (parameter 2) and length
@@ -136,7 +141,8 @@ inline int validate_set(int id, int kind, const int32_t *details, int64_t n_deta
     if (kind == 2 && model_structure(id)) return fail(SASHIP_EINVAL, where + "structure factors have no magnetic kernel");
     if (kind != 0 && model_oriented(id)) {
         const int tp = details[4 * mp + 3];
-        if (tp < 0 || tp + 1 >= model_npars(id)) return fail(SASHIP_EINVAL, where + "theta_par out of range");
+        const int last = tp + (model_abc(id) ? 2 : 1);
+        if (tp < 0 || last >= model_npars(id)) return fail(SASHIP_EINVAL, where + "theta_par out of range");
     }
     return 0;
 }
@@ -172,7 +178,7 @@ inline int ensure_host(char *&ptr, size_t &cap, size_t bytes)
 
 struct saship_model {
     int id = 0, quad_n = 0, dev = 0;
-    float *tab = nullptr;     // 3 * ntab floats: a | b | w
+    float *tab = nullptr;     // 5 * ntab floats: a | b | w | c | d
     int ntab = 0;
     sas::Quad quad{};
     // kernels and batches made from the model hold a reference; the tables

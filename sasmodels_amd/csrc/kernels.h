@@ -89,17 +89,33 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
     double base[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) base[i] = values[2 + i];
-    double view_theta = 0.0, view_phi = 0.0;
+    double view_theta = 0.0, view_phi = 0.0, view_psi = 0.0;
     if (OR2D) {
         view_theta = values[theta_par + 2];
         view_phi = values[theta_par + 3];
+        if (M::ABC) view_psi = values[theta_par + 4];
 #pragma unroll
         for (int i = 0; i < NP; i++)
-            if (i == theta_par || i == theta_par + 1) base[i] = 0.0;
+            if (i == theta_par || i == theta_par + 1 || (M::ABC && i == theta_par + 2)) base[i] = 0.0;
     }
-    double sv, cv, sp, cp;
+    double sv, cv, sp, cp, ss, cs;
     sincos(view_theta * DEG, &sv, &cv);
     sincos(view_phi * DEG, &sp, &cp);
+    sincos(view_psi * DEG, &ss, &cs);
+    // first two columns of view^-1 (kernel_iq.c:197-230, :251-296)
+    double v11, v12, v21, v22;
+    if (M::ABC) {
+        v11 = -sp * ss + cp * cs * cv;
+        v12 = sp * cs * cv + ss * cp;
+        v21 = -sp * cs - ss * cp * cv;
+        v22 = -sp * ss * cv + cp * cs;
+    } else {
+        v11 = cp * cv;
+        v12 = sp * cv;
+        v21 = -sp;
+        v22 = cp;
+    }
+    const double v31 = sv * cp, v32 = sp * sv;
 
     __shared__ int wave_cnt[EXPAND_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -124,12 +140,13 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
                     if (i == par) p[i] = v;
                 w0 = pd_weight[pd_off[k] + idx] * w0;
             }
-            double dth = 0.0, dph = 0.0;
+            double dth = 0.0, dph = 0.0, dps = 0.0;
             if (OR2D) {
 #pragma unroll
                 for (int i = 0; i < NP; i++) {
                     if (i == theta_par) dth = p[i];
                     if (i == theta_par + 1) dph = p[i];
+                    if (M::ABC && i == theta_par + 2) dps = p[i];
                 }
             }
             const double weight = OR2D ? fabs(cos(dth * DEG)) * w0 : w0;
@@ -149,19 +166,34 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
                         // rows of (jitter^-1 * view^-1) acting on (qx, qy, 0);
                         // row 3 is the reference's qac_rotation (kernel_iq.c:197-230),
                         // rows 1-2 give qab directly instead of sqrt(q^2 - qc^2).
-                        double sdt, cdt, sdp, cdp;
+                        // with psi (qabc models) the jitter also turns about
+                        // the particle's c axis (kernel_iq.c:269-286)
+                        double sdt, cdt, sdp, cdp, sds, cds;
                         sincos(dth * DEG, &sdt, &cdt);
                         sincos(dph * DEG, &sdp, &cdp);
-                        const double v11 = cp * cv, v12 = sp * cv, v21 = -sp, v22 = cp;
-                        const double v31 = sv * cp, v32 = sp * sv;
-                        const double j11 = cdt, j12 = sdp * sdt, j13 = -sdt * cdp;
-                        const double j22 = cdp, j23 = sdp;
+                        sincos(dps * DEG, &sds, &cds);
+                        double j11, j12, j13, j21, j22, j23;
+                        if (M::ABC) {
+                            j11 = cds * cdt;
+                            j12 = sdp * sdt * cds + sds * cdp;
+                            j13 = sdp * sds - sdt * cdp * cds;
+                            j21 = -sds * cdt;
+                            j22 = -sdp * sds * sdt + cdp * cds;
+                            j23 = sdp * cds + sds * sdt * cdp;
+                        } else {
+                            j11 = cdt;
+                            j12 = sdp * sdt;
+                            j13 = -sdt * cdp;
+                            j21 = 0.0;
+                            j22 = cdp;
+                            j23 = sdp;
+                        }
                         const double j31 = sdt, j32 = -sdp * cdt, j33 = cdp * cdt;
                         float *rot = rec + M::REC2D;      // six FP64 entries
                         put_d(rot, 0, j11 * v11 + j12 * v21 + j13 * v31);
                         put_d(rot, 2, j11 * v12 + j12 * v22 + j13 * v32);
-                        put_d(rot, 4, j22 * v21 + j23 * v31);
-                        put_d(rot, 6, j22 * v22 + j23 * v32);
+                        put_d(rot, 4, j21 * v11 + j22 * v21 + j23 * v31);
+                        put_d(rot, 6, j21 * v12 + j22 * v22 + j23 * v32);
                         put_d(rot, 8, j31 * v11 + j32 * v21 + j33 * v31);
                         put_d(rot, 10, j31 * v12 + j32 * v22 + j33 * v32);
                     }

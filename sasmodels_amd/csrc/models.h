@@ -1,4 +1,4 @@
-// models.h -- the five form factors as device functors.
+// models.h -- the form factors as device functors.
 //
 // Each model supplies, from its FP64 kernel-parameter vector p[]
 // (modelinfo order, no scale/background):
@@ -16,7 +16,8 @@
 //
 // Reference formulas (sasmodels/models): sphere.c:1-20, cylinder.c:1-102,
 // ellipsoid.c:1-105, core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30
-// with lib/core_shell.c:8-30.  Parameter order: sphere.py:66-72,
+// with lib/core_shell.c:8-30, parallelepiped.c:1-150,
+// triaxial_ellipsoid.c:1-160.  Parameter order: sphere.py:66-72,
 // cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 // core_shell_sphere.py:78-82.
 #pragma once
@@ -42,6 +43,8 @@ struct Quad {
     const float *a;     // sin(theta) | u^2
     const float *b;     // cos(theta) | unused
     const float *w;     // w*sin(theta) | w
+    const float *c;     // nested rules: outer-node functions (see the model)
+    const float *d;
     int n;              // fixed size, or 0 for adaptive
     int off[4];         // adaptive: offsets of the 20/76/500/5000 tables
 };
@@ -64,11 +67,32 @@ __device__ __forceinline__ void quad_range(const Quad &t, float qr, int &off, in
     }
 }
 
+// Nested rules (adaptive.c:11276-11298 called with outer_n = ADAPTIVE_MAX_OUTER
+// for the outer integral and outer_n = n_outer for the inner one): the outer
+// rule stops at 500 points and the inner rule may use one size less for every
+// size the outer rule went above 20 (20 -> up to 5000, 76 -> 500, 500 -> 76).
+// `level` is the outer rule's size index; a fixed rule uses t.n for both.
+template <bool FIXED>
+__device__ __forceinline__ void quad_nested(const Quad &t, float qr, int cap, int &level, int &off, int &n)
+{
+    if constexpr (FIXED) {
+        level = 0;
+        off = 0;
+        n = t.n;
+    } else {
+        const int k = min(adaptive_pick(qr), cap);
+        level = k;
+        off = k == 0 ? t.off[0] : k == 1 ? t.off[1] : k == 2 ? t.off[2] : t.off[3];
+        n = k == 0 ? 20 : k == 1 ? 76 : k == 2 ? 500 : 5000;
+    }
+}
+
 // ---------------------------------------------------------------- sphere
 struct Sphere {
     static constexpr int ID = 0, NPARS = 3, NMAG = 2, MAX_PD = 1, NCON = 1;
     static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = false;
+    static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr int REC1D = 4, REC2D = 4;
     static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // sld, sld_solvent
@@ -109,6 +133,7 @@ struct Cylinder {
     static constexpr int ID = 1, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
     static constexpr bool SHARED_1D = true;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
@@ -202,6 +227,7 @@ struct Ellipsoid {
     static constexpr int ID = 2, NPARS = 6, NMAG = 2, MAX_PD = 4, NCON = 1;
     static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
@@ -278,6 +304,7 @@ struct CoreShellCylinder {
     static constexpr int ID = 3, NPARS = 8, NMAG = 3, MAX_PD = 5, NCON = 2;
     static constexpr bool SHARED_1D = true;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr int REC1D = 8, REC2D = 12;
     static constexpr int A2D = 10;  // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // core, shell, solvent
@@ -395,6 +422,7 @@ struct CoreShellSphere {
     static constexpr int ID = 4, NPARS = 5, NMAG = 3, MAX_PD = 2, NCON = 2;
     static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = false;
+    static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 6;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return 2 + k; }        // core, shell, solvent
@@ -440,6 +468,241 @@ struct CoreShellSphere {
         con[0] = (float)(sld[0] - sld[1]);
         con[1] = (float)(sld[1] - sld[2]);
     }
+};
+
+// ---------------------------------------------------------------- parallelepiped
+// p = sld, sld_solvent, length_a, length_b, length_c, theta, phi, psi
+// (parallelepiped.py:216-231; parallelepiped.c:1-150).  1D: nested rules,
+// outer sigma = cos(alpha) in [0, 1] (tables c = sigma, d = sqrt(1 - sigma^2)),
+// inner angle pi/2 u (tables a = sin, b = cos), plain weights w.
+struct Parallelepiped {
+    static constexpr int ID = 6, NPARS = 8, NMAG = 2, MAX_PD = 5, NCON = 1;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = true;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 7;
+    SAS_HD static int sld_param(int k) { return k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return p[2] * p[3] * p[4]; }
+    __device__ static double excluded_volume_radius(double a, double b, double c)
+    {
+        // equivalent cylinder: the two closest sides make the cross-section
+        const double hi = fmax(a, fmax(b, c));
+        double s1 = hi, s2 = hi;
+        const double len[3] = {a, b, c};
+        for (int i = 0; i < 3; i++) {
+            if (len[i] < s1) {
+                s2 = s1;
+                s1 = len[i];
+            } else if (len[i] < s2) {
+                s2 = len[i];
+            }
+        }
+        double r, l;
+        if (s2 - s1 > hi - s2) {
+            r = sqrt(s2 * hi / PI_D);
+            l = s1;
+        } else {
+            r = sqrt(s1 * s2 / PI_D);
+            l = hi;
+        }
+        return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (PI_D * r + l)));
+    }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double a = p[2], b = p[3], c = p[4];
+        switch (mode) {
+        default:
+        case 1: return excluded_volume_radius(a, b, c);
+        case 2: return cbrt(a * b * c / FOUR_PI_3_D);
+        case 3: return 0.5 * a;
+        case 4: return 0.5 * b;
+        case 5: return 0.5 * c;
+        case 6: return sqrt(a * b / PI_D);
+        case 7: return 0.5 * sqrt(a * a + b * b);
+        case 8: return 0.5 * sqrt(a * a + b * b + c * c);
+        }
+    }
+    __device__ static double qr_scale(const double *p)
+    {
+        return 0.5 * fmax(sqrt(p[2] * p[2] + p[3] * p[3]), p[4]);
+    }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double s = (p[0] - p[1]) * volume(p);
+        r[0] = (float)(0.5 * p[3]);                 // b/2
+        r[1] = (float)(p[2] / p[3]);                // a/b
+        r[2] = (float)(p[4] / p[3]);                // c/b
+        r[3] = (float)qr_scale(p);
+        r[4] = (float)(w * 1e-2 * s * 0.25);        // two half-interval factors
+        r[5] = (float)(w * 1e-4 * s * s * 0.25);
+        r[6] = r[7] = 0.0f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        const float mu = q * r[0], amax = fmaxf(r[1], 1.0f), mc = mu * r[2];
+        int lo, oo, no;
+        quad_nested<FIXED>(t, q * r[3], 2, lo, oo, no);
+        float o1 = 0.0f, o2 = 0.0f;
+        for (int i = 0; i < no; i++) {
+            const float mp = mu * t.d[oo + i];
+            int li, io, ni;
+            quad_nested<FIXED>(t, mp * amax, 3 - lo, li, io, ni);
+            float i1 = 0.0f, i2 = 0.0f;
+            const float ma = mp * r[1];
+            for (int j = 0; j < ni; j++) {
+                const float f = sinc_prod(ma, t.a[io + j]) * sinc_prod(mp, t.b[io + j]);
+                const float wf = t.w[io + j] * f;
+                i1 += wf;
+                i2 = fmaf(wf, f, i2);
+            }
+            const float si = sinc_prod(mc, t.c[oo + i]);
+            const float ws = t.w[oo + i] * si;
+            o1 = fmaf(ws, i1, o1);
+            o2 = fmaf(ws * si, i2, o2);
+        }
+        f1 = r[4] * o1;
+        f2 = r[5] * o2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        put_d(r, 0, 0.5 * p[2]);
+        put_d(r, 2, 0.5 * p[3]);
+        put_d(r, 4, 0.5 * p[4]);
+        r[6] = (float)volume(p);
+        r[7] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        g[0] = r[6] * sinc_d(q.qa * get_d(r, 0)) * sinc_d(q.qb * get_d(r, 2)) * sinc_d(q.qc * get_d(r, 4));
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- triaxial ellipsoid
+// p = sld, sld_solvent, radius_equat_minor, radius_equat_major, radius_polar,
+// theta, phi, psi (triaxial_ellipsoid.py:143-158; triaxial_ellipsoid.c:1-160).
+// 1D: the radii sorted (minor <= major <= polar), nested rules, outer angle
+// phi = pi/4 (z + 1) (table c = sin^2 phi), inner u = (z + 1)/2 (table a = u^2),
+// plain weights; the inner rule is the same for every outer node.  2D uses
+// the radii in the given order.
+struct TriaxialEllipsoid {
+    static constexpr int ID = 7, NPARS = 8, NMAG = 2, MAX_PD = 5, NCON = 1;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = true;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 7;
+    SAS_HD static int sld_param(int k) { return k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * p[2] * p[3] * p[4]; }
+    __device__ static double curvature_radius(double ra, double rb, double rc)
+    {
+        // the reference's first test compares (ra == rb), a 0/1 value, with rc
+        if ((double)(ra == rb) == rc) return rc;
+        if (ra * rb * rc == 0.) return 0.;
+        const double hi = fmax(ra, fmax(rb, rc));
+        double r1 = hi, r2 = hi, r3 = hi;
+        const double rad[3] = {ra, rb, rc};
+        for (int i = 0; i < 3; i++) {
+            if (rad[i] < r1) {
+                r3 = r2;
+                r2 = r1;
+                r1 = rad[i];
+            } else if (rad[i] < r2) {
+                r2 = rad[i];
+            }
+        }
+        double re, rp;
+        if (r2 - r1 > r3 - r2) {
+            re = sqrt(r2 * r3);
+            rp = r1;
+        } else {
+            re = sqrt(r1 * r2);
+            rp = r3;
+        }
+        // mean curvature radius of the equivalent spheroid (Isihara)
+        const double ratio = rp < re ? rp / re : re / rp;
+        const double e1 = sqrt(1.0 - ratio * ratio);
+        const double b1 = 1.0 + asin(e1) / (e1 * ratio);
+        const double bl = (1.0 + e1) / (1.0 - e1);
+        const double b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bl);
+        const double delta = 0.75 * b1 * b2;
+        return 0.5 * cbrt(2.0 * (delta + 1.0) * rp * re * re);
+    }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double ra = p[2], rb = p[3], rc = p[4];
+        switch (mode) {
+        default:
+        case 1: return curvature_radius(ra, rb, rc);
+        case 2: return cbrt(ra * rb * rc);
+        case 3: {
+            const double m = ra < rb ? ra : rb;
+            return m < rc ? m : rc;
+        }
+        case 4: {
+            const double m = ra < rb ? rb : ra;
+            return m > rc ? m : rc;
+        }
+        }
+    }
+    __device__ static double qr_scale(const double *p) { return fmax(p[2], fmax(p[3], p[4])); }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double lo = fmin(p[2], p[3]), hi = fmax(p[2], p[3]);
+        const double mid = fmax(lo, p[4]);
+        const double minor = fmin(lo, p[4]), major = fmin(hi, mid), polar = fmax(hi, mid);
+        const double s = (p[0] - p[1]) * FOUR_PI_3_D * minor * major * polar;
+        r[0] = (float)major;
+        r[1] = (float)((minor / major) * (minor / major) - 1.0);
+        r[2] = (float)((polar / major) * (polar / major) - 1.0);
+        r[3] = (float)polar;
+        r[4] = (float)(w * 1e-2 * s * 0.25);
+        r[5] = (float)(w * 1e-4 * s * s * 0.25);
+        r[6] = r[7] = 0.0f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int lo, oo, no, li, io, ni;
+        quad_nested<FIXED>(t, q * r[3], 2, lo, oo, no);
+        quad_nested<FIXED>(t, q * r[0], 3 - lo, li, io, ni);
+        const float qb = q * r[0];
+        float o1 = 0.0f, o2 = 0.0f;
+        for (int i = 0; i < no; i++) {
+            const float pa = r[1] * t.c[oo + i];
+            float i1 = 0.0f, i2 = 0.0f;
+            for (int j = 0; j < ni; j++) {
+                const float u2 = t.a[io + j];
+                const float x = qb * sqrt_f(fmaf(pa, 1.0f - u2, fmaf(r[2], u2, 1.0f)));
+                const float f = sph_j1c_f(x);
+                const float wf = t.w[io + j] * f;
+                i1 += wf;
+                i2 = fmaf(wf, f, i2);
+            }
+            o1 = fmaf(t.w[oo + i], i1, o1);
+            o2 = fmaf(t.w[oo + i], i2, o2);
+        }
+        f1 = r[4] * o1;
+        f2 = r[5] * o2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        put_d(r, 0, p[2] * p[2]);
+        put_d(r, 2, p[3] * p[3]);
+        put_d(r, 4, p[4] * p[4]);
+        r[6] = (float)volume(p);
+        r[7] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double qr = sqrt(get_d(r, 0) * q.qa * q.qa + get_d(r, 2) * q.qb * q.qb + get_d(r, 4) * q.qc * q.qc);
+        g[0] = r[6] * sph_j1c_d(qr);
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 }  // namespace sas

@@ -289,7 +289,7 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
 {
     if (!out) return fail(SASHIP_EINVAL, "out is NULL");
     *out = nullptr;
-    if (model_id < 0 || model_id > MODEL_HARDSPHERE)
+    if (model_id < 0 || model_id > MODEL_LAST)
         return fail(SASHIP_EINVAL, "unknown model id " + std::to_string(model_id));
     if (quad_n < 0 || quad_n > 100000) return fail(SASHIP_EINVAL, "quadrature size must be 0 (adaptive) or 1..100000");
     HIP_TRY(hipSetDevice(current_device()));
@@ -301,28 +301,41 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
     // orientation-quadrature tables (the models whose 1D kernel averages orientation)
     const bool cyl_family = model_id == 1 || model_id == 3;
     const bool ell = model_id == 2;
+    const bool ppd = model_id == 6, tri = model_id == 7;
     std::vector<int> sizes;
     if (quad_n > 0) sizes = {quad_n};
     else sizes = {20, 76, 500, 5000};
-    std::vector<float> a, b, w;
+    std::vector<float> a, b, w, c, d;
     std::vector<int> offs;
-    if (cyl_family || ell) {
+    if (cyl_family || ell || ppd || tri) {
         for (int n : sizes) {
             std::vector<double> z, wt;
             gauss_legendre(n, z, wt);
             offs.push_back((int)a.size());
             for (int i = 0; i < n; i++) {
+                const double th = z[i] * sas::PI_4_D + sas::PI_4_D;   // [0, pi/2]
+                const double u = z[i] * 0.5 + 0.5;                       // [0, 1]
+                float ta = 0.0f, tb = 0.0f, tw = (float)wt[i], tc = 0.0f, td = 0.0f;
                 if (cyl_family) {
-                    const double th = z[i] * sas::PI_4_D + sas::PI_4_D;
-                    a.push_back((float)sin(th));
-                    b.push_back((float)cos(th));
-                    w.push_back((float)(wt[i] * sin(th)));
+                    ta = (float)sin(th);
+                    tb = (float)cos(th);
+                    tw = (float)(wt[i] * sin(th));
+                } else if (ell) {
+                    ta = (float)(u * u);
+                } else if (ppd) {
+                    ta = (float)sin(th);
+                    tb = (float)cos(th);
+                    tc = (float)u;
+                    td = (float)sqrt(1.0 - u * u);
                 } else {
-                    const double u = z[i] * 0.5 + 0.5;
-                    a.push_back((float)(u * u));
-                    b.push_back(0.0f);
-                    w.push_back((float)wt[i]);
+                    ta = (float)(u * u);
+                    tc = (float)(sin(th) * sin(th));
                 }
+                a.push_back(ta);
+                b.push_back(tb);
+                w.push_back(tw);
+                c.push_back(tc);
+                d.push_back(td);
             }
         }
     } else {
@@ -330,12 +343,14 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
         a.push_back(0.0f);
         b.push_back(0.0f);
         w.push_back(0.0f);
+        c.push_back(0.0f);
+        d.push_back(0.0f);
     }
     m->ntab = (int)a.size();
-    hipError_t e = hipMalloc((void **)&m->tab, sizeof(float) * 3 * m->ntab);
-    if (e == hipSuccess) e = hipMemcpy(m->tab, a.data(), sizeof(float) * m->ntab, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(m->tab + m->ntab, b.data(), sizeof(float) * m->ntab, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(m->tab + 2 * m->ntab, w.data(), sizeof(float) * m->ntab, hipMemcpyHostToDevice);
+    hipError_t e = hipMalloc((void **)&m->tab, sizeof(float) * 5 * m->ntab);
+    const std::vector<float> *parts[5] = {&a, &b, &w, &c, &d};
+    for (int k = 0; k < 5 && e == hipSuccess; k++)
+        e = hipMemcpy(m->tab + k * m->ntab, parts[k]->data(), sizeof(float) * m->ntab, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (m->tab) (void)hipFree(m->tab);
         delete m;
@@ -344,6 +359,8 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
     m->quad.a = m->tab;
     m->quad.b = m->tab + m->ntab;
     m->quad.w = m->tab + 2 * m->ntab;
+    m->quad.c = m->tab + 3 * m->ntab;
+    m->quad.d = m->tab + 4 * m->ntab;
     m->quad.n = quad_n;
     for (int i = 0; i < 4; i++) m->quad.off[i] = i < (int)offs.size() ? offs[i] : 0;
     *out = m;
@@ -448,7 +465,9 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     case 1: rc = launch<sas::Cylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     case 2: rc = launch<sas::Ellipsoid>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     case 3: rc = launch<sas::CoreShellCylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    default: rc = launch<sas::CoreShellSphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 4: rc = launch<sas::CoreShellSphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 6: rc = launch<sas::Parallelepiped>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    default: rc = launch<sas::TriaxialEllipsoid>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     }
     if (rc) return rc;
     return finish_call(k, rbytes, result);

@@ -16,7 +16,8 @@ path depends on (sasmodels/modelinfo.py):
 
 The parameter tables are the reference's own (sphere.py:66-72,
 cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
-core_shell_sphere.py:78-82, hardsphere.py:84-88), with the attributes the
+core_shell_sphere.py:78-82, parallelepiped.py:216-231,
+triaxial_ellipsoid.py:143-158, hardsphere.py:84-88), with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
 orientation mode, ``structure_factor``).
 """
@@ -252,6 +253,33 @@ _MODEL_TABLE = {
         ["sld_solvent", "1e-6/Ang^2", 3.0, [-inf, inf], "sld", "Solvent scattering length density"],
     ], dict(radius_effective_modes=["outer radius", "core radius"], xy_mode="qa",
             has_quadrature=False)),
+    "parallelepiped": (6, [
+        ["sld", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Parallelepiped scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["length_a", "Ang", 35, [0, inf], "volume", "Shorter side of the parallelepiped"],
+        ["length_b", "Ang", 75, [0, inf], "volume", "Second side of the parallelepiped"],
+        ["length_c", "Ang", 400, [0, inf], "volume", "Larger side of the parallelepiped"],
+        ["theta", "degrees", 60, [-180, 180], "orientation", "c axis to beam angle"],
+        ["phi", "degrees", 60, [-360, 360], "orientation", "rotation about beam"],
+        ["psi", "degrees", 60, [-360, 360], "orientation", "rotation about c axis"],
+    ], dict(radius_effective_modes=[
+                "equivalent cylinder excluded volume", "equivalent volume sphere",
+                "half length_a", "half length_b", "half length_c",
+                "equivalent circular cross-section", "half ab diagonal", "half diagonal"],
+            xy_mode="qabc", has_quadrature=True)),
+    "triaxial_ellipsoid": (7, [
+        ["sld", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Ellipsoid scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["radius_equat_minor", "Ang", 20, [0, inf], "volume", "Minor equatorial radius, Ra"],
+        ["radius_equat_major", "Ang", 400, [0, inf], "volume", "Major equatorial radius, Rb"],
+        ["radius_polar", "Ang", 10, [0, inf], "volume", "Polar radius, Rc"],
+        ["theta", "degrees", 60, [-180, 180], "orientation", "polar axis to beam angle"],
+        ["phi", "degrees", 60, [-360, 360], "orientation", "rotation about beam"],
+        ["psi", "degrees", 60, [-360, 360], "orientation", "rotation about polar axis"],
+    ], dict(radius_effective_modes=[
+                "equivalent biaxial ellipsoid average curvature", "equivalent volume sphere",
+                "min radius", "max radius"],
+            xy_mode="qabc", has_quadrature=True)),
     # structure factor: Iq only (no F), no volume, nothing polydisperse;
     # evaluated in FP64 (the reference marks it double precision only)
     "hardsphere": (5, [

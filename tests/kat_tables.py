@@ -97,8 +97,26 @@ CORE_SHELL_SPHERE = [
       "sld_solvent": 3.0, "background": 0.0}, 0.4, 0.000698838),
 ]
 
+# parallelepiped.py:253-259
+_ppd_qx, _ppd_qy = 0.2*cos(pi/6.), 0.2*sin(pi/6.)
+PARALLELEPIPED = [
+    ({}, 0.2, 0.17758004974),
+    ({}, [0.2], [0.17758004974]),
+    ({"theta": 10.0, "phi": 20.0}, (_ppd_qx, _ppd_qy), 0.0089517140475),
+    ({"theta": 10.0, "phi": 20.0}, [(_ppd_qx, _ppd_qy)], [0.0089517140475]),
+]
+
+# triaxial_ellipsoid.py:181-190
+_tri_qx, _tri_qy = 0.1*cos(pi/6.0), 0.1*sin(pi/6.0)
+TRIAXIAL_ELLIPSOID = [
+    ({}, 0.05, 24.8839548033),
+    ({"theta": 80., "phi": 10.}, (_tri_qx, _tri_qy), 166.712060266),
+]
+
 MODEL_KATS = {
     "sphere": SPHERE,
+    "parallelepiped": PARALLELEPIPED,
+    "triaxial_ellipsoid": TRIAXIAL_ELLIPSOID,
     "cylinder": CYLINDER,
     "ellipsoid": ELLIPSOID,
     "core_shell_cylinder": CORE_SHELL_CYLINDER,

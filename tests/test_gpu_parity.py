@@ -45,6 +45,12 @@ RANDOM_1D = {
                                         sld_core=r.uniform(-1, 7), sld_shell=r.uniform(-1, 7),
                                         sld_solvent=r.uniform(-1, 7), radius_pd=0.15, radius_pd_n=9,
                                         thickness_pd=0.1, thickness_pd_n=5),
+    "parallelepiped": lambda r: dict(length_a=r.uniform(10, 100), length_b=r.uniform(20, 200),
+                                     length_c=r.uniform(50, 600), length_a_pd=0.1,
+                                     length_a_pd_n=int(r.integers(1, 6)), length_c_pd=0.1, length_c_pd_n=3),
+    "triaxial_ellipsoid": lambda r: dict(radius_equat_minor=r.uniform(10, 200), radius_equat_major=r.uniform(10, 200),
+                                         radius_polar=r.uniform(10, 200), radius_polar_pd=0.1,
+                                         radius_polar_pd_n=int(r.integers(1, 6))),
 }
 
 
@@ -76,6 +82,14 @@ RANDOM_2D = {
                                           sld_core=r.uniform(-1, 7), sld_shell=r.uniform(-1, 7)),
     "core_shell_sphere": lambda r: dict(radius=r.uniform(10, 100), thickness=r.uniform(2, 30),
                                         radius_pd=0.1, radius_pd_n=5),
+    "parallelepiped": lambda r: dict(length_a=r.uniform(10, 60), length_b=r.uniform(20, 120),
+                                     length_c=r.uniform(50, 300), theta=r.uniform(0, 90), phi=r.uniform(0, 180),
+                                     psi=r.uniform(0, 180), theta_pd=8, theta_pd_n=3, phi_pd=10, phi_pd_n=3,
+                                     psi_pd=15, psi_pd_n=3, length_a_pd=0.1, length_a_pd_n=3),
+    "triaxial_ellipsoid": lambda r: dict(radius_equat_minor=r.uniform(10, 60), radius_equat_major=r.uniform(50, 200),
+                                         radius_polar=r.uniform(10, 100), theta=r.uniform(0, 90),
+                                         phi=r.uniform(0, 180), psi=r.uniform(0, 180), psi_pd=20, psi_pd_n=5,
+                                         theta_pd=5, theta_pd_n=3),
 }
 
 
@@ -111,6 +125,12 @@ MAGNETIC = {
     "core_shell_cylinder": dict(radius=30, thickness=10, length=150, theta=60, phi=30, sld_core_M0=1,
                                 sld_shell_M0=2, sld_shell_mtheta=20, up_frac_i=0.9, up_frac_f=0.4,
                                 up_theta=10, up_phi=30),
+    "parallelepiped": dict(length_a=30, length_b=60, length_c=200, theta=40, phi=20, psi=70, psi_pd=10,
+                           psi_pd_n=3, sld_M0=2.5, sld_mtheta=30, sld_mphi=60, up_frac_i=0.2,
+                           up_frac_f=0.6, up_theta=30),
+    "triaxial_ellipsoid": dict(radius_equat_minor=25, radius_equat_major=90, radius_polar=50, theta=20,
+                               phi=50, psi=10, sld_solvent_M0=1.5, sld_solvent_mphi=90, up_frac_i=0.6,
+                               up_frac_f=0.1, up_theta=60, up_phi=20),
 }
 
 

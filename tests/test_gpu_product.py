@@ -241,3 +241,21 @@ def test_product_batch_2d_and_smeared():
         kern.release()
         assert masked_relerr(got, want) <= RTOL
     batch.release()
+
+
+@pytest.mark.parametrize("p_name,pars", [
+    ("parallelepiped", dict(length_a=30.0, length_b=60.0, length_c=200.0, length_a_pd=0.1, length_a_pd_n=5,
+                            volfraction=0.15, structure_factor_mode=1, radius_effective_mode=1)),
+    ("parallelepiped", dict(volfraction=0.3, structure_factor_mode=0, radius_effective_mode=8)),
+    ("triaxial_ellipsoid", dict(radius_equat_minor=20.0, radius_equat_major=60.0, radius_polar=120.0,
+                                radius_polar_pd=0.1, radius_polar_pd_n=5, volfraction=0.2,
+                                structure_factor_mode=1, radius_effective_mode=1)),
+])
+def test_product_qabc_shapes(p_name, pars):
+    q = np.logspace(-3, np.log10(0.5), 80)
+    print("product", p_name, pars, flush=True)
+    model = core.load_model(p_name + "@hardsphere")
+    kern = model.make_kernel([q])
+    got = direct_model.call_kernel(kern, pars)
+    kern.release()
+    assert masked_relerr(got, oracle_product(p_name, q, pars)[0]) <= RTOL

@@ -23,7 +23,8 @@ from oracle import oracle
 from sasmodels_amd import direct_model, generate, modelinfo
 
 P = ct.POINTER(ct.c_double)
-NPARS = {"sphere": 3, "cylinder": 6, "ellipsoid": 6, "core_shell_cylinder": 8, "core_shell_sphere": 5}
+NPARS = {"sphere": 3, "cylinder": 6, "ellipsoid": 6, "core_shell_cylinder": 8, "core_shell_sphere": 5,
+         "parallelepiped": 8, "triaxial_ellipsoid": 8}
 
 
 def _ptr(a):
@@ -56,7 +57,7 @@ def test_special_functions_bit_exact(fn, refname, lib):
 
 def _rand_pars(model, rng):
     p = rng.uniform(1, 400, NPARS[model])
-    if model in ("sphere", "cylinder", "ellipsoid"):
+    if model in ("sphere", "cylinder", "ellipsoid", "parallelepiped", "triaxial_ellipsoid"):
         p[0:2] = rng.uniform(-2, 8, 2)
     elif model == "core_shell_cylinder":
         p[0:3] = rng.uniform(-2, 8, 3)
@@ -72,6 +73,9 @@ def _rand_pars(model, rng):
     ("core_shell_cylinder", "ref_core_shell_cylinder", 0),
     ("core_shell_cylinder", "ref_core_shell_cylinder_g76", 76),
     ("core_shell_sphere", "ref_core_shell_sphere", 0),
+    ("parallelepiped", "ref_parallelepiped", 0), ("parallelepiped", "ref_parallelepiped_g76", 76),
+    ("triaxial_ellipsoid", "ref_triaxial_ellipsoid", 0),
+    ("triaxial_ellipsoid", "ref_triaxial_ellipsoid_g76", 76),
 ])
 def test_form_factors_match_reference_sources(model, refname, quad):
     ref = _need_ref(refname)
@@ -89,12 +93,16 @@ def test_form_factors_match_reference_sources(model, refname, quad):
         worst = max(worst, abs(a2.value - b2.value)/abs(b2.value),
                     abs(a1.value - b1.value)/math.sqrt(abs(b2.value))*1e-2)
         assert L.sasoracle_form_volume(mid, _ptr(p)) == ref.ref_form_volume(_ptr(p))
-        for mode in range(1, 8):
+        for mode in range(1, 9):
             assert L.sasoracle_radius_effective(mid, mode, _ptr(p)) == ref.ref_reff(mode, _ptr(p))
         if hasattr(ref, "ref_iqac"):
             qab, qc = rng.uniform(0, 0.5, 2)
             u, v = L.sasoracle_iqac(mid, qab, qc, _ptr(p)), ref.ref_iqac(qab, qc, _ptr(p))
             worst = max(worst, abs(u - v)/abs(v) if v else abs(u))
+        if hasattr(ref, "ref_iqabc"):
+            qa, qb, qc = rng.uniform(-0.5, 0.5, 3)
+            u, v = L.sasoracle_iqabc(mid, qa, qb, qc, _ptr(p)), ref.ref_iqabc(qa, qb, qc, _ptr(p))
+            assert u == v, (u, v)
     # bit-exact without quadrature; the reference's printed Gauss tables
     # (15-16 digits, 500/5000-point weights to ~4e-10) bound the rest
     limit = 0.0 if model in ("sphere", "core_shell_sphere") else 1e-9
@@ -232,3 +240,51 @@ def test_ellipsoid_orientation_average_against_scipy_quad():
             return (3*special.spherical_jn(1, x)/x)**2
         num, _ = quad(f, 0, 1, epsabs=0, epsrel=1e-13)
         assert abs(f2/(1e-4*V**2*num) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("model", ["parallelepiped", "triaxial_ellipsoid"])
+def test_nested_orientation_average_against_dblquad(model):
+    """The 1D <F^2> of the qabc shapes is the orientation average of the 2D
+    amplitude: (2/pi) int_0^pi/2 int_0^pi/2 f^2(q n) sin(theta) dtheta dphi,
+    n = (sin t cos p, sin t sin p, cos t), by scipy's adaptive dblquad."""
+    from scipy import integrate
+    L = oracle.lib()
+    mid = oracle.MODEL_IDS[model]
+    p = np.ascontiguousarray([3.0, 1.0, 35.0, 75.0, 180.0, 0, 0, 0])
+    s = (p[0] - p[1])*L.sasoracle_form_volume(mid, _ptr(p))
+
+    def amp(q, t, f):
+        na, nb, nc = math.sin(t)*math.cos(f), math.sin(t)*math.sin(f), math.cos(t)
+        if model == "parallelepiped":
+            sinc = lambda x: math.sin(x)/x if x else 1.0
+            return sinc(0.5*q*p[2]*na)*sinc(0.5*q*p[3]*nb)*sinc(0.5*q*p[4]*nc)
+        x = q*math.sqrt((p[2]*na)**2 + (p[3]*nb)**2 + (p[4]*nc)**2)
+        return 3*(math.sin(x) - x*math.cos(x))/x**3
+
+    for q in (0.005, 0.03, 0.08):
+        f1, f2 = ct.c_double(), ct.c_double()
+        L.sasoracle_fq(mid, 150, q, _ptr(p), ct.byref(f1), ct.byref(f2))
+        avg2 = integrate.dblquad(lambda t, f: amp(q, t, f)**2*math.sin(t), 0, math.pi/2, 0, math.pi/2,
+                                 epsabs=0, epsrel=1e-11)[0]*2/math.pi
+        avg1 = integrate.dblquad(lambda t, f: amp(q, t, f)*math.sin(t), 0, math.pi/2, 0, math.pi/2,
+                                 epsabs=0, epsrel=1e-11)[0]*2/math.pi
+        assert f2.value == pytest.approx(1e-4*s*s*avg2, rel=1e-8), q
+        assert f1.value == pytest.approx(1e-2*s*avg1, rel=1e-8), q
+
+
+def test_qabc_rotation_is_orthonormal_and_reduces_to_qac():
+    """The (theta, phi, psi) view and jitter rotation: its first two columns
+    are orthonormal, psi = dpsi = 0 gives the qac rotation's third row, and
+    the view alone maps the beam-plane direction of the particle's c axis."""
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    r = np.zeros(6)
+    for _ in range(200):
+        th, ph, ps, dth, dph, dps = rng.uniform(-180, 180, 6)
+        L.sasoracle_view_jitter_abc(th, ph, ps, dth, dph, dps, _ptr(r))
+        c1, c2 = r[0::2], r[1::2]
+        assert abs(c1 @ c1 - 1) < 1e-13 and abs(c2 @ c2 - 1) < 1e-13 and abs(c1 @ c2) < 1e-13
+        L.sasoracle_view_jitter_abc(th, ph, 0.0, dth, dph, 0.0, _ptr(r))
+        r31, r32 = ct.c_double(), ct.c_double()
+        L.sasoracle_view_jitter_row(th, ph, dth, dph, ct.byref(r31), ct.byref(r32))
+        assert r[4] == pytest.approx(r31.value, abs=1e-15) and r[5] == pytest.approx(r32.value, abs=1e-15)
