@@ -1,8 +1,8 @@
 /*
  * saship.h -- C ABI of libsaship.so, the MI355X (gfx950) I(q) engine.
  *
- * The library evaluates the sasmodels polydispersity-mesh kernel for five
- * form factors on one GPU.  Callers pass the same buffers the reference's
+ * The library evaluates the sasmodels polydispersity-mesh kernel for ten
+ * form factors and the hard-sphere structure factor on one GPU.  Callers pass the same buffers the reference's
  * compiled kernels take:
  *
  *   void <model>_{Iq|Iqxy|Imagnetic}(int32 nq, int32 pd_start, int32 pd_stop,
@@ -48,6 +48,16 @@ extern "C" {
  * four trailing entries only (max_pd 0); Iq and Iqxy (q = |(qx, qy)|) give
  * result[i] = S(q_i) and sums (1, 1, 1, 0); evaluated in FP64 */
 #define SASHIP_HARDSPHERE 5
+/* shapes with three orientation angles (theta, phi, psi; xy_mode "qabc"):
+ * 1D kernels integrate the orientation with nested rules (adaptive as
+ * adaptive.c with outer_n limits, or the fixed size given) */
+#define SASHIP_PARALLELEPIPED 6
+#define SASHIP_TRIAXIAL_ELLIPSOID 7
+/* paracrystals: FP64, Gauss-150 product rule over (theta, phi) unless a
+ * size is given; no <F>: the 1D result has one output per q (nq + 4) */
+#define SASHIP_BCC_PARACRYSTAL 8
+#define SASHIP_FCC_PARACRYSTAL 9
+#define SASHIP_SC_PARACRYSTAL 10
 
 /* kernel kinds: the three kernels generated per model (generate.py:1144-1180) */
 #define SASHIP_KIND_IQ 0        /* <model>_Iq: 1D, <F> and <F^2> */
@@ -98,7 +108,8 @@ int saship_kernel_create(saship_model *model, int32_t nq, const double *q,
 /* One evaluation over mesh steps [pd_start, pd_stop).  details is the
  * CallDetails int32 buffer (4*max_pd + 4 entries, sasmodels/details.py:73-88),
  * values the packed parameter vector (sasmodels/details.py:225-254).
- * result has nout*nq + 4 doubles, nout = 2 for SASHIP_KIND_IQ, else 1.
+ * result has nout*nq + 4 doubles, nout = 2 for SASHIP_KIND_IQ of a model with
+ * <F>, else 1 (2D kinds, the paracrystals, the structure factor).
  * Replaces Kernel._call_kernel (sasmodels/kernel.py:198-205) as the DLL,
  * OpenCL and CUDA backends implement it (kerneldll.py:449-475,
  * kernelcl.py:599-654, kernelcuda.py:471-518). */

@@ -22,9 +22,12 @@ LIBPATH = os.path.join(HERE, "liboracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 KIND_IQ, KIND_IQXY, KIND_IMAGNETIC = 0, 1, 2
+#: models whose 1D kernel returns <F^2> only (one output per q)
+NO_FQ = (8, 9, 10)
 MODEL_IDS = {"sphere": 0, "cylinder": 1, "ellipsoid": 2,
              "core_shell_cylinder": 3, "core_shell_sphere": 4,
-             "parallelepiped": 6, "triaxial_ellipsoid": 7}
+             "parallelepiped": 6, "triaxial_ellipsoid": 7,
+             "bcc_paracrystal": 8, "fcc_paracrystal": 9, "sc_paracrystal": 10}
 
 _lib = None
 _D = ct.c_double
@@ -90,7 +93,7 @@ def raw_call(model, kind, quad_n, q, details, values, cutoff=0.0, reff_mode=0,
     nq = q.shape[0]
     details = np.ascontiguousarray(details, "i4")
     values = np.ascontiguousarray(values, "d")
-    nout = 2 if kind == KIND_IQ else 1
+    nout = 2 if kind == KIND_IQ and mid not in NO_FQ else 1
     if result is None:
         result = np.zeros(nout*nq + 4, "d")
     stop = int(details[-4]) if pd_stop is None else int(pd_stop)
@@ -149,7 +152,7 @@ class OracleKernel:
         return result
 
     def Fq(self, call_details, values, cutoff, magnetic, radius_effective_mode=0):
-        nout = 2 if self.dim == "1d" else 1
+        nout = 2 if self.dim == "1d" and getattr(self.info, "have_Fq", True) else 1
         res = self.raw(call_details, values, cutoff, magnetic, radius_effective_mode)
         return normalise(res, self.nq, nout)
 
@@ -192,6 +195,14 @@ def ref_lib(name):
     if hasattr(L, "ref_sq"):
         L.ref_sq.restype = _D
         L.ref_sq.argtypes = [_D, _D, _D]
+        return L
+    if hasattr(L, "ref_iq"):
+        L.ref_iq.restype = _D
+        L.ref_iq.argtypes = [_D, ct.c_void_p]
+        L.ref_iqabc.restype = _D
+        L.ref_iqabc.argtypes = [_D, _D, _D, ct.c_void_p]
+        L.ref_form_volume.restype = _D
+        L.ref_form_volume.argtypes = [ct.c_void_p]
         return L
     L.ref_fq.argtypes = [_D, ct.c_void_p, _P, _P]
     L.ref_form_volume.restype = _D

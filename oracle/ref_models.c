@@ -11,9 +11,10 @@
  * vector to the reference's static functions so that ctypes can call them.
  *
  * One shared object is built per (model, quadrature) pair:
- *   -DREF_MODEL=<1..8>  sphere, cylinder, ellipsoid, core_shell_cylinder,
+ *   -DREF_MODEL=<1..11> sphere, cylinder, ellipsoid, core_shell_cylinder,
  *                       core_shell_sphere, hardsphere (structure factor),
- *                       parallelepiped, triaxial_ellipsoid
+ *                       parallelepiped, triaxial_ellipsoid, bcc, fcc and
+ *                       sc paracrystal (with lib/gauss150.c)
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -91,8 +92,25 @@ double ref_sq(double q, double radius_effective, double volfraction) { return Iq
 #  define NPAR_IQABC(a, b, c, p) Iqabc(a, b, c, p[0], p[1], p[2], p[3], p[4])
 #  define NPAR_VOLUME(p) form_volume(p[2], p[3], p[4])
 #  define NPAR_REFF(m, p) radius_effective(m, p[2], p[3], p[4])
+#elif REF_MODEL >= 9 && REF_MODEL <= 11
+#  include "lib/sas_3j1x_x.c"
+#  include "lib/gauss150.c"
+#  include "lib/sphere_form.c"
+#  if REF_MODEL == 9
+#    include "bcc_paracrystal.c"
+#  elif REF_MODEL == 10
+#    include "fcc_paracrystal.c"
+#  else
+#    include "sc_paracrystal.c"
+#  endif
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4]); }
+double ref_iqabc(double qa, double qb, double qc, const double *p)
+{
+    return Iqabc(qa, qb, qc, p[0], p[1], p[2], p[3], p[4]);
+}
+double ref_form_volume(const double *p) { return form_volume(p[2]); }
 #else
-#  error "REF_MODEL must be 1..8"
+#  error "REF_MODEL must be 1..11"
 #endif
 
 #ifdef NPAR_CALL_FQ

@@ -24,7 +24,9 @@
  *   - models: sphere.c:1-20, cylinder.c:1-102, ellipsoid.c:1-105,
  *     core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30 with
  *     lib/core_shell.c:8-30, parallelepiped.c:1-150,
- *     triaxial_ellipsoid.c:1-160 (nested rules adaptive.c:11243-11298);
+ *     triaxial_ellipsoid.c:1-160 (nested rules adaptive.c:11243-11298),
+ *     bcc_paracrystal.c, fcc_paracrystal.c, sc_paracrystal.c with
+ *     lib/sphere_form.c (Gauss-150 product rule, lib/gauss150.c);
  *     view/jitter rotation with psi qabc_rotation/qabc_apply
  *     kernel_iq.c:251-310
  *   - hard-sphere structure factor hardsphere.c:1-90
@@ -49,6 +51,7 @@
 #define O_PI_180 0.017453292519943295
 #define O_4PI_3 4.18879020478639
 #define O_SQRT1_PI 0.56418958354775628
+#define O_SQRT1_2 0.70710678118654746
 
 /* ------------------------------------------------------------------ */
 /* Gauss-Legendre nodes: Newton iteration on P_n in long double.       */
@@ -145,6 +148,7 @@ static void quad_prime(int quad_n)
     if (quad_n > 0) { gl_table(quad_n, &z, &w); return; }
     gl_table(20, &z, &w); gl_table(76, &z, &w);
     gl_table(500, &z, &w); gl_table(5000, &z, &w);
+    gl_table(150, &z, &w);   /* the paracrystals' default product rule */
 }
 
 /* ------------------------------------------------------------------ */
@@ -600,6 +604,79 @@ static double tri_iqabc(double qa, double qb, double qc, const double *p)
     return 1.0e-4 * sq(tri_vol(p) * (p[0] - p[1]) * f);
 }
 
+/* ---- paracrystals (bcc/fcc/sc_paracrystal.c with lib/sphere_form.c):
+ *      p = dnn, d_factor, radius, sld, sld_solvent, theta, phi, psi ---- */
+static double pc_zq(int lat, double qa, double qb, double qc, double dnn, double d_factor)
+{
+    double a1, a2, a3, d_a;
+    if (lat == 0) {        /* bcc */
+        a1 = (-qa + qb + qc) / 2.0; a2 = (+qa - qb + qc) / 2.0; a3 = (+qa + qb - qc) / 2.0;
+        d_a = dnn / sqrt(0.75);
+    } else if (lat == 1) { /* fcc */
+        a1 = (qa + qb) / 2.0; a2 = (qa + qc) / 2.0; a3 = (qb + qc) / 2.0;
+        d_a = dnn * sqrt(2);
+    } else {               /* sc */
+        a1 = qa; a2 = qb; a3 = qc;
+        d_a = dnn;
+    }
+    const double arg = -0.5 * sq(dnn * d_factor) * (a1 * a1 + a2 * a2 + a3 * a3);
+    const double exp_arg = exp(arg);
+    return -cb(expm1(2.0 * arg))
+        / (((exp_arg - 2.0 * cos(d_a * a1)) * exp_arg + 1.0)
+           * ((exp_arg - 2.0 * cos(d_a * a2)) * exp_arg + 1.0)
+           * ((exp_arg - 2.0 * cos(d_a * a3)) * exp_arg + 1.0));
+}
+static double pc_vfrac(int lat, double radius, double dnn)
+{
+    if (lat == 0) return 2.0 * (O_4PI_3 * cb(sqrt(0.75) * radius / dnn));
+    if (lat == 1) return 4.0 * (O_4PI_3 * cb(O_SQRT1_2 * radius / dnn));
+    return O_4PI_3 * cb(radius / dnn);
+}
+static double pc_sphere_form(double q, const double *p)
+{
+    const double fq = O_4PI_3 * cb(p[2]) * o_sph_j1c(q * p[2]);
+    return 1.0e-4 * sq((p[3] - p[4]) * fq);
+}
+static double pc_vol(const double *p) { return O_4PI_3 * cb(p[2]); }
+static double pc_reff(int mode, const double *p) { (void)mode; (void)p; return 0.0; }
+static double pc_iq(int lat, double q, const double *p, int quad_n)
+{
+    const double *z, *w;
+    const int n = gl_table(quad_n > 0 ? quad_n : 150, &z, &w);
+    const double tm = lat == 2 ? O_PI_4 : O_PI / 2, tb = tm;
+    const double pm = lat == 2 ? O_PI_4 : O_PI, pb = pm;
+    double outer = 0.0;
+    for (int i = 0; i < n; i++) {
+        double inner = 0.0;
+        const double theta = z[i] * tm + tb;
+        const double st = sin(theta), ct = cos(theta);
+        const double qc = q * ct, qab = q * st;
+        for (int j = 0; j < n; j++) {
+            const double phi = z[j] * pm + pb;
+            const double sp = sin(phi), cp = cos(phi);
+            inner += w[j] * pc_zq(lat, qab * cp, qab * sp, qc, p[0], p[1]);
+        }
+        inner *= pm;
+        outer += w[i] * inner * st;
+    }
+    outer *= tm;
+    const double zq = lat == 2 ? outer / (O_PI / 2) : outer / (4.0 * O_PI);
+    return pc_vfrac(lat, p[2], p[0]) * pc_sphere_form(q, p) * zq;
+}
+static double pc_iqabc(int lat, double qa, double qb, double qc, const double *p)
+{
+    const double q = sqrt(qa * qa + qb * qb + qc * qc);
+    return pc_vfrac(lat, p[2], p[0]) * pc_sphere_form(q, p) * pc_zq(lat, qa, qb, qc, p[0], p[1]);
+}
+#define PC_FNS(L, NAME) \
+    static void NAME##_fq(double q, const double *p, int quad_n, double *f1, double *f2) \
+    { *f1 = 0.0; *f2 = pc_iq(L, q, p, quad_n); } \
+    static double NAME##_iqabc(double qa, double qb, double qc, const double *p) \
+    { return pc_iqabc(L, qa, qb, qc, p); }
+PC_FNS(0, bcc)
+PC_FNS(1, fcc)
+PC_FNS(2, scp)
+
 /* ------------------------------------------------------------------ */
 /* Model table                                                         */
 /* ------------------------------------------------------------------ */
@@ -612,23 +689,28 @@ typedef struct {
     void (*fq)(double q, const double *p, int quad_n, double *f1, double *f2);
     double (*iqac)(double qab, double qc, const double *p);
     double (*iqabc)(double qa, double qb, double qc, const double *p);
+    int no_fq;   /* Iq-only model: one output per q in 1D too */
 } OModel;
 
 static int always_valid(const double *p) { (void)p; return 1; }
 
 /* model ids: 0 sphere, 1 cylinder, 2 ellipsoid, 3 core_shell_cylinder,
- * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid (the ids
- * libsaship uses; 5, the hard-sphere structure factor, has its own entry) */
-#define N_MODELS 8
+ * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid, 8-10 bcc,
+ * fcc, sc paracrystal (the ids libsaship uses; 5, the hard-sphere structure
+ * factor, has its own entry) */
+#define N_MODELS 11
 static const OModel MODELS[N_MODELS] = {
-    {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0},
-    {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0},
-    {6, 2, 4, 1, {0, 1, -1}, always_valid, ell_vol, ell_reff, ell_fq, ell_iqac, 0},
-    {8, 3, 5, 1, {0, 1, 2}, always_valid, csc_vol, csc_reff, csc_fq, csc_iqac, 0},
-    {5, 3, 2, 0, {2, 3, 4}, always_valid, css_vol, css_reff, css_fq, 0, 0},
-    {0, 0, 0, 0, {-1, -1, -1}, 0, 0, 0, 0, 0, 0},
-    {8, 2, 5, 1, {0, 1, -1}, always_valid, ppd_vol, ppd_reff, ppd_fq, 0, ppd_iqabc},
-    {8, 2, 5, 1, {0, 1, -1}, always_valid, tri_vol, tri_reff, tri_fq, 0, tri_iqabc},
+    {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0, 0},
+    {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0, 0},
+    {6, 2, 4, 1, {0, 1, -1}, always_valid, ell_vol, ell_reff, ell_fq, ell_iqac, 0, 0},
+    {8, 3, 5, 1, {0, 1, 2}, always_valid, csc_vol, csc_reff, csc_fq, csc_iqac, 0, 0},
+    {5, 3, 2, 0, {2, 3, 4}, always_valid, css_vol, css_reff, css_fq, 0, 0, 0},
+    {0, 0, 0, 0, {-1, -1, -1}, 0, 0, 0, 0, 0, 0, 0},
+    {8, 2, 5, 1, {0, 1, -1}, always_valid, ppd_vol, ppd_reff, ppd_fq, 0, ppd_iqabc, 0},
+    {8, 2, 5, 1, {0, 1, -1}, always_valid, tri_vol, tri_reff, tri_fq, 0, tri_iqabc, 0},
+    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, bcc_fq, 0, bcc_iqabc, 1},
+    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, fcc_fq, 0, fcc_iqabc, 1},
+    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, scp_fq, 0, scp_iqabc, 1},
 };
 static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npars > 0; }
 
@@ -734,7 +816,7 @@ void sasoracle_set_threads(int n) { n_threads = n > 0 ? n : 1; }
  * with the model, the kernel kind (0: <model>_Iq, 1: _Iqxy, 2: _Imagnetic) and
  * the quadrature (0 adaptive, else fixed Gauss-n) named explicitly.
  * details: int32[4*max_pd+4]; values: f64; q: nq (1D) or 2*nq interleaved (2D);
- * result: nout*nq+4 with nout = 2 for the 1D kernel (all five models have Fq).
+ * result: nout*nq+4 with nout = 2 for the 1D kernel of models with Fq, else 1.
  * Returns 0, or -1 for a bad model/kind, -2 for a bad quadrature size.
  */
 int sasoracle_kernel(int model_id, int kind, int quad_n,
@@ -753,7 +835,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
     const int32_t theta_par = details[4 * maxpd + 3];
     const double *pd_value = values + (2 + np + (m->nmag ? 4 + 3 * m->nmag : 0));
     const double *pd_weight = pd_value + num_weights;
-    const int nout = (kind == 0) ? 2 : 1;
+    const int nout = (kind == 0 && !m->no_fq) ? 2 : 1;
     const int oriented2d = (kind != 0) && m->oriented;
     const int abc = m->iqabc != 0;
     quad_prime(quad_n);
@@ -835,8 +917,8 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
             if (kind == 0) {
                 double f1, f2;
                 m->fq(q[iq], p, quad_n, &f1, &f2);
-                result[2 * iq + 0] += weight * f2;
-                result[2 * iq + 1] += weight * f1;
+                result[nout * iq + 0] += weight * f2;
+                if (nout == 2) result[2 * iq + 1] += weight * f1;
                 continue;
             }
             const double qx = q[2 * iq], qy = q[2 * iq + 1];

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
         const int c1 = min(npts, c0 + a.pts_per_item);
         const float4 *rec = a.recs + a.roff[s] * RV;
         KSum a1, a2;
+        double d1 = 0.0, d2 = 0.0;    // FP64 models
         bool shared = false;
         if constexpr (FIXED && M::SHARED_1D) {
             // every point of the chunk has the same lengths (a radius-only
@@ -92,18 +93,26 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
             shared = c1 - c0 > 1 && M::shares_lengths(rf, 4 * RV, c0, c1);
             if (shared) M::eval1d_shared(q, rf, 4 * RV, c0, c1, a.quad, a1, a2);
         }
+        const double qd = M::FP64 && iq < nq ? a.q64[q0 + iq] : 0.0;
         for (int j = c0; j < c1 && !shared; j++) {
             float r[4 * RV];
             load_rec(rec + (size_t)j * RV, r, RV);
-            float f1, f2;
-            M::template eval1d<FIXED>(q, r, a.quad, f1, f2);
-            a1.add(f1);
-            a2.add(f2);
+            if constexpr (M::FP64) {
+                double f1, f2;
+                M::eval1d_d(qd, r, a.quad, f1, f2);
+                d1 += f1;
+                d2 += f2;
+            } else {
+                float f1, f2;
+                M::template eval1d<FIXED>(q, r, a.quad, f1, f2);
+                a1.add(f1);
+                a2.add(f2);
+            }
         }
         if (iq < nq) {
             double *dst = partial + a.poff[s] + (size_t)chunk * 2 * nq;
-            dst[iq] = a2.value();
-            dst[nq + iq] = a1.value();
+            dst[iq] = M::FP64 ? d2 : a2.value();
+            dst[nq + iq] = M::FP64 ? d1 : a1.value();
         }
     }
 }
@@ -206,7 +215,7 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
 // weight sums, and form I(q) = scale * <F^2>/<V> + background with the
 // guards of Kernel.Fq (total weight 0 -> 1, shell volume 0 -> 1).
 __global__ __launch_bounds__(256) void k_batch_final(
-    int nout, const int64_t *__restrict__ qoff, const double *__restrict__ values,
+    int nout, int nres, const int64_t *__restrict__ qoff, const double *__restrict__ values,
     const int64_t *__restrict__ voff, const int64_t *__restrict__ poff,
     const int64_t *__restrict__ resoff, const int *__restrict__ nchunk,
     const double *__restrict__ sums, const double *__restrict__ partial,
@@ -244,8 +253,8 @@ __global__ __launch_bounds__(256) void k_batch_final(
             if (nout == 2) f1 += part[(size_t)c * nout * nq + nq + e];
         }
         if (result) {
-            result[resoff[s] + (size_t)e * nout] = f2;
-            if (nout == 2) result[resoff[s] + (size_t)e * nout + 1] = f1;
+            result[resoff[s] + (size_t)e * nres] = f2;
+            if (nres == 2) result[resoff[s] + (size_t)e * nres + 1] = f1;
         }
         if (!iq_out) continue;
         double v = f2 / total;
@@ -259,7 +268,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
         iq_out[q0 + e] = cscale * v + background;
     }
     if (result && blockIdx.y == 0 && threadIdx.x < 4)
-        result[resoff[s] + (size_t)nout * nq + threadIdx.x] = sm[threadIdx.x];
+        result[resoff[s] + (size_t)nres * nq + threadIdx.x] = sm[threadIdx.x];
 }
 
 // Resolution smearing of the batch's I(q): one thread per smeared point,
@@ -289,7 +298,7 @@ namespace batch_host {
 
 // Host-side plan of one call: the staging layout and the work list.
 struct Plan {
-    int kind = 0, nout = 2, ndet = 0, rv = 1;
+    int kind = 0, nout = 2, nres = 2, ndet = 0, rv = 1;   // partial outputs, result outputs
     int64_t n_values = 0, rec_total = 0, partial_total = 0, result_total = 0, nq_total = 0;
     int n_items = 0;
     int sep_nlp = 0;                  // separable path: bilinear table columns (0: generic path)
@@ -390,7 +399,10 @@ int record_width(int id, int kind)
     case 3: return rv_of<sas::CoreShellCylinder>(kind);
     case 4: return rv_of<sas::CoreShellSphere>(kind);
     case 6: return rv_of<sas::Parallelepiped>(kind);
-    default: return rv_of<sas::TriaxialEllipsoid>(kind);
+    case 7: return rv_of<sas::TriaxialEllipsoid>(kind);
+    case 8: return rv_of<sas::BccParacrystal>(kind);
+    case 9: return rv_of<sas::FccParacrystal>(kind);
+    default: return rv_of<sas::ScParacrystal>(kind);
     }
 }
 
@@ -401,6 +413,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     const int id = b->m->id, ns = b->nsets;
     p.kind = kind;
     p.nout = kind == 0 ? 2 : 1;
+    p.nres = kind == 0 && !model_paracrystal(id) ? 2 : 1;    // models without <F>: <F^2> only
     p.ndet = n_details;
     p.rv = record_width(id, kind);
     p.n_values = voff_in[ns];
@@ -437,7 +450,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         poff[s] = p.partial_total;
         p.partial_total += (int64_t)nchunk[s] * p.nout * nq;
         resoff[s] = p.result_total;
-        p.result_total += (int64_t)p.nout * nq + 4;
+        p.result_total += (int64_t)p.nres * nq + 4;
         items += (int64_t)nchunk[s] * ((nq + sas::QTILE - 1) / sas::QTILE);
     }
     if (items > 0x7fffffff) return fail(SASHIP_EINVAL, "too many work items in one batch");
@@ -576,7 +589,7 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
     }
 
-    k_batch_final<<<dim3(b->nsets, std::max(1, (p.max_nq + 255) / 256)), 256, 0, b->st>>>(p.nout, b->d_qoff, d_values, d_voff, d_poff, d_resoff,
+    k_batch_final<<<dim3(b->nsets, std::max(1, (p.max_nq + 255) / 256)), 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, d_values, d_voff, d_poff, d_resoff,
                                                 d_nchunk, b->d_sums, b->d_partial,
                                                 p.structure ? (const double *)(d + p.off_spars) : nullptr,
                                                 b->q64, b->is2d,
@@ -610,7 +623,10 @@ int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, 
     case 3: return enqueue<sas::CoreShellCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 4: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 6: return enqueue<sas::Parallelepiped>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    default: return enqueue<sas::TriaxialEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 7: return enqueue<sas::TriaxialEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 8: return enqueue<sas::BccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 9: return enqueue<sas::FccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    default: return enqueue<sas::ScParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }
 

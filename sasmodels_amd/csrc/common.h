@@ -73,29 +73,30 @@ inline int cu_count(int dev)
     return n;
 }
 
-// Model ids 0-4 and 6-7 are form factors, 5 the hard-sphere structure
+// Model ids 0-4 and 6-10 are form factors, 5 the hard-sphere structure
 // factor (two parameters, nothing polydisperse, no sld: no magnetic values).
 constexpr int MODEL_HARDSPHERE = 5;
-constexpr int MODEL_LAST = 7;
+constexpr int MODEL_LAST = 10;
 inline int model_npars(int id)
 {
-    static const int np[8] = {3, 6, 6, 8, 5, 2, 8, 8};
+    static const int np[11] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[8] = {1, 4, 4, 5, 2, 0, 5, 5};
+    static const int mp[11] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[8] = {15, 18, 18, 23, 20, 4, 20, 20};
+    static const int nv[11] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20};
     return nv[id];
 }
-inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || id == 6 || id == 7; }
+inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || id >= 6; }
 // orientation (theta, phi, psi): view and jitter about three axes
-inline bool model_abc(int id) { return id == 6 || id == 7; }
-// 1D kernels integrating over orientation by nested (two-level) rules
+inline bool model_abc(int id) { return id >= 6; }
+// bcc, fcc, sc paracrystals: FP64 product rule (Gauss-150 by default)
+inline bool model_paracrystal(int id) { return id >= 8 && id <= 10; }
 inline bool model_nested(int id) { return id == 6 || id == 7; }
 inline bool model_structure(int id) { return id == MODEL_HARDSPHERE; }
 
@@ -179,6 +180,7 @@ inline int ensure_host(char *&ptr, size_t &cap, size_t bytes)
 struct saship_model {
     int id = 0, quad_n = 0, dev = 0;
     float *tab = nullptr;     // 5 * ntab floats: a | b | w | c | d
+    double *tab64 = nullptr;  // FP64 models: 5 * n doubles (Quad::g64)
     int ntab = 0;
     sas::Quad quad{};
     // kernels and batches made from the model hold a reference; the tables
@@ -192,6 +194,7 @@ inline void model_free(saship_model *m)
 {
     (void)hipSetDevice(m->dev);
     if (m->tab) (void)hipFree(m->tab);
+    if (m->tab64) (void)hipFree(m->tab64);
     delete m;
 }
 inline void model_retain(saship_model *m) { m->refs.fetch_add(1); }

@@ -285,7 +285,7 @@ __device__ __forceinline__ void load_rec(const float4 *src, float *r, int rv)
 // ------------------------------------------------------------------ 1D <F>, <F^2>
 template <class M, bool FIXED>
 __global__ __launch_bounds__(EVAL_THREADS) void k_fq1d(
-    int nq, const float *__restrict__ q32, const float4 *__restrict__ recs,
+    int nq, const float *__restrict__ q32, const double *__restrict__ q64, const float4 *__restrict__ recs,
     const int *__restrict__ npts_p, int nsplit, Quad quad, double *__restrict__ partial)
 {
     constexpr int RV = rv1<M>();
@@ -295,7 +295,9 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_fq1d(
     int s0, s1;
     split_range(*npts_p, nsplit, split, s0, s1);
     const float q = iq < nq ? q32[iq] : 0.0f;
+    const double qd = M::FP64 && iq < nq ? q64[iq] : 0.0;
     KSum a1, a2;
+    double d1 = 0.0, d2 = 0.0;     // FP64 models
     for (int t0 = s0; t0 < s1; t0 += TILE_PTS) {
         const int cnt = min(TILE_PTS, s1 - t0);
         for (int i = threadIdx.x; i < cnt * RV; i += EVAL_THREADS) tile[i] = recs[(size_t)t0 * RV + i];
@@ -308,17 +310,24 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_fq1d(
                     const float4 x = tile[j * RV + v];
                     r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
                 }
-                float f1, f2;
-                M::template eval1d<FIXED>(q, r, quad, f1, f2);
-                a1.add(f1);
-                a2.add(f2);
+                if constexpr (M::FP64) {
+                    double f1, f2;
+                    M::eval1d_d(qd, r, quad, f1, f2);
+                    d1 += f1;
+                    d2 += f2;
+                } else {
+                    float f1, f2;
+                    M::template eval1d<FIXED>(q, r, quad, f1, f2);
+                    a1.add(f1);
+                    a2.add(f2);
+                }
             }
         }
         __syncthreads();
     }
     if (iq < nq) {
-        partial[((size_t)0 * nq + iq) * nsplit + split] = a2.value();
-        partial[((size_t)1 * nq + iq) * nsplit + split] = a1.value();
+        partial[((size_t)0 * nq + iq) * nsplit + split] = M::FP64 ? d2 : a2.value();
+        partial[((size_t)1 * nq + iq) * nsplit + split] = M::FP64 ? d1 : a1.value();
     }
 }
 
@@ -365,6 +374,7 @@ struct Contrast {
     float con[NX][M::NCON];
     float xsw[NX];
     bool active[NX];
+    bool on = true;    // magnetic: q != 0 (the reference adds nothing at q = 0)
 
     __device__ __forceinline__ void setup(const double *values, double qx, double qy)
     {
@@ -383,6 +393,7 @@ struct Contrast {
             sincos(values[4 + NP] * DEG, &smt, &cmt);
             sincos(values[5 + NP] * DEG, &smp, &cmp);
             const bool q_ok = qx * qx + qy * qy > 1.e-16;   // kernel_iq.c:754
+            on = q_ok;
 #pragma unroll
             for (int x = 0; x < NX; x++) {
                 active[x] = w[x] > 1.e-8;                    // kernel_iq.c:761
@@ -401,6 +412,7 @@ struct Contrast {
     // sum over cross sections of w_xs * F_xs^2 for amplitude basis g
     __device__ __forceinline__ float f2(const float *g) const
     {
+        if (!on) return 0.0f;    // also when the geometry is not finite at q = 0
         float s = 0.0f;
 #pragma unroll
         for (int x = 0; x < NX; x++) {

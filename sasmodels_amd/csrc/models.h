@@ -17,7 +17,8 @@
 // Reference formulas (sasmodels/models): sphere.c:1-20, cylinder.c:1-102,
 // ellipsoid.c:1-105, core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30
 // with lib/core_shell.c:8-30, parallelepiped.c:1-150,
-// triaxial_ellipsoid.c:1-160.  Parameter order: sphere.py:66-72,
+// triaxial_ellipsoid.c:1-160, bcc/fcc/sc_paracrystal.c with
+// lib/sphere_form.c.  Parameter order: sphere.py:66-72,
 // cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 // core_shell_sphere.py:78-82.
 #pragma once
@@ -45,6 +46,7 @@ struct Quad {
     const float *w;     // w*sin(theta) | w
     const float *c;     // nested rules: outer-node functions (see the model)
     const float *d;
+    const double *g64;  // FP64 models: n nodes each of sin, cos of the outer and inner angles, and w
     int n;              // fixed size, or 0 for adaptive
     int off[4];         // adaptive: offsets of the 20/76/500/5000 tables
 };
@@ -93,6 +95,8 @@ struct Sphere {
     static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = false;
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
+    static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
+    static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
     static constexpr int REC1D = 4, REC2D = 4;
     static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // sld, sld_solvent
@@ -134,6 +138,8 @@ struct Cylinder {
     static constexpr bool SHARED_1D = true;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
+    static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
+    static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
@@ -228,6 +234,8 @@ struct Ellipsoid {
     static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
+    static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
+    static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
@@ -305,6 +313,8 @@ struct CoreShellCylinder {
     static constexpr bool SHARED_1D = true;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = true;
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
+    static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
+    static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
     static constexpr int REC1D = 8, REC2D = 12;
     static constexpr int A2D = 10;  // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // core, shell, solvent
@@ -423,6 +433,8 @@ struct CoreShellSphere {
     static constexpr bool SHARED_1D = false;   // eval1d_shared: lengths shared by a chunk
     static constexpr bool ORIENTED = false;
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
+    static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
+    static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 6;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return 2 + k; }        // core, shell, solvent
@@ -480,6 +492,8 @@ struct Parallelepiped {
     static constexpr bool SHARED_1D = false;
     static constexpr bool ORIENTED = true;
     static constexpr bool ABC = true;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 7;
     SAS_HD static int sld_param(int k) { return k; }
@@ -593,6 +607,8 @@ struct TriaxialEllipsoid {
     static constexpr bool SHARED_1D = false;
     static constexpr bool ORIENTED = true;
     static constexpr bool ABC = true;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 7;
     SAS_HD static int sld_param(int k) { return k; }
@@ -704,5 +720,123 @@ struct TriaxialEllipsoid {
     }
     SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
+
+// ---------------------------------------------------------------- paracrystals
+// Spheres on a distorted bcc, fcc or sc lattice (bcc_paracrystal.c,
+// fcc_paracrystal.c, sc_paracrystal.c with lib/sphere_form.c):
+// p = dnn, d_factor, radius, sld, sld_solvent, theta, phi, psi.
+// I = vf(R, dnn) P(q) Z(q), Z the paracrystal lattice factor
+//   Z = -expm1(2 x)^3 / prod_k ((e^x - 2 cos(d_a a_k)) e^x + 1),
+//   x = -(dnn d_factor)^2 |a|^2 / 2,
+// with the lattice's projections a_k of (qa, qb, qc).  The reference marks
+// these models double precision only: Z cancels near the Bragg peaks, so
+// the 1D orientation average (a fixed product rule over theta and phi,
+// Gauss-150 by default) and Z in 2D are evaluated in FP64.
+enum Lattice { BCC = 0, FCC = 1, SC = 2 };
+
+template <int L>
+__device__ __forceinline__ double paracrystal_z(double qa, double qb, double qc, double dnn, double dfac)
+{
+    double a1, a2, a3, da;
+    if (L == BCC) {
+        a1 = (-qa + qb + qc) * 0.5;
+        a2 = (qa - qb + qc) * 0.5;
+        a3 = (qa + qb - qc) * 0.5;
+        da = dnn / sqrt(0.75);
+    } else if (L == FCC) {
+        a1 = (qa + qb) * 0.5;
+        a2 = (qa + qc) * 0.5;
+        a3 = (qb + qc) * 0.5;
+        da = dnn * sqrt(2.0);
+    } else {
+        a1 = qa;
+        a2 = qb;
+        a3 = qc;
+        da = dnn;
+    }
+    const double sd = dnn * dfac;
+    const double x = -0.5 * sd * sd * (a1 * a1 + a2 * a2 + a3 * a3);
+    const double e = exp(x);
+    const double m = expm1(2.0 * x);
+    const double den = ((e - 2.0 * cos(da * a1)) * e + 1.0) * ((e - 2.0 * cos(da * a2)) * e + 1.0) *
+                       ((e - 2.0 * cos(da * a3)) * e + 1.0);
+    return -(m * m * m) / den;
+}
+
+template <int L>
+struct Paracrystal {
+    static constexpr int ID = 8 + L, NPARS = 8, NMAG = 2, MAX_PD = 4, NCON = 1;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = true;
+    static constexpr bool FQ = false;
+    static constexpr bool FP64 = true;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 7;
+    SAS_HD static int sld_param(int k) { return 3 + k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[2]); }
+    __device__ static double reff(int, const double *) { return 0.0; }
+    // lattice volume fraction of the spheres
+    __device__ static double vfrac(const double *p)
+    {
+        // 1/sqrt(2) with the reference's value (kernel_header.c:156-157)
+        const double x = L == BCC ? sqrt(0.75) * p[2] / p[0] : L == FCC ? 0.70710678118654746 * p[2] / p[0]
+                                                                        : p[2] / p[0];
+        return (L == BCC ? 2.0 : L == FCC ? 4.0 : 1.0) * FOUR_PI_3_D * cube_d(x);
+    }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double s = (p[3] - p[4]) * volume(p);
+        put_d(r, 0, p[2]);
+        put_d(r, 2, p[0]);
+        put_d(r, 4, p[1]);
+        put_d(r, 6, w * vfrac(p) * 1e-4 * s * s);
+    }
+    // orientation average of Z by the fixed product rule (outer theta with
+    // sin(theta) weight, inner phi): bcc/fcc theta in [0, pi], phi in
+    // [0, 2 pi], divided by 4 pi; sc theta, phi in [0, pi/2], divided by pi/2
+    __device__ static void eval1d_d(double q, const float *r, const Quad &t, double &f1, double &f2)
+    {
+        const int n = t.n;
+        const double *st = t.g64, *ct = t.g64 + n, *sp = t.g64 + 2 * n, *cp = t.g64 + 3 * n, *w = t.g64 + 4 * n;
+        const double dnn = get_d(r, 2), dfac = get_d(r, 4);
+        double outer = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double qc = q * ct[i], qab = q * st[i];
+            double inner = 0.0;
+            for (int j = 0; j < n; j++) inner += w[j] * paracrystal_z<L>(qab * cp[j], qab * sp[j], qc, dnn, dfac);
+            outer += w[i] * inner * st[i];
+        }
+        // theta_m * phi_m / (4 pi) = pi/8 (bcc, fcc); (pi/4)^2 / (pi/2) = pi/8 (sc)
+        const double z = outer * (PI_D / 8.0);
+        const double j = sph_j1c_dd(q * get_d(r, 0));
+        f1 = 0.0;
+        f2 = get_d(r, 6) * j * j * z;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float, const float *, const Quad &, float &f1, float &f2)
+    {
+        f1 = f2 = 0.0f;    // FP64 models use eval1d_d
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        put_d(r, 0, p[2]);
+        put_d(r, 2, p[0]);
+        put_d(r, 4, p[1]);
+        r[6] = (float)volume(p);
+        r[7] = (float)(w * 1e-4 * vfrac(p));
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double z = paracrystal_z<L>(q.qa, q.qb, q.qc, get_d(r, 2), get_d(r, 4));
+        const double qm = sqrt(q.qa * q.qa + q.qb * q.qb + q.qc * q.qc);
+        g[0] = (float)(r[6] * sph_j1c_dd(qm * get_d(r, 0)) * sqrt(z));
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+using BccParacrystal = Paracrystal<BCC>;
+using FccParacrystal = Paracrystal<FCC>;
+using ScParacrystal = Paracrystal<SC>;
 
 }  // namespace sas

@@ -145,9 +145,11 @@ int launch(saship_kernel *k, int kind, const int32_t *d_det, const double *d_val
     if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
     if (kind == 0) {
         if (k->m->quad_n > 0)
-            k_fq1d<M, true><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q32, k->d_recs, npts, nsplit, k->m->quad, k->d_partial);
+            k_fq1d<M, true><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q32, k->q64, k->d_recs, npts, nsplit, k->m->quad,
+                                                              k->d_partial);
         else
-            k_fq1d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q32, k->d_recs, npts, nsplit, k->m->quad, k->d_partial);
+            k_fq1d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q32, k->q64, k->d_recs, npts, nsplit, k->m->quad,
+                                                               k->d_partial);
     } else if (kind == 1) {
         k_iq2d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q64, d_val, k->d_recs, npts, nsplit, k->d_partial);
     } else {
@@ -156,8 +158,10 @@ int launch(saship_kernel *k, int kind, const int32_t *d_det, const double *d_val
     HIP_TRY(hipGetLastError());
     if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
 
-    const int fblocks = std::max(1, (nout * nq + 3) / 4);
-    k_finalize<<<fblocks, 256, 0, k->st>>>(nq, nout, nsplit, k->d_partial, sums, accumulate, k->d_result);
+    // models without <F> return only the <F^2> sums (nout 1 in 1D too)
+    const int nres = M::FQ ? nout : 1;
+    const int fblocks = std::max(1, (nres * nq + 3) / 4);
+    k_finalize<<<fblocks, 256, 0, k->st>>>(nq, nres, nsplit, k->d_partial, sums, accumulate, k->d_result);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -361,7 +365,35 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
     m->quad.w = m->tab + 2 * m->ntab;
     m->quad.c = m->tab + 3 * m->ntab;
     m->quad.d = m->tab + 4 * m->ntab;
-    m->quad.n = quad_n;
+    m->quad.g64 = nullptr;
+    if (model_paracrystal(model_id)) {
+        // FP64 product rule over (theta, phi): sin/cos of both angles and w;
+        // bcc, fcc: theta = pi/2 (z + 1), phi = pi (z + 1); sc: both pi/4 (z + 1)
+        const int n = quad_n > 0 ? quad_n : 150;
+        std::vector<double> z, wt, g(5 * (size_t)n);
+        gauss_legendre(n, z, wt);
+        const bool sc = model_id == 10;
+        for (int i = 0; i < n; i++) {
+            const double th = sc ? z[i] * sas::PI_4_D + sas::PI_4_D : (z[i] + 1.0) * (0.5 * sas::PI_D);
+            const double ph = sc ? z[i] * sas::PI_4_D + sas::PI_4_D : (z[i] + 1.0) * sas::PI_D;
+            g[i] = sin(th);
+            g[n + i] = cos(th);
+            g[2 * n + i] = sin(ph);
+            g[3 * n + i] = cos(ph);
+            g[4 * n + i] = wt[i];
+        }
+        e = hipMalloc((void **)&m->tab64, sizeof(double) * g.size());
+        if (e == hipSuccess) e = hipMemcpy(m->tab64, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(m->tab);
+            if (m->tab64) (void)hipFree(m->tab64);
+            delete m;
+            return fail(SASHIP_EHIP, std::string("quadrature upload: ") + hipGetErrorString(e));
+        }
+        m->quad.g64 = m->tab64;
+        m->quad_n = n;
+    }
+    m->quad.n = m->quad_n;
     for (int i = 0; i < 4; i++) m->quad.off[i] = i < (int)offs.size() ? offs[i] : 0;
     *out = m;
     return 0;
@@ -427,7 +459,7 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, "")) return e;
     std::lock_guard<std::mutex> lock(k->mu);
     HIP_TRY(hipSetDevice(k->dev));
-    const int nq = k->nq, nout = kind == 0 && !model_structure(id) ? 2 : 1;
+    const int nq = k->nq, nout = kind == 0 && !model_structure(id) && !model_paracrystal(id) ? 2 : 1;
     const size_t vbytes = sizeof(double) * (size_t)n_values;
     const size_t dbytes = sizeof(int32_t) * (size_t)n_details;
     const size_t doff = (vbytes + 255) & ~(size_t)255;
@@ -467,7 +499,10 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     case 3: rc = launch<sas::CoreShellCylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     case 4: rc = launch<sas::CoreShellSphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     case 6: rc = launch<sas::Parallelepiped>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    default: rc = launch<sas::TriaxialEllipsoid>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 7: rc = launch<sas::TriaxialEllipsoid>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 8: rc = launch<sas::BccParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 9: rc = launch<sas::FccParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    default: rc = launch<sas::ScParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     }
     if (rc) return rc;
     return finish_call(k, rbytes, result);

@@ -292,6 +292,23 @@ SAS_HD float j1c_d(double x)
 // FP64 helpers used where the kernels need double (volumes, weights).
 SAS_HD double cube_d(double x) { return x * x * x; }
 
+// 3 j1(x)/x in FP64: series sum_k (-1)^k 3 (2k+2) x^(2k) / (2k+3)! below
+// |x| = 0.25 (to x^10: remainder < 1e-17), the closed form above.
+SAS_HD double sph_j1c_dd(double x)
+{
+    const double z = x * x;
+    if (z < 0.0625) {
+        double p = fma(z, -1.0 / 172972800.0, 1.0 / 1330560.0);
+        p = fma(z, p, -1.0 / 15120.0);
+        p = fma(z, p, 1.0 / 280.0);
+        p = fma(z, p, -0.1);
+        return fma(z, p, 1.0);
+    }
+    double s, c;
+    sincos(x, &s, &c);
+    return 3.0 * (s - x * c) / (z * x);
+}
+
 // A double stored in two consecutive words of an FP32 record.
 SAS_HD double get_d(const float *r, int i)
 {

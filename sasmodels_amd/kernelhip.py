@@ -276,13 +276,17 @@ class HipBatch:
         vals = np.concatenate(vals).astype(np.float64, copy=False)
         return np.ascontiguousarray(det), np.ascontiguousarray(vals), voff, det.size//self.nsets
 
+    def _nout(self):
+        """Outputs per q in the raw result: <F^2> and <F> for 1D models with <F>."""
+        return 2 if self.dim == "1d" and self.info.have_Fq else 1
+
     def _kind(self, magnetic):
         return KIND_IQ if self.dim == "1d" else (KIND_IMAGNETIC if magnetic else KIND_IQXY)
 
     def raw(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0, want_result=True, want_iq=True):
         """(raw accumulators of all sets back to back, I(q) of all sets back to back)."""
         det, vals, voff, ndet = self._pack(calls)
-        nout = 2 if self.dim == "1d" else 1
+        nout = self._nout()
         result = np.empty(int(np.sum(nout*self.nq + 4)), "d") if want_result else None
         n_iq = int(self.qoff[-1]) if self._n_out is None else int(self._n_out.sum())
         iq = np.empty(n_iq, "d") if want_iq else None
@@ -302,7 +306,7 @@ class HipBatch:
     def Fq(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0):
         """List of (F1, F2, R_eff, V_shell, V_form/V_shell), one per set."""
         result, _ = self.raw(calls, cutoff, magnetic, radius_effective_mode, want_result=True, want_iq=False)
-        nout = 2 if self.dim == "1d" else 1
+        nout = self._nout()
         out = []
         off = 0
         for nq in self.nq:

@@ -17,7 +17,9 @@ path depends on (sasmodels/modelinfo.py):
 The parameter tables are the reference's own (sphere.py:66-72,
 cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 core_shell_sphere.py:78-82, parallelepiped.py:216-231,
-triaxial_ellipsoid.py:143-158, hardsphere.py:84-88), with the attributes the
+triaxial_ellipsoid.py:143-158, bcc_paracrystal.py:186-194,
+fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hardsphere.py:84-88),
+with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
 orientation mode, ``structure_factor``).
 """
@@ -200,6 +202,19 @@ class ModelInfo:
         return set()
 
 
+def _paracrystal(kernel_id, dnn_limits, sld, sld_solvent, angle, names):
+    return (kernel_id, [
+        ["dnn", "Ang", 220.0, dnn_limits, "", "Nearest neighbour distance"],
+        ["d_factor", "", 0.06, [-inf, inf], "", "Paracrystal distortion factor"],
+        ["radius", "Ang", 40.0, [0, inf], "volume", names[0]],
+        ["sld", "1e-6/Ang^2", sld[0], sld[1], "sld", names[1]],
+        ["sld_solvent", "1e-6/Ang^2", sld_solvent[0], sld_solvent[1], "sld", "Solvent scattering length density"],
+        ["theta", "degrees", angle, [-180, 180], "orientation", "c axis to beam angle"],
+        ["phi", "degrees", angle, [-360, 360], "orientation", "rotation about beam"],
+        ["psi", "degrees", angle, [-360, 360], "orientation", "rotation about c axis"],
+    ], dict(have_Fq=False, xy_mode="qabc", has_quadrature=True))
+
+
 # name -> (kernel id, parameter rows, extra attributes)
 _MODEL_TABLE = {
     "sphere": (0, [
@@ -280,6 +295,14 @@ _MODEL_TABLE = {
                 "equivalent biaxial ellipsoid average curvature", "equivalent volume sphere",
                 "min radius", "max radius"],
             xy_mode="qabc", has_quadrature=True)),
+    # spheres on distorted lattices: Iq only, FP64, Gauss-150 product rule
+    # over (theta, phi) unless an integration size is set
+    "bcc_paracrystal": _paracrystal(8, [-inf, inf], (4, [-inf, inf]), (1, [-inf, inf]), 60,
+                                    ("Particle radius", "Particle scattering length density")),
+    "fcc_paracrystal": _paracrystal(9, [-inf, inf], (4, [-inf, inf]), (1, [-inf, inf]), 60,
+                                    ("Particle radius", "Particle scattering length density")),
+    "sc_paracrystal": _paracrystal(10, [0.0, inf], (3.0, [0.0, inf]), (6.3, [0.0, inf]), 0,
+                                   ("Radius of sphere", "Sphere scattering length density")),
     # structure factor: Iq only (no F), no volume, nothing polydisperse;
     # evaluated in FP64 (the reference marks it double precision only)
     "hardsphere": (5, [

@@ -113,8 +113,26 @@ TRIAXIAL_ELLIPSOID = [
     ({"theta": 80., "phi": 10.}, (_tri_qx, _tri_qy), 166.712060266),
 ]
 
+# bcc_paracrystal.py:226-232, fcc_paracrystal.py:216-222, sc_paracrystal.py:209-216
+BCC_PARACRYSTAL = [
+    ({}, [0.001, 4.*pi/220., 0.25], [0.6945817843046642, 1.6885157981411993, 0.005367008206852725]),
+]
+FCC_PARACRYSTAL = [
+    ({}, [0.01, 0.035, 0.25], [0.0213498915484313, 69.03586722100925, 0.005713137547083953]),
+]
+SC_PARACRYSTAL = [
+    ({}, 0.001, 10.3048),
+    ({}, 0.215268, 0.00814889),
+    ({}, 0.414467, 0.001313289),
+    ({"theta": 10.0, "phi": 20, "psi": 30.0}, (0.045, -0.035), 18.0397138402),
+    ({"theta": 10.0, "phi": 20, "psi": 30.0}, (0.023, 0.045), 0.0177333171285),
+]
+
 MODEL_KATS = {
     "sphere": SPHERE,
+    "bcc_paracrystal": BCC_PARACRYSTAL,
+    "fcc_paracrystal": FCC_PARACRYSTAL,
+    "sc_paracrystal": SC_PARACRYSTAL,
     "parallelepiped": PARALLELEPIPED,
     "triaxial_ellipsoid": TRIAXIAL_ELLIPSOID,
     "cylinder": CYLINDER,

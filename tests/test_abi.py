@@ -33,7 +33,7 @@ def test_library_exports_every_declared_function():
 def test_error_reporting_without_device():
     lib = _lib.lib()
     out = ct.c_void_p()
-    assert lib.saship_model_create(8, 0, ct.byref(out)) == -1
+    assert lib.saship_model_create(99, 0, ct.byref(out)) == -1
     assert b"unknown model id" in lib.saship_last_error()
     assert lib.saship_model_create(1, -5, ct.byref(out)) == -1
     assert lib.saship_kernel_call(None, 0, None, 0, None, 0, 0.0, 0, 0, 0, None) == -1

@@ -51,6 +51,12 @@ RANDOM_1D = {
     "triaxial_ellipsoid": lambda r: dict(radius_equat_minor=r.uniform(10, 200), radius_equat_major=r.uniform(10, 200),
                                          radius_polar=r.uniform(10, 200), radius_polar_pd=0.1,
                                          radius_polar_pd_n=int(r.integers(1, 6))),
+    "bcc_paracrystal": lambda r: dict(dnn=r.uniform(100, 300), d_factor=r.uniform(0.03, 0.3),
+                                      radius=r.uniform(10, 40), radius_pd=0.1, radius_pd_n=3),
+    "fcc_paracrystal": lambda r: dict(dnn=r.uniform(100, 300), d_factor=r.uniform(0.03, 0.3),
+                                      radius=r.uniform(10, 40), radius_pd=0.1, radius_pd_n=3),
+    "sc_paracrystal": lambda r: dict(dnn=r.uniform(100, 300), d_factor=r.uniform(0.03, 0.3),
+                                     radius=r.uniform(10, 40), radius_pd=0.1, radius_pd_n=3),
 }
 
 
@@ -65,7 +71,7 @@ def test_1d_random(model, quad):
         hk, ok, det, val, mag = buffers(model, [q], pars, quad)
         hk._call_kernel(det, val, 0.0, mag, 1)
         ora = ok.raw(det, val, 0.0, mag, 1)
-        compare_raw(hk.result, ora, q.size, 2)
+        compare_raw(hk.result, ora, q.size, 2 if hk.info.have_Fq else 1)
 
 
 RANDOM_2D = {
@@ -90,6 +96,15 @@ RANDOM_2D = {
                                          radius_polar=r.uniform(10, 100), theta=r.uniform(0, 90),
                                          phi=r.uniform(0, 180), psi=r.uniform(0, 180), psi_pd=20, psi_pd_n=5,
                                          theta_pd=5, theta_pd_n=3),
+    "bcc_paracrystal": lambda r: dict(dnn=r.uniform(150, 300), d_factor=r.uniform(0.05, 0.2), radius=r.uniform(20, 50),
+                                      theta=r.uniform(0, 90), phi=r.uniform(0, 180), psi=r.uniform(0, 180),
+                                      theta_pd=5, theta_pd_n=3, psi_pd=10, psi_pd_n=3),
+    "fcc_paracrystal": lambda r: dict(dnn=r.uniform(150, 300), d_factor=r.uniform(0.05, 0.2), radius=r.uniform(20, 50),
+                                      theta=r.uniform(0, 90), phi=r.uniform(0, 180), psi=r.uniform(0, 180),
+                                      phi_pd=5, phi_pd_n=3),
+    "sc_paracrystal": lambda r: dict(dnn=r.uniform(150, 300), d_factor=r.uniform(0.05, 0.2), radius=r.uniform(20, 50),
+                                     theta=r.uniform(0, 90), phi=r.uniform(0, 180), psi=r.uniform(0, 180),
+                                     radius_pd=0.1, radius_pd_n=3),
 }
 
 
@@ -131,6 +146,8 @@ MAGNETIC = {
     "triaxial_ellipsoid": dict(radius_equat_minor=25, radius_equat_major=90, radius_polar=50, theta=20,
                                phi=50, psi=10, sld_solvent_M0=1.5, sld_solvent_mphi=90, up_frac_i=0.6,
                                up_frac_f=0.1, up_theta=60, up_phi=20),
+    "bcc_paracrystal": dict(dnn=200, d_factor=0.1, radius=40, theta=30, phi=40, psi=50, sld_M0=2.0,
+                            sld_mtheta=45, up_frac_i=0.3, up_frac_f=0.8, up_theta=20),
 }
 
 

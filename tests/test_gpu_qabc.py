@@ -1,5 +1,6 @@
 """
-The (theta, phi, psi) shapes -- parallelepiped and triaxial ellipsoid -- on
+The (theta, phi, psi) shapes -- parallelepiped, triaxial ellipsoid and the
+bcc, fcc and sc paracrystals -- on
 the batched persistent path: nested-rule 1D sets (adaptive and fixed
 Gauss-76) and oriented, jittered, magnetic 2D sets, each against the FP64
 oracle on the same buffers.  (The single-call path is covered with the
@@ -80,4 +81,30 @@ def test_batch_2d_magnetic(model):
     for s, (det, val) in enumerate(calls):
         ora = OracleKernel(hm.info, [qx, qy]).raw(det, val, 0.0, True, 0)
         compare_raw(raw[s*(nq + 4):(s + 1)*(nq + 4)], ora, nq, 1)
+    batch.release()
+
+
+@pytest.mark.parametrize("model", ["bcc_paracrystal", "fcc_paracrystal", "sc_paracrystal"])
+def test_batch_paracrystal_1d(model):
+    """FP64 product-rule sets through the persistent batch: <F^2> only, one
+    output per q in the raw result."""
+    rng = np.random.default_rng(90 + len(model))
+    hm = hip_model(model)
+    kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
+    sets = []
+    for _ in range(4):
+        q = np.logspace(-3, -0.5, int(rng.integers(10, 40)))
+        pars = dict(dnn=rng.uniform(100, 300), d_factor=rng.uniform(0.03, 0.3), radius=rng.uniform(10, 50),
+                    radius_pd=0.1, radius_pd_n=int(rng.integers(1, 4)))
+        sets.append((q, make_kernel_args(kl, direct_model.get_mesh(hm.info, pars, dim="1d"))[:2]))
+    batch = HipBatch(hm, [[q] for q, _ in sets])
+    print("paracrystal batch 1d", model, flush=True)
+    raw, iq = batch.raw([c for _, c in sets], 0.0, False, 0)
+    off = 0
+    for s, (q, (det, val)) in enumerate(sets):
+        r = raw[off:off + q.size + 4]
+        off += q.size + 4
+        ok = OracleKernel(hm.info, [q])
+        compare_raw(r, ok.raw(det, val, 0.0, False, 0), q.size, 1)
+        assert masked_relerr(iq[batch.qoff[s]:batch.qoff[s + 1]], ok(det, val, 0.0, False)) <= RTOL
     batch.release()
