@@ -311,6 +311,35 @@ class ProductBatch:
         self.batch = HipBatch(model.P, q_list, use_graph=use_graph)
         self.dim = self.batch.dim
         self.nsets = self.batch.nsets
+        self._det_cache = {}
+        self._idx_cache = {}
+
+    def _p_details(self, cd):
+        """P's mesh descriptor for P@S details *cd*, cached by layout (a fit
+        keeps the same polydispersity layout from call to call)."""
+        lay = self._layout
+        key = (cd.length[lay.p_detail].tobytes(), cd.offset[lay.p_detail].tobytes(), int(cd.num_weights))
+        det = self._det_cache.get(key)
+        if det is None:
+            det = make_details(lay.p_info, np.asarray(cd.length[lay.p_detail]),
+                               np.asarray(cd.offset[lay.p_detail]), int(cd.num_weights))
+            if len(self._det_cache) > 256:
+                self._det_cache.clear()
+            self._det_cache[key] = det
+        return det
+
+    def _p_index(self, nweights):
+        """Positions in a P@S values vector of P's values vector (scale and
+        background kept: the final kernel applies them), and its padded length."""
+        idx = self._idx_cache.get(nweights)
+        if idx is None:
+            lay = self._layout
+            idx = np.concatenate([[0, 1], np.arange(lay.p_values.start, lay.p_values.stop),
+                                  np.arange(lay.magnetic.start, lay.magnetic.stop),
+                                  np.arange(lay.nvalues, lay.nvalues + 2*nweights)]).astype(np.intp)
+            idx = (idx, idx.size + (32 - idx.size % 32) % 32)
+            self._idx_cache[nweights] = idx
+        return idx
 
     def _split(self, calls):
         lay = self._layout
@@ -319,7 +348,10 @@ class ProductBatch:
             scale, background, volfrac, er_mode, beta_mode = lay.modes(values)
             if beta_mode and self.dim == "2d":
                 raise NotImplementedError("beta not yet supported for 2D")
-            p_calls.append(lay.p_call(cd, values, scale, background))
+            idx, size = self._p_index(int(cd.num_weights))
+            p_values = np.zeros(size)
+            p_values[:idx.size] = values[idx]
+            p_calls.append((self._p_details(cd), p_values))
             spars[s] = (values[lay.er_index], volfrac, 1.0 if beta_mode else 0.0, er_mode)
             if er_mode > 0:
                 er_modes.add(er_mode)
