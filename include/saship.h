@@ -1,8 +1,8 @@
 /*
  * saship.h -- C ABI of libsaship.so, the MI355X (gfx950) I(q) engine.
  *
- * The library evaluates the sasmodels polydispersity-mesh kernel for ten
- * form factors and the hard-sphere structure factor on one GPU.  Callers pass the same buffers the reference's
+ * The library evaluates the sasmodels polydispersity-mesh kernel for
+ * thirteen form factors and the hard-sphere structure factor on one GPU.  Callers pass the same buffers the reference's
  * compiled kernels take:
  *
  *   void <model>_{Iq|Iqxy|Imagnetic}(int32 nq, int32 pd_start, int32 pd_stop,
@@ -58,6 +58,10 @@ extern "C" {
 #define SASHIP_BCC_PARACRYSTAL 8
 #define SASHIP_FCC_PARACRYSTAL 9
 #define SASHIP_SC_PARACRYSTAL 10
+/* hollow shapes (result[nout*nq + 2] is the shell volume sum) and the fuzzy sphere */
+#define SASHIP_HOLLOW_CYLINDER 11
+#define SASHIP_VESICLE 12
+#define SASHIP_FUZZY_SPHERE 13
 
 /* kernel kinds: the three kernels generated per model (generate.py:1144-1180) */
 #define SASHIP_KIND_IQ 0        /* <model>_Iq: 1D, <F> and <F^2> */

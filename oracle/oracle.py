@@ -27,7 +27,8 @@ NO_FQ = (8, 9, 10)
 MODEL_IDS = {"sphere": 0, "cylinder": 1, "ellipsoid": 2,
              "core_shell_cylinder": 3, "core_shell_sphere": 4,
              "parallelepiped": 6, "triaxial_ellipsoid": 7,
-             "bcc_paracrystal": 8, "fcc_paracrystal": 9, "sc_paracrystal": 10}
+             "bcc_paracrystal": 8, "fcc_paracrystal": 9, "sc_paracrystal": 10,
+             "hollow_cylinder": 11, "vesicle": 12, "fuzzy_sphere": 13}
 
 _lib = None
 _D = ct.c_double
@@ -67,6 +68,8 @@ def lib():
         L.sasoracle_view_jitter_abc.argtypes = [_D]*6 + [ct.c_void_p]
         L.sasoracle_hardsphere.restype = _D
         L.sasoracle_hardsphere.argtypes = [_D, _D, _D]
+        L.sasoracle_shell_volume.restype = _D
+        L.sasoracle_shell_volume.argtypes = [ct.c_int, ct.c_void_p]
         L.sasoracle_radius_effective.restype = _D
         L.sasoracle_radius_effective.argtypes = [ct.c_int, ct.c_int, ct.c_void_p]
         L.sasoracle_view_jitter_row.argtypes = [_D, _D, _D, _D, _P, _P]
@@ -170,20 +173,23 @@ def hardsphere(q, radius_effective, volfraction):
                     ).reshape(q.shape)
 
 
-def product(q, fq, scale, background, radius_effective, volfraction, beta_mode, er_mode):
+def product(q, fq, scale, background, radius_effective, volfraction, beta_mode, er_mode,
+            volfrac_in_p=False):
     """
     P@S intensity from P's normalised (F, F^2, R_eff, V_shell, V_form/V_shell)
-    (sasmodels/product.py:425-528, P without its own volfraction): S at
-    P's R_eff (er_mode > 0) or *radius_effective*, volume fraction
-    volfraction * V_form/V_shell; I = scale*volfraction/V_shell * PS +
-    background with PS = F^2 S, or F^2 + F1^2 (S - 1) with beta.  *q* is the
-    1D q or the |q| of 2D points.  Returns (I, S).
+    (sasmodels/product.py:425-528): S at P's R_eff (er_mode > 0) or
+    *radius_effective*, volume fraction volfraction * V_form/V_shell;
+    I = scale*volfraction/V_shell * PS + background with PS = F^2 S, or
+    F^2 + F1^2 (S - 1) with beta; when P has its own volfraction (already in
+    its F) the number density is scale/V_shell.  *q* is the 1D q or the |q|
+    of 2D points.  Returns (I, S).
     """
     F1, F2, reff, shell, ratio = fq
     radius = reff if er_mode > 0 else radius_effective
     S = hardsphere(q, radius, volfraction*ratio)
     PS = F2 + F1**2*(S - 1.0) if beta_mode else F2*S
-    return scale*volfraction/shell*PS + background, S
+    density = scale/shell if volfrac_in_p else scale*volfraction/shell
+    return density*PS + background, S
 
 
 def ref_lib(name):
@@ -209,6 +215,9 @@ def ref_lib(name):
     L.ref_form_volume.argtypes = [ct.c_void_p]
     L.ref_reff.restype = _D
     L.ref_reff.argtypes = [ct.c_int, ct.c_void_p]
+    if hasattr(L, "ref_shell_volume"):
+        L.ref_shell_volume.restype = _D
+        L.ref_shell_volume.argtypes = [ct.c_void_p]
     if hasattr(L, "ref_iqac"):
         L.ref_iqac.restype = _D
         L.ref_iqac.argtypes = [_D, _D, ct.c_void_p]

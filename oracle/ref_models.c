@@ -11,10 +11,11 @@
  * vector to the reference's static functions so that ctypes can call them.
  *
  * One shared object is built per (model, quadrature) pair:
- *   -DREF_MODEL=<1..11> sphere, cylinder, ellipsoid, core_shell_cylinder,
+ *   -DREF_MODEL=<1..14> sphere, cylinder, ellipsoid, core_shell_cylinder,
  *                       core_shell_sphere, hardsphere (structure factor),
  *                       parallelepiped, triaxial_ellipsoid, bcc, fcc and
- *                       sc paracrystal (with lib/gauss150.c)
+ *                       sc paracrystal (with lib/gauss150.c), hollow_cylinder,
+ *                       vesicle, fuzzy_sphere
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -109,8 +110,36 @@ double ref_iqabc(double qa, double qb, double qc, const double *p)
     return Iqabc(qa, qb, qc, p[0], p[1], p[2], p[3], p[4]);
 }
 double ref_form_volume(const double *p) { return form_volume(p[2]); }
+#elif REF_MODEL == 12
+#  include "lib/polevl.c"
+#  include "lib/sas_J1.c"
+#  ifdef REF_FIXED76
+#    include "lib/gauss76.c"
+#    include "lib/nonadaptive.c"
+#  else
+#    include "lib/adaptive.c"
+#  endif
+#  include "hollow_cylinder.c"
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4])
+#  define NPAR_IQAC(a, c, p) Iqac(a, c, p[0], p[1], p[2], p[3], p[4])
+#  define NPAR_VOLUME(p) form_volume(p[0], p[1], p[2])
+#  define NPAR_SHELL(p) shell_volume(p[0], p[1], p[2])
+#  define NPAR_REFF(m, p) radius_effective(m, p[0], p[1], p[2])
+#elif REF_MODEL == 13
+#  include "lib/sas_3j1x_x.c"
+#  include "vesicle.c"
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4])
+#  define NPAR_VOLUME(p) form_volume(p[3], p[4])
+#  define NPAR_SHELL(p) shell_volume(p[3], p[4])
+#  define NPAR_REFF(m, p) radius_effective(m, p[3], p[4])
+#elif REF_MODEL == 14
+#  include "lib/sas_3j1x_x.c"
+#  include "fuzzy_sphere.c"
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3])
+#  define NPAR_VOLUME(p) form_volume(p[2], p[3])
+#  define NPAR_REFF(m, p) radius_effective(m, p[2], p[3])
 #else
-#  error "REF_MODEL must be 1..11"
+#  error "REF_MODEL must be 1..14"
 #endif
 
 #ifdef NPAR_CALL_FQ
@@ -120,6 +149,9 @@ double ref_reff(int mode, const double *p) { return NPAR_REFF(mode, p); }
 #endif
 #ifdef NPAR_IQAC
 double ref_iqac(double qab, double qc, const double *p) { return NPAR_IQAC(qab, qc, p); }
+#endif
+#ifdef NPAR_SHELL
+double ref_shell_volume(const double *p) { return NPAR_SHELL(p); }
 #endif
 #ifdef NPAR_IQABC
 double ref_iqabc(double qa, double qb, double qc, const double *p) { return NPAR_IQABC(qa, qb, qc, p); }

@@ -26,7 +26,9 @@
  *     lib/core_shell.c:8-30, parallelepiped.c:1-150,
  *     triaxial_ellipsoid.c:1-160 (nested rules adaptive.c:11243-11298),
  *     bcc_paracrystal.c, fcc_paracrystal.c, sc_paracrystal.c with
- *     lib/sphere_form.c (Gauss-150 product rule, lib/gauss150.c);
+ *     lib/sphere_form.c (Gauss-150 product rule, lib/gauss150.c),
+ *     hollow_cylinder.c, vesicle.c, fuzzy_sphere.c (shell volumes of the
+ *     hollow shapes in the volume sums, kernel_iq.c:728-731);
  *     view/jitter rotation with psi qabc_rotation/qabc_apply
  *     kernel_iq.c:251-310
  *   - hard-sphere structure factor hardsphere.c:1-90
@@ -677,6 +679,89 @@ PC_FNS(0, bcc)
 PC_FNS(1, fcc)
 PC_FNS(2, scp)
 
+/* ---- hollow cylinder (hollow_cylinder.c:1-140): p = radius, thickness,
+ *      length, sld, sld_solvent, theta, phi ---- */
+static double hcy_vol(const double *p) { return O_PI * p[2] * sq(p[0] + p[1]); }
+static double hcy_svol(const double *p) { return O_PI * p[2] * (sq(p[0] + p[1]) - p[0] * p[0]); }
+static double hcy_reff(int mode, const double *p)
+{
+    const double ro = p[0] + p[1], l = p[2];
+    switch (mode) {
+    default:
+    case 1: return 0.5 * cbrt(0.75 * ro * (2.0 * ro * l + (ro + l) * (O_PI * ro + l)));
+    case 2: return cbrt(O_PI * sq(p[0] + p[1]) * l / O_4PI_3);
+    case 3: return ro;
+    case 4: return 0.5 * l;
+    case 5: return (ro < 0.5 * l ? ro : 0.5 * l);
+    case 6: return (ro > 0.5 * l ? ro : 0.5 * l);
+    case 7: return sqrt(sq(p[0] + p[1]) + 0.25 * sq(l));
+    }
+}
+static double hcy_amp(double qab, double qc, const double *p)
+{
+    const double r = p[0], t = p[1], l = p[2];
+    const double lam1 = o_2j1x_x((r + t) * qab);
+    const double lam2 = o_2j1x_x(r * qab);
+    const double gamma_sq = sq(r / (r + t));
+    const double psi = (lam1 - gamma_sq * lam2) / (1.0 - gamma_sq);
+    return psi * o_sinx_x(0.5 * l * qc);
+}
+static void hcy_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    const double *z, *w;
+    const int n = quad_select(quad_n, q * fmax(p[0] + p[1], 0.5 * p[2]), &z, &w);
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double ct = 0.5 * (z[i] * (1.0 - 0.0) + 0.0 + 1.0);
+        const double st = sqrt(1.0 - ct * ct);
+        const double form = hcy_amp(q * st, q * ct, p);
+        t1 += w[i] * form;
+        t2 += w[i] * form * form;
+    }
+    t1 *= 0.5 * (1.0 - 0.0);
+    t2 *= 0.5 * (1.0 - 0.0);
+    const double s = (p[3] - p[4]) * hcy_svol(p);
+    *f1 = 1e-2 * s * t1;
+    *f2 = 1e-4 * s * s * t2;
+}
+static double hcy_iqac(double qab, double qc, const double *p)
+{
+    const double form = hcy_amp(qab, qc, p);
+    const double s = (p[3] - p[4]) * hcy_svol(p);
+    return 1.0e-4 * sq(s * form);
+}
+
+/* ---- vesicle (vesicle.c:1-50): p = sld, sld_solvent, volfraction, radius, thickness ---- */
+static double ves_vol(const double *p) { return O_4PI_3 * cb(p[3] + p[4]); }
+static double ves_svol(const double *p) { return O_4PI_3 * (cb(p[3] + p[4]) - cb(p[3])); }
+static double ves_reff(int mode, const double *p) { (void)mode; return p[3] + p[4]; }
+static void ves_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    (void)quad_n;
+    double vol, contrast, f;
+    contrast = p[1] - p[0];
+    vol = O_4PI_3 * cb(p[3]);
+    f = vol * o_sph_j1c(q * p[3]) * contrast;
+    contrast = p[0] - p[1];
+    vol = O_4PI_3 * cb(p[3] + p[4]);
+    f += vol * o_sph_j1c(q * (p[3] + p[4])) * contrast;
+    *f1 = 1e-2 * sqrt(p[2]) * f;
+    *f2 = 1.0e-4 * p[2] * f * f;
+}
+
+/* ---- fuzzy sphere (fuzzy_sphere.c:1-35): p = sld, sld_solvent, radius, fuzziness ---- */
+static double fuz_vol(const double *p) { return O_4PI_3 * cb(p[2]); }
+static double fuz_reff(int mode, const double *p) { return mode == 2 ? p[2] + p[3] : p[2]; }
+static void fuz_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+{
+    (void)quad_n;
+    const double bes = o_sph_j1c(q * p[2]);
+    const double qf = exp(-0.5 * sq(q * p[3]));
+    const double form = (p[0] - p[1]) * fuz_vol(p) * bes * qf;
+    *f1 = 1.0e-2 * form;
+    *f2 = 1.0e-4 * form * form;
+}
+
 /* ------------------------------------------------------------------ */
 /* Model table                                                         */
 /* ------------------------------------------------------------------ */
@@ -690,27 +775,31 @@ typedef struct {
     double (*iqac)(double qab, double qc, const double *p);
     double (*iqabc)(double qa, double qb, double qc, const double *p);
     int no_fq;   /* Iq-only model: one output per q in 1D too */
+    double (*svol)(const double *p);   /* shell volume of hollow models (else the form volume) */
 } OModel;
 
 static int always_valid(const double *p) { (void)p; return 1; }
 
 /* model ids: 0 sphere, 1 cylinder, 2 ellipsoid, 3 core_shell_cylinder,
  * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid, 8-10 bcc,
- * fcc, sc paracrystal (the ids libsaship uses; 5, the hard-sphere structure
- * factor, has its own entry) */
-#define N_MODELS 11
+ * fcc, sc paracrystal, 11 hollow_cylinder, 12 vesicle, 13 fuzzy_sphere (the
+ * ids libsaship uses; 5, the hard-sphere structure factor, has its own entry) */
+#define N_MODELS 14
 static const OModel MODELS[N_MODELS] = {
-    {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0, 0},
-    {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0, 0},
-    {6, 2, 4, 1, {0, 1, -1}, always_valid, ell_vol, ell_reff, ell_fq, ell_iqac, 0, 0},
-    {8, 3, 5, 1, {0, 1, 2}, always_valid, csc_vol, csc_reff, csc_fq, csc_iqac, 0, 0},
-    {5, 3, 2, 0, {2, 3, 4}, always_valid, css_vol, css_reff, css_fq, 0, 0, 0},
-    {0, 0, 0, 0, {-1, -1, -1}, 0, 0, 0, 0, 0, 0, 0},
-    {8, 2, 5, 1, {0, 1, -1}, always_valid, ppd_vol, ppd_reff, ppd_fq, 0, ppd_iqabc, 0},
-    {8, 2, 5, 1, {0, 1, -1}, always_valid, tri_vol, tri_reff, tri_fq, 0, tri_iqabc, 0},
-    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, bcc_fq, 0, bcc_iqabc, 1},
-    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, fcc_fq, 0, fcc_iqabc, 1},
-    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, scp_fq, 0, scp_iqabc, 1},
+    {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0, 0, 0},
+    {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0, 0, 0},
+    {6, 2, 4, 1, {0, 1, -1}, always_valid, ell_vol, ell_reff, ell_fq, ell_iqac, 0, 0, 0},
+    {8, 3, 5, 1, {0, 1, 2}, always_valid, csc_vol, csc_reff, csc_fq, csc_iqac, 0, 0, 0},
+    {5, 3, 2, 0, {2, 3, 4}, always_valid, css_vol, css_reff, css_fq, 0, 0, 0, 0},
+    {0, 0, 0, 0, {-1, -1, -1}, 0, 0, 0, 0, 0, 0, 0, 0},
+    {8, 2, 5, 1, {0, 1, -1}, always_valid, ppd_vol, ppd_reff, ppd_fq, 0, ppd_iqabc, 0, 0},
+    {8, 2, 5, 1, {0, 1, -1}, always_valid, tri_vol, tri_reff, tri_fq, 0, tri_iqabc, 0, 0},
+    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, bcc_fq, 0, bcc_iqabc, 1, 0},
+    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, fcc_fq, 0, fcc_iqabc, 1, 0},
+    {8, 2, 4, 1, {3, 4, -1}, always_valid, pc_vol, pc_reff, scp_fq, 0, scp_iqabc, 1, 0},
+    {7, 2, 5, 1, {3, 4, -1}, always_valid, hcy_vol, hcy_reff, hcy_fq, hcy_iqac, 0, 0, hcy_svol},
+    {5, 2, 2, 0, {0, 1, -1}, always_valid, ves_vol, ves_reff, ves_fq, 0, 0, 0, ves_svol},
+    {4, 2, 2, 0, {0, 1, -1}, always_valid, fuz_vol, fuz_reff, fuz_fq, 0, 0, 0, 0},
 };
 static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npars > 0; }
 
@@ -888,7 +977,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
         const double form = m->vol(p);
         acc[0] += weight;
         acc[1] += weight * form;
-        acc[2] += weight * form;
+        acc[2] += weight * (m->svol ? m->svol(p) : form);
         if (reff_mode != 0) acc[3] += weight * m->reff(reff_mode, p);
         double *rec = pts + (size_t)nacc * rw;
         memcpy(rec, p, sizeof(double) * np);
@@ -1007,6 +1096,12 @@ void sasoracle_view_jitter_abc(double theta, double phi, double psi, double dthe
                                double dpsi, double *r)
 {
     view_jitter_abc(theta, phi, psi, dtheta, dphi, dpsi, r);
+}
+
+double sasoracle_shell_volume(int model_id, const double *p)
+{
+    if (!model_ok(model_id)) return NAN;
+    return MODELS[model_id].svol ? MODELS[model_id].svol(p) : MODELS[model_id].vol(p);
 }
 
 double sasoracle_form_volume(int model_id, const double *p)

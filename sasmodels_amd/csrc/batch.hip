@@ -402,7 +402,10 @@ int record_width(int id, int kind)
     case 7: return rv_of<sas::TriaxialEllipsoid>(kind);
     case 8: return rv_of<sas::BccParacrystal>(kind);
     case 9: return rv_of<sas::FccParacrystal>(kind);
-    default: return rv_of<sas::ScParacrystal>(kind);
+    case 10: return rv_of<sas::ScParacrystal>(kind);
+    case 11: return rv_of<sas::HollowCylinder>(kind);
+    case 12: return rv_of<sas::Vesicle>(kind);
+    default: return rv_of<sas::FuzzySphere>(kind);
     }
 }
 
@@ -626,7 +629,10 @@ int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, 
     case 7: return enqueue<sas::TriaxialEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 8: return enqueue<sas::BccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 9: return enqueue<sas::FccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    default: return enqueue<sas::ScParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 10: return enqueue<sas::ScParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 11: return enqueue<sas::HollowCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 12: return enqueue<sas::Vesicle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    default: return enqueue<sas::FuzzySphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }
 

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
     __shared__ int wave_cnt[EXPAND_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    double acc_w = 0.0, acc_v = 0.0, acc_r = 0.0;
+    double acc_w = 0.0, acc_v = 0.0, acc_s = 0.0, acc_r = 0.0;
     int running = 0;
     for (int b0 = pd_start; b0 < pd_stop; b0 += EXPAND_THREADS) {
         const int step = b0 + tid;
@@ -155,6 +155,7 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
                 const double vol = M::volume(p);
                 acc_w += weight;
                 acc_v += weight * vol;
+                if constexpr (M::HOLLOW) acc_s += weight * M::shell_volume(p);
                 if (a.reff_mode != 0) acc_r += weight * M::reff(a.reff_mode, p);
 #pragma unroll
                 for (int i = 0; i < 4 * RV; i++) rec[i] = 0.0f;
@@ -222,16 +223,16 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
     }
 
     // fixed-order FP64 tree reduction of the normalisation sums
-    __shared__ double red[3][EXPAND_THREADS];
+    __shared__ double red[4][EXPAND_THREADS];
     red[0][tid] = acc_w;
     red[1][tid] = acc_v;
-    red[2][tid] = acc_r;
+    red[2][tid] = acc_s;
+    red[3][tid] = acc_r;
     __syncthreads();
     for (int s = EXPAND_THREADS / 2; s > 0; s >>= 1) {
         if (tid < s) {
-            red[0][tid] += red[0][tid + s];
-            red[1][tid] += red[1][tid + s];
-            red[2][tid] += red[2][tid + s];
+#pragma unroll
+            for (int k = 0; k < 4; k++) red[k][tid] += red[k][tid + s];
         }
         __syncthreads();
     }
@@ -239,8 +240,8 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
         a.npts[set] = running;
         a.sums[4 * set + 0] = red[0][0];
         a.sums[4 * set + 1] = red[1][0];
-        a.sums[4 * set + 2] = red[1][0];   // shell volume == form volume for these solids
-        a.sums[4 * set + 3] = red[2][0];
+        a.sums[4 * set + 2] = M::HOLLOW ? red[2][0] : red[1][0];   // solids: shell == form
+        a.sums[4 * set + 3] = red[3][0];
     }
 }
 

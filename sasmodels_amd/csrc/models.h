@@ -18,7 +18,7 @@
 // ellipsoid.c:1-105, core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30
 // with lib/core_shell.c:8-30, parallelepiped.c:1-150,
 // triaxial_ellipsoid.c:1-160, bcc/fcc/sc_paracrystal.c with
-// lib/sphere_form.c.  Parameter order: sphere.py:66-72,
+// lib/sphere_form.c, hollow_cylinder.c, vesicle.c, fuzzy_sphere.c.  Parameter order: sphere.py:66-72,
 // cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 // core_shell_sphere.py:78-82.
 #pragma once
@@ -97,6 +97,7 @@ struct Sphere {
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
     static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
+    static constexpr bool HOLLOW = false;    // shell volume differs from the form volume
     static constexpr int REC1D = 4, REC2D = 4;
     static constexpr int A2D = 3;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // sld, sld_solvent
@@ -140,6 +141,7 @@ struct Cylinder {
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
     static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
+    static constexpr bool HOLLOW = false;    // shell volume differs from the form volume
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
@@ -236,6 +238,7 @@ struct Ellipsoid {
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
     static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
+    static constexpr bool HOLLOW = false;    // shell volume differs from the form volume
     static constexpr int REC1D = 4, REC2D = 8;
     static constexpr int A2D = 5;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }
@@ -315,6 +318,7 @@ struct CoreShellCylinder {
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
     static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
+    static constexpr bool HOLLOW = false;    // shell volume differs from the form volume
     static constexpr int REC1D = 8, REC2D = 12;
     static constexpr int A2D = 10;  // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return k; }            // core, shell, solvent
@@ -435,6 +439,7 @@ struct CoreShellSphere {
     static constexpr bool ABC = false;       // three view angles (theta, phi, psi)
     static constexpr bool FQ = true;         // 1D kernel returns <F> as well as <F^2>
     static constexpr bool FP64 = false;      // 1D evaluation in FP64 (eval1d_d)
+    static constexpr bool HOLLOW = false;    // shell volume differs from the form volume
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 6;   // slot of w*1e-4 in the 2D record
     SAS_HD static int sld_param(int k) { return 2 + k; }        // core, shell, solvent
@@ -494,6 +499,7 @@ struct Parallelepiped {
     static constexpr bool ABC = true;
     static constexpr bool FQ = true;
     static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = false;
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 7;
     SAS_HD static int sld_param(int k) { return k; }
@@ -609,6 +615,7 @@ struct TriaxialEllipsoid {
     static constexpr bool ABC = true;
     static constexpr bool FQ = true;
     static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = false;
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 7;
     SAS_HD static int sld_param(int k) { return k; }
@@ -771,6 +778,7 @@ struct Paracrystal {
     static constexpr bool ABC = true;
     static constexpr bool FQ = false;
     static constexpr bool FP64 = true;
+    static constexpr bool HOLLOW = false;
     static constexpr int REC1D = 8, REC2D = 8;
     static constexpr int A2D = 7;
     SAS_HD static int sld_param(int k) { return 3 + k; }
@@ -838,5 +846,196 @@ struct Paracrystal {
 using BccParacrystal = Paracrystal<BCC>;
 using FccParacrystal = Paracrystal<FCC>;
 using ScParacrystal = Paracrystal<SC>;
+
+// ---------------------------------------------------------------- hollow cylinder
+// p = radius, thickness, length, sld, sld_solvent, theta, phi
+// (hollow_cylinder.py:90-98; hollow_cylinder.c:1-140).  The amplitude is
+// normalised by the shell (wall) volume; the 1D orientation rule runs over
+// cos(theta) = (z + 1)/2 (tables a = sin, b = cos, plain w).
+struct HollowCylinder {
+    static constexpr int ID = 11, NPARS = 7, NMAG = 2, MAX_PD = 5, NCON = 1;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = false;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = true;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 7;
+    SAS_HD static int sld_param(int k) { return 3 + k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return PI_D * p[2] * (p[0] + p[1]) * (p[0] + p[1]); }
+    __device__ static double shell_volume(const double *p)
+    {
+        return PI_D * p[2] * ((p[0] + p[1]) * (p[0] + p[1]) - p[0] * p[0]);
+    }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double ro = p[0] + p[1], l = p[2];
+        switch (mode) {
+        default:
+        case 1: return 0.5 * cbrt(0.75 * ro * (2.0 * ro * l + (ro + l) * (PI_D * ro + l)));
+        case 2: return cbrt(PI_D * ro * ro * l / FOUR_PI_3_D);
+        case 3: return ro;
+        case 4: return 0.5 * l;
+        case 5: return ro < 0.5 * l ? ro : 0.5 * l;
+        case 6: return ro > 0.5 * l ? ro : 0.5 * l;
+        case 7: return sqrt(ro * ro + 0.25 * l * l);
+        }
+    }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double ro = p[0] + p[1], g2 = (p[0] / ro) * (p[0] / ro);
+        const double s = (p[3] - p[4]) * shell_volume(p);
+        r[0] = (float)ro;
+        r[1] = (float)p[0];
+        r[2] = (float)(0.5 * p[2]);
+        r[3] = (float)g2;
+        r[4] = (float)(1.0 / (1.0 - g2));
+        r[5] = (float)(w * 1e-2 * s * 0.5);
+        r[6] = (float)(w * 1e-4 * s * s * 0.5);
+        r[7] = 0.0f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int off, n;
+        quad_range<FIXED>(t, q * fmaxf(r[0], r[2]), off, n);
+        const float qo = q * r[0], qi = q * r[1], qh = q * r[2];
+        float a1 = 0.0f, a2 = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float st = t.a[off + i];
+            const float psi = (j1c_f(qo * st) - r[3] * j1c_f(qi * st)) * r[4];
+            const float f = psi * sinc_prod(qh, t.b[off + i]);
+            const float wf = t.w[off + i] * f;
+            a1 += wf;
+            a2 = fmaf(wf, f, a2);
+        }
+        f1 = r[5] * a1;
+        f2 = r[6] * a2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        put_d(r, 0, p[0] + p[1]);
+        put_d(r, 2, p[0]);
+        put_d(r, 4, 0.5 * p[2]);
+        r[6] = (float)shell_volume(p);
+        r[7] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double qab = sqrt(q.qa * q.qa + q.qb * q.qb);
+        const double ro = get_d(r, 0), ri = get_d(r, 2);
+        const double g2 = (ri / ro) * (ri / ro);
+        const float psi = (float)((j1c_d(qab * ro) - g2 * j1c_d(qab * ri)) / (1.0 - g2));
+        g[0] = r[6] * psi * sinc_d(q.qc * get_d(r, 4));
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- vesicle
+// p = sld, sld_solvent, volfraction, radius, thickness (vesicle.py:87-96;
+// vesicle.c:1-50): a shell of sld between radius and radius + thickness,
+// solvent inside and out; normalised by the shell volume.
+struct Vesicle {
+    static constexpr int ID = 12, NPARS = 5, NMAG = 2, MAX_PD = 2, NCON = 1;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = false;
+    static constexpr bool ABC = false;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = true;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 6;
+    SAS_HD static int sld_param(int k) { return k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[3] + p[4]); }
+    __device__ static double shell_volume(const double *p)
+    {
+        return FOUR_PI_3_D * (cube_d(p[3] + p[4]) - cube_d(p[3]));
+    }
+    __device__ static double reff(int, const double *p) { return p[3] + p[4]; }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double drho = p[0] - p[1];
+        r[0] = (float)p[3];
+        r[1] = (float)(p[3] + p[4]);
+        r[2] = (float)(FOUR_PI_3_D * cube_d(p[3]));
+        r[3] = (float)volume(p);
+        r[4] = (float)(w * 1e-2 * sqrt(p[2]) * drho);
+        r[5] = (float)(w * 1e-4 * p[2] * drho * drho);
+        r[6] = r[7] = 0.0f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &, float &f1, float &f2)
+    {
+        const float g = r[3] * sph_j1c_f(q * r[1]) - r[2] * sph_j1c_f(q * r[0]);
+        f1 = r[4] * g;
+        f2 = r[5] * g * g;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        put_d(r, 0, p[3]);
+        put_d(r, 2, p[3] + p[4]);
+        r[4] = (float)(FOUR_PI_3_D * cube_d(p[3]));
+        r[5] = (float)volume(p);
+        r[6] = (float)(w * 1e-4 * p[2]);
+        r[7] = 0.0f;
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        g[0] = r[5] * sph_j1c_d(q.q * get_d(r, 2)) - r[4] * sph_j1c_d(q.q * get_d(r, 0));
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- fuzzy sphere
+// p = sld, sld_solvent, radius, fuzziness (fuzzy_sphere.py:113-117;
+// fuzzy_sphere.c:1-35): the sphere amplitude damped by exp(-(q fuzziness)^2/2).
+struct FuzzySphere {
+    static constexpr int ID = 13, NPARS = 4, NMAG = 2, MAX_PD = 2, NCON = 1;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = false;
+    static constexpr bool ABC = false;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = false;
+    static constexpr int REC1D = 4, REC2D = 8;
+    static constexpr int A2D = 5;
+    SAS_HD static int sld_param(int k) { return k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[2]); }
+    __device__ static double reff(int mode, const double *p) { return mode == 2 ? p[2] + p[3] : p[2]; }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double s = (p[0] - p[1]) * volume(p);
+        r[0] = (float)p[2];
+        r[1] = (float)p[3];
+        r[2] = (float)(w * 1e-2 * s);
+        r[3] = (float)(w * 1e-4 * s * s);
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &, float &f1, float &f2)
+    {
+        const float qf = q * r[1];
+        const float f = sph_j1c_f(q * r[0]) * expf(-0.5f * qf * qf);
+        f1 = r[2] * f;
+        f2 = r[3] * f * f;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        put_d(r, 0, p[2]);
+        put_d(r, 2, p[3]);
+        r[4] = (float)volume(p);
+        r[5] = (float)(w * 1e-4);
+        r[6] = r[7] = 0.0f;
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double qf = q.q * get_d(r, 2);
+        g[0] = r[4] * sph_j1c_d(q.q * get_d(r, 0)) * (float)exp(-0.5 * qf * qf);
+    }
+    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
 
 }  // namespace sas

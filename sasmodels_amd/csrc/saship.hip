@@ -305,13 +305,13 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
     // orientation-quadrature tables (the models whose 1D kernel averages orientation)
     const bool cyl_family = model_id == 1 || model_id == 3;
     const bool ell = model_id == 2;
-    const bool ppd = model_id == 6, tri = model_id == 7;
+    const bool ppd = model_id == 6, tri = model_id == 7, hollow = model_id == 11;
     std::vector<int> sizes;
     if (quad_n > 0) sizes = {quad_n};
     else sizes = {20, 76, 500, 5000};
     std::vector<float> a, b, w, c, d;
     std::vector<int> offs;
-    if (cyl_family || ell || ppd || tri) {
+    if (cyl_family || ell || ppd || tri || hollow) {
         for (int n : sizes) {
             std::vector<double> z, wt;
             gauss_legendre(n, z, wt);
@@ -326,6 +326,10 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
                     tw = (float)(wt[i] * sin(th));
                 } else if (ell) {
                     ta = (float)(u * u);
+                } else if (hollow) {
+                    // cos(theta) = u on [0, 1]
+                    ta = (float)sqrt(1.0 - u * u);
+                    tb = (float)u;
                 } else if (ppd) {
                     ta = (float)sin(th);
                     tb = (float)cos(th);
@@ -502,7 +506,10 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     case 7: rc = launch<sas::TriaxialEllipsoid>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     case 8: rc = launch<sas::BccParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     case 9: rc = launch<sas::FccParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    default: rc = launch<sas::ScParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 10: rc = launch<sas::ScParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 11: rc = launch<sas::HollowCylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    case 12: rc = launch<sas::Vesicle>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    default: rc = launch<sas::FuzzySphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
     }
     if (rc) return rc;
     return finish_call(k, rbytes, result);

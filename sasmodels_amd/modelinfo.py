@@ -18,7 +18,8 @@ The parameter tables are the reference's own (sphere.py:66-72,
 cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 core_shell_sphere.py:78-82, parallelepiped.py:216-231,
 triaxial_ellipsoid.py:143-158, bcc_paracrystal.py:186-194,
-fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hardsphere.py:84-88),
+fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hollow_cylinder.py:90-98,
+vesicle.py:87-96, fuzzy_sphere.py:113-117, hardsphere.py:84-88),
 with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
 orientation mode, ``structure_factor``).
@@ -303,6 +304,32 @@ _MODEL_TABLE = {
                                     ("Particle radius", "Particle scattering length density")),
     "sc_paracrystal": _paracrystal(10, [0.0, inf], (3.0, [0.0, inf]), (6.3, [0.0, inf]), 0,
                                    ("Radius of sphere", "Sphere scattering length density")),
+    "hollow_cylinder": (11, [
+        ["radius", "Ang", 20.0, [0, inf], "volume", "Cylinder core radius"],
+        ["thickness", "Ang", 10.0, [0, inf], "volume", "Cylinder wall thickness"],
+        ["length", "Ang", 400.0, [0, inf], "volume", "Cylinder total length"],
+        ["sld", "1e-6/Ang^2", 6.3, [-inf, inf], "sld", "Cylinder sld"],
+        ["sld_solvent", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Solvent sld"],
+        ["theta", "degrees", 90, [-180, 180], "orientation", "Cylinder axis to beam angle"],
+        ["phi", "degrees", 0, [-360, 360], "orientation", "Rotation about beam"],
+    ], dict(radius_effective_modes=[
+                "excluded volume", "equivalent outer volume sphere", "outer radius", "half length",
+                "half outer min dimension", "half outer max dimension", "half outer diagonal"],
+            xy_mode="qac", has_quadrature=True)),
+    "vesicle": (12, [
+        ["sld", "1e-6/Ang^2", 0.5, [-inf, inf], "sld", "vesicle shell scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 6.36, [-inf, inf], "sld", "solvent scattering length density"],
+        ["volfraction", "", 0.05, [0, 1.0], "", "volume fraction of vesicles"],
+        ["radius", "Ang", 100, [0, inf], "volume", "radius of solvent filled core"],
+        ["thickness", "Ang", 30, [0, inf], "volume", "thickness of vesicle shell"],
+    ], dict(radius_effective_modes=["outer radius"], xy_mode="qa", has_quadrature=False)),
+    "fuzzy_sphere": (13, [
+        ["sld", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Scattering length density of sphere"],
+        ["sld_solvent", "1e-6/Ang^2", 3, [-inf, inf], "sld", "Scattering length density of solvent"],
+        ["radius", "Ang", 60, [0, inf], "volume", "Pseudo-radius of sphere"],
+        ["fuzziness", "Ang", 10, [0, inf], "volume", "Std dev of the fuzzy interface"],
+    ], dict(radius_effective_modes=["radius", "radius + fuzziness"], xy_mode="qa",
+            has_quadrature=False)),
     # structure factor: Iq only (no F), no volume, nothing polydisperse;
     # evaluated in FP64 (the reference marks it double precision only)
     "hardsphere": (5, [

@@ -128,8 +128,38 @@ SC_PARACRYSTAL = [
     ({"theta": 10.0, "phi": 20, "psi": 30.0}, (0.023, 0.045), 0.0177333171285),
 ]
 
+# hollow_cylinder.py:124-166 (the Fq entry's R_eff, shell volume and
+# form:shell ratio written out for the default radius 20, thickness 10, length 400)
+_hc_r, _hc_t, _hc_l = 20.0, 10.0, 400.0
+HOLLOW_CYLINDER = [
+    ({}, 0.00005, 1764.926),
+    ({}, 0.1, None, None,
+     0.5*(0.75*(_hc_r + _hc_t)*(2.0*(_hc_r + _hc_t)*_hc_l
+                               + ((_hc_r + _hc_t) + _hc_l)*(pi*(_hc_r + _hc_t) + _hc_l)))**(1./3.),
+     pi*((_hc_r + _hc_t)**2 - _hc_r**2)*_hc_l,
+     (_hc_r + _hc_t)**2/((_hc_r + _hc_t)**2 - _hc_r**2)),
+    ({}, 0.001, 1756.76),
+    ({}, (0.1*cos(pi/6.0), 0.1*sin(pi/6.0)), 2.36885476192),
+]
+
+# vesicle.py:118-124
+VESICLE = [
+    ({}, 0.0005, 859.916526646),
+    ({}, 0.100600200401, 1.77063682331),
+    ({}, 0.5, 0.00355351388906),
+    ({}, 0.1, None, None, 130., None, 1./0.54483386436),
+]
+
+# fuzzy_sphere.py:134-140
+FUZZY_SPHERE = [
+    ({}, 0.00301005, 359.2315),
+]
+
 MODEL_KATS = {
     "sphere": SPHERE,
+    "hollow_cylinder": HOLLOW_CYLINDER,
+    "vesicle": VESICLE,
+    "fuzzy_sphere": FUZZY_SPHERE,
     "bcc_paracrystal": BCC_PARACRYSTAL,
     "fcc_paracrystal": FCC_PARACRYSTAL,
     "sc_paracrystal": SC_PARACRYSTAL,

@@ -57,6 +57,15 @@ RANDOM_1D = {
                                       radius=r.uniform(10, 40), radius_pd=0.1, radius_pd_n=3),
     "sc_paracrystal": lambda r: dict(dnn=r.uniform(100, 300), d_factor=r.uniform(0.03, 0.3),
                                      radius=r.uniform(10, 40), radius_pd=0.1, radius_pd_n=3),
+    "hollow_cylinder": lambda r: dict(radius=r.uniform(5, 80), thickness=r.uniform(2, 30), length=r.uniform(20, 800),
+                                      sld=r.uniform(-1, 7), radius_pd=0.1, radius_pd_n=int(r.integers(1, 8)),
+                                      thickness_pd=0.1, thickness_pd_n=3),
+    "vesicle": lambda r: dict(radius=r.uniform(20, 200), thickness=r.uniform(5, 50), volfraction=r.uniform(0.01, 0.5),
+                              sld=r.uniform(-1, 7), radius_pd=0.1, radius_pd_n=int(r.integers(1, 20)),
+                              thickness_pd=0.1, thickness_pd_n=5),
+    "fuzzy_sphere": lambda r: dict(radius=r.uniform(10, 200), fuzziness=r.uniform(1, 30), sld=r.uniform(-1, 7),
+                                   radius_pd=0.15, radius_pd_n=int(r.integers(1, 30)), fuzziness_pd=0.1,
+                                   fuzziness_pd_n=3),
 }
 
 
@@ -105,6 +114,12 @@ RANDOM_2D = {
     "sc_paracrystal": lambda r: dict(dnn=r.uniform(150, 300), d_factor=r.uniform(0.05, 0.2), radius=r.uniform(20, 50),
                                      theta=r.uniform(0, 90), phi=r.uniform(0, 180), psi=r.uniform(0, 180),
                                      radius_pd=0.1, radius_pd_n=3),
+    "hollow_cylinder": lambda r: dict(radius=r.uniform(10, 60), thickness=r.uniform(5, 20), length=r.uniform(50, 400),
+                                      theta=r.uniform(0, 90), phi=r.uniform(0, 180), theta_pd=8, theta_pd_n=3,
+                                      phi_pd=10, phi_pd_n=3, radius_pd=0.1, radius_pd_n=3),
+    "vesicle": lambda r: dict(radius=r.uniform(20, 150), thickness=r.uniform(5, 40), radius_pd=0.1, radius_pd_n=5),
+    "fuzzy_sphere": lambda r: dict(radius=r.uniform(20, 150), fuzziness=r.uniform(2, 20), radius_pd=0.1,
+                                   radius_pd_n=5),
 }
 
 
@@ -148,6 +163,12 @@ MAGNETIC = {
                                up_frac_f=0.1, up_theta=60, up_phi=20),
     "bcc_paracrystal": dict(dnn=200, d_factor=0.1, radius=40, theta=30, phi=40, psi=50, sld_M0=2.0,
                             sld_mtheta=45, up_frac_i=0.3, up_frac_f=0.8, up_theta=20),
+    "hollow_cylinder": dict(radius=30, thickness=8, length=200, theta=50, phi=20, sld_M0=2.5, sld_mphi=30,
+                            up_frac_i=0.1, up_frac_f=0.9, up_theta=45),
+    "vesicle": dict(radius=80, thickness=25, sld_M0=1.5, sld_solvent_M0=0.7, sld_solvent_mtheta=60,
+                    up_frac_i=0.4, up_frac_f=0.6, up_theta=10),
+    "fuzzy_sphere": dict(radius=60, fuzziness=8, sld_M0=3, sld_mtheta=20, sld_mphi=70, up_frac_i=0.8,
+                         up_frac_f=0.3, up_theta=80),
 }
 
 

@@ -250,6 +250,11 @@ def test_product_batch_2d_and_smeared():
     ("triaxial_ellipsoid", dict(radius_equat_minor=20.0, radius_equat_major=60.0, radius_polar=120.0,
                                 radius_polar_pd=0.1, radius_polar_pd_n=5, volfraction=0.2,
                                 structure_factor_mode=1, radius_effective_mode=1)),
+    # hollow shapes: S at volfraction x form:shell ratio, number density by the shell volume
+    ("vesicle", dict(radius=60.0, thickness=15.0, radius_pd=0.1, radius_pd_n=9, volfraction=0.2,
+                     structure_factor_mode=1, radius_effective_mode=1)),
+    ("hollow_cylinder", dict(radius=20.0, thickness=8.0, length=200.0, volfraction=0.25,
+                             structure_factor_mode=0, radius_effective_mode=2)),
 ])
 def test_product_qabc_shapes(p_name, pars):
     q = np.logspace(-3, np.log10(0.5), 80)

@@ -123,10 +123,14 @@ def oracle_product(p_name, q, pars):
     reff = pars.pop("radius_effective", 50.0)
     beta = pars.pop("structure_factor_mode", 0) > 0
     er_mode = int(pars.pop("radius_effective_mode", 1))
+    in_p = "volfraction" in modelinfo.make_model_info(p_name).parameters
+    if in_p:
+        # P's own volfraction (vesicle): P and S both see the product's value
+        pars["volfraction"] = volfrac
     kern, det, val, mag = _p_fq(p_name, q, pars)
     fq = kern.Fq(det, val, 0.0, mag, er_mode)
-    I, S = oracle.product(np.asarray(q, "d"), fq, scale, background, reff, volfrac, beta, er_mode)
-    return I, S, fq, scale*volfrac/fq[3]
+    I, S = oracle.product(np.asarray(q, "d"), fq, scale, background, reff, volfrac, beta, er_mode, in_p)
+    return I, S, fq, (scale if in_p else scale*volfrac)/fq[3]
 
 
 @pytest.mark.parametrize("case", PRODUCT_KATS, ids=lambda c: "%s-%s" % (c[0], sorted(c[1].items())))
@@ -191,3 +195,28 @@ def test_product_kernel_split_reproduces_kats(case):
             assert all(is_near(w, g) for w, g in zip(case[4][key], parts[key][1])), key
         for key in ("volume", "volume_ratio", "radius_effective"):
             assert is_near(case[4][key], parts[key]), key
+
+
+def test_product_kernel_split_volfraction_in_p():
+    """A form factor with its own volfraction (vesicle): P keeps it, S's
+    volfraction slot borrows P's distribution slot (product.py:478-486), and
+    the number density is scale/V_shell (product.py:513-517); hollow, so the
+    S volume fraction carries the form:shell ratio."""
+    q = np.logspace(-3, np.log10(0.5), 40)
+    info = core.load_model_info("vesicle@hardsphere")
+    names = [p.id for p in info.parameters.call_parameters]
+    assert names[2:10] == ["sld", "sld_solvent", "volfraction", "radius", "thickness", "radius_effective",
+                           "structure_factor_mode", "radius_effective_mode"]
+    kl = type("K", (), {"info": info, "dtype": np.dtype("d")})()
+    for beta, er in ((1, 1), (0, 0)):
+        pars = dict(radius=60.0, thickness=15.0, radius_pd=0.1, radius_pd_n=9, thickness_pd=0.1,
+                    thickness_pd_n=3, volfraction=0.2, structure_factor_mode=beta, radius_effective_mode=er,
+                    scale=1.3, background=0.01)
+        pk = _OracleP("vesicle", q)
+        pk.q_input = None
+        kernel = product.ProductKernel(info, pk, _OracleS(q), [q])
+        det, val, mag = make_kernel_args(kl, direct_model.get_mesh(info, pars, dim="1d"))
+        got = kernel(det, val, 0.0, mag)
+        want, _, fq, _ = oracle_product("vesicle", q, pars)
+        assert fq[4] > 1.5                      # form:shell ratio of the hollow shape
+        np.testing.assert_allclose(got, want, rtol=1e-13)
