@@ -184,17 +184,18 @@ int saship_batch_set_resolution(saship_batch *batch, const int32_t *n_out, const
  * factor s_model (SASHIP_HARDSPHERE; 0 switches it off), combined in the
  * final kernel as ProductKernel.Iq does (sasmodels/product.py:425-528).
  * pars holds per set (radius_effective, volfraction,
- * structure_factor_mode, radius_effective_mode).  Later calls return in
- * `iq`
- *     scale*volfraction/V_shell * (<F^2> S)                 mode 0
- *     scale*volfraction/V_shell * (<F^2> + <F>^2 (S - 1))   mode > 0 (beta, 1D only)
+ * structure_factor_mode, radius_effective_mode, volfraction_in_form).
+ * Later calls return in `iq`
+ *     n * (<F^2> S)                 structure_factor_mode 0
+ *     n * (<F^2> + <F>^2 (S - 1))   structure_factor_mode > 0 (beta, 1D only)
  *   + background,
- * with S at the form's mean effective radius (radius_effective_mode > 0;
- * it must equal the call's radius_effective_mode) or at radius_effective
- * (mode 0), and at volume fraction volfraction * V_form/V_shell.  scale
- * and background are the set's values[0..1]; the raw result stays P's.
- * The five form factors have no volfraction parameter of their own, so
- * volfraction always scales the number density. */
+ * with number density n = scale*volfraction/V_shell, or scale/V_shell when
+ * volfraction_in_form is non-zero (a form factor with its own volfraction,
+ * already in its F); S at the form's mean effective radius
+ * (radius_effective_mode > 0; it must equal the call's
+ * radius_effective_mode) or at radius_effective (mode 0), and at volume
+ * fraction volfraction * V_form/V_shell.  scale and background are the
+ * set's values[0..1]; the raw result stays P's. */
 int saship_batch_set_structure(saship_batch *batch, int32_t s_model, const double *pars, int32_t nsets);
 
 /* Runs the device work of the last successful saship_batch_call again

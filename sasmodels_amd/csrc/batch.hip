@@ -237,14 +237,14 @@ __global__ __launch_bounds__(256) void k_batch_final(
     // P@S (sasmodels/product.py:425-528): S at the effective radius of the
     // form (mode > 0) or the given one, and at volfraction * form:shell
     // volume ratio; the number density is volfraction / shell volume
-    const double *sp = spars ? spars + 4 * s : nullptr;
+    const double *sp = spars ? spars + 5 * s : nullptr;
     double reff = 0.0, phi = 0.0;
     bool beta = false;
     if (sp) {
         reff = sp[3] > 0.0 ? sm[3] / total : sp[0];
         phi = sp[1] * ((sm[1] / total) / shell);
         beta = sp[2] > 0.0;
-        cscale *= sp[1];
+        if (sp[4] == 0.0) cscale *= sp[1];      // else the form carries its own volfraction
     }
     for (int e = blockIdx.y * 256 + threadIdx.x; e < nq; e += 256 * gridDim.y) {
         double f2 = 0.0, f1 = 0.0;
@@ -343,7 +343,8 @@ struct saship_batch {
     double *d_sw = nullptr;
     bool separable = true;            // use the separable path when every set allows it
     // P@S (saship_batch_set_structure): structure-factor model (0: none) and
-    // per set (radius_effective, volfraction, structure_factor_mode, radius_effective_mode)
+    // per set (radius_effective, volfraction, structure_factor_mode, radius_effective_mode,
+    // volfraction in the form factor)
     int s_model = 0;
     std::vector<double> spars;
     char *h_out = nullptr;
@@ -474,10 +475,10 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     p.structure = b->s_model != 0;
     if (p.structure) {
         for (int s = 0; s < ns; s++)
-            if (b->spars[4 * s + 2] > 0.0 && kind != 0)
+            if (b->spars[5 * s + 2] > 0.0 && kind != 0)
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": the beta approximation needs 1D q");
         p.off_spars = off;
-        off = align256(off + sizeof(double) * 4 * ns);
+        off = align256(off + sizeof(double) * 5 * ns);
     }
     p.bytes = off;
     if (int e = ensure_host(b->h_stage, b->stage_cap, p.bytes)) return e;
@@ -489,7 +490,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     memcpy(h + p.off_poff, poff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_resoff, resoff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_nchunk, nchunk.data(), sizeof(int32_t) * ns);
-    if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * 4 * ns);
+    if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * 5 * ns);
     // work list, longest items first: items are bucketed by the number of
     // mesh points their chunk can hold (an upper bound before pruning)
     int4 *it = (int4 *)(h + p.off_items);
@@ -870,7 +871,7 @@ int saship_batch_call(saship_batch *b, int32_t kind, const int32_t *details, int
     std::lock_guard<std::mutex> lock(b->mu);
     if (b->s_model)
         for (int s = 0; s < b->nsets; s++) {
-            const double er = b->spars[4 * s + 3];
+            const double er = b->spars[5 * s + 3];
             if (er != 0.0 && er != (double)radius_effective_mode)
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": structure radius_effective_mode " +
                                                std::to_string((int)er) + " differs from the call's " +
@@ -891,16 +892,16 @@ int saship_batch_set_structure(saship_batch *b, int32_t s_model, const double *p
     }
     if (!model_structure(s_model)) return fail(SASHIP_EINVAL, "model " + std::to_string(s_model) + " is not a structure factor");
     if (model_structure(b->m->id)) return fail(SASHIP_EINVAL, "the batch model is itself a structure factor");
-    if (!pars || nsets != b->nsets) return fail(SASHIP_EINVAL, "need 4 structure parameters for each of the batch's sets");
+    if (!pars || nsets != b->nsets) return fail(SASHIP_EINVAL, "need 5 structure parameters for each of the batch's sets");
     for (int s = 0; s < nsets; s++) {
-        const double *sp = pars + 4 * s;
+        const double *sp = pars + 5 * s;
         const std::string where = "set " + std::to_string(s) + ": ";
-        for (int k = 0; k < 4; k++)
+        for (int k = 0; k < 5; k++)
             if (!std::isfinite(sp[k])) return fail(SASHIP_EINVAL, where + "structure parameters must be finite");
         if (sp[3] < 0.0 || sp[3] != std::floor(sp[3]) || sp[3] > 100.0)
             return fail(SASHIP_EINVAL, where + "radius_effective_mode must be a small non-negative integer");
     }
-    b->spars.assign(pars, pars + 4 * (size_t)nsets);
+    b->spars.assign(pars, pars + 5 * (size_t)nsets);
     b->s_model = s_model;
     b->have_last = false;
     return 0;

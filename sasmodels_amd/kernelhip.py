@@ -254,15 +254,15 @@ class HipBatch:
     def set_structure(self, pars, s_model=5):
         """
         Multiply every set's form factor by a structure factor on the device
-        (saship_batch_set_structure): *pars* is an (nsets, 4) array of
+        (saship_batch_set_structure): *pars* is an (nsets, 5) array of
         (radius_effective, volfraction, structure_factor_mode,
-        radius_effective_mode) per set; None switches the product off.
-        Calls then return the P@S intensity of each set.
+        radius_effective_mode, volfraction_in_form) per set; None switches
+        the product off.  Calls then return the P@S intensity of each set.
         """
         if pars is None:
             _lib.check(_lib.lib().saship_batch_set_structure(self._handle, 0, None, 0))
             return
-        pars = np.ascontiguousarray(pars, "d").reshape(-1, 4)
+        pars = np.ascontiguousarray(pars, "d").reshape(-1, 5)
         _lib.check(_lib.lib().saship_batch_set_structure(
             self._handle, int(s_model), pars.ctypes.data, pars.shape[0]))
 

@@ -303,8 +303,6 @@ class ProductBatch:
             raise ValueError("batched products support the hardsphere structure factor")
         self.info = model.info
         self._layout = _Layout(model.info)
-        if self._layout.volfrac_in_p:
-            raise ValueError("batched products need volfraction in S")
         if any(p.id != q.id for p, q in zip(model.S.info.parameters.kernel_parameters,
                                             self._layout.s_info.parameters.kernel_parameters)):
             raise ValueError("structure factor table mismatch")
@@ -343,7 +341,7 @@ class ProductBatch:
 
     def _split(self, calls):
         lay = self._layout
-        p_calls, spars, er_modes = [], np.empty((len(calls), 4)), set()
+        p_calls, spars, er_modes = [], np.empty((len(calls), 5)), set()
         for s, (cd, values) in enumerate(calls):
             scale, background, volfrac, er_mode, beta_mode = lay.modes(values)
             if beta_mode and self.dim == "2d":
@@ -352,7 +350,8 @@ class ProductBatch:
             p_values = np.zeros(size)
             p_values[:idx.size] = values[idx]
             p_calls.append((self._p_details(cd), p_values))
-            spars[s] = (values[lay.er_index], volfrac, 1.0 if beta_mode else 0.0, er_mode)
+            spars[s] = (values[lay.er_index], volfrac, 1.0 if beta_mode else 0.0, er_mode,
+                        1.0 if lay.volfrac_in_p else 0.0)
             if er_mode > 0:
                 er_modes.add(er_mode)
         if len(er_modes) > 1:

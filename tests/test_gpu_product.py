@@ -186,12 +186,12 @@ def test_product_batch_structure_checks():
     q = np.logspace(-3, -1, 50)
     batch = HipBatch(hm, [[q], [q]])
     with pytest.raises(_lib.HipError):
-        batch.set_structure(np.zeros((3, 4)))
+        batch.set_structure(np.zeros((3, 5)))
     with pytest.raises(_lib.HipError):
-        batch.set_structure(np.array([[50, 0.2, 0, 1.5], [50, 0.2, 0, 0]]))
+        batch.set_structure(np.array([[50, 0.2, 0, 1.5, 0], [50, 0.2, 0, 0, 0]]))
     with pytest.raises(_lib.HipError):
-        batch.set_structure(np.array([[50, 0.2, 0, 0], [50, 0.2, 0, 0]]), s_model=1)
-    batch.set_structure(np.array([[50, 0.2, 0, 1], [50, 0.2, 0, 0]]))
+        batch.set_structure(np.array([[50, 0.2, 0, 0, 0], [50, 0.2, 0, 0, 0]]), s_model=1)
+    batch.set_structure(np.array([[50, 0.2, 0, 1, 0], [50, 0.2, 0, 0, 0]]))
     kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
     calls = [make_kernel_args(kl, direct_model.get_mesh(hm.info, dict(radius=40.0), dim="1d"))[:2]] * 2
     with pytest.raises(_lib.HipError):
@@ -210,6 +210,24 @@ def test_product_batch_structure_checks():
     base.release()
     with pytest.raises(_lib.HipError):
         HipBatch(HipModel(modelinfo.make_model_info("hardsphere")), [[q]])
+
+
+def test_product_batch_volfraction_in_form():
+    """vesicle@hardsphere batched: the vesicle's own volfraction stays in P, the
+    number density is scale/V_shell and S sees volfraction x form:shell ratio."""
+    rng = np.random.default_rng(33)
+    info = core.load_model_info("vesicle@hardsphere")
+    model = core.build_model(info)
+    q_list = [np.logspace(-3, np.log10(0.5), int(rng.integers(30, 120))) for _ in range(6)]
+    pars = [dict(radius=rng.uniform(30, 120), thickness=rng.uniform(5, 30), radius_pd=0.1, radius_pd_n=7,
+                 volfraction=rng.uniform(0.05, 0.3), structure_factor_mode=s % 2, radius_effective_mode=1,
+                 scale=rng.uniform(0.5, 2.0)) for s in range(6)]
+    batch = ProductBatch(model, [[q] for q in q_list])
+    print("product batch vesicle", flush=True)
+    out = batch(_calls(info, [(p, "1d") for p in pars]))
+    for q, p, got in zip(q_list, pars, out):
+        assert masked_relerr(got, oracle_product("vesicle", q, p)[0]) <= RTOL
+    batch.release()
 
 
 def test_product_batch_2d_and_smeared():
