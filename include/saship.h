@@ -48,6 +48,11 @@ extern "C" {
  * four trailing entries only (max_pd 0); Iq and Iqxy (q = |(qx, qy)|) give
  * result[i] = S(q_i) and sums (1, 1, 1, 0); evaluated in FP64 */
 #define SASHIP_HARDSPHERE 5
+/* more structure factors with the same conventions (FP64; four parameters
+ * radius_effective, volfraction, then perturb, stickiness / welldepth,
+ * wellwidth): sasmodels/models/stickyhardsphere.py, squarewell.py */
+#define SASHIP_STICKY_HARDSPHERE 14
+#define SASHIP_SQUAREWELL 15
 /* shapes with three orientation angles (theta, phi, psi; xy_mode "qabc"):
  * 1D kernels integrate the orientation with nested rules (adaptive as
  * adaptive.c with outer_n limits, or the fixed size given) */
@@ -181,10 +186,12 @@ int saship_batch_set_resolution(saship_batch *batch, const int32_t *n_out, const
                                 const double *weights, int64_t n_weights);
 
 /* P@S products on the device: the batch's form factor times structure
- * factor s_model (SASHIP_HARDSPHERE; 0 switches it off), combined in the
- * final kernel as ProductKernel.Iq does (sasmodels/product.py:425-528).
- * pars holds per set (radius_effective, volfraction,
- * structure_factor_mode, radius_effective_mode, volfraction_in_form).
+ * factor s_model (SASHIP_HARDSPHERE, SASHIP_STICKY_HARDSPHERE or
+ * SASHIP_SQUAREWELL; 0 switches it off), combined in the final kernel as
+ * ProductKernel.Iq does (sasmodels/product.py:425-528).  pars holds per
+ * set (radius_effective, volfraction, structure_factor_mode,
+ * radius_effective_mode, volfraction_in_form, s3, s4) with s3, s4 the
+ * structure factor's further parameters (0 for the hard sphere).
  * Later calls return in `iq`
  *     n * (<F^2> S)                 structure_factor_mode 0
  *     n * (<F^2> + <F>^2 (S - 1))   structure_factor_mode > 0 (beta, 1D only)

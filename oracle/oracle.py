@@ -68,6 +68,9 @@ def lib():
         L.sasoracle_view_jitter_abc.argtypes = [_D]*6 + [ct.c_void_p]
         L.sasoracle_hardsphere.restype = _D
         L.sasoracle_hardsphere.argtypes = [_D, _D, _D]
+        for name in ("sasoracle_stickyhardsphere", "sasoracle_squarewell"):
+            getattr(L, name).restype = _D
+            getattr(L, name).argtypes = [_D]*5
         L.sasoracle_shell_volume.restype = _D
         L.sasoracle_shell_volume.argtypes = [ct.c_int, ct.c_void_p]
         L.sasoracle_radius_effective.restype = _D
@@ -173,20 +176,32 @@ def hardsphere(q, radius_effective, volfraction):
                     ).reshape(q.shape)
 
 
+def structure(name, q, radius_effective, volfraction, *extra):
+    """S(q) of structure factor *name* ("hardsphere", "stickyhardsphere",
+    "squarewell") with its further parameters *extra*; q scalar or array."""
+    q = np.asarray(q, "d")
+    if name == "hardsphere":
+        return hardsphere(q, radius_effective, volfraction)
+    f = getattr(lib(), "sasoracle_" + name)
+    return np.array([f(float(v), float(radius_effective), float(volfraction), *map(float, extra))
+                     for v in q.ravel()]).reshape(q.shape)
+
+
 def product(q, fq, scale, background, radius_effective, volfraction, beta_mode, er_mode,
-            volfrac_in_p=False):
+            volfrac_in_p=False, s_name="hardsphere", s_extra=()):
     """
     P@S intensity from P's normalised (F, F^2, R_eff, V_shell, V_form/V_shell)
     (sasmodels/product.py:425-528): S at P's R_eff (er_mode > 0) or
     *radius_effective*, volume fraction volfraction * V_form/V_shell;
     I = scale*volfraction/V_shell * PS + background with PS = F^2 S, or
     F^2 + F1^2 (S - 1) with beta; when P has its own volfraction (already in
-    its F) the number density is scale/V_shell.  *q* is the 1D q or the |q|
+    its F) the number density is scale/V_shell.  S is structure factor
+    *s_name* with further parameters *s_extra*.  *q* is the 1D q or the |q|
     of 2D points.  Returns (I, S).
     """
     F1, F2, reff, shell, ratio = fq
     radius = reff if er_mode > 0 else radius_effective
-    S = hardsphere(q, radius, volfraction*ratio)
+    S = structure(s_name, q, radius, volfraction*ratio, *s_extra)
     PS = F2 + F1**2*(S - 1.0) if beta_mode else F2*S
     density = scale/shell if volfrac_in_p else scale*volfraction/shell
     return density*PS + background, S
@@ -201,6 +216,10 @@ def ref_lib(name):
     if hasattr(L, "ref_sq"):
         L.ref_sq.restype = _D
         L.ref_sq.argtypes = [_D, _D, _D]
+        return L
+    if hasattr(L, "ref_sq4"):
+        L.ref_sq4.restype = _D
+        L.ref_sq4.argtypes = [_D]*5
         return L
     if hasattr(L, "ref_iq"):
         L.ref_iq.restype = _D

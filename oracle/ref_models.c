@@ -11,11 +11,11 @@
  * vector to the reference's static functions so that ctypes can call them.
  *
  * One shared object is built per (model, quadrature) pair:
- *   -DREF_MODEL=<1..14> sphere, cylinder, ellipsoid, core_shell_cylinder,
+ *   -DREF_MODEL=<1..16> sphere, cylinder, ellipsoid, core_shell_cylinder,
  *                       core_shell_sphere, hardsphere (structure factor),
  *                       parallelepiped, triaxial_ellipsoid, bcc, fcc and
  *                       sc paracrystal (with lib/gauss150.c), hollow_cylinder,
- *                       vesicle, fuzzy_sphere
+ *                       vesicle, fuzzy_sphere, stickyhardsphere, squarewell
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -138,8 +138,15 @@ double ref_form_volume(const double *p) { return form_volume(p[2]); }
 #  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3])
 #  define NPAR_VOLUME(p) form_volume(p[2], p[3])
 #  define NPAR_REFF(m, p) radius_effective(m, p[2], p[3])
+#elif REF_MODEL == 15 || REF_MODEL == 16
+#  if REF_MODEL == 15
+#    include "stickyhardsphere.c"
+#  else
+#    include "squarewell.c"
+#  endif
+double ref_sq4(double q, double r, double phi, double a, double b) { return Iq(q, r, phi, a, b); }
 #else
-#  error "REF_MODEL must be 1..14"
+#  error "REF_MODEL must be 1..16"
 #endif
 
 #ifdef NPAR_CALL_FQ

@@ -1144,6 +1144,69 @@ void sasoracle_spin_weights(double in_spin, double out_spin, double *wt)
  * with A = ((1 + 2 phi)/(1 - phi)^2)^2, B = -6 phi ((1 + phi/2)/(1 - phi)^2)^2,
  * G = phi A / 2 (the Percus-Yevick direct-correlation coefficients).
  */
+/*
+ * Sticky hard spheres (sasmodels/models/stickyhardsphere.c:1-70) and the
+ * square-well fluid (sasmodels/models/squarewell.c:1-60), restated step by
+ * step in FP64: the perturbative Baxter solution (smaller root lambda of the
+ * contact quadratic; -1 without a physical root) and the MSA square well.
+ */
+double sasoracle_stickyhardsphere(double q, double radius, double phi, double perturb, double stickiness)
+{
+    const double onemineps = 1.0 - perturb;
+    const double eta = phi / onemineps / onemineps / onemineps;
+    const double sig = 2.0 * radius;
+    const double aa = sig / onemineps;
+    const double etam1 = 1.0 - eta;
+    const double etam1sq = etam1 * etam1;
+    const double qa = eta / 6.0;
+    const double qb = stickiness + eta / etam1;
+    const double qc = (1.0 + eta / 2.0) / etam1sq;
+    double radic = qb * qb - 2.0 * qa * qc;
+    if (radic < 0) return -1.0;
+    radic = sqrt(radic);
+    double lam = (qb - radic) / qa;
+    const double lam2 = (qb + radic) / qa;
+    if (lam2 < lam) lam = lam2;
+    const double test = 1.0 + 2.0 * eta;
+    const double mu = lam * eta * etam1;
+    if (mu > test) return -1.0;
+    const double alpha = (1.0 + 2.0 * eta - mu) / etam1sq;
+    const double beta = (mu - 3.0 * eta) / (2.0 * etam1sq);
+    const double kk = q * aa, k2 = kk * kk, k3 = kk * k2;
+    const double ds = sin(kk), dc = cos(kk);
+    const double aq1 = ((ds - kk * dc) * alpha) / k3;
+    const double aq2 = (beta * (1.0 - dc)) / k2;
+    const double aq3 = (lam * ds) / (12.0 * kk);
+    const double aq = 1.0 + 12.0 * eta * (aq1 + aq2 - aq3);
+    const double bq1 = alpha * (0.5 / kk - ds / k2 + (1.0 - dc) / k3);
+    const double bq2 = beta * (1.0 / kk - ds / k2);
+    const double bq3 = (lam / 12.0) * ((1.0 - dc) / kk);
+    const double bq = 12.0 * eta * (bq1 + bq2 - bq3);
+    return 1.0 / (aq * aq + bq * bq);
+}
+
+double sasoracle_squarewell(double q, double radius, double phi, double welldepth, double wellwidth)
+{
+    const double sigma = radius * 2.;
+    const double eta = phi, eta2 = eta * eta, eta3 = eta * eta2, eta4 = eta * eta3;
+    const double etam1 = 1. - eta;
+    const double etam14 = etam1 * etam1 * etam1 * etam1;
+    double sk = 1.0 + 2.0 * eta;
+    sk *= sk;
+    const double alpha = (sk + eta3 * (eta - 4.0)) / etam14;
+    const double beta = -(eta / 3.0) * (18. + 20. * eta - 12. * eta2 + eta4) / etam14;
+    const double gamm = 0.5 * eta * (sk + eta3 * (eta - 4.)) / etam14;
+    sk = q * sigma;
+    const double sk2 = sk * sk, sk3 = sk * sk2, sk4 = sk3 * sk;
+    const double S = sin(sk), C = cos(sk), SL = sin(wellwidth * sk), CL = cos(wellwidth * sk);
+    const double t1 = alpha * sk3 * (S - sk * C);
+    const double t2 = beta * sk2 * 2.0 * (sk * S - (0.5 * sk2 - 1.) * C - 1.0);
+    const double t3 = gamm * ((4.0 * sk3 - 24. * sk) * S - (sk4 - 12.0 * sk2 + 24.0) * C + 24.0);
+    const double t4 = -welldepth * sk3 * (SL + sk * (C - wellwidth * CL) - S);
+    const double ck = -24.0 * eta * (t1 + t2 + t3 + t4) / sk3 / sk3;
+    return 1. / (1. - ck);
+}
+
 double sasoracle_hardsphere(double q, double radius, double phi)
 {
     if (fabs(radius) < 1e-12) return 1.0;

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
     const int64_t *__restrict__ voff, const int64_t *__restrict__ poff,
     const int64_t *__restrict__ resoff, const int *__restrict__ nchunk,
     const double *__restrict__ sums, const double *__restrict__ partial,
-    const double *__restrict__ spars, const double *__restrict__ q64, int is2d,
+    int s_model, const double *__restrict__ spars, const double *__restrict__ q64, int is2d,
     double *__restrict__ result, double *__restrict__ iq_out)
 {
     const int s = blockIdx.x;
@@ -237,12 +237,14 @@ __global__ __launch_bounds__(256) void k_batch_final(
     // P@S (sasmodels/product.py:425-528): S at the effective radius of the
     // form (mode > 0) or the given one, and at volfraction * form:shell
     // volume ratio; the number density is volfraction / shell volume
-    const double *sp = spars ? spars + 5 * s : nullptr;
-    double reff = 0.0, phi = 0.0;
+    const double *sp = spars ? spars + 7 * s : nullptr;
+    double sq_pars[4] = {0.0, 0.0, 0.0, 0.0};     // radius, volume fraction, extra S parameters
     bool beta = false;
     if (sp) {
-        reff = sp[3] > 0.0 ? sm[3] / total : sp[0];
-        phi = sp[1] * ((sm[1] / total) / shell);
+        sq_pars[0] = sp[3] > 0.0 ? sm[3] / total : sp[0];
+        sq_pars[1] = sp[1] * ((sm[1] / total) / shell);
+        sq_pars[2] = sp[5];
+        sq_pars[3] = sp[6];
         beta = sp[2] > 0.0;
         if (sp[4] == 0.0) cscale *= sp[1];      // else the form carries its own volfraction
     }
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
         if (sp) {
             const int64_t g = q0 + e;
             const double q = is2d ? sqrt(q64[2 * g] * q64[2 * g] + q64[2 * g + 1] * q64[2 * g + 1]) : q64[g];
-            const double S = hardsphere_sq(q, reff, phi);
+            const double S = structure_sq(s_model, q, sq_pars);
             const double F1 = f1 / total;
             v = beta ? v + F1 * F1 * (S - 1.0) : v * S;
         }
@@ -344,7 +346,7 @@ struct saship_batch {
     bool separable = true;            // use the separable path when every set allows it
     // P@S (saship_batch_set_structure): structure-factor model (0: none) and
     // per set (radius_effective, volfraction, structure_factor_mode, radius_effective_mode,
-    // volfraction in the form factor)
+    // volfraction in the form factor, the structure factor's third and fourth parameter)
     int s_model = 0;
     std::vector<double> spars;
     char *h_out = nullptr;
@@ -475,10 +477,10 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     p.structure = b->s_model != 0;
     if (p.structure) {
         for (int s = 0; s < ns; s++)
-            if (b->spars[5 * s + 2] > 0.0 && kind != 0)
+            if (b->spars[7 * s + 2] > 0.0 && kind != 0)
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": the beta approximation needs 1D q");
         p.off_spars = off;
-        off = align256(off + sizeof(double) * 5 * ns);
+        off = align256(off + sizeof(double) * 7 * ns);
     }
     p.bytes = off;
     if (int e = ensure_host(b->h_stage, b->stage_cap, p.bytes)) return e;
@@ -490,7 +492,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     memcpy(h + p.off_poff, poff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_resoff, resoff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_nchunk, nchunk.data(), sizeof(int32_t) * ns);
-    if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * 5 * ns);
+    if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * 7 * ns);
     // work list, longest items first: items are bucketed by the number of
     // mesh points their chunk can hold (an upper bound before pruning)
     int4 *it = (int4 *)(h + p.off_items);
@@ -595,7 +597,7 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
 
     k_batch_final<<<dim3(b->nsets, std::max(1, (p.max_nq + 255) / 256)), 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, d_values, d_voff, d_poff, d_resoff,
                                                 d_nchunk, b->d_sums, b->d_partial,
-                                                p.structure ? (const double *)(d + p.off_spars) : nullptr,
+                                                b->s_model, p.structure ? (const double *)(d + p.off_spars) : nullptr,
                                                 b->q64, b->is2d,
                                                 want_result ? d_result : nullptr, want_iq ? d_iq : nullptr);
     HIP_TRY(hipGetLastError());
@@ -871,7 +873,7 @@ int saship_batch_call(saship_batch *b, int32_t kind, const int32_t *details, int
     std::lock_guard<std::mutex> lock(b->mu);
     if (b->s_model)
         for (int s = 0; s < b->nsets; s++) {
-            const double er = b->spars[5 * s + 3];
+            const double er = b->spars[7 * s + 3];
             if (er != 0.0 && er != (double)radius_effective_mode)
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": structure radius_effective_mode " +
                                                std::to_string((int)er) + " differs from the call's " +
@@ -892,16 +894,16 @@ int saship_batch_set_structure(saship_batch *b, int32_t s_model, const double *p
     }
     if (!model_structure(s_model)) return fail(SASHIP_EINVAL, "model " + std::to_string(s_model) + " is not a structure factor");
     if (model_structure(b->m->id)) return fail(SASHIP_EINVAL, "the batch model is itself a structure factor");
-    if (!pars || nsets != b->nsets) return fail(SASHIP_EINVAL, "need 5 structure parameters for each of the batch's sets");
+    if (!pars || nsets != b->nsets) return fail(SASHIP_EINVAL, "need 7 structure parameters for each of the batch's sets");
     for (int s = 0; s < nsets; s++) {
-        const double *sp = pars + 5 * s;
+        const double *sp = pars + 7 * s;
         const std::string where = "set " + std::to_string(s) + ": ";
-        for (int k = 0; k < 5; k++)
+        for (int k = 0; k < 7; k++)
             if (!std::isfinite(sp[k])) return fail(SASHIP_EINVAL, where + "structure parameters must be finite");
         if (sp[3] < 0.0 || sp[3] != std::floor(sp[3]) || sp[3] > 100.0)
             return fail(SASHIP_EINVAL, where + "radius_effective_mode must be a small non-negative integer");
     }
-    b->spars.assign(pars, pars + 5 * (size_t)nsets);
+    b->spars.assign(pars, pars + 7 * (size_t)nsets);
     b->s_model = s_model;
     b->have_last = false;
     return 0;

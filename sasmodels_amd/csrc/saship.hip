@@ -60,19 +60,22 @@ __global__ __launch_bounds__(256) void k_finalize(
     }
 }
 
-// Structure-factor evaluation (hard spheres, FP64): nothing is polydisperse,
-// so the mesh is one point of weight 1 whatever pd_start/pd_stop are
-// (kernel_iq.c with MAX_PD 0), counted when 1 > cutoff.  One thread per q,
-// q = |(qx, qy)| for 2D input; the weight sums are (1, 1, 1, 0): volume 1,
-// no effective radius.  Adds to the result when accumulate is set.
-__global__ __launch_bounds__(256) void k_hardsphere(int nq, int is2d, const double *__restrict__ q64,
-                                                    const double *__restrict__ values, int include,
-                                                    int accumulate, double *__restrict__ result)
+// Structure-factor evaluation (FP64): nothing is polydisperse, so the mesh
+// is one point of weight 1 whatever pd_start/pd_stop are (kernel_iq.c with
+// MAX_PD 0), counted when 1 > cutoff.  One thread per q, q = |(qx, qy)| for
+// 2D input; the weight sums are (1, 1, 1, 0): volume 1, no effective
+// radius.  Adds to the result when accumulate is set.
+__global__ __launch_bounds__(256) void k_structure(int model, int npar, int nq, int is2d,
+                                                   const double *__restrict__ q64,
+                                                   const double *__restrict__ values, int include,
+                                                   int accumulate, double *__restrict__ result)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < nq) {
         const double q = is2d ? sqrt(q64[2 * i] * q64[2 * i] + q64[2 * i + 1] * q64[2 * i + 1]) : q64[i];
-        const double s = include ? hardsphere_sq(q, values[2], values[3]) : 0.0;
+        double p[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < npar && k < 4; k++) p[k] = values[2 + k];
+        const double s = include ? structure_sq(model, q, p) : 0.0;
         result[i] = accumulate ? result[i] + s : s;
     }
     if (blockIdx.x == 0 && threadIdx.x < 4) {
@@ -484,8 +487,8 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     if (model_structure(id)) {
         if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
         const int include = 1.0 > cutoff ? 1 : 0;
-        sas::k_hardsphere<<<std::max(1, (nq + 255) / 256), 256, 0, k->st>>>(nq, k->is2d, k->q64, d_val, include,
-                                                                            accumulate, k->d_result);
+        sas::k_structure<<<std::max(1, (nq + 255) / 256), 256, 0, k->st>>>(id, model_npars(id), nq, k->is2d, k->q64,
+                                                                           d_val, include, accumulate, k->d_result);
         HIP_TRY(hipGetLastError());
         if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
         return finish_call(k, rbytes, result);

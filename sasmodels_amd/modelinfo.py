@@ -19,7 +19,8 @@ cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 core_shell_sphere.py:78-82, parallelepiped.py:216-231,
 triaxial_ellipsoid.py:143-158, bcc_paracrystal.py:186-194,
 fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hollow_cylinder.py:90-98,
-vesicle.py:87-96, fuzzy_sphere.py:113-117, hardsphere.py:84-88),
+vesicle.py:87-96, fuzzy_sphere.py:113-117, hardsphere.py:84-88,
+stickyhardsphere.py:110-121, squarewell.py:83-92),
 with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
 orientation mode, ``structure_factor``).
@@ -335,6 +336,18 @@ _MODEL_TABLE = {
     "hardsphere": (5, [
         ["radius_effective", "Ang", 50.0, [0, inf], "", "effective radius of hard sphere"],
         ["volfraction", "", 0.2, [0, 0.74], "", "volume fraction of hard spheres"],
+    ], dict(structure_factor=True, have_Fq=False, xy_mode="qa", has_quadrature=False)),
+    "stickyhardsphere": (14, [
+        ["radius_effective", "Ang", 50.0, [0, inf], "", "effective radius of hard sphere"],
+        ["volfraction", "", 0.2, [0, 0.74], "", "volume fraction of hard spheres"],
+        ["perturb", "", 0.05, [0.01, 0.1], "", "perturbation parameter, tau"],
+        ["stickiness", "", 0.20, [-inf, inf], "", "stickiness, epsilon"],
+    ], dict(structure_factor=True, have_Fq=False, xy_mode="qa", has_quadrature=False)),
+    "squarewell": (15, [
+        ["radius_effective", "Ang", 50.0, [0, inf], "", "effective radius of hard sphere"],
+        ["volfraction", "", 0.04, [0, 0.08], "", "volume fraction of spheres"],
+        ["welldepth", "kT", 1.5, [0.0, 1.5], "", "depth of well, epsilon"],
+        ["wellwidth", "diameters", 1.2, [1.0, inf], "", "width of well in diameters (=2R) units, must be > 1"],
     ], dict(structure_factor=True, have_Fq=False, xy_mode="qa", has_quadrature=False)),
 }
 

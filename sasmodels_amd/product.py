@@ -292,15 +292,18 @@ class ProductBatch:
     :class:`ProductKernel` on that set.
 
     *model* is a :class:`ProductModel` whose P is a hip model and whose S
-    is the hard-sphere structure factor; *q_list* holds one ``[q]`` or
+    is the hard-sphere, sticky hard-sphere or square-well structure factor;
+    *q_list* holds one ``[q]`` or
     ``[qx, qy]`` per set.  Sets with radius_effective_mode > 0 in one call
     must share the mode (the batch computes one effective radius per set).
     """
 
     def __init__(self, model, q_list, use_graph=False):
         from .kernelhip import HipBatch
-        if model.S.info.kernel_id != 5:
-            raise ValueError("batched products support the hardsphere structure factor")
+        if model.S.info.kernel_id not in (5, 14, 15):
+            raise ValueError("batched products support the hardsphere, stickyhardsphere and "
+                             "squarewell structure factors")
+        self._s_model = model.S.info.kernel_id
         self.info = model.info
         self._layout = _Layout(model.info)
         if any(p.id != q.id for p, q in zip(model.S.info.parameters.kernel_parameters,
@@ -341,7 +344,7 @@ class ProductBatch:
 
     def _split(self, calls):
         lay = self._layout
-        p_calls, spars, er_modes = [], np.empty((len(calls), 5)), set()
+        p_calls, spars, er_modes = [], np.zeros((len(calls), 7)), set()
         for s, (cd, values) in enumerate(calls):
             scale, background, volfrac, er_mode, beta_mode = lay.modes(values)
             if beta_mode and self.dim == "2d":
@@ -350,8 +353,10 @@ class ProductBatch:
             p_values = np.zeros(size)
             p_values[:idx.size] = values[idx]
             p_calls.append((self._p_details(cd), p_values))
-            spars[s] = (values[lay.er_index], volfrac, 1.0 if beta_mode else 0.0, er_mode,
-                        1.0 if lay.volfrac_in_p else 0.0)
+            spars[s, :5] = (values[lay.er_index], volfrac, 1.0 if beta_mode else 0.0, er_mode,
+                            1.0 if lay.volfrac_in_p else 0.0)
+            extra = values[lay.s_values]
+            spars[s, 5:5 + len(extra)] = extra
             if er_mode > 0:
                 er_modes.add(er_mode)
         if len(er_modes) > 1:
@@ -362,7 +367,7 @@ class ProductBatch:
     def __call__(self, calls, cutoff=0.0, magnetic=False):
         """List of P@S I(q), one per set (smeared when set_resolution is on)."""
         p_calls, spars, er_mode = self._split(calls)
-        self.batch.set_structure(spars)
+        self.batch.set_structure(spars, self._s_model)
         return self.batch(p_calls, cutoff, magnetic, radius_effective_mode=er_mode)
 
     def set_resolution(self, resolutions):
@@ -374,7 +379,7 @@ class ProductBatch:
 
     def autotune(self, calls, cutoff=0.0, magnetic=False):
         p_calls, spars, _ = self._split(calls)
-        self.batch.set_structure(spars)
+        self.batch.set_structure(spars, self._s_model)
         return self.batch.autotune(p_calls, cutoff, magnetic)
 
     def replay(self, nsteps):

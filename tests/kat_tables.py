@@ -235,6 +235,18 @@ HARDSPHERE = [
      [0.001, 0.1], [0.209128, 0.930587]),
 ]
 
+# stickyhardsphere.py:145-149
+STICKYHARDSPHERE = [
+    ({"scale": 1.0, "background": 0.0, "radius_effective": 50.0, "perturb": 0.05, "stickiness": 0.2,
+      "volfraction": 0.1}, [0.001, 0.003], [1.09718, 1.087830]),
+]
+
+# squarewell.py:116-120
+SQUAREWELL = [
+    ({"scale": 1.0, "background": 0.0, "radius_effective": 50.0, "volfraction": 0.04, "welldepth": 1.5,
+      "wellwidth": 1.2}, [0.001], [0.97665742]),
+]
+
 # direct_model.py:562-603, rtol 1e-6: (model, pars, q, I)
 DIRECT_MODEL = [
     ("sphere", dict(radius=200, background=0), [0.1], [0.6190146273894904]),

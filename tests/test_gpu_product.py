@@ -186,12 +186,12 @@ def test_product_batch_structure_checks():
     q = np.logspace(-3, -1, 50)
     batch = HipBatch(hm, [[q], [q]])
     with pytest.raises(_lib.HipError):
-        batch.set_structure(np.zeros((3, 5)))
+        batch.set_structure(np.zeros((3, 7)))
     with pytest.raises(_lib.HipError):
-        batch.set_structure(np.array([[50, 0.2, 0, 1.5, 0], [50, 0.2, 0, 0, 0]]))
+        batch.set_structure(np.array([[50, 0.2, 0, 1.5, 0, 0, 0], [50, 0.2, 0, 0, 0, 0, 0]]))
     with pytest.raises(_lib.HipError):
-        batch.set_structure(np.array([[50, 0.2, 0, 0, 0], [50, 0.2, 0, 0, 0]]), s_model=1)
-    batch.set_structure(np.array([[50, 0.2, 0, 1, 0], [50, 0.2, 0, 0, 0]]))
+        batch.set_structure(np.array([[50, 0.2, 0, 0, 0, 0, 0], [50, 0.2, 0, 0, 0, 0, 0]]), s_model=1)
+    batch.set_structure(np.array([[50, 0.2, 0, 1, 0, 0, 0], [50, 0.2, 0, 0, 0, 0, 0]]))
     kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
     calls = [make_kernel_args(kl, direct_model.get_mesh(hm.info, dict(radius=40.0), dim="1d"))[:2]] * 2
     with pytest.raises(_lib.HipError):
@@ -282,3 +282,59 @@ def test_product_qabc_shapes(p_name, pars):
     got = direct_model.call_kernel(kern, pars)
     kern.release()
     assert masked_relerr(got, oracle_product(p_name, q, pars)[0]) <= RTOL
+
+
+def _s_call(info, pars):
+    n = len(pars)
+    det = make_details(info, np.ones(n, int), np.arange(n), n)
+    values = np.zeros(32)
+    values[:2 + n] = [1.0, 0.0] + list(pars)
+    values[2 + n:2 + 2*n] = pars
+    values[2 + 2*n:2 + 3*n] = 1.0
+    return det, values
+
+
+@pytest.mark.parametrize("name", ["stickyhardsphere", "squarewell"])
+def test_structure_factor_kernels(name):
+    hm = HipModel(modelinfo.make_model_info(name))
+    q = np.logspace(-4, 0, 300)
+    kern = hm.make_kernel([q])
+    rng = np.random.default_rng(7)
+    for _ in range(6):
+        r = 10**rng.uniform(0.5, 2.5)
+        pars = ([r, rng.uniform(0.01, 0.4), rng.uniform(0.01, 0.1), rng.uniform(0.05, 1.5)]
+                if name == "stickyhardsphere" else
+                [r, rng.uniform(0.005, 0.08), rng.uniform(0, 1.5), rng.uniform(1.0, 1.8)])
+        print(name, pars, flush=True)
+        got = kern(*_s_call(hm.info, pars), 0.0, False)
+        want = oracle.structure(name, q, *pars)
+        np.testing.assert_allclose(got, want, rtol=1e-9)
+    kern.release()
+    hm.release()
+
+
+@pytest.mark.parametrize("s_name,pars", [
+    ("stickyhardsphere", dict(radius=30.0, radius_pd=0.1, radius_pd_n=9, volfraction=0.15, perturb=0.05,
+                              stickiness=0.3, structure_factor_mode=1, radius_effective_mode=1)),
+    ("squarewell", dict(radius=30.0, radius_pd=0.1, radius_pd_n=9, volfraction=0.05, welldepth=1.0,
+                        wellwidth=1.3, structure_factor_mode=0, radius_effective_mode=0,
+                        radius_effective=40.0)),
+])
+def test_products_with_other_structure_factors(s_name, pars):
+    """sphere@stickyhardsphere and sphere@squarewell, one call and batched."""
+    q = np.logspace(-3, np.log10(0.5), 90)
+    model = core.load_model("sphere@" + s_name)
+    kern = model.make_kernel([q])
+    print("product sphere@" + s_name, flush=True)
+    got = direct_model.call_kernel(kern, pars)
+    kern.release()
+    want = oracle_product("sphere", q, pars, s_name)[0]
+    assert masked_relerr(got, want) <= RTOL
+    rng = np.random.default_rng(3)
+    sets = [dict(pars, radius=rng.uniform(20, 60)) for _ in range(5)]
+    batch = ProductBatch(model, [[q]]*len(sets), use_graph=True)
+    for rep in range(2):
+        out = batch(_calls(model.info, [(p, "1d") for p in sets]))
+        for p, o in zip(sets, out):
+            assert masked_relerr(o, oracle_product("sphere", q, p, s_name)[0]) <= RTOL
+    batch.release()
