@@ -31,8 +31,9 @@ launch from the counter summary in profiles/ when present.  cpu_baseline: the FP
 host cores, on a bounded sample of the workload.
 
 other_configs: the sphere 1D, pinhole-smeared cylinder, cylinder@hardsphere
-(beta), ellipsoid 2D, batched core-shell cylinder and batched magnetic
-core-shell sphere configurations, each with device throughput, FP64 accuracy on a subset and
+(beta), ellipsoid 2D, batched core-shell cylinder, batched magnetic
+core-shell sphere, batched parallelepiped (nested orientation rules) and
+batched bcc paracrystal (FP64) configurations, each with device throughput, FP64 accuracy on a subset and
 the oracle's rate on the host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
@@ -498,6 +499,43 @@ def bench_configs(threads, cpu=True):
         ent["cpu_sets_per_s"] = 2/t
     out["magnetic_core_shell_sphere_batched"] = ent
     model.release()
+
+    # more models on the same machinery: a nested-rule 1D shape and an FP64 one
+    for name, nsets, nq, mk, evals_per_point in (
+            ("parallelepiped", 256, 200,
+             lambda: dict(length_a=rng.uniform(20, 60), length_b=rng.uniform(40, 120),
+                          length_c=rng.uniform(100, 400), length_a_pd=0.1, length_a_pd_n=5,
+                          length_b_pd=0.1, length_b_pd_n=5), None),
+            ("bcc_paracrystal", 64, 200,
+             lambda: dict(dnn=rng.uniform(150, 300), d_factor=rng.uniform(0.05, 0.2), radius=rng.uniform(20, 50),
+                          radius_pd=0.1, radius_pd_n=5), 150*150)):
+        info = core.load_model_info(name)
+        model = core.build_model(info, platform="hip")
+        kl = _kernel_like(info)
+        q = np.logspace(-3, np.log10(0.5), nq)
+        calls = [make_kernel_args(kl, direct_model.get_mesh(info, mk(), dim="1d"))[:2] for _ in range(nsets)]
+        b = HipBatch(model, [[q]]*nsets, use_graph=True)
+        b.autotune(calls)
+        dt, iq = _timed(lambda: b(calls), 3)
+        b.replay(1)
+        res = b.replay(5)*1e-3/5
+        b.release()
+        npts = sum(int(d.num_eval) for d, _ in calls)
+        ent = {"workload": "%d %s sets, %d q in [1e-3, 0.5], %d mesh points in all, one persistent launch%s"
+                           % (nsets, name, nq, npts, "" if evals_per_point is None else
+                              "; FP64, Gauss-150 x 150 lattice average per (q, point)"),
+               "curves_per_s": nsets/res, "curves_per_s_host_inclusive": nsets/dt, "ms_per_launch": res*1e3,
+               "qpd_evals_per_s": nq*npts/res}
+        if evals_per_point:
+            ent["lattice_evals_per_s"] = nq*npts*evals_per_point/res
+        if cpu:
+            sub = 4
+            t, refs = _oracle_leg(info, [([q], d, v, False) for d, v in calls[:sub]], threads)
+            ent["max_rel_err_vs_fp64"] = float(max(masked_relerr(np.asarray(a), r).max()
+                                                    for a, r in zip(iq, refs)))
+            ent["cpu_curves_per_s"] = sub/t
+        out[name + "_batched"] = ent
+        model.release()
     if cpu:
         for e in out.values():
             e["cpu_threads"] = threads
