@@ -512,14 +512,31 @@ namespace sep {
 constexpr int NRMAX = 64;    // radius and length points per set (either axis)
 constexpr int RCHUNK = 8;    // bilinear mode: rows per pass
 
+// Moment series (rotation mode).  With J(x) = 2 J1(x)/x = sum_k a_k x^2k and
+// the radius grid R_i <= Rmax, r_i = R_i / Rmax,
+//   U1(qs) = sum_i u1_i J(qs R_i)   = sum_k [a_k sum_i u1_i r_i^2k] z^k,
+//   U2(qs) = sum_i u2_i J(qs R_i)^2 = sum_k [b_k sum_i u2_i r_i^2k] z^k,
+// z = (qs Rmax)^2, b = a * a (Cauchy product); likewise the length factors
+// with sin(y)/y and its square.  Each factor becomes one FP64 polynomial in z
+// whose coefficients are computed once per set; it is used while every
+// argument of the wave is <= SER_X, where the truncation and the
+// cancellation among the terms stay below 2e-9 of the factor.
+constexpr int SER_NU1 = 16, SER_NU2 = 30, SER_NT1 = 16, SER_NT2 = 26;
+constexpr int SER_U1 = 2, SER_U2 = SER_U1 + SER_NU1, SER_T1 = SER_U2 + SER_NU2, SER_T2 = SER_T1 + SER_NT1;
+constexpr int SER_STRIDE = 96;   // doubles per set: [Rmax, Ymax, U1 | U2 | T1 | T2 coefficients]
+constexpr int SER_MIN_POINTS = 4;   // fewer grid points: the per-point sums are cheaper
+constexpr double SER_X = 6.0;
+static_assert(SER_T2 + SER_NT2 <= SER_STRIDE, "series table overflows its stride");
+
 // Per-set tables, set s at ROW/COL/GRP + s*NRMAX, AT/BT + s*nlp*NRMAX,
-// dims + 4*s.  Rotation mode: ROW/COL in grid order with the grid step in
-// .w, no groups.  Prefix mode: ROW[r] = (R, u1, u2, 0) for the rows that accept
+// dims + 4*s, SER + s*SER_STRIDE.  Rotation mode: ROW/COL in grid order
+// with the grid step in .w, no groups.  Prefix mode: ROW[r] = (R, u1, u2, 0) for the rows that accept
 // at least one column, by ascending accepted count; COL[m] = (L/2, v1, v2, 0)
 // by descending weight; GRP[g] = (columns, rows) at the end of the g-th run
 // of rows with equal accepted count.  Bilinear mode: ROW[i].x = R_i,
 // COL[j].x = L_j/2 in mesh order.
 struct Tables {
+    double *SER;     // rotation mode: moment series (Rmax and Ymax +inf where unused)
     float4 *ROW;
     float4 *COL;
     int2 *GRP;
@@ -531,7 +548,8 @@ struct Tables {
 
 inline size_t tables_bytes(int nsets, int nlp)
 {
-    return (size_t)nsets * (NRMAX * (2 * sizeof(float4) + sizeof(int2) + 2 * sizeof(float) * (size_t)nlp) +
+    return (size_t)nsets * (SER_STRIDE * sizeof(double) +
+                            NRMAX * (2 * sizeof(float4) + sizeof(int2) + 2 * sizeof(float) * (size_t)nlp) +
                             4 * sizeof(int)) + 256;
 }
 
@@ -540,7 +558,8 @@ inline Tables carve(void *base, int nsets, int nlp)
 {
     Tables t;
     const size_t n = (size_t)nsets * NRMAX;
-    t.ROW = (float4 *)base;
+    t.SER = (double *)base;
+    t.ROW = (float4 *)(t.SER + (size_t)nsets * SER_STRIDE);
     t.COL = t.ROW + n;
     t.GRP = (int2 *)(t.COL + n);
     t.AT = (float *)(t.GRP + n);
@@ -549,6 +568,39 @@ inline Tables carve(void *base, int nsets, int nlp)
     t.nlp = nlp;
     return t;
 }
+
+// Power series coefficients, evaluated at compile time: 2 J1(x)/x =
+// sum_k a_k x^2k with a_k = (-1)^k / (4^k k! (k+1)!); its square (Cauchy
+// product); sin(y)/y = sum_k (-1)^k y^2k / (2k+1)!; (sin(y)/y)^2 =
+// (1 - cos 2y)/(2 y^2) = sum_k (-1)^k 2^(2k+1) y^2k / (2k+2)!.
+struct SerCoef {
+    double j1c[SER_NU2], j1c_sq[SER_NU2], sinc[SER_NT2], sinc_sq[SER_NT2];
+};
+constexpr SerCoef make_ser_coef()
+{
+    SerCoef c{};
+    double a = 1.0;
+    for (int k = 0; k < SER_NU2; k++) {
+        if (k > 0) a *= -1.0 / (4.0 * k * (k + 1));
+        c.j1c[k] = a;
+    }
+    for (int k = 0; k < SER_NU2; k++) {
+        double b = 0.0;
+        for (int j = 0; j <= k; j++) b += c.j1c[j] * c.j1c[k - j];
+        c.j1c_sq[k] = b;
+    }
+    double s = 1.0, t = 1.0;
+    for (int k = 0; k < SER_NT2; k++) {
+        if (k > 0) {
+            s *= -1.0 / ((2.0 * k) * (2.0 * k + 1.0));
+            t *= -4.0 / ((2.0 * k + 1.0) * (2.0 * k + 2.0));
+        }
+        c.sinc[k] = s;
+        c.sinc_sq[k] = t;
+    }
+    return c;
+}
+__constant__ constexpr SerCoef SER_COEF = make_ser_coef();
 
 // One workgroup per set: find the radius and length levels, walk the set's
 // mesh steps (FP64 normalisation sums as k_expand, acceptance mask and the
@@ -753,17 +805,17 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         d[2] = mode;
         d[3] = ngroups;
     }
+    // single-point levels multiply in, in mesh order like w0
+    double wo = 1.0;
+    for (int lev = MP - 1; lev >= 0; lev--)
+        if (lev != kr && lev != kl) wo = pd_weight[pd_off[lev]] * wo;
+    const double drho = base[0] - base[1];
+    const double c1 = wo * 1e-2 * drho * PI_D * PI_4_D;
+    const double c2 = wo * 1e-4 * drho * drho * PI_D * PI_D * PI_4_D;
     if (tid < NRMAX) {
         const int m = tid;
         float4 row = make_float4(0.0f, 0.0f, 0.0f, 0.0f), col = row;
         if (prefix) {
-            // single-point levels multiply in, in mesh order like w0
-            double wo = 1.0;
-            for (int lev = MP - 1; lev >= 0; lev--)
-                if (lev != kr && lev != kl) wo = pd_weight[pd_off[lev]] * wo;
-            const double drho = base[0] - base[1];
-            const double c1 = wo * 1e-2 * drho * PI_D * PI_4_D;
-            const double c2 = wo * 1e-4 * drho * drho * PI_D * PI_D * PI_4_D;
             // rotation and interval modes: grid order, grid steps in .w
             const bool rot = mode >= 2;
             if (m < (rot ? NR : NR - nzero)) {
@@ -784,6 +836,63 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         }
         ROW[m] = row;
         COL[m] = col;
+    }
+    // rotation mode: the moment series of the four factors from the same
+    // FP32-rounded grid values and weights as ROW and COL (u1 = w R^2,
+    // u2 = w R^4; v1 = w L c1, v2 = w L^2 c2): powers r^2k of the scaled
+    // grids in LDS, then one coefficient per thread
+    __shared__ double pw[2][SER_NU2][NRMAX];
+    __shared__ double gmax[2];
+    static_assert(SER_NT2 <= SER_NU2, "power table too short for the length series");
+    if (tid < 2) {
+        double m = 0.0;
+        for (int i = 0; i < (tid ? NL : NR); i++)
+            m = fmax(m, tid ? (double)(float)(0.5 * lv[i]) : (double)(float)rv[i]);
+        gmax[tid] = m;
+    }
+    __syncthreads();
+    const double rmax = gmax[0], ymax = gmax[1];
+    const bool ur = mode == 2 && NR >= SER_MIN_POINTS && rmax > 0.0;
+    const bool ul = mode == 2 && NL >= SER_MIN_POINTS && ymax > 0.0;
+    if (tid < 2 * NRMAX) {
+        const int g = tid / NRMAX, i = tid % NRMAX;
+        if (i < (g ? NL : NR) && (g ? ul : ur)) {
+            const double r = g ? (double)(float)(0.5 * lv[i]) / ymax : (double)(float)rv[i] / rmax;
+            double p = 1.0;
+            for (int k = 0; k < SER_NU2; k++) {
+                pw[g][k][i] = p;
+                p *= r * r;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < SER_STRIDE) {
+        const int c = tid;
+        double v = 0.0;
+        if (c == 0) {
+            v = ur ? rmax : __builtin_inf();
+        } else if (c == 1) {
+            v = ul ? ymax : __builtin_inf();
+        } else if (c < SER_T1 && ur) {
+            const bool sq = c >= SER_U2;
+            const int k = sq ? c - SER_U2 : c - SER_U1;
+            for (int i = 0; i < NR; i++) {
+                const double x = rv[i];
+                const double u = sq ? (double)(float)(rw[i] * x * x * x * x) : (double)(float)(rw[i] * x * x);
+                v = fma(u, pw[0][k][i], v);
+            }
+            v *= sq ? SER_COEF.j1c_sq[k] : SER_COEF.j1c[k];
+        } else if (c >= SER_T1 && c < SER_T2 + SER_NT2 && ul) {
+            const bool sq = c >= SER_T2;
+            const int k = sq ? c - SER_T2 : c - SER_T1;
+            for (int j = 0; j < NL; j++) {
+                const double x = lv[j];
+                const double u = sq ? (double)(float)(lw[j] * x * x * c2) : (double)(float)(lw[j] * x * c1);
+                v = fma(u, pw[1][k][j], v);
+            }
+            v *= sq ? SER_COEF.sinc_sq[k] : SER_COEF.sinc[k];
+        }
+        t.SER[(size_t)set * SER_STRIDE + c] = v;
     }
 }
 
@@ -809,6 +918,37 @@ SAS_HD bool wave_all(bool p)
 #endif
 }
 
+// One length point m: (s, c) = sin and cos of its argument, advanced by the
+// step rotation (sd, cd) for the next.  MIXED: some argument of the wave
+// is below 2 (Taylor series there); the two instantiations keep the series
+// out of the loops of waves that never take it (a masked-off block still
+// issues).  A run: n <= RESTART points from m0, unrolled when whole.
+template <bool MIXED, class P4>
+SAS_HD void length_point(P4 COL, int m, float qc, float &s, float &c, float sd, float cd, float &T1, float &T2)
+{
+    const float4 e = ldv(COL + m);
+    const float y = qc * e.x;
+    float S = s * rcp_f(y);
+    if (MIXED && y < 2.0f) S = sinc_taylor(y);
+    T1 = fmaf(e.y, S, T1);
+    T2 = fmaf(e.z, S * S, T2);
+    const float s2 = fmaf(s, cd, c * sd);
+    c = fmaf(c, cd, -s * sd);
+    s = s2;
+}
+
+template <bool MIXED, class P4>
+SAS_HD void length_run(P4 COL, int m0, int n, float qc, float s, float c, float sd, float cd, float &T1,
+                       float &T2)
+{
+    if (n == RESTART) {
+#pragma unroll
+        for (int t = 0; t < RESTART; t++) length_point<MIXED>(COL, m0 + t, qc, s, c, sd, cd, T1, T2);
+    } else {
+        for (int t = 0; t < n; t++) length_point<MIXED>(COL, m0 + t, qc, s, c, sd, cd, T1, T2);
+    }
+}
+
 // Length factors T1 = sum v1 S, T2 = sum v2 S^2 (grid order, step COL[0].w).
 template <class P4>
 SAS_HD void length_sums(P4 COL, int NL, float qc, float &T1, float &T2)
@@ -832,18 +972,38 @@ SAS_HD void length_sums(P4 COL, int NL, float qc, float &T1, float &T2)
     for (int m0 = 0; m0 < NL; m0 += RESTART) {
         float s, c;
         sincos_prod(qc, ldv(COL + m0).x, s, c);
-        const int m1 = m0 + RESTART < NL ? m0 + RESTART : NL;
-        for (int m = m0; m < m1; m++) {
-            const float4 e = ldv(COL + m);
-            const float y = qc * e.x;
-            float S = s * rcp_f(y);
-            if (mixed && y < 2.0f) S = sinc_taylor(y);
-            T1 = fmaf(e.y, S, T1);
-            T2 = fmaf(e.z, S * S, T2);
-            const float s2 = fmaf(s, cd, c * sd);
-            c = fmaf(c, cd, -s * sd);
-            s = s2;
-        }
+        const int n = NL - m0 < RESTART ? NL - m0 : RESTART;
+        if (mixed) length_run<true>(COL, m0, n, qc, s, c, sd, cd, T1, T2);
+        else length_run<false>(COL, m0, n, qc, s, c, sd, cd, T1, T2);
+    }
+}
+
+// One radius point and a run of them (as for the length; MIXED: some
+// argument of the wave is at most 2, polynomial branch there).
+template <bool MIXED, class P4>
+SAS_HD void radius_point(P4 ROW, int i, float qs, float &s, float &c, float sd, float cd, float &U1, float &U2)
+{
+    const float4 e = ldv(ROW + i);
+    const float x = qs * e.x;
+    float J;
+    if (MIXED && x <= 2.0f) J = j1c_small(x);
+    else J = j1c_big(x, s, c);
+    U1 = fmaf(e.y, J, U1);
+    U2 = fmaf(e.z, J * J, U2);
+    const float s2 = fmaf(s, cd, c * sd);
+    c = fmaf(c, cd, -s * sd);
+    s = s2;
+}
+
+template <bool MIXED, class P4>
+SAS_HD void radius_run(P4 ROW, int i0, int n, float qs, float s, float c, float sd, float cd, float &U1,
+                       float &U2)
+{
+    if (n == RESTART) {
+#pragma unroll
+        for (int t = 0; t < RESTART; t++) radius_point<MIXED>(ROW, i0 + t, qs, s, c, sd, cd, U1, U2);
+    } else {
+        for (int t = 0; t < n; t++) radius_point<MIXED>(ROW, i0 + t, qs, s, c, sd, cd, U1, U2);
     }
 }
 
@@ -870,19 +1030,9 @@ SAS_HD void radius_sums(P4 ROW, int NR, float qs, float &U1, float &U2)
     for (int i0 = 0; i0 < NR; i0 += RESTART) {
         float s, c;
         sincos_prod(qs, ldv(ROW + i0).x, s, c);
-        const int i1 = i0 + RESTART < NR ? i0 + RESTART : NR;
-        for (int i = i0; i < i1; i++) {
-            const float4 e = ldv(ROW + i);
-            const float x = qs * e.x;
-            float J;
-            if (mixed && x <= 2.0f) J = j1c_small(x);
-            else J = j1c_big(x, s, c);
-            U1 = fmaf(e.y, J, U1);
-            U2 = fmaf(e.z, J * J, U2);
-            const float s2 = fmaf(s, cd, c * sd);
-            c = fmaf(c, cd, -s * sd);
-            s = s2;
-        }
+        const int n = NR - i0 < RESTART ? NR - i0 : RESTART;
+        if (mixed) radius_run<true>(ROW, i0, n, qs, s, c, sd, cd, U1, U2);
+        else radius_run<false>(ROW, i0, n, qs, s, c, sd, cd, U1, U2);
     }
 }
 
@@ -901,6 +1051,58 @@ SAS_HD void rank_one_sums(P4 ROW, P4 COL, int NR, int NL, float q, float sn, flo
     float T1, T2, U1, U2;
     length_sums(COL, NL, fabsf(q * cs), T1, T2);
     radius_sums(ROW, NR, fabsf(q * sn), U1, U2);
+    f1 = U1 * T1;
+    f2 = U2 * T2;
+}
+
+// Two series in one z, FP64: P = sum_k<N1 a_k z^k and Q = sum_k<N2 b_k z^k
+// (N1 <= N2) as power sums over shared powers of z, the coefficients read
+// through the scalar cache as multiplicands (one FMA per coefficient and
+// one multiply per power; a Horner chain would first copy every
+// coefficient into vector registers).
+template <int N1, int N2>
+__device__ __forceinline__ void ser_pair(const SAS_CONST double *a, const SAS_CONST double *b, double z, double &P,
+                                         double &Q)
+{
+    static_assert(N1 <= N2, "the second series must be the longer one");
+    double p = z, u = a[0], v = b[0];
+#pragma unroll
+    for (int k = 1; k < N2; k++) {
+        if (k < N1) u = fma(a[k], p, u);
+        v = fma(b[k], p, v);
+        p *= z;
+    }
+    P = u;
+    Q = v;
+}
+
+// The rotation-mode sums with the moment series: a factor whose arguments
+// are all <= SER_X across the wave (largest argument q Rmax) is one
+// polynomial in (q Rmax)^2; otherwise the per-point sums above.  Rmax is
+// +inf where a set has no series, which fails the vote (also at q = 0).
+__device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const SAS_CONST float4 *COL,
+                                              const SAS_CONST double *ser, int NR, int NL, float q, float sn,
+                                              float cs, float &f1, float &f2)
+{
+    float T1, T2, U1, U2;
+    const float qc = fabsf(q * cs), qs = fabsf(q * sn);
+    const double yl = (double)qc * ser[1], xl = (double)qs * ser[0];
+    if (wave_all(yl <= SER_X)) {
+        double P, Q;
+        ser_pair<SER_NT1, SER_NT2>(ser + SER_T1, ser + SER_T2, yl * yl, P, Q);
+        T1 = (float)P;
+        T2 = (float)Q;
+    } else {
+        length_sums(COL, NL, qc, T1, T2);
+    }
+    if (wave_all(xl <= SER_X)) {
+        double P, Q;
+        ser_pair<SER_NU1, SER_NU2>(ser + SER_U1, ser + SER_U2, xl * xl, P, Q);
+        U1 = (float)P;
+        U2 = (float)Q;
+    } else {
+        radius_sums(ROW, NR, qs, U1, U2);
+    }
     f1 = U1 * T1;
     f2 = U2 * T2;
 }
@@ -1031,7 +1233,7 @@ __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, flo
     const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
     const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
     if (dims[2] == 2) {
-        rank_one_sums(ROW, COL, NR, NL, q, sn, cs, f1, f2);
+        rank_one_sums(ROW, COL, as_const(t.SER) + (size_t)set * SER_STRIDE, NR, NL, q, sn, cs, f1, f2);
         return;
     }
     if (dims[2] == 3) {
