@@ -56,11 +56,42 @@ struct BatchArgs {
 // arguments: with every store of a kernel going through them, the compiler
 // can read the read-only tables through the scalar cache.
 
-__device__ __forceinline__ int next_item(int *queue)
+// A wave's position in the work queue: it starts on its own counter, moves
+// on when that one runs dry and stops after finding every counter dry in a
+// row (counters only grow, so every item is handed out exactly once).
+struct QueueCursor {
+    int c, dry;
+};
+
+__device__ __forceinline__ QueueCursor queue_start()
 {
-    int item = 0;
-    if ((threadIdx.x & 63) == 0) item = atomicAdd(queue, 1);
-    return __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    // wave-uniform (the compiler cannot tell from threadIdx): keeps the
+    // counter, the item and every table address derived from it in scalars
+    const int c = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % WORK_QUEUES);
+    return QueueCursor{__builtin_amdgcn_readfirstlane(c), 0};
+}
+
+// Next item of the wave, or -1 when the queue is empty.
+__device__ __forceinline__ int queue_next(int *queue, int n_items, QueueCursor &cur)
+{
+    for (;;) {
+        const int first = (int)((int64_t)cur.c * n_items / WORK_QUEUES);
+        const int last = (int)((int64_t)(cur.c + 1) * n_items / WORK_QUEUES);
+        int got = last;
+        if ((threadIdx.x & 63) == 0) {
+            int *ctr = queue + cur.c * WORK_QUEUE_STRIDE;
+            // a stale read can only be low: then the atomic tells
+            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < last - first)
+                got = first + atomicAdd(ctr, 1);
+        }
+        const int item = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
+        if (item < last) {
+            cur.dry = 0;
+            return item;
+        }
+        if (++cur.dry == WORK_QUEUES) return -1;
+        cur.c = cur.c + 1 == WORK_QUEUES ? 0 : cur.c + 1;
+    }
 }
 
 template <class M, bool FIXED>
@@ -70,9 +101,10 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
     constexpr int RV = rv1<M>();
     const int lane = threadIdx.x & 63;
     const int n_items = *a.n_items;
+    QueueCursor cur = queue_start();
     for (;;) {
-        const int item = next_item(queue);
-        if (item >= n_items) break;
+        const int item = queue_next(queue, n_items, cur);
+        if (item < 0) break;
         const int4 it = a.items[item];
         const int s = it.x, tile = it.y, chunk = it.z;
         const int64_t q0 = a.qoff[s];
@@ -124,9 +156,10 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a, int *__restr
     constexpr int RV = rv2<M>();
     const int lane = threadIdx.x & 63;
     const int n_items = *a.n_items;
+    QueueCursor cur = queue_start();
     for (;;) {
-        const int item = next_item(queue);
-        if (item >= n_items) break;
+        const int item = queue_next(queue, n_items, cur);
+        if (item < 0) break;
         const int4 it = a.items[item];
         const int s = it.x, tile = it.y, chunk = it.z;
         const int64_t q0 = a.qoff[s];
@@ -178,16 +211,26 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
     const int lane = threadIdx.x & 63;
     const SAS_CONST int *ipre = as_const(a.ipre);
     const int n_items = ipre[a.nsets];
+    QueueCursor cur = queue_start();
+    int pos = 0;
     for (;;) {
-        const int item = next_item(queue);
-        if (item >= n_items) break;
-        int lo = 0, hi = a.nsets;            // last position p with ipre[p] <= item
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (ipre[mid] <= item) lo = mid;
-            else hi = mid;
+        const int item = queue_next(queue, n_items, cur);
+        if (item < 0) break;
+        // last position p with ipre[p] <= item: the previous item's set
+        // when it still holds (items of one counter are consecutive); kept
+        // wave-uniform so that every table read of the set is a scalar load
+        pos = __builtin_amdgcn_readfirstlane(pos);
+        if (item < ipre[pos] || item >= ipre[pos + 1]) {
+            int lo = 0, hi = a.nsets;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (ipre[mid] <= item) lo = mid;
+                else hi = mid;
+            }
+            pos = __builtin_amdgcn_readfirstlane(lo);
         }
-        const int s = as_const(a.order)[lo];
+        const int lo = pos;
+        const int s = __builtin_amdgcn_readfirstlane(as_const(a.order)[lo]);
         const int rem = item - ipre[lo];
         const int tile = rem / a.nblk, blk = rem - tile * a.nblk;
         const int64_t q0 = a.qoff[s];
@@ -758,7 +801,8 @@ int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_qoff, sizeof(int64_t) * (nsets + 1));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_npts, sizeof(int) * nsets);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_sums, sizeof(double) * 4 * nsets);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->d_queue, 256);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&b->d_queue, sizeof(int) * sas::WORK_QUEUES * sas::WORK_QUEUE_STRIDE);
     if (e == hipSuccess && nqv) e = hipMemcpy(b->q32, q32.data(), sizeof(float) * nqv, hipMemcpyHostToDevice);
     if (e == hipSuccess && nqv) e = hipMemcpy(b->q64, q, sizeof(double) * nqv, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_qoff, qoff.data(), sizeof(int64_t) * (nsets + 1), hipMemcpyHostToDevice);
@@ -955,7 +999,7 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
     int best_p = b->pts_per_item, best_w = b->waves_per_cu;
     double best_t = 1e300;
     const int pts[] = {4, 8, 16, 32, 64, 128};
-    const int waves[] = {8, 16, 32};
+    const int waves[] = {8, 16, 24, 32};
     int rc = 0;
     for (int P : pts) {
         for (int W : waves) {

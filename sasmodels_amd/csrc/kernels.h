@@ -22,6 +22,11 @@
 namespace sas {
 
 constexpr int EXPAND_THREADS = 512;
+// Work queue of the persistent batched kernels: WORK_QUEUES counters
+// WORK_QUEUE_STRIDE ints apart (separate memory channels); counter c hands
+// out the items [c n / WORK_QUEUES, (c+1) n / WORK_QUEUES).  One counter for
+// every item serialises thousands of atomics per launch.
+constexpr int WORK_QUEUES = 16, WORK_QUEUE_STRIDE = 1024;
 constexpr int EVAL_THREADS = 256;
 constexpr int TILE_PTS = 64;
 constexpr double DEG = 0.017453292519943295;   // pi/180
@@ -62,7 +67,7 @@ struct ExpandArgs {
     float4 *recs;
     int *npts;                  // [nsets] accepted points
     double *sums;               // [4*nsets] sum w, sum w*V_form, sum w*V_shell, sum w*R_eff
-    int *queue;                 // reset to 0 by workgroup 0 when not NULL
+    int *queue;                 // counters reset to 0 by workgroup 0 when not NULL
 };
 
 template <class M, int KIND>
@@ -75,7 +80,7 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
     const int32_t *det = a.det + (size_t)set * a.ndet;
     const double *values = a.values + (a.voff ? a.voff[set] : 0);
     float4 *recs = a.recs + (a.roff ? a.roff[set] : 0) * RV;
-    if (a.queue && set == 0 && threadIdx.x == 0) a.queue[0] = 0;
+    if (a.queue && set == 0 && threadIdx.x < WORK_QUEUES) a.queue[threadIdx.x * WORK_QUEUE_STRIDE] = 0;
 
     const int32_t *pd_par = det, *pd_len = det + MP, *pd_off = det + 2 * MP, *pd_stride = det + 3 * MP;
     const int num_eval = det[4 * MP];
@@ -539,6 +544,8 @@ struct Tables {
     double *SER;     // rotation mode: moment series (Rmax and Ymax +inf where unused)
     float4 *ROW;
     float4 *COL;
+    float4 *ROWB;    // rotation mode: (1/R, u1 R^-3/2, u2 R^-3, 1 in entry 0 when usable)
+    float4 *COLB;    // rotation mode: (2/L, v1 2/L, v2 (2/L)^2, 1 in entry 0 when usable)
     int2 *GRP;
     float *AT;       // bilinear mode [nlp][NRMAX]: w * 1e-2 * drho * V * pi/4 at (row, column)
     float *BT;       // bilinear mode [nlp][NRMAX]: w * 1e-4 * (drho * V)^2 * pi/4
@@ -549,7 +556,7 @@ struct Tables {
 inline size_t tables_bytes(int nsets, int nlp)
 {
     return (size_t)nsets * (SER_STRIDE * sizeof(double) +
-                            NRMAX * (2 * sizeof(float4) + sizeof(int2) + 2 * sizeof(float) * (size_t)nlp) +
+                            NRMAX * (4 * sizeof(float4) + sizeof(int2) + 2 * sizeof(float) * (size_t)nlp) +
                             4 * sizeof(int)) + 256;
 }
 
@@ -561,7 +568,9 @@ inline Tables carve(void *base, int nsets, int nlp)
     t.SER = (double *)base;
     t.ROW = (float4 *)(t.SER + (size_t)nsets * SER_STRIDE);
     t.COL = t.ROW + n;
-    t.GRP = (int2 *)(t.COL + n);
+    t.ROWB = t.COL + n;
+    t.COLB = t.ROWB + n;
+    t.GRP = (int2 *)(t.COLB + n);
     t.AT = (float *)(t.GRP + n);
     t.BT = t.AT + n * nlp;
     t.dims = (int *)(t.BT + n * nlp);
@@ -636,7 +645,7 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     __shared__ unsigned char acc[NRMAX * NRMAX];
     __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad, nzero, ngroups, mode;
     __shared__ float steps[2];
-    if (a.queue && set == 0 && tid == 0) a.queue[0] = 0;
+    if (a.queue && set == 0 && tid < WORK_QUEUES) a.queue[tid * WORK_QUEUE_STRIDE] = 0;
     for (int i = tid; i < NRMAX * nlp; i += EXPAND_THREADS) {
         AT[i] = 0.0f;
         BT[i] = 0.0f;
@@ -842,16 +851,38 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     // u2 = w R^4; v1 = w L c1, v2 = w L^2 c2): powers r^2k of the scaled
     // grids in LDS, then one coefficient per thread
     __shared__ double pw[2][SER_NU2][NRMAX];
-    __shared__ double gmax[2];
+    __shared__ double gmax[2], gmin[2];
     static_assert(SER_NT2 <= SER_NU2, "power table too short for the length series");
     if (tid < 2) {
-        double m = 0.0;
-        for (int i = 0; i < (tid ? NL : NR); i++)
-            m = fmax(m, tid ? (double)(float)(0.5 * lv[i]) : (double)(float)rv[i]);
+        double m = 0.0, mn = __builtin_inf();
+        for (int i = 0; i < (tid ? NL : NR); i++) {
+            const double v = tid ? (double)(float)(0.5 * lv[i]) : (double)(float)rv[i];
+            m = fmax(m, v);
+            mn = fmin(mn, v);
+        }
         gmax[tid] = m;
+        gmin[tid] = mn;
     }
     __syncthreads();
     const double rmax = gmax[0], ymax = gmax[1];
+    // rotation mode, grids without zero: the scaled tables of the rotation
+    // loops (J = sqrt(2) (qs R)^-3/2 j1_mc, sin(y)/y = sin(y) / (qc L/2))
+    if (tid < NRMAX) {
+        const int m = tid;
+        float4 rb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cb = rb;
+        if (mode == 2 && gmin[0] > 0.0 && m < NR) {
+            const double x = rv[m], r = 1.0 / (double)(float)x;
+            const double u1 = (double)(float)(rw[m] * x * x), u2 = (double)(float)(rw[m] * x * x * x * x);
+            rb = make_float4((float)r, (float)(u1 * r * sqrt(r)), (float)(u2 * r * r * r), m == 0 ? 1.0f : 0.0f);
+        }
+        if (mode == 2 && gmin[1] > 0.0 && m < NL) {
+            const double x = lv[m], r = 1.0 / (double)(float)(0.5 * x);
+            const double v1 = (double)(float)(lw[m] * x * c1), v2 = (double)(float)(lw[m] * x * x * c2);
+            cb = make_float4((float)r, (float)(v1 * r), (float)(v2 * r * r), m == 0 ? 1.0f : 0.0f);
+        }
+        t.ROWB[(size_t)set * NRMAX + m] = rb;
+        t.COLB[(size_t)set * NRMAX + m] = cb;
+    }
     const bool ur = mode == 2 && NR >= SER_MIN_POINTS && rmax > 0.0;
     const bool ul = mode == 2 && NL >= SER_MIN_POINTS && ymax > 0.0;
     if (tid < 2 * NRMAX) {
@@ -1055,34 +1086,137 @@ SAS_HD void rank_one_sums(P4 ROW, P4 COL, int NR, int NL, float q, float sn, flo
     f2 = U2 * T2;
 }
 
-// Two series in one z, FP64: P = sum_k<N1 a_k z^k and Q = sum_k<N2 b_k z^k
-// (N1 <= N2) as power sums over shared powers of z, the coefficients read
-// through the scalar cache as multiplicands (one FMA per coefficient and
-// one multiply per power; a Horner chain would first copy every
-// coefficient into vector registers).
-template <int N1, int N2>
-__device__ __forceinline__ void ser_pair(const SAS_CONST double *a, const SAS_CONST double *b, double z, double &P,
-                                         double &Q)
+// Two series in one z, FP64: P = sum_k<N1 a_k z^k and Q = sum_k<N2 b_k z^k.
+// Blocks of 8 coefficients are power sums over the shared powers z..z^7
+// (the coefficients, read through the scalar cache, enter as
+// multiplicands: one FMA each), and the blocks combine by Horner in z^8.
+// Fewer vector registers than one power sum over every power, and no copy
+// of each coefficient into vector registers as a plain Horner chain needs.
+template <int N, class PD>
+__device__ __forceinline__ double ser_blocks(PD c, const double *zp, double z8)
 {
-    static_assert(N1 <= N2, "the second series must be the longer one");
-    double p = z, u = a[0], v = b[0];
+    double u = 0.0;
 #pragma unroll
-    for (int k = 1; k < N2; k++) {
-        if (k < N1) u = fma(a[k], p, u);
-        v = fma(b[k], p, v);
-        p *= z;
+    for (int b = (N - 1) / 8; b >= 0; b--) {
+        double blk = c[8 * b];
+#pragma unroll
+        for (int k = 1; k < 8; k++)
+            if (8 * b + k < N) blk = fma(c[8 * b + k], zp[k], blk);
+        u = b == (N - 1) / 8 ? blk : fma(u, z8, blk);
     }
-    P = u;
-    Q = v;
+    return u;
+}
+
+template <int N1, int N2, class PD>
+__device__ __forceinline__ void ser_pair(PD a, PD b, double z, double &P, double &Q)
+{
+    double zp[8];
+    zp[0] = 1.0;
+    zp[1] = z;
+#pragma unroll
+    for (int k = 2; k < 8; k++) zp[k] = zp[k - 1] * z;
+    const double z8 = zp[7] * z;
+    P = ser_blocks<N1>(a, zp, z8);
+    Q = ser_blocks<N2>(b, zp, z8);
+}
+
+// Pure rotation waves (every argument of the wave above 2) on the scaled
+// tables: T1 = (1/qc) sum v1' sin y, T2 = (1/qc^2) sum v2' sin^2 y with
+// v' = v (2/L)^k, and U1 = sqrt(2) qs^-3/2 sum u1' j1_mc(q, sin x, cos x),
+// U2 = 2 qs^-3 sum u2' j1_mc^2 with u' = u R^-3k/2 and q = (1/qs)(1/R):
+// no reciprocal or square root per point.  Each run of RESTART points
+// starts from an anchor advanced by the rotation through RESTART grid steps
+// (sin and cos computed directly for the first point and the two steps).
+__device__ __forceinline__ void rotate(float &s, float &c, float sd, float cd)
+{
+    const float s2 = fmaf(s, cd, c * sd);
+    c = fmaf(c, cd, -s * sd);
+    s = s2;
+}
+
+template <class P4>
+__device__ __forceinline__ void length_sums_b(P4 COLB, float y0, float step, int NL, float qc, float &T1, float &T2)
+{
+    float sd, cd, sa, ca, s0, c0;
+    sincos_prod(qc, step, sd, cd);
+    sincos_prod(qc, RESTART * step, sa, ca);
+    sincos_prod(qc, y0, s0, c0);
+    float A = 0.0f, B = 0.0f;
+    for (int m0 = 0; m0 < NL; m0 += RESTART) {
+        float s = s0, c = c0;
+        const int n = NL - m0 < RESTART ? NL - m0 : RESTART;
+        if (n == RESTART) {
+#pragma unroll
+            for (int t = 0; t < RESTART; t++) {
+                const float4 e = ldv(COLB + m0 + t);
+                A = fmaf(e.y, s, A);
+                B = fmaf(e.z, s * s, B);
+                rotate(s, c, sd, cd);
+            }
+            rotate(s0, c0, sa, ca);
+        } else {
+            for (int t = 0; t < n; t++) {
+                const float4 e = ldv(COLB + m0 + t);
+                A = fmaf(e.y, s, A);
+                B = fmaf(e.z, s * s, B);
+                rotate(s, c, sd, cd);
+            }
+        }
+    }
+    const float r = rcp_f(qc);
+    T1 = A * r;
+    T2 = B * (r * r);
+}
+
+template <class P4>
+__device__ __forceinline__ void radius_sums_b(P4 ROWB, float x0, float step, int NR, float qs, float &U1, float &U2)
+{
+    float sd, cd, sa, ca, s0, c0;
+    sincos_prod(qs, step, sd, cd);
+    sincos_prod(qs, RESTART * step, sa, ca);
+    sincos_prod(qs, x0, s0, c0);
+    const float iq = rcp_f(qs);
+    float A = 0.0f, B = 0.0f;
+    for (int i0 = 0; i0 < NR; i0 += RESTART) {
+        float s = s0, c = c0;
+        const int n = NR - i0 < RESTART ? NR - i0 : RESTART;
+        if (n == RESTART) {
+#pragma unroll
+            for (int t = 0; t < RESTART; t++) {
+                const float4 e = ldv(ROWB + i0 + t);
+                const float j = j1_mc(iq * e.x, s, c);
+                A = fmaf(e.y, j, A);
+                B = fmaf(e.z, j * j, B);
+                rotate(s, c, sd, cd);
+            }
+            rotate(s0, c0, sa, ca);
+        } else {
+            for (int t = 0; t < n; t++) {
+                const float4 e = ldv(ROWB + i0 + t);
+                const float j = j1_mc(iq * e.x, s, c);
+                A = fmaf(e.y, j, A);
+                B = fmaf(e.z, j * j, B);
+                rotate(s, c, sd, cd);
+            }
+        }
+    }
+    const float f = iq * sqrt_f(iq);   // qs^-3/2
+    U1 = A * (1.41421356f * f);
+    U2 = B * (2.0f * f * f);
 }
 
 // The rotation-mode sums with the moment series: a factor whose arguments
 // are all <= SER_X across the wave (largest argument q Rmax) is one
-// polynomial in (q Rmax)^2; otherwise the per-point sums above.  Rmax is
-// +inf where a set has no series, which fails the vote (also at q = 0).
-__device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const SAS_CONST float4 *COL,
-                                              const SAS_CONST double *ser, int NR, int NL, float q, float sn,
-                                              float cs, float &f1, float &f2)
+// polynomial in (q Rmax)^2; a factor whose arguments all exceed 2 runs on
+// the scaled tables; otherwise the per-point sums above (ROW, COL).  Rmax
+// is +inf where a set has no series, which fails the vote (also at q = 0).
+// The rotation tables come as RB, CB (ROWB, COLB), the series table `ser`
+// and the grid ends and steps `ends` = (R first, R last, R step, L/2 first,
+// L/2 last, L/2 step).
+template <class PF4, class PD, class PF>
+__device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const SAS_CONST float4 *COL, PF4 RB,
+                                              PF4 CB, PD ser, PF ends, int NR, int NL, float q, float sn, float cs,
+                                              float &f1, float &f2)
 {
     float T1, T2, U1, U2;
     const float qc = fabsf(q * cs), qs = fabsf(q * sn);
@@ -1092,6 +1226,8 @@ __device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const
         ser_pair<SER_NT1, SER_NT2>(ser + SER_T1, ser + SER_T2, yl * yl, P, Q);
         T1 = (float)P;
         T2 = (float)Q;
+    } else if (ldv(CB).w != 0.0f && wave_all(qc * fminf(ends[3], ends[4]) >= 2.0f)) {
+        length_sums_b(CB, ends[3], ends[5], NL, qc, T1, T2);
     } else {
         length_sums(COL, NL, qc, T1, T2);
     }
@@ -1100,6 +1236,8 @@ __device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const
         ser_pair<SER_NU1, SER_NU2>(ser + SER_U1, ser + SER_U2, xl * xl, P, Q);
         U1 = (float)P;
         U2 = (float)Q;
+    } else if (ldv(RB).w != 0.0f && wave_all(qs * fminf(ends[0], ends[1]) > 2.0f)) {
+        radius_sums_b(RB, ends[0], ends[2], NR, qs, U1, U2);
     } else {
         radius_sums(ROW, NR, qs, U1, U2);
     }
@@ -1233,7 +1371,10 @@ __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, flo
     const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
     const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
     if (dims[2] == 2) {
-        rank_one_sums(ROW, COL, as_const(t.SER) + (size_t)set * SER_STRIDE, NR, NL, q, sn, cs, f1, f2);
+        const float ends[6] = {ldv(ROW).x, ldv(ROW + NR - 1).x, ldv(ROW).w, ldv(COL).x, ldv(COL + NL - 1).x,
+                               ldv(COL).w};
+        rank_one_sums(ROW, COL, as_const(t.ROWB) + (size_t)set * NRMAX, as_const(t.COLB) + (size_t)set * NRMAX,
+                      as_const(t.SER) + (size_t)set * SER_STRIDE, ends, NR, NL, q, sn, cs, f1, f2);
         return;
     }
     if (dims[2] == 3) {

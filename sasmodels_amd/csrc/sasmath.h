@@ -182,13 +182,13 @@ SAS_HD float j1c_small(float x)
     return 2.0f * (z - 14.681970596f) * p;
 }
 
-// 2 J1(x)/x for x = ax > 2 given sx = sin(ax), cx = cos(ax) (Cephes j1f
-// modulus/phase form)
-SAS_HD float j1c_big(float ax, float sx, float cx)
+// Modulus and phase part of J1 for x > 2 (Cephes j1f modulus/phase form):
+// with q = 1/x, sx = sin x, cx = cos x,
+//   J1(x) = sqrt(q) / sqrt(2) * j1_mc(q, sx, cx),
+//   j1_mc = M(q) [sin xn (sx + cx) + cos xn (sx - cx)],  xn = q PH(q^2),
+// i.e. sqrt(q) M(q) cos(x + xn - 3pi/4).
+SAS_HD float j1_mc(float q, float sx, float cx)
 {
-    const float q = rcp_f(ax);
-    const float w = sqrt_f(q);
-    // modulus sqrt(1/x) * M(1/x)
     float m = fmaf(q, 6.913942741265801e-2f, -2.284801500053359e-1f);
     m = fmaf(q, m, 3.138238455499697e-1f);
     m = fmaf(q, m, -2.102302420403875e-1f);
@@ -196,8 +196,6 @@ SAS_HD float j1c_big(float ax, float sx, float cx)
     m = fmaf(q, m, 1.493389585089498e-1f);
     m = fmaf(q, m, 4.976029650847191e-6f);
     m = fmaf(q, m, 7.978845453073848e-1f);
-    const float amp = w * m;
-    // phase correction xn = (1/x) * PH(1/x^2); J1 = amp * cos(x + xn - 3pi/4)
     const float q2 = q * q;
     float ph = fmaf(q2, -4.497014141919556e+1f, 5.073465654089319e+1f);
     ph = fmaf(q2, ph, -2.485774108720340e+1f);
@@ -211,8 +209,16 @@ SAS_HD float j1c_big(float ax, float sx, float cx)
     const float xn2 = xn * xn;
     const float sxn = xn * fmaf(xn2, fmaf(xn2, 8.33333333e-3f, -1.66666667e-1f), 1.0f);
     const float cxn = fmaf(xn2, fmaf(xn2, fmaf(xn2, -1.38888889e-3f, 4.16666667e-2f), -0.5f), 1.0f);
-    // cos(x + xn - 3pi/4) = [sin xn (sin x + cos x) + cos xn (sin x - cos x)] / sqrt(2)
-    const float j1 = amp * 0.70710678118654752f * (sxn * (sx + cx) + cxn * (sx - cx));
+    // cos(x + xn - 3pi/4) sqrt(2) = sin xn (sin x + cos x) + cos xn (sin x - cos x)
+    return m * (sxn * (sx + cx) + cxn * (sx - cx));
+}
+
+// 2 J1(x)/x for x = ax > 2 given sx = sin(ax), cx = cos(ax)
+SAS_HD float j1c_big(float ax, float sx, float cx)
+{
+    const float q = rcp_f(ax);
+    const float w = sqrt_f(q);
+    const float j1 = w * 0.70710678118654752f * j1_mc(q, sx, cx);
     return 2.0f * j1 * q;   // even function: sign of x cancels
 }
 
@@ -263,29 +269,9 @@ SAS_HD float j1c_d(double x)
     if (ax <= 2.0f) return j1c_f(ax);
     const float q = rcp_f(ax);
     const float w = sqrt_f(q);
-    float m = fmaf(q, 6.913942741265801e-2f, -2.284801500053359e-1f);
-    m = fmaf(q, m, 3.138238455499697e-1f);
-    m = fmaf(q, m, -2.102302420403875e-1f);
-    m = fmaf(q, m, 5.435364690523026e-3f);
-    m = fmaf(q, m, 1.493389585089498e-1f);
-    m = fmaf(q, m, 4.976029650847191e-6f);
-    m = fmaf(q, m, 7.978845453073848e-1f);
-    const float amp = w * m;
-    const float q2 = q * q;
-    float ph = fmaf(q2, -4.497014141919556e+1f, 5.073465654089319e+1f);
-    ph = fmaf(q2, ph, -2.485774108720340e+1f);
-    ph = fmaf(q2, ph, 7.222973196770240e+0f);
-    ph = fmaf(q2, ph, -1.544842782180211e+0f);
-    ph = fmaf(q2, ph, 3.503787691653334e-1f);
-    ph = fmaf(q2, ph, -1.637986776941202e-1f);
-    ph = fmaf(q2, ph, 3.749989509080821e-1f);
-    const float xn = q * ph;
-    const float xn2 = xn * xn;
-    const float sxn = xn * fmaf(xn2, fmaf(xn2, 8.33333333e-3f, -1.66666667e-1f), 1.0f);
-    const float cxn = fmaf(xn2, fmaf(xn2, fmaf(xn2, -1.38888889e-3f, 4.16666667e-2f), -0.5f), 1.0f);
     float sx, cx;
     sincos_d(axd, sx, cx);
-    const float j1 = amp * 0.70710678118654752f * (sxn * (sx + cx) + cxn * (sx - cx));
+    const float j1 = w * 0.70710678118654752f * j1_mc(q, sx, cx);
     return 2.0f * j1 * q;
 }
 
