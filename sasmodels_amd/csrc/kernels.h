@@ -639,9 +639,18 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         if (pd_par[k] == 3) kl = k;
     }
     const int NR = pd_len[kr], NL = pd_len[kl];
-    const double *rv = pd_value + pd_off[kr], *rw = pd_weight + pd_off[kr];
-    const double *lv = pd_value + pd_off[kl], *lw = pd_weight + pd_off[kl];
     const int tid = threadIdx.x;
+    // the radius and length levels in LDS: the loops below walk them
+    // serially, and each step from device memory would wait out its latency
+    __shared__ double grid[4][NRMAX];
+    if (tid < NR) {
+        grid[0][tid] = pd_value[pd_off[kr] + tid];
+        grid[1][tid] = pd_weight[pd_off[kr] + tid];
+    } else if (tid >= NRMAX && tid - NRMAX < NL) {
+        grid[2][tid - NRMAX] = pd_value[pd_off[kl] + tid - NRMAX];
+        grid[3][tid - NRMAX] = pd_weight[pd_off[kl] + tid - NRMAX];
+    }
+    const double *rv = grid[0], *rw = grid[1], *lv = grid[2], *lw = grid[3];
     __shared__ unsigned char acc[NRMAX * NRMAX];
     __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad, nzero, ngroups, mode;
     __shared__ float steps[2];
@@ -651,29 +660,54 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         BT[i] = 0.0f;
     }
     for (int i = tid; i < NRMAX * NRMAX; i += EXPAND_THREADS) acc[i] = 0;
-    if (tid == 0) bad = 0;
+    __shared__ int nacc, notap;
+    if (tid == 0) {
+        bad = 0;
+        nacc = 0;
+        notap = 0;
+    }
     __syncthreads();
+    // equally spaced grids (every distribution of the reference is a linspace)
+    const double dr = NR > 1 ? (rv[NR - 1] - rv[0]) / (NR - 1) : 0.0;
+    const double dl = NL > 1 ? (lv[NL - 1] - lv[0]) / (NL - 1) : 0.0;
+    if (tid < NR) {
+        if (!(fabs(rv[tid] - (rv[0] + tid * dr)) <= 1e-9 * fabs(rv[NR - 1]))) notap = 1;
+    } else if (tid >= NRMAX && tid - NRMAX < NL) {
+        const int j = tid - NRMAX;
+        if (!(fabs(lv[j] - (lv[0] + j * dl)) <= 1e-9 * fabs(lv[NL - 1]))) notap = 1;
+    }
     double base[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) base[i] = values[2 + i];
     double acc_w = 0.0, acc_v = 0.0, acc_r = 0.0;
+    int nmine = 0;
+    // the levels other than radius and length have one point each (a
+    // separable mesh): their values and weights once, the grids from LDS
+    double pfix[NP];
+#pragma unroll
+    for (int i = 0; i < NP; i++) pfix[i] = base[i];
+    double wfix[MP];
+#pragma unroll
+    for (int k = 0; k < MP; k++) {
+        wfix[k] = pd_weight[pd_off[k]];
+        if (k == kr || k == kl) continue;
+        const int par = pd_par[k];
+#pragma unroll
+        for (int i = 0; i < NP; i++)
+            if (i == par) pfix[i] = pd_value[pd_off[k]];
+    }
+    const int sr = pd_stride[kr], sl = pd_stride[kl];
     for (int step = pd_start + tid; step < pd_stop; step += EXPAND_THREADS) {
+        const int ir = (step / sr) % NR, il = (step / sl) % NL;
         double p[NP];
 #pragma unroll
-        for (int i = 0; i < NP; i++) p[i] = base[i];
+        for (int i = 0; i < NP; i++) p[i] = pfix[i];
+        p[2] = rv[ir];
+        p[3] = lv[il];
+        // the product in mesh order (outermost level first), as k_expand
         double w0 = 1.0;
-        int ir = 0, il = 0;
-        for (int k = MP - 1; k >= 0; k--) {
-            const int idx = (step / pd_stride[k]) % pd_len[k];
-            const double v = pd_value[pd_off[k] + idx];
-            const int par = pd_par[k];
 #pragma unroll
-            for (int i = 0; i < NP; i++)
-                if (i == par) p[i] = v;
-            w0 = pd_weight[pd_off[k] + idx] * w0;
-            if (k == kr) ir = idx;
-            if (k == kl) il = idx;
-        }
+        for (int k = MP - 1; k >= 0; k--) w0 = (k == kr ? rw[ir] : k == kl ? lw[il] : wfix[k]) * w0;
         if (Cylinder::valid(p) && w0 > a.cutoff) {
             const double vol = Cylinder::volume(p);
             acc_w += w0;
@@ -683,49 +717,73 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
             AT[il * NRMAX + ir] = (float)(w0 * 1e-2 * s * PI_4_D);
             BT[il * NRMAX + ir] = (float)(w0 * 1e-4 * s * s * PI_4_D);
             acc[ir * NRMAX + il] = 1;
+            nmine++;
         }
     }
-    __shared__ double red[3][EXPAND_THREADS];
-    red[0][tid] = acc_w;
-    red[1][tid] = acc_v;
-    red[2][tid] = acc_r;
-    // columns by descending weight (invalid lengths last), ties by index
-    if (tid < NL) {
-        const double key = lv[tid] >= 0.0 ? lw[tid] : -__builtin_inf();
-        int rank = 0;
-        for (int j = 0; j < NL; j++) {
-            const double kj = lv[j] >= 0.0 ? lw[j] : -__builtin_inf();
-            rank += (kj > key || (kj == key && j < tid)) ? 1 : 0;
-        }
-        colorder[rank] = tid;
+    if (nmine) atomicAdd(&nacc, nmine);
+    // weight sums: within each wave by shuffles, then over the waves
+    constexpr int NW = EXPAND_THREADS / 64;
+    __shared__ double red[3][NW];
+    for (int off = 32; off > 0; off >>= 1) {
+        acc_w += __shfl_down(acc_w, off);
+        acc_v += __shfl_down(acc_v, off);
+        acc_r += __shfl_down(acc_r, off);
+    }
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = acc_w;
+        red[1][tid >> 6] = acc_v;
+        red[2][tid >> 6] = acc_r;
     }
     __syncthreads();
-    for (int s = EXPAND_THREADS / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            red[0][tid] += red[0][tid + s];
-            red[1][tid] += red[1][tid + s];
-            red[2][tid] += red[2][tid + s];
+    if (tid == 0) {
+        for (int w = 1; w < NW; w++) {
+            red[0][0] += red[0][w];
+            red[1][0] += red[1][w];
+            red[2][0] += red[2][w];
+        }
+    }
+    __syncthreads();
+    // every point accepted on equally spaced grids: rotation mode, without
+    // the orders and groups below (a serial walk of the grids each)
+    const bool fast = nacc == NR * NL && notap == 0;
+    if (!fast) {
+        // columns by descending weight (invalid lengths last), ties by index
+        if (tid < NL) {
+            const double key = lv[tid] >= 0.0 ? lw[tid] : -__builtin_inf();
+            int rank = 0;
+            for (int j = 0; j < NL; j++) {
+                const double kj = lv[j] >= 0.0 ? lw[j] : -__builtin_inf();
+                rank += (kj > key || (kj == key && j < tid)) ? 1 : 0;
+            }
+            colorder[rank] = tid;
+        }
+        __syncthreads();
+        // accepted columns of each row, and whether they are a prefix of colorder
+        if (tid < NR) {
+            int k = 0;
+            for (int j = 0; j < NL; j++) k += acc[tid * NRMAX + j];
+            rowk[tid] = k;
+            for (int m = 0; m < NL; m++)
+                if ((acc[tid * NRMAX + colorder[m]] != 0) != (m < k)) bad = 1;
+        }
+        __syncthreads();
+        if (tid < NR) {
+            const int k = rowk[tid];
+            int rank = 0;
+            for (int i = 0; i < NR; i++) rank += (rowk[i] < k || (rowk[i] == k && i < tid)) ? 1 : 0;
+            roworder[rank] = tid;
         }
         __syncthreads();
     }
-    // accepted columns of each row, and whether they are a prefix of colorder
-    if (tid < NR) {
-        int k = 0;
-        for (int j = 0; j < NL; j++) k += acc[tid * NRMAX + j];
-        rowk[tid] = k;
-        for (int m = 0; m < NL; m++)
-            if ((acc[tid * NRMAX + colorder[m]] != 0) != (m < k)) bad = 1;
-    }
-    __syncthreads();
-    if (tid < NR) {
-        const int k = rowk[tid];
-        int rank = 0;
-        for (int i = 0; i < NR; i++) rank += (rowk[i] < k || (rowk[i] == k && i < tid)) ? 1 : 0;
-        roworder[rank] = tid;
-    }
-    __syncthreads();
     const bool prefix = bad == 0;
-    if (tid == 0) {
+    if (tid == 0 && fast) {
+        nzero = 0;
+        ngroups = 1;
+        GRP[0] = make_int2(NL, NR);
+        steps[0] = (float)dr;
+        steps[1] = (float)(0.5 * dl);
+        mode = 2;
+    } else if (tid == 0) {
         // rows accepting nothing drop out; runs of equal accepted count form groups
         int z = 0;
         while (z < NR && rowk[roworder[z]] == 0) z++;
@@ -738,13 +796,8 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         }
         nzero = z;
         ngroups = g;
-        // equally spaced grids (every distribution of the reference is a linspace)
         int md = prefix ? 1 : 0;
-        bool ap = prefix;
-        const double dr = NR > 1 ? (rv[NR - 1] - rv[0]) / (NR - 1) : 0.0;
-        const double dl = NL > 1 ? (lv[NL - 1] - lv[0]) / (NL - 1) : 0.0;
-        for (int i = 0; i < NR && ap; i++) ap = fabs(rv[i] - (rv[0] + i * dr)) <= 1e-9 * fabs(rv[NR - 1]);
-        for (int j = 0; j < NL && ap; j++) ap = fabs(lv[j] - (lv[0] + j * dl)) <= 1e-9 * fabs(lv[NL - 1]);
+        const bool ap = prefix && notap == 0;
         if (ap) {
             steps[0] = (float)dr;
             steps[1] = (float)(0.5 * dl);
@@ -848,52 +901,46 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     }
     // rotation mode: the moment series of the four factors from the same
     // FP32-rounded grid values and weights as ROW and COL (u1 = w R^2,
-    // u2 = w R^4; v1 = w L c1, v2 = w L^2 c2): powers r^2k of the scaled
-    // grids in LDS, then one coefficient per thread
+    // u2 = w R^4; v1 = w L c1, v2 = w L^2 c2): the weights and the powers
+    // r^2k of the scaled grids in LDS, then one coefficient per thread.
+    // Rotation-mode grids are equally spaced: their extremes are the ends.
     __shared__ double pw[2][SER_NU2][NRMAX];
-    __shared__ double gmax[2], gmin[2];
+    __shared__ double wt[4][NRMAX];
     static_assert(SER_NT2 <= SER_NU2, "power table too short for the length series");
-    if (tid < 2) {
-        double m = 0.0, mn = __builtin_inf();
-        for (int i = 0; i < (tid ? NL : NR); i++) {
-            const double v = tid ? (double)(float)(0.5 * lv[i]) : (double)(float)rv[i];
-            m = fmax(m, v);
-            mn = fmin(mn, v);
-        }
-        gmax[tid] = m;
-        gmin[tid] = mn;
-    }
-    __syncthreads();
-    const double rmax = gmax[0], ymax = gmax[1];
-    // rotation mode, grids without zero: the scaled tables of the rotation
-    // loops (J = sqrt(2) (qs R)^-3/2 j1_mc, sin(y)/y = sin(y) / (qc L/2))
-    if (tid < NRMAX) {
-        const int m = tid;
-        float4 rb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cb = rb;
-        if (mode == 2 && gmin[0] > 0.0 && m < NR) {
-            const double x = rv[m], r = 1.0 / (double)(float)x;
-            const double u1 = (double)(float)(rw[m] * x * x), u2 = (double)(float)(rw[m] * x * x * x * x);
-            rb = make_float4((float)r, (float)(u1 * r * sqrt(r)), (float)(u2 * r * r * r), m == 0 ? 1.0f : 0.0f);
-        }
-        if (mode == 2 && gmin[1] > 0.0 && m < NL) {
-            const double x = lv[m], r = 1.0 / (double)(float)(0.5 * x);
-            const double v1 = (double)(float)(lw[m] * x * c1), v2 = (double)(float)(lw[m] * x * x * c2);
-            cb = make_float4((float)r, (float)(v1 * r), (float)(v2 * r * r), m == 0 ? 1.0f : 0.0f);
-        }
-        t.ROWB[(size_t)set * NRMAX + m] = rb;
-        t.COLB[(size_t)set * NRMAX + m] = cb;
-    }
+    const double r0 = (double)(float)rv[0], r1 = (double)(float)rv[NR - 1];
+    const double y0 = (double)(float)(0.5 * lv[0]), y1 = (double)(float)(0.5 * lv[NL - 1]);
+    const double rmax = fmax(r0, r1), ymax = fmax(y0, y1);
+    const bool rpos = fmin(r0, r1) > 0.0, lpos = fmin(y0, y1) > 0.0;
     const bool ur = mode == 2 && NR >= SER_MIN_POINTS && rmax > 0.0;
     const bool ul = mode == 2 && NL >= SER_MIN_POINTS && ymax > 0.0;
     if (tid < 2 * NRMAX) {
         const int g = tid / NRMAX, i = tid % NRMAX;
-        if (i < (g ? NL : NR) && (g ? ul : ur)) {
-            const double r = g ? (double)(float)(0.5 * lv[i]) / ymax : (double)(float)rv[i] / rmax;
-            double p = 1.0;
-            for (int k = 0; k < SER_NU2; k++) {
-                pw[g][k][i] = p;
-                p *= r * r;
+        if (i < (g ? NL : NR)) {
+            const double x = g ? lv[i] : rv[i], w = g ? lw[i] : rw[i];
+            const double a = (double)(float)(g ? w * x * c1 : w * x * x);
+            const double b = (double)(float)(g ? w * x * x * c2 : w * x * x * x * x);
+            wt[2 * g][i] = a;
+            wt[2 * g + 1][i] = b;
+            // rotation mode, grids without zero: the scaled tables of the
+            // rotation loops (J = sqrt(2) (qs R)^-3/2 j1_mc, sin(y)/y =
+            // sin(y) / (qc L/2))
+            float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (mode == 2 && (g ? lpos : rpos)) {
+                const double r = 1.0 / (double)(float)(g ? 0.5 * x : x);
+                e = g ? make_float4((float)r, (float)(a * r), (float)(b * r * r), i == 0 ? 1.0f : 0.0f)
+                      : make_float4((float)r, (float)(a * r * sqrt(r)), (float)(b * r * r * r), i == 0 ? 1.0f : 0.0f);
             }
+            (g ? t.COLB : t.ROWB)[(size_t)set * NRMAX + i] = e;
+            if (g ? ul : ur) {
+                const double r = g ? (double)(float)(0.5 * x) / ymax : (double)(float)x / rmax;
+                double p = 1.0;
+                for (int k = 0; k < SER_NU2; k++) {
+                    pw[g][k][i] = p;
+                    p *= r * r;
+                }
+            }
+        } else {
+            (g ? t.COLB : t.ROWB)[(size_t)set * NRMAX + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
     __syncthreads();
@@ -904,24 +951,21 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
             v = ur ? rmax : __builtin_inf();
         } else if (c == 1) {
             v = ul ? ymax : __builtin_inf();
-        } else if (c < SER_T1 && ur) {
-            const bool sq = c >= SER_U2;
-            const int k = sq ? c - SER_U2 : c - SER_U1;
-            for (int i = 0; i < NR; i++) {
-                const double x = rv[i];
-                const double u = sq ? (double)(float)(rw[i] * x * x * x * x) : (double)(float)(rw[i] * x * x);
-                v = fma(u, pw[0][k][i], v);
+        } else if (c < SER_T1 ? ur : (c < SER_T2 + SER_NT2 && ul)) {
+            // term k of series U1 (J), U2 (J^2), T1 (sinc) or T2 (sinc^2)
+            const int sr = c < SER_U2 ? 0 : c < SER_T1 ? 1 : c < SER_T2 ? 2 : 3;
+            const int k = c - (sr == 0 ? SER_U1 : sr == 1 ? SER_U2 : sr == 2 ? SER_T1 : SER_T2);
+            const int g = sr >> 1, n = g ? NL : NR;
+            const double *u = wt[sr], *pk = pw[g][k];
+            double v0 = 0.0, v1 = 0.0;
+            int i = 0;
+            for (; i + 1 < n; i += 2) {
+                v0 = fma(u[i], pk[i], v0);
+                v1 = fma(u[i + 1], pk[i + 1], v1);
             }
-            v *= sq ? SER_COEF.j1c_sq[k] : SER_COEF.j1c[k];
-        } else if (c >= SER_T1 && c < SER_T2 + SER_NT2 && ul) {
-            const bool sq = c >= SER_T2;
-            const int k = sq ? c - SER_T2 : c - SER_T1;
-            for (int j = 0; j < NL; j++) {
-                const double x = lv[j];
-                const double u = sq ? (double)(float)(lw[j] * x * x * c2) : (double)(float)(lw[j] * x * c1);
-                v = fma(u, pw[1][k][j], v);
-            }
-            v *= sq ? SER_COEF.sinc_sq[k] : SER_COEF.sinc[k];
+            if (i < n) v0 = fma(u[i], pk[i], v0);
+            v = (v0 + v1) * (sr == 0 ? SER_COEF.j1c[k] : sr == 1 ? SER_COEF.j1c_sq[k] :
+                             sr == 2 ? SER_COEF.sinc[k] : SER_COEF.sinc_sq[k]);
         }
         t.SER[(size_t)set * SER_STRIDE + c] = v;
     }
