@@ -53,6 +53,11 @@ extern "C" {
  * wellwidth): sasmodels/models/stickyhardsphere.py, squarewell.py */
 #define SASHIP_STICKY_HARDSPHERE 14
 #define SASHIP_SQUAREWELL 15
+/* charged spheres, Hayter-Penfold rescaled MSA (sasmodels/models/
+ * hayter_msa.py): six parameters radius_effective, volfraction, charge,
+ * temperature, concentration_salt, dielectconst; NaN where the reference's
+ * coefficient iteration fails */
+#define SASHIP_HAYTER_MSA 16
 /* shapes with three orientation angles (theta, phi, psi; xy_mode "qabc"):
  * 1D kernels integrate the orientation with nested rules (adaptive as
  * adaptive.c with outer_n limits, or the fixed size given) */
@@ -186,12 +191,13 @@ int saship_batch_set_resolution(saship_batch *batch, const int32_t *n_out, const
                                 const double *weights, int64_t n_weights);
 
 /* P@S products on the device: the batch's form factor times structure
- * factor s_model (SASHIP_HARDSPHERE, SASHIP_STICKY_HARDSPHERE or
- * SASHIP_SQUAREWELL; 0 switches it off), combined in the final kernel as
- * ProductKernel.Iq does (sasmodels/product.py:425-528).  pars holds per
- * set (radius_effective, volfraction, structure_factor_mode,
- * radius_effective_mode, volfraction_in_form, s3, s4) with s3, s4 the
- * structure factor's further parameters (0 for the hard sphere).
+ * factor s_model (SASHIP_HARDSPHERE, SASHIP_STICKY_HARDSPHERE,
+ * SASHIP_SQUAREWELL or SASHIP_HAYTER_MSA; 0 switches it off), combined in
+ * the final kernel as ProductKernel.Iq does (sasmodels/product.py:425-528).
+ * pars holds SASHIP_SPARS doubles per set: (radius_effective, volfraction,
+ * structure_factor_mode, radius_effective_mode, volfraction_in_form, s2,
+ * s3, s4, s5) with s2..s5 the structure factor's further parameters in
+ * order, 0 past its last one (all 0 for the hard sphere).
  * Later calls return in `iq`
  *     n * (<F^2> S)                 structure_factor_mode 0
  *     n * (<F^2> + <F>^2 (S - 1))   structure_factor_mode > 0 (beta, 1D only)
@@ -203,6 +209,7 @@ int saship_batch_set_resolution(saship_batch *batch, const int32_t *n_out, const
  * radius_effective_mode) or at radius_effective (mode 0), and at volume
  * fraction volfraction * V_form/V_shell.  scale and background are the
  * set's values[0..1]; the raw result stays P's. */
+#define SASHIP_SPARS 9
 int saship_batch_set_structure(saship_batch *batch, int32_t s_model, const double *pars, int32_t nsets);
 
 /* Runs the device work of the last successful saship_batch_call again

@@ -71,6 +71,8 @@ def lib():
         for name in ("sasoracle_stickyhardsphere", "sasoracle_squarewell"):
             getattr(L, name).restype = _D
             getattr(L, name).argtypes = [_D]*5
+        L.sasoracle_hayter_msa.restype = _D
+        L.sasoracle_hayter_msa.argtypes = [_D]*7
         L.sasoracle_shell_volume.restype = _D
         L.sasoracle_shell_volume.argtypes = [ct.c_int, ct.c_void_p]
         L.sasoracle_radius_effective.restype = _D
@@ -178,7 +180,8 @@ def hardsphere(q, radius_effective, volfraction):
 
 def structure(name, q, radius_effective, volfraction, *extra):
     """S(q) of structure factor *name* ("hardsphere", "stickyhardsphere",
-    "squarewell") with its further parameters *extra*; q scalar or array."""
+    "squarewell", "hayter_msa") with its further parameters *extra*; q
+    scalar or array."""
     q = np.asarray(q, "d")
     if name == "hardsphere":
         return hardsphere(q, radius_effective, volfraction)
@@ -220,6 +223,10 @@ def ref_lib(name):
     if hasattr(L, "ref_sq4"):
         L.ref_sq4.restype = _D
         L.ref_sq4.argtypes = [_D]*5
+        return L
+    if hasattr(L, "ref_sq6"):
+        L.ref_sq6.restype = _D
+        L.ref_sq6.argtypes = [_D]*7
         return L
     if hasattr(L, "ref_iq"):
         L.ref_iq.restype = _D

@@ -11,11 +11,12 @@
  * vector to the reference's static functions so that ctypes can call them.
  *
  * One shared object is built per (model, quadrature) pair:
- *   -DREF_MODEL=<1..16> sphere, cylinder, ellipsoid, core_shell_cylinder,
+ *   -DREF_MODEL=<1..17> sphere, cylinder, ellipsoid, core_shell_cylinder,
  *                       core_shell_sphere, hardsphere (structure factor),
  *                       parallelepiped, triaxial_ellipsoid, bcc, fcc and
  *                       sc paracrystal (with lib/gauss150.c), hollow_cylinder,
- *                       vesicle, fuzzy_sphere, stickyhardsphere, squarewell
+ *                       vesicle, fuzzy_sphere, stickyhardsphere, squarewell,
+ *                       hayter_msa
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -145,8 +146,14 @@ double ref_form_volume(const double *p) { return form_volume(p[2]); }
 #    include "squarewell.c"
 #  endif
 double ref_sq4(double q, double r, double phi, double a, double b) { return Iq(q, r, phi, a, b); }
+#elif REF_MODEL == 17
+#  include "hayter_msa.c"
+double ref_sq6(double q, double r, double phi, double a, double b, double c, double d)
+{
+    return Iq(q, r, phi, a, b, c, d);
+}
 #else
-#  error "REF_MODEL must be 1..16"
+#  error "REF_MODEL must be 1..17"
 #endif
 
 #ifdef NPAR_CALL_FQ

@@ -280,14 +280,13 @@ __global__ __launch_bounds__(256) void k_batch_final(
     // P@S (sasmodels/product.py:425-528): S at the effective radius of the
     // form (mode > 0) or the given one, and at volfraction * form:shell
     // volume ratio; the number density is volfraction / shell volume
-    const double *sp = spars ? spars + 7 * s : nullptr;
-    double sq_pars[4] = {0.0, 0.0, 0.0, 0.0};     // radius, volume fraction, extra S parameters
+    const double *sp = spars ? spars + SASHIP_SPARS * s : nullptr;
+    double sq_pars[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};     // radius, volume fraction, further S parameters
     bool beta = false;
     if (sp) {
         sq_pars[0] = sp[3] > 0.0 ? sm[3] / total : sp[0];
         sq_pars[1] = sp[1] * ((sm[1] / total) / shell);
-        sq_pars[2] = sp[5];
-        sq_pars[3] = sp[6];
+        for (int k = 2; k < 6; k++) sq_pars[k] = sp[3 + k];
         beta = sp[2] > 0.0;
         if (sp[4] == 0.0) cscale *= sp[1];      // else the form carries its own volfraction
     }
@@ -520,10 +519,10 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     p.structure = b->s_model != 0;
     if (p.structure) {
         for (int s = 0; s < ns; s++)
-            if (b->spars[7 * s + 2] > 0.0 && kind != 0)
+            if (b->spars[SASHIP_SPARS * s + 2] > 0.0 && kind != 0)
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": the beta approximation needs 1D q");
         p.off_spars = off;
-        off = align256(off + sizeof(double) * 7 * ns);
+        off = align256(off + sizeof(double) * SASHIP_SPARS * ns);
     }
     p.bytes = off;
     if (int e = ensure_host(b->h_stage, b->stage_cap, p.bytes)) return e;
@@ -535,7 +534,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     memcpy(h + p.off_poff, poff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_resoff, resoff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_nchunk, nchunk.data(), sizeof(int32_t) * ns);
-    if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * 7 * ns);
+    if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * SASHIP_SPARS * ns);
     // work list, longest items first: items are bucketed by the number of
     // mesh points their chunk can hold (an upper bound before pruning)
     int4 *it = (int4 *)(h + p.off_items);
@@ -917,7 +916,7 @@ int saship_batch_call(saship_batch *b, int32_t kind, const int32_t *details, int
     std::lock_guard<std::mutex> lock(b->mu);
     if (b->s_model)
         for (int s = 0; s < b->nsets; s++) {
-            const double er = b->spars[7 * s + 3];
+            const double er = b->spars[SASHIP_SPARS * s + 3];
             if (er != 0.0 && er != (double)radius_effective_mode)
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": structure radius_effective_mode " +
                                                std::to_string((int)er) + " differs from the call's " +
@@ -938,16 +937,17 @@ int saship_batch_set_structure(saship_batch *b, int32_t s_model, const double *p
     }
     if (!model_structure(s_model)) return fail(SASHIP_EINVAL, "model " + std::to_string(s_model) + " is not a structure factor");
     if (model_structure(b->m->id)) return fail(SASHIP_EINVAL, "the batch model is itself a structure factor");
-    if (!pars || nsets != b->nsets) return fail(SASHIP_EINVAL, "need 7 structure parameters for each of the batch's sets");
+    if (!pars || nsets != b->nsets)
+        return fail(SASHIP_EINVAL, "need " + std::to_string(SASHIP_SPARS) + " structure parameters for each of the batch's sets");
     for (int s = 0; s < nsets; s++) {
-        const double *sp = pars + 7 * s;
+        const double *sp = pars + SASHIP_SPARS * s;
         const std::string where = "set " + std::to_string(s) + ": ";
-        for (int k = 0; k < 7; k++)
+        for (int k = 0; k < SASHIP_SPARS; k++)
             if (!std::isfinite(sp[k])) return fail(SASHIP_EINVAL, where + "structure parameters must be finite");
         if (sp[3] < 0.0 || sp[3] != std::floor(sp[3]) || sp[3] > 100.0)
             return fail(SASHIP_EINVAL, where + "radius_effective_mode must be a small non-negative integer");
     }
-    b->spars.assign(pars, pars + 7 * (size_t)nsets);
+    b->spars.assign(pars, pars + SASHIP_SPARS * (size_t)nsets);
     b->s_model = s_model;
     b->have_last = false;
     return 0;

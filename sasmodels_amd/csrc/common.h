@@ -73,25 +73,25 @@ inline int cu_count(int dev)
     return n;
 }
 
-// Model ids 0-4 and 6-13 are form factors; 5, 14 and 15 the hard-sphere,
-// sticky hard-sphere and square-well structure factors (radius_effective,
-// volfraction and up to two more parameters, nothing polydisperse, no sld:
-// no magnetic values).
+// Model ids 0-4 and 6-13 are form factors; 5, 14, 15 and 16 the
+// hard-sphere, sticky hard-sphere, square-well and Hayter-Penfold MSA
+// structure factors (radius_effective, volfraction and up to four more
+// parameters, nothing polydisperse, no sld: no magnetic values).
 constexpr int MODEL_HARDSPHERE = 5;
-constexpr int MODEL_LAST = 15;
+constexpr int MODEL_LAST = 16;
 inline int model_npars(int id)
 {
-    static const int np[16] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4};
+    static const int np[17] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[16] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0};
+    static const int mp[17] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[16] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6};
+    static const int nv[17] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8};
     return nv[id];
 }
 inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || (id >= 6 && id <= 11); }
@@ -100,7 +100,7 @@ inline bool model_abc(int id) { return id >= 6 && id <= 10; }
 // bcc, fcc, sc paracrystals: FP64 product rule (Gauss-150 by default)
 inline bool model_paracrystal(int id) { return id >= 8 && id <= 10; }
 inline bool model_nested(int id) { return id == 6 || id == 7; }
-inline bool model_structure(int id) { return id == MODEL_HARDSPHERE || id == 14 || id == 15; }
+inline bool model_structure(int id) { return id == MODEL_HARDSPHERE || id == 14 || id == 15 || id == 16; }
 
 // Separable cylinder 1D meshes: only the radius (parameter 2) and length
 // (parameter 3) levels have more than one point, each at most SEP_MAX.

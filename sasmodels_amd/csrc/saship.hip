@@ -73,8 +73,8 @@ __global__ __launch_bounds__(256) void k_structure(int model, int npar, int nq, 
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < nq) {
         const double q = is2d ? sqrt(q64[2 * i] * q64[2 * i] + q64[2 * i + 1] * q64[2 * i + 1]) : q64[i];
-        double p[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int k = 0; k < npar && k < 4; k++) p[k] = values[2 + k];
+        double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < npar && k < 6; k++) p[k] = values[2 + k];
         const double s = include ? structure_sq(model, q, p) : 0.0;
         result[i] = accumulate ? result[i] + s : s;
     }

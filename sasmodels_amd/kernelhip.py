@@ -254,18 +254,18 @@ class HipBatch:
     def set_structure(self, pars, s_model=5):
         """
         Multiply every set's form factor by a structure factor on the device
-        (saship_batch_set_structure): *pars* is an (nsets, 7) array of
+        (saship_batch_set_structure): *pars* is an (nsets, 9) array of
         (radius_effective, volfraction, structure_factor_mode,
-        radius_effective_mode, volfraction_in_form, s3, s4) per set, s3 and
-        s4 the structure factor's further parameters; *s_model* its kernel
-        id (5 hard sphere, 14 sticky hard sphere, 15 square well); None
-        switches the product off.  Calls then return the P@S intensity of
-        each set.
+        radius_effective_mode, volfraction_in_form, s2, s3, s4, s5) per set,
+        s2..s5 the structure factor's further parameters (0 past its last);
+        *s_model* its kernel id (5 hard sphere, 14 sticky hard sphere,
+        15 square well, 16 Hayter MSA); None switches the product off.
+        Calls then return the P@S intensity of each set.
         """
         if pars is None:
             _lib.check(_lib.lib().saship_batch_set_structure(self._handle, 0, None, 0))
             return
-        pars = np.ascontiguousarray(pars, "d").reshape(-1, 7)
+        pars = np.ascontiguousarray(pars, "d").reshape(-1, 9)
         _lib.check(_lib.lib().saship_batch_set_structure(
             self._handle, int(s_model), pars.ctypes.data, pars.shape[0]))
 

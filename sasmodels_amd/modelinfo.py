@@ -20,7 +20,7 @@ core_shell_sphere.py:78-82, parallelepiped.py:216-231,
 triaxial_ellipsoid.py:143-158, bcc_paracrystal.py:186-194,
 fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hollow_cylinder.py:90-98,
 vesicle.py:87-96, fuzzy_sphere.py:113-117, hardsphere.py:84-88,
-stickyhardsphere.py:110-121, squarewell.py:83-92),
+stickyhardsphere.py:110-121, squarewell.py:83-92, hayter_msa.py:110-117),
 with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
 orientation mode, ``structure_factor``).
@@ -348,6 +348,15 @@ _MODEL_TABLE = {
         ["volfraction", "", 0.04, [0, 0.08], "", "volume fraction of spheres"],
         ["welldepth", "kT", 1.5, [0.0, 1.5], "", "depth of well, epsilon"],
         ["wellwidth", "diameters", 1.2, [1.0, inf], "", "width of well in diameters (=2R) units, must be > 1"],
+    ], dict(structure_factor=True, have_Fq=False, xy_mode="qa", has_quadrature=False)),
+    "hayter_msa": (16, [
+        ["radius_effective", "Ang", 20.75, [0, inf], "", "effective radius of charged sphere"],
+        ["volfraction", "None", 0.0192, [0, 0.74], "", "volume fraction of spheres"],
+        ["charge", "e", 19.0, [0.000001, 200], "", "charge on sphere (in electrons)"],
+        ["temperature", "K", 318.16, [0, 450], "", "temperature, in Kelvin, for Debye length calculation"],
+        ["concentration_salt", "M", 0.0, [0, inf], "", "conc of salt, moles/litre, 1:1 electolyte, for Debye length"],
+        ["dielectconst", "None", 71.08, [-inf, inf], "",
+         "dielectric constant (relative permittivity) of solvent, kappa, default water, for Debye length"],
     ], dict(structure_factor=True, have_Fq=False, xy_mode="qa", has_quadrature=False)),
 }
 

@@ -34,6 +34,8 @@ from .modelinfo import (NUM_COMMON_PARS, NUM_MAGFIELD_PARS, NUM_MAGNETIC_PARS,
                         ModelInfo, ParameterTable, parse_parameter)
 
 STRUCTURE_MODE_ID = "structure_factor_mode"
+#: per-set structure parameters of the batched product (SASHIP_SPARS)
+SPARS = 9
 RADIUS_MODE_ID = "radius_effective_mode"
 RADIUS_ID = "radius_effective"
 VOLFRAC_ID = "volfraction"
@@ -292,7 +294,8 @@ class ProductBatch:
     :class:`ProductKernel` on that set.
 
     *model* is a :class:`ProductModel` whose P is a hip model and whose S
-    is the hard-sphere, sticky hard-sphere or square-well structure factor;
+    is the hard-sphere, sticky hard-sphere, square-well or Hayter MSA
+    structure factor;
     *q_list* holds one ``[q]`` or
     ``[qx, qy]`` per set.  Sets with radius_effective_mode > 0 in one call
     must share the mode (the batch computes one effective radius per set).
@@ -300,9 +303,9 @@ class ProductBatch:
 
     def __init__(self, model, q_list, use_graph=False):
         from .kernelhip import HipBatch
-        if model.S.info.kernel_id not in (5, 14, 15):
-            raise ValueError("batched products support the hardsphere, stickyhardsphere and "
-                             "squarewell structure factors")
+        if model.S.info.kernel_id not in (5, 14, 15, 16):
+            raise ValueError("batched products support the hardsphere, stickyhardsphere, "
+                             "squarewell and hayter_msa structure factors")
         self._s_model = model.S.info.kernel_id
         self.info = model.info
         self._layout = _Layout(model.info)
@@ -344,7 +347,7 @@ class ProductBatch:
 
     def _split(self, calls):
         lay = self._layout
-        p_calls, spars, er_modes = [], np.zeros((len(calls), 7)), set()
+        p_calls, spars, er_modes = [], np.zeros((len(calls), SPARS)), set()
         for s, (cd, values) in enumerate(calls):
             scale, background, volfrac, er_mode, beta_mode = lay.modes(values)
             if beta_mode and self.dim == "2d":

@@ -247,6 +247,13 @@ SQUAREWELL = [
       "wellwidth": 1.2}, [0.001], [0.97665742]),
 ]
 
+# hayter_msa.py:156-165
+HAYTER_MSA = [
+    ({"scale": 1.0, "background": 0.0, "radius_effective": 20.75, "charge": 19.0, "volfraction": 0.0192,
+      "temperature": 298.0, "concentration_salt": 0, "dielectconst": 78.0},
+     [0.00001, 0.0010, 0.01, 0.075], [0.0711646, 0.0712928, 0.0847006, 1.07150]),
+]
+
 # direct_model.py:562-603, rtol 1e-6: (model, pars, q, I)
 DIRECT_MODEL = [
     ("sphere", dict(radius=200, background=0), [0.1], [0.6190146273894904]),
