@@ -611,11 +611,14 @@ constexpr SerCoef make_ser_coef()
 }
 __constant__ constexpr SerCoef SER_COEF = make_ser_coef();
 
-// One workgroup per set: find the radius and length levels, walk the set's
-// mesh steps (FP64 normalisation sums as k_expand, acceptance mask and the
-// bilinear tables), then test the prefix structure and, when it holds,
-// write the sorted prefix-mode tables.  (Templated so that both
-// translation units can instantiate it.)
+// One workgroup per set: find the radius and length levels (staged in LDS),
+// walk the set's mesh steps (FP64 normalisation sums as k_expand,
+// acceptance mask and the bilinear tables), then pick the mode: rotation
+// directly when every point is accepted on equally spaced grids, else from
+// the prefix, interval and group structure of the acceptance mask; write the
+// tables of that mode, and for the rotation mode the scaled tables and the
+// moment series.  (Templated so that both translation units can
+// instantiate it.)
 template <int VARIANT = 0>
 __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tables t)
 {
@@ -971,7 +974,7 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     }
 }
 
-// Loads for rank_one_sums from host or device tables.
+// Loads of the rotation-mode tables from host or device memory.
 SAS_HD float4 ldv(const float4 *p) { return *p; }
 __device__ __forceinline__ float4 ldv(const SAS_CONST float4 *p) { return ld4(p); }
 
@@ -979,10 +982,13 @@ constexpr int RESTART = 8;   // rotation mode: direct sin/cos every RESTART poin
 
 // Rotation mode (every mesh point accepted, both grids equally spaced, in
 // grid order, grid step in .w): F = (sum_i u1_i J_i)(sum_j v1_j S_j) and
-// F^2 = (sum_i u2_i J_i^2)(sum_j v2_j S_j^2), with sin and cos along each
+// F^2 = (sum_i u2_i J_i^2)(sum_j v2_j S_j^2).  Per factor and wave: the
+// moment series when every argument is small, else sin and cos along the
 // grid advanced by rotating through the step angle (the exact product of
 // q and the step, reduced like any argument) and recomputed directly every
-// RESTART points; S = sin(y)/y by its Taylor series below |y| = 2.
+// RESTART points, S = sin(y)/y by its Taylor series below |y| = 2 and J by
+// its polynomial below 2 where some argument of the wave is that small.
+
 // Wave-uniform vote (every lane of the wave) on the device, the value itself on the host.
 SAS_HD bool wave_all(bool p)
 {
@@ -1109,25 +1115,6 @@ SAS_HD void radius_sums(P4 ROW, int NR, float qs, float &U1, float &U2)
         if (mixed) radius_run<true>(ROW, i0, n, qs, s, c, sd, cd, U1, U2);
         else radius_run<false>(ROW, i0, n, qs, s, c, sd, cd, U1, U2);
     }
-}
-
-// Rotation mode (every mesh point accepted, both grids equally spaced, in
-// grid order, grid step in .w): F = (sum_i u1_i J_i)(sum_j v1_j S_j) and
-// F^2 = (sum_i u2_i J_i^2)(sum_j v2_j S_j^2), with sin and cos along each
-// grid advanced by rotating through the step angle (the exact product of
-// q and the step, reduced like any argument) and recomputed directly every
-// RESTART points; S = sin(y)/y by its Taylor series below |y| = 2, J by its
-// polynomial below 2.  Loops are specialised on wave-uniform votes: all
-// arguments of the wave small (series only), all large (rotation only),
-// or mixed.
-template <class P4>
-SAS_HD void rank_one_sums(P4 ROW, P4 COL, int NR, int NL, float q, float sn, float cs, float &f1, float &f2)
-{
-    float T1, T2, U1, U2;
-    length_sums(COL, NL, fabsf(q * cs), T1, T2);
-    radius_sums(ROW, NR, fabsf(q * sn), U1, U2);
-    f1 = U1 * T1;
-    f2 = U2 * T2;
 }
 
 // Two series in one z, FP64: P = sum_k<N1 a_k z^k and Q = sum_k<N2 b_k z^k.
