@@ -181,3 +181,27 @@ def test_batched_separable_graph_replay_and_options():
             compare_raw(other[off:off + 2*nq + 4], eager[off:off + 2*nq + 4], nq, 2, rtol=1e-6)
             off += 2*nq + 4
     batch.release()
+
+
+@pytest.mark.parametrize("separable", [True, False])
+def test_batched_few_items(separable):
+    """Fewer work items than queue counters (down to one item): every item
+    is still handed out once, by the separable and the generic kernel."""
+    from gpu_helpers import hip_model
+    from oracle.oracle import OracleKernel
+    from sasmodels_amd.kernelhip import HipBatch
+    hm = hip_model("cylinder", 76)
+    q = np.logspace(-3, -0.5, 40)
+    hk, _, det, val, mag = buffers("cylinder", [q], _pars(35, 35), 76)
+    ref = OracleKernel(hm.info, [q]).raw(det, val, 0.0, False, 0)
+    for nsets in (1, 3):
+        batch = HipBatch(hm, [[q]]*nsets)
+        batch.set_separable(separable)
+        for nodes in (128, 16, 1):
+            print("few items", separable, nsets, nodes, flush=True)
+            batch.set_options(pts_per_item=nodes, waves_per_cu=8)
+            raw, _ = batch.raw([(det, val)]*nsets, 0.0)
+            n = 2*q.size + 4
+            for s in range(nsets):
+                compare_raw(raw[s*n:(s + 1)*n], ref, q.size, 2)
+        batch.release()
