@@ -254,16 +254,48 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
     }
 }
 
+// P@S (sasmodels/product.py:425-528): S's parameters for set s, at the
+// effective radius of the form (mode > 0) or the given one and at
+// volfraction * form:shell volume ratio, from the staged structure
+// parameters sp and the set's weight sums sm; total and shell with the
+// guards of Kernel.Fq (total weight 0 -> 1, shell volume 0 -> 1).
+__device__ __forceinline__ void structure_inputs(const double *sp, const double *sm, double *sq_pars, double &total,
+                                                 double &shell)
+{
+    total = sm[0];
+    if (total == 0.0) total = 1.0;
+    shell = sm[2] / total;
+    if (shell == 0.0) shell = 1.0;
+    sq_pars[0] = sp[3] > 0.0 ? sm[3] / total : sp[0];
+    sq_pars[1] = sp[1] * ((sm[1] / total) / shell);
+    for (int k = 2; k < 6; k++) sq_pars[k] = sp[3 + k];
+}
+
+// One thread per set: the q-independent part of S (the Hayter MSA's
+// coefficient iteration), before k_batch_final.
+__global__ __launch_bounds__(64) void k_structure_prep(int nsets, int s_model, const double *__restrict__ spars,
+                                                       const double *__restrict__ sums, double *__restrict__ sprep)
+{
+    const int s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= nsets) return;
+    double sq_pars[6], total, shell;
+    structure_inputs(spars + SASHIP_SPARS * s, sums + 4 * s, sq_pars, total, shell);
+    structure_prepare(s_model, sq_pars, sprep + S_PREP * s);
+}
+
 // Per set: sum the chunk partials of every output in chunk order, add the
 // weight sums, and form I(q) = scale * <F^2>/<V> + background with the
-// guards of Kernel.Fq (total weight 0 -> 1, shell volume 0 -> 1).
+// guards of Kernel.Fq (total weight 0 -> 1, shell volume 0 -> 1); for P@S
+// with structure factor SM (0: none), one instance per structure factor so
+// that a plain batch carries none of their code and registers.
+template <int SM>
 __global__ __launch_bounds__(256) void k_batch_final(
     int nout, int nres, const int64_t *__restrict__ qoff, const double *__restrict__ values,
     const int64_t *__restrict__ voff, const int64_t *__restrict__ poff,
     const int64_t *__restrict__ resoff, const int *__restrict__ nchunk,
     const double *__restrict__ sums, const double *__restrict__ partial,
-    int s_model, const double *__restrict__ spars, const double *__restrict__ q64, int is2d,
-    double *__restrict__ result, double *__restrict__ iq_out)
+    const double *__restrict__ spars, const double *__restrict__ sprep,
+    const double *__restrict__ q64, int is2d, double *__restrict__ result, double *__restrict__ iq_out)
 {
     const int s = blockIdx.x;
     const int64_t q0 = qoff[s];
@@ -277,16 +309,13 @@ __global__ __launch_bounds__(256) void k_batch_final(
     if (shell == 0.0) shell = 1.0;
     const double scale = values[voff[s]], background = values[voff[s] + 1];
     double cscale = scale / shell;
-    // P@S (sasmodels/product.py:425-528): S at the effective radius of the
-    // form (mode > 0) or the given one, and at volfraction * form:shell
-    // volume ratio; the number density is volfraction / shell volume
-    const double *sp = spars ? spars + SASHIP_SPARS * s : nullptr;
+    // P@S: S's parameters (structure_inputs); the number density is
+    // volfraction / shell volume
+    const double *sp = SM != 0 ? spars + SASHIP_SPARS * s : nullptr;
     double sq_pars[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};     // radius, volume fraction, further S parameters
     bool beta = false;
     if (sp) {
-        sq_pars[0] = sp[3] > 0.0 ? sm[3] / total : sp[0];
-        sq_pars[1] = sp[1] * ((sm[1] / total) / shell);
-        for (int k = 2; k < 6; k++) sq_pars[k] = sp[3 + k];
+        structure_inputs(sp, sm, sq_pars, total, shell);
         beta = sp[2] > 0.0;
         if (sp[4] == 0.0) cscale *= sp[1];      // else the form carries its own volfraction
     }
@@ -305,7 +334,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
         if (sp) {
             const int64_t g = q0 + e;
             const double q = is2d ? sqrt(q64[2 * g] * q64[2 * g] + q64[2 * g + 1] * q64[2 * g + 1]) : q64[g];
-            const double S = structure_sq(s_model, q, sq_pars);
+            const double S = structure_sq_prepared(SM, q, sq_pars, sprep + S_PREP * s);
             const double F1 = f1 / total;
             v = beta ? v + F1 * F1 * (S - 1.0) : v * S;
         }
@@ -373,6 +402,7 @@ struct saship_batch {
     size_t recs_cap = 0;
     int *d_npts = nullptr;
     double *d_sums = nullptr;
+    double *d_sprep = nullptr;        // P@S: the q-independent part of S per set (S_PREP doubles)
     int *d_queue = nullptr;
     double *d_partial = nullptr;
     size_t partial_cap = 0;
@@ -637,11 +667,26 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
     }
 
-    k_batch_final<<<dim3(b->nsets, std::max(1, (p.max_nq + 255) / 256)), 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, d_values, d_voff, d_poff, d_resoff,
-                                                d_nchunk, b->d_sums, b->d_partial,
-                                                b->s_model, p.structure ? (const double *)(d + p.off_spars) : nullptr,
-                                                b->q64, b->is2d,
-                                                want_result ? d_result : nullptr, want_iq ? d_iq : nullptr);
+    const double *d_spars = p.structure ? (const double *)(d + p.off_spars) : nullptr;
+    if (p.structure) {
+        k_structure_prep<<<(b->nsets + 63) / 64, 64, 0, b->st>>>(b->nsets, b->s_model, d_spars, b->d_sums, b->d_sprep);
+        HIP_TRY(hipGetLastError());
+    }
+    const dim3 fgrid(b->nsets, std::max(1, (p.max_nq + 255) / 256));
+    double *f_result = want_result ? d_result : nullptr, *f_iq = want_iq ? d_iq : nullptr;
+#define SAS_FINAL(SM)                                                                                        \
+    k_batch_final<SM><<<fgrid, 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, d_values, d_voff, d_poff, d_resoff, \
+                                                d_nchunk, b->d_sums, b->d_partial, d_spars, b->d_sprep, b->q64,  \
+                                                b->is2d, f_result, f_iq)
+    switch (p.structure ? b->s_model : 0) {
+    case 0: SAS_FINAL(0); break;
+    case S_HARDSPHERE: SAS_FINAL(S_HARDSPHERE); break;
+    case S_STICKY: SAS_FINAL(S_STICKY); break;
+    case S_SQUAREWELL: SAS_FINAL(S_SQUAREWELL); break;
+    case S_HAYTER: SAS_FINAL(S_HAYTER); break;
+    default: return fail(SASHIP_EINVAL, "unknown structure factor " + std::to_string(b->s_model));
+    }
+#undef SAS_FINAL
     HIP_TRY(hipGetLastError());
     double *d_siq = d_iq + p.nq_total;
     if (want_iq && b->n_smear > 0) {
@@ -800,6 +845,7 @@ int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_qoff, sizeof(int64_t) * (nsets + 1));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_npts, sizeof(int) * nsets);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_sums, sizeof(double) * 4 * nsets);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_sprep, sizeof(double) * sas::S_PREP * nsets);
     if (e == hipSuccess)
         e = hipMalloc((void **)&b->d_queue, sizeof(int) * sas::WORK_QUEUES * sas::WORK_QUEUE_STRIDE);
     if (e == hipSuccess && nqv) e = hipMemcpy(b->q32, q32.data(), sizeof(float) * nqv, hipMemcpyHostToDevice);
@@ -1062,7 +1108,7 @@ int saship_batch_release(saship_batch *b)
     (void)hipSetDevice(b->dev);
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
-    void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums,
+    void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums, b->d_sprep,
                         b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw};
     for (void *ptr : dev_ptrs)
         if (ptr) (void)hipFree(ptr);

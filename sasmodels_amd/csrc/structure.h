@@ -394,14 +394,15 @@ __host__ __device__ inline double sq(const State &st, double qsig)
 
 }  // namespace msa
 
-// Hayter-Penfold rescaled MSA S(q): sphere radius (A), volume fraction,
+// Hayter-Penfold rescaled MSA for sphere radius (A), volume fraction,
 // charge (e), temperature (K), 1:1 salt concentration (mol/l) and the
-// solvent's relative permittivity.  The ionic strength counts the
+// solvent's relative permittivity: the unscaled volume fraction, contact
+// potential and screening constant.  The ionic strength counts the
 // monovalent counterions of the spheres and the salt; the Debye screening
 // constant and the contact potential follow in SI units with the constants
-// of the reference.  NaN when the coefficients do not converge.
-__host__ __device__ inline double hayter_msa_sq(double q, double radius, double phi, double charge, double temperature,
-                                                double csalt, double dielec)
+// of the reference.
+__host__ __device__ inline void hayter_msa_setup(double radius, double phi, double charge, double temperature,
+                                                 double csalt, double dielec, msa::State &st)
 {
     #pragma clang fp contract(off)
     const double e_charge = 1.602189e-19, k_b = 1.380662e-23, eps0 = 8.85418782E-12;
@@ -415,13 +416,20 @@ __host__ __device__ inline double hayter_msa_sq(double q, double radius, double 
     const double cs = csalt * 6.022E23 * 1.0E3;
     const double ionst = 0.5 * e_charge * e_charge * (charge * phi / vp + 2.0 * cs);
     const double kappa = sqrt(2 * beta * ionst / perm);
-    msa::State st;
     st.gek0 = beta * zc * zc / (3.141592653589793 * perm * sidiam * ((2.0 + kappa * sidiam) * (2.0 + kappa * sidiam)));
     st.ak0 = kappa * sidiam;
     st.eta0 = phi;
     st.eta = phi;
+}
+
+// S(q) of the Hayter MSA; NaN when the coefficients do not converge.
+__host__ __device__ inline double hayter_msa_sq(double q, double radius, double phi, double charge, double temperature,
+                                                double csalt, double dielec)
+{
+    msa::State st;
+    hayter_msa_setup(radius, phi, charge, temperature, csalt, dielec, st);
     if (msa::solve(st) < 0) return __builtin_nan("");
-    return msa::sq(st, q * diam);
+    return msa::sq(st, q * (2.0 * radius));
 }
 
 // S(q) of structure-factor model `id` with p = (radius_effective,
@@ -431,6 +439,49 @@ __host__ __device__ inline double structure_sq(int id, double q, const double *p
     if (id == S_STICKY) return stickyhardsphere_sq(q, p[0], p[1], p[2], p[3]);
     if (id == S_SQUAREWELL) return squarewell_sq(q, p[0], p[1], p[2], p[3]);
     if (id == S_HAYTER) return hayter_msa_sq(q, p[0], p[1], p[2], p[3], p[4], p[5]);
+    return hardsphere_sq(q, p[0], p[1]);
+}
+
+// The q-independent part of a structure factor for one parameter set, in
+// S_PREP doubles: the solved Hayter MSA coefficients (a, b, c, f, eta, gek,
+// k, s; a = NaN when they do not converge), nothing for the closed forms.
+// Solving once per set keeps the iteration (and its registers) out of the
+// kernel that evaluates S at every q.
+constexpr int S_PREP = 8;
+__host__ __device__ inline void structure_prepare(int id, const double *p, double *prep)
+{
+    if (id != S_HAYTER) return;
+    msa::State st;
+    hayter_msa_setup(p[0], p[1], p[2], p[3], p[4], p[5], st);
+    const bool ok = msa::solve(st) >= 0;
+    prep[0] = ok ? st.a : __builtin_nan("");
+    prep[1] = st.b;
+    prep[2] = st.c;
+    prep[3] = st.f;
+    prep[4] = st.eta;
+    prep[5] = st.gek;
+    prep[6] = st.ak;
+    prep[7] = st.s;
+}
+
+// S(q) from structure_prepare's output (same value as structure_sq).
+__host__ __device__ inline double structure_sq_prepared(int id, double q, const double *p, const double *prep)
+{
+    if (id == S_STICKY) return stickyhardsphere_sq(q, p[0], p[1], p[2], p[3]);
+    if (id == S_SQUAREWELL) return squarewell_sq(q, p[0], p[1], p[2], p[3]);
+    if (id == S_HAYTER) {
+        if (prep[0] != prep[0]) return prep[0];
+        msa::State st;
+        st.a = prep[0];
+        st.b = prep[1];
+        st.c = prep[2];
+        st.f = prep[3];
+        st.eta = prep[4];
+        st.gek = prep[5];
+        st.ak = prep[6];
+        st.s = prep[7];
+        return msa::sq(st, q * (2.0 * p[0]));
+    }
     return hardsphere_sq(q, p[0], p[1]);
 }
 
