@@ -470,7 +470,7 @@ __host__ __device__ inline double structure_sq_prepared(int id, double q, const 
     if (id == S_STICKY) return stickyhardsphere_sq(q, p[0], p[1], p[2], p[3]);
     if (id == S_SQUAREWELL) return squarewell_sq(q, p[0], p[1], p[2], p[3]);
     if (id == S_HAYTER) {
-        if (prep[0] != prep[0]) return prep[0];
+        if (prep[0] != prep[0]) return prep[0];   // NaN: the iteration did not converge
         msa::State st;
         st.a = prep[0];
         st.b = prep[1];
