@@ -320,10 +320,27 @@ __global__ __launch_bounds__(256) void k_batch_final(
         if (sp[4] == 0.0) cscale *= sp[1];      // else the form carries its own volfraction
     }
     for (int e = blockIdx.y * 256 + threadIdx.x; e < nq; e += 256 * gridDim.y) {
+        // chunk order is kept; the loads of 8 chunks are issued before their
+        // adds so that their latencies overlap
         double f2 = 0.0, f1 = 0.0;
-        for (int c = 0; c < nc; c++) {
-            f2 += part[(size_t)c * nout * nq + e];
-            if (nout == 2) f1 += part[(size_t)c * nout * nq + nq + e];
+        const size_t cstride = (size_t)nout * nq;
+        int c = 0;
+        for (; c + 8 <= nc; c += 8) {
+            double a[8], b[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                a[k] = part[(size_t)(c + k) * cstride + e];
+                b[k] = nout == 2 ? part[(size_t)(c + k) * cstride + nq + e] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                f2 += a[k];
+                if (nout == 2) f1 += b[k];
+            }
+        }
+        for (; c < nc; c++) {
+            f2 += part[(size_t)c * cstride + e];
+            if (nout == 2) f1 += part[(size_t)c * cstride + nq + e];
         }
         if (result) {
             result[resoff[s] + (size_t)e * nres] = f2;
