@@ -600,6 +600,17 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
             ipre[i + 1] = ipre[i] + ((nq + sas::QTILE - 1) / sas::QTILE) * nblk;
         }
     }
+    // Counter c of the device queue hands out the slice
+    // [c n / WORK_QUEUES, (c + 1) n / WORK_QUEUES) of the list in order, so the
+    // longest-first order is dealt round-robin over the slices: every counter
+    // starts on long items and ends on short ones, and waves that move to
+    // another counter after theirs runs dry find short items there.
+    int64_t slot_next[sas::WORK_QUEUES], slot_end[sas::WORK_QUEUES];
+    for (int c = 0; c < sas::WORK_QUEUES; c++) {
+        slot_next[c] = (int64_t)c * p.n_items / sas::WORK_QUEUES;
+        slot_end[c] = (int64_t)(c + 1) * p.n_items / sas::WORK_QUEUES;
+    }
+    int deal = 0;
     for (int fill = p.sep_nlp ? 0 : P; fill >= 1; fill--) {
         for (int s = 0; s < ns; s++) {
             const int64_t num_eval = details[(size_t)s * n_details + n_details - 4];
@@ -608,10 +619,16 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
             for (int c = 0; c < nchunk[s]; c++) {
                 const int64_t here = std::min<int64_t>(P, num_eval - (int64_t)c * P);
                 if (std::max<int64_t>(1, here) != fill) continue;
-                for (int t = 0; t < ntile; t++) it[k++] = make_int4(s, t, c, 0);
+                for (int t = 0; t < ntile; t++) {
+                    while (slot_next[deal] == slot_end[deal]) deal = (deal + 1) % sas::WORK_QUEUES;
+                    it[slot_next[deal]++] = make_int4(s, t, c, 0);
+                    deal = (deal + 1) % sas::WORK_QUEUES;
+                    k++;
+                }
             }
         }
     }
+    if (k != p.n_items && !p.sep_nlp) return fail(SASHIP_EINVAL, "internal: work list size mismatch");
     *(int32_t *)(h + p.off_nitems) = p.n_items;
     return 0;
 }
@@ -891,6 +908,7 @@ int saship_batch_set_options(saship_batch *b, int32_t use_graph, int32_t pts_per
     b->pts_per_item = pts_per_item;
     b->waves_per_cu = waves_per_cu;
     drop_graph(b);
+    b->have_last = false;      // the staged work list was cut for the old item size
     return 0;
 }
 
@@ -900,6 +918,7 @@ int saship_batch_set_path(saship_batch *b, int32_t separable)
     std::lock_guard<std::mutex> lock(b->mu);
     b->separable = separable != 0;
     drop_graph(b);
+    b->have_last = false;      // the staged plan belongs to the other path
     return 0;
 }
 
@@ -910,6 +929,7 @@ int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int
     std::lock_guard<std::mutex> lock(b->mu);
     HIP_TRY(hipSetDevice(b->dev));
     drop_graph(b);
+    b->have_last = false;      // the output buffer was sized for the old smearing
     void *old[] = {b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw};
     for (void *ptr : old)
         if (ptr) (void)hipFree(ptr);
@@ -1093,6 +1113,7 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
     b->t_all_ms = t_all_keep;
     b->ncalls = n_keep;
     drop_graph(b);
+    b->have_last = false;      // the last staged plan is the sweep's last point
     if (rc) return rc;
     *best_pts_per_item = best_p;
     *best_waves_per_cu = best_w;

@@ -12,6 +12,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -114,84 +115,163 @@ struct saship_kernel {
 
 namespace {
 
-// Launch the kernels of one evaluation for model M.
+// Where one evaluation reads q and leaves its intermediate and final sums:
+// the kernel handle's own buffers, or the no-allocation scratch.
+struct Target {
+    hipStream_t st;
+    const saship_model *m;
+    int nq, is2d, cus;
+    const float *q32;
+    const double *q64;
+    float4 *recs;        // compacted mesh records
+    char *meta;          // npts (int) at 0, sums (4 doubles) at 64
+    double *partial;     // per-split partial sums
+    double *result;      // nout*nq + 4 doubles, reference layout
+    char *sep;           // separable tables, sas::sep::tables_bytes(1, SEP_MAX)
+    hipEvent_t ev0, ev1; // NULL: untimed
+};
+
+// Calls f((M *)nullptr) with M the device functor of model id.
+template <class F>
+int with_model(int id, F &&f)
+{
+    switch (id) {
+    case 0: return f((sas::Sphere *)nullptr);
+    case 1: return f((sas::Cylinder *)nullptr);
+    case 2: return f((sas::Ellipsoid *)nullptr);
+    case 3: return f((sas::CoreShellCylinder *)nullptr);
+    case 4: return f((sas::CoreShellSphere *)nullptr);
+    case 6: return f((sas::Parallelepiped *)nullptr);
+    case 7: return f((sas::TriaxialEllipsoid *)nullptr);
+    case 8: return f((sas::BccParacrystal *)nullptr);
+    case 9: return f((sas::FccParacrystal *)nullptr);
+    case 10: return f((sas::ScParacrystal *)nullptr);
+    case 11: return f((sas::HollowCylinder *)nullptr);
+    case 12: return f((sas::Vesicle *)nullptr);
+    case 13: return f((sas::FuzzySphere *)nullptr);
+    default: return fail(SASHIP_EINVAL, "model " + std::to_string(id) + " has no mesh kernel");
+    }
+}
+
+// Record width (float4 units) of model id for kernel kind.
+int record_width(int id, int kind)
+{
+    return with_model(id, [&](auto *tag) {
+        using M = std::remove_pointer_t<decltype(tag)>;
+        return kind == 0 ? sas::rv1<M>() : sas::rv2<M>();
+    });
+}
+
+// Mesh splits of the generic path: about 8 workgroups on every CU.
+int split_count(int cus, int nq, int npts_max)
+{
+    const int ntiles = std::max(1, (nq + sas::EVAL_THREADS - 1) / sas::EVAL_THREADS);
+    const int target = std::max(1, (cus * 8) / ntiles);
+    const int ppb = std::max(1, (npts_max + target - 1) / target);
+    return std::max(1, std::min(65535, (npts_max + ppb - 1) / ppb));
+}
+
+// Launch the kernels of one evaluation for model M with nsplit mesh splits
+// (t.recs holds pd_stop - pd_start records, t.partial nout*nq*nsplit doubles).
 template <class M>
-int launch(saship_kernel *k, int kind, const int32_t *d_det, const double *d_val, int n_details,
-           int pd_start, int pd_stop, double cutoff, int reff_mode, int accumulate)
+int launch(const Target &t, int kind, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
+           int pd_stop, double cutoff, int reff_mode, int accumulate, int nsplit)
 {
     using namespace sas;
-    const int nq = k->nq;
+    const int nq = t.nq;
     const int nout = kind == 0 ? 2 : 1;
-    const int npts_max = std::max(0, pd_stop - pd_start);
-    const int rv = kind == 0 ? rv1<M>() : rv2<M>();
-    if (int e = ensure(k->d_recs, k->recs_cap, sizeof(float4) * (size_t)std::max(1, npts_max) * rv)) return e;
-    int *npts = (int *)k->d_meta;
-    double *sums = (double *)(k->d_meta + 64);
+    int *npts = (int *)t.meta;
+    double *sums = (double *)(t.meta + 64);
 
     ExpandArgs ea{d_det, d_val, nullptr, nullptr, n_details, pd_start, pd_stop, cutoff, reff_mode,
-                  k->d_recs, npts, sums, nullptr};
+                  t.recs, npts, sums, nullptr};
     switch (kind) {
-    case 0: k_expand<M, 0><<<1, EXPAND_THREADS, 0, k->st>>>(ea); break;
-    case 1: k_expand<M, 1><<<1, EXPAND_THREADS, 0, k->st>>>(ea); break;
-    default: k_expand<M, 2><<<1, EXPAND_THREADS, 0, k->st>>>(ea); break;
+    case 0: k_expand<M, 0><<<1, EXPAND_THREADS, 0, t.st>>>(ea); break;
+    case 1: k_expand<M, 1><<<1, EXPAND_THREADS, 0, t.st>>>(ea); break;
+    default: k_expand<M, 2><<<1, EXPAND_THREADS, 0, t.st>>>(ea); break;
     }
     HIP_TRY(hipGetLastError());
 
-    // Split the mesh so that about 8 workgroups land on every CU.
     const int ntiles = std::max(1, (nq + EVAL_THREADS - 1) / EVAL_THREADS);
-    const int target = std::max(1, (k->cus * 8) / ntiles);
-    const int ppb = std::max(1, (npts_max + target - 1) / target);
-    const int nsplit = std::max(1, std::min(65535, (npts_max + ppb - 1) / ppb));
-    if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * (size_t)nout * nq * nsplit)) return e;
-
     const dim3 grid(ntiles, nsplit);
-    if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
+    if (t.ev0) HIP_TRY(hipEventRecord(t.ev0, t.st));
     if (kind == 0) {
-        if (k->m->quad_n > 0)
-            k_fq1d<M, true><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q32, k->q64, k->d_recs, npts, nsplit, k->m->quad,
-                                                              k->d_partial);
+        if (t.m->quad_n > 0)
+            k_fq1d<M, true><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q32, t.q64, t.recs, npts, nsplit, t.m->quad,
+                                                             t.partial);
         else
-            k_fq1d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q32, k->q64, k->d_recs, npts, nsplit, k->m->quad,
-                                                               k->d_partial);
+            k_fq1d<M, false><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q32, t.q64, t.recs, npts, nsplit, t.m->quad,
+                                                              t.partial);
     } else if (kind == 1) {
-        k_iq2d<M, false><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q64, d_val, k->d_recs, npts, nsplit, k->d_partial);
+        k_iq2d<M, false><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q64, d_val, t.recs, npts, nsplit, t.partial);
     } else {
-        k_iq2d<M, true><<<grid, EVAL_THREADS, 0, k->st>>>(nq, k->q64, d_val, k->d_recs, npts, nsplit, k->d_partial);
+        k_iq2d<M, true><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q64, d_val, t.recs, npts, nsplit, t.partial);
     }
     HIP_TRY(hipGetLastError());
-    if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
+    if (t.ev1) HIP_TRY(hipEventRecord(t.ev1, t.st));
 
     // models without <F> return only the <F^2> sums (nout 1 in 1D too)
     const int nres = M::FQ ? nout : 1;
     const int fblocks = std::max(1, (nres * nq + 3) / 4);
-    k_finalize<<<fblocks, 256, 0, k->st>>>(nq, nres, nsplit, k->d_partial, sums, accumulate, k->d_result);
+    k_finalize<<<fblocks, 256, 0, t.st>>>(nq, nres, nsplit, t.partial, sums, accumulate, t.result);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-// Separable cylinder path: tables, one wave per (q tile, node), FP64 node sums.
-int launch_sep(saship_kernel *k, int nlp, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
+int launch_model(const Target &t, int kind, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
+                 int pd_stop, double cutoff, int reff_mode, int accumulate, int nsplit)
+{
+    return with_model(t.m->id, [&](auto *tag) {
+        using M = std::remove_pointer_t<decltype(tag)>;
+        return launch<M>(t, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, nsplit);
+    });
+}
+
+// Separable cylinder path: tables, one wave per (q tile, node), FP64 node sums
+// (t.sep holds the tables, t.partial 2*nq*nodes doubles).
+int launch_sep(const Target &t, int nlp, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
                int pd_stop, double cutoff, int reff_mode, int accumulate)
 {
     using namespace sas;
-    const int nq = k->nq, nnode = k->m->quad.n;
-    if (!k->d_sep) HIP_TRY(hipMalloc((void **)&k->d_sep, sep::tables_bytes(1, SEP_MAX)));
-    const sep::Tables t = sep::carve(k->d_sep, 1, nlp);
-    double *sums = (double *)(k->d_meta + 64);
+    const int nq = t.nq, nnode = t.m->quad.n;
+    const sep::Tables tb = sep::carve(t.sep, 1, nlp);
+    double *sums = (double *)(t.meta + 64);
     ExpandArgs ea{d_det, d_val, nullptr, nullptr, n_details, pd_start, pd_stop, cutoff, reff_mode,
-                  nullptr, (int *)k->d_meta, sums, nullptr};
-    sep::k_sep_expand<0><<<1, EXPAND_THREADS, 0, k->st>>>(ea, t);
+                  nullptr, (int *)t.meta, sums, nullptr};
+    sep::k_sep_expand<0><<<1, EXPAND_THREADS, 0, t.st>>>(ea, tb);
     HIP_TRY(hipGetLastError());
-    if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * 2 * (size_t)nq * nnode)) return e;
     const dim3 grid(std::max(1, (nq + 63) / 64), (nnode + 3) / 4);
-    if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
-    sep::k_fq1d_sep<0><<<grid, 256, 0, k->st>>>(nq, k->q32, t, k->m->quad, k->d_partial);
+    if (t.ev0) HIP_TRY(hipEventRecord(t.ev0, t.st));
+    sep::k_fq1d_sep<0><<<grid, 256, 0, t.st>>>(nq, t.q32, tb, t.m->quad, t.partial);
     HIP_TRY(hipGetLastError());
-    if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
+    if (t.ev1) HIP_TRY(hipEventRecord(t.ev1, t.st));
     const int fblocks = std::max(1, (2 * nq + 3) / 4);
-    k_finalize<<<fblocks, 256, 0, k->st>>>(nq, 2, nnode, k->d_partial, sums, accumulate, k->d_result);
+    k_finalize<<<fblocks, 256, 0, t.st>>>(nq, 2, nnode, t.partial, sums, accumulate, t.result);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+// Structure-factor evaluation (nothing polydisperse).
+int launch_structure(const Target &t, const double *d_val, double cutoff, int accumulate)
+{
+    const int id = t.m->id;
+    if (t.ev0) HIP_TRY(hipEventRecord(t.ev0, t.st));
+    const int include = 1.0 > cutoff ? 1 : 0;
+    sas::k_structure<<<std::max(1, (t.nq + 255) / 256), 256, 0, t.st>>>(id, model_npars(id), t.nq, t.is2d, t.q64,
+                                                                       d_val, include, accumulate, t.result);
+    HIP_TRY(hipGetLastError());
+    if (t.ev1) HIP_TRY(hipEventRecord(t.ev1, t.st));
+    return 0;
+}
+
+// Which path evaluates a call: 0 structure factor, 1 separable cylinder
+// (*nlp set), 2 generic mesh kernels.
+int choose_path(const saship_model *m, int kind, const int32_t *details, bool separable, int *nlp)
+{
+    *nlp = 0;
+    if (model_structure(m->id)) return 0;
+    if (m->id == 1 && kind == 0 && m->quad_n > 0 && separable) *nlp = cylinder_separable(details);
+    return *nlp ? 1 : 2;
 }
 
 static_assert(SEP_MAX == sas::sep::NRMAX, "separable table size");
@@ -483,36 +563,29 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     }
     const int32_t *d_det = (const int32_t *)(k->d_args + doff);
     const double *d_val = (const double *)k->d_args;
+    Target t{k->st, k->m, nq, k->is2d, k->cus, k->q32, k->q64, nullptr, k->d_meta, nullptr, k->d_result, nullptr,
+             k->timing ? k->ev0 : nullptr, k->timing ? k->ev1 : nullptr};
+    int nlp = 0;
+    const int path = choose_path(k->m, kind, details, k->separable, &nlp);
     int rc;
-    if (model_structure(id)) {
-        if (k->timing) HIP_TRY(hipEventRecord(k->ev0, k->st));
-        const int include = 1.0 > cutoff ? 1 : 0;
-        sas::k_structure<<<std::max(1, (nq + 255) / 256), 256, 0, k->st>>>(id, model_npars(id), nq, k->is2d, k->q64,
-                                                                           d_val, include, accumulate, k->d_result);
-        HIP_TRY(hipGetLastError());
-        if (k->timing) HIP_TRY(hipEventRecord(k->ev1, k->st));
-        return finish_call(k, rbytes, result);
-    }
-    const int nlp = id == 1 && kind == 0 && k->m->quad_n > 0 && k->separable ? cylinder_separable(details) : 0;
-    if (nlp) {
-        rc = launch_sep(k, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
-        if (rc) return rc;
-        return finish_call(k, rbytes, result);
-    }
-    switch (id) {
-    case 0: rc = launch<sas::Sphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 1: rc = launch<sas::Cylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 2: rc = launch<sas::Ellipsoid>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 3: rc = launch<sas::CoreShellCylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 4: rc = launch<sas::CoreShellSphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 6: rc = launch<sas::Parallelepiped>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 7: rc = launch<sas::TriaxialEllipsoid>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 8: rc = launch<sas::BccParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 9: rc = launch<sas::FccParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 10: rc = launch<sas::ScParacrystal>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 11: rc = launch<sas::HollowCylinder>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    case 12: rc = launch<sas::Vesicle>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
-    default: rc = launch<sas::FuzzySphere>(k, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate); break;
+    if (path == 0) {
+        rc = launch_structure(t, d_val, cutoff, accumulate);
+    } else if (path == 1) {
+        if (!k->d_sep) HIP_TRY(hipMalloc((void **)&k->d_sep, sas::sep::tables_bytes(1, SEP_MAX)));
+        if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * 2 * (size_t)nq * k->m->quad.n)) return e;
+        t.sep = k->d_sep;
+        t.partial = k->d_partial;
+        rc = launch_sep(t, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
+    } else {
+        const int npts_max = std::max(0, pd_stop - pd_start);
+        const int rv = record_width(id, kind);
+        const int nsplit = split_count(k->cus, nq, npts_max);
+        if (int e = ensure(k->d_recs, k->recs_cap, sizeof(float4) * (size_t)std::max(1, npts_max) * rv)) return e;
+        if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * (kind == 0 ? 2 : 1) * (size_t)nq * nsplit))
+            return e;
+        t.recs = k->d_recs;
+        t.partial = k->d_partial;
+        rc = launch_model(t, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, nsplit);
     }
     if (rc) return rc;
     return finish_call(k, rbytes, result);

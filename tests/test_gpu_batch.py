@@ -177,3 +177,35 @@ def test_replay_resident():
     again, _ = batch.raw(calls)
     assert np.array_equal(first, again)
     batch.release()
+
+
+def test_replay_needs_a_fresh_call_after_changes():
+    """A replay reruns the staged plan of the last call; any change of options,
+    path, smearing or structure factor invalidates it until the next call."""
+    from sasmodels_amd._lib import HipError
+    from sasmodels_amd.resolution import Pinhole1D
+    model = hip_model("cylinder", 76)
+    q = np.logspace(-3, -0.5, 150)
+    pars = dict(radius=25, length=300, radius_pd=0.1, radius_pd_n=7, length_pd=0.1, length_pd_n=5)
+    kl = type("K", (), {"info": model.info, "dtype": np.dtype("d")})()
+    calls = [_pack(model, kl, pars, "1d")[:2]]*3
+    batch = HipBatch(model, [[q]]*3)
+    want = batch(calls)
+    changes = [
+        lambda: batch.set_options(pts_per_item=8),
+        lambda: batch.set_separable(False),
+        lambda: batch.set_separable(True),
+        lambda: batch.set_resolution([Pinhole1D(q, 0.05*q, q_calc=q)]*3),
+        lambda: batch.set_resolution(None),
+        lambda: batch.autotune(calls),
+    ]
+    for change in changes:
+        batch(calls)
+        assert batch.replay(1) > 0.0
+        change()
+        with pytest.raises(HipError):
+            batch.replay(1)
+    got = batch(calls)
+    for a, b in zip(got, want):
+        np.testing.assert_allclose(a, b, rtol=2e-6)
+    batch.release()
