@@ -133,6 +133,29 @@ int saship_kernel_call(saship_kernel *kernel, int32_t kind,
                        double cutoff, int32_t radius_effective_mode,
                        int32_t pd_start, int32_t pd_stop, double *result);
 
+/* The evaluation of saship_kernel_call without any allocation: no kernel
+ * handle, no hipMalloc or hipFree, no allocating library call.  No
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  q (nq doubles for SASHIP_KIND_IQ, 2*nq interleaved (qx, qy)
+ * otherwise) and result (nout*nq + 4 doubles, the layout and accumulate
+ * semantics of saship_kernel_call) are DEVICE memory; details and values are
+ * host memory, read before the call returns.  Work is enqueued on `stream`
+ * (a hipStream_t; NULL is the null stream) and the call returns without
+ * waiting for it.  The intermediate buffers (staged values, FP32 q, mesh
+ * records, partial sums) live in one fixed 32 MiB scratch per device,
+ * owned by the library until the process ends; calls that do not fit it
+ * (mesh records plus q-dependent partial sums) fail with SASHIP_EINVAL.
+ * Every call on a device shares that device's scratch: calls whose work
+ * overlaps in time on one device race, so serialise them (one stream, or
+ * synchronise between calls).  Same evaluation path as saship_kernel_call
+ * with the separable cylinder path allowed. */
+int saship_kernel_call_noalloc(saship_model *model, int32_t kind, int32_t nq, const double *q,
+                               const int32_t *details, int32_t n_details,
+                               const double *values, int32_t n_values,
+                               double cutoff, int32_t radius_effective_mode,
+                               int32_t pd_start, int32_t pd_stop, double *result, void *stream);
+
 /* Evaluation strategy: with separable != 0 (the default) a cylinder 1D call
  * with a fixed Gauss rule and a mesh over radius and length only is
  * evaluated as bilinear forms of the radius and length factors (same sums,

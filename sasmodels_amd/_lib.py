@@ -31,6 +31,10 @@ SIGNATURES = {
     "saship_kernel_call": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
                                       ct.c_void_p, ct.c_int32, ct.c_double, ct.c_int32,
                                       ct.c_int32, ct.c_int32, ct.c_void_p]),
+    "saship_kernel_call_noalloc": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_void_p,
+                                              ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
+                                              ct.c_double, ct.c_int32, ct.c_int32, ct.c_int32,
+                                              ct.c_void_p, ct.c_void_p]),
     "saship_probe_fma": (ct.c_int, [ct.c_int32, ct.POINTER(ct.c_double)]),
     "saship_batch_replay": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_double)]),
     "saship_batch_set_resolution": (ct.c_int, [ct.c_void_p]*7 + [ct.c_int64]),

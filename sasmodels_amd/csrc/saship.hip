@@ -298,6 +298,58 @@ __global__ __launch_bounds__(256) void k_fma_probe(float *out, int iters, float 
 
 }  // namespace
 
+// ------------------------------------------------------------------ no-allocation scratch
+// Device memory of saship_kernel_call_noalloc: staged values and details, FP32
+// q, mesh records, the weight sums and the per-split partial sums of one call,
+// carved per call.  Each device holds its own copy from the library's load
+// until the process ends.  Every buffer in it is written by the call before it
+// is read (nothing has to start at zero); calls that overlap on one device
+// share it and race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace sas {
+__global__ __launch_bounds__(256) void k_q_to_f32(int64_t n, const double *__restrict__ q64, float *__restrict__ q32)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) q32[i] = (float)q64[i];
+}
+}  // namespace sas
+
+namespace {
+std::mutex scratch_mu;
+std::vector<char *> scratch_dev;   // device address of `scratch` per device (NULL: not looked up)
+
+// Device address of `scratch` on device dev, which must be current.  The
+// symbol's host-side name is not a device address: kernels and copies get
+// the address looked up here.
+int scratch_address(int dev, char **out)
+{
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if ((int)scratch_dev.size() <= dev) scratch_dev.resize(dev + 1, nullptr);
+    if (!scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(SASHIP_EHIP, std::string("scratch address: ") + hipGetErrorString(e));
+        scratch_dev[dev] = (char *)ptr;
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
+// Is p device memory (or host memory the device can address)?
+bool device_accessible(const void *p)
+{
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.devicePointer != nullptr;
+}
+}  // namespace
+
 namespace {
 // Download the result of the evaluation just enqueued and account its time.
 int finish_call(saship_kernel *k, size_t rbytes, double *result)
@@ -336,7 +388,8 @@ int saship_init(int32_t device)
         return fail(SASHIP_EINVAL, "device " + std::to_string(device) + " out of range (" + std::to_string(n) + " devices)");
     HIP_TRY(hipSetDevice(device));
     current_device() = device;
-    return 0;
+    char *sp = nullptr;
+    return scratch_address(device, &sp);
 }
 
 int saship_probe_fma(int32_t iters, double *tflops)
@@ -589,6 +642,81 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     }
     if (rc) return rc;
     return finish_call(k, rbytes, result);
+}
+
+int saship_kernel_call_noalloc(saship_model *m, int32_t kind, int32_t nq, const double *q,
+                               const int32_t *details, int32_t n_details, const double *values, int32_t n_values,
+                               double cutoff, int32_t reff_mode, int32_t pd_start, int32_t pd_stop,
+                               double *result, void *stream)
+{
+    if (!m) return fail(SASHIP_EINVAL, "model is NULL");
+    if (!details || !values || !result) return fail(SASHIP_EINVAL, "details, values and result must not be NULL");
+    if (kind < 0 || kind > 2) return fail(SASHIP_EINVAL, "kind must be 0 (Iq), 1 (Iqxy) or 2 (Imagnetic)");
+    if (nq < 0) return fail(SASHIP_EINVAL, "nq must be >= 0");
+    if (nq > 0 && !q) return fail(SASHIP_EINVAL, "q is NULL");
+    const int id = m->id;
+    if (pd_stop < pd_start) return fail(SASHIP_EINVAL, "need 0 <= pd_start <= pd_stop");
+    if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, "")) return e;
+    HIP_TRY(hipSetDevice(m->dev));
+    if ((nq > 0 && !device_accessible(q)) || !device_accessible(result))
+        return fail(SASHIP_EINVAL, "q and result must be device memory");
+    char *base = nullptr;
+    if (int e = scratch_address(m->dev, &base)) return e;
+    hipStream_t st = (hipStream_t)stream;
+    const int is2d = kind != 0;
+    const int cus = cu_count(m->dev);
+
+    // carve: values | details | q32 | meta | records | separable tables | partials
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t o_val = take(sizeof(double) * (size_t)n_values);
+    const size_t o_det = take(sizeof(int32_t) * (size_t)n_details);
+    const int64_t nqv = (int64_t)nq * (is2d ? 2 : 1);
+    const size_t o_q32 = take(sizeof(float) * (size_t)std::max<int64_t>(1, nqv));
+    const size_t o_meta = take(256);
+    int nlp = 0;
+    const int path = choose_path(m, kind, details, true, &nlp);
+    const int npts_max = std::max(0, pd_stop - pd_start);
+    const size_t o_recs = take(path == 2 ? sizeof(float4) * (size_t)std::max(1, npts_max) * record_width(id, kind) : 0);
+    const size_t o_sep = take(path == 1 ? sas::sep::tables_bytes(1, SEP_MAX) : 0);
+    const size_t o_part = off;
+    if (o_part > NOALLOC_SCRATCH_BYTES)
+        return fail(SASHIP_EINVAL, "call needs more than the " + std::to_string(NOALLOC_SCRATCH_BYTES >> 20) +
+                                       " MiB no-allocation scratch (mesh or values too large)");
+    const size_t room = NOALLOC_SCRATCH_BYTES - o_part;
+    int nsplit = 1;
+    size_t part_bytes = 0;
+    if (path == 1) {
+        part_bytes = sizeof(double) * 2 * (size_t)nq * m->quad.n;
+    } else if (path == 2) {
+        const size_t per_split = sizeof(double) * (kind == 0 ? 2 : 1) * (size_t)std::max(1, nq);
+        nsplit = split_count(cus, nq, npts_max);
+        nsplit = (int)std::min<size_t>((size_t)nsplit, room / per_split);
+        part_bytes = per_split * (size_t)nsplit;
+    }
+    if (nsplit < 1 || part_bytes > room)
+        return fail(SASHIP_EINVAL, "call needs more than the " + std::to_string(NOALLOC_SCRATCH_BYTES >> 20) +
+                                       " MiB no-allocation scratch (too many q points)");
+
+    double *d_val = (double *)(base + o_val);
+    int32_t *d_det = (int32_t *)(base + o_det);
+    float *q32 = (float *)(base + o_q32);
+    HIP_TRY(hipMemcpyAsync(d_val, values, sizeof(double) * (size_t)n_values, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_det, details, sizeof(int32_t) * (size_t)n_details, hipMemcpyHostToDevice, st));
+    if (nqv > 0) {
+        sas::k_q_to_f32<<<(unsigned)((nqv + 255) / 256), 256, 0, st>>>(nqv, q, q32);
+        HIP_TRY(hipGetLastError());
+    }
+    Target t{st, m, nq, is2d, cus, q32, q, (float4 *)(base + o_recs), base + o_meta, (double *)(base + o_part),
+             result, base + o_sep, nullptr, nullptr};
+    const int accumulate = pd_start > 0 ? 1 : 0;
+    if (path == 0) return launch_structure(t, d_val, cutoff, accumulate);
+    if (path == 1) return launch_sep(t, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
+    return launch_model(t, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, nsplit);
 }
 
 int saship_kernel_set_path(saship_kernel *k, int32_t separable)
