@@ -1,178 +1,139 @@
 """
-Kernel call details: the int32 mesh descriptor and the packed values vector.
+The two buffers of a kernel call: the int32 mesh descriptor and the values
+vector.
 
-Produces buffers with the same layout as the reference
-(sasmodels/details.py:43-254), so buffers built by either side can be
-passed to either backend:
+Their layouts are the drop-in contract with the reference's kernels
+(sasmodels/details.py:73-88, :225-254; kernel_iq.c:47-58), so buffers built
+here or by the reference work with either backend:
 
-* ``CallDetails.buffer`` = ``pd_par[max_pd] | pd_length[max_pd] |
-  pd_offset[max_pd] | pd_stride[max_pd] | num_eval | num_weights |
-  num_active | theta_par``;
-* mesh levels are the parameters sorted by decreasing distribution length
-  (``argsort(length)[::-1]``), level 0 innermost with stride 1;
-* ``values`` = scalars, then every parameter's distribution values, then its
-  weights, zero padded to a multiple of 32, in the kernel dtype;
-* magnetic (M0, theta, phi) triples are converted in place to (mx, my, mz).
+    descriptor  pd_par[max_pd] | pd_length[max_pd] | pd_offset[max_pd] |
+                pd_stride[max_pd] | num_eval | num_weights | num_active |
+                theta_par
+    values      one scalar per call parameter (scale, background, the kernel
+                parameters, then the magnetic block), the distribution points
+                of every kernel parameter back to back, their weights in the
+                same order, zeros up to a multiple of 32, in the kernel dtype
+
+Mesh level 0 is the innermost loop (stride 1); levels are the parameters in
+decreasing distribution length, ties in the order that reversing numpy's
+argsort of the lengths gives (details.py:207) -- the order decides the
+kernels' summation order and what a pd_start/pd_stop chunk of the mesh is.
+In the magnetic block each (M0, mtheta, mphi) triple becomes the vector
+(mx, my, mz) = M0 (sin t cos p, sin t sin p, cos t) in place, the kernels'
+convention (details.py:288-306).
+
+:func:`make_kernel_args` fills both buffers in one pass over the mesh.
 """
 
 import numpy as np
-from numpy import cos, radians, sin
 
 from .modelinfo import NUM_COMMON_PARS, NUM_MAGNETIC_PARS
 
 
+class _Trailer:
+    """One of the four scalars after the per-level arrays (-4 .. -1)."""
+    def __init__(self, index, doc):
+        self.index = index
+        self.__doc__ = doc
+
+    def __get__(self, obj, owner=None):
+        return self if obj is None else obj.buffer[self.index]
+
+    def __set__(self, obj, value):
+        obj.buffer[self.index] = value
+
+
 class CallDetails:
-    """Mesh descriptor for one kernel call (details.py:43-180)."""
-    parts = None
+    """
+    Mesh descriptor of one call: one int32 ``buffer`` and views of its
+    fields.  ``length``/``offset`` keep the per-parameter distribution sizes
+    and starts the descriptor was made from (products and mixtures re-slice
+    them per part).
+    """
+    num_eval = _Trailer(-4, "points in the whole mesh")
+    num_weights = _Trailer(-3, "length of the points (and of the weights) block of the values")
+    num_active = _Trailer(-2, "parameters with more than one point")
+    theta_par = _Trailer(-1, "kernel index of theta, -1 without orientation")
 
     def __init__(self, model_info):
-        parameters = model_info.parameters
-        max_pd = parameters.max_pd
-        self.buffer = np.empty(4*max_pd + 4, "i4")
-        self._pd_par = self.buffer[0*max_pd:1*max_pd]
-        self._pd_length = self.buffer[1*max_pd:2*max_pd]
-        self._pd_offset = self.buffer[2*max_pd:3*max_pd]
-        self._pd_stride = self.buffer[3*max_pd:4*max_pd]
-        self.theta_par = parameters.theta_offset
-        self.offset = None
-        self.length = None
+        max_pd = model_info.parameters.max_pd
         self.info = model_info
-
-    @property
-    def pd_par(self):
-        """Parameter index of each mesh level"""
-        return self._pd_par
-
-    @property
-    def pd_length(self):
-        """Distribution length of each mesh level"""
-        return self._pd_length
-
-    @property
-    def pd_offset(self):
-        """Offset of each level's distribution in the value/weight vectors"""
-        return self._pd_offset
-
-    @property
-    def pd_stride(self):
-        """Mesh stride of each level"""
-        return self._pd_stride
-
-    @property
-    def num_eval(self):
-        """Total mesh size"""
-        return self.buffer[-4]
-
-    @num_eval.setter
-    def num_eval(self, v):
-        self.buffer[-4] = v
-
-    @property
-    def num_weights(self):
-        """Total length of the distribution vectors"""
-        return self.buffer[-3]
-
-    @num_weights.setter
-    def num_weights(self, v):
-        self.buffer[-3] = v
-
-    @property
-    def num_active(self):
-        """Number of distributions longer than one point"""
-        return self.buffer[-2]
-
-    @num_active.setter
-    def num_active(self, v):
-        self.buffer[-2] = v
-
-    @property
-    def theta_par(self):
-        """Kernel-parameter index of theta, or -1"""
-        return self.buffer[-1]
-
-    @theta_par.setter
-    def theta_par(self, v):
-        self.buffer[-1] = v
-
-    def show(self, values=None):
-        """Print the mesh descriptor."""
-        print("===== %s details ====" % self.info.name)
-        print("num_active:%d  num_eval:%d  num_weights:%d  theta=%d"
-              % (self.num_active, self.num_eval, self.num_weights, self.theta_par))
-        if self.pd_par.size:
-            print("pd_par", self.pd_par)
-            print("pd_length", self.pd_length)
-            print("pd_offset", self.pd_offset)
-            print("pd_stride", self.pd_stride)
-        if values is not None:
-            nvalues = self.info.parameters.nvalues
-            print("scale, background", values[:2])
-            print("val", values[2:nvalues])
-            print("pd", values[nvalues:nvalues + self.num_weights])
-            print("wt", values[nvalues + self.num_weights:nvalues + 2*self.num_weights])
+        self.buffer = np.zeros(4 * max_pd + 4, np.int32)
+        fields = self.buffer[:4 * max_pd].reshape(4, max_pd)
+        self.pd_par, self.pd_length, self.pd_offset, self.pd_stride = fields
+        self.theta_par = model_info.parameters.theta_offset
+        self.length = self.offset = None
 
 
 def make_details(model_info, length, offset, num_weights):
     """
-    Mesh descriptor for distribution *length* and *offset* per kernel
-    parameter (details.py:183-221).
+    Descriptor for kernel parameters whose distributions have *length*
+    points starting at *offset* in the points block (*num_weights* long).
     """
-    num_active = np.sum(length > 1)
+    length = np.asarray(length)
+    offset = np.asarray(offset)
     max_pd = model_info.parameters.max_pd
-    if num_active > max_pd:
-        raise ValueError("Too many polydisperse parameters")
-    idx = np.argsort(length)[::-1][:max_pd]
-    pd_stride = np.cumprod(np.hstack((1, length[idx])))
-    call_details = CallDetails(model_info)
-    call_details.pd_par[:max_pd] = idx
-    call_details.pd_length[:max_pd] = length[idx]
-    call_details.pd_offset[:max_pd] = offset[idx]
-    call_details.pd_stride[:max_pd] = pd_stride[:-1]
-    call_details.num_eval = pd_stride[-1]
-    call_details.num_weights = num_weights
-    call_details.num_active = num_active
-    call_details.length = length
-    call_details.offset = offset
-    return call_details
-
-
-_ZEROS = tuple([0.]*31)
+    active = int(np.count_nonzero(length > 1))
+    if active > max_pd:
+        raise ValueError("%s takes at most %d polydisperse parameters, got %d"
+                         % (model_info.name, max_pd, active))
+    cd = CallDetails(model_info)
+    levels = np.argsort(length)[::-1][:max_pd]
+    cd.pd_par[:] = levels
+    cd.pd_length[:] = length[levels]
+    cd.pd_offset[:] = offset[levels]
+    stride = 1
+    for k, n in enumerate(length[levels]):
+        cd.pd_stride[k] = stride
+        stride *= int(n)
+    if stride > np.iinfo(np.int32).max:
+        raise ValueError("mesh of %d points exceeds the int32 descriptor" % stride)
+    cd.num_eval = stride
+    cd.num_weights = num_weights
+    cd.num_active = active
+    cd.length, cd.offset = length, offset
+    return cd
 
 
 def make_kernel_args(kernel, mesh):
     """
-    Convert a mesh of (value, dispersity, weight) per call parameter into
-    ``(call_details, values, is_magnetic)`` (details.py:225-254).
+    ``(call_details, values, is_magnetic)`` for *mesh*, one
+    ``(value, points, weights)`` per call parameter (direct_model.get_mesh);
+    *kernel* supplies ``info`` and ``dtype``.
     """
-    npars = kernel.info.parameters.npars
-    nvalues = kernel.info.parameters.nvalues
-    scalars = [value for value, _dispersity, _weight in mesh]
-    _, dispersity, weight = (zip(*mesh[NUM_COMMON_PARS:npars + NUM_COMMON_PARS])
-                             if npars else ((), (), ()))
-    length = np.array([len(w) for w in weight])
-    offset = np.cumsum(np.hstack((0, length)))
-    call_details = make_details(kernel.info, length, offset[:-1], offset[-1])
-    data_len = nvalues + 2*sum(len(v) for v in dispersity)
-    extra = (32 - data_len % 32) % 32
-    data = np.hstack((scalars,) + tuple(dispersity) + tuple(weight) + _ZEROS[:extra])
-    data = data.astype(kernel.dtype)
-    is_magnetic = convert_magnetism(kernel.info.parameters, data)
-    return call_details, data, is_magnetic
+    table = kernel.info.parameters
+    nvalues, npars = table.nvalues, table.npars
+    dists = mesh[NUM_COMMON_PARS:NUM_COMMON_PARS + npars]
+    length = np.array([len(w) for _, _, w in dists], dtype=int)
+    offset = np.zeros(npars, dtype=int)
+    if npars:
+        np.cumsum(length[:-1], out=offset[1:])
+    num_weights = int(length.sum())
+
+    used = nvalues + 2 * num_weights
+    values = np.zeros(used + (-used) % 32, dtype=kernel.dtype)
+    values[:len(mesh)] = [v for v, _, _ in mesh]
+    points = values[nvalues:nvalues + num_weights]
+    weights = values[nvalues + num_weights:used]
+    for start, n, (_, x, w) in zip(offset, length, dists):
+        points[start:start + n] = x
+        weights[start:start + n] = w
+
+    call_details = make_details(kernel.info, length, offset, num_weights)
+    return call_details, values, _spin_vectors(table, values)
 
 
-def convert_magnetism(parameters, values):
-    """
-    Convert (M0, theta, phi) per sld to (mx, my, mz) in place; True if any
-    M0 is non-zero (details.py:288-306).
-    """
-    nmagpars = NUM_MAGNETIC_PARS*parameters.nmagnetic
-    mag = values[parameters.nvalues - nmagpars:parameters.nvalues]
-    mag = mag.reshape(-1, NUM_MAGNETIC_PARS)
-    if np.any(mag[:, 0] != 0.0):
-        M0 = mag[:, 0].copy()
-        theta, phi = radians(mag[:, 1]), radians(mag[:, 2])
-        mag[:, 0] = M0*sin(theta)*cos(phi)
-        mag[:, 1] = M0*sin(theta)*sin(phi)
-        mag[:, 2] = M0*cos(theta)
-        return True
-    return False
+def _spin_vectors(table, values):
+    """(M0, mtheta, mphi) -> (mx, my, mz) in place; True when any M0 != 0."""
+    if not table.nmagnetic:
+        return False
+    end = table.nvalues
+    block = values[end - NUM_MAGNETIC_PARS * table.nmagnetic:end].reshape(-1, NUM_MAGNETIC_PARS)
+    M0 = block[:, 0].copy()
+    if not M0.any():
+        return False
+    theta, phi = np.radians(block[:, 1]), np.radians(block[:, 2])
+    block[:, 0] = M0 * np.sin(theta) * np.cos(phi)
+    block[:, 1] = M0 * np.sin(theta) * np.sin(phi)
+    block[:, 2] = M0 * np.cos(theta)
+    return True

@@ -1,12 +1,21 @@
 """
-Script-level evaluation: build the mesh for a parameter set and call a kernel.
+Script-level evaluation: the mesh of a parameter dictionary, and one kernel
+call on it.
 
-Restates the reference's call assembly (sasmodels/direct_model.py:41-154):
-every call parameter becomes ``(value, dispersity, weights)``; a parameter
-is polydisperse only if it is active for the kernel's dimension
-(``pd_1d``/``pd_2d``) and has ``<name>_pd_n`` > 0 and ``<name>_pd`` != 0;
-defaults are ``_pd_n`` = 0, ``_pd_nsigma`` = 3 and ``_pd_type`` =
-'gaussian'; inactive orientation parameters carry jitter 0.
+The mesh has one ``(value, points, weights)`` entry per call parameter,
+assembled by the rules of the reference's call path
+(sasmodels/direct_model.py:41-154), which decide the values vector a fit
+sends to the kernel:
+
+* a parameter reads ``name`` (default: its table default); a polydisperse
+  one also reads ``name_pd`` (width, 0), ``name_pd_n`` (points, 0),
+  ``name_pd_nsigma`` (3) and ``name_pd_type`` ('gaussian');
+* it is dispersed only when it is active for the kernel's dimension (1D:
+  volume parameters; 2D: volume and orientation), ``name_pd_n`` is not 0 and
+  ``name_pd`` is not 0; otherwise it is the single point of weight 1 at its
+  value, or at 0 for an orientation parameter, whose points are jitter
+  about the view angle;
+* a key that names no call parameter raises TypeError.
 """
 
 import numpy as np
@@ -17,58 +26,55 @@ from .details import make_kernel_args
 #: parameter holding the effective-radius mode in Fq calls (product.py)
 RADIUS_MODE_ID = "radius_effective_mode"
 
+#: suffix -> default of the keys describing a parameter's distribution
+PD_KEYS = (("_pd", 0.0), ("_pd_n", 0), ("_pd_nsigma", 3.0), ("_pd_type", "gaussian"))
+
 
 def call_kernel(calculator, pars, cutoff=0., mono=False):
-    """Evaluate *calculator* (a :class:`~kernel.Kernel`) at *pars*; returns I(q)."""
-    mesh = get_mesh(calculator.info, pars, dim=calculator.dim, mono=mono)
-    call_details, values, is_magnetic = make_kernel_args(calculator, mesh)
-    return calculator(call_details, values, cutoff, is_magnetic)
+    """I(q) of *calculator* (a :class:`~kernel.Kernel`) at *pars*."""
+    details, values, magnetic = make_kernel_args(calculator, get_mesh(calculator.info, pars, calculator.dim, mono))
+    return calculator(details, values, cutoff, magnetic)
 
 
 def call_Fq(calculator, pars, cutoff=0., mono=False):
-    """Like :func:`call_kernel` but returns (F, F^2, R_eff, V_shell, V_form/V_shell)."""
+    """(F, F^2, R_eff, V_shell, V_form/V_shell) of *calculator* at *pars*;
+    ``radius_effective_mode`` in *pars* selects R_eff (default 1)."""
     pars = dict(pars)
-    R_eff_type = int(pars.pop(RADIUS_MODE_ID, 1.0))
-    mesh = get_mesh(calculator.info, pars, dim=calculator.dim, mono=mono)
-    call_details, values, is_magnetic = make_kernel_args(calculator, mesh)
-    return calculator.Fq(call_details, values, cutoff, is_magnetic, R_eff_type)
+    mode = int(pars.pop(RADIUS_MODE_ID, 1.0))
+    details, values, magnetic = make_kernel_args(calculator, get_mesh(calculator.info, pars, calculator.dim, mono))
+    return calculator.Fq(details, values, cutoff, magnetic, mode)
 
 
 def get_mesh(model_info, values, dim="1d", mono=False):
     """
-    List of *(value, dispersity, weights)* per call parameter; unknown keys
-    raise TypeError.
+    One ``(value, points, weights)`` per call parameter of *model_info* for
+    the parameter dictionary *values*.  *dim* ('1d', '2d', or anything else
+    for "every polydisperse parameter") selects the active distributions;
+    *mono* switches all of them off.
     """
-    parameters = model_info.parameters
+    table = model_info.parameters
     if mono:
-        active = lambda name: False
+        active = frozenset()
     elif dim == "1d":
-        active = lambda name: name in parameters.pd_1d
+        active = table.pd_1d
     elif dim == "2d":
-        active = lambda name: name in parameters.pd_2d
+        active = table.pd_2d
     else:
-        active = lambda name: True
-    values = dict(values)
-    mesh = [_pop_par_weights(p, values, active(p.name))
-            for p in parameters.call_parameters]
-    if values:
-        raise TypeError("Unused parameters in call: %s" % ", ".join(values.keys()))
+        active = frozenset(p.name for p in table.call_parameters if p.polydisperse)
+    unread = dict(values)
+    mesh = [_distribution(p, unread, p.name in active) for p in table.call_parameters]
+    if unread:
+        raise TypeError("parameters not used by %s: %s" % (model_info.name, ", ".join(sorted(unread))))
     return mesh
 
 
-def _pop_par_weights(parameter, values, active=True):
-    value = float(values.pop(parameter.name, parameter.default))
-    if parameter.polydisperse:
-        npts = values.pop(parameter.name + "_pd_n", 0)
-        width = values.pop(parameter.name + "_pd", 0.0)
-        nsigma = values.pop(parameter.name + "_pd_nsigma", 3.0)
-        distribution = values.pop(parameter.name + "_pd_type", "gaussian")
-        relative = parameter.relative_pd
-        if npts == 0 or width == 0.0 or not active:
-            pd = [value if relative else 0.0], [1.0]
-        else:
-            pd = weights.get_weights(distribution, npts, width, nsigma,
-                                     value, parameter.limits, relative)
-    else:
-        pd = [value], [1.0]
-    return value, np.asarray(pd[0], "d"), np.asarray(pd[1], "d")
+def _distribution(par, unread, active):
+    """Pop *par*'s keys from *unread*; return its mesh entry."""
+    value = float(unread.pop(par.name, par.default))
+    if not par.polydisperse:
+        return value, np.array([value]), np.array([1.0])
+    width, npts, nsigma, kind = [unread.pop(par.name + suffix, default) for suffix, default in PD_KEYS]
+    if active and npts != 0 and width != 0.0:
+        x, w = weights.get_weights(kind, npts, width, nsigma, value, par.limits, par.relative_pd)
+        return value, np.asarray(x, "d"), np.asarray(w, "d")
+    return value, np.array([value if par.relative_pd else 0.0]), np.array([1.0])

@@ -181,3 +181,123 @@ def test_backend_selection_errors():
     info = core.load_model_info("cylinder")
     generate.set_integration_size(info, 150)
     assert info.integration_size == 150 and core.list_models()[1] == "cylinder"
+
+
+# ---------------------------------------------------------------- bit identity of the contract
+def _closed_form(kind, c, s, n, ns, lb=-np.inf, ub=np.inf):
+    """Points and weights written out from the definitions, in the floating
+    point order of sasmodels/weights.py:86-176 (each weight is one numpy
+    expression of x, c and s)."""
+    if kind in ("lognormal", "schulz"):
+        lb, ub = max(lb, 1e-8), max(ub, 1e-8)
+    if kind == "uniform":
+        x = np.linspace(c - s, c + s, n)
+    else:
+        x = c + np.linspace(-ns*s, +ns*s, n)
+    x = x[(x >= lb) & (x <= ub)]
+    if kind == "gaussian":
+        w = np.exp((x - c)**2/(-2.0*s*s))
+    elif kind == "lognormal":
+        t = np.fabs(s/c)
+        w = np.exp(-0.5*((np.log(x) - np.log(c))/t)**2)/(x*t)
+    elif kind == "schulz":
+        from scipy.special import gammaln
+        z = (c/s)**2
+        w = np.exp(z*np.log(z) + (z - 1)*np.log(x/c) - (x/c)*z - np.log(c) - gammaln(z))
+    elif kind == "boltzmann":
+        w = np.exp(-np.abs(x - c)/np.abs(s))
+    elif kind == "rectangle":
+        x = x[np.fabs(x - c) <= np.fabs(s)*np.sqrt(3.0)]
+        w = np.ones_like(x)
+    else:
+        w = np.ones_like(x)
+    return x, w/np.sum(w)
+
+
+@pytest.mark.parametrize("kind,n,ns", [("gaussian", 35, 3), ("lognormal", 35, 3), ("lognormal", 80, 8),
+                                       ("schulz", 80, 8), ("schulz", 21, 2.5), ("boltzmann", 35, 3),
+                                       ("uniform", 35, None), ("rectangle", 35, 1.73205)])
+def test_distribution_points_and_weights_bit_identical(kind, n, ns):
+    rng = np.random.default_rng(hash(kind) % 1000 + n)
+    for _ in range(20):
+        c = float(rng.uniform(1, 300))
+        rel = float(rng.uniform(0.01, 0.4))
+        lb = float(rng.choice([0.0, 0.8*c, -np.inf]))
+        x, w = weights.get_weights(kind, n, rel, ns, c, (lb, np.inf), True)
+        xr, wr = _closed_form(kind, c, rel*c, n, ns, lb, np.inf)
+        np.testing.assert_array_equal(x, xr)
+        np.testing.assert_array_equal(w, wr)
+    if kind in ("lognormal", "schulz"):
+        return      # defined for x > 0 only: no orientation jitter
+    # orientation: absolute width, centred on zero, clipped to the limits
+    x, w = weights.get_weights(kind, n, 12.5, ns, 60.0, (-90, 90), False)
+    xr, wr = _closed_form(kind, 0.0, 12.5, n, ns, -90, 90)
+    np.testing.assert_array_equal(x, xr)
+    np.testing.assert_array_equal(w, wr)
+
+
+def test_distribution_registry_and_classes():
+    assert list(weights.DISTRIBUTIONS) == ["rectangle", "uniform", "array", "lognormal",
+                                           "gaussian", "schulz", "boltzmann"]
+    assert weights.MODELS is weights.DISTRIBUTIONS
+    defaults = {name: (cls.default["npts"], cls.default["nsigmas"]) for name, cls in weights.MODELS.items()}
+    assert defaults == {"rectangle": (35, 1.73205), "uniform": (35, None), "array": (35, 1),
+                        "lognormal": (80, 8), "gaussian": (35, 3), "schulz": (80, 8), "boltzmann": (35, 3)}
+    g = weights.GaussianDispersion(npts=9, width=0.2)
+    assert g.get_pars() == dict(type="gaussian", npts=9, width=0.2, nsigmas=3)
+    with pytest.raises(RuntimeError):
+        g.set_weights([1.0], [1.0])
+    a = weights.ArrayDispersion(width=1.0)     # width 0 means "no distribution"
+    a.set_weights([1.0, 2.0, 3.0, 4.0], [0.1, 0.2, 0.3, 0.4])
+    x, w = a.get_weights(2.0, 1.5, 3.5, True)
+    assert list(x) == [2.0, 3.0] and list(w) == [0.2, 0.3]
+    assert a.npts == 4
+    # a single point outside the limits gives an empty distribution
+    x, w = weights.GaussianDispersion(npts=1, width=0.1).get_weights(5.0, 6.0, 9.0, True)
+    assert x.size == 0 and w.size == 0
+
+
+def _reference_layout(info, mesh, dtype):
+    """The values vector assembled the straightforward way: every scalar,
+    then each distribution's points, then each one's weights, zero padded to
+    a multiple of 32, cast to the kernel dtype at the end."""
+    npars = info.parameters.npars
+    scalars = [m[0] for m in mesh]
+    dists = mesh[2:2 + npars]
+    flat = np.hstack([np.asarray(scalars, "d")] + [np.asarray(m[1], "d") for m in dists]
+                     + [np.asarray(m[2], "d") for m in dists])
+    flat = np.hstack([flat, np.zeros((-flat.size) % 32)])
+    return flat.astype(dtype)
+
+
+@pytest.mark.parametrize("name,dim,pars", [
+    ("cylinder", "1d", dict(radius_pd=0.1, radius_pd_n=35, radius_pd_type="lognormal",
+                            length_pd=0.1, length_pd_n=35, length_pd_type="lognormal")),
+    ("ellipsoid", "2d", dict(radius_polar=20, radius_equatorial=400, theta_pd=10, theta_pd_n=20,
+                             phi_pd=20, phi_pd_n=20, radius_equatorial_pd=0.1, radius_equatorial_pd_n=20)),
+    ("core_shell_cylinder", "1d", dict(radius_pd=0.15, radius_pd_n=17, thickness_pd=0.1, thickness_pd_n=5,
+                                       length_pd=0.2, length_pd_n=5, length_pd_type="schulz")),
+    ("sphere", "1d", dict(radius_pd=0.1, radius_pd_n=35)),
+    ("parallelepiped", "2d", dict(length_a_pd=0.1, length_a_pd_n=5, psi_pd=8, psi_pd_n=4, theta_pd=5, theta_pd_n=3)),
+])
+def test_values_vector_and_descriptor_bit_identical(name, dim, pars):
+    k = _Stub(name, dim)
+    mesh = direct_model.get_mesh(k.info, pars, dim=dim)
+    for dtype in ("d", "f"):
+        k.dtype = np.dtype(dtype)
+        cd, values, mag = details.make_kernel_args(k, mesh)
+        want = _reference_layout(k.info, mesh, k.dtype)
+        assert values.dtype == want.dtype
+        np.testing.assert_array_equal(values, want)
+    # descriptor: levels by decreasing length (ties: reversed argsort), strides
+    # the running products, the whole mesh their product
+    length = np.array([len(m[2]) for m in mesh[2:2 + k.info.parameters.npars]])
+    mp = k.info.parameters.max_pd
+    order = np.argsort(length)[::-1][:mp]
+    assert list(cd.pd_par) == list(order)
+    assert list(cd.pd_length) == list(length[order])
+    assert list(cd.pd_stride) == list(np.cumprod(np.concatenate([[1], length[order]]))[:-1])
+    assert cd.num_eval == np.prod(length[order]) and cd.num_weights == length.sum()
+    assert cd.num_active == np.count_nonzero(length > 1)
+    starts = np.concatenate([[0], np.cumsum(length)[:-1]])
+    assert list(cd.pd_offset) == list(starts[order])
