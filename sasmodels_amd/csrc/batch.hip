@@ -166,22 +166,20 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a, int *__restr
         const int nq = (int)(a.qoff[s + 1] - q0);
         const int iq = tile * QTILE + lane;
         const bool ok = iq < nq;
-        const double qx = ok ? a.q64[2 * (q0 + iq)] : 0.0;
-        const double qy = ok ? a.q64[2 * (q0 + iq) + 1] : 0.0;
-        const double qmag = sqrt(qx * qx + qy * qy);
+        Lane2D L;
+        L.set(ok, ok ? a.q64[2 * (q0 + iq)] : 0.0, ok ? a.q64[2 * (q0 + iq) + 1] : 0.0);
         Contrast<M, MAG> con;
-        con.setup(a.values + a.voff[s], qx, qy);
+        con.setup(a.values + a.voff[s], L.qx, L.qy);
         const int npts = a.npts[s];
         const int c0 = min(npts, chunk * a.pts_per_item);
         const int c1 = min(npts, c0 + a.pts_per_item);
-        const float4 *rec = a.recs + a.roff[s] * RV;
+        const SAS_CONST float4 *rec = as_const(a.recs) + a.roff[s] * RV;
         KSum acc;
-        for (int j = c0; j < c1; j++) {
-            float r[4 * RV];
-            load_rec(rec + (size_t)j * RV, r, RV);
-            float g[M::NCON];
-            geom_at<M>(r, qx, qy, qmag, g);
-            acc.add(r[M::A2D] * con.f2(g));
+        for (int b0 = c0; b0 < c1; b0 += SUM_BLOCK) {
+            const int b1 = min(c1, b0 + SUM_BLOCK);
+            float blk = 0.0f;
+            for (int j = b0; j < b1; j++) blk += point2d<M, MAG>(rec + (size_t)j * RV, L, con);
+            acc.add(blk);
         }
         if (ok) partial[a.poff[s] + (size_t)chunk * nq + iq] = acc.value();
     }

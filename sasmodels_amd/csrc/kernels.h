@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "models.h"
 
@@ -31,10 +32,30 @@ constexpr int EVAL_THREADS = 256;
 constexpr int TILE_PTS = 64;
 constexpr double DEG = 0.017453292519943295;   // pi/180
 
-// Record widths in float4 units.
+// Does model M have the FP32 2D path (REC2F, rec2f, geom2f)?
+template <class M, class = void> struct has_f32_2d : std::false_type {};
+template <class M> struct has_f32_2d<M, std::void_t<decltype(M::REC2F)>> : std::true_type {};
+template <class M> constexpr int rec2f_floats()
+{
+    if constexpr (has_f32_2d<M>::value) return M::REC2F + (M::ORIENTED ? 6 : 0);
+    else return 0;
+}
+
+// Record widths in float4 units.  A 2D record is the FP64 part (model
+// fields, then six FP64 rotation entries for oriented models) followed, for
+// models with the FP32 path, by the FP32 part (six FP32 rotation entries
+// for oriented models, then the model's FP32 fields).
 template <class M> constexpr int rv1() { return (M::REC1D + 3) / 4; }
-template <class M> constexpr int rv2() { return (M::REC2D + (M::ORIENTED ? 12 : 0) + 3) / 4; }
+template <class M> constexpr int rv2d() { return (M::REC2D + (M::ORIENTED ? 12 : 0) + 3) / 4; }
+template <class M> constexpr int rv2f() { return (rec2f_floats<M>() + 3) / 4; }
+template <class M> constexpr int rv2() { return rv2d<M>() + rv2f<M>(); }
 template <class M, int KIND> constexpr int rvk() { return KIND == 0 ? rv1<M>() : rv2<M>(); }
+
+// Largest phase (radians) the FP32 2D path takes: a wave whose largest
+// |q| times a record's length scale exceeds it evaluates that record in
+// FP64 (rotation, argument and its reduction by pi/2).  At 400 rad the FP32
+// phase is good to about 4e-5 rad.
+constexpr float XF32_2D = 400.0f;
 
 template <class M> constexpr int nvalues()
 {
@@ -202,6 +223,14 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
                         put_d(rot, 6, j21 * v12 + j22 * v22 + j23 * v32);
                         put_d(rot, 8, j31 * v11 + j32 * v21 + j33 * v31);
                         put_d(rot, 10, j31 * v12 + j32 * v22 + j33 * v32);
+                    }
+                    if constexpr (KIND != 0 && has_f32_2d<M>::value) {
+                        float *f = rec + 4 * rv2d<M>();
+                        if (OR2D) {
+                            const float *rot = rec + M::REC2D;
+                            for (int k = 0; k < 6; k++) f[k] = (float)get_d(rot, 2 * k);
+                        }
+                        M::rec2f(p, weight, f);
                     }
                 }
             }
@@ -448,6 +477,67 @@ __device__ __forceinline__ void geom_at(const float *r, double qx, double qy, do
     M::geom2d(q, r, g);
 }
 
+// One lane's q of a 2D kernel, in FP64 and FP32, and the wave's largest |q|.
+struct Lane2D {
+    double qx, qy, qmag;
+    float fx, fy, fmag, wave_qmax;
+    __device__ __forceinline__ void set(bool ok, double x, double y)
+    {
+        qx = ok ? x : 0.0;
+        qy = ok ? y : 0.0;
+        qmag = sqrt(qx * qx + qy * qy);
+        fx = (float)qx;
+        fy = (float)qy;
+        fmag = (float)qmag;
+        float m = fmag;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        wave_qmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(m)));
+    }
+};
+
+// w * sum over cross sections of w_xs F_xs^2 for one 2D record (rec points
+// at its first float4; the wave reads it with uniform loads): the FP32 path
+// when the model has one and every phase of the wave is within XF32_2D,
+// else FP64 rotation and arguments.
+template <class M, bool MAG, class P>
+__device__ __forceinline__ float point2d(P rec, const Lane2D &L, const Contrast<M, MAG> &con)
+{
+    float g[M::NCON];
+    if constexpr (has_f32_2d<M>::value) {
+        constexpr int RF = rv2f<M>();
+        float f[4 * RF];
+#pragma unroll
+        for (int v = 0; v < RF; v++) {
+            const float4 x = ld4(rec + rv2d<M>() + v);
+            f[4 * v] = x.x; f[4 * v + 1] = x.y; f[4 * v + 2] = x.z; f[4 * v + 3] = x.w;
+        }
+        if (L.wave_qmax * f[M::XS2F] <= XF32_2D) {
+            QRotF q{L.fmag, 0.0f, 0.0f, L.fmag};
+            if constexpr (M::ORIENTED) {
+                q.qa = fmaf(f[0], L.fx, f[1] * L.fy);
+                q.qb = fmaf(f[2], L.fx, f[3] * L.fy);
+                q.qc = fmaf(f[4], L.fx, f[5] * L.fy);
+            }
+            M::geom2f(q, f, g);
+            return f[M::A2F] * con.f2(g);
+        }
+    }
+    constexpr int RD = rv2d<M>();
+    float r[4 * RD];
+#pragma unroll
+    for (int v = 0; v < RD; v++) {
+        const float4 x = ld4(rec + v);
+        r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
+    }
+    geom_at<M>(r, L.qx, L.qy, L.qmag, g);
+    return r[M::A2D] * con.f2(g);
+}
+
+// Points summed in FP32 between compensated adds (TwoSum) of the block sums:
+// the block's FP32 rounding stays below 16 ulp of its terms.
+constexpr int SUM_BLOCK = 16;
+
 template <class M, bool MAG>
 __global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
     int nq, const double *__restrict__ q64,
@@ -455,37 +545,24 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
     const int *__restrict__ npts_p, int nsplit, double *__restrict__ partial)
 {
     constexpr int RV = rv2<M>();
-    __shared__ float4 tile[TILE_PTS * RV];
     const int iq = blockIdx.x * EVAL_THREADS + threadIdx.x;
     const int split = blockIdx.y;
     int s0, s1;
     split_range(*npts_p, nsplit, split, s0, s1);
-    const double qx = iq < nq ? q64[2 * iq] : 0.0, qy = iq < nq ? q64[2 * iq + 1] : 0.0;
-    const double qmag = sqrt(qx * qx + qy * qy);
+    const bool ok = iq < nq;
+    Lane2D L;
+    L.set(ok, ok ? q64[2 * iq] : 0.0, ok ? q64[2 * iq + 1] : 0.0);
     Contrast<M, MAG> con;
-    con.setup(values, qx, qy);
-
+    con.setup(values, L.qx, L.qy);
+    const SAS_CONST float4 *rc = as_const(recs);
     KSum acc;
-    for (int t0 = s0; t0 < s1; t0 += TILE_PTS) {
-        const int cnt = min(TILE_PTS, s1 - t0);
-        for (int i = threadIdx.x; i < cnt * RV; i += EVAL_THREADS) tile[i] = recs[(size_t)t0 * RV + i];
-        __syncthreads();
-        if (iq < nq) {
-            for (int j = 0; j < cnt; j++) {
-                float r[4 * RV];
-#pragma unroll
-                for (int v = 0; v < RV; v++) {
-                    const float4 x = tile[j * RV + v];
-                    r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
-                }
-                float g[M::NCON];
-                geom_at<M>(r, qx, qy, qmag, g);
-                acc.add(r[M::A2D] * con.f2(g));
-            }
-        }
-        __syncthreads();
+    for (int b0 = s0; b0 < s1; b0 += SUM_BLOCK) {
+        const int b1 = min(s1, b0 + SUM_BLOCK);
+        float blk = 0.0f;
+        for (int j = b0; j < b1; j++) blk += point2d<M, MAG>(rc + (size_t)j * RV, L, con);
+        acc.add(blk);
     }
-    if (iq < nq) partial[(size_t)iq * nsplit + split] = acc.value();
+    if (ok) partial[(size_t)iq * nsplit + split] = acc.value();
 }
 
 }  // namespace sas

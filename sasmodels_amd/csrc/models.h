@@ -40,6 +40,17 @@ struct QRot {
     double qa, qb, qc, q;
 };
 
+// q in the particle frame in FP32 (the 2D FP32 path).  Models that have it
+// define REC2F (FP32 record floats after the rotation rows of oriented
+// models), A2F (slot of w*1e-4) and XS2F (slot of the largest length by
+// which the geometry multiplies |q|: the phase bound of the record), and
+// rec2f/geom2f, the FP32 counterparts of rec2d/geom2d with every length
+// divided by 2 pi so that the phases come out in revolutions.
+struct QRotF {
+    float qa, qb, qc, q;
+};
+constexpr double INV_TWO_PI_D = 0.15915494309189533577;
+
 struct Quad {
     const float *a;     // sin(theta) | u^2
     const float *b;     // cos(theta) | unused
@@ -129,6 +140,20 @@ struct Sphere {
     __device__ static void geom2d(const QRot &q, const float *r, float *g)
     {
         g[0] = r[2] * sph_j1c_d(q.q * get_d(r, 0));
+    }
+    // FP32 record: R/2pi, V, w*1e-4, R
+    static constexpr int REC2F = 4, A2F = 2, XS2F = 3;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        f[0] = (float)(p[2] * INV_TWO_PI_D);
+        f[1] = (float)volume(p);
+        f[2] = (float)(w * 1e-4);
+        f[3] = (float)p[2];
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float t = q.q * f[0];
+        g[0] = f[1] * sph_j1c_rev2(t * t);
     }
     SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
@@ -227,6 +252,21 @@ struct Cylinder {
         const double qab = sqrt(q.qa * q.qa + q.qb * q.qb);
         g[0] = r[4] * j1c_d(qab * get_d(r, 0)) * sinc_d(q.qc * get_d(r, 2));
     }
+    // FP32 record after the rotation rows: R/2pi, (L/2)/2pi, V, w*1e-4, max(R, L/2)
+    static constexpr int REC2F = 5, A2F = 9, XS2F = 10;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        f[6] = (float)(p[2] * INV_TWO_PI_D);
+        f[7] = (float)(0.5 * p[3] * INV_TWO_PI_D);
+        f[8] = (float)volume(p);
+        f[9] = (float)(w * 1e-4);
+        f[10] = (float)fmax(p[2], 0.5 * p[3]);
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float qab = sqrt_f(fmaf(q.qa, q.qa, q.qb * q.qb));
+        g[0] = f[8] * j1c_rev(qab * f[6]) * sinc_rev(q.qc * f[7]);
+    }
     SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
@@ -306,6 +346,22 @@ struct Ellipsoid {
     {
         const double qr = sqrt(get_d(r, 0) * (q.qa * q.qa + q.qb * q.qb) + get_d(r, 2) * (q.qc * q.qc));
         g[0] = r[4] * sph_j1c_d(qr);
+    }
+    // FP32 record after the rotation rows: (re/2pi)^2, (rp/2pi)^2, V, w*1e-4, max(re, rp)
+    static constexpr int REC2F = 5, A2F = 9, XS2F = 10;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        const double re = p[3] * INV_TWO_PI_D, rp = p[2] * INV_TWO_PI_D;
+        f[6] = (float)(re * re);
+        f[7] = (float)(rp * rp);
+        f[8] = (float)volume(p);
+        f[9] = (float)(w * 1e-4);
+        f[10] = (float)fmax(p[2], p[3]);
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float t2 = fmaf(f[6], fmaf(q.qa, q.qa, q.qb * q.qb), f[7] * (q.qc * q.qc));
+        g[0] = f[8] * sph_j1c_rev2(t2);
     }
     SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
@@ -424,6 +480,27 @@ struct CoreShellCylinder {
         g[0] = r[8] * sinc_d(q.qc * get_d(r, 2)) * j1c_d(qab * get_d(r, 0));
         g[1] = r[9] * sinc_d(q.qc * get_d(r, 6)) * j1c_d(qab * get_d(r, 4));
     }
+    // FP32 record after the rotation rows: R, L/2, R+t, L/2+t (each / 2pi),
+    // V_core, V_shell, w*1e-4, max(R+t, L/2+t)
+    static constexpr int REC2F = 8, A2F = 12, XS2F = 13;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        const double rr = p[3], t = p[4], l = p[5];
+        f[6] = (float)(rr * INV_TWO_PI_D);
+        f[7] = (float)(0.5 * l * INV_TWO_PI_D);
+        f[8] = (float)((rr + t) * INV_TWO_PI_D);
+        f[9] = (float)((0.5 * l + t) * INV_TWO_PI_D);
+        f[10] = (float)vol3(rr, 0.0, l);
+        f[11] = (float)vol3(rr, t, l);
+        f[12] = (float)(w * 1e-4);
+        f[13] = (float)fmax(rr + t, 0.5 * l + t);
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float qab = sqrt_f(fmaf(q.qa, q.qa, q.qb * q.qb));
+        g[0] = f[10] * sinc_rev(q.qc * f[7]) * j1c_rev(qab * f[6]);
+        g[1] = f[11] * sinc_rev(q.qc * f[9]) * j1c_rev(qab * f[8]);
+    }
     SAS_HD static void contrasts(const double *sld, float *con)
     {
         con[0] = (float)(sld[0] - sld[1]);
@@ -479,6 +556,24 @@ struct CoreShellSphere {
     {
         g[0] = r[4] * sph_j1c_d(q.q * get_d(r, 0));
         g[1] = r[5] * sph_j1c_d(q.q * get_d(r, 2));
+    }
+    // FP32 record: r_core/2pi, r_shell/2pi, V_core, V_shell, w*1e-4, r_shell
+    static constexpr int REC2F = 6, A2F = 4, XS2F = 5;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        const double rc = p[0], rs = p[0] + p[1];
+        f[0] = (float)(rc * INV_TWO_PI_D);
+        f[1] = (float)(rs * INV_TWO_PI_D);
+        f[2] = (float)(FOUR_PI_3_D * cube_d(rc));
+        f[3] = (float)(FOUR_PI_3_D * cube_d(rs));
+        f[4] = (float)(w * 1e-4);
+        f[5] = (float)rs;
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float tc = q.q * f[0], ts = q.q * f[1];
+        g[0] = f[2] * sph_j1c_rev2(tc * tc);
+        g[1] = f[3] * sph_j1c_rev2(ts * ts);
     }
     SAS_HD static void contrasts(const double *sld, float *con)
     {
