@@ -113,6 +113,56 @@ def raw_call(model, kind, quad_n, q, details, values, cutoff=0.0, reff_mode=0,
     return result
 
 
+FLOPCOUNT_PATH = os.path.join(HERE, "libflopcount.so")
+_flib = None
+#: categories tallied by libflopcount.so, in its order
+FLOP_CATEGORIES = ("add", "mul", "div", "sqrt", "trans")
+
+
+def flopcount_lib():
+    """Load libflopcount.so (the oracle on a counting scalar), building it first if needed."""
+    global _flib
+    if _flib is None:
+        if not os.path.exists(FLOPCOUNT_PATH):
+            build()
+        L = ct.CDLL(FLOPCOUNT_PATH)
+        L.flopcount_kernel.restype = ct.c_int
+        L.flopcount_kernel.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int32, ct.c_int32, ct.c_int32,
+                                       ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_void_p,
+                                       ct.c_int32, _D, ct.c_int32, ct.c_void_p]
+        L.flopcount_set_threads.argtypes = [ct.c_int]
+        _flib = L
+    return _flib
+
+
+def count_flops(model, kind, quad_n, q, details, values, cutoff=0.0, reff_mode=0, threads=1):
+    """
+    Operations the reference formulas take for one whole-mesh kernel call
+    (sasoracle.c on a counting scalar): dict of the FLOP_CATEGORIES tallies,
+    "flops" (their sum: add, sub, mul, div, sqrt and each transcendental
+    count 1) and "result" (the raw result, equal to raw_call's).
+    """
+    mid = MODEL_IDS[model] if isinstance(model, str) else int(model)
+    q = np.ascontiguousarray(q, "d")
+    nq = q.shape[0]
+    details = np.ascontiguousarray(getattr(details, "buffer", details), "i4")
+    values = np.ascontiguousarray(values, "d")
+    nout = 2 if kind == KIND_IQ and mid not in NO_FQ else 1
+    result = np.zeros(nout*nq + 4, "d")
+    counts = np.zeros(len(FLOP_CATEGORIES), np.uint64)
+    L = flopcount_lib()
+    L.flopcount_set_threads(int(threads))
+    rc = L.flopcount_kernel(mid, kind, int(quad_n), nq, 0, int(details[-4]), details.ctypes.data,
+                            values.ctypes.data, values.size, q.ctypes.data, result.ctypes.data, result.size,
+                            float(cutoff), int(reff_mode), counts.ctypes.data)
+    if rc != 0:
+        raise ValueError("flopcount_kernel failed with %d" % rc)
+    out = {k: int(v) for k, v in zip(FLOP_CATEGORIES, counts)}
+    out["flops"] = sum(out[k] for k in FLOP_CATEGORIES)
+    out["result"] = result
+    return out
+
+
 def normalise(result, nq, nout):
     """(F1, F2, R_eff, V_shell, V_form/V_shell) from raw accumulators (kernel.py:170-188)."""
     total = result[nout*nq]

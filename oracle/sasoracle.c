@@ -47,6 +47,13 @@
 #include <omp.h>
 #endif
 
+/* Arithmetic type of the restatement: double, except in oracle/flopcount.cpp,
+ * which compiles this file as C++ with a counting scalar in its place to
+ * count the floating-point operations of the same formulas. */
+#ifndef SASORACLE_COUNTING
+typedef double oreal;
+#endif
+
 /* constants with the values the reference's C uses (kernel_header.c:144-169) */
 #define O_PI 3.14159265358979323846
 #define O_PI_4 0.78539816339744830962
@@ -60,10 +67,10 @@
 /* ------------------------------------------------------------------ */
 #define GL_MAX_TABLES 16
 static int gl_n[GL_MAX_TABLES];
-static double *gl_z[GL_MAX_TABLES], *gl_w[GL_MAX_TABLES];
+static oreal *gl_z[GL_MAX_TABLES], *gl_w[GL_MAX_TABLES];
 static int gl_count = 0;
 
-static void gl_compute(int n, double *z, double *w)
+static void gl_compute(int n, oreal *z, oreal *w)
 {
     for (int i = 0; i < (n + 1) / 2; i++) {
         /* root i counted from the top (x near +1) */
@@ -92,24 +99,24 @@ static void gl_compute(int n, double *z, double *w)
         }
         long double wt = 2.0L / ((1.0L - x * x) * dp * dp);
         /* ascending order, as numpy.polynomial.legendre.leggauss returns */
-        z[n - 1 - i] = (double)x;
-        z[i] = (double)(-x);
-        w[n - 1 - i] = (double)wt;
-        w[i] = (double)wt;
+        z[n - 1 - i] = (oreal)x;
+        z[i] = (oreal)(-x);
+        w[n - 1 - i] = (oreal)wt;
+        w[i] = (oreal)wt;
     }
     if (n % 2) z[n / 2] = 0.0;
 }
 
 /* Returns the table for n points, computing it on first use.
  * Not thread safe on first use: callers prime it before parallel regions. */
-static int gl_table(int n, const double **z, const double **w)
+static int gl_table(int n, const oreal **z, const oreal **w)
 {
     for (int t = 0; t < gl_count; t++) {
         if (gl_n[t] == n) { *z = gl_z[t]; *w = gl_w[t]; return n; }
     }
     if (gl_count >= GL_MAX_TABLES || n < 1) return -1;
-    double *zz = (double *)malloc(sizeof(double) * n);
-    double *ww = (double *)malloc(sizeof(double) * n);
+    oreal *zz = (oreal *)malloc(sizeof(oreal) * n);
+    oreal *ww = (oreal *)malloc(sizeof(oreal) * n);
     gl_compute(n, zz, ww);
     gl_n[gl_count] = n; gl_z[gl_count] = zz; gl_w[gl_count] = ww; gl_count++;
     *z = zz; *w = ww;
@@ -118,7 +125,7 @@ static int gl_table(int n, const double **z, const double **w)
 
 /* quad_n == 0: adaptive choice by q*r_max (adaptive.c:11290-11297);
  * quad_n > 0: fixed table (nonadaptive.c:9-13 after set_integration_size). */
-static int quad_select(int quad_n, double qr, const double **z, const double **w)
+static int quad_select(int quad_n, oreal qr, const oreal **z, const oreal **w)
 {
     int n;
     if (quad_n > 0) n = quad_n;
@@ -133,7 +140,7 @@ static int quad_select(int quad_n, double qr, const double **z, const double **w
  * outer_n = ADAPTIVE_MAX_OUTER, the inner call the outer rule's size. */
 #define O_ADAPTIVE_MAX_N 100000
 #define O_ADAPTIVE_MAX_OUTER (O_ADAPTIVE_MAX_N / 500)
-static int quad_select_nested(int quad_n, double qr, int outer_n, const double **z, const double **w)
+static int quad_select_nested(int quad_n, oreal qr, int outer_n, const oreal **z, const oreal **w)
 {
     int n;
     if (quad_n > 0) n = quad_n;
@@ -146,7 +153,7 @@ static int quad_select_nested(int quad_n, double qr, int outer_n, const double *
 
 static void quad_prime(int quad_n)
 {
-    const double *z, *w;
+    const oreal *z, *w;
     if (quad_n > 0) { gl_table(quad_n, &z, &w); return; }
     gl_table(20, &z, &w); gl_table(76, &z, &w);
     gl_table(500, &z, &w); gl_table(5000, &z, &w);
@@ -157,117 +164,117 @@ static void quad_prime(int quad_n)
 /* Special functions                                                   */
 /* ------------------------------------------------------------------ */
 /* c[0]*x^n + c[1]*x^(n-1) + ... + c[n]   (polevl.c:52-66) */
-static double horner(double x, const double *c, int n)
+static oreal horner(oreal x, const oreal *c, int n)
 {
-    double acc = c[0];
+    oreal acc = c[0];
     for (int i = 1; i <= n; i++) acc = acc * x + c[i];
     return acc;
 }
 
 /* x^n + c[0]*x^(n-1) + ... + c[n-1]   (polevl.c:73-85) */
-static double horner_monic(double x, const double *c, int n)
+static oreal horner_monic(oreal x, const oreal *c, int n)
 {
-    double acc = x + c[0];
+    oreal acc = x + c[0];
     for (int i = 1; i < n; i++) acc = acc * x + c[i];
     return acc;
 }
 
 /* Cephes j1.c rational approximations (S. L. Moshier, Cephes 2.8), the
  * published coefficient sets RP/RQ on |x|<=5 and PP/PQ/QP/QQ beyond. */
-static const double J1_RP[4] = {
+static const oreal J1_RP[4] = {
     -8.99971225705559398224E8, 4.52228297998194034323E11,
     -7.27494245221818276015E13, 3.68295732863852883286E15};
-static const double J1_RQ[8] = {
+static const oreal J1_RQ[8] = {
     6.20836478118054335476E2, 2.56987256757748830383E5,
     8.35146791431949253037E7, 2.21511595479792499675E10,
     4.74914122079991414898E12, 7.84369607876235854894E14,
     8.95222336184627338078E16, 5.32278620332680085395E18};
-static const double J1_PP[7] = {
+static const oreal J1_PP[7] = {
     7.62125616208173112003E-4, 7.31397056940917570436E-2,
     1.12719608129684925192E0, 5.11207951146807644818E0,
     8.42404590141772420927E0, 5.21451598682361504063E0,
     1.00000000000000000254E0};
-static const double J1_PQ[7] = {
+static const oreal J1_PQ[7] = {
     5.71323128072548699714E-4, 6.88455908754495404082E-2,
     1.10514232634061696926E0, 5.07386386128601488557E0,
     8.39985554327604159757E0, 5.20982848682361821619E0,
     9.99999999999999997461E-1};
-static const double J1_QP[8] = {
+static const oreal J1_QP[8] = {
     5.10862594750176621635E-2, 4.98213872951233449420E0,
     7.58238284132545283818E1, 3.66779609360150777800E2,
     7.10856304998926107277E2, 5.97489612400613639965E2,
     2.11688757100572135698E2, 2.52070205858023719784E1};
-static const double J1_QQ[7] = {
+static const oreal J1_QQ[7] = {
     7.42373277035675149943E1, 1.05644886038262816351E3,
     4.98641058337653607651E3, 9.56231892404756170795E3,
     7.99704160447350683650E3, 2.82619278517639096600E3,
     3.36093607810698293419E2};
 
 /* J1(x), odd extension to x<0 (sas_J1.c:107-153) */
-static double o_j1(double x)
+static oreal o_j1(oreal x)
 {
-    const double Z1 = 1.46819706421238932572E1;  /* first zero squared */
-    const double Z2 = 4.92184563216946036703E1;  /* second zero squared */
-    const double ax = x < 0 ? -x : x;
-    const double sgn = x < 0 ? -1.0 : 1.0;
+    const oreal Z1 = 1.46819706421238932572E1;  /* first zero squared */
+    const oreal Z2 = 4.92184563216946036703E1;  /* second zero squared */
+    const oreal ax = x < 0 ? -x : x;
+    const oreal sgn = x < 0 ? -1.0 : 1.0;
     if (ax <= 5.0) {
-        const double z = ax * ax;
-        double r = horner(z, J1_RP, 3) / horner_monic(z, J1_RQ, 8);
+        const oreal z = ax * ax;
+        oreal r = horner(z, J1_RP, 3) / horner_monic(z, J1_RQ, 8);
         r = r * ax * (z - Z1) * (z - Z2);
         return sgn * r;
     }
-    const double u = 5.0 / ax;
-    const double z = u * u;
-    double p = horner(z, J1_PP, 6) / horner(z, J1_PQ, 6);
-    const double qq = horner(z, J1_QP, 7) / horner_monic(z, J1_QQ, 7);
-    const double s = sin(ax), c = cos(ax);
+    const oreal u = 5.0 / ax;
+    const oreal z = u * u;
+    oreal p = horner(z, J1_PP, 6) / horner(z, J1_PQ, 6);
+    const oreal qq = horner(z, J1_QP, 7) / horner_monic(z, J1_QQ, 7);
+    const oreal s = sin(ax), c = cos(ax);
     /* p*cos(x-3pi/4) - u*q*sin(x-3pi/4) expanded in sin(x), cos(x) */
     p = p * (s - c) + u * qq * (s + c);
     return sgn * p * O_SQRT1_PI / sqrt(ax);
 }
 
-static double o_2j1x_x(double x) { return (x != 0.0) ? 2.0 * o_j1(x) / x : 1.0; }
+static oreal o_2j1x_x(oreal x) { return (x != 0.0) ? 2.0 * o_j1(x) / x : 1.0; }
 
-static double o_sinx_x(double x) { return x == 0 ? 1.0 : sin(x) / x; }
+static oreal o_sinx_x(oreal x) { return x == 0 ? 1.0 : sin(x) / x; }
 
 /* 3 j1(x)/x: 4-term Taylor series below 0.1, closed form above */
-static double o_sph_j1c(double x)
+static oreal o_sph_j1c(oreal x)
 {
     if (fabs(x) < 0.1) {
-        const double x2 = x * x;
+        const oreal x2 = x * x;
         return (1.0 + x2 * (-3. / 30. + x2 * (3. / 840. + x2 * (-3. / 45360.))));
     }
-    const double s = sin(x), c = cos(x);
+    const oreal s = sin(x), c = cos(x);
     return 3.0 * (s / x - c) / (x * x);
 }
 
-static double sq(double x) { return x * x; }
-static double cb(double x) { return x * x * x; }
+static oreal sq(oreal x) { return x * x; }
+static oreal cb(oreal x) { return x * x * x; }
 
 /* ------------------------------------------------------------------ */
 /* Models.  p[] is the kernel parameter vector (no scale/background).  */
 /* ------------------------------------------------------------------ */
 
 /* ---- sphere (sphere.c:1-20): p = sld, sld_solvent, radius ---- */
-static double sph_vol(const double *p) { return O_4PI_3 * cb(p[2]); }
-static double sph_reff(int mode, const double *p) { (void)mode; return p[2]; }
-static void sph_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static oreal sph_vol(const oreal *p) { return O_4PI_3 * cb(p[2]); }
+static oreal sph_reff(int mode, const oreal *p) { (void)mode; return p[2]; }
+static void sph_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
     (void)quad_n;
-    const double bes = o_sph_j1c(q * p[2]);
-    const double contrast = (p[0] - p[1]);
-    const double form = contrast * sph_vol(p) * bes;
+    const oreal bes = o_sph_j1c(q * p[2]);
+    const oreal contrast = (p[0] - p[1]);
+    const oreal form = contrast * sph_vol(p) * bes;
     *f1 = 1.0e-2 * form;
     *f2 = 1.0e-4 * form * form;
 }
 
 /* ---- cylinder (cylinder.c:1-102): p = sld, sld_solvent, radius, length, theta, phi ---- */
-static double cyl_volume(double r, double l) { return O_PI * r * r * l; }
-static double cyl_vol(const double *p) { return cyl_volume(p[2], p[3]); }
-static int cyl_valid(const double *p) { return p[2] >= 0.0 && p[3] >= 0.0; }
-static double cyl_reff(int mode, const double *p)
+static oreal cyl_volume(oreal r, oreal l) { return O_PI * r * r * l; }
+static oreal cyl_vol(const oreal *p) { return cyl_volume(p[2], p[3]); }
+static int cyl_valid(const oreal *p) { return p[2] >= 0.0 && p[3] >= 0.0; }
+static oreal cyl_reff(int mode, const oreal *p)
 {
-    const double r = p[2], l = p[3];
+    const oreal r = p[2], l = p[3];
     switch (mode) {
     default:
     case 1: return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (O_PI * r + l)));
@@ -279,55 +286,55 @@ static double cyl_reff(int mode, const double *p)
     case 7: return sqrt(r * r + 0.25 * l * l);
     }
 }
-static double cyl_amp(double qab, double qc, double r, double l)
+static oreal cyl_amp(oreal qab, oreal qc, oreal r, oreal l)
 {
     return o_2j1x_x(qab * r) * o_sinx_x(qc * 0.5 * l);
 }
-static void cyl_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static void cyl_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
-    const double r = p[2], l = p[3];
-    const double *z, *w;
+    const oreal r = p[2], l = p[3];
+    const oreal *z, *w;
     const int n = quad_select(quad_n, q * fmax(r, 0.5 * l), &z, &w);
-    double t1 = 0.0, t2 = 0.0;
+    oreal t1 = 0.0, t2 = 0.0;
     for (int i = 0; i < n; i++) {
-        const double th = z[i] * O_PI_4 + O_PI_4;
-        const double s = sin(th), c = cos(th);
-        const double form = cyl_amp(q * s, q * c, r, l);
+        const oreal th = z[i] * O_PI_4 + O_PI_4;
+        const oreal s = sin(th), c = cos(th);
+        const oreal form = cyl_amp(q * s, q * c, r, l);
         t1 += w[i] * form * s;
         t2 += w[i] * form * form * s;
     }
     t1 *= O_PI_4;
     t2 *= O_PI_4;
-    const double sv = (p[0] - p[1]) * cyl_volume(r, l);
+    const oreal sv = (p[0] - p[1]) * cyl_volume(r, l);
     *f1 = 1e-2 * sv * t1;
     *f2 = 1e-4 * sv * sv * t2;
 }
-static double cyl_iqac(double qab, double qc, const double *p)
+static oreal cyl_iqac(oreal qab, oreal qc, const oreal *p)
 {
-    const double form = cyl_amp(qab, qc, p[2], p[3]);
-    const double sv = (p[0] - p[1]) * cyl_volume(p[2], p[3]);
+    const oreal form = cyl_amp(qab, qc, p[2], p[3]);
+    const oreal sv = (p[0] - p[1]) * cyl_volume(p[2], p[3]);
     return 1.0e-4 * sq(sv * form);
 }
 
 /* ---- ellipsoid (ellipsoid.c:1-105): p = sld, sld_solvent, rp, re, theta, phi ---- */
-static double ell_volume(double rp, double re) { return O_4PI_3 * rp * re * re; }
-static double ell_vol(const double *p) { return ell_volume(p[2], p[3]); }
-static double ell_reff(int mode, const double *p)
+static oreal ell_volume(oreal rp, oreal re) { return O_4PI_3 * rp * re * re; }
+static oreal ell_vol(const oreal *p) { return ell_volume(p[2], p[3]); }
+static oreal ell_reff(int mode, const oreal *p)
 {
-    const double rp = p[2], re = p[3];
+    const oreal rp = p[2], re = p[3];
     switch (mode) {
     default:
     case 1: {
         if (rp == re) return rp;
         if (rp * re == 0.) return 0.;
         /* Isihara (1950) eq. 26 mean curvature radius */
-        const double ratio = (rp < re ? rp / re : re / rp);
-        const double e1 = sqrt(1.0 - ratio * ratio);
-        const double b1 = 1.0 + asin(e1) / (e1 * ratio);
-        const double bL = (1.0 + e1) / (1.0 - e1);
-        const double b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
-        const double delta = 0.75 * b1 * b2;
-        const double ddd = 2.0 * (delta + 1.0) * rp * re * re;
+        const oreal ratio = (rp < re ? rp / re : re / rp);
+        const oreal e1 = sqrt(1.0 - ratio * ratio);
+        const oreal b1 = 1.0 + asin(e1) / (e1 * ratio);
+        const oreal bL = (1.0 + e1) / (1.0 - e1);
+        const oreal b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
+        const oreal delta = 0.75 * b1 * b2;
+        const oreal ddd = 2.0 * (delta + 1.0) * rp * re * re;
         return 0.5 * cbrt(ddd);
     }
     case 2: return cbrt(rp * re * re);
@@ -335,46 +342,46 @@ static double ell_reff(int mode, const double *p)
     case 4: return (rp > re ? rp : re);
     }
 }
-static void ell_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static void ell_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
-    const double rp = p[2], re = p[3];
-    const double vm1 = sq(rp / re) - 1.0;
-    const double *z, *w;
+    const oreal rp = p[2], re = p[3];
+    const oreal vm1 = sq(rp / re) - 1.0;
+    const oreal *z, *w;
     const int n = quad_select(quad_n, q * fmax(rp, re), &z, &w);
-    double t2 = 0.0, t1 = 0.0;
+    oreal t2 = 0.0, t1 = 0.0;
     for (int i = 0; i < n; i++) {
-        const double u = z[i] * 0.5 + 0.5;
-        const double r = re * sqrt(1.0 + u * u * vm1);
-        const double f = o_sph_j1c(q * r);
+        const oreal u = z[i] * 0.5 + 0.5;
+        const oreal r = re * sqrt(1.0 + u * u * vm1);
+        const oreal f = o_sph_j1c(q * r);
         t2 += w[i] * f * f;
         t1 += w[i] * f;
     }
     t1 *= 0.5;
     t2 *= 0.5;
-    const double sv = (p[0] - p[1]) * ell_volume(rp, re);
+    const oreal sv = (p[0] - p[1]) * ell_volume(rp, re);
     *f1 = 1e-2 * sv * t1;
     *f2 = 1e-4 * sv * sv * t2;
 }
-static double ell_iqac(double qab, double qc, const double *p)
+static oreal ell_iqac(oreal qab, oreal qc, const oreal *p)
 {
-    const double rp = p[2], re = p[3];
-    const double qr = sqrt(sq(re * qab) + sq(rp * qc));
-    const double f = o_sph_j1c(qr);
-    const double sv = (p[0] - p[1]) * ell_volume(rp, re);
+    const oreal rp = p[2], re = p[3];
+    const oreal qr = sqrt(sq(re * qab) + sq(rp * qc));
+    const oreal f = o_sph_j1c(qr);
+    const oreal sv = (p[0] - p[1]) * ell_volume(rp, re);
     return 1.0e-4 * sq(f * sv);
 }
 
 /* ---- core_shell_cylinder (core_shell_cylinder.c:1-123):
  *      p = sld_core, sld_shell, sld_solvent, radius, thickness, length, theta, phi ---- */
-static double csc_volume(double r, double t, double l) { return O_PI * sq(r + t) * (l + 2.0 * t); }
-static double csc_vol(const double *p) { return csc_volume(p[3], p[4], p[5]); }
-static double csc_reff(int mode, const double *p)
+static oreal csc_volume(oreal r, oreal t, oreal l) { return O_PI * sq(r + t) * (l + 2.0 * t); }
+static oreal csc_vol(const oreal *p) { return csc_volume(p[3], p[4], p[5]); }
+static oreal csc_reff(int mode, const oreal *p)
 {
-    const double r = p[3], t = p[4], l = p[5];
+    const oreal r = p[3], t = p[4], l = p[5];
     switch (mode) {
     default:
     case 1: {
-        const double rt = r + t, lt = l + 2.0 * t;
+        const oreal rt = r + t, lt = l + 2.0 * t;
         return 0.5 * cbrt(0.75 * rt * (2.0 * rt * lt + (rt + lt) * (O_PI * rt + lt)));
     }
     case 2: return cbrt(csc_volume(r, t, l) / O_4PI_3);
@@ -383,53 +390,53 @@ static double csc_reff(int mode, const double *p)
     case 5: return (r < 0.5 * l ? r + t : 0.5 * l + t);
     case 6: return (r > 0.5 * l ? r + t : 0.5 * l + t);
     case 7: {
-        const double ro = r + t, lo = l + 2.0 * t;
+        const oreal ro = r + t, lo = l + 2.0 * t;
         return sqrt(ro * ro + 0.25 * lo * lo);
     }
     }
 }
-static double csc_part(double vd, double besarg, double siarg)
+static oreal csc_part(oreal vd, oreal besarg, oreal siarg)
 {
     return vd * o_sinx_x(siarg) * o_2j1x_x(besarg);
 }
-static double csc_amp(double qab, double qc, const double *p)
+static oreal csc_amp(oreal qab, oreal qc, const oreal *p)
 {
-    const double r = p[3], t = p[4], l = p[5];
-    const double core_r = r;
-    const double core_h = 0.5 * l;
-    const double core_vd = csc_volume(r, 0, l) * (p[0] - p[1]);
-    const double shell_r = (r + t);
-    const double shell_h = (0.5 * l + t);
-    const double shell_vd = csc_volume(r, t, l) * (p[1] - p[2]);
+    const oreal r = p[3], t = p[4], l = p[5];
+    const oreal core_r = r;
+    const oreal core_h = 0.5 * l;
+    const oreal core_vd = csc_volume(r, 0, l) * (p[0] - p[1]);
+    const oreal shell_r = (r + t);
+    const oreal shell_h = (0.5 * l + t);
+    const oreal shell_vd = csc_volume(r, t, l) * (p[1] - p[2]);
     return csc_part(core_vd, core_r * qab, core_h * qc)
          + csc_part(shell_vd, shell_r * qab, shell_h * qc);
 }
-static void csc_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static void csc_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
-    const double r = p[3], t = p[4], l = p[5];
-    const double *z, *w;
+    const oreal r = p[3], t = p[4], l = p[5];
+    const oreal *z, *w;
     const int n = quad_select(quad_n, q * fmax(r + t, 0.5 * l + t), &z, &w);
-    double t1 = 0.0, t2 = 0.0;
+    oreal t1 = 0.0, t2 = 0.0;
     for (int i = 0; i < n; i++) {
-        const double th = z[i] * O_PI_4 + O_PI_4;
-        const double s = sin(th), c = cos(th);
-        const double fq = csc_amp(q * s, q * c, p);
+        const oreal th = z[i] * O_PI_4 + O_PI_4;
+        const oreal s = sin(th), c = cos(th);
+        const oreal fq = csc_amp(q * s, q * c, p);
         t1 += w[i] * fq * s;
         t2 += w[i] * fq * fq * s;
     }
     *f1 = 1.0e-2 * t1 * O_PI_4;
     *f2 = 1.0e-4 * t2 * O_PI_4;
 }
-static double csc_iqac(double qab, double qc, const double *p)
+static oreal csc_iqac(oreal qab, oreal qc, const oreal *p)
 {
-    const double fq = csc_amp(qab, qc, p);
+    const oreal fq = csc_amp(qab, qc, p);
     return 1.0e-4 * fq * fq;
 }
 
 /* ---- core_shell_sphere (core_shell_sphere.c:1-30, lib/core_shell.c:8-30):
  *      p = radius, thickness, sld_core, sld_shell, sld_solvent ---- */
-static double css_vol(const double *p) { return O_4PI_3 * cb(p[0] + p[1]); }
-static double css_reff(int mode, const double *p)
+static oreal css_vol(const oreal *p) { return O_4PI_3 * cb(p[0] + p[1]); }
+static oreal css_reff(int mode, const oreal *p)
 {
     switch (mode) {
     default:
@@ -437,19 +444,19 @@ static double css_reff(int mode, const double *p)
     case 2: return p[0];
     }
 }
-static void css_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static void css_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
     (void)quad_n;
-    const double r = p[0], t = p[1];
-    const double core_qr = q * r;
-    const double core_contrast = p[2] - p[3];
-    const double core_bes = o_sph_j1c(core_qr);
-    const double core_volume = O_4PI_3 * cb(r);
-    double f = core_volume * core_bes * core_contrast;
-    const double shell_qr = q * (r + t);
-    const double shell_contrast = p[3] - p[4];
-    const double shell_bes = o_sph_j1c(shell_qr);
-    const double shell_volume = O_4PI_3 * cb(r + t);
+    const oreal r = p[0], t = p[1];
+    const oreal core_qr = q * r;
+    const oreal core_contrast = p[2] - p[3];
+    const oreal core_bes = o_sph_j1c(core_qr);
+    const oreal core_volume = O_4PI_3 * cb(r);
+    oreal f = core_volume * core_bes * core_contrast;
+    const oreal shell_qr = q * (r + t);
+    const oreal shell_contrast = p[3] - p[4];
+    const oreal shell_bes = o_sph_j1c(shell_qr);
+    const oreal shell_volume = O_4PI_3 * cb(r + t);
     f += shell_volume * shell_bes * shell_contrast;
     *f1 = 1.0e-2 * f;
     *f2 = 1.0e-4 * f * f;
@@ -457,25 +464,25 @@ static void css_fq(double q, const double *p, int quad_n, double *f1, double *f2
 
 /* ---- parallelepiped (parallelepiped.c:1-150):
  *      p = sld, sld_solvent, length_a, length_b, length_c, theta, phi, psi ---- */
-static double ppd_vol(const double *p) { return p[2] * p[3] * p[4]; }
-static double ppd_excluded(double a, double b, double c)
+static oreal ppd_vol(const oreal *p) { return p[2] * p[3] * p[4]; }
+static oreal ppd_excluded(oreal a, oreal b, oreal c)
 {
-    const double len[3] = {a, b, c};
-    const double lmax = fmax(len[0], fmax(len[1], len[2]));
-    double l1 = lmax, l2 = lmax;
-    const double l3 = lmax;
+    const oreal len[3] = {a, b, c};
+    const oreal lmax = fmax(len[0], fmax(len[1], len[2]));
+    oreal l1 = lmax, l2 = lmax;
+    const oreal l3 = lmax;
     for (int i = 0; i < 3; i++) {
         if (len[i] < l1) { l2 = l1; l1 = len[i]; }
         else if (len[i] < l2) l2 = len[i];
     }
-    double r, l;
+    oreal r, l;
     if (l2 - l1 > l3 - l2) { r = sqrt(l2 * l3 / O_PI); l = l1; }
     else { r = sqrt(l1 * l2 / O_PI); l = l3; }
     return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (O_PI * r + l)));
 }
-static double ppd_reff(int mode, const double *p)
+static oreal ppd_reff(int mode, const oreal *p)
 {
-    const double a = p[2], b = p[3], c = p[4];
+    const oreal a = p[2], b = p[3], c = p[4];
     switch (mode) {
     default:
     case 1: return ppd_excluded(a, b, c);
@@ -488,105 +495,105 @@ static double ppd_reff(int mode, const double *p)
     case 8: return 0.5 * sqrt(a * a + b * b + c * c);
     }
 }
-static void ppd_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static void ppd_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
-    const double a = p[2], b = p[3], c = p[4];
-    const double mu = 0.5 * q * b;
-    const double as = a / b, cs = c / b;
-    const double *zo, *wo;
+    const oreal a = p[2], b = p[3], c = p[4];
+    const oreal mu = 0.5 * q * b;
+    const oreal as = a / b, cs = c / b;
+    const oreal *zo, *wo;
     const int no = quad_select_nested(quad_n, q * fmax(sqrt(a * a + b * b), c) / 2, O_ADAPTIVE_MAX_OUTER, &zo, &wo);
-    double o1 = 0.0, o2 = 0.0;
+    oreal o1 = 0.0, o2 = 0.0;
     for (int i = 0; i < no; i++) {
-        const double sigma = 0.5 * (zo[i] + 1.0);
-        const double mp = mu * sqrt(1.0 - sigma * sigma);
-        const double *zi, *wi;
+        const oreal sigma = 0.5 * (zo[i] + 1.0);
+        const oreal mp = mu * sqrt(1.0 - sigma * sigma);
+        const oreal *zi, *wi;
         const int ni = quad_select_nested(quad_n, mp * fmax(as, 1.0), no, &zi, &wi);
-        double i1 = 0.0, i2 = 0.0;
+        oreal i1 = 0.0, i2 = 0.0;
         for (int j = 0; j < ni; j++) {
-            const double uu = 0.5 * (zi[j] + 1.0);
-            const double su = sin(O_PI / 2 * uu), cu = cos(O_PI / 2 * uu);
-            const double fq = o_sinx_x(mp * su * as) * o_sinx_x(mp * cu);
+            const oreal uu = 0.5 * (zi[j] + 1.0);
+            const oreal su = sin(O_PI / 2 * uu), cu = cos(O_PI / 2 * uu);
+            const oreal fq = o_sinx_x(mp * su * as) * o_sinx_x(mp * cu);
             i1 += wi[j] * fq;
             i2 += wi[j] * fq * fq;
         }
         i1 *= 0.5;
         i2 *= 0.5;
-        const double si = o_sinx_x(mu * cs * sigma);
+        const oreal si = o_sinx_x(mu * cs * sigma);
         o1 += wo[i] * i1 * si;
         o2 += wo[i] * i2 * si * si;
     }
     o1 *= 0.5;
     o2 *= 0.5;
-    const double s = (p[0] - p[1]) * ppd_vol(p);
+    const oreal s = (p[0] - p[1]) * ppd_vol(p);
     *f1 = 1.0e-2 * s * o1;
     *f2 = 1.0e-4 * s * s * o2;
 }
-static double ppd_iqabc(double qa, double qb, double qc, const double *p)
+static oreal ppd_iqabc(oreal qa, oreal qb, oreal qc, const oreal *p)
 {
-    const double f = ppd_vol(p) * (p[0] - p[1]) * o_sinx_x(0.5 * p[2] * qa) * o_sinx_x(0.5 * p[3] * qb)
+    const oreal f = ppd_vol(p) * (p[0] - p[1]) * o_sinx_x(0.5 * p[2] * qa) * o_sinx_x(0.5 * p[3] * qb)
                      * o_sinx_x(0.5 * p[4] * qc);
     return 1.0e-4 * f * f;
 }
 
 /* ---- triaxial ellipsoid (triaxial_ellipsoid.c:1-160): p = sld, sld_solvent,
  *      radius_equat_minor, radius_equat_major, radius_polar, theta, phi, psi ---- */
-static double tri_vol(const double *p) { return O_4PI_3 * p[2] * p[3] * p[4]; }
-static double tri_curvature(double ra, double rb, double rc)
+static oreal tri_vol(const oreal *p) { return O_4PI_3 * p[2] * p[3] * p[4]; }
+static oreal tri_curvature(oreal ra, oreal rb, oreal rc)
 {
     /* the reference compares the 0/1 result of ra == rb with rc */
     if ((ra == rb) == rc) return rc;
     if (ra * rb * rc == 0.) return 0.;
-    const double rad[3] = {ra, rb, rc};
-    const double rmax = fmax(rad[0], fmax(rad[1], rad[2]));
-    double r1 = rmax, r2 = rmax, r3 = rmax;
+    const oreal rad[3] = {ra, rb, rc};
+    const oreal rmax = fmax(rad[0], fmax(rad[1], rad[2]));
+    oreal r1 = rmax, r2 = rmax, r3 = rmax;
     for (int i = 0; i < 3; i++) {
         if (rad[i] < r1) { r3 = r2; r2 = r1; r1 = rad[i]; }
         else if (rad[i] < r2) r2 = rad[i];
     }
-    double re, rp;
+    oreal re, rp;
     if (r2 - r1 > r3 - r2) { re = sqrt(r2 * r3); rp = r1; }
     else { re = sqrt(r1 * r2); rp = r3; }
-    const double ratio = (rp < re ? rp / re : re / rp);
-    const double e1 = sqrt(1.0 - ratio * ratio);
-    const double b1 = 1.0 + asin(e1) / (e1 * ratio);
-    const double bL = (1.0 + e1) / (1.0 - e1);
-    const double b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
-    const double delta = 0.75 * b1 * b2;
-    const double ddd = 2.0 * (delta + 1.0) * rp * re * re;
+    const oreal ratio = (rp < re ? rp / re : re / rp);
+    const oreal e1 = sqrt(1.0 - ratio * ratio);
+    const oreal b1 = 1.0 + asin(e1) / (e1 * ratio);
+    const oreal bL = (1.0 + e1) / (1.0 - e1);
+    const oreal b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
+    const oreal delta = 0.75 * b1 * b2;
+    const oreal ddd = 2.0 * (delta + 1.0) * rp * re * re;
     return 0.5 * cbrt(ddd);
 }
-static double tri_reff(int mode, const double *p)
+static oreal tri_reff(int mode, const oreal *p)
 {
-    const double ra = p[2], rb = p[3], rc = p[4];
+    const oreal ra = p[2], rb = p[3], rc = p[4];
     switch (mode) {
     default:
     case 1: return tri_curvature(ra, rb, rc);
     case 2: return cbrt(ra * rb * rc);
-    case 3: { const double m = (ra < rb ? ra : rb); return (m < rc ? m : rc); }
-    case 4: { const double m = (ra < rb ? rb : ra); return (m > rc ? m : rc); }
+    case 3: { const oreal m = (ra < rb ? ra : rb); return (m < rc ? m : rc); }
+    case 4: { const oreal m = (ra < rb ? rb : ra); return (m > rc ? m : rc); }
     }
 }
-static void tri_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static void tri_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
-    const double lo = fmin(p[2], p[3]), hi = fmax(p[2], p[3]);
-    const double mid = fmax(lo, p[4]);
-    const double minor = fmin(lo, p[4]), major = fmin(hi, mid), polar = fmax(hi, mid);
-    const double pa = sq(minor / major) - 1.0;
-    const double pc = sq(polar / major) - 1.0;
-    const double qr_inner = fmax(q * minor, q * major);
-    const double qr_max = fmax(qr_inner, q * polar);
-    const double *zo, *wo, *zi, *wi;
+    const oreal lo = fmin(p[2], p[3]), hi = fmax(p[2], p[3]);
+    const oreal mid = fmax(lo, p[4]);
+    const oreal minor = fmin(lo, p[4]), major = fmin(hi, mid), polar = fmax(hi, mid);
+    const oreal pa = sq(minor / major) - 1.0;
+    const oreal pc = sq(polar / major) - 1.0;
+    const oreal qr_inner = fmax(q * minor, q * major);
+    const oreal qr_max = fmax(qr_inner, q * polar);
+    const oreal *zo, *wo, *zi, *wi;
     const int no = quad_select_nested(quad_n, qr_max, O_ADAPTIVE_MAX_OUTER, &zo, &wo);
     const int ni = quad_select_nested(quad_n, qr_inner, no, &zi, &wi);
-    double o1 = 0.0, o2 = 0.0;
+    oreal o1 = 0.0, o2 = 0.0;
     for (int i = 0; i < no; i++) {
-        const double phi = zo[i] * O_PI_4 + O_PI_4;
-        const double pas = pa * sq(sin(phi));
-        double i1 = 0.0, i2 = 0.0;
+        const oreal phi = zo[i] * O_PI_4 + O_PI_4;
+        const oreal pas = pa * sq(sin(phi));
+        oreal i1 = 0.0, i2 = 0.0;
         for (int j = 0; j < ni; j++) {
-            const double usq = sq(zi[j] * 0.5 + 0.5);
-            const double r = major * sqrt(pas * (1.0 - usq) + 1.0 + pc * usq);
-            const double fq = o_sph_j1c(q * r);
+            const oreal usq = sq(zi[j] * 0.5 + 0.5);
+            const oreal r = major * sqrt(pas * (1.0 - usq) + 1.0 + pc * usq);
+            const oreal fq = o_sph_j1c(q * r);
             i1 += wi[j] * fq;
             i2 += wi[j] * fq * fq;
         }
@@ -595,22 +602,22 @@ static void tri_fq(double q, const double *p, int quad_n, double *f1, double *f2
     }
     o1 *= 0.25;
     o2 *= 0.25;
-    const double s = (p[0] - p[1]) * O_4PI_3 * minor * major * polar;
+    const oreal s = (p[0] - p[1]) * O_4PI_3 * minor * major * polar;
     *f1 = 1.0e-2 * s * o1;
     *f2 = 1.0e-4 * s * s * o2;
 }
-static double tri_iqabc(double qa, double qb, double qc, const double *p)
+static oreal tri_iqabc(oreal qa, oreal qb, oreal qc, const oreal *p)
 {
-    const double qr = sqrt(sq(p[2] * qa) + sq(p[3] * qb) + sq(p[4] * qc));
-    const double f = o_sph_j1c(qr);
+    const oreal qr = sqrt(sq(p[2] * qa) + sq(p[3] * qb) + sq(p[4] * qc));
+    const oreal f = o_sph_j1c(qr);
     return 1.0e-4 * sq(tri_vol(p) * (p[0] - p[1]) * f);
 }
 
 /* ---- paracrystals (bcc/fcc/sc_paracrystal.c with lib/sphere_form.c):
  *      p = dnn, d_factor, radius, sld, sld_solvent, theta, phi, psi ---- */
-static double pc_zq(int lat, double qa, double qb, double qc, double dnn, double d_factor)
+static oreal pc_zq(int lat, oreal qa, oreal qb, oreal qc, oreal dnn, oreal d_factor)
 {
-    double a1, a2, a3, d_a;
+    oreal a1, a2, a3, d_a;
     if (lat == 0) {        /* bcc */
         a1 = (-qa + qb + qc) / 2.0; a2 = (+qa - qb + qc) / 2.0; a3 = (+qa + qb - qc) / 2.0;
         d_a = dnn / sqrt(0.75);
@@ -621,59 +628,59 @@ static double pc_zq(int lat, double qa, double qb, double qc, double dnn, double
         a1 = qa; a2 = qb; a3 = qc;
         d_a = dnn;
     }
-    const double arg = -0.5 * sq(dnn * d_factor) * (a1 * a1 + a2 * a2 + a3 * a3);
-    const double exp_arg = exp(arg);
+    const oreal arg = -0.5 * sq(dnn * d_factor) * (a1 * a1 + a2 * a2 + a3 * a3);
+    const oreal exp_arg = exp(arg);
     return -cb(expm1(2.0 * arg))
         / (((exp_arg - 2.0 * cos(d_a * a1)) * exp_arg + 1.0)
            * ((exp_arg - 2.0 * cos(d_a * a2)) * exp_arg + 1.0)
            * ((exp_arg - 2.0 * cos(d_a * a3)) * exp_arg + 1.0));
 }
-static double pc_vfrac(int lat, double radius, double dnn)
+static oreal pc_vfrac(int lat, oreal radius, oreal dnn)
 {
     if (lat == 0) return 2.0 * (O_4PI_3 * cb(sqrt(0.75) * radius / dnn));
     if (lat == 1) return 4.0 * (O_4PI_3 * cb(O_SQRT1_2 * radius / dnn));
     return O_4PI_3 * cb(radius / dnn);
 }
-static double pc_sphere_form(double q, const double *p)
+static oreal pc_sphere_form(oreal q, const oreal *p)
 {
-    const double fq = O_4PI_3 * cb(p[2]) * o_sph_j1c(q * p[2]);
+    const oreal fq = O_4PI_3 * cb(p[2]) * o_sph_j1c(q * p[2]);
     return 1.0e-4 * sq((p[3] - p[4]) * fq);
 }
-static double pc_vol(const double *p) { return O_4PI_3 * cb(p[2]); }
-static double pc_reff(int mode, const double *p) { (void)mode; (void)p; return 0.0; }
-static double pc_iq(int lat, double q, const double *p, int quad_n)
+static oreal pc_vol(const oreal *p) { return O_4PI_3 * cb(p[2]); }
+static oreal pc_reff(int mode, const oreal *p) { (void)mode; (void)p; return 0.0; }
+static oreal pc_iq(int lat, oreal q, const oreal *p, int quad_n)
 {
-    const double *z, *w;
+    const oreal *z, *w;
     const int n = gl_table(quad_n > 0 ? quad_n : 150, &z, &w);
-    const double tm = lat == 2 ? O_PI_4 : O_PI / 2, tb = tm;
-    const double pm = lat == 2 ? O_PI_4 : O_PI, pb = pm;
-    double outer = 0.0;
+    const oreal tm = lat == 2 ? O_PI_4 : O_PI / 2, tb = tm;
+    const oreal pm = lat == 2 ? O_PI_4 : O_PI, pb = pm;
+    oreal outer = 0.0;
     for (int i = 0; i < n; i++) {
-        double inner = 0.0;
-        const double theta = z[i] * tm + tb;
-        const double st = sin(theta), ct = cos(theta);
-        const double qc = q * ct, qab = q * st;
+        oreal inner = 0.0;
+        const oreal theta = z[i] * tm + tb;
+        const oreal st = sin(theta), ct = cos(theta);
+        const oreal qc = q * ct, qab = q * st;
         for (int j = 0; j < n; j++) {
-            const double phi = z[j] * pm + pb;
-            const double sp = sin(phi), cp = cos(phi);
+            const oreal phi = z[j] * pm + pb;
+            const oreal sp = sin(phi), cp = cos(phi);
             inner += w[j] * pc_zq(lat, qab * cp, qab * sp, qc, p[0], p[1]);
         }
         inner *= pm;
         outer += w[i] * inner * st;
     }
     outer *= tm;
-    const double zq = lat == 2 ? outer / (O_PI / 2) : outer / (4.0 * O_PI);
+    const oreal zq = lat == 2 ? outer / (O_PI / 2) : outer / (4.0 * O_PI);
     return pc_vfrac(lat, p[2], p[0]) * pc_sphere_form(q, p) * zq;
 }
-static double pc_iqabc(int lat, double qa, double qb, double qc, const double *p)
+static oreal pc_iqabc(int lat, oreal qa, oreal qb, oreal qc, const oreal *p)
 {
-    const double q = sqrt(qa * qa + qb * qb + qc * qc);
+    const oreal q = sqrt(qa * qa + qb * qb + qc * qc);
     return pc_vfrac(lat, p[2], p[0]) * pc_sphere_form(q, p) * pc_zq(lat, qa, qb, qc, p[0], p[1]);
 }
 #define PC_FNS(L, NAME) \
-    static void NAME##_fq(double q, const double *p, int quad_n, double *f1, double *f2) \
+    static void NAME##_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2) \
     { *f1 = 0.0; *f2 = pc_iq(L, q, p, quad_n); } \
-    static double NAME##_iqabc(double qa, double qb, double qc, const double *p) \
+    static oreal NAME##_iqabc(oreal qa, oreal qb, oreal qc, const oreal *p) \
     { return pc_iqabc(L, qa, qb, qc, p); }
 PC_FNS(0, bcc)
 PC_FNS(1, fcc)
@@ -681,11 +688,11 @@ PC_FNS(2, scp)
 
 /* ---- hollow cylinder (hollow_cylinder.c:1-140): p = radius, thickness,
  *      length, sld, sld_solvent, theta, phi ---- */
-static double hcy_vol(const double *p) { return O_PI * p[2] * sq(p[0] + p[1]); }
-static double hcy_svol(const double *p) { return O_PI * p[2] * (sq(p[0] + p[1]) - p[0] * p[0]); }
-static double hcy_reff(int mode, const double *p)
+static oreal hcy_vol(const oreal *p) { return O_PI * p[2] * sq(p[0] + p[1]); }
+static oreal hcy_svol(const oreal *p) { return O_PI * p[2] * (sq(p[0] + p[1]) - p[0] * p[0]); }
+static oreal hcy_reff(int mode, const oreal *p)
 {
-    const double ro = p[0] + p[1], l = p[2];
+    const oreal ro = p[0] + p[1], l = p[2];
     switch (mode) {
     default:
     case 1: return 0.5 * cbrt(0.75 * ro * (2.0 * ro * l + (ro + l) * (O_PI * ro + l)));
@@ -697,48 +704,48 @@ static double hcy_reff(int mode, const double *p)
     case 7: return sqrt(sq(p[0] + p[1]) + 0.25 * sq(l));
     }
 }
-static double hcy_amp(double qab, double qc, const double *p)
+static oreal hcy_amp(oreal qab, oreal qc, const oreal *p)
 {
-    const double r = p[0], t = p[1], l = p[2];
-    const double lam1 = o_2j1x_x((r + t) * qab);
-    const double lam2 = o_2j1x_x(r * qab);
-    const double gamma_sq = sq(r / (r + t));
-    const double psi = (lam1 - gamma_sq * lam2) / (1.0 - gamma_sq);
+    const oreal r = p[0], t = p[1], l = p[2];
+    const oreal lam1 = o_2j1x_x((r + t) * qab);
+    const oreal lam2 = o_2j1x_x(r * qab);
+    const oreal gamma_sq = sq(r / (r + t));
+    const oreal psi = (lam1 - gamma_sq * lam2) / (1.0 - gamma_sq);
     return psi * o_sinx_x(0.5 * l * qc);
 }
-static void hcy_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static void hcy_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
-    const double *z, *w;
+    const oreal *z, *w;
     const int n = quad_select(quad_n, q * fmax(p[0] + p[1], 0.5 * p[2]), &z, &w);
-    double t1 = 0.0, t2 = 0.0;
+    oreal t1 = 0.0, t2 = 0.0;
     for (int i = 0; i < n; i++) {
-        const double ct = 0.5 * (z[i] * (1.0 - 0.0) + 0.0 + 1.0);
-        const double st = sqrt(1.0 - ct * ct);
-        const double form = hcy_amp(q * st, q * ct, p);
+        const oreal ct = 0.5 * (z[i] * (1.0 - 0.0) + 0.0 + 1.0);
+        const oreal st = sqrt(1.0 - ct * ct);
+        const oreal form = hcy_amp(q * st, q * ct, p);
         t1 += w[i] * form;
         t2 += w[i] * form * form;
     }
     t1 *= 0.5 * (1.0 - 0.0);
     t2 *= 0.5 * (1.0 - 0.0);
-    const double s = (p[3] - p[4]) * hcy_svol(p);
+    const oreal s = (p[3] - p[4]) * hcy_svol(p);
     *f1 = 1e-2 * s * t1;
     *f2 = 1e-4 * s * s * t2;
 }
-static double hcy_iqac(double qab, double qc, const double *p)
+static oreal hcy_iqac(oreal qab, oreal qc, const oreal *p)
 {
-    const double form = hcy_amp(qab, qc, p);
-    const double s = (p[3] - p[4]) * hcy_svol(p);
+    const oreal form = hcy_amp(qab, qc, p);
+    const oreal s = (p[3] - p[4]) * hcy_svol(p);
     return 1.0e-4 * sq(s * form);
 }
 
 /* ---- vesicle (vesicle.c:1-50): p = sld, sld_solvent, volfraction, radius, thickness ---- */
-static double ves_vol(const double *p) { return O_4PI_3 * cb(p[3] + p[4]); }
-static double ves_svol(const double *p) { return O_4PI_3 * (cb(p[3] + p[4]) - cb(p[3])); }
-static double ves_reff(int mode, const double *p) { (void)mode; return p[3] + p[4]; }
-static void ves_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static oreal ves_vol(const oreal *p) { return O_4PI_3 * cb(p[3] + p[4]); }
+static oreal ves_svol(const oreal *p) { return O_4PI_3 * (cb(p[3] + p[4]) - cb(p[3])); }
+static oreal ves_reff(int mode, const oreal *p) { (void)mode; return p[3] + p[4]; }
+static void ves_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
     (void)quad_n;
-    double vol, contrast, f;
+    oreal vol, contrast, f;
     contrast = p[1] - p[0];
     vol = O_4PI_3 * cb(p[3]);
     f = vol * o_sph_j1c(q * p[3]) * contrast;
@@ -750,14 +757,14 @@ static void ves_fq(double q, const double *p, int quad_n, double *f1, double *f2
 }
 
 /* ---- fuzzy sphere (fuzzy_sphere.c:1-35): p = sld, sld_solvent, radius, fuzziness ---- */
-static double fuz_vol(const double *p) { return O_4PI_3 * cb(p[2]); }
-static double fuz_reff(int mode, const double *p) { return mode == 2 ? p[2] + p[3] : p[2]; }
-static void fuz_fq(double q, const double *p, int quad_n, double *f1, double *f2)
+static oreal fuz_vol(const oreal *p) { return O_4PI_3 * cb(p[2]); }
+static oreal fuz_reff(int mode, const oreal *p) { return mode == 2 ? p[2] + p[3] : p[2]; }
+static void fuz_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
 {
     (void)quad_n;
-    const double bes = o_sph_j1c(q * p[2]);
-    const double qf = exp(-0.5 * sq(q * p[3]));
-    const double form = (p[0] - p[1]) * fuz_vol(p) * bes * qf;
+    const oreal bes = o_sph_j1c(q * p[2]);
+    const oreal qf = exp(-0.5 * sq(q * p[3]));
+    const oreal form = (p[0] - p[1]) * fuz_vol(p) * bes * qf;
     *f1 = 1.0e-2 * form;
     *f2 = 1.0e-4 * form * form;
 }
@@ -768,17 +775,17 @@ static void fuz_fq(double q, const double *p, int quad_n, double *f1, double *f2
 typedef struct {
     int npars, nmag, max_pd, oriented;
     int sld_idx[3];
-    int (*valid)(const double *p);
-    double (*vol)(const double *p);
-    double (*reff)(int mode, const double *p);
-    void (*fq)(double q, const double *p, int quad_n, double *f1, double *f2);
-    double (*iqac)(double qab, double qc, const double *p);
-    double (*iqabc)(double qa, double qb, double qc, const double *p);
+    int (*valid)(const oreal *p);
+    oreal (*vol)(const oreal *p);
+    oreal (*reff)(int mode, const oreal *p);
+    void (*fq)(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2);
+    oreal (*iqac)(oreal qab, oreal qc, const oreal *p);
+    oreal (*iqabc)(oreal qa, oreal qb, oreal qc, const oreal *p);
     int no_fq;   /* Iq-only model: one output per q in 1D too */
-    double (*svol)(const double *p);   /* shell volume of hollow models (else the form volume) */
+    oreal (*svol)(const oreal *p);   /* shell volume of hollow models (else the form volume) */
 } OModel;
 
-static int always_valid(const double *p) { (void)p; return 1; }
+static int always_valid(const oreal *p) { (void)p; return 1; }
 
 /* model ids: 0 sphere, 1 cylinder, 2 ellipsoid, 3 core_shell_cylinder,
  * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid, 8-10 bcc,
@@ -807,47 +814,47 @@ static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npar
 /* Rotation and magnetism helpers                                      */
 /* ------------------------------------------------------------------ */
 /* Third row of (jitter^-1 * view^-1); maps (qx, qy) to qc (kernel_iq.c:197-230) */
-static void view_jitter_row(double theta, double phi, double dtheta, double dphi,
-                            double *r31, double *r32)
+static void view_jitter_row(oreal theta, oreal phi, oreal dtheta, oreal dphi,
+                            oreal *r31, oreal *r32)
 {
-    const double st = sin(theta * O_PI_180), ct = cos(theta * O_PI_180);
-    const double sp = sin(phi * O_PI_180), cp = cos(phi * O_PI_180);
-    const double v11 = cp * ct, v12 = sp * ct;
-    const double v21 = -sp, v22 = cp;
-    const double v31 = st * cp, v32 = sp * st;
-    const double sdt = sin(dtheta * O_PI_180), cdt = cos(dtheta * O_PI_180);
-    const double sdp = sin(dphi * O_PI_180), cdp = cos(dphi * O_PI_180);
-    const double j31 = sdt, j32 = -sdp * cdt, j33 = cdp * cdt;
+    const oreal st = sin(theta * O_PI_180), ct = cos(theta * O_PI_180);
+    const oreal sp = sin(phi * O_PI_180), cp = cos(phi * O_PI_180);
+    const oreal v11 = cp * ct, v12 = sp * ct;
+    const oreal v21 = -sp, v22 = cp;
+    const oreal v31 = st * cp, v32 = sp * st;
+    const oreal sdt = sin(dtheta * O_PI_180), cdt = cos(dtheta * O_PI_180);
+    const oreal sdp = sin(dphi * O_PI_180), cdp = cos(dphi * O_PI_180);
+    const oreal j31 = sdt, j32 = -sdp * cdt, j33 = cdp * cdt;
     *r31 = j31 * v11 + j32 * v21 + j33 * v31;
     *r32 = j31 * v12 + j32 * v22 + j33 * v32;
 }
 
 /* First two columns of (jitter^-1 * view^-1) with psi: (qx, qy) -> (qa, qb, qc)
  * (kernel_iq.c:251-296), r[0..5] = R11, R12, R21, R22, R31, R32 */
-static void view_jitter_abc(double theta, double phi, double psi,
-                            double dtheta, double dphi, double dpsi, double r[6])
+static void view_jitter_abc(oreal theta, oreal phi, oreal psi,
+                            oreal dtheta, oreal dphi, oreal dpsi, oreal r[6])
 {
-    double st = sin(theta * O_PI_180), ct = cos(theta * O_PI_180);
-    double sp = sin(phi * O_PI_180), cp = cos(phi * O_PI_180);
-    double ss = sin(psi * O_PI_180), cs = cos(psi * O_PI_180);
-    const double v11 = -sp * ss + cp * cs * ct;
-    const double v12 = sp * cs * ct + ss * cp;
-    const double v21 = -sp * cs - ss * cp * ct;
-    const double v22 = -sp * ss * ct + cp * cs;
-    const double v31 = st * cp;
-    const double v32 = sp * st;
+    oreal st = sin(theta * O_PI_180), ct = cos(theta * O_PI_180);
+    oreal sp = sin(phi * O_PI_180), cp = cos(phi * O_PI_180);
+    oreal ss = sin(psi * O_PI_180), cs = cos(psi * O_PI_180);
+    const oreal v11 = -sp * ss + cp * cs * ct;
+    const oreal v12 = sp * cs * ct + ss * cp;
+    const oreal v21 = -sp * cs - ss * cp * ct;
+    const oreal v22 = -sp * ss * ct + cp * cs;
+    const oreal v31 = st * cp;
+    const oreal v32 = sp * st;
     st = sin(dtheta * O_PI_180); ct = cos(dtheta * O_PI_180);
     sp = sin(dphi * O_PI_180); cp = cos(dphi * O_PI_180);
     ss = sin(dpsi * O_PI_180); cs = cos(dpsi * O_PI_180);
-    const double j11 = cs * ct;
-    const double j12 = sp * st * cs + ss * cp;
-    const double j13 = sp * ss - st * cp * cs;
-    const double j21 = -ss * ct;
-    const double j22 = -sp * ss * st + cp * cs;
-    const double j23 = sp * cs + ss * st * cp;
-    const double j31 = st;
-    const double j32 = -sp * ct;
-    const double j33 = cp * ct;
+    const oreal j11 = cs * ct;
+    const oreal j12 = sp * st * cs + ss * cp;
+    const oreal j13 = sp * ss - st * cp * cs;
+    const oreal j21 = -ss * ct;
+    const oreal j22 = -sp * ss * st + cp * cs;
+    const oreal j23 = sp * cs + ss * st * cp;
+    const oreal j31 = st;
+    const oreal j32 = -sp * ct;
+    const oreal j33 = cp * ct;
     r[0] = j11 * v11 + j12 * v21 + j13 * v31;
     r[1] = j11 * v12 + j12 * v22 + j13 * v32;
     r[2] = j21 * v11 + j22 * v21 + j23 * v31;
@@ -857,11 +864,11 @@ static void view_jitter_abc(double theta, double phi, double psi,
 }
 
 /* spin cross-section weights dd, du.re, ud.re, uu, du.im, ud.im (kernel_iq.c:84-124) */
-static void spin_weights(double in_spin, double out_spin, double wt[6])
+static void spin_weights(oreal in_spin, oreal out_spin, oreal wt[6])
 {
     in_spin = in_spin < 0.0 ? 0.0 : in_spin > 1.0 ? 1.0 : in_spin;
     out_spin = out_spin < 0.0 ? 0.0 : out_spin > 1.0 ? 1.0 : out_spin;
-    const double norm = out_spin < 0.5 ? 1 - out_spin : out_spin;
+    const oreal norm = out_spin < 0.5 ? 1 - out_spin : out_spin;
     wt[0] = (1.0 - in_spin) * (1.0 - out_spin) / norm;
     wt[1] = (1.0 - in_spin) * out_spin / norm;
     wt[2] = in_spin * (1.0 - out_spin) / norm;
@@ -870,20 +877,20 @@ static void spin_weights(double in_spin, double out_spin, double wt[6])
     wt[5] = wt[2];
 }
 
-static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static oreal dot3(const oreal *a, const oreal *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
 /* effective sld for cross section xs (kernel_iq.c:127-175) */
-static double magnetic_sld(int xs, double qx, double qy, double cmt, double smt,
-                           double cmp, double smp, double sld, double mx, double my, double mz)
+static oreal magnetic_sld(int xs, oreal qx, oreal qy, oreal cmt, oreal smt,
+                           oreal cmp, oreal smp, oreal sld, oreal mx, oreal my, oreal mz)
 {
-    const double qn = sqrt(qx * qx + qy * qy);
-    const double qhat[3] = {qx / qn, qy / qn, 0};
-    const double m[3] = {mx, my, mz};
-    const double pol[3] = {smt * cmp, smt * smp, cmt};
-    const double py[3] = {-smp, cmp, 0};
-    const double pz[3] = {-cmt * cmp, -cmt * smp, smt};
-    const double k = dot3(m, qhat) / dot3(qhat, qhat);
-    const double mp[3] = {m[0] - k * qhat[0], m[1] - k * qhat[1], m[2] - k * qhat[2]};
+    const oreal qn = sqrt(qx * qx + qy * qy);
+    const oreal qhat[3] = {qx / qn, qy / qn, 0};
+    const oreal m[3] = {mx, my, mz};
+    const oreal pol[3] = {smt * cmp, smt * smp, cmt};
+    const oreal py[3] = {-smp, cmp, 0};
+    const oreal pz[3] = {-cmt * cmp, -cmt * smp, smt};
+    const oreal k = dot3(m, qhat) / dot3(qhat, qhat);
+    const oreal mp[3] = {m[0] - k * qhat[0], m[1] - k * qhat[1], m[2] - k * qhat[2]};
     switch (xs) {
     case 0: return sld - dot3(pol, mp);
     case 1: case 2: return dot3(py, mp);
@@ -910,9 +917,9 @@ void sasoracle_set_threads(int n) { n_threads = n > 0 ? n : 1; }
  */
 int sasoracle_kernel(int model_id, int kind, int quad_n,
                      int32_t nq, int32_t pd_start, int32_t pd_stop,
-                     const int32_t *details, const double *values,
-                     const double *q, double *result,
-                     double cutoff, int32_t reff_mode)
+                     const int32_t *details, const oreal *values,
+                     const oreal *q, oreal *result,
+                     oreal cutoff, int32_t reff_mode)
 {
     if (!model_ok(model_id) || kind < 0 || kind > 2) return -1;
     if (quad_n < 0) return -2;
@@ -922,20 +929,20 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
     const int32_t *pd_off = details + 2 * maxpd, *pd_stride = details + 3 * maxpd;
     const int32_t num_weights = details[4 * maxpd + 1];
     const int32_t theta_par = details[4 * maxpd + 3];
-    const double *pd_value = values + (2 + np + (m->nmag ? 4 + 3 * m->nmag : 0));
-    const double *pd_weight = pd_value + num_weights;
+    const oreal *pd_value = values + (2 + np + (m->nmag ? 4 + 3 * m->nmag : 0));
+    const oreal *pd_weight = pd_value + num_weights;
     const int nout = (kind == 0 && !m->no_fq) ? 2 : 1;
     const int oriented2d = (kind != 0) && m->oriented;
     const int abc = m->iqabc != 0;
     quad_prime(quad_n);
 
-    double acc[4];
+    oreal acc[4];
     for (int k = 0; k < 4; k++) acc[k] = pd_start == 0 ? 0.0 : result[nout * nq + k];
-    if (pd_start == 0) memset(result, 0, sizeof(double) * nout * nq);
+    if (pd_start == 0) memset(result, 0, sizeof(oreal) * nout * nq);
 
-    double base[8];
+    oreal base[8];
     for (int i = 0; i < np; i++) base[i] = values[2 + i];
-    double view_theta = 0, view_phi = 0, view_psi = 0;
+    oreal view_theta = 0, view_phi = 0, view_psi = 0;
     if (oriented2d) {
         view_theta = values[theta_par + 2];
         view_phi = values[theta_par + 3];
@@ -948,7 +955,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
         }
     }
 
-    double xsw[6] = {0, 0, 0, 0, 0, 0}, smt = 0, cmt = 1, smp = 0, cmp = 1;
+    oreal xsw[6] = {0, 0, 0, 0, 0, 0}, smt = 0, cmt = 1, smp = 0, cmp = 1;
     if (kind == 2) {
         spin_weights(values[np + 2], values[np + 3], xsw);
         smt = sin(values[np + 4] * O_PI_180); cmt = cos(values[np + 4] * O_PI_180);
@@ -958,29 +965,29 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
     /* Pass 1: walk the mesh in order, keep the accepted points. */
     const int npts = pd_stop > pd_start ? pd_stop - pd_start : 0;
     const int rw = np + 7;   /* parameters, weight, rotation */
-    double *pts = (double *)malloc(sizeof(double) * (size_t)(npts ? npts : 1) * rw);
+    oreal *pts = (oreal *)malloc(sizeof(oreal) * (size_t)(npts ? npts : 1) * rw);
     int nacc = 0;
     for (int step = pd_start; step < pd_stop; step++) {
-        double p[8];
-        memcpy(p, base, sizeof(double) * np);
-        double w0 = 1.0;
+        oreal p[8];
+        memcpy(p, base, sizeof(oreal) * np);
+        oreal w0 = 1.0;
         for (int k = maxpd - 1; k >= 0; k--) {
             const int idx = (step / pd_stride[k]) % pd_len[k];
             p[pd_par[k]] = pd_value[pd_off[k] + idx];
             w0 = pd_weight[pd_off[k] + idx] * w0;
         }
         if (!m->valid(p)) continue;
-        double weight = w0;
-        double r31 = 0, r32 = 0;
+        oreal weight = w0;
+        oreal r31 = 0, r32 = 0;
         if (oriented2d) weight = fabs(cos(p[theta_par] * O_PI_180)) * w0;
         if (!(weight > cutoff)) continue;
-        const double form = m->vol(p);
+        const oreal form = m->vol(p);
         acc[0] += weight;
         acc[1] += weight * form;
         acc[2] += weight * (m->svol ? m->svol(p) : form);
         if (reff_mode != 0) acc[3] += weight * m->reff(reff_mode, p);
-        double *rec = pts + (size_t)nacc * rw;
-        memcpy(rec, p, sizeof(double) * np);
+        oreal *rec = pts + (size_t)nacc * rw;
+        memcpy(rec, p, sizeof(oreal) * np);
         rec[np] = weight;
         if (oriented2d && abc) {
             view_jitter_abc(view_theta, view_phi, view_psi, p[theta_par], p[theta_par + 1],
@@ -999,37 +1006,37 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
 #endif
     for (int iq = 0; iq < nq; iq++) {
         for (int a = 0; a < nacc; a++) {
-            const double *rec = pts + (size_t)a * rw;
-            double p[8];
-            memcpy(p, rec, sizeof(double) * np);
-            const double weight = rec[np];
+            const oreal *rec = pts + (size_t)a * rw;
+            oreal p[8];
+            memcpy(p, rec, sizeof(oreal) * np);
+            const oreal weight = rec[np];
             if (kind == 0) {
-                double f1, f2;
+                oreal f1, f2;
                 m->fq(q[iq], p, quad_n, &f1, &f2);
                 result[nout * iq + 0] += weight * f2;
                 if (nout == 2) result[2 * iq + 1] += weight * f1;
                 continue;
             }
-            const double qx = q[2 * iq], qy = q[2 * iq + 1];
-            double qab = 0, qc = 0, qa = 0, qb = 0;
+            const oreal qx = q[2 * iq], qy = q[2 * iq + 1];
+            oreal qab = 0, qc = 0, qa = 0, qb = 0;
             if (oriented2d && abc) {
                 /* qabc_apply (kernel_iq.c:298-310) */
                 qa = rec[np + 1] * qx + rec[np + 2] * qy;
                 qb = rec[np + 3] * qx + rec[np + 4] * qy;
                 qc = rec[np + 5] * qx + rec[np + 6] * qy;
             } else if (oriented2d) {
-                const double dqc = rec[np + 1] * qx + rec[np + 2] * qy;
-                const double dqab_sq = -dqc * dqc + qx * qx + qy * qy;
+                const oreal dqc = rec[np + 1] * qx + rec[np + 2] * qy;
+                const oreal dqab_sq = -dqc * dqc + qx * qx + qy * qy;
                 qab = dqab_sq > 0.0 ? sqrt(dqab_sq) : 0.0;
                 qc = dqc;
             }
-            double f2 = 0.0;
+            oreal f2 = 0.0;
             if (kind == 1) {
                 if (oriented2d && abc) f2 = m->iqabc(qa, qb, qc, p);
                 else if (oriented2d) f2 = m->iqac(qab, qc, p);
-                else { double f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &f2); }
+                else { oreal f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &f2); }
             } else {
-                const double qsq = qx * qx + qy * qy;
+                const oreal qsq = qx * qx + qy * qy;
                 if (qsq > 1.e-16) {
                     for (int xs = 0; xs < 6; xs++) {
                         if (!(xsw[xs] > 1.e-8)) continue;
@@ -1039,10 +1046,10 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
                             p[si] = magnetic_sld(xs, qx, qy, cmt, smt, cmp, smp, values[si + 2],
                                                  values[mi], values[mi + 1], values[mi + 2]);
                         }
-                        double fx;
+                        oreal fx;
                         if (oriented2d && abc) fx = m->iqabc(qa, qb, qc, p);
                         else if (oriented2d) fx = m->iqac(qab, qc, p);
-                        else { double f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &fx); }
+                        else { oreal f1; m->fq(sqrt(qx * qx + qy * qy), p, quad_n, &f1, &fx); }
                         f2 += xsw[xs] * fx;
                     }
                 }
@@ -1058,21 +1065,21 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
 /* ------------------------------------------------------------------ */
 /* Per-function entry points used by the tests                         */
 /* ------------------------------------------------------------------ */
-double sasoracle_j1(double x) { return o_j1(x); }
-double sasoracle_2j1x_x(double x) { return o_2j1x_x(x); }
-double sasoracle_sph_j1c(double x) { return o_sph_j1c(x); }
-double sasoracle_sinx_x(double x) { return o_sinx_x(x); }
+oreal sasoracle_j1(oreal x) { return o_j1(x); }
+oreal sasoracle_2j1x_x(oreal x) { return o_2j1x_x(x); }
+oreal sasoracle_sph_j1c(oreal x) { return o_sph_j1c(x); }
+oreal sasoracle_sinx_x(oreal x) { return o_sinx_x(x); }
 
-int sasoracle_gauss(int n, double *z, double *w)
+int sasoracle_gauss(int n, oreal *z, oreal *w)
 {
-    const double *zz, *ww;
+    const oreal *zz, *ww;
     if (gl_table(n, &zz, &ww) != n) return -1;
-    memcpy(z, zz, sizeof(double) * n);
-    memcpy(w, ww, sizeof(double) * n);
+    memcpy(z, zz, sizeof(oreal) * n);
+    memcpy(w, ww, sizeof(oreal) * n);
     return n;
 }
 
-int sasoracle_fq(int model_id, int quad_n, double q, const double *p, double *f1, double *f2)
+int sasoracle_fq(int model_id, int quad_n, oreal q, const oreal *p, oreal *f1, oreal *f2)
 {
     if (!model_ok(model_id)) return -1;
     quad_prime(quad_n);
@@ -1080,56 +1087,56 @@ int sasoracle_fq(int model_id, int quad_n, double q, const double *p, double *f1
     return 0;
 }
 
-double sasoracle_iqac(int model_id, double qab, double qc, const double *p)
+oreal sasoracle_iqac(int model_id, oreal qab, oreal qc, const oreal *p)
 {
     if (!model_ok(model_id) || !MODELS[model_id].iqac) return NAN;
     return MODELS[model_id].iqac(qab, qc, p);
 }
 
-double sasoracle_iqabc(int model_id, double qa, double qb, double qc, const double *p)
+oreal sasoracle_iqabc(int model_id, oreal qa, oreal qb, oreal qc, const oreal *p)
 {
     if (!model_ok(model_id) || !MODELS[model_id].iqabc) return NAN;
     return MODELS[model_id].iqabc(qa, qb, qc, p);
 }
 
-void sasoracle_view_jitter_abc(double theta, double phi, double psi, double dtheta, double dphi,
-                               double dpsi, double *r)
+void sasoracle_view_jitter_abc(oreal theta, oreal phi, oreal psi, oreal dtheta, oreal dphi,
+                               oreal dpsi, oreal *r)
 {
     view_jitter_abc(theta, phi, psi, dtheta, dphi, dpsi, r);
 }
 
-double sasoracle_shell_volume(int model_id, const double *p)
+oreal sasoracle_shell_volume(int model_id, const oreal *p)
 {
     if (!model_ok(model_id)) return NAN;
     return MODELS[model_id].svol ? MODELS[model_id].svol(p) : MODELS[model_id].vol(p);
 }
 
-double sasoracle_form_volume(int model_id, const double *p)
+oreal sasoracle_form_volume(int model_id, const oreal *p)
 {
     if (!model_ok(model_id)) return NAN;
     return MODELS[model_id].vol(p);
 }
 
-double sasoracle_radius_effective(int model_id, int mode, const double *p)
+oreal sasoracle_radius_effective(int model_id, int mode, const oreal *p)
 {
     if (!model_ok(model_id)) return NAN;
     return MODELS[model_id].reff(mode, p);
 }
 
-void sasoracle_view_jitter_row(double theta, double phi, double dtheta, double dphi,
-                               double *r31, double *r32)
+void sasoracle_view_jitter_row(oreal theta, oreal phi, oreal dtheta, oreal dphi,
+                               oreal *r31, oreal *r32)
 {
     view_jitter_row(theta, phi, dtheta, dphi, r31, r32);
 }
 
-double sasoracle_magnetic_sld(int xs, double qx, double qy, double mtheta_deg, double mphi_deg,
-                              double sld, double mx, double my, double mz)
+oreal sasoracle_magnetic_sld(int xs, oreal qx, oreal qy, oreal mtheta_deg, oreal mphi_deg,
+                              oreal sld, oreal mx, oreal my, oreal mz)
 {
     return magnetic_sld(xs, qx, qy, cos(mtheta_deg * O_PI_180), sin(mtheta_deg * O_PI_180),
                         cos(mphi_deg * O_PI_180), sin(mphi_deg * O_PI_180), sld, mx, my, mz);
 }
 
-void sasoracle_spin_weights(double in_spin, double out_spin, double *wt)
+void sasoracle_spin_weights(oreal in_spin, oreal out_spin, oreal *wt)
 {
     spin_weights(in_spin, out_spin, wt);
 }
@@ -1150,60 +1157,60 @@ void sasoracle_spin_weights(double in_spin, double out_spin, double *wt)
  * step in FP64: the perturbative Baxter solution (smaller root lambda of the
  * contact quadratic; -1 without a physical root) and the MSA square well.
  */
-double sasoracle_stickyhardsphere(double q, double radius, double phi, double perturb, double stickiness)
+oreal sasoracle_stickyhardsphere(oreal q, oreal radius, oreal phi, oreal perturb, oreal stickiness)
 {
-    const double onemineps = 1.0 - perturb;
-    const double eta = phi / onemineps / onemineps / onemineps;
-    const double sig = 2.0 * radius;
-    const double aa = sig / onemineps;
-    const double etam1 = 1.0 - eta;
-    const double etam1sq = etam1 * etam1;
-    const double qa = eta / 6.0;
-    const double qb = stickiness + eta / etam1;
-    const double qc = (1.0 + eta / 2.0) / etam1sq;
-    double radic = qb * qb - 2.0 * qa * qc;
+    const oreal onemineps = 1.0 - perturb;
+    const oreal eta = phi / onemineps / onemineps / onemineps;
+    const oreal sig = 2.0 * radius;
+    const oreal aa = sig / onemineps;
+    const oreal etam1 = 1.0 - eta;
+    const oreal etam1sq = etam1 * etam1;
+    const oreal qa = eta / 6.0;
+    const oreal qb = stickiness + eta / etam1;
+    const oreal qc = (1.0 + eta / 2.0) / etam1sq;
+    oreal radic = qb * qb - 2.0 * qa * qc;
     if (radic < 0) return -1.0;
     radic = sqrt(radic);
-    double lam = (qb - radic) / qa;
-    const double lam2 = (qb + radic) / qa;
+    oreal lam = (qb - radic) / qa;
+    const oreal lam2 = (qb + radic) / qa;
     if (lam2 < lam) lam = lam2;
-    const double test = 1.0 + 2.0 * eta;
-    const double mu = lam * eta * etam1;
+    const oreal test = 1.0 + 2.0 * eta;
+    const oreal mu = lam * eta * etam1;
     if (mu > test) return -1.0;
-    const double alpha = (1.0 + 2.0 * eta - mu) / etam1sq;
-    const double beta = (mu - 3.0 * eta) / (2.0 * etam1sq);
-    const double kk = q * aa, k2 = kk * kk, k3 = kk * k2;
-    const double ds = sin(kk), dc = cos(kk);
-    const double aq1 = ((ds - kk * dc) * alpha) / k3;
-    const double aq2 = (beta * (1.0 - dc)) / k2;
-    const double aq3 = (lam * ds) / (12.0 * kk);
-    const double aq = 1.0 + 12.0 * eta * (aq1 + aq2 - aq3);
-    const double bq1 = alpha * (0.5 / kk - ds / k2 + (1.0 - dc) / k3);
-    const double bq2 = beta * (1.0 / kk - ds / k2);
-    const double bq3 = (lam / 12.0) * ((1.0 - dc) / kk);
-    const double bq = 12.0 * eta * (bq1 + bq2 - bq3);
+    const oreal alpha = (1.0 + 2.0 * eta - mu) / etam1sq;
+    const oreal beta = (mu - 3.0 * eta) / (2.0 * etam1sq);
+    const oreal kk = q * aa, k2 = kk * kk, k3 = kk * k2;
+    const oreal ds = sin(kk), dc = cos(kk);
+    const oreal aq1 = ((ds - kk * dc) * alpha) / k3;
+    const oreal aq2 = (beta * (1.0 - dc)) / k2;
+    const oreal aq3 = (lam * ds) / (12.0 * kk);
+    const oreal aq = 1.0 + 12.0 * eta * (aq1 + aq2 - aq3);
+    const oreal bq1 = alpha * (0.5 / kk - ds / k2 + (1.0 - dc) / k3);
+    const oreal bq2 = beta * (1.0 / kk - ds / k2);
+    const oreal bq3 = (lam / 12.0) * ((1.0 - dc) / kk);
+    const oreal bq = 12.0 * eta * (bq1 + bq2 - bq3);
     return 1.0 / (aq * aq + bq * bq);
 }
 
-double sasoracle_squarewell(double q, double radius, double phi, double welldepth, double wellwidth)
+oreal sasoracle_squarewell(oreal q, oreal radius, oreal phi, oreal welldepth, oreal wellwidth)
 {
-    const double sigma = radius * 2.;
-    const double eta = phi, eta2 = eta * eta, eta3 = eta * eta2, eta4 = eta * eta3;
-    const double etam1 = 1. - eta;
-    const double etam14 = etam1 * etam1 * etam1 * etam1;
-    double sk = 1.0 + 2.0 * eta;
+    const oreal sigma = radius * 2.;
+    const oreal eta = phi, eta2 = eta * eta, eta3 = eta * eta2, eta4 = eta * eta3;
+    const oreal etam1 = 1. - eta;
+    const oreal etam14 = etam1 * etam1 * etam1 * etam1;
+    oreal sk = 1.0 + 2.0 * eta;
     sk *= sk;
-    const double alpha = (sk + eta3 * (eta - 4.0)) / etam14;
-    const double beta = -(eta / 3.0) * (18. + 20. * eta - 12. * eta2 + eta4) / etam14;
-    const double gamm = 0.5 * eta * (sk + eta3 * (eta - 4.)) / etam14;
+    const oreal alpha = (sk + eta3 * (eta - 4.0)) / etam14;
+    const oreal beta = -(eta / 3.0) * (18. + 20. * eta - 12. * eta2 + eta4) / etam14;
+    const oreal gamm = 0.5 * eta * (sk + eta3 * (eta - 4.)) / etam14;
     sk = q * sigma;
-    const double sk2 = sk * sk, sk3 = sk * sk2, sk4 = sk3 * sk;
-    const double S = sin(sk), C = cos(sk), SL = sin(wellwidth * sk), CL = cos(wellwidth * sk);
-    const double t1 = alpha * sk3 * (S - sk * C);
-    const double t2 = beta * sk2 * 2.0 * (sk * S - (0.5 * sk2 - 1.) * C - 1.0);
-    const double t3 = gamm * ((4.0 * sk3 - 24. * sk) * S - (sk4 - 12.0 * sk2 + 24.0) * C + 24.0);
-    const double t4 = -welldepth * sk3 * (SL + sk * (C - wellwidth * CL) - S);
-    const double ck = -24.0 * eta * (t1 + t2 + t3 + t4) / sk3 / sk3;
+    const oreal sk2 = sk * sk, sk3 = sk * sk2, sk4 = sk3 * sk;
+    const oreal S = sin(sk), C = cos(sk), SL = sin(wellwidth * sk), CL = cos(wellwidth * sk);
+    const oreal t1 = alpha * sk3 * (S - sk * C);
+    const oreal t2 = beta * sk2 * 2.0 * (sk * S - (0.5 * sk2 - 1.) * C - 1.0);
+    const oreal t3 = gamm * ((4.0 * sk3 - 24. * sk) * S - (sk4 - 12.0 * sk2 + 24.0) * C + 24.0);
+    const oreal t4 = -welldepth * sk3 * (SL + sk * (C - wellwidth * CL) - S);
+    const oreal ck = -24.0 * eta * (t1 + t2 + t3 + t4) / sk3 / sk3;
     return 1. / (1. - ck);
 }
 
@@ -1221,9 +1228,9 @@ double sasoracle_squarewell(double q, double radius, double phi, double welldept
  */
 
 typedef struct {
-    double eta0, gek0, ak0;          // volume fraction, contact potential / kT, kappa * sigma
-    double eta, gek, ak, s;          // in use (rescaled): s = (eta0 / eta)^(1/3)
-    double a, b, c, f, u, v, g1;     // MSA coefficients, g1 = g(1+)
+    oreal eta0, gek0, ak0;          // volume fraction, contact potential / kT, kappa * sigma
+    oreal eta, gek, ak, s;          // in use (rescaled): s = (eta0 / eta)^(1/3)
+    oreal a, b, c, f, u, v, g1;     // MSA coefficients, g1 = g(1+)
 } OMsa;
 
 // Coefficients at the trial volume fraction `et` (hayter_msa.c:179-467):
@@ -1232,41 +1239,41 @@ typedef struct {
 // residual of the other equation in `fval`, 3 the full MSA started near
 // the Gillan condition, 4 the Gillan-condition coefficients with g(1+) in
 // `fval`.  Returns the Newton iterations, or -2 when Newton fails.
-static int omsa_coefficients(OMsa *st, double et, int mode, double *fval)
+static int omsa_coefficients(OMsa *st, oreal et, int mode, oreal *fval)
 {
-    double a2 = 0.0, a3 = 0.0, b2 = 0.0, b3 = 0.0, v2 = 0.0, v3 = 0.0, p2 = 0.0, p3 = 0.0;
+    oreal a2 = 0.0, a3 = 0.0, b2 = 0.0, b3 = 0.0, v2 = 0.0, v3 = 0.0, p2 = 0.0, p3 = 0.0;
     int ir = 0;
-    const double eta2 = et * et, eta3 = eta2 * et;
-    const double e12 = 12.0 * et, e24 = e12 + e12;
+    const oreal eta2 = et * et, eta3 = eta2 * et;
+    const oreal e12 = 12.0 * et, e24 = e12 + e12;
     st->s = pow(st->eta0 / et, 1.0 / 3.0);
     st->ak = st->ak0 / st->s;
     const int big = st->ak > 15.0 && mode == 1;
     st->gek = st->gek0 * st->s * exp(st->ak0 - st->ak);
-    const double gek = st->gek, ak = st->ak;
-    const double ak2 = ak * ak, ak1 = 1.0 + ak, dak2 = 1.0 / ak2, dak4 = dak2 * dak2;
-    const double d = 1.0 - et, d2 = d * d, dak = d / ak, dd2 = 1.0 / d2, dd4 = dd2 * dd2;
-    const double dd45 = dd4 * 2.0e-1;
-    const double eta3d = 3.0 * et, eta6d = eta3d + eta3d, eta32 = eta3 + eta3;
-    const double eta2d = et + 2.0, eta2d2 = eta2d * eta2d, eta21 = 2.0 * et + 1.0, eta22 = eta21 * eta21;
+    const oreal gek = st->gek, ak = st->ak;
+    const oreal ak2 = ak * ak, ak1 = 1.0 + ak, dak2 = 1.0 / ak2, dak4 = dak2 * dak2;
+    const oreal d = 1.0 - et, d2 = d * d, dak = d / ak, dd2 = 1.0 / d2, dd4 = dd2 * dd2;
+    const oreal dd45 = dd4 * 2.0e-1;
+    const oreal eta3d = 3.0 * et, eta6d = eta3d + eta3d, eta32 = eta3 + eta3;
+    const oreal eta2d = et + 2.0, eta2d2 = eta2d * eta2d, eta21 = 2.0 * et + 1.0, eta22 = eta21 * eta21;
     // alpha, beta, nu, phi, tau of the paper
-    const double al1 = -eta21 * dak, al2 = (14.0 * eta2 - 4.0 * et - 1.0) * dak2, al3 = 36.0 * eta2 * dak4;
-    const double be1 = -(eta2 + 7.0 * et + 1.0) * dak, be2 = 9.0 * et * (eta2 + 4.0 * et - 2.0) * dak2;
-    const double be3 = 12.0 * et * (2.0 * eta2 + 8.0 * et - 1.0) * dak4;
-    const double vu1 = -(eta3 + 3.0 * eta2 + 45.0 * et + 5.0) * dak;
-    const double vu2 = (eta32 + 3.0 * eta2 + 42.0 * et - 2.0e1) * dak2;
-    const double vu3 = (eta32 + 3.0e1 * et - 5.0) * dak4;
-    const double vu4 = vu1 + e24 * ak * vu3, vu5 = eta6d * (vu2 + 4.0 * vu3);
-    const double ph1 = eta6d / ak, ph2 = d - e12 * dak2;
-    const double ta1 = (et + 5.0) / (5.0 * ak), ta2 = eta2d * dak2;
-    const double ta3 = -e12 * gek * (ta1 + ta2), ta4 = eta3d * ak2 * (ta1 * ta1 - ta2 * ta2);
-    const double ta5 = eta3d * (et + 8.0) * 1.0e-1 - 2.0 * eta22 * dak2;
+    const oreal al1 = -eta21 * dak, al2 = (14.0 * eta2 - 4.0 * et - 1.0) * dak2, al3 = 36.0 * eta2 * dak4;
+    const oreal be1 = -(eta2 + 7.0 * et + 1.0) * dak, be2 = 9.0 * et * (eta2 + 4.0 * et - 2.0) * dak2;
+    const oreal be3 = 12.0 * et * (2.0 * eta2 + 8.0 * et - 1.0) * dak4;
+    const oreal vu1 = -(eta3 + 3.0 * eta2 + 45.0 * et + 5.0) * dak;
+    const oreal vu2 = (eta32 + 3.0 * eta2 + 42.0 * et - 2.0e1) * dak2;
+    const oreal vu3 = (eta32 + 3.0e1 * et - 5.0) * dak4;
+    const oreal vu4 = vu1 + e24 * ak * vu3, vu5 = eta6d * (vu2 + 4.0 * vu3);
+    const oreal ph1 = eta6d / ak, ph2 = d - e12 * dak2;
+    const oreal ta1 = (et + 5.0) / (5.0 * ak), ta2 = eta2d * dak2;
+    const oreal ta3 = -e12 * gek * (ta1 + ta2), ta4 = eta3d * ak2 * (ta1 * ta1 - ta2 * ta2);
+    const oreal ta5 = eta3d * (et + 8.0) * 1.0e-1 - 2.0 * eta22 * dak2;
     // sinh k, cosh k (e^-k dropped beyond k = 20)
-    const double ex1 = exp(ak), ex2 = ak < 20.0 ? exp(-ak) : 0.0;
-    const double sk = 0.5 * (ex1 - ex2), ck = 0.5 * (ex1 + ex2);
-    const double ckma = ck - 1.0 - ak * sk, skma = sk - ak * ck;
-    const double a1 = (e24 * gek * (al1 + al2 + ak1 * al3) - eta22) * dd4;
-    const double b1 = (1.5 * et * eta2d2 - e12 * gek * (be1 + be2 + ak1 * be3)) * dd4;
-    const double v1 = (eta21 * (eta2 - 2.0 * et + 1.0e1) * 2.5e-1 - gek * (vu4 + vu5)) * dd45;
+    const oreal ex1 = exp(ak), ex2 = ak < 20.0 ? exp(-ak) : 0.0;
+    const oreal sk = 0.5 * (ex1 - ex2), ck = 0.5 * (ex1 + ex2);
+    const oreal ckma = ck - 1.0 - ak * sk, skma = sk - ak * ck;
+    const oreal a1 = (e24 * gek * (al1 + al2 + ak1 * al3) - eta22) * dd4;
+    const oreal b1 = (1.5 * et * eta2d2 - e12 * gek * (be1 + be2 + ak1 * be3)) * dd4;
+    const oreal v1 = (eta21 * (eta2 - 2.0 * et + 1.0e1) * 2.5e-1 - gek * (vu4 + vu5)) * dd45;
     if (!big) {
         a2 = e24 * (al3 * skma + al2 * sk - al1 * ck) * dd4;
         a3 = e24 * (eta22 * dak2 - 0.5 * d2 + al3 * ckma - al1 * sk + al2 * ck) * dd4;
@@ -1276,25 +1283,25 @@ static int omsa_coefficients(OMsa *st, double et, int mode, double *fval)
         v3 = ((eta3 - 6.0 * eta2 + 5.0) * d - eta6d * (2.0 * eta3 - 3.0 * eta2 + 18.0 * et + 1.0e1) * dak2 +
               e24 * vu3 + vu4 * sk - vu5 * ck) * dd45;
     }
-    const double pp1 = ph1 * ph1, pp2 = ph2 * ph2, pp = pp1 + pp2, p1p2 = ph1 * ph2 * 2.0;
-    const double p1 = (gek * (pp1 + pp2 - p1p2) - 0.5 * eta2d) * dd2;
+    const oreal pp1 = ph1 * ph1, pp2 = ph2 * ph2, pp = pp1 + pp2, p1p2 = ph1 * ph2 * 2.0;
+    const oreal p1 = (gek * (pp1 + pp2 - p1p2) - 0.5 * eta2d) * dd2;
     if (!big) {
         p2 = (pp * sk + p1p2 * ck) * dd2;
         p3 = (pp * ck + p1p2 * sk + pp1 - pp2) * dd2;
     }
-    const double t1 = ta3 + ta4 * a1 + ta5 * b1;
-    double fa, ca;
+    const oreal t1 = ta3 + ta4 * a1 + ta5 * b1;
+    oreal fa, ca;
     if (big) {
         // very large screening: the asymptotic solution
         v3 = ((eta3 - 6.0 * eta2 + 5.0) * d - eta6d * (2.0 * eta3 - 3.0 * eta2 + 18.0 * et + 1.0e1) * dak2 +
               e24 * vu3) * dd45;
-        const double t3 = ta4 * a3 + ta5 * b3 + e12 * ta2 - 4.0e-1 * et * (et + 1.0e1) - 1.0;
+        const oreal t3 = ta4 * a3 + ta5 * b3 + e12 * ta2 - 4.0e-1 * et * (et + 1.0e1) - 1.0;
         p3 = (pp1 - pp2) * dd2;
         b3 = e12 * (0.5 * d2 * eta2d - eta3d * eta2d2 * dak2 + be3) * dd4;
         a3 = e24 * (eta22 * dak2 - 0.5 * d2 - al3) * dd4;
-        const double um6 = t3 * a3 - e12 * v3 * v3, um5 = t1 * a3 + a1 * t3 - e24 * v1 * v3;
-        const double um4 = t1 * a1 - e12 * v1 * v1;
-        const double l6 = e12 * p3 * p3, l5 = e24 * p1 * p3 - b3 - b3 - ak2, l4 = e12 * p1 * p1 - b1 - b1;
+        const oreal um6 = t3 * a3 - e12 * v3 * v3, um5 = t1 * a3 + a1 * t3 - e24 * v1 * v3;
+        const oreal um4 = t1 * a1 - e12 * v1 * v1;
+        const oreal l6 = e12 * p3 * p3, l5 = e24 * p1 * p3 - b3 - b3 - ak2, l4 = e12 * p1 * p1 - b1 - b1;
         fa = -(um4 * l6 - l4 * um6) / (um5 * l6 - l5 * um6);
         ca = -fa;
         st->f = fa;
@@ -1306,53 +1313,53 @@ static int omsa_coefficients(OMsa *st, double et, int mode, double *fval)
         *fval = fabs(st->g1) < 1.0e-3 ? 0.0 : st->g1;
         st->eta = et;
     } else {
-        const double t2 = ta4 * a2 + ta5 * b2 + e12 * (ta1 * ck - ta2 * sk);
-        const double t3 = ta4 * a3 + ta5 * b3 + e12 * (ta1 * sk - ta2 * (ck - 1.0)) - 4.0e-1 * et * (et + 1.0e1) - 1.0;
+        const oreal t2 = ta4 * a2 + ta5 * b2 + e12 * (ta1 * ck - ta2 * sk);
+        const oreal t3 = ta4 * a3 + ta5 * b3 + e12 * (ta1 * sk - ta2 * (ck - 1.0)) - 4.0e-1 * et * (et + 1.0e1) - 1.0;
         // mu
-        const double um1 = t2 * a2 - e12 * v2 * v2, um2 = t1 * a2 + t2 * a1 - e24 * v1 * v2;
-        const double um3 = t2 * a3 + t3 * a2 - e24 * v2 * v3, um4 = t1 * a1 - e12 * v1 * v1;
-        const double um5 = t1 * a3 + t3 * a1 - e24 * v1 * v3, um6 = t3 * a3 - e12 * v3 * v3;
+        const oreal um1 = t2 * a2 - e12 * v2 * v2, um2 = t1 * a2 + t2 * a1 - e24 * v1 * v2;
+        const oreal um3 = t2 * a3 + t3 * a2 - e24 * v2 * v3, um4 = t1 * a1 - e12 * v1 * v1;
+        const oreal um5 = t1 * a3 + t3 * a1 - e24 * v1 * v3, um6 = t3 * a3 - e12 * v3 * v3;
         if (mode == 1 || mode == 3) {
             // the full MSA: lambda, omega and the quartic in f
-            const double l1 = e12 * p2 * p2, l2 = e24 * p1 * p2 - b2 - b2, l3 = e24 * p2 * p3;
-            const double l4 = e12 * p1 * p1 - b1 - b1, l5 = e24 * p1 * p3 - b3 - b3 - ak2, l6 = e12 * p3 * p3;
-            const double w16 = um1 * l6 - l1 * um6, w15 = um1 * l5 - l1 * um5, w14 = um1 * l4 - l1 * um4;
-            const double w13 = um1 * l3 - l1 * um3, w12 = um1 * l2 - l1 * um2;
-            const double w26 = um2 * l6 - l2 * um6, w25 = um2 * l5 - l2 * um5, w24 = um2 * l4 - l2 * um4;
-            const double w36 = um3 * l6 - l3 * um6, w35 = um3 * l5 - l3 * um5, w34 = um3 * l4 - l3 * um4;
-            const double w32 = um3 * l2 - l3 * um2;
-            const double w46 = um4 * l6 - l4 * um6, w56 = um5 * l6 - l5 * um6;
-            const double w3526 = w35 + w26, w3425 = w34 + w25;
-            const double q4 = w16 * w16 - w13 * w36;
-            const double q3 = 2.0 * w16 * w15 - w13 * w3526 - w12 * w36;
-            const double q2 = w15 * w15 + 2.0 * w16 * w14 - w13 * w3425 - w12 * w3526;
-            const double q1 = 2.0 * w15 * w14 - w13 * w24 - w12 * w3425;
-            const double q0 = w14 * w14 - w12 * w24;
-            double fap;
+            const oreal l1 = e12 * p2 * p2, l2 = e24 * p1 * p2 - b2 - b2, l3 = e24 * p2 * p3;
+            const oreal l4 = e12 * p1 * p1 - b1 - b1, l5 = e24 * p1 * p3 - b3 - b3 - ak2, l6 = e12 * p3 * p3;
+            const oreal w16 = um1 * l6 - l1 * um6, w15 = um1 * l5 - l1 * um5, w14 = um1 * l4 - l1 * um4;
+            const oreal w13 = um1 * l3 - l1 * um3, w12 = um1 * l2 - l1 * um2;
+            const oreal w26 = um2 * l6 - l2 * um6, w25 = um2 * l5 - l2 * um5, w24 = um2 * l4 - l2 * um4;
+            const oreal w36 = um3 * l6 - l3 * um6, w35 = um3 * l5 - l3 * um5, w34 = um3 * l4 - l3 * um4;
+            const oreal w32 = um3 * l2 - l3 * um2;
+            const oreal w46 = um4 * l6 - l4 * um6, w56 = um5 * l6 - l5 * um6;
+            const oreal w3526 = w35 + w26, w3425 = w34 + w25;
+            const oreal q4 = w16 * w16 - w13 * w36;
+            const oreal q3 = 2.0 * w16 * w15 - w13 * w3526 - w12 * w36;
+            const oreal q2 = w15 * w15 + 2.0 * w16 * w14 - w13 * w3425 - w12 * w3526;
+            const oreal q1 = 2.0 * w15 * w14 - w13 * w24 - w12 * w3425;
+            const oreal q0 = w14 * w14 - w12 * w24;
+            oreal fap;
             if (mode == 1) {
                 fap = (w14 - w34 - w46) / (w12 - w15 + w35 - w26 + w56 - w32);   // large k
             } else {
                 // near the Gillan condition; the Percus-Yevick-like estimate
                 // when both the contact potential and k are small
-                double g = 0.5 * eta2d * dd2 * exp(-gek);
+                oreal g = 0.5 * eta2d * dd2 * exp(-gek);
                 if (gek <= 2.0 && gek >= 0.0 && ak <= 1.0) {
-                    const double e24g = e24 * gek * exp(ak), pwk = sqrt(e24g);
-                    const double qpw = (1.0 - sqrt(1.0 + 2.0 * d2 * d * pwk / eta22)) * eta21 / d;
+                    const oreal e24g = e24 * gek * exp(ak), pwk = sqrt(e24g);
+                    const oreal qpw = (1.0 - sqrt(1.0 + 2.0 * d2 * d * pwk / eta22)) * eta21 / d;
                     g = -qpw * qpw / e24 + 0.5 * eta2d * dd2;
                 }
-                const double pg = p1 + g;
-                double cst = ak2 * pg + 2.0 * (b3 * pg - b1 * p3) + e12 * g * g * p3;
+                const oreal pg = p1 + g;
+                oreal cst = ak2 * pg + 2.0 * (b3 * pg - b1 * p3) + e12 * g * g * p3;
                 cst = -cst / (ak2 * p2 + 2.0 * (b3 * p2 - b2 * p3));
                 fap = -(pg + p2 * cst) / p3;
             }
             // Newton on the quartic
-            double del;
+            oreal del;
             int it = 0;
             do {
                 if (++it > 40) return -2;
                 fa = fap;
-                const double fun = q0 + (q1 + (q2 + (q3 + q4 * fa) * fa) * fa) * fa;
-                const double fund = q1 + (2.0 * q2 + (3.0 * q3 + 4.0 * q4 * fa) * fa) * fa;
+                const oreal fun = q0 + (q1 + (q2 + (q3 + q4 * fa) * fa) * fa) * fa;
+                const oreal fund = q1 + (2.0 * q2 + (3.0 * q3 + 4.0 * q4 * fa) * fa) * fa;
                 fap = fa - fun / fund;
                 del = fabs((fap - fa) / fa);
             } while (del > 1.0e-6);
@@ -1376,7 +1383,7 @@ static int omsa_coefficients(OMsa *st, double et, int mode, double *fval)
         st->a = a1 + a2 * ca + a3 * fa;
         st->v = (v1 + v2 * ca + v3 * fa) / st->a;
     }
-    const double g24 = e24 * gek * ex1;
+    const oreal g24 = e24 * gek * ex1;
     st->u = (ak * ak2 * ca - g24) / (ak2 * g24);
     return ir;
 }
@@ -1386,7 +1393,7 @@ static int omsa_coefficients(OMsa *st, double et, int mode, double *fval)
 // iteration) to meet the Gillan condition.  Negative on failure.
 static int omsa_solve(OMsa *st)
 {
-    double fval = 0.0;
+    oreal fval = 0.0;
     int ir = 0;
     int ig = 1;
     if (st->ak0 >= 1.0 + 8.0 * st->eta0) {
@@ -1395,17 +1402,17 @@ static int omsa_solve(OMsa *st)
         if (ir < 0 || fval >= 0.0) return ir;
         st->g1 = fval;
     }
-    double e = fmin(st->eta0, 0.20);
-    const double gamk = 2.0 * pow(st->eta0, 1.0 / 3.0) * st->gek0 * exp(st->ak0 - st->ak0 / pow(st->eta0, 1.0 / 3.0));
+    oreal e = fmin(st->eta0, 0.20);
+    const oreal gamk = 2.0 * pow(st->eta0, 1.0 / 3.0) * st->gek0 * exp(st->ak0 - st->ak0 / pow(st->eta0, 1.0 / 3.0));
     if (!ig || gamk >= 0.15) {
         int ii = 0;
-        double del, e2;
+        oreal del, e2;
         do {
             if (++ii > 40) return -1;
             if (e <= 0.0) e = st->eta0 / ii;
             if (e > 0.6) e = 0.35 / ii;
-            const double e1 = e;
-            double f1, f2;
+            const oreal e1 = e;
+            oreal f1, f2;
             omsa_coefficients(st, e1, 2, &f1);
             omsa_coefficients(st, e * 1.01, 2, &f2);
             e2 = e1 - (e * 1.01 - e1) * f1 / (f2 - f1);
@@ -1427,30 +1434,30 @@ static int omsa_solve(OMsa *st)
 // S(q) from solved coefficients at q sigma (hayter_msa.c:469-541): the
 // closed form of the Fourier transform of c(x), its Taylor expansion in
 // (q sigma / s)^2 below 0.01 and -1/a at zero.
-static double omsa_sq(const OMsa *st, double qsig)
+static oreal omsa_sq(const OMsa *st, oreal qsig)
 {
-    const double eta = st->eta, ak = st->ak, gek = st->gek;
-    const double e24 = 24.0 * eta;
-    const double x1 = exp(ak), x2 = ak < 20.0 ? exp(-ak) : 0.0;
-    const double ck = 0.5 * (x1 + x2), sk = 0.5 * (x1 - x2), ak2 = ak * ak;
-    const double qk = qsig / st->s, q2k = qk * qk;
+    const oreal eta = st->eta, ak = st->ak, gek = st->gek;
+    const oreal e24 = 24.0 * eta;
+    const oreal x1 = exp(ak), x2 = ak < 20.0 ? exp(-ak) : 0.0;
+    const oreal ck = 0.5 * (x1 + x2), sk = 0.5 * (x1 - x2), ak2 = ak * ak;
+    const oreal qk = qsig / st->s, q2k = qk * qk;
     if (qk <= 1.0e-08) return -1.0 / st->a;
     if (qk <= 0.01) {
-        const double aqk = st->a * (8.0 + 2.0 * eta) + 6 * st->b - 12.0 * st->f -
+        const oreal aqk = st->a * (8.0 + 2.0 * eta) + 6 * st->b - 12.0 * st->f -
                            24 * (gek * (1.0 + ak) - ck * ak * st->c + st->f * (ck - 1.0) + (st->c - st->f * ak) * sk) / ak2 +
                            q2k * (-(st->a * (48.0 + 15.0 * eta) + 40.0 * st->b) / 60.0 + st->f +
                                   (4.0 / ak2) * (gek * (9.0 + 7.0 * ak) + ck * (9.0 * st->f - 7.0 * st->c * ak) +
                                                  sk * (9.0 * st->c - 7.0 * st->f * ak)));
         return 1.0 / (1.0 - eta * aqk);
     }
-    const double qk2 = 1.0 / q2k, qk3 = qk2 / qk, qqk = 1.0 / (qk * (q2k + ak2));
-    double sink, cosk;
+    const oreal qk2 = 1.0 / q2k, qk3 = qk2 / qk, qqk = 1.0 / (qk * (q2k + ak2));
+    oreal sink, cosk;
     sink = sin(qk);
     cosk = cos(qk);
-    const double asink = ak * sink, qcosk = qk * cosk;
-    double aqk = st->a * (sink - qcosk);
+    const oreal asink = ak * sink, qcosk = qk * cosk;
+    oreal aqk = st->a * (sink - qcosk);
     aqk = aqk + st->b * ((2.0 * qk2 - 1.0) * qcosk + 2.0 * sink - 2.0 / qk);
-    const double inter = 24.0 * qk3 + 4.0 * (1.0 - 6.0 * qk2) * sink;
+    const oreal inter = 24.0 * qk3 + 4.0 * (1.0 - 6.0 * qk2) * sink;
     aqk = (aqk + 0.5 * eta * st->a * (inter - (1.0 - 12.0 * qk2 + 24.0 * qk2 * qk2) * qcosk)) * qk3;
     aqk = aqk + st->c * (ck * asink - sk * qcosk) * qqk;
     aqk = aqk + st->f * (sk * asink - qk * (ck * cosk - 1.0)) * qqk;
@@ -1466,20 +1473,20 @@ static double omsa_sq(const OMsa *st, double qsig)
 // monovalent counterions of the spheres and the salt; the Debye screening
 // constant and the contact potential follow in SI units with the constants
 // of the reference.  NaN when the coefficients do not converge.
-double sasoracle_hayter_msa(double q, double radius, double phi, double charge, double temperature,
-                             double csalt, double dielec)
+oreal sasoracle_hayter_msa(oreal q, oreal radius, oreal phi, oreal charge, oreal temperature,
+                             oreal csalt, oreal dielec)
 {
-    const double e_charge = 1.602189e-19, k_b = 1.380662e-23, eps0 = 8.85418782E-12;
-    const double diam = 2.0 * radius;
-    const double beta = 1.0 / (k_b * temperature);
-    const double perm = dielec * eps0;
-    const double zc = charge * e_charge;
-    const double sidiam = diam * 1.0E-10;
-    const double half = sidiam / 2.0;
-    const double vp = 4.18879020478639 * (half * half * half);   // 4 pi / 3 as the reference's header rounds it
-    const double cs = csalt * 6.022E23 * 1.0E3;
-    const double ionst = 0.5 * e_charge * e_charge * (charge * phi / vp + 2.0 * cs);
-    const double kappa = sqrt(2 * beta * ionst / perm);
+    const oreal e_charge = 1.602189e-19, k_b = 1.380662e-23, eps0 = 8.85418782E-12;
+    const oreal diam = 2.0 * radius;
+    const oreal beta = 1.0 / (k_b * temperature);
+    const oreal perm = dielec * eps0;
+    const oreal zc = charge * e_charge;
+    const oreal sidiam = diam * 1.0E-10;
+    const oreal half = sidiam / 2.0;
+    const oreal vp = 4.18879020478639 * (half * half * half);   // 4 pi / 3 as the reference's header rounds it
+    const oreal cs = csalt * 6.022E23 * 1.0E3;
+    const oreal ionst = 0.5 * e_charge * e_charge * (charge * phi / vp + 2.0 * cs);
+    const oreal kappa = sqrt(2 * beta * ionst / perm);
     OMsa st_, *st = &st_;
     st->gek0 = beta * zc * zc / (3.141592653589793 * perm * sidiam * ((2.0 + kappa * sidiam) * (2.0 + kappa * sidiam)));
     st->ak0 = kappa * sidiam;
@@ -1490,27 +1497,27 @@ double sasoracle_hayter_msa(double q, double radius, double phi, double charge, 
 }
 
 
-double sasoracle_hardsphere(double q, double radius, double phi)
+oreal sasoracle_hardsphere(oreal q, oreal radius, oreal phi)
 {
     if (fabs(radius) < 1e-12) return 1.0;
-    const double inv = 1.0 / (1.0 - phi);
-    const double d = inv * inv;
-    const double a = sq((1.0 + 2.0 * phi) * d);
-    const double x = fabs(2.0 * q * radius);
+    const oreal inv = 1.0 / (1.0 - phi);
+    const oreal d = inv * inv;
+    const oreal a = sq((1.0 + 2.0 * phi) * d);
+    const oreal x = fabs(2.0 * q * radius);
     if (x < 5e-6) return 1.0 / a;
-    const double b = -6.0 * phi * sq((1.0 + 0.5 * phi) * d);
-    const double g = 0.5 * phi * a;
-    const double x2 = x * x;
+    const oreal b = -6.0 * phi * sq((1.0 + 0.5 * phi) * d);
+    const oreal g = 0.5 * phi * a;
+    const oreal x2 = x * x;
     if (x < 0.05) {
-        const double c0 = 8.0 * a + 6.0 * b + 4.0 * g;
-        const double c2 = -0.8 * a - b / 1.5 - 0.5 * g;
-        const double c4 = a / 35.0 + 0.0125 * b + 0.02 * g;
+        const oreal c0 = 8.0 * a + 6.0 * b + 4.0 * g;
+        const oreal c2 = -0.8 * a - b / 1.5 - 0.5 * g;
+        const oreal c4 = a / 35.0 + 0.0125 * b + 0.02 * g;
         return 1.0 / (1.0 + phi * (c0 + (c2 + c4 * x2) * x2));
     }
-    const double s = sin(x), c = cos(x), x4 = x2 * x2;
-    const double ta = a * (s - x * c);
-    const double tb = b * (2.0 * x * s - (x2 - 2.0) * c - 2.0);
-    const double tg = g * ((4.0 * x2 - 24.0) * x * s - (x4 - 12.0 * x2 + 24.0) * c + 24.0);
-    const double ff = ((tg / x2 + tb) / x + ta) / x;
+    const oreal s = sin(x), c = cos(x), x4 = x2 * x2;
+    const oreal ta = a * (s - x * c);
+    const oreal tb = b * (2.0 * x * s - (x2 - 2.0) * c - 2.0);
+    const oreal tg = g * ((4.0 * x2 - 24.0) * x * s - (x4 - 12.0 * x2 + 24.0) * c + 24.0);
+    const oreal ff = ((tg / x2 + tb) / x + ta) / x;
     return 1.0 / (1.0 + 24.0 * phi * ff / x2);
 }
