@@ -461,6 +461,12 @@ def bench_surrogate(steps=20, ncurves=4096, nq=512, threshold=1e-3, judge_points
             "curves_per_s_inference_only": ncurves/t_inf, "ms_inference": t_inf*1e3,
             "ms_judge": t_judge*1e3, "ms_plain_whole_batch": t_plain*1e3,
             "fallback_frac": float(reject.mean()), "judge_points": int(jidx.size), "threshold": threshold,
+            "served_at_threshold": bool(sur.serves(threshold)),
+            "policy": "CylinderSurrogate.evaluate serves threshold %g with these weights" % threshold
+                      if sur.serves(threshold) else
+                      "CylinderSurrogate.evaluate sends every request at threshold %g to the plain kernel "
+                      "(validation record below %g of curves): the path runs at the plain rate; the gated "
+                      "figures are what serving would cost" % (threshold, 0.5),
             "plain_curves_per_s_same_curves": ncurves/t_plain,
             "speedup_vs_plain_gated": t_plain/t_gated,
             "frac_curves_within_1e-3_ungated": float(np.mean(err <= 1e-3)),

@@ -31,7 +31,9 @@ when the relative error there exceeds the threshold.
 """
 
 import ctypes as ct
+import json
 import math
+import os
 
 import numpy as np
 
@@ -51,6 +53,8 @@ BOX = {
 LOG10Q = (-3.0, 0.0)
 PD_N = 35
 QUAD = 76
+#: least fraction of held-out curves within a threshold for the weights to serve it
+MIN_SERVED = 0.5
 
 
 def _torch():
@@ -178,6 +182,7 @@ class CylinderSurrogate:
         self.norm = torch.zeros(12, dtype=torch.float32, device="cuda")
         self._work = None
         self._work_batch = 0
+        self.validation = None
         self.init_params(seed)
         self.plain = PlainCylinder()
 
@@ -310,17 +315,37 @@ class CylinderSurrogate:
             int(logq_t.shape[0]), ct.c_void_p(self.norm.data_ptr()), 1 if out_iq else 0,
             ct.c_void_p(out_t.data_ptr()), int(grid), _stream(self.torch)))
 
+    def serves(self, threshold):
+        """
+        Whether the loaded weights take requests at relative-error
+        *threshold*: their validation record (the JSON written beside the
+        weights by train_surrogate) must show at least MIN_SERVED of the
+        held-out curves within it.  Below that most curves would be computed
+        twice (inference, then the plain kernel), which is slower than the
+        plain kernel alone.  Weights without a record (trained in this
+        process) are served.
+        """
+        if not self.validation:
+            return True
+        key = "val_frac_below_1e3" if threshold < 1e-2 else "val_frac_below_1e2"
+        return float(self.validation.get(key, 0.0)) >= MIN_SERVED
+
     def evaluate(self, curves, log10q, threshold=1e-3, judge_points=8, batch=None):
         """
         I(q) for every curve with the plain-kernel gate: (Iq [n, nq], used_fallback [n],
         judged_err [n]).  Curves outside the box, or whose relative error at
-        `judge_points` log-spaced q exceeds `threshold`, come from the plain kernel.
+        `judge_points` log-spaced q exceeds `threshold`, come from the plain kernel;
+        when the weights do not serve *threshold* (:meth:`serves`) every curve
+        does, without inference (judged_err is then NaN).
         """
         curves = np.asarray(curves, np.float64)
         nq = len(log10q)
+        q = 10.0**np.asarray(log10q, np.float64)
+        if not self.serves(threshold):
+            I = np.array(self.plain.iq(curves, [q]*len(curves))).reshape(len(curves), nq)
+            return I, np.ones(len(curves), bool), np.full(len(curves), np.nan)
         I = self.predict(curves, log10q).reshape(len(curves), nq).double().cpu().numpy()
         jidx = np.unique(np.linspace(0, nq - 1, judge_points).round().astype(int))
-        q = 10.0**np.asarray(log10q, np.float64)
         judge = np.array(self.plain.iq(curves, [q[jidx]]*len(curves)))
         err = np.max(np.abs(I[:, jidx] - judge)/np.abs(judge), axis=1)
         bad = (err > threshold) | ~in_box(curves, log10q)
@@ -333,6 +358,12 @@ class CylinderSurrogate:
         np.savez(path, theta=self.theta.cpu().numpy(), norm=self.norm.cpu().numpy())
 
     def load(self, path):
+        """Weights from *path* (.npz) and their validation record (the .json beside it, if any)."""
         d = np.load(path)
         self.theta = self.torch.as_tensor(d["theta"], device="cuda").contiguous()
         self.norm.copy_(self.torch.as_tensor(d["norm"]))
+        record = os.path.splitext(path)[0] + ".json"
+        self.validation = None
+        if os.path.exists(record):
+            with open(record) as f:
+                self.validation = json.load(f)

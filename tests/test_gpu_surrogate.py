@@ -138,3 +138,29 @@ def test_trained_network_gains_over_the_envelope():
     rms_network = float(np.sqrt(np.mean((r_net - r_true)**2)))
     print("RMS log I error: envelope alone %.4f, trained network %.4f" % (rms_envelope, rms_network))
     assert rms_network < 0.2*rms_envelope
+
+
+def test_weights_that_miss_the_threshold_route_to_the_plain_kernel():
+    """Weights whose validation record shows fewer than half the held-out
+    curves within a threshold do not serve it: evaluate() returns the plain
+    kernel's curves for every input without running inference."""
+    import os
+    from sasmodels_amd.surrogate import MIN_SERVED, CylinderSurrogate, sample_box
+    path = os.path.join(os.path.dirname(os.path.abspath(omlp.__file__)), "..", "sasmodels_amd", "data",
+                        "cylinder_surrogate.npz")
+    s = CylinderSurrogate(seed=0)
+    s.load(path)
+    assert s.validation is not None
+    served = s.validation["val_frac_below_1e3"] >= MIN_SERVED
+    assert s.serves(1e-3) == served
+    curves = sample_box(5, np.random.default_rng(4))
+    lq = np.linspace(-3, 0, 40)
+    I, bad, err = s.evaluate(curves, lq, threshold=1e-3)
+    if not served:
+        assert bad.all() and np.isnan(err).all()
+        plain = np.array(s.plain.iq(curves, [10.0**lq]*5))
+        np.testing.assert_allclose(I, plain, rtol=1e-12)
+    s.validation = None          # weights without a record are served and judged
+    assert s.serves(1e-3)
+    I2, bad2, err2 = s.evaluate(curves, lq, threshold=1e-3)
+    assert np.isfinite(err2).all()
