@@ -367,25 +367,27 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_fq1d(
 }
 
 // ------------------------------------------------------------------ 2D, plain and magnetic
-// Effective SLD for spin cross section xs (kernel_iq.c:127-175): the
-// magnetisation component perpendicular to q, projected on the polarisation
-// (non-spin-flip) or on the two axes normal to it (spin-flip real/imag).
-__device__ __forceinline__ double mag_sld(int xs, double qx, double qy, double cmt, double smt,
-                                          double cmp, double smp, double sld,
-                                          double mx, double my, double mz)
+// Effective SLDs of the six spin cross sections (kernel_iq.c:127-175) for
+// one magnetised SLD at one q: the magnetisation component perpendicular to
+// q, projected on the polarisation (non-spin-flip: sld -+ P.Mperp) or on
+// the two axes normal to it (spin-flip, real and imaginary).  ux, uy is the
+// unit vector of q and uu its squared norm (which the reference divides by);
+// both depend on q only, the projections on the SLD only, so a lane works
+// them out once instead of once per cross section.
+__device__ __forceinline__ void mag_slds(double ux, double uy, double uu, double cmt, double smt, double cmp,
+                                         double smp, double sld, double mx, double my, double mz, double *eff)
 {
-    const double qn = sqrt(qx * qx + qy * qy);
-    const double ux = qx / qn, uy = qy / qn;
-    const double k = (mx * ux + my * uy) / (ux * ux + uy * uy);
+    const double k = (mx * ux + my * uy) / uu;
     const double px = mx - k * ux, py = my - k * uy, pz = mz;
-    switch (xs) {
-    case 0: return sld - (smt * cmp * px + smt * smp * py + cmt * pz);
-    case 1:
-    case 2: return -smp * px + cmp * py;
-    case 3: return sld + (smt * cmp * px + smt * smp * py + cmt * pz);
-    case 4: return -(-cmt * cmp * px - cmt * smp * py + smt * pz);
-    default: return -cmt * cmp * px - cmt * smp * py + smt * pz;
-    }
+    const double pol = smt * cmp * px + smt * smp * py + cmt * pz;
+    const double flip = -smp * px + cmp * py;
+    const double perp = -cmt * cmp * px - cmt * smp * py + smt * pz;
+    eff[0] = sld - pol;
+    eff[1] = flip;
+    eff[2] = flip;
+    eff[3] = sld + pol;
+    eff[4] = -perp;
+    eff[5] = perp;
 }
 
 // Spin cross-section weights dd, du.re, ud.re, uu, du.im, ud.im (kernel_iq.c:84-124)
@@ -429,17 +431,23 @@ struct Contrast {
             sincos(values[5 + NP] * DEG, &smp, &cmp);
             const bool q_ok = qx * qx + qy * qy > 1.e-16;   // kernel_iq.c:754
             on = q_ok;
+            const double qn = q_ok ? sqrt(qx * qx + qy * qy) : 1.0;
+            const double ux = qx / qn, uy = qy / qn;
+            const double uu = q_ok ? ux * ux + uy * uy : 1.0;
+            double eff[NX][M::NMAG];
+#pragma unroll
+            for (int k = 0; k < M::NMAG; k++) {
+                const double *mv = values + 2 + NP + 4 + 3 * k;
+                double e[6];
+                mag_slds(ux, uy, uu, cmt, smt, cmp, smp, sld[k], mv[0], mv[1], mv[2], e);
+#pragma unroll
+                for (int x = 0; x < NX; x++) eff[x][k] = q_ok ? e[x] : 0.0;
+            }
 #pragma unroll
             for (int x = 0; x < NX; x++) {
                 active[x] = w[x] > 1.e-8;                    // kernel_iq.c:761
                 xsw[x] = (active[x] && q_ok) ? (float)w[x] : 0.0f;
-                double eff[M::NMAG];
-#pragma unroll
-                for (int k = 0; k < M::NMAG; k++) {
-                    const double *mv = values + 2 + NP + 4 + 3 * k;
-                    eff[k] = q_ok ? mag_sld(x, qx, qy, cmt, smt, cmp, smp, sld[k], mv[0], mv[1], mv[2]) : 0.0;
-                }
-                M::contrasts(eff, con[x]);
+                M::contrasts(eff[x], con[x]);
             }
         }
     }
