@@ -156,6 +156,18 @@ int saship_kernel_call_noalloc(saship_model *model, int32_t kind, int32_t nq, co
                                double cutoff, int32_t radius_effective_mode,
                                int32_t pd_start, int32_t pd_stop, double *result, void *stream);
 
+/* Diagnostic, no reference counterpart: one parameter set of structure
+ * factor s_model (SASHIP_HARDSPHERE, SASHIP_STICKY_HARDSPHERE,
+ * SASHIP_SQUAREWELL, SASHIP_HAYTER_MSA; pars in the order of its kernel
+ * call, unused trailing entries ignored): the q-independent coefficients
+ * the batched product path solves once per set (prep, 8 doubles: for the
+ * Hayter MSA a, b, c, f, eta, gek, k, s with a = NaN where the coefficient
+ * iteration fails; zeros for the closed forms) and S at nq q points (sq),
+ * computed on the host (on_device == 0) or on the current device by the
+ * same code.  q, prep and sq are host memory. */
+int saship_structure_eval(int32_t s_model, const double *pars, int32_t nq, const double *q,
+                          int32_t on_device, double *prep, double *sq);
+
 /* Evaluation strategy: with separable != 0 (the default) a cylinder 1D call
  * with a fixed Gauss rule and a mesh over radius and length only is
  * evaluated as bilinear forms of the radius and length factors (same sums,

@@ -35,6 +35,8 @@ SIGNATURES = {
                                               ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
                                               ct.c_double, ct.c_int32, ct.c_int32, ct.c_int32,
                                               ct.c_void_p, ct.c_void_p]),
+    "saship_structure_eval": (ct.c_int, [ct.c_int32, ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
+                                         ct.c_void_p, ct.c_void_p]),
     "saship_probe_fma": (ct.c_int, [ct.c_int32, ct.POINTER(ct.c_double)]),
     "saship_batch_replay": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_double)]),
     "saship_batch_set_resolution": (ct.c_int, [ct.c_void_p]*7 + [ct.c_int64]),

@@ -350,6 +350,23 @@ bool device_accessible(const void *p)
 }
 }  // namespace
 
+namespace sas {
+// Diagnostic: one structure factor's coefficients (thread 0) and S at every q.
+__global__ __launch_bounds__(256) void k_structure_eval(int model, const double *__restrict__ pars, int nq,
+                                                        const double *__restrict__ q, double *__restrict__ prep,
+                                                        double *__restrict__ sq)
+{
+    __shared__ double pr[S_PREP];
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < S_PREP; k++) pr[k] = 0.0;
+        structure_prepare(model, pars, pr);
+        for (int k = 0; k < S_PREP; k++) prep[k] = pr[k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nq; i += 256) sq[i] = structure_sq_prepared(model, q[i], pars, pr);
+}
+}  // namespace sas
+
 namespace {
 // Download the result of the evaluation just enqueued and account its time.
 int finish_call(saship_kernel *k, size_t rbytes, double *result)
@@ -717,6 +734,38 @@ int saship_kernel_call_noalloc(saship_model *m, int32_t kind, int32_t nq, const 
     if (path == 0) return launch_structure(t, d_val, cutoff, accumulate);
     if (path == 1) return launch_sep(t, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
     return launch_model(t, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, nsplit);
+}
+
+int saship_structure_eval(int32_t s_model, const double *pars, int32_t nq, const double *q, int32_t on_device,
+                          double *prep, double *sq)
+{
+    if (!pars || !prep || (nq > 0 && (!q || !sq))) return fail(SASHIP_EINVAL, "NULL argument");
+    if (!model_structure(s_model)) return fail(SASHIP_EINVAL, "model " + std::to_string(s_model) + " is not a structure factor");
+    if (nq < 0) return fail(SASHIP_EINVAL, "nq must be >= 0");
+    double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < model_npars(s_model); k++) p[k] = pars[k];
+    const int id = s_model;     // the structure-factor ids are the model ids
+    if (!on_device) {
+        for (int k = 0; k < sas::S_PREP; k++) prep[k] = 0.0;
+        sas::structure_prepare(id, p, prep);
+        for (int i = 0; i < nq; i++) sq[i] = sas::structure_sq_prepared(id, q[i], p, prep);
+        return 0;
+    }
+    HIP_TRY(hipSetDevice(current_device()));
+    double *d = nullptr;
+    const size_t n = 6 + (size_t)sas::S_PREP + 2 * (size_t)nq;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(double) * n));
+    hipError_t e = hipMemcpy(d, p, sizeof(double) * 6, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nq > 0) e = hipMemcpy(d + 6 + sas::S_PREP, q, sizeof(double) * nq, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        sas::k_structure_eval<<<1, 256>>>(id, d, nq, d + 6 + sas::S_PREP, d + 6, d + 6 + sas::S_PREP + nq);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(prep, d + 6, sizeof(double) * sas::S_PREP, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nq > 0) e = hipMemcpy(sq, d + 6 + sas::S_PREP + nq, sizeof(double) * nq, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SASHIP_EHIP, std::string("structure eval: ") + hipGetErrorString(e));
+    return 0;
 }
 
 int saship_kernel_set_path(saship_kernel *k, int32_t separable)

