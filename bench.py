@@ -515,6 +515,8 @@ def bench_workload(wl, kernel_name, stem, fma_tflops, threads, cpu=True, reps=10
     calls = wl["calls"]
     b.autotune(calls, wl["cutoff"], magnetic=mag)
     dt, iq = _timed(lambda: b(calls, wl["cutoff"], magnetic=mag), max(2, reps//3))
+    buf = np.empty(int(b.qoff[-1]))
+    dt_out, _ = _timed(lambda: b(calls, wl["cutoff"], magnetic=mag, out=buf), max(2, reps//3))
     ms = kernel_ms(b, calls, max(2, reps//3), cutoff=wl["cutoff"], magnetic=mag)
     b(calls, wl["cutoff"], magnetic=mag)
     res = resident_s(b, reps)
@@ -523,6 +525,7 @@ def bench_workload(wl, kernel_name, stem, fma_tflops, threads, cpu=True, reps=10
     n = len(calls)
     work = [np.asarray(qv[0]).size*int(d.num_eval) for qv, (d, _) in zip(wl["q_list"], calls)]
     ent = {"workload": wl["workload"], "sets_per_s": n/res, "sets_per_s_host_inclusive": n/dt,
+           "sets_per_s_host_inclusive_reused_output": n/dt_out,
            "ms_per_launch": res*1e3, "eval_kernel_ms": ms, "qpd_evals_per_s": sum(work)/(ms*1e-3),
            "roofline": roofline(wl["name"], kernel_name, ms, stem=stem, fma_tflops=fma_tflops)}
     if cpu:

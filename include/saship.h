@@ -175,6 +175,14 @@ int saship_structure_eval(int32_t s_model, const double *pars, int32_t nq, const
  * No reference counterpart. */
 int saship_kernel_set_path(saship_kernel *kernel, int32_t separable);
 
+/* Graph replay (default off): the upload and kernels of a call
+ * are captured once as a hipGraph and replayed for later calls of the same
+ * shape (kind, path, mesh range, cutoff, buffer sizes); enable = 0 launches
+ * every call eagerly (measured faster for single calls, whose eager
+ * sequence is a few short kernels).  Timed calls are always eager.  No
+ * reference counterpart. */
+int saship_kernel_set_graph(saship_kernel *kernel, int32_t enable);
+
 /* Kernel timing: when enabled, the device time of the evaluation kernel of
  * every call is summed (HIP events on the handle's stream).  No reference
  * counterpart; used by the benchmark. */

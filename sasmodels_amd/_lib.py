@@ -43,6 +43,7 @@ SIGNATURES = {
     "saship_batch_set_structure": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32]),
     "saship_kernel_set_path": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_batch_set_path": (ct.c_int, [ct.c_void_p, ct.c_int32]),
+    "saship_kernel_set_graph": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_kernel_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "saship_kernel_get_timing": (ct.c_int, [ct.c_void_p, c_double_p, c_int32_p]),
     "saship_kernel_release": (ct.c_int, [ct.c_void_p]),

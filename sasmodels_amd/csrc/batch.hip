@@ -455,6 +455,10 @@ struct saship_batch {
     // timing of the evaluation kernel
     hipEvent_t ev0 = nullptr, ev1 = nullptr;     // around the evaluation kernel
     hipEvent_t ev2 = nullptr, ev3 = nullptr;     // around the whole device sequence
+    // large I(q) downloads go in chunks after the launch, each copied out
+    // to the caller while the next one is in flight
+    bool defer_iq = false;
+    std::vector<hipEvent_t> chunk_ev;
     bool timing = false;
     double t_ms = 0.0, t_all_ms = 0.0;
     int ncalls = 0;
@@ -731,7 +735,7 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     if (!copies) return 0;
     if (want_result)
         HIP_TRY(hipMemcpyAsync(b->h_out, d_result, sizeof(double) * p.result_total, hipMemcpyDeviceToHost, b->st));
-    if (want_iq)
+    if (want_iq && !b->defer_iq)
         HIP_TRY(hipMemcpyAsync(b->h_out + sizeof(double) * p.result_total, b->n_smear > 0 ? d_siq : d_iq,
                                sizeof(double) * (b->n_smear > 0 ? b->n_smear : p.nq_total),
                                hipMemcpyDeviceToHost, b->st));
@@ -790,10 +794,13 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     if (int e = plan_and_stage(b, kind, details, n_details, values, voff, p)) return e;
     if (int e = size_buffers(b, p)) return e;
     const bool want_result = result != nullptr, want_iq = iq != nullptr;
+    const size_t iq_bytes = want_iq ? sizeof(double) * (size_t)(b->n_smear > 0 ? b->n_smear : p.nq_total) : 0;
+    constexpr size_t DEFER_IQ_FROM = (size_t)8 << 20, IQ_CHUNK = (size_t)4 << 20;
+    b->defer_iq = iq_bytes >= DEFER_IQ_FROM;
     if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
-                                    p.sep_nlp, b->n_smear, b->s_model};
+                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
@@ -816,6 +823,26 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     } else {
         if (int e = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq)) return e;
     }
+    const size_t nchunk = b->defer_iq ? (iq_bytes + IQ_CHUNK - 1) / IQ_CHUNK : 0;
+    if (nchunk) {
+        const char *src = (const char *)(b->d_out + p.result_total + (b->n_smear > 0 ? p.nq_total : 0));
+        char *stage = b->h_out + sizeof(double) * p.result_total;
+        while (b->chunk_ev.size() < nchunk) {
+            hipEvent_t ev = nullptr;
+            HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            b->chunk_ev.push_back(ev);
+        }
+        for (size_t c = 0; c < nchunk; c++) {
+            const size_t off = c * IQ_CHUNK, len = std::min(IQ_CHUNK, iq_bytes - off);
+            HIP_TRY(hipMemcpyAsync(stage + off, src + off, len, hipMemcpyDeviceToHost, b->st));
+            HIP_TRY(hipEventRecord(b->chunk_ev[c], b->st));
+        }
+        for (size_t c = 0; c < nchunk; c++) {
+            const size_t off = c * IQ_CHUNK, len = std::min(IQ_CHUNK, iq_bytes - off);
+            HIP_TRY(hipEventSynchronize(b->chunk_ev[c]));
+            copy_out((char *)iq + off, stage + off, len);
+        }
+    }
     HIP_TRY(hipStreamSynchronize(b->st));
     b->last = p;
     b->last_cutoff = cutoff;
@@ -823,10 +850,8 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     b->last_result = want_result;
     b->last_iq = want_iq;
     b->have_last = true;
-    if (want_result) memcpy(result, b->h_out, sizeof(double) * p.result_total);
-    if (want_iq)
-        memcpy(iq, b->h_out + sizeof(double) * p.result_total,
-               sizeof(double) * (b->n_smear > 0 ? b->n_smear : p.nq_total));
+    if (want_result) copy_out(result, b->h_out, sizeof(double) * p.result_total);
+    if (want_iq && !nchunk) copy_out(iq, b->h_out + sizeof(double) * p.result_total, iq_bytes);
     if (b->timing) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -1154,6 +1179,7 @@ int saship_batch_release(saship_batch *b)
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->ev2) (void)hipEventDestroy(b->ev2);
     if (b->ev3) (void)hipEventDestroy(b->ev3);
+    for (hipEvent_t ev : b->chunk_ev) (void)hipEventDestroy(ev);
     if (b->st) (void)hipStreamDestroy(b->st);
     if (b->m) model_drop(b->m);
     delete b;

@@ -6,7 +6,11 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/saship.h"
@@ -204,4 +208,93 @@ inline void model_drop(saship_model *m)
 {
     if (m->refs.fetch_sub(1) == 1 && m->released.load()) model_free(m);
 }
+// Host copy of a downloaded result into the caller's buffer: large copies
+// (a batch of 2D images is tens of MB) are split over a small pool of
+// worker threads that lives as long as the library, which also spreads the
+// page faults of a freshly allocated destination.  One copy at a time goes
+// through the pool; smaller ones are a plain memcpy.
+class CopyPool {
+public:
+    static CopyPool &get()
+    {
+        static CopyPool pool;
+        return pool;
+    }
+    void copy(void *dst, const void *src, size_t bytes)
+    {
+        std::lock_guard<std::mutex> one(submit_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = (char *)dst;
+            src_ = (const char *)src;
+            bytes_ = bytes;
+            part_ = ((bytes / (workers_.size() + 1)) + 4095) & ~(size_t)4095;
+            pending_ = (int)workers_.size();
+            ++generation_;
+        }
+        cv_.notify_all();
+        memcpy(dst, src, std::min(part_, bytes));        // the caller's share: piece 0
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+
+private:
+    CopyPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned n = std::min(7u, hw > 1 ? hw - 1 : 1u);
+        for (unsigned i = 0; i < n; i++) workers_.emplace_back([this, i] { run(i + 1); });
+    }
+    void run(unsigned piece)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            char *dst;
+            const char *src;
+            size_t bytes, part;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || generation_ != seen; });
+                if (stop_) return;
+                seen = generation_;
+                dst = dst_;
+                src = src_;
+                bytes = bytes_;
+                part = part_;
+            }
+            const size_t off = part * piece;
+            if (off < bytes) memcpy(dst + off, src + off, std::min(part, bytes - off));
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (--pending_ == 0) done_.notify_one();
+            }
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_, submit_;
+    std::condition_variable cv_, done_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t bytes_ = 0, part_ = 0;
+    int pending_ = 0;
+    uint64_t generation_ = 0;
+    bool stop_ = false;
+};
+
+inline void copy_out(void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t PARALLEL_FROM = (size_t)2 << 20;
+    if (bytes < PARALLEL_FROM) memcpy(dst, src, bytes);
+    else CopyPool::get().copy(dst, src, bytes);
+}
+
 }  // namespace saship_host

@@ -100,16 +100,21 @@ struct saship_kernel {
     char *d_meta = nullptr;          // npts (int) at 0, sums (4 doubles) at 64
     double *d_partial = nullptr;
     size_t partial_cap = 0;
-    double *d_result = nullptr;
     char *d_sep = nullptr;          // separable cylinder tables (R, H, A, B, dims)
     char *h_stage = nullptr;
     size_t stage_cap = 0;
-    double *h_result = nullptr;
+    double *h_result = nullptr;      // pinned, mapped: the final kernel writes the result here
+    double *hd_result = nullptr;     // device address of h_result
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing = false;
     bool separable = true;           // use the separable cylinder path when it applies
     double t_ms = 0.0;
     int ncalls = 0;
+    // graph replay of repeated calls of one shape (upload, launches, download)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<int64_t> gkey;
     std::mutex mu;
 };
 
@@ -351,6 +356,25 @@ bool device_accessible(const void *p)
 }  // namespace
 
 namespace sas {
+// Small uploads (values and details of one call) as a kernel argument: a
+// one-workgroup copy into device memory instead of a DMA transfer, which
+// costs more latency than the few hundred bytes take to move.
+constexpr int STAGE_ARG_BYTES = 2048;
+struct StageIn {
+    int nbytes;
+    alignas(16) unsigned char data[STAGE_ARG_BYTES];
+};
+__global__ __launch_bounds__(128) void k_stage_in(StageIn s, unsigned char *__restrict__ dst)
+{
+    for (int i = threadIdx.x * 16; i < s.nbytes; i += 128 * 16) {
+        const int n = min(16, s.nbytes - i);
+        if (n == 16) *(uint4 *)(dst + i) = *(const uint4 *)(s.data + i);
+        else for (int k = 0; k < n; k++) dst[i + k] = s.data[i + k];
+    }
+}
+}  // namespace sas
+
+namespace sas {
 // Diagnostic: one structure factor's coefficients (thread 0) and S at every q.
 __global__ __launch_bounds__(256) void k_structure_eval(int model, const double *__restrict__ pars, int nq,
                                                         const double *__restrict__ q, double *__restrict__ prep,
@@ -368,10 +392,19 @@ __global__ __launch_bounds__(256) void k_structure_eval(int model, const double 
 }  // namespace sas
 
 namespace {
-// Download the result of the evaluation just enqueued and account its time.
+void drop_graph(saship_kernel *k)
+{
+    if (k->gexec) (void)hipGraphExecDestroy(k->gexec);
+    if (k->graph) (void)hipGraphDestroy(k->graph);
+    k->gexec = nullptr;
+    k->graph = nullptr;
+    k->gkey.clear();
+}
+
+// Wait for the call enqueued on the handle's stream (its download included),
+// hand the result over and account its time.
 int finish_call(saship_kernel *k, size_t rbytes, double *result)
 {
-    HIP_TRY(hipMemcpyAsync(k->h_result, k->d_result, rbytes, hipMemcpyDeviceToHost, k->st));
     HIP_TRY(hipStreamSynchronize(k->st));
     memcpy(result, k->h_result, rbytes);
     if (k->timing) {
@@ -587,8 +620,9 @@ int saship_kernel_create(saship_model *m, int32_t nq, const double *q, int32_t i
     if (e == hipSuccess) e = hipMalloc((void **)&k->q32, sizeof(float) * std::max<size_t>(nqv, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&k->q64, sizeof(double) * std::max<size_t>(nqv, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&k->d_meta, 256);
-    if (e == hipSuccess) e = hipMalloc((void **)&k->d_result, sizeof(double) * (2 * (size_t)nq + 4));
-    if (e == hipSuccess) e = hipHostMalloc((void **)&k->h_result, sizeof(double) * (2 * (size_t)nq + 4), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&k->h_result, sizeof(double) * (2 * (size_t)nq + 4),
+                                             hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&k->hd_result, k->h_result, 0);
     if (e == hipSuccess && nqv) e = hipMemcpy(k->q32, q32.data(), sizeof(float) * nqv, hipMemcpyHostToDevice);
     if (e == hipSuccess && nqv) e = hipMemcpy(k->q64, q, sizeof(double) * nqv, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreate(&k->ev0);
@@ -624,40 +658,76 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     if (int e = ensure(k->d_args, k->args_cap, doff + dbytes)) return e;
     memcpy(k->h_stage, values, vbytes);
     memcpy(k->h_stage + doff, details, dbytes);
-    HIP_TRY(hipMemcpyAsync(k->d_args, k->h_stage, doff + dbytes, hipMemcpyHostToDevice, k->st));
     const size_t rbytes = sizeof(double) * ((size_t)nout * nq + 4);
     const int accumulate = pd_start > 0 ? 1 : 0;
-    if (accumulate) {
-        memcpy(k->h_result, result, rbytes);
-        HIP_TRY(hipMemcpyAsync(k->d_result, k->h_result, rbytes, hipMemcpyHostToDevice, k->st));
+    if (accumulate) memcpy(k->h_result, result, rbytes);
+    // device buffers before any capture (allocations are never captured)
+    int nlp = 0;
+    const int path = choose_path(k->m, kind, details, k->separable, &nlp);
+    const int npts_max = std::max(0, pd_stop - pd_start);
+    int nsplit = 0;
+    if (path == 1) {
+        if (!k->d_sep) HIP_TRY(hipMalloc((void **)&k->d_sep, sas::sep::tables_bytes(1, SEP_MAX)));
+        if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * 2 * (size_t)nq * k->m->quad.n)) return e;
+    } else if (path == 2) {
+        nsplit = split_count(k->cus, nq, npts_max);
+        if (int e = ensure(k->d_recs, k->recs_cap, sizeof(float4) * (size_t)std::max(1, npts_max) * record_width(id, kind)))
+            return e;
+        if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * (kind == 0 ? 2 : 1) * (size_t)nq * nsplit))
+            return e;
     }
     const int32_t *d_det = (const int32_t *)(k->d_args + doff);
     const double *d_val = (const double *)k->d_args;
-    Target t{k->st, k->m, nq, k->is2d, k->cus, k->q32, k->q64, nullptr, k->d_meta, nullptr, k->d_result, nullptr,
+    // the final kernel reads (accumulate) and writes the result in mapped
+    // host memory: no transfer of the result in either direction
+    Target t{k->st, k->m, nq, k->is2d, k->cus, k->q32, k->q64, k->d_recs, k->d_meta, k->d_partial, k->hd_result, k->d_sep,
              k->timing ? k->ev0 : nullptr, k->timing ? k->ev1 : nullptr};
-    int nlp = 0;
-    const int path = choose_path(k->m, kind, details, k->separable, &nlp);
-    int rc;
-    if (path == 0) {
-        rc = launch_structure(t, d_val, cutoff, accumulate);
-    } else if (path == 1) {
-        if (!k->d_sep) HIP_TRY(hipMalloc((void **)&k->d_sep, sas::sep::tables_bytes(1, SEP_MAX)));
-        if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * 2 * (size_t)nq * k->m->quad.n)) return e;
-        t.sep = k->d_sep;
-        t.partial = k->d_partial;
-        rc = launch_sep(t, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
+    const size_t staged = doff + dbytes;
+    const bool graphed = k->use_graph && !k->timing;
+    // upload and evaluation: one sequence on the handle's stream (a captured
+    // graph copies from the staging buffer: kernel arguments would be frozen)
+    auto enqueue = [&]() -> int {
+        if (!graphed && staged <= (size_t)sas::STAGE_ARG_BYTES) {
+            sas::StageIn in;
+            in.nbytes = (int)staged;
+            memcpy(in.data, k->h_stage, staged);
+            sas::k_stage_in<<<1, 128, 0, k->st>>>(in, (unsigned char *)k->d_args);
+            HIP_TRY(hipGetLastError());
+        } else {
+            HIP_TRY(hipMemcpyAsync(k->d_args, k->h_stage, staged, hipMemcpyHostToDevice, k->st));
+        }
+        int rc;
+        if (path == 0) rc = launch_structure(t, d_val, cutoff, accumulate);
+        else if (path == 1) rc = launch_sep(t, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate);
+        else rc = launch_model(t, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, nsplit);
+        return rc;
+    };
+    if (graphed) {
+        int64_t cbits;
+        memcpy(&cbits, &cutoff, sizeof cbits);
+        const std::vector<int64_t> key = {kind, path, nlp, nsplit, (int64_t)(doff + dbytes), (int64_t)doff, accumulate, pd_start,
+                                          pd_stop, reff_mode, cbits, (int64_t)rbytes, (int64_t)(intptr_t)k->d_args,
+                                          (int64_t)(intptr_t)k->d_recs, (int64_t)(intptr_t)k->d_partial,
+                                          (int64_t)(intptr_t)k->d_sep, (int64_t)(intptr_t)k->h_stage};
+        if (!k->gexec || key != k->gkey) {
+            drop_graph(k);
+            HIP_TRY(hipStreamBeginCapture(k->st, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue();
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(k->st, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ce != hipSuccess) return fail(SASHIP_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            k->graph = g;
+            HIP_TRY(hipGraphInstantiate(&k->gexec, g, nullptr, nullptr, 0));
+            k->gkey = key;
+        }
+        HIP_TRY(hipGraphLaunch(k->gexec, k->st));
     } else {
-        const int npts_max = std::max(0, pd_stop - pd_start);
-        const int rv = record_width(id, kind);
-        const int nsplit = split_count(k->cus, nq, npts_max);
-        if (int e = ensure(k->d_recs, k->recs_cap, sizeof(float4) * (size_t)std::max(1, npts_max) * rv)) return e;
-        if (int e = ensure(k->d_partial, k->partial_cap, sizeof(double) * (kind == 0 ? 2 : 1) * (size_t)nq * nsplit))
-            return e;
-        t.recs = k->d_recs;
-        t.partial = k->d_partial;
-        rc = launch_model(t, kind, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate, nsplit);
+        if (int e = enqueue()) return e;
     }
-    if (rc) return rc;
     return finish_call(k, rbytes, result);
 }
 
@@ -776,6 +846,15 @@ int saship_kernel_set_path(saship_kernel *k, int32_t separable)
     return 0;
 }
 
+int saship_kernel_set_graph(saship_kernel *k, int32_t enable)
+{
+    if (!k) return fail(SASHIP_EINVAL, "kernel is NULL");
+    std::lock_guard<std::mutex> lock(k->mu);
+    k->use_graph = enable != 0;
+    drop_graph(k);
+    return 0;
+}
+
 int saship_kernel_set_timing(saship_kernel *k, int32_t enable)
 {
     if (!k) return fail(SASHIP_EINVAL, "kernel is NULL");
@@ -800,13 +879,13 @@ int saship_kernel_release(saship_kernel *k)
     if (!k) return 0;
     (void)hipSetDevice(k->dev);
     if (k->st) (void)hipStreamSynchronize(k->st);
+    drop_graph(k);
     if (k->q32) (void)hipFree(k->q32);
     if (k->q64) (void)hipFree(k->q64);
     if (k->d_args) (void)hipFree(k->d_args);
     if (k->d_recs) (void)hipFree(k->d_recs);
     if (k->d_meta) (void)hipFree(k->d_meta);
     if (k->d_partial) (void)hipFree(k->d_partial);
-    if (k->d_result) (void)hipFree(k->d_result);
     if (k->d_sep) (void)hipFree(k->d_sep);
     if (k->h_stage) (void)hipHostFree(k->h_stage);
     if (k->h_result) (void)hipHostFree(k->h_result);

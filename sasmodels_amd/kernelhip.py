@@ -145,6 +145,10 @@ class HipKernel(Kernel):
         """Allow (default) or forbid the separable cylinder evaluation."""
         _lib.check(_lib.lib().saship_kernel_set_path(self._handle, 1 if enable else 0))
 
+    def set_graph(self, enable=True):
+        """Replay repeated calls of one shape from a captured hipGraph, or launch eagerly (default)."""
+        _lib.check(_lib.lib().saship_kernel_set_graph(self._handle, int(bool(enable))))
+
     def set_timing(self, enable=True):
         """Sum the device time of the evaluation kernel over later calls."""
         _lib.check(_lib.lib().saship_kernel_set_timing(self._handle, 1 if enable else 0))
@@ -200,6 +204,7 @@ class HipBatch:
             1 if self.dim == "2d" else 0, ct.byref(h)))
         self._handle = h
         self._n_out = None          # smeared points per set (set_resolution)
+        self._uniform = bool(self.nq.size) and int(self.nq.min()) == int(self.nq.max()) > 0
         if use_graph:
             self.set_options(use_graph=True)
 
@@ -286,24 +291,35 @@ class HipBatch:
     def _kind(self, magnetic):
         return KIND_IQ if self.dim == "1d" else (KIND_IMAGNETIC if magnetic else KIND_IQXY)
 
-    def raw(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0, want_result=True, want_iq=True):
-        """(raw accumulators of all sets back to back, I(q) of all sets back to back)."""
+    def raw(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0, want_result=True, want_iq=True,
+            out=None):
+        """(raw accumulators of all sets back to back, I(q) of all sets back to back).
+        *out*: a float64 array of the I(q) size to write I(q) into (a fit
+        loop that reuses it avoids faulting in new memory on every call)."""
         det, vals, voff, ndet = self._pack(calls)
         nout = self._nout()
         result = np.empty(int(np.sum(nout*self.nq + 4)), "d") if want_result else None
         n_iq = int(self.qoff[-1]) if self._n_out is None else int(self._n_out.sum())
-        iq = np.empty(n_iq, "d") if want_iq else None
+        if out is not None and want_iq:
+            if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.size == n_iq
+                    and out.flags.c_contiguous and out.flags.writeable):
+                raise ValueError("out must be a writeable contiguous float64 array of %d values" % n_iq)
+            iq = out.reshape(-1)
+        else:
+            iq = np.empty(n_iq, "d") if want_iq else None
         _lib.check(_lib.lib().saship_batch_call(
             self._handle, self._kind(magnetic), det.ctypes.data, ndet, vals.ctypes.data, voff.ctypes.data,
             float(cutoff), int(radius_effective_mode),
             result.ctypes.data if want_result else None, iq.ctypes.data if want_iq else None))
         return result, iq
 
-    def __call__(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0):
-        """List of I(q), one per set."""
-        _, iq = self.raw(calls, cutoff, magnetic, radius_effective_mode, want_result=False, want_iq=True)
+    def __call__(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0, out=None):
+        """List of I(q), one per set (views of one array: *out* when given, see :meth:`raw`)."""
+        _, iq = self.raw(calls, cutoff, magnetic, radius_effective_mode, want_result=False, want_iq=True, out=out)
         if self._n_out is not None:
             return np.split(iq, np.cumsum(self._n_out)[:-1])
+        if self._uniform:
+            return list(iq.reshape(self.nsets, -1))      # rows of one array: views, as np.split gives
         return np.split(iq, self.qoff[1:-1])
 
     def Fq(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0):
