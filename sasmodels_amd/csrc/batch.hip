@@ -204,15 +204,29 @@ struct SepArgs {
     const int64_t *poff;
 };
 
+// One workgroup of four waves per item (set, 64-q tile, block of Gauss
+// nodes): the block's nodes are dealt round-robin to the waves, each wave
+// sums its nodes with compensation, and the four sums are combined in a
+// fixed order through LDS, so one partial per output and q leaves the
+// workgroup for the whole block (larger blocks: fewer partials in HBM, no
+// loss of balance inside the workgroup).
 __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ queue, double *__restrict__ partial)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ double red[2][4][64];
+    __shared__ int next_item;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const SAS_CONST int *ipre = as_const(a.ipre);
     const int n_items = ipre[a.nsets];
-    QueueCursor cur = queue_start();
+    // the workgroup's cursor lives in wave 0; it starts on counter blockIdx % WORK_QUEUES
+    QueueCursor cur{(int)(blockIdx.x % WORK_QUEUES), 0};
     int pos = 0;
     for (;;) {
-        const int item = queue_next(queue, n_items, cur);
+        if (wave == 0) {
+            const int it = queue_next(queue, n_items, cur);
+            if (lane == 0) next_item = it;
+        }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(next_item);
         if (item < 0) break;
         // last position p with ipre[p] <= item: the previous item's set
         // when it still holds (items of one counter are consecutive); kept
@@ -237,18 +251,22 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
         const float q = iq < nq ? a.q32[q0 + iq] : 0.0f;
         const int n0 = blk * a.nodes_per_item, n1 = min(a.quad.n, n0 + a.nodes_per_item);
         KSum a1, a2;
-        for (int node = n0; node < n1; node++) {
+        for (int node = n0 + wave; node < n1; node += 4) {
             float f1, f2;
             sep::node_sums(a.t, s, q, a.quad.a[node], a.quad.b[node], f1, f2);
             const float w = a.quad.w[node];
             a1.add(w * f1);
             a2.add(w * f2);
         }
-        if (iq < nq) {
-            double *dst = partial + a.poff[s] + (size_t)blk * 2 * nq;
-            dst[iq] = a2.value();
-            dst[nq + iq] = a1.value();
+        red[0][wave][lane] = a2.value();
+        red[1][wave][lane] = a1.value();
+        __syncthreads();
+        if (wave < 2 && iq < nq) {
+            // wave 0: <F^2>, wave 1: <F>, the four waves' sums in wave order
+            const double v = ((red[wave][0][lane] + red[wave][1][lane]) + red[wave][2][lane]) + red[wave][3][lane];
+            partial[a.poff[s] + (size_t)(blk * 2 + wave) * nq + iq] = v;
         }
+        __syncthreads();
     }
 }
 
