@@ -16,7 +16,7 @@
  *                       parallelepiped, triaxial_ellipsoid, bcc, fcc and
  *                       sc paracrystal (with lib/gauss150.c), hollow_cylinder,
  *                       vesicle, fuzzy_sphere, stickyhardsphere, squarewell,
- *                       hayter_msa
+ *                       hayter_msa, core_multi_shell (18), onion (19)
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -152,8 +152,23 @@ double ref_sq6(double q, double r, double phi, double a, double b, double c, dou
 {
     return Iq(q, r, phi, a, b, c, d);
 }
+#elif REF_MODEL == 18
+#  include "lib/sas_3j1x_x.c"
+#  include "core_multi_shell.c"
+/* vector parameters sld[10], thickness[10] go by pointer, as the generator
+ * passes them (kernel_iq.c:317-328 calls with the values' slots in place) */
+#  define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], (double *)p + 4, (double *)p + 14)
+#  define NPAR_VOLUME(p) form_volume(p[1], p[3], (double *)p + 14)
+#  define NPAR_REFF(m, p) radius_effective(m, p[1], p[3], (double *)p + 14)
+#elif REF_MODEL == 19
+#  include "lib/sas_3j1x_x.c"
+#  include "onion.c"
+#  define NPAR_CALL_FQ(q, f1, f2, p) \
+    Fq(q, f1, f2, p[0], p[1], p[2], p[3], (double *)p + 4, (double *)p + 14, (double *)p + 24, (double *)p + 34)
+#  define NPAR_VOLUME(p) form_volume(p[1], p[3], (double *)p + 24)
+#  define NPAR_REFF(m, p) radius_effective(m, p[1], p[3], (double *)p + 24)
 #else
-#  error "REF_MODEL must be 1..17"
+#  error "REF_MODEL must be 1..19"
 #endif
 
 #ifdef NPAR_CALL_FQ

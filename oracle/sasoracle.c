@@ -28,7 +28,9 @@
  *     bcc_paracrystal.c, fcc_paracrystal.c, sc_paracrystal.c with
  *     lib/sphere_form.c (Gauss-150 product rule, lib/gauss150.c),
  *     hollow_cylinder.c, vesicle.c, fuzzy_sphere.c (shell volumes of the
- *     hollow shapes in the volume sums, kernel_iq.c:728-731);
+ *     hollow shapes in the volume sums, kernel_iq.c:728-731),
+ *     core_multi_shell.c and onion.c (vector parameters: n shells of
+ *     sld[n], thickness[n], ... packed as ten slots each);
  *     view/jitter rotation with psi qabc_rotation/qabc_apply
  *     kernel_iq.c:251-310
  *   - hard-sphere structure factor hardsphere.c:1-90
@@ -769,12 +771,103 @@ static void fuz_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
     *f2 = 1.0e-4 * form * form;
 }
 
+/* ---- shells of vector parameters: n = (int)(fp_n + 0.5) of a control
+ * parameter with limits [0, 10] (core_multi_shell.c:12-18, onion.c:31-38) ---- */
+#define O_NSHELL 10
+static int shell_count(oreal fp_n) { return (int)(fp_n + 0.5); }
+
+/* ---- core_multi_shell (core_multi_shell.c:1-60): p = sld_core, radius,
+ * sld_solvent, n, sld[10], thickness[10].  Boundary k sits at radius plus
+ * the first k thicknesses and carries (sld outside - sld inside). ---- */
+static oreal cms_outer(const oreal *p)
+{
+    const int n = shell_count(p[3]);
+    oreal r = p[1];
+    for (int i = 0; i < n; i++) r += p[14 + i];
+    return r;
+}
+static oreal cms_vol(const oreal *p) { return O_4PI_3 * cb(cms_outer(p)); }
+static oreal cms_reff(int mode, const oreal *p) { return mode == 2 ? p[1] : cms_outer(p); }
+static void cms_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const int n = shell_count(p[3]);
+    oreal r = p[1], inside = p[0], f = 0.0;
+    for (int i = 0; i < n; i++) {
+        f += O_4PI_3 * cb(r) * (p[4 + i] - inside) * o_sph_j1c(q * r);
+        inside = p[4 + i];
+        r += p[14 + i];
+    }
+    f += O_4PI_3 * cb(r) * (p[2] - inside) * o_sph_j1c(q * r);
+    *f1 = 1e-2 * f;
+    *f2 = 1e-4 * f * f;
+}
+
+/* ---- onion (onion.c:1-73): p = sld_core, radius_core, sld_solvent,
+ * n_shells, sld_in[10], sld_out[10], thickness[10], A[10].  Shell i has
+ * sld(x) = sld_in + (sld_out - sld_in)(exp(A x) - 1)/(exp(A) - 1) over its
+ * fractional depth x; flat (sld_in) when A = 0.  The amplitude of a sphere
+ * of radius r is evaluated at the shell's inner (side 0) and outer (side 1)
+ * radius and differenced. ---- */
+static oreal oni_sphere_amp(oreal q, oreal r, oreal sld_in, oreal sld_out, oreal thickness, oreal A,
+                            oreal side)
+{
+    const oreal vol = O_4PI_3 * cb(r);
+    const oreal qr = q * r;
+    const oreal bes = o_sph_j1c(qr);
+    const oreal alpha = A * r / thickness;
+    oreal amp;
+    if (qr == 0.0) {
+        amp = 1.0;
+    } else if (fabs(A) > 0.0) {
+        const oreal qr2 = qr * qr, alpha2 = alpha * alpha;
+        const oreal sum2 = alpha2 + qr2;
+        const oreal s = sin(qr), c = cos(qr);
+        const oreal t1 = (alpha2 - qr2) * s / qr - 2.0 * alpha * c;
+        const oreal t2 = alpha * s / qr - c;
+        const oreal fun = -3.0 * (t1 / sum2 - t2) / sum2;
+        const oreal slope = (sld_out - sld_in) / expm1(A);
+        const oreal edge = slope * exp(A * side);
+        amp = edge * fun + (sld_in - slope) * bes;
+    } else {
+        amp = sld_in * bes;
+    }
+    return vol * amp;
+}
+static oreal oni_outer(const oreal *p)
+{
+    const int n = shell_count(p[3]);
+    oreal r = p[1];
+    for (int i = 0; i < n; i++) r += p[24 + i];
+    return r;
+}
+static oreal oni_vol(const oreal *p) { return O_4PI_3 * cb(oni_outer(p)); }
+static oreal oni_reff(int mode, const oreal *p) { (void)mode; return oni_outer(p); }
+static void oni_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const int n = shell_count(p[3]);
+    oreal r_out = p[1];
+    oreal f = oni_sphere_amp(q, r_out, p[0], 0.0, 0.0, 0.0, 0.0);
+    for (int i = 0; i < n; i++) {
+        const oreal r_in = r_out;
+        r_out += p[24 + i];
+        f -= oni_sphere_amp(q, r_in, p[4 + i], p[14 + i], p[24 + i], p[34 + i], 0.0);
+        f += oni_sphere_amp(q, r_out, p[4 + i], p[14 + i], p[24 + i], p[34 + i], 1.0);
+    }
+    f -= oni_sphere_amp(q, r_out, p[2], 0.0, 0.0, 0.0, 0.0);
+    *f1 = 1e-2 * f;
+    *f2 = 1e-4 * f * f;
+}
+
 /* ------------------------------------------------------------------ */
 /* Model table                                                         */
 /* ------------------------------------------------------------------ */
+#define O_MAX_PARS 48   /* kernel parameters of the largest model (onion: 44) */
+#define O_MAX_SLD 24    /* sld parameters of the largest model (onion: 22) */
 typedef struct {
     int npars, nmag, max_pd, oriented;
-    int sld_idx[3];
+    int sld_idx[O_MAX_SLD];
     int (*valid)(const oreal *p);
     oreal (*vol)(const oreal *p);
     oreal (*reff)(int mode, const oreal *p);
@@ -789,9 +882,10 @@ static int always_valid(const oreal *p) { (void)p; return 1; }
 
 /* model ids: 0 sphere, 1 cylinder, 2 ellipsoid, 3 core_shell_cylinder,
  * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid, 8-10 bcc,
- * fcc, sc paracrystal, 11 hollow_cylinder, 12 vesicle, 13 fuzzy_sphere (the
- * ids libsaship uses; 5, the hard-sphere structure factor, has its own entry) */
-#define N_MODELS 14
+ * fcc, sc paracrystal, 11 hollow_cylinder, 12 vesicle, 13 fuzzy_sphere,
+ * 17 core_multi_shell, 18 onion (the ids libsaship uses; 5 and 14-16, the
+ * structure factors, have entries of their own) */
+#define N_MODELS 19
 static const OModel MODELS[N_MODELS] = {
     {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0, 0, 0},
     {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0, 0, 0},
@@ -807,6 +901,13 @@ static const OModel MODELS[N_MODELS] = {
     {7, 2, 5, 1, {3, 4, -1}, always_valid, hcy_vol, hcy_reff, hcy_fq, hcy_iqac, 0, 0, hcy_svol},
     {5, 2, 2, 0, {0, 1, -1}, always_valid, ves_vol, ves_reff, ves_fq, 0, 0, 0, ves_svol},
     {4, 2, 2, 0, {0, 1, -1}, always_valid, fuz_vol, fuz_reff, fuz_fq, 0, 0, 0, 0},
+    {0, 0, 0, 0, {-1}, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, {-1}, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, {-1}, 0, 0, 0, 0, 0, 0, 0, 0},
+    {24, 12, 5, 0, {0, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13}, always_valid, cms_vol, cms_reff, cms_fq,
+     0, 0, 0, 0},
+    {44, 22, 5, 0, {0, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23},
+     always_valid, oni_vol, oni_reff, oni_fq, 0, 0, 0, 0},
 };
 static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npars > 0; }
 
@@ -940,7 +1041,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
     for (int k = 0; k < 4; k++) acc[k] = pd_start == 0 ? 0.0 : result[nout * nq + k];
     if (pd_start == 0) memset(result, 0, sizeof(oreal) * nout * nq);
 
-    oreal base[8];
+    oreal base[O_MAX_PARS];
     for (int i = 0; i < np; i++) base[i] = values[2 + i];
     oreal view_theta = 0, view_phi = 0, view_psi = 0;
     if (oriented2d) {
@@ -968,7 +1069,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
     oreal *pts = (oreal *)malloc(sizeof(oreal) * (size_t)(npts ? npts : 1) * rw);
     int nacc = 0;
     for (int step = pd_start; step < pd_stop; step++) {
-        oreal p[8];
+        oreal p[O_MAX_PARS];
         memcpy(p, base, sizeof(oreal) * np);
         oreal w0 = 1.0;
         for (int k = maxpd - 1; k >= 0; k--) {
@@ -1007,7 +1108,7 @@ int sasoracle_kernel(int model_id, int kind, int quad_n,
     for (int iq = 0; iq < nq; iq++) {
         for (int a = 0; a < nacc; a++) {
             const oreal *rec = pts + (size_t)a * rw;
-            oreal p[8];
+            oreal p[O_MAX_PARS];
             memcpy(p, rec, sizeof(oreal) * np);
             const oreal weight = rec[np];
             if (kind == 0) {

@@ -127,6 +127,14 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
         }
         const double qd = M::FP64 && iq < nq ? a.q64[q0 + iq] : 0.0;
         for (int j = c0; j < c1 && !shared; j++) {
+            if constexpr (M::FP64 && RV > BIG_RECORD) {
+                // a record too large for registers is read where it lies
+                double f1, f2;
+                M::eval1d_d(qd, (const float *)(rec + (size_t)j * RV), a.quad, f1, f2);
+                d1 += f1;
+                d2 += f2;
+                continue;
+            }
             float r[4 * RV];
             load_rec(rec + (size_t)j * RV, r, RV);
             if constexpr (M::FP64) {
@@ -517,6 +525,8 @@ int record_width(int id, int kind)
     case 10: return rv_of<sas::ScParacrystal>(kind);
     case 11: return rv_of<sas::HollowCylinder>(kind);
     case 12: return rv_of<sas::Vesicle>(kind);
+    case 17: return rv_of<sas::CoreMultiShell>(kind);
+    case 18: return rv_of<sas::Onion>(kind);
     default: return rv_of<sas::FuzzySphere>(kind);
     }
 }
@@ -776,6 +786,8 @@ int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, 
     case 10: return enqueue<sas::ScParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 11: return enqueue<sas::HollowCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 12: return enqueue<sas::Vesicle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 17: return enqueue<sas::CoreMultiShell>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 18: return enqueue<sas::Onion>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     default: return enqueue<sas::FuzzySphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }

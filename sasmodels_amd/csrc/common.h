@@ -77,25 +77,26 @@ inline int cu_count(int dev)
     return n;
 }
 
-// Model ids 0-4 and 6-13 are form factors; 5, 14, 15 and 16 the
+// Model ids 0-4, 6-13 and 17-18 are form factors; 5, 14, 15 and 16 the
 // hard-sphere, sticky hard-sphere, square-well and Hayter-Penfold MSA
 // structure factors (radius_effective, volfraction and up to four more
-// parameters, nothing polydisperse, no sld: no magnetic values).
+// parameters, nothing polydisperse, no sld: no magnetic values).  17
+// (core_multi_shell) and 18 (onion) have vector parameters, ten slots each.
 constexpr int MODEL_HARDSPHERE = 5;
-constexpr int MODEL_LAST = 16;
+constexpr int MODEL_LAST = 18;
 inline int model_npars(int id)
 {
-    static const int np[17] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6};
+    static const int np[19] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6, 24, 44};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[17] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0};
+    static const int mp[19] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0, 5, 5};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[17] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8};
+    static const int nv[19] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8, 66, 116};
     return nv[id];
 }
 inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || (id >= 6 && id <= 11); }

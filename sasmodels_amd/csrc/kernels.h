@@ -30,6 +30,9 @@ constexpr int EXPAND_THREADS = 512;
 constexpr int WORK_QUEUES = 16, WORK_QUEUE_STRIDE = 1024;
 constexpr int EVAL_THREADS = 256;
 constexpr int TILE_PTS = 64;
+// Records wider than this many float4 are not copied to registers by the
+// 1D kernels (the evaluation reads them in place).
+constexpr int BIG_RECORD = 8;
 constexpr double DEG = 0.017453292519943295;   // pi/180
 
 // Does model M have the FP32 2D path (REC2F, rec2f, geom2f)?
@@ -339,6 +342,14 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_fq1d(
         __syncthreads();
         if (iq < nq) {
             for (int j = 0; j < cnt; j++) {
+                if constexpr (M::FP64 && RV > BIG_RECORD) {
+                    // a record too large for registers is read where it lies in LDS
+                    double f1, f2;
+                    M::eval1d_d(qd, (const float *)(tile + j * RV), quad, f1, f2);
+                    d1 += f1;
+                    d2 += f2;
+                    continue;
+                }
                 float r[4 * RV];
 #pragma unroll
                 for (int v = 0; v < RV; v++) {
@@ -420,7 +431,7 @@ struct Contrast {
 #pragma unroll
         for (int k = 0; k < M::NMAG; k++) sld[k] = values[2 + M::sld_param(k)];
         if constexpr (!MAG) {
-            M::contrasts(sld, con[0]);
+            M::contrasts(values + 2, sld, con[0]);
             xsw[0] = 1.0f;
             active[0] = true;
         } else {
@@ -447,7 +458,7 @@ struct Contrast {
             for (int x = 0; x < NX; x++) {
                 active[x] = w[x] > 1.e-8;                    // kernel_iq.c:761
                 xsw[x] = (active[x] && q_ok) ? (float)w[x] : 0.0f;
-                M::contrasts(eff[x], con[x]);
+                M::contrasts(values + 2, eff[x], con[x]);
             }
         }
     }

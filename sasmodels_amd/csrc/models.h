@@ -9,7 +9,9 @@
 //   rec2d(p, w, r)    2D record: lengths that enter a phase in FP64 (two
 //                     words each), per-layer volumes and w*1e-4 in FP32
 //   geom2d(qr, r, g)  amplitude basis g[k] at the rotated q (FP64); F = sum_k con[k]*g[k]
-//   contrasts(sld, con)          con[k] from the SLD slot values (FP64)
+//   contrasts(p, sld, con)       con[k] from the SLD slot values (FP64); p, the
+//                                kernel parameters, gives the layer count of
+//                                models with vector parameters
 // The amplitude of every model is linear in its SLDs, which lets the
 // magnetic kernel evaluate the geometry once and reuse it for each spin
 // cross section.
@@ -18,7 +20,8 @@
 // ellipsoid.c:1-105, core_shell_cylinder.c:1-123, core_shell_sphere.c:1-30
 // with lib/core_shell.c:8-30, parallelepiped.c:1-150,
 // triaxial_ellipsoid.c:1-160, bcc/fcc/sc_paracrystal.c with
-// lib/sphere_form.c, hollow_cylinder.c, vesicle.c, fuzzy_sphere.c.  Parameter order: sphere.py:66-72,
+// lib/sphere_form.c, hollow_cylinder.c, vesicle.c, fuzzy_sphere.c,
+// core_multi_shell.c, onion.c.  Parameter order: sphere.py:66-72,
 // cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 // core_shell_sphere.py:78-82.
 #pragma once
@@ -155,7 +158,7 @@ struct Sphere {
         const float t = q.q * f[0];
         g[0] = f[1] * sph_j1c_rev2(t * t);
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 // ---------------------------------------------------------------- cylinder
@@ -267,7 +270,7 @@ struct Cylinder {
         const float qab = sqrt_f(fmaf(q.qa, q.qa, q.qb * q.qb));
         g[0] = f[8] * j1c_rev(qab * f[6]) * sinc_rev(q.qc * f[7]);
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 // ---------------------------------------------------------------- ellipsoid
@@ -363,7 +366,7 @@ struct Ellipsoid {
         const float t2 = fmaf(f[6], fmaf(q.qa, q.qa, q.qb * q.qb), f[7] * (q.qc * q.qc));
         g[0] = f[8] * sph_j1c_rev2(t2);
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 // ---------------------------------------------------------------- core_shell_cylinder
@@ -501,7 +504,7 @@ struct CoreShellCylinder {
         g[0] = f[10] * sinc_rev(q.qc * f[7]) * j1c_rev(qab * f[6]);
         g[1] = f[11] * sinc_rev(q.qc * f[9]) * j1c_rev(qab * f[8]);
     }
-    SAS_HD static void contrasts(const double *sld, float *con)
+    SAS_HD static void contrasts(const double *, const double *sld, float *con)
     {
         con[0] = (float)(sld[0] - sld[1]);
         con[1] = (float)(sld[1] - sld[2]);
@@ -575,7 +578,7 @@ struct CoreShellSphere {
         g[0] = f[2] * sph_j1c_rev2(tc * tc);
         g[1] = f[3] * sph_j1c_rev2(ts * ts);
     }
-    SAS_HD static void contrasts(const double *sld, float *con)
+    SAS_HD static void contrasts(const double *, const double *sld, float *con)
     {
         con[0] = (float)(sld[0] - sld[1]);
         con[1] = (float)(sld[1] - sld[2]);
@@ -693,7 +696,7 @@ struct Parallelepiped {
     {
         g[0] = r[6] * sinc_d(q.qa * get_d(r, 0)) * sinc_d(q.qb * get_d(r, 2)) * sinc_d(q.qc * get_d(r, 4));
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 // ---------------------------------------------------------------- triaxial ellipsoid
@@ -820,7 +823,7 @@ struct TriaxialEllipsoid {
         const double qr = sqrt(get_d(r, 0) * q.qa * q.qa + get_d(r, 2) * q.qb * q.qb + get_d(r, 4) * q.qc * q.qc);
         g[0] = r[6] * sph_j1c_d(qr);
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 // ---------------------------------------------------------------- paracrystals
@@ -936,7 +939,7 @@ struct Paracrystal {
         const double qm = sqrt(q.qa * q.qa + q.qb * q.qb + q.qc * q.qc);
         g[0] = (float)(r[6] * sph_j1c_dd(qm * get_d(r, 0)) * sqrt(z));
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 using BccParacrystal = Paracrystal<BCC>;
 using FccParacrystal = Paracrystal<FCC>;
@@ -1025,7 +1028,7 @@ struct HollowCylinder {
         const float psi = (float)((j1c_d(qab * ro) - g2 * j1c_d(qab * ri)) / (1.0 - g2));
         g[0] = r[6] * psi * sinc_d(q.qc * get_d(r, 4));
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 // ---------------------------------------------------------------- vesicle
@@ -1081,7 +1084,7 @@ struct Vesicle {
     {
         g[0] = r[5] * sph_j1c_d(q.q * get_d(r, 2)) - r[4] * sph_j1c_d(q.q * get_d(r, 0));
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
 };
 
 // ---------------------------------------------------------------- fuzzy sphere
@@ -1130,7 +1133,341 @@ struct FuzzySphere {
         const double qf = q.q * get_d(r, 2);
         g[0] = r[4] * sph_j1c_d(q.q * get_d(r, 0)) * (float)exp(-0.5 * qf * qf);
     }
-    SAS_HD static void contrasts(const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- concentric shells
+// Vector parameters: a control parameter p[3] (limits [0, 10]) gives
+// n = (int)(p[3] + 0.5) shells, read from the first n of ten slots per
+// vector (core_multi_shell.c:12-18, onion.c:31-38).  Boundary k = 0..n
+// sits at the core radius plus the first k thicknesses.
+constexpr int NSHELL = 10, NBOUND = NSHELL + 1;
+SAS_HD int shell_count(double fp_n)
+{
+    const int n = (int)(fp_n + 0.5);
+    return n < 0 ? 0 : n > NSHELL ? NSHELL : n;
+}
+
+// p = sld_core, radius, sld_solvent, n, sld[10], thickness[10]
+// (core_multi_shell.py:91-103; core_multi_shell.c:1-60): uniform shells;
+// the amplitude is sum_k 4 pi/3 r_k^3 (sld outside - sld inside) 3 j1(q r_k)/(q r_k).
+// Records carry the eleven boundaries with zero volume beyond n, so the
+// evaluation skips them.
+struct CoreMultiShell {
+    static constexpr int ID = 17, NPARS = 4 + 2 * NSHELL, NMAG = 2 + NSHELL, MAX_PD = 5, NCON = NBOUND;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = false;
+    static constexpr bool ABC = false;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = false;
+    // 1D: (r_k, V_k * jump_k) pairs, w*1e-2, w*1e-4
+    // 2D: r_k in FP64, V_k, w*1e-4
+    static constexpr int REC1D = 2 * NBOUND + 2, REC2D = 3 * NBOUND + 1;
+    static constexpr int A2D = 3 * NBOUND;
+    SAS_HD static int sld_param(int k) { return k == 0 ? 0 : k == 1 ? 2 : 2 + k; }   // core, solvent, sld1..
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double outer(const double *p)
+    {
+        const int n = shell_count(p[3]);
+        double r = p[1];
+        for (int i = 0; i < n; i++) r += p[4 + NSHELL + i];
+        return r;
+    }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(outer(p)); }
+    __device__ static double reff(int mode, const double *p) { return mode == 2 ? p[1] : outer(p); }
+    // radius of boundary k, 0 beyond the last one
+    __device__ static void radii(const double *p, double *r)
+    {
+        const int n = shell_count(p[3]);
+        double rk = p[1];
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            r[k] = k <= n ? rk : 0.0;
+            if (k < NSHELL && k < n) rk += p[4 + NSHELL + k];
+        }
+    }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        double rad[NBOUND], sld[NMAG], con[NCON];
+        radii(p, rad);
+#pragma unroll
+        for (int k = 0; k < NMAG; k++) sld[k] = p[sld_param(k)];
+        jumps(p, sld, con);
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            r[2 * k] = (float)rad[k];
+            r[2 * k + 1] = (float)(FOUR_PI_3_D * cube_d(rad[k]) * con[k]);
+        }
+        r[2 * NBOUND] = (float)(w * 1e-2);
+        r[2 * NBOUND + 1] = (float)(w * 1e-4);
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &, float &f1, float &f2)
+    {
+        float f = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            const float a = r[2 * k + 1];   // the same for every lane: a uniform branch
+            if (a != 0.0f) f = fmaf(a, sph_j1c_f(q * r[2 * k]), f);
+        }
+        f1 = r[2 * NBOUND] * f;
+        f2 = r[2 * NBOUND + 1] * f * f;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        double rad[NBOUND];
+        radii(p, rad);
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            put_d(r, 2 * k, rad[k]);
+            r[2 * NBOUND + k] = (float)(FOUR_PI_3_D * cube_d(rad[k]));
+        }
+        r[A2D] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            const float v = r[2 * NBOUND + k];
+            g[k] = v != 0.0f ? v * sph_j1c_d(q.q * get_d(r, 2 * k)) : 0.0f;
+        }
+    }
+    // FP32 record: r_k/2pi, V_k, w*1e-4, outer radius
+    static constexpr int REC2F = 2 * NBOUND + 2, A2F = 2 * NBOUND, XS2F = 2 * NBOUND + 1;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        double rad[NBOUND];
+        radii(p, rad);
+        double rmax = 0.0;
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            f[k] = (float)(rad[k] * INV_TWO_PI_D);
+            f[NBOUND + k] = (float)(FOUR_PI_3_D * cube_d(rad[k]));
+            rmax = fmax(rmax, rad[k]);
+        }
+        f[A2F] = (float)(w * 1e-4);
+        f[XS2F] = (float)rmax;
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            const float v = f[NBOUND + k];
+            const float t = q.q * f[k];
+            g[k] = v != 0.0f ? v * sph_j1c_rev2(t * t) : 0.0f;
+        }
+    }
+    // jump of the sld across boundary k, outside minus inside (the
+    // reference's sign); zero beyond the last boundary
+    SAS_HD static void jumps(const double *p, const double *sld, double *con)
+    {
+        const int n = shell_count(p[3]);
+        double inside = sld[0];
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            const double outside = k < n ? sld[2 + (k < NSHELL ? k : 0)] : sld[1];
+            con[k] = k <= n ? outside - inside : 0.0;
+            inside = outside;
+        }
+    }
+    SAS_HD static void contrasts(const double *p, const double *sld, float *con)
+    {
+        double c[NCON];
+        jumps(p, sld, c);
+#pragma unroll
+        for (int k = 0; k < NCON; k++) con[k] = (float)c[k];
+    }
+};
+
+// 3 (sin x - x cos x)/x^3 weighted by the exponential profile
+// exp(alpha r/R) of onion.c:12-21 (qr = x): the amplitude, per unit
+// volume and unit slope, of a sphere whose sld rises as exp(alpha r/R).
+SAS_HD double onion_profile_amp(double alpha, double x)
+{
+    const double x2 = x * x, a2 = alpha * alpha;
+    const double sum2 = a2 + x2;
+    double s, c;
+    sincos(x, &s, &c);
+    const double t1 = (a2 - x2) * s / x - 2.0 * alpha * c;
+    const double t2 = alpha * s / x - c;
+    return -3.0 * (t1 / sum2 - t2) / sum2;
+}
+
+// p = sld_core, radius_core, sld_solvent, n_shells, sld_in[10], sld_out[10],
+// thickness[10], A[10] (onion.py:323-333; onion.c:1-73), FP64 like the
+// reference (single = False).  Shell i's sld runs from sld_in to sld_out as
+// sld_in + slope_i (exp(A_i x) - 1), slope_i = (sld_out - sld_in)/expm1(A_i),
+// over its fractional depth x; flat at sld_in for A_i = 0.  Taking the
+// reference's difference of sphere amplitudes at each shell's inner and
+// outer radius apart gives, with b_k = 3 j1(q r_k)/(q r_k), V_k = 4 pi/3 r_k^3,
+//   F = sum_k V_k b_k (sld inside - sld outside at r_k)
+//     + sum_i slope_i [V_i (b_i - u(a_i r_i)) - e^A_i V_i+1 (b_i+1 - u(a_i r_i+1))]
+// with a_i = A_i/thickness_i and u = onion_profile_amp: one term per
+// boundary and one per exponential shell, each linear in the slds, so the
+// magnetic kernel reuses the geometry as for the other models.  At q = 0
+// every sphere amplitude of the reference is its bare volume and they
+// cancel: F = 0.
+struct Onion {
+    static constexpr int ID = 18, NPARS = 4 + 4 * NSHELL, NMAG = 2 + 2 * NSHELL, MAX_PD = 5;
+    static constexpr int NCON = NBOUND + NSHELL;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = false;
+    static constexpr bool ABC = false;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = true;
+    static constexpr bool HOLLOW = false;
+    static constexpr int P_SIN = 4, P_SOUT = 4 + NSHELL, P_T = 4 + 2 * NSHELL, P_A = 4 + 3 * NSHELL;
+    // 1D record, in doubles: r_k, V_k jump_k, slope_i, e^A_i, A_i/t_i, w*1e-2, w*1e-4
+    static constexpr int D_R = 0, D_C = NBOUND, D_S = 2 * NBOUND, D_E = 2 * NBOUND + NSHELL,
+                         D_A = 2 * NBOUND + 2 * NSHELL, D_W = 2 * NBOUND + 3 * NSHELL;
+    static constexpr int REC1D = 2 * (D_W + 2);
+    // 2D record, in doubles: r_k, e^A_i (0: flat or unused shell), A_i/t_i; then w*1e-4
+    static constexpr int E_R = 0, E_E = NBOUND, E_A = NBOUND + NSHELL;
+    static constexpr int A2D = 2 * (NBOUND + 2 * NSHELL), REC2D = A2D + 1;
+    SAS_HD static int sld_param(int k) { return k == 0 ? 0 : k == 1 ? 2 : 2 + k; }   // core, solvent, in.., out..
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double outer(const double *p)
+    {
+        const int n = shell_count(p[3]);
+        double r = p[1];
+        for (int i = 0; i < n; i++) r += p[P_T + i];
+        return r;
+    }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(outer(p)); }
+    __device__ static double reff(int, const double *p) { return outer(p); }
+    __device__ static bool profiled(const double *p, int i) { return i < shell_count(p[3]) && fabs(p[P_A + i]) > 0.0; }
+    __device__ static void radii(const double *p, double *r)
+    {
+        const int n = shell_count(p[3]);
+        double rk = p[1];
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            r[k] = k <= n ? rk : 0.0;
+            if (k < NSHELL && k < n) rk += p[P_T + k];
+        }
+    }
+    // jumps (inside - outside) at the boundaries, then the shells' slopes
+    SAS_HD static void contrasts_d(const double *p, const double *sld, double *con)
+    {
+        const int n = shell_count(p[3]);
+        double inside = sld[0];
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            double jump = 0.0, slope = 0.0;
+            if (k < n) {
+                const int i = k < NSHELL ? k : 0;
+                const double in = sld[2 + i], A = p[P_A + i];
+                jump = inside - in;
+                inside = in;
+                if (fabs(A) > 0.0) {
+                    const double out = sld[2 + NSHELL + i];
+                    slope = (out - in) / expm1(A);
+                    inside = out;
+                }
+            } else if (k == n) {
+                jump = inside - sld[1];
+            }
+            con[k] = jump;
+            if (k < NSHELL) con[NBOUND + k] = slope;
+        }
+    }
+    SAS_HD static void contrasts(const double *p, const double *sld, float *con)
+    {
+        double c[NCON];
+        contrasts_d(p, sld, c);
+#pragma unroll
+        for (int k = 0; k < NCON; k++) con[k] = (float)c[k];
+    }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        double rad[NBOUND], sld[NMAG], con[NCON];
+        radii(p, rad);
+#pragma unroll
+        for (int k = 0; k < NMAG; k++) sld[k] = p[sld_param(k)];
+        contrasts_d(p, sld, con);
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            put_d(r, 2 * (D_R + k), rad[k]);
+            put_d(r, 2 * (D_C + k), FOUR_PI_3_D * cube_d(rad[k]) * con[k]);
+        }
+#pragma unroll
+        for (int i = 0; i < NSHELL; i++) {
+            const bool on = profiled(p, i);
+            put_d(r, 2 * (D_S + i), on ? con[NBOUND + i] : 0.0);
+            put_d(r, 2 * (D_E + i), on ? exp(p[P_A + i]) : 0.0);
+            put_d(r, 2 * (D_A + i), on ? p[P_A + i] / p[P_T + i] : 0.0);
+        }
+        put_d(r, 2 * D_W, w * 1e-2);
+        put_d(r, 2 * (D_W + 1), w * 1e-4);
+    }
+    // the profile term of shell i between radii ri and ro
+    __device__ static double shell_term(double q, double ri, double ro, double bi, double bo, double e, double a)
+    {
+        const double vi = ri > 0.0 ? FOUR_PI_3_D * cube_d(ri) * (bi - onion_profile_amp(a * ri, q * ri)) : 0.0;
+        const double vo = ro > 0.0 ? FOUR_PI_3_D * cube_d(ro) * (bo - onion_profile_amp(a * ro, q * ro)) : 0.0;
+        return vi - e * vo;
+    }
+    __device__ static void eval1d_d(double q, const float *r, const Quad &, double &f1, double &f2)
+    {
+        double f = 0.0;
+        if (q != 0.0) {
+            double b[NBOUND];
+#pragma unroll
+            for (int k = 0; k < NBOUND; k++) {
+                const double c = get_d(r, 2 * (D_C + k));
+                b[k] = sph_j1c_dd(q * get_d(r, 2 * (D_R + k)));
+                f += c * b[k];
+            }
+#pragma unroll
+            for (int i = 0; i < NSHELL; i++) {
+                const double s = get_d(r, 2 * (D_S + i));
+                if (s != 0.0)
+                    f += s * shell_term(q, get_d(r, 2 * (D_R + i)), get_d(r, 2 * (D_R + i + 1)), b[i], b[i + 1],
+                                        get_d(r, 2 * (D_E + i)), get_d(r, 2 * (D_A + i)));
+            }
+        }
+        f1 = get_d(r, 2 * D_W) * f;
+        f2 = get_d(r, 2 * (D_W + 1)) * f * f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float, const float *, const Quad &, float &f1, float &f2)
+    {
+        f1 = f2 = 0.0f;    // FP64 models use eval1d_d
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        double rad[NBOUND];
+        radii(p, rad);
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) put_d(r, 2 * (E_R + k), rad[k]);
+#pragma unroll
+        for (int i = 0; i < NSHELL; i++) {
+            const bool on = profiled(p, i);
+            put_d(r, 2 * (E_E + i), on ? exp(p[P_A + i]) : 0.0);
+            put_d(r, 2 * (E_A + i), on ? p[P_A + i] / p[P_T + i] : 0.0);
+        }
+        r[A2D] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double qm = q.q;
+        double b[NBOUND];
+#pragma unroll
+        for (int k = 0; k < NBOUND; k++) {
+            const double rk = get_d(r, 2 * (E_R + k));
+            b[k] = sph_j1c_dd(qm * rk);
+            g[k] = qm != 0.0 ? (float)(FOUR_PI_3_D * cube_d(rk) * b[k]) : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < NSHELL; i++) {
+            const double e = get_d(r, 2 * (E_E + i));
+            g[NBOUND + i] = (qm != 0.0 && e != 0.0)
+                                ? (float)shell_term(qm, get_d(r, 2 * (E_R + i)), get_d(r, 2 * (E_R + i + 1)), b[i],
+                                                    b[i + 1], e, get_d(r, 2 * (E_A + i)))
+                                : 0.0f;
+        }
+    }
 };
 
 }  // namespace sas

@@ -154,6 +154,8 @@ int with_model(int id, F &&f)
     case 11: return f((sas::HollowCylinder *)nullptr);
     case 12: return f((sas::Vesicle *)nullptr);
     case 13: return f((sas::FuzzySphere *)nullptr);
+    case 17: return f((sas::CoreMultiShell *)nullptr);
+    case 18: return f((sas::Onion *)nullptr);
     default: return fail(SASHIP_EINVAL, "model " + std::to_string(id) + " has no mesh kernel");
     }
 }

@@ -12,14 +12,18 @@ path depends on (sasmodels/modelinfo.py):
 * ``polydisperse`` for volume and orientation parameters and
   ``relative_pd`` for volume parameters (:176-177);
 * ``theta_offset``, ``max_pd``, ``pd_1d``/``pd_2d`` (:470-495);
-* choice-list limits ``[[name, ...]]`` -> ``(0, len - 1)`` (:95-108).
+* choice-list limits ``[[name, ...]]`` -> ``(0, len - 1)`` (:95-108);
+* vector parameters ``id[ctl]`` (:135-180, :585-606): ``ctl``'s upper limit
+  is the length, ``ctl`` becomes a non-polydisperse control parameter, and
+  the call list holds one slot per element, ``id1`` .. ``idN`` (:634-647).
 
 The parameter tables are the reference's own (sphere.py:66-72,
 cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 core_shell_sphere.py:78-82, parallelepiped.py:216-231,
 triaxial_ellipsoid.py:143-158, bcc_paracrystal.py:186-194,
 fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hollow_cylinder.py:90-98,
-vesicle.py:87-96, fuzzy_sphere.py:113-117, hardsphere.py:84-88,
+vesicle.py:87-96, fuzzy_sphere.py:113-117, core_multi_shell.py:91-103,
+onion.py:323-333, hardsphere.py:84-88,
 stickyhardsphere.py:110-121, squarewell.py:83-92, hayter_msa.py:110-117),
 with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
@@ -46,18 +50,21 @@ class Parameter:
     """
     One model parameter: *name*, *units*, *default*, *limits*, *type*
     ('volume', 'orientation', 'sld', 'magnetic' or '') and *description*.
+
+    A vector parameter is named ``id[k]`` (fixed length k) or ``id[ctl]``,
+    whose length is the upper limit of the control parameter *ctl*; ``id``
+    drops the brackets and ``length``/``length_control`` keep the rest
+    (modelinfo.py:135-180).
     """
     def __init__(self, name, units="", default=None, limits=(-np.inf, np.inf),
                  type="", description=""):
-        self.id = name
         self.name = name
+        self.id, self.length, self.length_control = _split_vector_name(name)
         self.units = units
         self.default = default
         self.limits = tuple(limits)
         self.type = type
         self.description = description
-        self.length = 1
-        self.length_control = None
         self.is_control = False
         self.polydisperse = type in ("volume", "orientation")
         self.relative_pd = type == "volume"
@@ -67,10 +74,27 @@ class Parameter:
         return "P<%s>" % self.name
 
 
+def _split_vector_name(name):
+    """``(id, length, length_control)`` of a parameter name."""
+    if "[" not in name:
+        return name, 1, None
+    if not name.endswith("]"):
+        raise ValueError("Expected name[len] for vector parameter %s" % name)
+    pid, ref = name[:-1].split("[", 1)
+    ref = ref.strip()
+    if not ref:
+        raise ValueError("Need to specify vector length for %s" % name)
+    if ref.isdigit():
+        return pid, int(ref), None
+    # the length comes from the control parameter's upper limit once the
+    # whole table is known (ParameterTable)
+    return pid, 1, ref
+
+
 def parse_parameter(name, units, default, user_limits, ptype, description):
     """
     Build a :class:`Parameter` from a table row, checking limits and type
-    (the subset of modelinfo.py:80-181 the five models exercise).
+    (modelinfo.py:80-181).
     """
     if ptype not in ("volume", "orientation", "sld", "magnetic", ""):
         raise ValueError("unexpected type %r for %s" % (ptype, name))
@@ -87,11 +111,37 @@ def parse_parameter(name, units, default, user_limits, ptype, description):
             raise ValueError("require lower limit < upper limit")
     if default < limits[0] or default > limits[1]:
         raise ValueError("default value %r not in range for %s" % (default, name))
-    if ptype == "" and (name.startswith("sld") or name.endswith("sld")):
+    pid = _split_vector_name(name)[0]
+    if ptype == "" and (pid.startswith("sld") or pid.endswith("sld")):
         ptype = "sld"
     par = Parameter(name, units, float(default), limits, ptype, description)
     par.choices = choices
     return par
+
+
+def expand_pars(table, pars=None):
+    """
+    Defaults of *table* updated by *pars*, with a vector parameter given
+    as a whole expanded to its slots: ``thickness=[20, 10]`` sets thickness1
+    and thickness2, ``thickness=5`` sets every slot not named on its own
+    (modelinfo.py:183-225).
+    """
+    result = dict(table.defaults)
+    if not pars:
+        return result
+    vector = {p.id: p for p in table.kernel_parameters if p.length > 1}
+    scalars = {k: v for k, v in pars.items() if k not in vector}
+    for name, value in pars.items():
+        if name not in vector:
+            continue
+        if np.isscalar(value):
+            for k in range(1, vector[name].length + 1):
+                scalars.setdefault(name + str(k), value)
+        else:
+            for k, v in enumerate(value, 1):
+                scalars[name + str(k)] = v
+    result.update(scalars)
+    return result
 
 
 class ParameterTable:
@@ -101,6 +151,7 @@ class ParameterTable:
     def __init__(self, parameters, allow_magnetic=True):
         self.common_parameters = [Parameter(*p) for p in COMMON_PARAMETERS]
         self.kernel_parameters = parameters
+        self._set_vector_lengths()
         self.npars = sum(p.length for p in parameters)
         self.nmagnetic = (sum(p.length for p in parameters if p.type == "sld")
                           if allow_magnetic else 0)
@@ -130,8 +181,38 @@ class ParameterTable:
                          if p.polydisperse and p.type not in ("orientation", "magnetic"))
         self.pd_2d = set(p.name for p in self.call_parameters if p.polydisperse)
 
+    def _set_vector_lengths(self):
+        """
+        Length of each ``id[ctl]`` parameter from the upper limit of *ctl*,
+        which becomes a control parameter: an integer in [0, 20], never
+        polydisperse (modelinfo.py:585-606).
+        """
+        by_id = {p.id: p for p in self.kernel_parameters}
+        for p in self.kernel_parameters:
+            if p.length_control is None:
+                continue
+            ctl = by_id.get(p.length_control)
+            if ctl is None:
+                raise ValueError("no reference variable %r for %s" % (p.length_control, p.name))
+            low, high = ctl.limits
+            if int(low) != low or int(high) != high or low < 0 or high > 20:
+                raise ValueError("expected limits on %s to be within [0, 20]" % ctl.name)
+            ctl.is_control = True
+            ctl.polydisperse = False
+            p.length = int(high)
+
     def _get_call_parameters(self):
-        full = self.common_parameters[:] + list(self.kernel_parameters)
+        # a vector parameter takes one call slot per element, id1 .. idN,
+        # with the element's distribution flags (modelinfo.py:634-647)
+        full = self.common_parameters[:]
+        for p in self.kernel_parameters:
+            if p.length == 1:
+                full.append(p)
+                continue
+            for k in range(1, p.length + 1):
+                pk = Parameter(p.id + str(k), p.units, p.default, p.limits, p.type, p.description)
+                pk.polydisperse, pk.relative_pd, pk.choices = p.polydisperse, p.relative_pd, p.choices
+                full.append(pk)
         if self.nmagnetic > 0:
             full.extend([
                 Parameter("up_frac_i", "", 0., [0., 1.], "magnetic",
@@ -198,10 +279,28 @@ class ModelInfo:
     integration_size = 0
     #: models whose 1D kernel integrates over orientation
     has_quadrature = False
+    #: False for models the reference runs in double precision only
+    single = True
 
     def get_hidden_parameters(self, control):
-        """No hidden parameters for these models."""
-        return set()
+        """
+        Call parameters a model with one control parameter ignores at
+        control value *control*: the vector slots above it and their
+        magnetic parameters (modelinfo.py:1205-1228).  Empty without vectors.
+        """
+        table = self.parameters
+        vectors = [p for p in table.kernel_parameters if p.length > 1]
+        if not vectors:
+            return set()
+        if sum(p.is_control for p in table.kernel_parameters) != 1:
+            raise ValueError("more than one control parameter")
+        hidden = set()
+        for p in vectors:
+            for k in range(int(control) + 1, p.length + 1):
+                hidden.add(p.id + str(k))
+                if p.type == "sld":
+                    hidden.update(p.id + str(k) + s for s in ("_M0", "_mtheta", "_mphi"))
+        return hidden
 
 
 def _paracrystal(kernel_id, dnn_limits, sld, sld_solvent, angle, names):
@@ -331,6 +430,29 @@ _MODEL_TABLE = {
         ["fuzziness", "Ang", 10, [0, inf], "volume", "Std dev of the fuzzy interface"],
     ], dict(radius_effective_modes=["radius", "radius + fuzziness"], xy_mode="qa",
             has_quadrature=False)),
+    # concentric shells: n (a control parameter, integer in [0, 10]) sets how
+    # many of the ten sld/thickness slots are used
+    "core_multi_shell": (17, [
+        ["sld_core", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "Core scattering length density"],
+        ["radius", "Ang", 200., [0, inf], "volume", "Radius of the core"],
+        ["sld_solvent", "1e-6/Ang^2", 6.4, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["n", "", 1, [0, 10], "volume", "number of shells"],
+        ["sld[n]", "1e-6/Ang^2", 1.7, [-inf, inf], "sld", "scattering length density of shell k"],
+        ["thickness[n]", "Ang", 40., [0, inf], "volume", "Thickness of shell k"],
+    ], dict(radius_effective_modes=["outer radius", "core radius"], xy_mode="qa", has_quadrature=False)),
+    # shells with exponential sld profiles; FP64 like the reference (single = False)
+    "onion": (18, [
+        ["sld_core", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "Core scattering length density"],
+        ["radius_core", "Ang", 200., [0, inf], "volume", "Radius of the core"],
+        ["sld_solvent", "1e-6/Ang^2", 6.4, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["n_shells", "", 1, [0, 10], "volume", "number of shells (must be integer)"],
+        ["sld_in[n_shells]", "1e-6/Ang^2", 1.7, [-inf, inf], "sld",
+         "scattering length density at the inner radius of shell k"],
+        ["sld_out[n_shells]", "1e-6/Ang^2", 2.0, [-inf, inf], "sld",
+         "scattering length density at the outer radius of shell k"],
+        ["thickness[n_shells]", "Ang", 40., [0, inf], "volume", "Thickness of shell k"],
+        ["A[n_shells]", "", 1.0, [-inf, inf], "", "Decay rate of shell k"],
+    ], dict(radius_effective_modes=["outer radius"], xy_mode="qa", has_quadrature=False, single=False)),
     # structure factor: Iq only (no F), no volume, nothing polydisperse;
     # evaluated in FP64 (the reference marks it double precision only)
     "hardsphere": (5, [
