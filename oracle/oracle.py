@@ -29,7 +29,8 @@ MODEL_IDS = {"sphere": 0, "cylinder": 1, "ellipsoid": 2,
              "parallelepiped": 6, "triaxial_ellipsoid": 7,
              "bcc_paracrystal": 8, "fcc_paracrystal": 9, "sc_paracrystal": 10,
              "hollow_cylinder": 11, "vesicle": 12, "fuzzy_sphere": 13,
-             "core_multi_shell": 17, "onion": 18}
+             "core_multi_shell": 17, "onion": 18, "core_shell_ellipsoid": 19,
+             "elliptical_cylinder": 20, "core_shell_bicelle": 21}
 
 _lib = None
 _D = ct.c_double

@@ -16,7 +16,9 @@
  *                       parallelepiped, triaxial_ellipsoid, bcc, fcc and
  *                       sc paracrystal (with lib/gauss150.c), hollow_cylinder,
  *                       vesicle, fuzzy_sphere, stickyhardsphere, squarewell,
- *                       hayter_msa, core_multi_shell (18), onion (19)
+ *                       hayter_msa, core_multi_shell (18), onion (19),
+ *                       core_shell_ellipsoid (20), elliptical_cylinder (21),
+ *                       core_shell_bicelle (22)
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -167,8 +169,43 @@ double ref_sq6(double q, double r, double phi, double a, double b, double c, dou
     Fq(q, f1, f2, p[0], p[1], p[2], p[3], (double *)p + 4, (double *)p + 14, (double *)p + 24, (double *)p + 34)
 #  define NPAR_VOLUME(p) form_volume(p[1], p[3], (double *)p + 24)
 #  define NPAR_REFF(m, p) radius_effective(m, p[1], p[3], (double *)p + 24)
+#elif REF_MODEL >= 20 && REF_MODEL <= 22
+#  if REF_MODEL == 20
+#    include "lib/sas_3j1x_x.c"
+#  else
+#    if REF_MODEL == 22
+#      include "lib/sas_Si.c"
+#    endif
+#    include "lib/polevl.c"
+#    include "lib/sas_J1.c"
+#  endif
+#  ifdef REF_FIXED76
+#    include "lib/gauss76.c"
+#    include "lib/nonadaptive.c"
+#  else
+#    include "lib/adaptive.c"
+#  endif
+#  if REF_MODEL == 20
+#    include "core_shell_ellipsoid.c"
+#    define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4], p[5], p[6])
+#    define NPAR_IQAC(a, c, p) Iqac(a, c, p[0], p[1], p[2], p[3], p[4], p[5], p[6])
+#    define NPAR_VOLUME(p) form_volume(p[0], p[1], p[2], p[3])
+#    define NPAR_REFF(m, p) radius_effective(m, p[0], p[1], p[2], p[3])
+#  elif REF_MODEL == 21
+#    include "elliptical_cylinder.c"
+#    define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4])
+#    define NPAR_IQABC(a, b, c, p) Iqabc(a, b, c, p[0], p[1], p[2], p[3], p[4])
+#    define NPAR_VOLUME(p) form_volume(p[0], p[1], p[2])
+#    define NPAR_REFF(m, p) radius_effective(m, p[0], p[1], p[2])
+#  else
+#    include "core_shell_bicelle.c"
+#    define NPAR_CALL_FQ(q, f1, f2, p) Fq(q, f1, f2, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7])
+#    define NPAR_IQAC(a, c, p) Iqac(a, c, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7])
+#    define NPAR_VOLUME(p) form_volume(p[0], p[1], p[2], p[3])
+#    define NPAR_REFF(m, p) radius_effective(m, p[0], p[1], p[2], p[3])
+#  endif
 #else
-#  error "REF_MODEL must be 1..19"
+#  error "REF_MODEL must be 1..22"
 #endif
 
 #ifdef NPAR_CALL_FQ

@@ -30,7 +30,9 @@
  *     hollow_cylinder.c, vesicle.c, fuzzy_sphere.c (shell volumes of the
  *     hollow shapes in the volume sums, kernel_iq.c:728-731),
  *     core_multi_shell.c and onion.c (vector parameters: n shells of
- *     sld[n], thickness[n], ... packed as ten slots each);
+ *     sld[n], thickness[n], ... packed as ten slots each),
+ *     core_shell_ellipsoid.c, elliptical_cylinder.c (nested rules),
+ *     core_shell_bicelle.c;
  *     view/jitter rotation with psi qabc_rotation/qabc_apply
  *     kernel_iq.c:251-310
  *   - hard-sphere structure factor hardsphere.c:1-90
@@ -771,6 +773,183 @@ static void fuz_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
     *f2 = 1.0e-4 * form * form;
 }
 
+/* ---- core_shell_ellipsoid (core_shell_ellipsoid.c:1-170): p = radius_equat_core,
+ * x_core, thick_shell, x_polar_shell, sld_core, sld_shell, sld_solvent, theta, phi.
+ * Core radii (re, re x_core); shell radii (re + t, re x_core + t x_polar_shell). ---- */
+static oreal cse_amp(oreal qab, oreal qc, oreal re_c, oreal rp_c, oreal re_s, oreal rp_s,
+                     oreal d_core_shell, oreal d_shell_solvent)
+{
+    const oreal x_c = sqrt(sq(re_c * qab) + sq(rp_c * qc));
+    const oreal x_s = sqrt(sq(re_s * qab) + sq(rp_s * qc));
+    return o_sph_j1c(x_c) * (O_4PI_3 * re_c * re_c * rp_c) * d_core_shell
+         + o_sph_j1c(x_s) * (O_4PI_3 * re_s * re_s * rp_s) * d_shell_solvent;
+}
+static oreal cse_vol(const oreal *p)
+{
+    const oreal re_s = p[0] + p[2];
+    const oreal rp_s = p[0] * p[1] + p[2] * p[3];
+    return O_4PI_3 * re_s * re_s * rp_s;
+}
+static oreal cse_reff(int mode, const oreal *p)
+{
+    const oreal re = p[0] + p[2], rp = p[0] * p[1] + p[2] * p[3];
+    switch (mode) {
+    default:
+    case 1: {
+        if (1.0 == p[1] && 1.0 == p[3]) return re;
+        if (re * rp == 0.) return 0.;
+        /* Isihara (1950) eq. 26 for the outer spheroid */
+        const oreal ratio = (rp < re ? rp / re : re / rp);
+        const oreal e1 = sqrt(1.0 - ratio * ratio);
+        const oreal b1 = 1.0 + asin(e1) / (e1 * ratio);
+        const oreal bL = (1.0 + e1) / (1.0 - e1);
+        const oreal b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
+        const oreal delta = 0.75 * b1 * b2;
+        return 0.5 * cbrt(2.0 * (delta + 1.0) * rp * re * re);
+    }
+    case 2: return cbrt(cse_vol(p) / O_4PI_3);
+    case 3: return (rp < re ? rp : re);
+    case 4: return (rp > re ? rp : re);
+    }
+}
+static void cse_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    const oreal d1 = p[4] - p[5], d2 = p[5] - p[6];
+    const oreal rp_c = p[0] * p[1], re_s = p[0] + p[2], rp_s = p[0] * p[1] + p[2] * p[3];
+    const oreal *z, *w;
+    const int n = quad_select(quad_n, q * fmax(re_s, rp_s), &z, &w);
+    oreal t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        const oreal u = z[i] * 0.5 + 0.5;
+        const oreal s = sqrt(1.0 - u * u);
+        const oreal f = cse_amp(q * s, q * u, p[0], rp_c, re_s, rp_s, d1, d2);
+        t1 += w[i] * f;
+        t2 += w[i] * f * f;
+    }
+    *f1 = 1.0e-2 * (t1 * 0.5);
+    *f2 = 1.0e-4 * (t2 * 0.5);
+}
+static oreal cse_iqac(oreal qab, oreal qc, const oreal *p)
+{
+    const oreal f = cse_amp(qab, qc, p[0], p[0] * p[1], p[0] + p[2], p[0] * p[1] + p[2] * p[3],
+                            p[4] - p[5], p[5] - p[6]);
+    return 1.0e-4 * f * f;
+}
+
+/* ---- elliptical_cylinder (elliptical_cylinder.c:1-150): p = radius_minor,
+ * axis_ratio, length, sld, sld_solvent, theta, phi, psi.  1D: nested rules,
+ * outer cos(alpha) = (z + 1)/2 on [0, 1], inner angle (z + 1) pi/2 on [0, pi]
+ * around the elliptical cross-section. ---- */
+static oreal ecy_vol(const oreal *p) { return O_PI * p[0] * p[0] * p[1] * p[2]; }
+static oreal ecy_reff(int mode, const oreal *p)
+{
+    const oreal rm = p[0], ratio = p[1], l = p[2];
+    const oreal rmin = ratio > 1.0 ? rm : ratio * rm, rmax = ratio < 1.0 ? rm : ratio * rm;
+    switch (mode) {
+    default:
+    case 1: {
+        const oreal r = sqrt(rm * rm * ratio);
+        return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (O_PI * r + l)));
+    }
+    case 2: return cbrt(ecy_vol(p) / O_4PI_3);
+    case 3: return 0.5 * rm * (1.0 + ratio);
+    case 4: return rmin;
+    case 5: return rmax;
+    case 6: return sqrt(rm * rm * ratio);
+    case 7: return 0.5 * l;
+    case 8: return (rmin < 0.5 * l ? rmin : 0.5 * l);
+    case 9: return (rmax > 0.5 * l ? rmax : 0.5 * l);
+    case 10: {
+        const oreal r = ratio > 1.0 ? rm * ratio : rm;
+        return sqrt(r * r + 0.25 * l * l);
+    }
+    }
+}
+static void ecy_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    const oreal rm = p[0], rmaj = p[1] * p[0], l = p[2];
+    const oreal ra = 0.5 * (sq(rmaj) + sq(rm)), rb = 0.5 * (sq(rmaj) - sq(rm));
+    const oreal *zo, *wo, *zi, *wi;
+    const int no = quad_select_nested(quad_n, q * fmax(0.5 * l, fmax(rm, rm * p[1])), O_ADAPTIVE_MAX_OUTER,
+                                      &zo, &wo);
+    const int ni = quad_select_nested(quad_n, q * fmax(rm, rmaj), no, &zi, &wi);
+    oreal o1 = 0.0, o2 = 0.0;
+    for (int i = 0; i < no; i++) {
+        const oreal u = (zo[i] + 1.0) / 2.0;
+        const oreal s = sqrt(1.0 - u * u);
+        oreal i1 = 0.0, i2 = 0.0;
+        for (int j = 0; j < ni; j++) {
+            const oreal th = (zi[j] * O_PI + O_PI) / 2.0;
+            const oreal be = o_2j1x_x(q * (s * sqrt(ra - rb * cos(th))));
+            i1 += wi[j] * be;
+            i2 += wi[j] * be * be;
+        }
+        i1 *= 0.5 * O_PI;
+        i2 *= 0.5 * O_PI;
+        const oreal si = o_sinx_x(q * 0.5 * l * u);
+        o1 += wo[i] * i1 * si;
+        o2 += wo[i] * i2 * si * si;
+    }
+    o1 *= 0.5 / O_PI;
+    o2 *= 0.5 / O_PI;
+    const oreal sv = (p[3] - p[4]) * ecy_vol(p);
+    *f1 = 1.0e-2 * sv * o1;
+    *f2 = 1.0e-4 * sv * sv * o2;
+}
+static oreal ecy_iqabc(oreal qa, oreal qb, oreal qc, const oreal *p)
+{
+    const oreal be = o_2j1x_x(p[0] * sqrt(sq(p[1] * qb) + sq(qa)));
+    const oreal si = o_sinx_x(qc * 0.5 * p[2]);
+    return 1.0e-4 * sq((p[3] - p[4]) * ecy_vol(p) * (be * si));
+}
+
+/* ---- core_shell_bicelle (core_shell_bicelle.c:1-160): p = radius, thick_rim,
+ * thick_face, length, sld_core, sld_face, sld_rim, sld_solvent, theta, phi.
+ * A core cylinder capped by face layers and wrapped by a rim. ---- */
+static oreal csb_vol(const oreal *p) { return O_PI * sq(p[0] + p[1]) * (p[3] + 2.0 * p[2]); }
+static oreal csb_reff(int mode, const oreal *p)
+{
+    const oreal r = p[0] + p[1], l = p[3] + 2.0 * p[2];
+    switch (mode) {
+    default:
+    case 1: return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (O_PI * r + l)));
+    case 2: return cbrt(csb_vol(p) / O_4PI_3);
+    case 3: return p[0] + p[1];
+    case 4: return 0.5 * p[3] + p[2];
+    case 5: return sqrt(r * r + 0.25 * l * l);
+    }
+}
+static oreal csb_amp(oreal qab, oreal qc, const oreal *p)
+{
+    const oreal r = p[0], tr = p[1], tf = p[2], h = 0.5 * p[3];
+    const oreal be1 = o_2j1x_x(r * qab), be2 = o_2j1x_x((r + tr) * qab);
+    const oreal si1 = o_sinx_x(h * qc), si2 = o_sinx_x((h + tf) * qc);
+    const oreal v1 = O_PI * sq(r) * 2.0 * h;
+    const oreal v2 = O_PI * sq(r + tr) * 2.0 * (h + tf);
+    const oreal v3 = O_PI * sq(r) * 2.0 * (h + tf);
+    return v1 * (p[4] - p[5]) * si1 * be1 + v2 * (p[6] - p[7]) * si2 * be2 + v3 * (p[5] - p[6]) * si2 * be1;
+}
+static void csb_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    const oreal *z, *w;
+    const int n = quad_select(quad_n, q * fmax(p[0] + p[1], p[3] + p[2]), &z, &w);
+    oreal t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        const oreal th = (z[i] + 1.0) * O_PI_4;
+        const oreal s = sin(th), c = cos(th);
+        const oreal f = csb_amp(q * s, q * c, p);
+        t1 += w[i] * f * s;
+        t2 += w[i] * f * f * s;
+    }
+    *f1 = 1.0e-2 * (t1 * O_PI_4);
+    *f2 = 1.0e-4 * (t2 * O_PI_4);
+}
+static oreal csb_iqac(oreal qab, oreal qc, const oreal *p)
+{
+    const oreal f = csb_amp(qab, qc, p);
+    return 1.0e-4 * f * f;
+}
+
 /* ---- shells of vector parameters: n = (int)(fp_n + 0.5) of a control
  * parameter with limits [0, 10] (core_multi_shell.c:12-18, onion.c:31-38) ---- */
 #define O_NSHELL 10
@@ -883,9 +1062,10 @@ static int always_valid(const oreal *p) { (void)p; return 1; }
 /* model ids: 0 sphere, 1 cylinder, 2 ellipsoid, 3 core_shell_cylinder,
  * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid, 8-10 bcc,
  * fcc, sc paracrystal, 11 hollow_cylinder, 12 vesicle, 13 fuzzy_sphere,
- * 17 core_multi_shell, 18 onion (the ids libsaship uses; 5 and 14-16, the
+ * 17 core_multi_shell, 18 onion, 19 core_shell_ellipsoid, 20
+ * elliptical_cylinder, 21 core_shell_bicelle (the ids libsaship uses; 5 and 14-16, the
  * structure factors, have entries of their own) */
-#define N_MODELS 19
+#define N_MODELS 22
 static const OModel MODELS[N_MODELS] = {
     {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0, 0, 0},
     {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0, 0, 0},
@@ -908,6 +1088,9 @@ static const OModel MODELS[N_MODELS] = {
      0, 0, 0, 0},
     {44, 22, 5, 0, {0, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23},
      always_valid, oni_vol, oni_reff, oni_fq, 0, 0, 0, 0},
+    {9, 3, 5, 1, {4, 5, 6}, always_valid, cse_vol, cse_reff, cse_fq, cse_iqac, 0, 0, 0},
+    {8, 2, 5, 1, {3, 4}, always_valid, ecy_vol, ecy_reff, ecy_fq, 0, ecy_iqabc, 0, 0},
+    {10, 4, 5, 1, {4, 5, 6, 7}, always_valid, csb_vol, csb_reff, csb_fq, csb_iqac, 0, 0, 0},
 };
 static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npars > 0; }
 

@@ -527,6 +527,9 @@ int record_width(int id, int kind)
     case 12: return rv_of<sas::Vesicle>(kind);
     case 17: return rv_of<sas::CoreMultiShell>(kind);
     case 18: return rv_of<sas::Onion>(kind);
+    case 19: return rv_of<sas::CoreShellEllipsoid>(kind);
+    case 20: return rv_of<sas::EllipticalCylinder>(kind);
+    case 21: return rv_of<sas::CoreShellBicelle>(kind);
     default: return rv_of<sas::FuzzySphere>(kind);
     }
 }
@@ -788,6 +791,9 @@ int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, 
     case 12: return enqueue<sas::Vesicle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 17: return enqueue<sas::CoreMultiShell>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 18: return enqueue<sas::Onion>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 19: return enqueue<sas::CoreShellEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 20: return enqueue<sas::EllipticalCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 21: return enqueue<sas::CoreShellBicelle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     default: return enqueue<sas::FuzzySphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }

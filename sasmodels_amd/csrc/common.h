@@ -81,30 +81,31 @@ inline int cu_count(int dev)
 // hard-sphere, sticky hard-sphere, square-well and Hayter-Penfold MSA
 // structure factors (radius_effective, volfraction and up to four more
 // parameters, nothing polydisperse, no sld: no magnetic values).  17
-// (core_multi_shell) and 18 (onion) have vector parameters, ten slots each.
+// (core_multi_shell) and 18 (onion) have vector parameters, ten slots each;
+// 19-21 are core_shell_ellipsoid, elliptical_cylinder, core_shell_bicelle.
 constexpr int MODEL_HARDSPHERE = 5;
-constexpr int MODEL_LAST = 18;
+constexpr int MODEL_LAST = 21;
 inline int model_npars(int id)
 {
-    static const int np[19] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6, 24, 44};
+    static const int np[22] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6, 24, 44, 9, 8, 10};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[19] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0, 5, 5};
+    static const int mp[22] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0, 5, 5, 5, 5, 5};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[19] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8, 66, 116};
+    static const int nv[22] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8, 66, 116, 24, 20, 28};
     return nv[id];
 }
-inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || (id >= 6 && id <= 11); }
+inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || (id >= 6 && id <= 11) || (id >= 19 && id <= 21); }
 // orientation (theta, phi, psi): view and jitter about three axes
-inline bool model_abc(int id) { return id >= 6 && id <= 10; }
+inline bool model_abc(int id) { return (id >= 6 && id <= 10) || id == 20; }
 // bcc, fcc, sc paracrystals: FP64 product rule (Gauss-150 by default)
 inline bool model_paracrystal(int id) { return id >= 8 && id <= 10; }
-inline bool model_nested(int id) { return id == 6 || id == 7; }
+inline bool model_nested(int id) { return id == 6 || id == 7 || id == 20; }
 inline bool model_structure(int id) { return id == MODEL_HARDSPHERE || id == 14 || id == 15 || id == 16; }
 
 // Separable cylinder 1D meshes: only the radius (parameter 2) and length

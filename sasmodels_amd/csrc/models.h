@@ -21,7 +21,8 @@
 // with lib/core_shell.c:8-30, parallelepiped.c:1-150,
 // triaxial_ellipsoid.c:1-160, bcc/fcc/sc_paracrystal.c with
 // lib/sphere_form.c, hollow_cylinder.c, vesicle.c, fuzzy_sphere.c,
-// core_multi_shell.c, onion.c.  Parameter order: sphere.py:66-72,
+// core_multi_shell.c, onion.c, core_shell_ellipsoid.c, elliptical_cylinder.c,
+// core_shell_bicelle.c.  Parameter order: sphere.py:66-72,
 // cylinder.py:135-142, ellipsoid.py:145-152, core_shell_cylinder.py:118-134,
 // core_shell_sphere.py:78-82.
 #pragma once
@@ -1134,6 +1135,390 @@ struct FuzzySphere {
         g[0] = r[4] * sph_j1c_d(q.q * get_d(r, 0)) * (float)exp(-0.5 * qf * qf);
     }
     SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- core_shell_ellipsoid
+// p = radius_equat_core, x_core, thick_shell, x_polar_shell, sld_core,
+// sld_shell, sld_solvent, theta, phi (core_shell_ellipsoid.py:139-149;
+// core_shell_ellipsoid.c:1-170).  Core radii (re, re x_core), shell radii
+// (re + t, re x_core + t x_polar_shell).  1D: u = cos(theta) = (z + 1)/2
+// (table a = u^2, plain weights); the argument of each spheroid is
+// q sqrt(re^2 + (rp^2 - re^2) u^2).
+struct CoreShellEllipsoid {
+    static constexpr int ID = 19, NPARS = 9, NMAG = 3, MAX_PD = 5, NCON = 2;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = false;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = false;
+    static constexpr int REC1D = 9, REC2D = 11;
+    static constexpr int A2D = 10;
+    SAS_HD static int sld_param(int k) { return 4 + k; }   // core, shell, solvent
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static void radii(const double *p, double &re_c, double &rp_c, double &re_s, double &rp_s)
+    {
+        re_c = p[0];
+        rp_c = p[0] * p[1];
+        re_s = p[0] + p[2];
+        rp_s = p[0] * p[1] + p[2] * p[3];
+    }
+    __device__ static double volume(const double *p)
+    {
+        const double re_s = p[0] + p[2], rp_s = p[0] * p[1] + p[2] * p[3];
+        return FOUR_PI_3_D * re_s * re_s * rp_s;
+    }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double re = p[0] + p[2], rp = p[0] * p[1] + p[2] * p[3];
+        switch (mode) {
+        default:
+        case 1: {
+            if (1.0 == p[1] && 1.0 == p[3]) return re;
+            if (re * rp == 0.) return 0.;
+            // mean curvature radius of the outer spheroid (Isihara 1950, eq. 26)
+            const double ratio = rp < re ? rp / re : re / rp;
+            const double e1 = sqrt(1.0 - ratio * ratio);
+            const double b1 = 1.0 + asin(e1) / (e1 * ratio);
+            const double bL = (1.0 + e1) / (1.0 - e1);
+            const double b2 = 1.0 + 0.5 * ratio * ratio / e1 * log(bL);
+            const double delta = 0.75 * b1 * b2;
+            return 0.5 * cbrt(2.0 * (delta + 1.0) * rp * re * re);
+        }
+        case 2: return cbrt(volume(p) / FOUR_PI_3_D);
+        case 3: return rp < re ? rp : re;
+        case 4: return rp > re ? rp : re;
+        }
+    }
+    // 1D record: re_c^2, rp_c^2 - re_c^2, re_s^2, rp_s^2 - re_s^2,
+    // V_core (core - shell), V_shell (shell - solvent), w*1e-2/2, w*1e-4/2, max radius
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        double re_c, rp_c, re_s, rp_s;
+        radii(p, re_c, rp_c, re_s, rp_s);
+        r[0] = (float)(re_c * re_c);
+        r[1] = (float)(rp_c * rp_c - re_c * re_c);
+        r[2] = (float)(re_s * re_s);
+        r[3] = (float)(rp_s * rp_s - re_s * re_s);
+        r[4] = (float)(FOUR_PI_3_D * re_c * re_c * rp_c * (p[4] - p[5]));
+        r[5] = (float)(FOUR_PI_3_D * re_s * re_s * rp_s * (p[5] - p[6]));
+        r[6] = (float)(w * 1e-2 * 0.5);
+        r[7] = (float)(w * 1e-4 * 0.5);
+        r[8] = (float)fmax(re_s, rp_s);
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int off, n;
+        quad_range<FIXED>(t, q * r[8], off, n);
+        float a1 = 0.0f, a2 = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float u2 = t.a[off + i];
+            const float xc = q * sqrt_f(fmaf(r[1], u2, r[0]));
+            const float xs = q * sqrt_f(fmaf(r[3], u2, r[2]));
+            const float f = fmaf(r[4], sph_j1c_f(xc), r[5] * sph_j1c_f(xs));
+            const float wf = t.w[off + i] * f;
+            a1 += wf;
+            a2 = fmaf(wf, f, a2);
+        }
+        f1 = r[6] * a1;
+        f2 = r[7] * a2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        double re_c, rp_c, re_s, rp_s;
+        radii(p, re_c, rp_c, re_s, rp_s);
+        put_d(r, 0, re_c * re_c);
+        put_d(r, 2, rp_c * rp_c);
+        put_d(r, 4, re_s * re_s);
+        put_d(r, 6, rp_s * rp_s);
+        r[8] = (float)(FOUR_PI_3_D * re_c * re_c * rp_c);
+        r[9] = (float)(FOUR_PI_3_D * re_s * re_s * rp_s);
+        r[10] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double ab = q.qa * q.qa + q.qb * q.qb, c = q.qc * q.qc;
+        g[0] = r[8] * sph_j1c_d(sqrt(get_d(r, 0) * ab + get_d(r, 2) * c));
+        g[1] = r[9] * sph_j1c_d(sqrt(get_d(r, 4) * ab + get_d(r, 6) * c));
+    }
+    // FP32 record after the rotation rows: (re_c, rp_c, re_s, rp_s)/2pi squared,
+    // V_core, V_shell, w*1e-4, largest radius
+    static constexpr int REC2F = 8, A2F = 12, XS2F = 13;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        double re_c, rp_c, re_s, rp_s;
+        radii(p, re_c, rp_c, re_s, rp_s);
+        const double k = INV_TWO_PI_D * INV_TWO_PI_D;
+        f[6] = (float)(re_c * re_c * k);
+        f[7] = (float)(rp_c * rp_c * k);
+        f[8] = (float)(re_s * re_s * k);
+        f[9] = (float)(rp_s * rp_s * k);
+        f[10] = (float)(FOUR_PI_3_D * re_c * re_c * rp_c);
+        f[11] = (float)(FOUR_PI_3_D * re_s * re_s * rp_s);
+        f[12] = (float)(w * 1e-4);
+        f[13] = (float)fmax(fmax(re_c, rp_c), fmax(re_s, rp_s));
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float ab = fmaf(q.qa, q.qa, q.qb * q.qb), c = q.qc * q.qc;
+        g[0] = f[10] * sph_j1c_rev2(fmaf(f[6], ab, f[7] * c));
+        g[1] = f[11] * sph_j1c_rev2(fmaf(f[8], ab, f[9] * c));
+    }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con)
+    {
+        con[0] = (float)(sld[0] - sld[1]);
+        con[1] = (float)(sld[1] - sld[2]);
+    }
+};
+
+// ---------------------------------------------------------------- elliptical_cylinder
+// p = radius_minor, axis_ratio, length, sld, sld_solvent, theta, phi, psi
+// (elliptical_cylinder.py:115-122; elliptical_cylinder.c:1-150).  1D: nested
+// rules, outer u = cos(alpha) = (z + 1)/2 (tables c = u, d = sqrt(1 - u^2)),
+// inner angle theta = (z + 1) pi/2 around the cross-section (tables
+// a = cos^2(theta/2), b = sin^2(theta/2)), plain weights; the cross-section
+// radius at theta is sqrt(r_minor^2 a + r_major^2 b), which is the
+// reference's sqrt((r_major^2 + r_minor^2)/2 - (r_major^2 - r_minor^2)/2 cos theta)
+// without its cancellation.  One inner rule serves every outer node.
+struct EllipticalCylinder {
+    static constexpr int ID = 20, NPARS = 8, NMAG = 2, MAX_PD = 5, NCON = 1;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = true;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = false;
+    static constexpr int REC1D = 8, REC2D = 8;
+    static constexpr int A2D = 7;
+    SAS_HD static int sld_param(int k) { return 3 + k; }
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p) { return PI_D * p[0] * p[0] * p[1] * p[2]; }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double rm = p[0], ratio = p[1], l = p[2];
+        const double rmin = ratio > 1.0 ? rm : ratio * rm, rmax = ratio < 1.0 ? rm : ratio * rm;
+        switch (mode) {
+        default:
+        case 1: {
+            const double r = sqrt(rm * rm * ratio);
+            return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (PI_D * r + l)));
+        }
+        case 2: return cbrt(volume(p) / FOUR_PI_3_D);
+        case 3: return 0.5 * rm * (1.0 + ratio);
+        case 4: return rmin;
+        case 5: return rmax;
+        case 6: return sqrt(rm * rm * ratio);
+        case 7: return 0.5 * l;
+        case 8: return rmin < 0.5 * l ? rmin : 0.5 * l;
+        case 9: return rmax > 0.5 * l ? rmax : 0.5 * l;
+        case 10: {
+            const double r = ratio > 1.0 ? rm * ratio : rm;
+            return sqrt(r * r + 0.25 * l * l);
+        }
+        }
+    }
+    // 1D record: r_minor, axis_ratio^2, L/2, outer qr scale, inner qr scale,
+    // w*1e-2 s/4, w*1e-4 s^2/4 with s = (sld - solvent) V
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        const double s = (p[3] - p[4]) * volume(p);
+        const double rmaj = p[1] * p[0];
+        r[0] = (float)p[0];
+        r[1] = (float)(p[1] * p[1]);
+        r[2] = (float)(0.5 * p[2]);
+        r[3] = (float)fmax(0.5 * p[2], fmax(p[0], rmaj));
+        r[4] = (float)fmax(p[0], rmaj);
+        r[5] = (float)(w * 1e-2 * s * 0.25);
+        r[6] = (float)(w * 1e-4 * s * s * 0.25);
+        r[7] = 0.0f;
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int lo, oo, no, li, io, ni;
+        quad_nested<FIXED>(t, q * r[3], 2, lo, oo, no);
+        quad_nested<FIXED>(t, q * r[4], 3 - lo, li, io, ni);
+        const float qr = q * r[0], qh = q * r[2];
+        float o1 = 0.0f, o2 = 0.0f;
+        for (int i = 0; i < no; i++) {
+            const float qs = qr * t.d[oo + i];
+            float i1 = 0.0f, i2 = 0.0f;
+            for (int j = 0; j < ni; j++) {
+                const float be = j1c_f(qs * sqrt_f(fmaf(r[1], t.b[io + j], t.a[io + j])));
+                const float wb = t.w[io + j] * be;
+                i1 += wb;
+                i2 = fmaf(wb, be, i2);
+            }
+            const float si = sinc_prod(qh, t.c[oo + i]);
+            const float ws = t.w[oo + i] * si;
+            o1 = fmaf(ws, i1, o1);
+            o2 = fmaf(ws * si, i2, o2);
+        }
+        f1 = r[5] * o1;
+        f2 = r[6] * o2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        put_d(r, 0, p[0]);
+        put_d(r, 2, p[1]);
+        put_d(r, 4, 0.5 * p[2]);
+        r[6] = (float)volume(p);
+        r[7] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double k = get_d(r, 2) * q.qb;
+        g[0] = r[6] * j1c_d(get_d(r, 0) * sqrt(k * k + q.qa * q.qa)) * sinc_d(q.qc * get_d(r, 4));
+    }
+    // FP32 record after the rotation rows: r_minor/2pi, axis_ratio, L/(4 pi), V,
+    // w*1e-4, largest length
+    static constexpr int REC2F = 6, A2F = 10, XS2F = 11;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        f[6] = (float)(p[0] * INV_TWO_PI_D);
+        f[7] = (float)p[1];
+        f[8] = (float)(0.5 * p[2] * INV_TWO_PI_D);
+        f[9] = (float)volume(p);
+        f[10] = (float)(w * 1e-4);
+        f[11] = (float)fmax(0.5 * p[2], p[0] * fmax(p[1], 1.0));
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float k = f[7] * q.qb;
+        g[0] = f[9] * j1c_rev(f[6] * sqrt_f(fmaf(k, k, q.qa * q.qa))) * sinc_rev(q.qc * f[8]);
+    }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con) { con[0] = (float)(sld[0] - sld[1]); }
+};
+
+// ---------------------------------------------------------------- core_shell_bicelle
+// p = radius, thick_rim, thick_face, length, sld_core, sld_face, sld_rim,
+// sld_solvent, theta, phi (core_shell_bicelle.py:140-151; core_shell_bicelle.c:1-160).
+// F = V1 (core - face) S(h) B(R) + V2 (rim - solvent) S(h + tf) B(R + tr)
+//   + V3 (face - rim) S(h + tf) B(R), S = sinc(qc .), B = 2 J1(qab .)/(qab .),
+// h = L/2, V1 = 2 pi R^2 h, V2 = 2 pi (R + tr)^2 (h + tf), V3 = 2 pi R^2 (h + tf).
+// 1D: theta = (z + 1) pi/4 (tables a = sin, b = cos, w sin: the cylinder family's).
+struct CoreShellBicelle {
+    static constexpr int ID = 21, NPARS = 10, NMAG = 4, MAX_PD = 5, NCON = 3;
+    static constexpr bool SHARED_1D = false;
+    static constexpr bool ORIENTED = true;
+    static constexpr bool ABC = false;
+    static constexpr bool FQ = true;
+    static constexpr bool FP64 = false;
+    static constexpr bool HOLLOW = false;
+    static constexpr int REC1D = 10, REC2D = 12;
+    static constexpr int A2D = 11;
+    SAS_HD static int sld_param(int k) { return 4 + k; }   // core, face, rim, solvent
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static double volume(const double *p)
+    {
+        const double ro = p[0] + p[1];
+        return PI_D * ro * ro * (p[3] + 2.0 * p[2]);
+    }
+    __device__ static double reff(int mode, const double *p)
+    {
+        const double r = p[0] + p[1], l = p[3] + 2.0 * p[2];
+        switch (mode) {
+        default:
+        case 1: return 0.5 * cbrt(0.75 * r * (2.0 * r * l + (r + l) * (PI_D * r + l)));
+        case 2: return cbrt(volume(p) / FOUR_PI_3_D);
+        case 3: return p[0] + p[1];
+        case 4: return 0.5 * p[3] + p[2];
+        case 5: return sqrt(r * r + 0.25 * l * l);
+        }
+    }
+    __device__ static void vols(const double *p, double &v1, double &v2, double &v3)
+    {
+        const double r = p[0], h = 0.5 * p[3];
+        v1 = PI_D * r * r * 2.0 * h;
+        v2 = PI_D * (r + p[1]) * (r + p[1]) * 2.0 * (h + p[2]);
+        v3 = PI_D * r * r * 2.0 * (h + p[2]);
+    }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+        double v1, v2, v3;
+        vols(p, v1, v2, v3);
+        r[0] = (float)p[0];
+        r[1] = (float)(0.5 * p[3]);
+        r[2] = (float)(p[0] + p[1]);
+        r[3] = (float)(0.5 * p[3] + p[2]);
+        r[4] = (float)(v1 * (p[4] - p[5]));
+        r[5] = (float)(v2 * (p[6] - p[7]));
+        r[6] = (float)(v3 * (p[5] - p[6]));
+        r[7] = (float)(w * 1e-2 * PI_4_D);
+        r[8] = (float)(w * 1e-4 * PI_4_D);
+        r[9] = (float)fmax(p[0] + p[1], p[3] + p[2]);    // the reference's q r_max
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float q, const float *r, const Quad &t, float &f1, float &f2)
+    {
+        int off, n;
+        quad_range<FIXED>(t, q * r[9], off, n);
+        const float qr1 = q * r[0], qh1 = q * r[1], qr2 = q * r[2], qh2 = q * r[3];
+        float a1 = 0.0f, a2 = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float s = t.a[off + i], c = t.b[off + i];
+            const float b1 = j1c_f(qr1 * s), s2 = sinc_prod(qh2, c);
+            const float f = r[4] * sinc_prod(qh1, c) * b1 + s2 * fmaf(r[5], j1c_f(qr2 * s), r[6] * b1);
+            const float wf = t.w[off + i] * f;
+            a1 += wf;
+            a2 = fmaf(wf, f, a2);
+        }
+        f1 = r[7] * a1;
+        f2 = r[8] * a2;
+    }
+    __device__ static void rec2d(const double *p, double w, float *r)
+    {
+        double v1, v2, v3;
+        vols(p, v1, v2, v3);
+        put_d(r, 0, p[0]);
+        put_d(r, 2, 0.5 * p[3]);
+        put_d(r, 4, p[0] + p[1]);
+        put_d(r, 6, 0.5 * p[3] + p[2]);
+        r[8] = (float)v1;
+        r[9] = (float)v2;
+        r[10] = (float)v3;
+        r[11] = (float)(w * 1e-4);
+    }
+    __device__ static void geom2d(const QRot &q, const float *r, float *g)
+    {
+        const double qab = sqrt(q.qa * q.qa + q.qb * q.qb);
+        const float b1 = j1c_d(qab * get_d(r, 0)), s2 = sinc_d(q.qc * get_d(r, 6));
+        g[0] = r[8] * sinc_d(q.qc * get_d(r, 2)) * b1;
+        g[1] = r[9] * s2 * j1c_d(qab * get_d(r, 4));
+        g[2] = r[10] * s2 * b1;
+    }
+    // FP32 record after the rotation rows: R, h, R + tr, h + tf (each / 2pi),
+    // V1, V2, V3, w*1e-4, largest length
+    static constexpr int REC2F = 9, A2F = 13, XS2F = 14;
+    __device__ static void rec2f(const double *p, double w, float *f)
+    {
+        double v1, v2, v3;
+        vols(p, v1, v2, v3);
+        f[6] = (float)(p[0] * INV_TWO_PI_D);
+        f[7] = (float)(0.5 * p[3] * INV_TWO_PI_D);
+        f[8] = (float)((p[0] + p[1]) * INV_TWO_PI_D);
+        f[9] = (float)((0.5 * p[3] + p[2]) * INV_TWO_PI_D);
+        f[10] = (float)v1;
+        f[11] = (float)v2;
+        f[12] = (float)v3;
+        f[13] = (float)(w * 1e-4);
+        f[14] = (float)fmax(p[0] + p[1], 0.5 * p[3] + p[2]);
+    }
+    __device__ static void geom2f(const QRotF &q, const float *f, float *g)
+    {
+        const float qab = sqrt_f(fmaf(q.qa, q.qa, q.qb * q.qb));
+        const float b1 = j1c_rev(qab * f[6]), s2 = sinc_rev(q.qc * f[9]);
+        g[0] = f[10] * sinc_rev(q.qc * f[7]) * b1;
+        g[1] = f[11] * s2 * j1c_rev(qab * f[8]);
+        g[2] = f[12] * s2 * b1;
+    }
+    SAS_HD static void contrasts(const double *, const double *sld, float *con)
+    {
+        con[0] = (float)(sld[0] - sld[1]);
+        con[1] = (float)(sld[2] - sld[3]);
+        con[2] = (float)(sld[1] - sld[2]);
+    }
 };
 
 // ---------------------------------------------------------------- concentric shells

@@ -156,6 +156,9 @@ int with_model(int id, F &&f)
     case 13: return f((sas::FuzzySphere *)nullptr);
     case 17: return f((sas::CoreMultiShell *)nullptr);
     case 18: return f((sas::Onion *)nullptr);
+    case 19: return f((sas::CoreShellEllipsoid *)nullptr);
+    case 20: return f((sas::EllipticalCylinder *)nullptr);
+    case 21: return f((sas::CoreShellBicelle *)nullptr);
     default: return fail(SASHIP_EINVAL, "model " + std::to_string(id) + " has no mesh kernel");
     }
 }
@@ -491,15 +494,15 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
     m->quad_n = quad_n;
     m->dev = current_device();
     // orientation-quadrature tables (the models whose 1D kernel averages orientation)
-    const bool cyl_family = model_id == 1 || model_id == 3;
-    const bool ell = model_id == 2;
-    const bool ppd = model_id == 6, tri = model_id == 7, hollow = model_id == 11;
+    const bool cyl_family = model_id == 1 || model_id == 3 || model_id == 21;
+    const bool ell = model_id == 2 || model_id == 19;
+    const bool ppd = model_id == 6, tri = model_id == 7, hollow = model_id == 11, ecy = model_id == 20;
     std::vector<int> sizes;
     if (quad_n > 0) sizes = {quad_n};
     else sizes = {20, 76, 500, 5000};
     std::vector<float> a, b, w, c, d;
     std::vector<int> offs;
-    if (cyl_family || ell || ppd || tri || hollow) {
+    if (cyl_family || ell || ppd || tri || hollow || ecy) {
         for (int n : sizes) {
             std::vector<double> z, wt;
             gauss_legendre(n, z, wt);
@@ -518,6 +521,13 @@ int saship_model_create(int32_t model_id, int32_t quad_n, saship_model **out)
                     // cos(theta) = u on [0, 1]
                     ta = (float)sqrt(1.0 - u * u);
                     tb = (float)u;
+                } else if (ecy) {
+                    // inner angle theta = 2 th on [0, pi]: cos^2 and sin^2 of th;
+                    // outer cos(alpha) = u on [0, 1]
+                    ta = (float)(cos(th) * cos(th));
+                    tb = (float)(sin(th) * sin(th));
+                    tc = (float)u;
+                    td = (float)sqrt(1.0 - u * u);
                 } else if (ppd) {
                     ta = (float)sin(th);
                     tb = (float)cos(th);

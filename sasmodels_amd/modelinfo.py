@@ -23,7 +23,8 @@ core_shell_sphere.py:78-82, parallelepiped.py:216-231,
 triaxial_ellipsoid.py:143-158, bcc_paracrystal.py:186-194,
 fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hollow_cylinder.py:90-98,
 vesicle.py:87-96, fuzzy_sphere.py:113-117, core_multi_shell.py:91-103,
-onion.py:323-333, hardsphere.py:84-88,
+onion.py:323-333, core_shell_ellipsoid.py:139-149,
+elliptical_cylinder.py:115-122, core_shell_bicelle.py:140-151, hardsphere.py:84-88,
 stickyhardsphere.py:110-121, squarewell.py:83-92, hayter_msa.py:110-117),
 with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
@@ -453,6 +454,47 @@ _MODEL_TABLE = {
         ["thickness[n_shells]", "Ang", 40., [0, inf], "volume", "Thickness of shell k"],
         ["A[n_shells]", "", 1.0, [-inf, inf], "", "Decay rate of shell k"],
     ], dict(radius_effective_modes=["outer radius"], xy_mode="qa", has_quadrature=False, single=False)),
+    "core_shell_ellipsoid": (19, [
+        ["radius_equat_core", "Ang", 20, [0, inf], "volume", "Equatorial radius of core"],
+        ["x_core", "None", 3, [0, inf], "volume", "axial ratio of core, X = r_polar/r_equatorial"],
+        ["thick_shell", "Ang", 30, [0, inf], "volume", "thickness of shell at equator"],
+        ["x_polar_shell", "", 1, [0, inf], "volume", "ratio of thickness of shell at pole to that at equator"],
+        ["sld_core", "1e-6/Ang^2", 2, [-inf, inf], "sld", "Core scattering length density"],
+        ["sld_shell", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Shell scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 6.3, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["theta", "degrees", 0, [-180, 180], "orientation", "elipsoid axis to beam angle"],
+        ["phi", "degrees", 0, [-360, 360], "orientation", "rotation about beam"],
+    ], dict(radius_effective_modes=["average outer curvature", "equivalent volume sphere",
+                                    "min outer radius", "max outer radius"],
+            xy_mode="qac", has_quadrature=True)),
+    "elliptical_cylinder": (20, [
+        ["radius_minor", "Ang", 20.0, [0, inf], "volume", "Ellipse minor radius"],
+        ["axis_ratio", "", 1.5, [1, inf], "volume", "Ratio of major radius over minor radius"],
+        ["length", "Ang", 400.0, [1, inf], "volume", "Length of the cylinder"],
+        ["sld", "1e-6/Ang^2", 4.0, [-inf, inf], "sld", "Cylinder scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["theta", "degrees", 90.0, [-180, 180], "orientation", "cylinder axis to beam angle"],
+        ["phi", "degrees", 0, [-360, 360], "orientation", "rotation about beam"],
+        ["psi", "degrees", 0, [-360, 360], "orientation", "rotation about cylinder axis"],
+    ], dict(radius_effective_modes=[
+                "equivalent cylinder excluded volume", "equivalent volume sphere", "average radius",
+                "min radius", "max radius", "equivalent circular cross-section", "half length",
+                "half min dimension", "half max dimension", "half diagonal"],
+            xy_mode="qabc", has_quadrature=True)),
+    "core_shell_bicelle": (21, [
+        ["radius", "Ang", 80, [0, inf], "volume", "Cylinder core radius"],
+        ["thick_rim", "Ang", 10, [0, inf], "volume", "Rim shell thickness"],
+        ["thick_face", "Ang", 10, [0, inf], "volume", "Cylinder face thickness"],
+        ["length", "Ang", 50, [0, inf], "volume", "Cylinder length"],
+        ["sld_core", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Cylinder core scattering length density"],
+        ["sld_face", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Cylinder face scattering length density"],
+        ["sld_rim", "1e-6/Ang^2", 4, [-inf, inf], "sld", "Cylinder rim scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["theta", "degrees", 90, [-180, 180], "orientation", "cylinder axis to beam angle"],
+        ["phi", "degrees", 0, [-360, 360], "orientation", "rotation about beam"],
+    ], dict(radius_effective_modes=["excluded volume", "equivalent volume sphere", "outer rim radius",
+                                    "half outer thickness", "half diagonal"],
+            xy_mode="qac", has_quadrature=True)),
     # structure factor: Iq only (no F), no volume, nothing polydisperse;
     # evaluated in FP64 (the reference marks it double precision only)
     "hardsphere": (5, [

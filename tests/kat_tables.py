@@ -155,7 +155,39 @@ FUZZY_SPHERE = [
     ({}, 0.00301005, 359.2315),
 ]
 
+# core_shell_ellipsoid.py:182-231
+_cse_q = 0.1
+CORE_SHELL_ELLIPSOID = [
+    ({"radius_equat_core": 200.0, "x_core": 0.1, "thick_shell": 50.0, "x_polar_shell": 0.2,
+      "sld_core": 2.0, "sld_shell": 1.0, "sld_solvent": 6.3, "background": 0.001, "scale": 1.0},
+     1.0, 0.00199393783025264),
+    ({"background": 0.01}, 0.1, 11.6915291814002),
+    ({"radius_equat_core": 20.0, "x_core": 200.0, "thick_shell": 54.0, "x_polar_shell": 3.0,
+      "sld_core": 20.0, "sld_shell": 10.0, "sld_solvent": 6.0, "background": 0.0, "scale": 1.0},
+     0.01, 8688.52423647087),
+    ({"background": 0.001, "theta": 90.0, "phi": 0.0}, (0.4, 0.5), 0.00690673646263621),
+    ({"radius_equat_core": 20.0, "x_core": 200.0, "thick_shell": 54.0, "x_polar_shell": 3.0,
+      "sld_core": 20.0, "sld_shell": 10.0, "sld_solvent": 6.0, "background": 0.01, "scale": 0.01,
+      "theta": 90.0, "phi": 0.0},
+     (_cse_q*cos(pi/6.0), _cse_q*sin(pi/6.0)), 0.010000250020321),
+]
+
+# elliptical_cylinder.py:157-166
+ELLIPTICAL_CYLINDER = [
+    ({"radius_minor": 20.0, "axis_ratio": 1.5, "sld": 4.0, "length": 400.0, "sld_solvent": 1.0,
+      "background": 0.0}, 0.001, 675.504402),
+]
+
+# core_shell_bicelle.py:173-180
+CORE_SHELL_BICELLE = [
+    ({}, 0.05, 7.4883545957),
+    ({"theta": 80., "phi": 10.}, (0.1*cos(pi/6.0), 0.1*sin(pi/6.0)), 2.81048892474),
+]
+
 MODEL_KATS = {
+    "core_shell_ellipsoid": CORE_SHELL_ELLIPSOID,
+    "elliptical_cylinder": ELLIPTICAL_CYLINDER,
+    "core_shell_bicelle": CORE_SHELL_BICELLE,
     "sphere": SPHERE,
     "hollow_cylinder": HOLLOW_CYLINDER,
     "vesicle": VESICLE,

@@ -66,6 +66,20 @@ RANDOM_1D = {
     "fuzzy_sphere": lambda r: dict(radius=r.uniform(10, 200), fuzziness=r.uniform(1, 30), sld=r.uniform(-1, 7),
                                    radius_pd=0.15, radius_pd_n=int(r.integers(1, 30)), fuzziness_pd=0.1,
                                    fuzziness_pd_n=3),
+    "core_shell_ellipsoid": lambda r: dict(radius_equat_core=r.uniform(10, 200), x_core=10**r.uniform(-1, 1),
+                                           thick_shell=r.uniform(2, 50), x_polar_shell=10**r.uniform(-1, 1),
+                                           sld_core=r.uniform(-1, 7), sld_shell=r.uniform(-1, 7),
+                                           radius_equat_core_pd=0.1, radius_equat_core_pd_n=int(r.integers(1, 8)),
+                                           thick_shell_pd=0.1, thick_shell_pd_n=3),
+    "elliptical_cylinder": lambda r: dict(radius_minor=r.uniform(5, 60), axis_ratio=1 + 10**r.uniform(-2, 1),
+                                          length=r.uniform(20, 600), sld=r.uniform(-1, 7),
+                                          radius_minor_pd=0.1, radius_minor_pd_n=int(r.integers(1, 5)),
+                                          length_pd=0.1, length_pd_n=3),
+    "core_shell_bicelle": lambda r: dict(radius=r.uniform(10, 120), thick_rim=r.uniform(2, 30),
+                                         thick_face=r.uniform(2, 30), length=r.uniform(10, 400),
+                                         sld_core=r.uniform(-1, 7), sld_face=r.uniform(-1, 7),
+                                         sld_rim=r.uniform(-1, 7), radius_pd=0.1,
+                                         radius_pd_n=int(r.integers(1, 8)), length_pd=0.1, length_pd_n=3),
 }
 
 
@@ -120,6 +134,20 @@ RANDOM_2D = {
     "vesicle": lambda r: dict(radius=r.uniform(20, 150), thickness=r.uniform(5, 40), radius_pd=0.1, radius_pd_n=5),
     "fuzzy_sphere": lambda r: dict(radius=r.uniform(20, 150), fuzziness=r.uniform(2, 20), radius_pd=0.1,
                                    radius_pd_n=5),
+    "core_shell_ellipsoid": lambda r: dict(radius_equat_core=r.uniform(10, 80), x_core=10**r.uniform(-1, 1),
+                                           thick_shell=r.uniform(2, 30), x_polar_shell=10**r.uniform(-1, 1),
+                                           theta=r.uniform(0, 90), phi=r.uniform(0, 180), theta_pd=8,
+                                           theta_pd_n=3, phi_pd=10, phi_pd_n=3, thick_shell_pd=0.1,
+                                           thick_shell_pd_n=3),
+    "elliptical_cylinder": lambda r: dict(radius_minor=r.uniform(5, 40), axis_ratio=1 + 10**r.uniform(-2, 0.7),
+                                          length=r.uniform(20, 300), theta=r.uniform(0, 90), phi=r.uniform(0, 180),
+                                          psi=r.uniform(0, 180), psi_pd=15, psi_pd_n=3, theta_pd=5, theta_pd_n=3,
+                                          radius_minor_pd=0.1, radius_minor_pd_n=3),
+    "core_shell_bicelle": lambda r: dict(radius=r.uniform(10, 80), thick_rim=r.uniform(2, 20),
+                                         thick_face=r.uniform(2, 20), length=r.uniform(10, 200),
+                                         theta=r.uniform(0, 90), phi=r.uniform(0, 180), theta_pd=8, theta_pd_n=3,
+                                         sld_core=r.uniform(-1, 7), sld_face=r.uniform(-1, 7),
+                                         sld_rim=r.uniform(-1, 7), radius_pd=0.1, radius_pd_n=3),
 }
 
 
@@ -169,6 +197,15 @@ MAGNETIC = {
                     up_frac_i=0.4, up_frac_f=0.6, up_theta=10),
     "fuzzy_sphere": dict(radius=60, fuzziness=8, sld_M0=3, sld_mtheta=20, sld_mphi=70, up_frac_i=0.8,
                          up_frac_f=0.3, up_theta=80),
+    "core_shell_ellipsoid": dict(radius_equat_core=40, x_core=2, thick_shell=15, x_polar_shell=0.5, theta=30,
+                                 phi=60, sld_core_M0=2, sld_core_mtheta=40, sld_shell_M0=1, sld_solvent_M0=0.5,
+                                 sld_solvent_mphi=120, up_frac_i=0.2, up_frac_f=0.7, up_theta=35, up_phi=15),
+    "elliptical_cylinder": dict(radius_minor=25, axis_ratio=2.5, length=180, theta=40, phi=20, psi=60,
+                                psi_pd=10, psi_pd_n=3, sld_M0=2.5, sld_mtheta=30, sld_mphi=45, up_frac_i=0.6,
+                                up_frac_f=0.3, up_theta=50),
+    "core_shell_bicelle": dict(radius=50, thick_rim=10, thick_face=8, length=60, theta=70, phi=10,
+                               sld_core_M0=1.5, sld_face_M0=2, sld_face_mtheta=60, sld_rim_M0=3,
+                               sld_rim_mphi=150, sld_solvent_M0=0.4, up_frac_i=0.7, up_frac_f=0.2, up_theta=20),
 }
 
 
