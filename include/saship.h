@@ -233,6 +233,20 @@ int saship_batch_set_resolution(saship_batch *batch, const int32_t *n_out, const
                                 const int32_t *length, const int64_t *stride, const int64_t *offset,
                                 const double *weights, int64_t n_weights);
 
+/* Resolution smearing with repeated bands: as saship_batch_set_resolution,
+ * and the o-th smeared point sums its band repeat[o] times, the j-th copy
+ * starting repeat_step[o]*j points further on, with the copies weighted by
+ * repeat_weights[repeat_offset[o] + j].  This is the 2D slit of an oriented
+ * sample (Slit2D, sasmodels/resolution2d.py:177-241): the model on a
+ * (qy, qx) grid, the qx band of each point's pinhole weights repeated over
+ * the qy rows with the trapezoid weights.  Every band is validated against
+ * the set's q count and both weight arrays. */
+int saship_batch_set_resolution_repeat(saship_batch *batch, const int32_t *n_out, const int32_t *start,
+                                       const int32_t *length, const int64_t *stride, const int64_t *offset,
+                                       const double *weights, int64_t n_weights, const int32_t *repeat,
+                                       const int64_t *repeat_step, const int64_t *repeat_offset,
+                                       const double *repeat_weights, int64_t n_repeat_weights);
+
 /* P@S products on the device: the batch's form factor times structure
  * factor s_model (SASHIP_HARDSPHERE, SASHIP_STICKY_HARDSPHERE,
  * SASHIP_SQUAREWELL or SASHIP_HAYTER_MSA; 0 switches it off), combined in

@@ -389,11 +389,20 @@ __global__ __launch_bounds__(256) void k_batch_final(
 // out[o] = sum_k w[off[o] + k] * iq[from[o] + k*step[o]] for k < len[o]
 // (FP64): contiguous bands of a 1D weight matrix (step 1) or the strided
 // polar bins of a 2D pinhole (step = pixels of the set).
+// Repeated bands (rep != NULL): the band is taken rep[o] times, each
+// rstep[o] points further on, and the band sums combined with the weights
+// rw[roff[o] ..] (the qy rows of a 2D slit).
+struct SmearRepeat {
+    const int32_t *rep;
+    const int64_t *rstep, *roff;
+    const double *rw;
+};
+
 __global__ __launch_bounds__(256) void k_batch_smear(int64_t n_out, const int64_t *__restrict__ from,
                                                      const int32_t *__restrict__ len,
                                                      const int64_t *__restrict__ step,
                                                      const int64_t *__restrict__ off,
-                                                     const double *__restrict__ w,
+                                                     const double *__restrict__ w, SmearRepeat r,
                                                      const double *__restrict__ iq, double *__restrict__ out)
 {
     const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -401,9 +410,23 @@ __global__ __launch_bounds__(256) void k_batch_smear(int64_t n_out, const int64_
     const double *src = iq + from[o];
     const double *wt = w + off[o];
     const int64_t st = step[o];
-    double acc = 0.0;
-    for (int k = 0; k < len[o]; k++) acc = fma(wt[k], src[k * st], acc);
-    out[o] = acc;
+    const int n = len[o];
+    if (!r.rep) {
+        double acc = 0.0;
+        for (int k = 0; k < n; k++) acc = fma(wt[k], src[k * st], acc);
+        out[o] = acc;
+        return;
+    }
+    const int64_t rs = r.rstep[o];
+    const double *rw = r.rw + r.roff[o];
+    double total = 0.0;
+    for (int j = 0; j < r.rep[o]; j++) {
+        const double *row = src + j * rs;
+        double acc = 0.0;
+        for (int k = 0; k < n; k++) acc = fma(wt[k], row[k * st], acc);
+        total = fma(rw[j], acc, total);
+    }
+    out[o] = total;
 }
 
 }  // namespace sas
@@ -456,6 +479,9 @@ struct saship_batch {
     int64_t *d_sfrom = nullptr, *d_soff = nullptr, *d_sstep = nullptr;
     int32_t *d_slen = nullptr;
     double *d_sw = nullptr;
+    int32_t *d_srep = nullptr;        // repeated bands (NULL: none)
+    int64_t *d_srstep = nullptr, *d_sroff = nullptr;
+    double *d_srw = nullptr;
     bool separable = true;            // use the separable path when every set allows it
     // P@S (saship_batch_set_structure): structure-factor model (0: none) and
     // per set (radius_effective, volfraction, structure_factor_mode, radius_effective_mode,
@@ -758,8 +784,9 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     double *d_siq = d_iq + p.nq_total;
     if (want_iq && b->n_smear > 0) {
         const int64_t blocks = (b->n_smear + 255) / 256;
+        const SmearRepeat rep{b->d_srep, b->d_srstep, b->d_sroff, b->d_srw};
         k_batch_smear<<<(unsigned)blocks, 256, 0, b->st>>>(b->n_smear, b->d_sfrom, b->d_slen, b->d_sstep, b->d_soff,
-                                                          b->d_sw, d_iq, d_siq);
+                                                          b->d_sw, rep, d_iq, d_siq);
         HIP_TRY(hipGetLastError());
     }
     if (b->timing) HIP_TRY(hipEventRecord(b->ev3, b->st));
@@ -981,24 +1008,31 @@ int saship_batch_set_path(saship_batch *b, int32_t separable)
     return 0;
 }
 
-int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int32_t *start, const int32_t *length,
-                                const int64_t *stride, const int64_t *offset, const double *weights, int64_t n_weights)
+namespace batch_host {
+
+// Shared body of the two resolution entry points; repeat == NULL: plain bands.
+int set_resolution(saship_batch *b, const int32_t *n_out, const int32_t *start, const int32_t *length,
+                   const int64_t *stride, const int64_t *offset, const double *weights, int64_t n_weights,
+                   const int32_t *repeat, const int64_t *repeat_step, const int64_t *repeat_offset,
+                   const double *repeat_weights, int64_t n_repeat_weights)
 {
     if (!b) return fail(SASHIP_EINVAL, "batch is NULL");
     std::lock_guard<std::mutex> lock(b->mu);
     HIP_TRY(hipSetDevice(b->dev));
     drop_graph(b);
     b->have_last = false;      // the output buffer was sized for the old smearing
-    void *old[] = {b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw};
+    void *old[] = {b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw, b->d_srep, b->d_srstep, b->d_sroff, b->d_srw};
     for (void *ptr : old)
         if (ptr) (void)hipFree(ptr);
-    b->d_sfrom = b->d_soff = b->d_sstep = nullptr;
-    b->d_slen = nullptr;
-    b->d_sw = nullptr;
+    b->d_sfrom = b->d_soff = b->d_sstep = b->d_srstep = b->d_sroff = nullptr;
+    b->d_slen = b->d_srep = nullptr;
+    b->d_sw = b->d_srw = nullptr;
     b->n_smear = 0;
     if (!n_out) return 0;                          // smearing off
     if (!start || !length || !offset || (n_weights > 0 && !weights) || n_weights < 0)
         return fail(SASHIP_EINVAL, "start, length, offset and weights must not be NULL");
+    if (repeat && (!repeat_step || !repeat_offset || (n_repeat_weights > 0 && !repeat_weights) || n_repeat_weights < 0))
+        return fail(SASHIP_EINVAL, "repeat_step, repeat_offset and repeat_weights must not be NULL");
     int64_t total = 0;
     for (int s = 0; s < b->nsets; s++) {
         if (n_out[s] < 0) return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": n_out < 0");
@@ -1009,14 +1043,19 @@ int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int
         const int64_t nq = b->qoff[s + 1] - b->qoff[s];
         for (int k = 0; k < n_out[s]; k++, o++) {
             const int64_t st = stride ? stride[o] : 1;
+            const int64_t nr = repeat ? repeat[o] : 1, rs = repeat ? repeat_step[o] : 0;
             steps[o] = st;
-            if (start[o] < 0 || length[o] < 0 || st < 1 ||
-                (length[o] > 0 && start[o] + (int64_t)(length[o] - 1) * st >= nq))
+            // the last point read: start + (nr - 1) rs + (length - 1) st
+            if (start[o] < 0 || length[o] < 0 || st < 1 || nr < 0 || rs < 0 ||
+                (length[o] > 0 && nr > 0 && start[o] + (nr - 1) * rs + (int64_t)(length[o] - 1) * st >= nq))
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": band of point " + std::to_string(k) +
                                                " outside the set's q points");
             if (offset[o] < 0 || offset[o] + length[o] > n_weights)
                 return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": weights of point " + std::to_string(k) +
                                                " outside the weight array");
+            if (repeat && (repeat_offset[o] < 0 || repeat_offset[o] + nr > n_repeat_weights))
+                return fail(SASHIP_EINVAL, "set " + std::to_string(s) + ": repeat weights of point " +
+                                               std::to_string(k) + " outside the repeat weight array");
             from[o] = b->qoff[s] + start[o];
         }
     }
@@ -1031,9 +1070,40 @@ int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int
     if (e == hipSuccess) e = hipMemcpy(b->d_soff, offset, sizeof(int64_t) * total, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_slen, length, sizeof(int32_t) * total, hipMemcpyHostToDevice);
     if (e == hipSuccess && n_weights > 0) e = hipMemcpy(b->d_sw, weights, sizeof(double) * n_weights, hipMemcpyHostToDevice);
+    if (repeat) {
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_srep, sizeof(int32_t) * total);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_srstep, sizeof(int64_t) * total);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_sroff, sizeof(int64_t) * total);
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_srw, sizeof(double) * std::max<int64_t>(n_repeat_weights, 1));
+        if (e == hipSuccess) e = hipMemcpy(b->d_srep, repeat, sizeof(int32_t) * total, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_srstep, repeat_step, sizeof(int64_t) * total, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_sroff, repeat_offset, sizeof(int64_t) * total, hipMemcpyHostToDevice);
+        if (e == hipSuccess && n_repeat_weights > 0)
+            e = hipMemcpy(b->d_srw, repeat_weights, sizeof(double) * n_repeat_weights, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) return fail(SASHIP_EHIP, std::string("set resolution: ") + hipGetErrorString(e));
     b->n_smear = total;
     return 0;
+}
+
+}  // namespace batch_host
+
+int saship_batch_set_resolution(saship_batch *b, const int32_t *n_out, const int32_t *start, const int32_t *length,
+                                const int64_t *stride, const int64_t *offset, const double *weights, int64_t n_weights)
+{
+    return batch_host::set_resolution(b, n_out, start, length, stride, offset, weights, n_weights, nullptr, nullptr,
+                                      nullptr, nullptr, 0);
+}
+
+int saship_batch_set_resolution_repeat(saship_batch *b, const int32_t *n_out, const int32_t *start,
+                                       const int32_t *length, const int64_t *stride, const int64_t *offset,
+                                       const double *weights, int64_t n_weights, const int32_t *repeat,
+                                       const int64_t *repeat_step, const int64_t *repeat_offset,
+                                       const double *repeat_weights, int64_t n_repeat_weights)
+{
+    if (!repeat) return fail(SASHIP_EINVAL, "repeat is NULL (use saship_batch_set_resolution)");
+    return batch_host::set_resolution(b, n_out, start, length, stride, offset, weights, n_weights, repeat,
+                                      repeat_step, repeat_offset, repeat_weights, n_repeat_weights);
 }
 
 int saship_batch_get_options(saship_batch *b, int32_t *use_graph, int32_t *pts_per_item, int32_t *waves_per_cu)
@@ -1206,7 +1276,8 @@ int saship_batch_release(saship_batch *b)
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
     void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums, b->d_sprep,
-                        b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw};
+                        b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw,
+                        b->d_srep, b->d_srstep, b->d_sroff, b->d_srw};
     for (void *ptr : dev_ptrs)
         if (ptr) (void)hipFree(ptr);
     if (b->h_stage) (void)hipHostFree(b->h_stage);

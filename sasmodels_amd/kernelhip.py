@@ -229,7 +229,7 @@ class HipBatch:
         """
         Smear every set's I(q) on the device: *resolutions* holds one
         resolution object per set whose q_calc is that set's q (Pinhole1D,
-        Slit1D, Pinhole2D; None or Perfect1D: no smearing for that set), or
+        Slit1D, Pinhole2D, Slit2D; None or Perfect1D: no smearing for that set), or
         is None to switch smearing off.  Calls then return I at each
         resolution's measured q (pixels for Pinhole2D).
         """
@@ -251,8 +251,28 @@ class HipBatch:
                 np.ascontiguousarray(np.concatenate([p[3] for p in parts]).astype(np.int64)),
                 np.ascontiguousarray(np.concatenate(offsets).astype(np.int64)),
                 np.ascontiguousarray(np.concatenate([p[5] for p in parts]) if total else np.zeros(1))]
-        _lib.check(_lib.lib().saship_batch_set_resolution(
-            self._handle, *[k.ctypes.data for k in keep], int(total)))
+        if not any(len(p) > 6 for p in parts):
+            _lib.check(_lib.lib().saship_batch_set_resolution(
+                self._handle, *[k.ctypes.data for k in keep], int(total)))
+        else:
+            # repeated bands (Slit2D); a set without them repeats its band once
+            reps, steps, roffs, rws, rtotal = [], [], [], [], 0
+            for p in parts:
+                n = int(p[0])
+                rep, step, roff, rw = p[6] if len(p) > 6 else (np.ones(n, np.int32), np.zeros(n, np.int64),
+                                                              np.zeros(n, np.int64), np.ones(1))
+                reps.append(rep)
+                steps.append(step)
+                roffs.append(np.asarray(roff, np.int64) + rtotal)
+                rws.append(np.asarray(rw, float))
+                rtotal += len(rw)
+            keep += [np.ascontiguousarray(np.concatenate(reps).astype(np.int32)),
+                     np.ascontiguousarray(np.concatenate(steps).astype(np.int64)),
+                     np.ascontiguousarray(np.concatenate(roffs).astype(np.int64)),
+                     np.ascontiguousarray(np.concatenate(rws))]
+            _lib.check(_lib.lib().saship_batch_set_resolution_repeat(
+                self._handle, *[k.ctypes.data for k in keep[:6]], int(total),
+                *[k.ctypes.data for k in keep[6:]], int(rtotal)))
         self._res_keep = keep
         self._n_out = keep[0].astype(np.int64)
 

@@ -24,13 +24,16 @@ Construction follows the reference's definitions:
 * 2D pinhole: every pixel sampled at nr x nphi polar offsets (radii at bin
   centres of [0, nsigma] in units of (dq_parallel, dq_perp), angles
   2 pi k/nphi, rotated to the pixel's direction), averaged with the
-  Gaussian mass of each radial bin (resolution2d.py:29-175).
+  Gaussian mass of each radial bin (resolution2d.py:29-175);
+* 2D slit (an oriented sample measured along qx): trapezoid integral over
+  qy of a (qy, qx) grid, then the pinhole weights along qx
+  (resolution2d.py:177-241).
 """
 
 import numpy as np
 from scipy.special import erf
 
-__all__ = ["Resolution", "Perfect1D", "Pinhole1D", "Slit1D", "Pinhole2D", "apply_resolution_matrix",
+__all__ = ["Resolution", "Perfect1D", "Pinhole1D", "Slit1D", "Pinhole2D", "Slit2D", "apply_resolution_matrix",
            "pinhole_resolution", "slit_resolution", "pinhole_extend_q", "slit_extend_q",
            "bin_edges", "interpolate", "linear_extrapolation", "geometric_extrapolation", "band",
            "smear_bands"]
@@ -155,6 +158,47 @@ class Pinhole2D(Resolution):
             return theory
         t = np.reshape(theory, (len(self.q_calc_weights), len(self.qx_data)))
         return np.average(t, axis=0, weights=self.q_calc_weights)
+
+
+N_SLIT_PERP = {"xhigh": 1000, "high": 500, "med": 200, "low": 50}
+
+
+class Slit2D(Resolution):
+    """
+    Slit smearing of an oriented sample measured along qx
+    (resolution2d.py:177-241): the model is evaluated on a (qy, qx) grid,
+    integrated over qy in [-q_width, q_width] by the trapezoid rule on
+    2 N + 1 log-spaced qy (N = N_SLIT_PERP[accuracy], both signs and 0), then
+    smeared along qx with a Gaussian of width q_length (the pinhole weights)
+    when any q_length > 0.  q_calc = [qx, qy] of the grid, qy-major (the
+    meshgrid layout), so ``apply`` reshapes to (ny, nx).
+    """
+
+    def __init__(self, q, q_length, q_width=0.0, q_calc=None, accuracy="low"):
+        self.q, self.q_length, self.q_width = q, q_length, q_width
+        length = np.full(len(q), float(q_length)) if np.isscalar(q_length) else np.asarray(q_length, float)
+        width = float(q_width) if np.isscalar(q_width) else float(np.max(q_width))
+        qx = np.sort(q_calc) if q_calc is not None else pinhole_extend_q(q, length, nsigma=3)
+        half = np.logspace(np.log10(np.min(q)), np.log10(width), N_SLIT_PERP[accuracy])
+        qy = np.concatenate([-half[::-1], [0.0], half])
+        grid_x, grid_y = np.meshgrid(qx, qy)
+        self.q_calc = [grid_x.ravel(), grid_y.ravel()]
+        self.qx_calc, self.qy_calc = qx, qy
+        self.nx, self.ny = len(qx), len(qy)
+        self.dy = 2*width/self.ny
+        # trapezoid weights over qy (the rule numpy's trapz applies)
+        dq = np.diff(qy)
+        self.qy_weights = np.concatenate([[0.0], dq/2]) + np.concatenate([dq/2, [0.0]])
+        if np.any(length > 0):
+            self.weights = pinhole_resolution(qx, q, np.maximum(length, MINIMUM_RESOLUTION))
+        elif len(qx) == len(q) and np.all(qx == q):
+            self.weights = None
+        else:
+            raise ValueError("Slit2D fails with q_calc != q")
+
+    def apply(self, theory):
+        iq = np.trapezoid(np.reshape(theory, (self.ny, self.nx)), x=self.qy_calc, axis=0)
+        return iq if self.weights is None else apply_resolution_matrix(self.weights, iq)
 
 
 def apply_resolution_matrix(weight_matrix, theory):
@@ -310,11 +354,29 @@ def band(weight_matrix):
 def smear_bands(res, n_calc):
     """
     Device form of a resolution for a set with *n_calc* q_calc points:
-    (n_out, start, length, stride, offset, weights).  1D weight matrices
-    give contiguous bands; Pinhole2D gives one strided band per pixel over
-    its polar bins, all pixels sharing one normalised weight vector; None
-    or a resolution without weights is the identity on the set's points.
+    (n_out, start, length, stride, offset, weights), and for Slit2D a
+    seventh entry (repeat, repeat_step, repeat_offset, repeat_weights): the
+    band of each output is repeated at repeat_step along the set's points
+    and the repeats summed with repeat_weights.  1D weight matrices give
+    contiguous bands; Pinhole2D gives one strided band per pixel over its
+    polar bins, all pixels sharing one normalised weight vector; Slit2D the
+    qx band of each point repeated over the qy rows with the trapezoid
+    weights; None or a resolution without weights is the identity on the
+    set's points.
     """
+    if isinstance(res, Slit2D):
+        if res.nx*res.ny != n_calc:
+            raise ValueError("Slit2D has %d q_calc points, the set %d" % (res.nx*res.ny, n_calc))
+        if res.weights is None:
+            n_out = res.nx
+            start, length = np.arange(n_out, dtype=np.int32), np.ones(n_out, np.int32)
+            offset, weights = np.zeros(n_out, np.int64), np.ones(1)
+        else:
+            start, length, offset, weights = band(res.weights)
+            n_out = res.weights.shape[1]
+        rep = (np.full(n_out, res.ny, np.int32), np.full(n_out, res.nx, np.int64), np.zeros(n_out, np.int64),
+               np.asarray(res.qy_weights, float))
+        return n_out, start, length, np.ones(n_out, np.int64), offset, weights, rep
     W = getattr(res, "weight_matrix", None)
     wb = getattr(res, "q_calc_weights", None)
     if isinstance(res, Pinhole2D) and wb is not None:

@@ -84,6 +84,26 @@ def test_smearing_rejects_bad_bands():
         _lib.check(_lib.lib().saship_batch_set_resolution(batch._handle, n_out.ctypes.data, start.ctypes.data,
                                                           length.ctypes.data, stride.ctypes.data,
                                                           offset.ctypes.data, w.ctypes.data, 6))
+    # repeated bands: rows 0..4 at repeats 0, 9, 18, 27, 36 reach row 40 of 40
+    start, length, stride = np.array([0], np.int32), np.array([5], np.int32), np.array([1], np.int64)
+    rep, rstep, roff, rw = np.array([5], np.int32), np.array([9], np.int64), np.array([0], np.int64), np.ones(5)
+    args = [n_out, start, length, stride, offset, w]
+    with pytest.raises(HipError):
+        _lib.check(_lib.lib().saship_batch_set_resolution_repeat(
+            batch._handle, *[a.ctypes.data for a in args], 6, rep.ctypes.data, rstep.ctypes.data,
+            roff.ctypes.data, rw.ctypes.data, 5))
+    rstep[0] = 8                          # ..., 32..36: inside; but repeat weights past the array
+    with pytest.raises(HipError):
+        _lib.check(_lib.lib().saship_batch_set_resolution_repeat(
+            batch._handle, *[a.ctypes.data for a in args], 6, rep.ctypes.data, rstep.ctypes.data,
+            roff.ctypes.data, rw.ctypes.data, 4))
+    _lib.check(_lib.lib().saship_batch_set_resolution_repeat(
+        batch._handle, *[a.ctypes.data for a in args], 6, rep.ctypes.data, rstep.ctypes.data,
+        roff.ctypes.data, rw.ctypes.data, 5))
+    got = batch(make_calls(hm, q))[0][0]      # the wrapper still splits by q count: the first value
+    batch.set_resolution(None)
+    plain = batch(make_calls(hm, q))[0]
+    np.testing.assert_allclose(got, sum(plain[8*j:8*j + 5].sum() for j in range(5)), rtol=1e-13)
     W = np.eye(40)
     W[39, 0] = 0.5                        # band of point 0 spans all rows: still inside
     ok = type("Res", (), {"weight_matrix": W})()
@@ -129,4 +149,37 @@ def test_batch_pinhole2d():
         np.testing.assert_allclose(sm, r.apply(p), rtol=1e-12, atol=0)
         ref = r.apply(OracleKernel(hm.info, list(r.q_calc))(det, val, 0.0, False))
         assert masked_relerr(sm, ref) <= RTOL
+    batch.release()
+
+
+def test_batch_slit2d():
+    """Oriented samples measured along qx (Slit2D): the (qy, qx) grid through
+    the 2D kernel, the qx bands repeated over the qy rows on the device."""
+    from sasmodels_amd import resolution2d
+    hm = hip_model("cylinder")
+    rng = np.random.default_rng(8)
+    kl = type("K", (), {"info": hm.info, "dtype": np.dtype("d")})()
+    res, calls = [], []
+    for s, (length, acc) in enumerate(((0.0, "low"), (0.004, "low"), (0.002, "med"))):
+        q = np.linspace(0.004, 0.12, 24 + 8*s)
+        kw = dict(q_calc=q) if length == 0.0 else {}
+        res.append(resolution2d.Slit2D(q, q_length=length, q_width=0.01 + 0.005*s, accuracy=acc, **kw))
+        pars = dict(radius=rng.uniform(20, 60), length=rng.uniform(100, 400), theta=rng.uniform(0, 90),
+                    phi=rng.uniform(0, 90), theta_pd=5, theta_pd_n=3)
+        calls.append(make_kernel_args(kl, direct_model.get_mesh(hm.info, pars, dim="2d"))[:2])
+    batch = HipBatch(hm, [list(r.q_calc) for r in res])
+    plain = batch(calls)
+    batch.set_resolution(res)
+    smeared = batch(calls)
+    for r, p, sm, (det, val) in zip(res, plain, smeared, calls):
+        print("slit2d", r.nx, r.ny, flush=True)
+        assert len(sm) == len(r.q)
+        np.testing.assert_allclose(sm, r.apply(p), rtol=1e-12, atol=0)
+        ref = r.apply(OracleKernel(hm.info, list(r.q_calc))(det, val, 0.0, False))
+        assert masked_relerr(sm, ref) <= RTOL
+    # mixed with a set that has no repeated bands
+    batch.set_resolution([res[0], None, res[2]])
+    mixed = batch(calls)
+    np.testing.assert_allclose(mixed[0], smeared[0], rtol=1e-14)
+    np.testing.assert_array_equal(mixed[1], plain[1])
     batch.release()

@@ -224,3 +224,60 @@ def test_smear_bands_forms():
     np.testing.assert_allclose(dev, p2.apply(theory), rtol=1e-13)
     with pytest.raises(ValueError):
         R.smear_bands(p2, 100)
+
+
+# ---------------------------------------------------------------- Slit2D (resolution2d.py:177-241)
+def test_slit2d_grid_and_constants():
+    from sasmodels_amd import resolution2d
+    q = np.linspace(0.002, 0.1, 40)
+    s = resolution2d.Slit2D(q, q_length=0.003, q_width=0.01)
+    assert s.ny == 2*resolution2d.N_SLIT_PERP["low"] + 1 and s.nx == len(s.qx_calc)
+    gx, gy = (np.reshape(v, (s.ny, s.nx)) for v in s.q_calc)
+    assert (gx == s.qx_calc[None, :]).all() and (gy == s.qy_calc[:, None]).all()     # meshgrid layout
+    assert s.qy_calc[s.ny//2] == 0.0 and np.allclose(s.qy_calc, -s.qy_calc[::-1])
+    assert np.isclose(s.qy_calc[-1], 0.01) and np.isclose(s.qy_calc[s.ny//2 + 1], q.min())
+    # a constant integrates to 2 q_width; a function odd in qy to 0
+    np.testing.assert_allclose(s.apply(np.full(s.nx*s.ny, 3.0)), 6.0*0.01, rtol=1e-13)
+    np.testing.assert_allclose(s.apply(s.q_calc[1]*(1 + s.q_calc[0])), 0.0, atol=1e-17)
+    # without a slit length and with q_calc = q the qx smearing is skipped
+    t = resolution2d.Slit2D(q, q_length=0.0, q_width=0.01, q_calc=q)
+    assert t.weights is None and t.nx == len(q)
+    assert resolution2d.Slit2D(q, q_length=0.0, q_width=0.01).weights is None    # q_calc extends to q itself
+    with pytest.raises(ValueError):
+        resolution2d.Slit2D(q, q_length=0.0, q_width=0.01, q_calc=1.1*q)
+
+
+def test_slit2d_is_the_integral_across_the_slit():
+    """Isotropic I(|q|): Slit2D without slit length is the integral of I over
+    qy in [-w, w] at each qx; against adaptive quadrature."""
+    from sasmodels_amd import resolution2d
+    # the grid's first interval, [0, min(q)], is one trapezoid: a small min(q)
+    # keeps its error (about min(q)^3 f''/6) below the bound
+    q = np.logspace(-4, -1, 25)
+    w = 0.02
+    f = lambda qq: 1.0/(1.0 + (80.0*qq)**2)**2      # a smooth Debye-Bueche-like curve
+    s = resolution2d.Slit2D(q, q_length=0.0, q_width=w, q_calc=q, accuracy="high")
+    got = s.apply(f(np.hypot(*s.q_calc)))
+    want = [2*integrate.quad(lambda y: f(np.hypot(qi, y)), 0, w, epsabs=0, epsrel=1e-12, limit=200)[0] for qi in q]
+    np.testing.assert_allclose(got, want, rtol=2e-5)
+
+
+def test_slit2d_smear_bands_match_apply():
+    from sasmodels_amd import resolution2d
+    rng = np.random.default_rng(3)
+    q = np.linspace(0.003, 0.08, 30)
+    for length in (0.0, 0.004):
+        kw = dict(q_calc=q) if length == 0.0 else {}
+        s = resolution2d.Slit2D(q, q_length=length, q_width=0.015, **kw)
+        n = s.nx*s.ny
+        n_out, start, ln, stride, offset, w, (rep, rstep, roff, rw) = R.smear_bands(s, n)
+        assert n_out == len(q) and (rep == s.ny).all() and (rstep == s.nx).all()
+        theory = rng.random(n)
+        dev = []
+        for o in range(n_out):
+            rows = [np.dot(w[offset[o]:offset[o] + ln[o]], theory[start[o] + j*rstep[o]:][:ln[o]*stride[o]:stride[o]])
+                    for j in range(rep[o])]
+            dev.append(np.dot(rw[roff[o]:roff[o] + rep[o]], rows))
+        np.testing.assert_allclose(dev, s.apply(theory), rtol=1e-13)
+        with pytest.raises(ValueError):
+            R.smear_bands(s, n - 1)
