@@ -72,6 +72,17 @@ extern "C" {
 #define SASHIP_HOLLOW_CYLINDER 11
 #define SASHIP_VESICLE 12
 #define SASHIP_FUZZY_SPHERE 13
+/* concentric shells with vector parameters (sasmodels/models/core_multi_shell.py,
+ * onion.py): the control parameter n (0..10) and ten value slots per vector
+ * (sld1..sld10, thickness1..thickness10, ...), magnetic triples for every
+ * sld slot (modelinfo.py:634-672); onion is evaluated in FP64 */
+#define SASHIP_CORE_MULTI_SHELL 17
+#define SASHIP_ONION 18
+/* more oriented shapes: core_shell_ellipsoid and core_shell_bicelle (theta,
+ * phi), elliptical_cylinder (theta, phi, psi; nested 1D rules) */
+#define SASHIP_CORE_SHELL_ELLIPSOID 19
+#define SASHIP_ELLIPTICAL_CYLINDER 20
+#define SASHIP_CORE_SHELL_BICELLE 21
 
 /* kernel kinds: the three kernels generated per model (generate.py:1144-1180) */
 #define SASHIP_KIND_IQ 0        /* <model>_Iq: 1D, <F> and <F^2> */
