@@ -9,7 +9,7 @@
 // the given inputs, on the branches actually taken (the J1 |x| <= 5 split,
 // the 3j1(x)/x Taylor branch, the adaptive rule sizes).  Convention:
 //   add, sub, mul, div, sqrt: 1 each; sin, cos, exp, log, pow, cbrt, asin,
-//   expm1: 1 each, also tallied as transcendental; an FMA would be 2 (the
+//   expm1, atan, tgamma: 1 each, also tallied as transcendental; an FMA would be 2 (the
 //   restatement has none: it is built without contraction).  Sign changes,
 //   fabs, fmin/fmax and comparisons are not counted.
 // Per-thread tallies (OpenMP over q) are summed after each call.
@@ -94,6 +94,8 @@ FLOPCOUNT_TRANS(log)
 FLOPCOUNT_TRANS(cbrt)
 FLOPCOUNT_TRANS(asin)
 FLOPCOUNT_TRANS(expm1)
+FLOPCOUNT_TRANS(atan)
+FLOPCOUNT_TRANS(tgamma)
 #undef FLOPCOUNT_TRANS
 inline Counted pow(Counted a, Counted b)
 {

@@ -23,14 +23,16 @@ REF_DIR = os.path.join(HERE, "_ref")
 
 KIND_IQ, KIND_IQXY, KIND_IMAGNETIC = 0, 1, 2
 #: models whose 1D kernel returns <F^2> only (one output per q)
-NO_FQ = (8, 9, 10)
+NO_FQ = (8, 9, 10) + tuple(range(22, 33))
 MODEL_IDS = {"sphere": 0, "cylinder": 1, "ellipsoid": 2,
              "core_shell_cylinder": 3, "core_shell_sphere": 4,
              "parallelepiped": 6, "triaxial_ellipsoid": 7,
              "bcc_paracrystal": 8, "fcc_paracrystal": 9, "sc_paracrystal": 10,
              "hollow_cylinder": 11, "vesicle": 12, "fuzzy_sphere": 13,
              "core_multi_shell": 17, "onion": 18, "core_shell_ellipsoid": 19,
-             "elliptical_cylinder": 20, "core_shell_bicelle": 21}
+             "elliptical_cylinder": 20, "core_shell_bicelle": 21, "guinier": 22, "gaussian_peak": 23,
+             "lorentz": 24, "dab": 25, "gel_fit": 26, "mass_fractal": 27, "surface_fractal": 28,
+             "mass_surface_fractal": 29, "star_polymer": 30, "mono_gauss_coil": 31, "fractal": 32}
 
 _lib = None
 _D = ct.c_double
@@ -283,8 +285,12 @@ def ref_lib(name):
     if hasattr(L, "ref_iq"):
         L.ref_iq.restype = _D
         L.ref_iq.argtypes = [_D, ct.c_void_p]
-        L.ref_iqabc.restype = _D
-        L.ref_iqabc.argtypes = [_D, _D, _D, ct.c_void_p]
+        if hasattr(L, "ref_iqabc"):
+            L.ref_iqabc.restype = _D
+            L.ref_iqabc.argtypes = [_D, _D, _D, ct.c_void_p]
+        if hasattr(L, "ref_reff"):
+            L.ref_reff.restype = _D
+            L.ref_reff.argtypes = [ct.c_int, ct.c_void_p]
         L.ref_form_volume.restype = _D
         L.ref_form_volume.argtypes = [ct.c_void_p]
         return L

@@ -18,7 +18,10 @@
  *                       vesicle, fuzzy_sphere, stickyhardsphere, squarewell,
  *                       hayter_msa, core_multi_shell (18), onion (19),
  *                       core_shell_ellipsoid (20), elliptical_cylinder (21),
- *                       core_shell_bicelle (22)
+ *                       core_shell_bicelle (22), and the Iq-only models
+ *                       guinier, gaussian_peak, lorentz, dab, gel_fit,
+ *                       mass_fractal, surface_fractal, mass_surface_fractal,
+ *                       star_polymer, mono_gauss_coil, fractal (23-33)
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -204,8 +207,58 @@ double ref_sq6(double q, double r, double phi, double a, double b, double c, dou
 #    define NPAR_VOLUME(p) form_volume(p[0], p[1], p[2], p[3])
 #    define NPAR_REFF(m, p) radius_effective(m, p[0], p[1], p[2], p[3])
 #  endif
+#elif REF_MODEL >= 23 && REF_MODEL <= 33
+/* models defined by Iq alone: ref_iq(q, p) and their form volume (1 when
+ * the model has no volume parameter, as the generator makes it) */
+#  if REF_MODEL == 23
+#    include "guinier.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0]); }
+#  elif REF_MODEL == 24
+#    include "gaussian_peak.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1]); }
+#  elif REF_MODEL == 25
+#    include "lorentz.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0]); }
+#  elif REF_MODEL == 26
+#    include "dab.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0]); }
+#  elif REF_MODEL == 27
+#    include "gel_fit.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4]); }
+#  elif REF_MODEL == 28
+#    include "lib/sas_3j1x_x.c"
+#    include "lib/sas_gamma.c"
+#    include "mass_fractal.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2]); }
+#  elif REF_MODEL == 29
+#    include "lib/sas_3j1x_x.c"
+#    include "lib/sas_gamma.c"
+#    include "surface_fractal.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2]); }
+#  elif REF_MODEL == 30
+#    include "mass_surface_fractal.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3]); }
+#  elif REF_MODEL == 31
+#    include "star_polymer.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1]); }
+#  elif REF_MODEL == 32
+#    include "mono_gauss_coil.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1]); }
+double ref_form_volume(const double *p) { return form_volume(p[1]); }
+double ref_reff(int mode, const double *p) { return radius_effective(mode, p[1]); }
+#  else
+#    include "lib/sas_3j1x_x.c"
+#    include "lib/sas_gamma.c"
+#    include "lib/fractal_sq.c"
+#    include "fractal.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4], p[5]); }
+double ref_form_volume(const double *p) { return form_volume(p[1]); }
+#  endif
+#  if REF_MODEL != 32 && REF_MODEL != 33
+double ref_form_volume(const double *p) { (void)p; return 1.0; }
+#  endif
 #else
-#  error "REF_MODEL must be 1..22"
+#  error "REF_MODEL must be 1..33"
 #endif
 
 #ifdef NPAR_CALL_FQ

@@ -32,7 +32,10 @@
  *     core_multi_shell.c and onion.c (vector parameters: n shells of
  *     sld[n], thickness[n], ... packed as ten slots each),
  *     core_shell_ellipsoid.c, elliptical_cylinder.c (nested rules),
- *     core_shell_bicelle.c;
+ *     core_shell_bicelle.c; the Iq-only models guinier.c, gaussian_peak.c,
+ *     lorentz.c, dab.c, gel_fit.c, mass_fractal.c, surface_fractal.c,
+ *     mass_surface_fractal.c, star_polymer.c, mono_gauss_coil.c, fractal.c
+ *     with lib/fractal_sq.c and lib/sas_gamma.c:140-153;
  *     view/jitter rotation with psi qabc_rotation/qabc_apply
  *     kernel_iq.c:251-310
  *   - hard-sphere structure factor hardsphere.c:1-90
@@ -950,6 +953,168 @@ static oreal csb_iqac(oreal qab, oreal qc, const oreal *p)
     return 1.0e-4 * f * f;
 }
 
+/* ---- models defined by Iq alone (no <F>); volume 1 unless stated ---- */
+/* gamma extended below 1 (lib/sas_gamma.c:140-153) */
+static oreal o_gamma(oreal x) { return x < 0. ? O_PI / tgamma(1. - x) / sin(O_PI * x) : tgamma(x + 1) / x; }
+static oreal one_vol(const oreal *p) { (void)p; return 1.0; }
+static oreal zero_reff(int mode, const oreal *p) { (void)mode; (void)p; return 0.0; }
+
+/* guinier.c: p = rg */
+static void gui_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal e = fabs(p[0]) * p[0] * q * q / 3.0;
+    *f1 = 0.0;
+    *f2 = exp(-e);
+}
+/* gaussian_peak.c: p = peak_pos, sigma */
+static void gpk_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal x = (q - p[0]) / p[1];
+    *f1 = 0.0;
+    *f2 = exp(-0.5 * x * x);
+}
+/* lorentz.c: p = cor_length */
+static void lor_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal d = 1 + (q * p[0]) * (q * p[0]);
+    *f1 = 0.0;
+    *f2 = 1 / d;
+}
+/* dab.c: p = cor_length */
+static void dab_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    *f1 = 0.0;
+    *f2 = cb(p[0]) / sq(1 + sq(q * p[0]));
+}
+/* gel_fit.c: p = guinier_scale, lorentz_scale, rg, fractal_dim, cor_length */
+static void gel_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    oreal lor = sq(q * p[4]);
+    lor = 1.0 + ((p[3] + 1.0) / 3.0) * lor;
+    lor = pow(lor, p[3] / 2.0);
+    oreal ex = sq(q * p[2]);
+    ex = exp(-ex / 3.0);
+    *f1 = 0.0;
+    *f2 = p[1] / lor + p[0] * ex;
+}
+/* mass_fractal.c: p = radius, fractal_dim_mass, cutoff_length */
+static void mfr_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal pq = sq(o_sph_j1c(q * p[0]));
+    oreal s;
+    if (q > 0. && p[1] > 1.) {
+        const oreal dm = p[1] - 1.0;
+        const oreal t1 = o_gamma(dm) * sin(dm * atan(q * p[2]));
+        const oreal t2 = pow(p[2], dm);
+        const oreal t3 = pow(1.0 + sq(q * p[2]), -0.5 * dm);
+        s = t1 * t2 * t3 / q;
+    } else if (q > 0.) {
+        s = atan(q * p[2]) / q;
+    } else if (p[1] > 1.) {
+        s = pow(p[2], p[1]) * o_gamma(p[1]);
+    } else {
+        s = p[2];
+    }
+    *f1 = 0.0;
+    *f2 = pq * s;
+}
+/* surface_fractal.c: p = radius, fractal_dim_surf, cutoff_length */
+static void sfr_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal pq = sq(o_sph_j1c(q * p[0]));
+    const oreal m = 5.0 - p[1];
+    const oreal s = o_gamma(m) * pow(p[2], m) * pow(1.0 + sq(q * p[2]), -0.5 * m) * sin(-m * atan(q * p[2])) / q;
+    const oreal r = pq * s;
+    *f1 = 0.0;
+    *f2 = r > 0. ? r : 0.;
+}
+/* mass_surface_fractal.c: p = fractal_dim_mass, fractal_dim_surf, rg_cluster, rg_primary */
+static void msf_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal dm = 0.5 * p[0];
+    const oreal dt = 0.5 * (6.0 - (p[0] + p[1]));
+    const oreal t1 = dm == 0. ? 1.0 : pow(1.0 + sq(q * p[2]) / (3.0 * dm), -dm);
+    const oreal t2 = dt == 0. ? 1.0 : pow(1.0 + sq(q * p[3]) / (3.0 * dt), -dt);
+    *f1 = 0.0;
+    *f2 = t1 * t2;
+}
+/* star_polymer.c (double-precision series cutoff 0.03): p = rg_squared, arms */
+static void star_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    *f1 = 0.0;
+    if (q == 0.) { *f2 = 1.0; return; }
+    const int n = (int)(p[1] + 0.5);
+    const oreal u2 = p[0] * q * q;
+    const oreal v = (u2 * n) / (3 * n - 2);
+    oreal t1;
+    if (u2 < 0.03) t1 = 1. + v * (-1. / 3. + v * (1. / 12. + v * (-1. / 60. + v * (1. / 360. + v * (-1. / 2520)))));
+    else t1 = 2.0 * (v + expm1(-v)) / (v * v);
+    const oreal t2 = sq(expm1(-v) / v);
+    *f2 = (t1 - t2) / n + t2;
+}
+/* mono_gauss_coil.c (double precision: Pade below x = 0.5): p = i_zero, rg */
+static oreal mgc_reff(int mode, const oreal *p)
+{
+    switch (mode) {
+    default:
+    case 1: return p[1];
+    case 2: return 2.0 * p[1];
+    case 3: return 3.0 * p[1];
+    case 4: return sqrt(5.0 / 3.0) * p[1];
+    }
+}
+static void mgc_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal x = sq(q * p[1]);
+    oreal g;
+    if (x < 0.5) {
+        const oreal A1 = 1. / 12., A2 = 2. / 99., A3 = 1. / 2640., A4 = 1. / 23760., A5 = -1. / 1995840.;
+        const oreal B1 = 5. / 12., B2 = 5. / 66., B3 = 1. / 132., B4 = 1. / 2376., B5 = 1. / 95040.;
+        g = (((((A5 * x + A4) * x + A3) * x + A2) * x + A1) * x + 1.)
+          / (((((B5 * x + B4) * x + B3) * x + B2) * x + B1) * x + 1.);
+    } else {
+        g = 2.0 * (expm1(-x) + x) / (x * x);
+    }
+    *f1 = 0.0;
+    *f2 = p[0] * g;
+}
+/* fractal.c with lib/fractal_sq.c: p = volfraction, radius, fractal_dim,
+ * cor_length, sld_block, sld_solvent; volume 4/3 pi r^3 */
+static oreal frc_vol(const oreal *p) { return O_4PI_3 * cb(p[1]); }
+static void frc_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal r = p[1], D = p[2], xi = p[3];
+    oreal term;
+    if (q == 0.) {
+        term = pow(xi / r, D) * o_gamma(D + 1.);
+    } else if (D == 0.) {
+        term = 1.0;
+    } else if (D == 1.) {
+        term = atan(q * xi) / (q * r);
+    } else {
+        const oreal dm = D - 1.0;
+        const oreal t1 = o_gamma(D + 1.) / dm * sin(dm * atan(q * xi));
+        const oreal t2 = pow(q * r, -D);
+        const oreal t3 = pow(1.0 + 1.0 / sq(q * xi), -0.5 * dm);
+        term = t1 * t2 * t3;
+    }
+    const oreal s = 1.0 + term;
+    const oreal fq = frc_vol(p) * (p[4] - p[5]) * o_sph_j1c(q * r);
+    *f1 = 0.0;
+    *f2 = 1.e-4 * p[0] * s * fq * fq;
+}
+
 /* ---- shells of vector parameters: n = (int)(fp_n + 0.5) of a control
  * parameter with limits [0, 10] (core_multi_shell.c:12-18, onion.c:31-38) ---- */
 #define O_NSHELL 10
@@ -1063,9 +1228,12 @@ static int always_valid(const oreal *p) { (void)p; return 1; }
  * 4 core_shell_sphere, 6 parallelepiped, 7 triaxial_ellipsoid, 8-10 bcc,
  * fcc, sc paracrystal, 11 hollow_cylinder, 12 vesicle, 13 fuzzy_sphere,
  * 17 core_multi_shell, 18 onion, 19 core_shell_ellipsoid, 20
- * elliptical_cylinder, 21 core_shell_bicelle (the ids libsaship uses; 5 and 14-16, the
+ * elliptical_cylinder, 21 core_shell_bicelle, 22-32 the Iq-only models guinier,
+ * gaussian_peak, lorentz, dab, gel_fit, mass_fractal, surface_fractal,
+ * mass_surface_fractal, star_polymer, mono_gauss_coil, fractal (the ids
+ * libsaship uses; 5 and 14-16, the
  * structure factors, have entries of their own) */
-#define N_MODELS 22
+#define N_MODELS 33
 static const OModel MODELS[N_MODELS] = {
     {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0, 0, 0},
     {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0, 0, 0},
@@ -1091,6 +1259,17 @@ static const OModel MODELS[N_MODELS] = {
     {9, 3, 5, 1, {4, 5, 6}, always_valid, cse_vol, cse_reff, cse_fq, cse_iqac, 0, 0, 0},
     {8, 2, 5, 1, {3, 4}, always_valid, ecy_vol, ecy_reff, ecy_fq, 0, ecy_iqabc, 0, 0},
     {10, 4, 5, 1, {4, 5, 6, 7}, always_valid, csb_vol, csb_reff, csb_fq, csb_iqac, 0, 0, 0},
+    {1, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, gui_fq, 0, 0, 1, 0},
+    {2, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, gpk_fq, 0, 0, 1, 0},
+    {1, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, lor_fq, 0, 0, 1, 0},
+    {1, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, dab_fq, 0, 0, 1, 0},
+    {5, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, gel_fq, 0, 0, 1, 0},
+    {3, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, mfr_fq, 0, 0, 1, 0},
+    {3, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, sfr_fq, 0, 0, 1, 0},
+    {4, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, msf_fq, 0, 0, 1, 0},
+    {2, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, star_fq, 0, 0, 1, 0},
+    {2, 0, 1, 0, {-1}, always_valid, one_vol, mgc_reff, mgc_fq, 0, 0, 1, 0},
+    {6, 2, 1, 0, {4, 5}, always_valid, frc_vol, zero_reff, frc_fq, 0, 0, 1, 0},
 };
 static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npars > 0; }
 

@@ -556,6 +556,17 @@ int record_width(int id, int kind)
     case 19: return rv_of<sas::CoreShellEllipsoid>(kind);
     case 20: return rv_of<sas::EllipticalCylinder>(kind);
     case 21: return rv_of<sas::CoreShellBicelle>(kind);
+    case 22: return rv_of<sas::Guinier>(kind);
+    case 23: return rv_of<sas::GaussianPeak>(kind);
+    case 24: return rv_of<sas::Lorentz>(kind);
+    case 25: return rv_of<sas::Dab>(kind);
+    case 26: return rv_of<sas::GelFit>(kind);
+    case 27: return rv_of<sas::MassFractal>(kind);
+    case 28: return rv_of<sas::SurfaceFractal>(kind);
+    case 29: return rv_of<sas::MassSurfaceFractal>(kind);
+    case 30: return rv_of<sas::StarPolymer>(kind);
+    case 31: return rv_of<sas::MonoGaussCoil>(kind);
+    case 32: return rv_of<sas::Fractal>(kind);
     default: return rv_of<sas::FuzzySphere>(kind);
     }
 }
@@ -567,7 +578,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     const int id = b->m->id, ns = b->nsets;
     p.kind = kind;
     p.nout = kind == 0 ? 2 : 1;
-    p.nres = kind == 0 && !model_paracrystal(id) ? 2 : 1;    // models without <F>: <F^2> only
+    p.nres = kind == 0 && model_has_fq(id) ? 2 : 1;    // models without <F>: <F^2> only
     p.ndet = n_details;
     p.rv = record_width(id, kind);
     p.n_values = voff_in[ns];
@@ -821,6 +832,17 @@ int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, 
     case 19: return enqueue<sas::CoreShellEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 20: return enqueue<sas::EllipticalCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 21: return enqueue<sas::CoreShellBicelle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 22: return enqueue<sas::Guinier>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 23: return enqueue<sas::GaussianPeak>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 24: return enqueue<sas::Lorentz>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 25: return enqueue<sas::Dab>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 26: return enqueue<sas::GelFit>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 27: return enqueue<sas::MassFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 28: return enqueue<sas::SurfaceFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 29: return enqueue<sas::MassSurfaceFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 30: return enqueue<sas::StarPolymer>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 31: return enqueue<sas::MonoGaussCoil>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 32: return enqueue<sas::Fractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     default: return enqueue<sas::FuzzySphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }

@@ -82,22 +82,28 @@ inline int cu_count(int dev)
 // structure factors (radius_effective, volfraction and up to four more
 // parameters, nothing polydisperse, no sld: no magnetic values).  17
 // (core_multi_shell) and 18 (onion) have vector parameters, ten slots each;
-// 19-21 are core_shell_ellipsoid, elliptical_cylinder, core_shell_bicelle.
+// 19-21 are core_shell_ellipsoid, elliptical_cylinder, core_shell_bicelle;
+// 22-32 models defined by their intensity alone (no <F>, FP64): guinier,
+// gaussian_peak, lorentz, dab, gel_fit, mass_fractal, surface_fractal,
+// mass_surface_fractal, star_polymer, mono_gauss_coil, fractal.
 constexpr int MODEL_HARDSPHERE = 5;
-constexpr int MODEL_LAST = 21;
+constexpr int MODEL_LAST = 32;
 inline int model_npars(int id)
 {
-    static const int np[22] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6, 24, 44, 9, 8, 10};
+    static const int np[33] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6, 24, 44, 9, 8, 10,
+                               1, 2, 1, 1, 5, 3, 3, 4, 2, 2, 6};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[22] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0, 5, 5, 5, 5, 5};
+    static const int mp[33] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0, 5, 5, 5, 5, 5,
+                               0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[22] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8, 66, 116, 24, 20, 28};
+    static const int nv[33] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8, 66, 116, 24, 20, 28,
+                               3, 4, 3, 3, 7, 5, 5, 6, 4, 4, 18};
     return nv[id];
 }
 inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || (id >= 6 && id <= 11) || (id >= 19 && id <= 21); }
@@ -107,6 +113,12 @@ inline bool model_abc(int id) { return (id >= 6 && id <= 10) || id == 20; }
 inline bool model_paracrystal(int id) { return id >= 8 && id <= 10; }
 inline bool model_nested(int id) { return id == 6 || id == 7 || id == 20; }
 inline bool model_structure(int id) { return id == MODEL_HARDSPHERE || id == 14 || id == 15 || id == 16; }
+// models defined by I(q) alone (FP64 direct evaluation)
+inline bool model_direct(int id) { return id >= 22 && id <= 32; }
+// the 1D kernel returns <F> beside <F^2> (two outputs per q)
+inline bool model_has_fq(int id) { return !model_structure(id) && !model_paracrystal(id) && !model_direct(id); }
+// models with sld parameters have a magnetic kernel
+inline bool model_magnetic(int id) { return model_nvalues(id) > 2 + model_npars(id); }
 
 // Separable cylinder 1D meshes: only the radius (parameter 2) and length
 // (parameter 3) levels have more than one point, each at most SEP_MAX.
@@ -148,6 +160,7 @@ inline int validate_set(int id, int kind, const int32_t *details, int64_t n_deta
             return fail(SASHIP_EINVAL, where + "inconsistent details buffer at pd level " + std::to_string(i));
     }
     if (kind == 2 && model_structure(id)) return fail(SASHIP_EINVAL, where + "structure factors have no magnetic kernel");
+    if (kind == 2 && !model_magnetic(id)) return fail(SASHIP_EINVAL, where + "the model has no sld: no magnetic kernel");
     if (kind != 0 && model_oriented(id)) {
         const int tp = details[4 * mp + 3];
         const int last = tp + (model_abc(id) ? 2 : 1);

@@ -36,6 +36,9 @@ constexpr int BIG_RECORD = 8;
 constexpr double DEG = 0.017453292519943295;   // pi/180
 
 // Does model M have the FP32 2D path (REC2F, rec2f, geom2f)?
+// direct-intensity models (Direct<D> in models.h): I itself, not an amplitude
+template <class M, class = void> struct is_direct : std::false_type {};
+template <class M> struct is_direct<M, std::void_t<decltype(M::DIRECT)>> : std::bool_constant<M::DIRECT> {};
 template <class M, class = void> struct has_f32_2d : std::false_type {};
 template <class M> struct has_f32_2d<M, std::void_t<decltype(M::REC2F)>> : std::true_type {};
 template <class M> constexpr int rec2f_floats()
@@ -419,15 +422,19 @@ __device__ __forceinline__ void spin_weights(double in_spin, double out_spin, do
 template <class M, bool MAG>
 struct Contrast {
     static constexpr int NX = MAG ? 6 : 1;
+    static constexpr bool DIRECT = is_direct<M>::value;
+    static constexpr int NM1 = M::NMAG > 0 ? M::NMAG : 1;
     float con[NX][M::NCON];
     float xsw[NX];
     bool active[NX];
     bool on = true;    // magnetic: q != 0 (the reference adds nothing at q = 0)
+    // direct-intensity models: the effective slds of each cross section
+    double effd[DIRECT && MAG ? NX : 1][DIRECT && MAG ? NM1 : 1];
 
     __device__ __forceinline__ void setup(const double *values, double qx, double qy)
     {
         constexpr int NP = M::NPARS;
-        double sld[M::NMAG];
+        double sld[NM1];
 #pragma unroll
         for (int k = 0; k < M::NMAG; k++) sld[k] = values[2 + M::sld_param(k)];
         if constexpr (!MAG) {
@@ -445,7 +452,7 @@ struct Contrast {
             const double qn = q_ok ? sqrt(qx * qx + qy * qy) : 1.0;
             const double ux = qx / qn, uy = qy / qn;
             const double uu = q_ok ? ux * ux + uy * uy : 1.0;
-            double eff[NX][M::NMAG];
+            double eff[NX][NM1];
 #pragma unroll
             for (int k = 0; k < M::NMAG; k++) {
                 const double *mv = values + 2 + NP + 4 + 3 * k;
@@ -458,7 +465,12 @@ struct Contrast {
             for (int x = 0; x < NX; x++) {
                 active[x] = w[x] > 1.e-8;                    // kernel_iq.c:761
                 xsw[x] = (active[x] && q_ok) ? (float)w[x] : 0.0f;
-                M::contrasts(values + 2, eff[x], con[x]);
+                if constexpr (DIRECT) {
+#pragma unroll
+                    for (int k = 0; k < NM1; k++) effd[x][k] = eff[x][k];
+                } else {
+                    M::contrasts(values + 2, eff[x], con[x]);
+                }
             }
         }
     }
@@ -522,6 +534,34 @@ struct Lane2D {
 template <class M, bool MAG, class P>
 __device__ __forceinline__ float point2d(P rec, const Lane2D &L, const Contrast<M, MAG> &con)
 {
+    if constexpr (is_direct<M>::value) {
+        // I(|q|) at the record's parameters; magnetic: once per active cross
+        // section with its effective slds in place
+        constexpr int RD = rv2d<M>();
+        float r[4 * RD];
+#pragma unroll
+        for (int v = 0; v < RD; v++) {
+            const float4 x = ld4(rec + v);
+            r[4 * v] = x.x; r[4 * v + 1] = x.y; r[4 * v + 2] = x.z; r[4 * v + 3] = x.w;
+        }
+        double p[M::NPARS];
+        const double w = M::load(r, p);
+        if constexpr (!MAG) {
+            return (float)(w * M::iq(L.qmag, p));
+        } else {
+            if (!con.on) return 0.0f;
+            double s = 0.0;
+#pragma unroll
+            for (int x = 0; x < Contrast<M, MAG>::NX; x++) {
+                if (con.active[x]) {
+#pragma unroll
+                    for (int k = 0; k < M::NMAG; k++) p[M::sld_param(k)] = con.effd[x][k];
+                    s += (double)con.xsw[x] * M::iq(L.qmag, p);
+                }
+            }
+            return (float)(w * s);
+        }
+    }
     float g[M::NCON];
     if constexpr (has_f32_2d<M>::value) {
         constexpr int RF = rv2f<M>();

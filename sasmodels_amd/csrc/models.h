@@ -1855,4 +1855,241 @@ struct Onion {
     }
 };
 
+// ---------------------------------------------------------------- direct-intensity models
+// Models the reference defines by Iq alone (no <F>): I(q) is evaluated in
+// FP64 at every (q, mesh point) from the point's parameters.  2D data use
+// I(|q|), the reference's Iqxy of an isotropic Iq; the magnetic kernel
+// evaluates I once per active spin cross section with that cross section's
+// effective slds in place, as the reference does.  Records hold the kernel
+// parameters and the weight in FP64.  A model D derives from DirectBase and
+// supplies NPARS, iq(q, p) and, where it has them, NMAG and sld_param(k),
+// MAX_PD, volume(p) and reff(mode, p).
+
+// Gamma with the reference's extension below 1 (lib/sas_gamma.c:140-153):
+// tgamma(x + 1)/x for x >= 0, the reflection formula below 0.
+SAS_HD double sas_gamma_d(double x)
+{
+    return x < 0.0 ? PI_D / tgamma(1.0 - x) / sin(PI_D * x) : tgamma(x + 1.0) / x;
+}
+
+struct DirectBase {
+    static constexpr int NMAG = 0, MAX_PD = 0;
+    SAS_HD static int sld_param(int) { return 0; }
+    __device__ static double volume(const double *) { return 1.0; }
+    __device__ static double reff(int, const double *) { return 0.0; }
+};
+
+template <class D>
+struct Direct : D {
+    static constexpr int NPARS = D::NPARS, NCON = 1;
+    static constexpr bool SHARED_1D = false, ORIENTED = false, ABC = false, FQ = false, FP64 = true;
+    static constexpr bool HOLLOW = false, DIRECT = true;
+    static constexpr int REC1D = 2 * (NPARS + 1), REC2D = REC1D, A2D = 0;
+    __device__ static bool valid(const double *) { return true; }
+    __device__ static void rec1d(const double *p, double w, float *r)
+    {
+#pragma unroll
+        for (int i = 0; i < NPARS; i++) put_d(r, 2 * i, p[i]);
+        put_d(r, 2 * NPARS, w);
+    }
+    // the record's parameters and weight
+    __device__ static double load(const float *r, double *p)
+    {
+#pragma unroll
+        for (int i = 0; i < NPARS; i++) p[i] = get_d(r, 2 * i);
+        return get_d(r, 2 * NPARS);
+    }
+    __device__ static void eval1d_d(double q, const float *r, const Quad &, double &f1, double &f2)
+    {
+        double p[NPARS];
+        const double w = load(r, p);
+        f1 = 0.0;
+        f2 = w * D::iq(q, p);
+    }
+    template <bool FIXED>
+    __device__ static void eval1d(float, const float *, const Quad &, float &f1, float &f2)
+    {
+        f1 = f2 = 0.0f;    // FP64 models use eval1d_d
+    }
+    __device__ static void rec2d(const double *p, double w, float *r) { rec1d(p, w, r); }
+    __device__ static void geom2d(const QRot &, const float *, float *g) { g[0] = 0.0f; }   // not used
+    SAS_HD static void contrasts(const double *, const double *, float *con) { con[0] = 0.0f; }   // not used
+};
+
+// guinier.c: exp(-|rg| rg q^2/3)  (guinier.py; p = rg)
+struct GuinierD : DirectBase {
+    static constexpr int ID = 22, NPARS = 1;
+    __device__ static double iq(double q, const double *p) { return exp(-(fabs(p[0]) * p[0] * q * q / 3.0)); }
+};
+// gaussian_peak.c: exp(-((q - q0)/sigma)^2/2)  (p = peak_pos, sigma)
+struct GaussianPeakD : DirectBase {
+    static constexpr int ID = 23, NPARS = 2;
+    __device__ static double iq(double q, const double *p)
+    {
+        const double x = (q - p[0]) / p[1];
+        return exp(-0.5 * x * x);
+    }
+};
+// lorentz.c: 1/(1 + (q xi)^2)  (p = cor_length)
+struct LorentzD : DirectBase {
+    static constexpr int ID = 24, NPARS = 1;
+    __device__ static double iq(double q, const double *p)
+    {
+        const double x = q * p[0];
+        return 1.0 / (1.0 + x * x);
+    }
+};
+// dab.c (Debye-Anderson-Brumberger): xi^3/(1 + (q xi)^2)^2  (p = cor_length)
+struct DabD : DirectBase {
+    static constexpr int ID = 25, NPARS = 1;
+    __device__ static double iq(double q, const double *p)
+    {
+        const double x = q * p[0], d = 1.0 + x * x;
+        return cube_d(p[0]) / (d * d);
+    }
+};
+// gel_fit.c: Lorentz-like term over (1 + (D + 1)/3 (q xi)^2)^(D/2) plus a
+// Guinier term  (p = guinier_scale, lorentz_scale, rg, fractal_dim, cor_length)
+struct GelFitD : DirectBase {
+    static constexpr int ID = 26, NPARS = 5;
+    __device__ static double iq(double q, const double *p)
+    {
+        const double x = q * p[4], g = q * p[2];
+        const double lorentz = pow(1.0 + ((p[3] + 1.0) / 3.0) * (x * x), 0.5 * p[3]);
+        return p[1] / lorentz + p[0] * exp(-(g * g) / 3.0);
+    }
+};
+// mass_fractal.c: sphere form factor times the mass-fractal S(q) with its
+// limits at q = 0 and D = 1  (p = radius, fractal_dim_mass, cutoff_length)
+struct MassFractalD : DirectBase {
+    static constexpr int ID = 27, NPARS = 3;
+    __device__ static double iq(double q, const double *p)
+    {
+        const double f = sph_j1c_dd(q * p[0]), c = p[2], D = p[1];
+        double s;
+        if (q > 0.0 && D > 1.0) {
+            const double dm = D - 1.0, x = q * c;
+            s = sas_gamma_d(dm) * sin(dm * atan(x)) * pow(c, dm) * pow(1.0 + x * x, -0.5 * dm) / q;
+        } else if (q > 0.0) {
+            s = atan(q * c) / q;
+        } else if (D > 1.0) {
+            s = pow(c, D) * sas_gamma_d(D);
+        } else {
+            s = c;
+        }
+        return f * f * s;
+    }
+};
+// surface_fractal.c: sphere form factor times the surface-fractal S(q),
+// negative results clipped to 0  (p = radius, fractal_dim_surf, cutoff_length)
+struct SurfaceFractalD : DirectBase {
+    static constexpr int ID = 28, NPARS = 3;
+    __device__ static double iq(double q, const double *p)
+    {
+        const double f = sph_j1c_dd(q * p[0]), m = 5.0 - p[1], x = q * p[2];
+        const double s = sas_gamma_d(m) * pow(p[2], m) * pow(1.0 + x * x, -0.5 * m) * sin(-m * atan(x)) / q;
+        const double i = f * f * s;
+        return i > 0.0 ? i : 0.0;
+    }
+};
+// mass_surface_fractal.c: product of two generalised Guinier-Porod terms
+// (p = fractal_dim_mass, fractal_dim_surf, rg_cluster, rg_primary)
+struct MassSurfaceFractalD : DirectBase {
+    static constexpr int ID = 29, NPARS = 4;
+    __device__ static double iq(double q, const double *p)
+    {
+        const double dm = 0.5 * p[0], dt = 0.5 * (6.0 - (p[0] + p[1]));
+        const double a = q * p[2], b = q * p[3];
+        const double t1 = dm == 0.0 ? 1.0 : pow(1.0 + a * a / (3.0 * dm), -dm);
+        const double t2 = dt == 0.0 ? 1.0 : pow(1.0 + b * b / (3.0 * dt), -dt);
+        return t1 * t2;
+    }
+};
+// star_polymer.c: Benoit's star of f Gaussian arms (f rounded to the
+// nearest integer), the series of (v + e^-v - 1) 2/v^2 below (q Rg)^2 = 0.03
+// (p = rg_squared, arms)
+struct StarPolymerD : DirectBase {
+    static constexpr int ID = 30, NPARS = 2;
+    __device__ static double iq(double q, const double *p)
+    {
+        if (q == 0.0) return 1.0;
+        const int n = (int)(p[1] + 0.5);
+        const double u2 = p[0] * q * q;
+        const double v = (u2 * n) / (3 * n - 2);
+        const double t1 = u2 < 0.03
+            ? 1.0 + v * (-1.0 / 3.0 + v * (1.0 / 12.0 + v * (-1.0 / 60.0 + v * (1.0 / 360.0 + v * (-1.0 / 2520.0)))))
+            : 2.0 * (v + expm1(-v)) / (v * v);
+        const double e = expm1(-v) / v;
+        return (t1 - e * e) / n + e * e;
+    }
+};
+// mono_gauss_coil.c: Debye's Gaussian coil, i_zero 2(e^-x + x - 1)/x^2 with
+// x = (q Rg)^2 and a Pade approximant below x = 0.5  (p = i_zero, rg)
+struct MonoGaussCoilD : DirectBase {
+    static constexpr int ID = 31, NPARS = 2, MAX_PD = 1;
+    __device__ static double reff(int mode, const double *p)
+    {
+        switch (mode) {
+        default:
+        case 1: return p[1];
+        case 2: return 2.0 * p[1];
+        case 3: return 3.0 * p[1];
+        case 4: return sqrt(5.0 / 3.0) * p[1];
+        }
+    }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double qr = q * p[1], x = qr * qr;
+        double g;
+        if (x < 0.5) {
+            // [5/5] Pade approximant of 2 (e^-x + x - 1)/x^2 about 0
+            const double n = ((((-1.0 / 1995840.0 * x + 1.0 / 23760.0) * x + 1.0 / 2640.0) * x + 2.0 / 99.0) * x
+                              + 1.0 / 12.0) * x + 1.0;
+            const double d = ((((1.0 / 95040.0 * x + 1.0 / 2376.0) * x + 1.0 / 132.0) * x + 5.0 / 66.0) * x
+                              + 5.0 / 12.0) * x + 1.0;
+            g = n / d;
+        } else {
+            g = 2.0 * (expm1(-x) + x) / (x * x);
+        }
+        return p[0] * g;
+    }
+};
+// fractal.c with lib/fractal_sq.c (Teixeira's S(q), eq. 15): spheres of
+// radius r in a fractal aggregate  (p = volfraction, radius, fractal_dim,
+// cor_length, sld_block, sld_solvent)
+struct FractalD : DirectBase {
+    static constexpr int ID = 32, NPARS = 6, NMAG = 2, MAX_PD = 1;
+    SAS_HD static int sld_param(int k) { return 4 + k; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[1]); }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double r = p[1], D = p[2], xi = p[3];
+        double term;
+        if (q == 0.0) {
+            term = pow(xi / r, D) * sas_gamma_d(D + 1.0);
+        } else if (D == 0.0) {
+            term = 1.0;
+        } else if (D == 1.0) {
+            term = atan(q * xi) / (q * r);
+        } else {
+            const double dm = D - 1.0, x = q * xi;
+            term = sas_gamma_d(D + 1.0) / dm * sin(dm * atan(x)) * pow(q * r, -D) * pow(1.0 + 1.0 / (x * x), -0.5 * dm);
+        }
+        const double f = volume(p) * (p[4] - p[5]) * sph_j1c_dd(q * r);
+        return 1e-4 * p[0] * (1.0 + term) * f * f;
+    }
+};
+
+using Guinier = Direct<GuinierD>;
+using GaussianPeak = Direct<GaussianPeakD>;
+using Lorentz = Direct<LorentzD>;
+using Dab = Direct<DabD>;
+using GelFit = Direct<GelFitD>;
+using MassFractal = Direct<MassFractalD>;
+using SurfaceFractal = Direct<SurfaceFractalD>;
+using MassSurfaceFractal = Direct<MassSurfaceFractalD>;
+using StarPolymer = Direct<StarPolymerD>;
+using MonoGaussCoil = Direct<MonoGaussCoilD>;
+using Fractal = Direct<FractalD>;
+
 }  // namespace sas

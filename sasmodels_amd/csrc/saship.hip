@@ -159,6 +159,17 @@ int with_model(int id, F &&f)
     case 19: return f((sas::CoreShellEllipsoid *)nullptr);
     case 20: return f((sas::EllipticalCylinder *)nullptr);
     case 21: return f((sas::CoreShellBicelle *)nullptr);
+    case 22: return f((sas::Guinier *)nullptr);
+    case 23: return f((sas::GaussianPeak *)nullptr);
+    case 24: return f((sas::Lorentz *)nullptr);
+    case 25: return f((sas::Dab *)nullptr);
+    case 26: return f((sas::GelFit *)nullptr);
+    case 27: return f((sas::MassFractal *)nullptr);
+    case 28: return f((sas::SurfaceFractal *)nullptr);
+    case 29: return f((sas::MassSurfaceFractal *)nullptr);
+    case 30: return f((sas::StarPolymer *)nullptr);
+    case 31: return f((sas::MonoGaussCoil *)nullptr);
+    case 32: return f((sas::Fractal *)nullptr);
     default: return fail(SASHIP_EINVAL, "model " + std::to_string(id) + " has no mesh kernel");
     }
 }
@@ -662,7 +673,7 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, "")) return e;
     std::lock_guard<std::mutex> lock(k->mu);
     HIP_TRY(hipSetDevice(k->dev));
-    const int nq = k->nq, nout = kind == 0 && !model_structure(id) && !model_paracrystal(id) ? 2 : 1;
+    const int nq = k->nq, nout = kind == 0 && model_has_fq(id) ? 2 : 1;
     const size_t vbytes = sizeof(double) * (size_t)n_values;
     const size_t dbytes = sizeof(int32_t) * (size_t)n_details;
     const size_t doff = (vbytes + 255) & ~(size_t)255;

@@ -24,7 +24,10 @@ triaxial_ellipsoid.py:143-158, bcc_paracrystal.py:186-194,
 fcc_paracrystal.py:184-192, sc_paracrystal.py:181-189, hollow_cylinder.py:90-98,
 vesicle.py:87-96, fuzzy_sphere.py:113-117, core_multi_shell.py:91-103,
 onion.py:323-333, core_shell_ellipsoid.py:139-149,
-elliptical_cylinder.py:115-122, core_shell_bicelle.py:140-151, hardsphere.py:84-88,
+elliptical_cylinder.py:115-122, core_shell_bicelle.py:140-151, guinier.py, gaussian_peak.py,
+lorentz.py, dab.py, gel_fit.py, mass_fractal.py, surface_fractal.py,
+mass_surface_fractal.py, star_polymer.py, mono_gauss_coil.py, fractal.py
+(the ``parameters`` tables), hardsphere.py:84-88,
 stickyhardsphere.py:110-121, squarewell.py:83-92, hayter_msa.py:110-117),
 with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
@@ -495,6 +498,59 @@ _MODEL_TABLE = {
     ], dict(radius_effective_modes=["excluded volume", "equivalent volume sphere", "outer rim radius",
                                     "half outer thickness", "half diagonal"],
             xy_mode="qac", has_quadrature=True)),
+    # models defined by I(q) alone (no <F>), evaluated in FP64
+    "guinier": (22, [
+        ["rg", "Ang", 60.0, [-inf, inf], "", "Radius of Gyration"],
+    ], dict(have_Fq=False)),
+    "gaussian_peak": (23, [
+        ["peak_pos", "1/Ang", 0.05, [-inf, inf], "", "Peak position"],
+        ["sigma", "1/Ang", 0.005, [0, inf], "", "Peak width (standard deviation)"],
+    ], dict(have_Fq=False)),
+    "lorentz": (24, [
+        ["cor_length", "Ang", 50.0, [0, inf], "", "Screening length"],
+    ], dict(have_Fq=False)),
+    "dab": (25, [
+        ["cor_length", "Ang", 50.0, [0, inf], "", "correlation length"],
+    ], dict(have_Fq=False)),
+    "gel_fit": (26, [
+        ["guinier_scale", "cm^-1", 1.7, [-inf, inf], "", "Guinier term scale"],
+        ["lorentz_scale", "cm^-1", 3.5, [-inf, inf], "", "Lorentz term scale"],
+        ["rg", "Ang", 104.0, [2, inf], "", "Radius of gyration"],
+        ["fractal_dim", "", 2.0, [0, inf], "", "Fractal exponent"],
+        ["cor_length", "Ang", 16.0, [0, inf], "", "Correlation length"],
+    ], dict(have_Fq=False)),
+    "mass_fractal": (27, [
+        ["radius", "Ang", 10.0, [0.0, inf], "", "Particle radius"],
+        ["fractal_dim_mass", "", 1.9, [1.0, 6.0], "", "Mass fractal dimension"],
+        ["cutoff_length", "Ang", 100.0, [0.0, inf], "", "Cut-off length"],
+    ], dict(have_Fq=False)),
+    "surface_fractal": (28, [
+        ["radius", "Ang", 10.0, [0, inf], "", "Particle radius"],
+        ["fractal_dim_surf", "", 2.0, [1, 3], "", "Surface fractal dimension"],
+        ["cutoff_length", "Ang", 500., [0.0, inf], "", "Cut-off Length"],
+    ], dict(have_Fq=False)),
+    "mass_surface_fractal": (29, [
+        ["fractal_dim_mass", "", 1.8, [0.0, 6.0], "", "Mass fractal dimension"],
+        ["fractal_dim_surf", "", 2.3, [0.0, 6.0], "", "Surface fractal dimension"],
+        ["rg_cluster", "Ang", 4000., [0.0, inf], "", "Cluster radius of gyration"],
+        ["rg_primary", "Ang", 86.7, [0.0, inf], "", "Primary particle radius of gyration"],
+    ], dict(have_Fq=False)),
+    "star_polymer": (30, [
+        ["rg_squared", "Ang^2", 100.0, [0.0, inf], "", "Ensemble radius of gyration SQUARED of the full polymer"],
+        ["arms", "", 3, [1.0, 6.0], "", "Number of arms in the model (integer)"],
+    ], dict(have_Fq=False)),
+    "mono_gauss_coil": (31, [
+        ["i_zero", "1/cm", 70.0, [0.0, inf], "", "Intensity at q=0"],
+        ["rg", "Ang", 75.0, [0.0, inf], "volume", "Radius of gyration"],
+    ], dict(have_Fq=False, radius_effective_modes=["R_g", "2R_g", "3R_g", "sqrt(5/3)*R_g"])),
+    "fractal": (32, [
+        ["volfraction", "", 0.05, [0.0, 1], "", "volume fraction of blocks"],
+        ["radius", "Ang", 5.0, [0.0, inf], "volume", "radius of particles"],
+        ["fractal_dim", "", 2.0, [0.0, 6.0], "", "fractal dimension"],
+        ["cor_length", "Ang", 100.0, [0.0, inf], "", "cluster correlation length"],
+        ["sld_block", "1e-6/Ang^2", 2.0, [-inf, inf], "sld", "scattering length density of particles"],
+        ["sld_solvent", "1e-6/Ang^2", 6.4, [-inf, inf], "sld", "scattering length density of solvent"],
+    ], dict(have_Fq=False)),
     # structure factor: Iq only (no F), no volume, nothing polydisperse;
     # evaluated in FP64 (the reference marks it double precision only)
     "hardsphere": (5, [

@@ -184,7 +184,63 @@ CORE_SHELL_BICELLE = [
     ({"theta": 80., "phi": 10.}, (0.1*cos(pi/6.0), 0.1*sin(pi/6.0)), 2.81048892474),
 ]
 
+# guinier.py, lorentz.py, gel_fit.py, mass_fractal.py, surface_fractal.py,
+# mass_surface_fractal.py, star_polymer.py, mono_gauss_coil.py, fractal.py
+# (the ``tests`` lists; gaussian_peak.py and dab.py have none)
+GUINIER = [({"rg": 31.5}, 0.005, 0.992756)]
+LORENTZ = [({"cor_length": 250}, 0.01, 0.138931)]
+GEL_FIT = [
+    ({"guinier_scale": 1.0, "lorentz_scale": 1.0, "rg": 10.0, "fractal_dim": 10.0, "cor_length": 20.0,
+      "background": 0.0}, 0.1, 0.716532),
+    ({"guinier_scale": 4.0, "lorentz_scale": 10.0, "rg": 500.0, "fractal_dim": 1.0, "cor_length": 20.0,
+      "background": 20.0}, 5.0, 20.1224653026),
+]
+MASS_FRACTAL = [
+    ({"radius": 10.0, "fractal_dim_mass": 1.9, "cutoff_length": 100.0}, 0.05, 279.59422),
+    ({"radius": 2.0, "fractal_dim_mass": 3.3, "cutoff_length": 1.0}, 0.5, 1.29116774904),
+    ({"radius": 1.0, "fractal_dim_mass": 1.3, "cutoff_length": 1.0, "background": 0.8}, 0.001, 1.69747015932),
+    ({"radius": 1.0, "fractal_dim_mass": 2.3, "cutoff_length": 1.0, "scale": 10.0}, 0.051, 11.6237966145),
+]
+SURFACE_FRACTAL = [
+    ({"radius": 10.0, "fractal_dim_surf": 2.0, "cutoff_length": 500.0}, 0.05, 301428.66016),
+    ({"radius": 1.0, "fractal_dim_surf": 1.0, "cutoff_length": 10.0}, 0.332070182643, 1125.00421004),
+    ({"radius": 3.5, "fractal_dim_surf": 0.1, "cutoff_length": 30.0, "background": 0.01}, 5.0, 0.00999998891322),
+    ({"radius": 3.0, "fractal_dim_surf": 1.0, "cutoff_length": 33.0, "scale": 0.1}, 0.51, 2.50120147004),
+]
+MASS_SURFACE_FRACTAL = [
+    ({"fractal_dim_mass": 1.8, "fractal_dim_surf": 2.3, "rg_cluster": 86.7, "rg_primary": 4000.0,
+      "background": 0.0}, 0.05, 1.77537e-05),
+    ({"fractal_dim_mass": 3.3, "fractal_dim_surf": 1.0, "rg_cluster": 4000.0, "rg_primary": 90.0},
+     0.001, 0.0932516614456),
+    ({"fractal_dim_mass": 1.3, "fractal_dim_surf": 2.0, "rg_cluster": 2000.0, "rg_primary": 90.0,
+      "background": 0.8}, 0.001, 1.28296431786),
+    ({"fractal_dim_mass": 2.3, "fractal_dim_surf": 3.1, "rg_cluster": 1000.0, "rg_primary": 30.0,
+      "scale": 10.0, "background": 0.0}, 0.051, 0.00333804044899),
+]
+STAR_POLYMER = [
+    ({"rg_squared": 2.0, "arms": 3.3}, 0.5, 0.8518871750393033),
+    ({"rg_squared": 2.0, "arms": 3.0}, 0.5, 0.8518871750393033),
+    ({"rg_squared": 1.0, "arms": 2.0, "background": 1.8}, 1.0, 2.53575888234),
+]
+MONO_GAUSS_COIL = [
+    ({"scale": 1.0, "i_zero": 70.0, "rg": 75.0, "background": 0.0}, [0.0106939, 0.469418], [57.1241, 0.112859]),
+]
+FRACTAL = [
+    ({}, 0.0005, 40.4980069872),
+    ({}, 0.234734468938, 0.0947143166058),
+    ({}, 0.5, 0.0176878183458),
+]
+
 MODEL_KATS = {
+    "guinier": GUINIER,
+    "lorentz": LORENTZ,
+    "gel_fit": GEL_FIT,
+    "mass_fractal": MASS_FRACTAL,
+    "surface_fractal": SURFACE_FRACTAL,
+    "mass_surface_fractal": MASS_SURFACE_FRACTAL,
+    "star_polymer": STAR_POLYMER,
+    "mono_gauss_coil": MONO_GAUSS_COIL,
+    "fractal": FRACTAL,
     "core_shell_ellipsoid": CORE_SHELL_ELLIPSOID,
     "elliptical_cylinder": ELLIPTICAL_CYLINDER,
     "core_shell_bicelle": CORE_SHELL_BICELLE,

@@ -1,0 +1,108 @@
+"""
+GPU parity of the models defined by I(q) alone (no <F>; FP64 evaluation on
+the device): 1D with polydispersity where the model has a volume parameter,
+2D at |q| with a pixel at q = 0, the magnetic kernel of the one model with
+slds (fractal), batched sets, and the magnetic call refused for models
+without slds.
+"""
+
+import zlib
+
+import numpy as np
+import pytest
+
+from gpu_helpers import buffers, compare_raw, hip_model, masked_relerr
+from oracle.oracle import OracleKernel
+from sasmodels_amd import direct_model
+from sasmodels_amd.details import make_kernel_args
+from sasmodels_amd.kernelhip import HipBatch
+
+pytestmark = pytest.mark.gpu
+
+PARS = {
+    "guinier": lambda r: dict(rg=r.uniform(-40, 120)),
+    "gaussian_peak": lambda r: dict(peak_pos=r.uniform(0.01, 0.3), sigma=10**r.uniform(-3, -1)),
+    "lorentz": lambda r: dict(cor_length=10**r.uniform(0, 3)),
+    "dab": lambda r: dict(cor_length=10**r.uniform(0, 3)),
+    "gel_fit": lambda r: dict(guinier_scale=r.uniform(0, 5), lorentz_scale=r.uniform(0, 5), rg=r.uniform(2, 300),
+                              fractal_dim=r.uniform(0.5, 4), cor_length=r.uniform(1, 100)),
+    "mass_fractal": lambda r: dict(radius=r.uniform(1, 30), fractal_dim_mass=r.uniform(1, 3),
+                                   cutoff_length=r.uniform(10, 500)),
+    "surface_fractal": lambda r: dict(radius=r.uniform(1, 30), fractal_dim_surf=r.uniform(1, 3),
+                                      cutoff_length=r.uniform(10, 1000)),
+    "mass_surface_fractal": lambda r: dict(fractal_dim_mass=r.uniform(0.5, 3), fractal_dim_surf=r.uniform(1, 3),
+                                           rg_cluster=r.uniform(100, 5000), rg_primary=r.uniform(5, 100)),
+    "star_polymer": lambda r: dict(rg_squared=r.uniform(10, 1e4), arms=r.uniform(1, 6)),
+    "mono_gauss_coil": lambda r: dict(i_zero=r.uniform(1, 100), rg=r.uniform(10, 300), rg_pd=0.2,
+                                      rg_pd_n=int(r.integers(1, 20))),
+    "fractal": lambda r: dict(volfraction=r.uniform(0.01, 0.5), radius=r.uniform(2, 20), fractal_dim=r.uniform(1.2, 3),
+                              cor_length=r.uniform(20, 500), sld_block=r.uniform(-1, 7), radius_pd=0.1,
+                              radius_pd_n=int(r.integers(1, 12))),
+}
+
+
+def _rng(model, salt):
+    return np.random.default_rng(salt + zlib.crc32(model.encode()) % 1000)
+
+
+@pytest.mark.parametrize("model", list(PARS))
+def test_iq_only_1d(model):
+    r = _rng(model, 800)
+    q = np.concatenate([[0.0], np.logspace(-4, 0, 120)])
+    for trial in range(4):
+        pars = PARS[model](r)
+        print("1d", model, trial, pars, flush=True)
+        hk, ok, det, val, mag = buffers(model, [q], pars)
+        hk._call_kernel(det, val, 0.0, mag, 1)
+        compare_raw(hk.result, ok.raw(det, val, 0.0, mag, 1), q.size, 1, rtol=1e-9)
+        hk.release()
+
+
+@pytest.mark.parametrize("model", list(PARS))
+def test_iq_only_2d(model):
+    r = _rng(model, 900)
+    x = np.linspace(-0.3, 0.3, 17)             # the centre pixel is q = 0
+    qv = [a.ravel() for a in np.meshgrid(x, x)]
+    pars = PARS[model](r)
+    print("2d", model, pars, flush=True)
+    hk, ok, det, val, mag = buffers(model, qv, pars)
+    hk._call_kernel(det, val, 0.0, mag, 0)
+    compare_raw(hk.result, ok.raw(det, val, 0.0, mag, 0), qv[0].size, 1)
+    hk.release()
+
+
+def test_iq_only_magnetic_fractal():
+    x = np.linspace(-0.3, 0.3, 20)
+    qv = [a.ravel() for a in np.meshgrid(x, x)]
+    qv[0][0] = qv[1][0] = 0.0
+    pars = dict(volfraction=0.1, radius=8, fractal_dim=2.2, cor_length=150, sld_block_M0=3, sld_block_mtheta=40,
+                sld_block_mphi=20, sld_solvent_M0=1, up_frac_i=0.2, up_frac_f=0.7, up_theta=30, up_phi=10,
+                radius_pd=0.1, radius_pd_n=5)
+    hk, ok, det, val, mag = buffers("fractal", qv, pars)
+    assert mag
+    hk._call_kernel(det, val, 0.0, mag, 0)
+    compare_raw(hk.result, ok.raw(det, val, 0.0, mag, 0), qv[0].size, 1)
+    assert hk.result[0] == 0.0
+    hk.release()
+
+
+def test_iq_only_batch_and_no_magnetic_kernel():
+    from sasmodels_amd._lib import HipError
+    m = hip_model("gel_fit")
+    r = np.random.default_rng(3)
+    sets = [([np.logspace(-3, 0, int(r.integers(30, 200)))], PARS["gel_fit"](r)) for _ in range(10)]
+    batch = HipBatch(m, [qv for qv, _ in sets])
+    calls = []
+    for qv, pars in sets:
+        k = m.make_kernel(qv)
+        calls.append(make_kernel_args(k, direct_model.get_mesh(m.info, dict(pars), dim="1d"))[:2])
+        k.release()
+    iq = batch(calls)
+    for (qv, _), (det, val), got in zip(sets, calls, iq):
+        assert masked_relerr(got, OracleKernel(m.info, qv)(det, val, 0.0, False)) <= 1e-9
+    batch.release()
+    # no sld parameter: the reference has no magnetic kernel for these models
+    hk, _, det, val, _ = buffers("lorentz", [np.array([0.01]), np.array([0.02])], {})
+    with pytest.raises(HipError):
+        hk._call_kernel(det, val, 0.0, True, 0)
+    hk.release()

@@ -398,3 +398,70 @@ def test_new_shapes_reduce_to_known_ones():
         l1, l2 = _fq_o("elliptical_cylinder", q, [r, 1.0, length, sld, solvent, 0, 0, 0], 76)
         y1, y2 = _fq_o("cylinder", q, [sld, solvent, r, length, 0, 0], 76)
         assert math.isclose(l2, y2, rel_tol=1e-10) and math.isclose(l1, y1, rel_tol=1e-10)
+
+
+# ---------------------------------------------------------------- models defined by Iq alone
+IQ_ONLY = {
+    "guinier": lambda r: [r.uniform(-300, 300)],
+    "gaussian_peak": lambda r: [r.uniform(-0.1, 0.3), 10**r.uniform(-4, -1)],
+    "lorentz": lambda r: [10**r.uniform(0, 3)],
+    "dab": lambda r: [10**r.uniform(0, 3)],
+    "gel_fit": lambda r: [r.uniform(-5, 5), r.uniform(-5, 5), r.uniform(2, 500), r.uniform(0, 10), r.uniform(1, 100)],
+    "mass_fractal": lambda r: [r.uniform(0, 50), r.choice([1.0, r.uniform(1, 6)]), r.uniform(0, 500)],
+    "surface_fractal": lambda r: [r.uniform(0, 50), r.uniform(1, 3), r.uniform(1, 1000)],
+    "mass_surface_fractal": lambda r: [r.choice([0.0, r.uniform(0, 6)]), r.uniform(0, 6), r.uniform(1, 5000),
+                                       r.uniform(1, 200)],
+    "star_polymer": lambda r: [r.uniform(0, 1e4), r.uniform(1, 6)],
+    "mono_gauss_coil": lambda r: [r.uniform(0, 100), r.uniform(0, 300)],
+    "fractal": lambda r: [r.uniform(0, 1), r.uniform(0.5, 50), r.choice([0.0, 1.0, r.uniform(0, 6)]),
+                          r.uniform(1, 500), r.uniform(-2, 8), r.uniform(-2, 8)],
+}
+
+
+@pytest.mark.parametrize("model", list(IQ_ONLY))
+def test_iq_only_models_match_reference_sources(model):
+    ref = _need_ref("ref_" + model)
+    L = oracle.lib()
+    mid = oracle.MODEL_IDS[model]
+    rng = np.random.default_rng(51 + mid)
+    for trial in range(500):
+        p = np.ascontiguousarray(IQ_ONLY[model](rng), "d")
+        q = 0.0 if trial % 50 == 0 else 10**rng.uniform(-4, 0.5)
+        f1, f2 = _fq_o(model, q, p)
+        want = ref.ref_iq(q, _ptr(p))
+        assert f1 == 0.0 and (f2 == want or (math.isnan(f2) and math.isnan(want))), (trial, q, p, f2, want)
+        assert L.sasoracle_form_volume(mid, _ptr(p)) == ref.ref_form_volume(_ptr(p))
+        if hasattr(ref, "ref_reff"):
+            for mode in (1, 2, 3, 4):
+                assert L.sasoracle_radius_effective(mid, mode, _ptr(p)) == ref.ref_reff(mode, _ptr(p))
+
+
+def test_iq_only_models_against_closed_forms():
+    """Formulas that share nothing with the restatement: numpy closed forms,
+    the Debye function for the coil and for a one-arm star, scipy's gamma in
+    the fractal structure factors."""
+    from scipy.special import gamma
+    rng = np.random.default_rng(61)
+    debye = lambda x: 2*(np.expm1(-x) + x)/x**2
+    for _ in range(300):
+        q = 10**rng.uniform(-3, 0)
+        rg = rng.uniform(-200, 200)
+        with np.errstate(over="ignore"):          # negative rg grows without bound, to inf as in C
+            want = float(np.exp(-abs(rg)*rg*q*q/3))
+        assert math.isclose(_fq_o("guinier", q, [rg])[1], want, rel_tol=1e-14)
+        xi = 10**rng.uniform(0, 3)
+        assert math.isclose(_fq_o("lorentz", q, [xi])[1], 1/(1 + (q*xi)**2), rel_tol=1e-14)
+        assert math.isclose(_fq_o("dab", q, [xi])[1], xi**3/(1 + (q*xi)**2)**2, rel_tol=1e-14)
+        r0 = rng.uniform(1, 300)
+        x = (q*r0)**2
+        if x > 1e-2:      # away from the cancellation of the closed form
+            assert math.isclose(_fq_o("mono_gauss_coil", q, [3.0, r0])[1], 3.0*debye(x), rel_tol=1e-11)
+            # one arm: the star is a Gaussian coil of the same Rg
+            assert math.isclose(_fq_o("star_polymer", q, [r0*r0, 1.0])[1], debye(x), rel_tol=1e-11)
+        # fractal: (1 + D Gamma(D-1) sin((D-1) atan(q xi)) / ((q r)^D (1 + 1/(q xi)^2)^((D-1)/2))) P(q)
+        vf, r, D, xi = rng.uniform(0, 1), rng.uniform(1, 20), rng.uniform(1.2, 3), rng.uniform(50, 500)
+        sb, ss = rng.uniform(-2, 8, 2)
+        S = 1 + D*gamma(D - 1)*math.sin((D - 1)*math.atan(q*xi))/((q*r)**D*(1 + 1/(q*xi)**2)**((D - 1)/2))
+        x = q*r
+        P = (4/3*math.pi*r**3*(sb - ss)*3*(math.sin(x) - x*math.cos(x))/x**3)**2
+        assert math.isclose(_fq_o("fractal", q, [vf, r, D, xi, sb, ss])[1], 1e-4*vf*S*P, rel_tol=1e-9)
