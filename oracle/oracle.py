@@ -23,7 +23,7 @@ REF_DIR = os.path.join(HERE, "_ref")
 
 KIND_IQ, KIND_IQXY, KIND_IMAGNETIC = 0, 1, 2
 #: models whose 1D kernel returns <F^2> only (one output per q)
-NO_FQ = (8, 9, 10) + tuple(range(22, 33))
+NO_FQ = (8, 9, 10) + tuple(range(22, 41))
 MODEL_IDS = {"sphere": 0, "cylinder": 1, "ellipsoid": 2,
              "core_shell_cylinder": 3, "core_shell_sphere": 4,
              "parallelepiped": 6, "triaxial_ellipsoid": 7,
@@ -32,7 +32,10 @@ MODEL_IDS = {"sphere": 0, "cylinder": 1, "ellipsoid": 2,
              "core_multi_shell": 17, "onion": 18, "core_shell_ellipsoid": 19,
              "elliptical_cylinder": 20, "core_shell_bicelle": 21, "guinier": 22, "gaussian_peak": 23,
              "lorentz": 24, "dab": 25, "gel_fit": 26, "mass_fractal": 27, "surface_fractal": 28,
-             "mass_surface_fractal": 29, "star_polymer": 30, "mono_gauss_coil": 31, "fractal": 32}
+             "mass_surface_fractal": 29, "star_polymer": 30, "mono_gauss_coil": 31, "fractal": 32,
+             "fractal_core_shell": 33, "lamellar_hg": 34, "lamellar_stack_caille": 35,
+             "lamellar_hg_stack_caille": 36, "lamellar_stack_paracrystal": 37, "linear_pearls": 38,
+             "polymer_micelle": 39, "raspberry": 40}
 
 _lib = None
 _D = ct.c_double

@@ -21,7 +21,10 @@
  *                       core_shell_bicelle (22), and the Iq-only models
  *                       guinier, gaussian_peak, lorentz, dab, gel_fit,
  *                       mass_fractal, surface_fractal, mass_surface_fractal,
- *                       star_polymer, mono_gauss_coil, fractal (23-33)
+ *                       star_polymer, mono_gauss_coil, fractal (23-33),
+ *                       fractal_core_shell, lamellar_hg, lamellar_stack_caille,
+ *                       lamellar_hg_stack_caille, lamellar_stack_paracrystal,
+ *                       linear_pearls, polymer_micelle, raspberry (34-41)
  *   -DREF_FIXED76       use lib/gauss76.c + lib/nonadaptive.c instead of
  *                       lib/adaptive.c (the set_integration_size(info, 76) case)
  * kernel_iq.c is not built: it needs generator-emitted macros.
@@ -257,8 +260,48 @@ double ref_form_volume(const double *p) { return form_volume(p[1]); }
 #  if REF_MODEL != 32 && REF_MODEL != 33
 double ref_form_volume(const double *p) { (void)p; return 1.0; }
 #  endif
+#elif REF_MODEL >= 34 && REF_MODEL <= 41
+#  if REF_MODEL == 34
+#    include "lib/sas_3j1x_x.c"
+#    include "lib/sas_gamma.c"
+#    include "lib/core_shell.c"
+#    include "lib/fractal_sq.c"
+#    include "fractal_core_shell.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]); }
+double ref_form_volume(const double *p) { return form_volume(p[0], p[1]); }
+#  elif REF_MODEL == 35
+#    include "lamellar_hg.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4]); }
+#  elif REF_MODEL == 36
+#    include "lamellar_stack_caille.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4], p[5]); }
+#  elif REF_MODEL == 37
+#    include "lamellar_hg_stack_caille.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]); }
+#  elif REF_MODEL == 38
+#    include "lamellar_stack_paracrystal.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4], p[5]); }
+#  elif REF_MODEL == 39
+#    include "lib/sas_3j1x_x.c"
+#    include "linear_pearls.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4]); }
+#  elif REF_MODEL == 40
+#    include "lib/sas_3j1x_x.c"
+#    include "polymer_micelle.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9]); }
+#  else
+#    include "lib/sas_3j1x_x.c"
+#    include "raspberry.c"
+double ref_iq(double q, const double *p) { return Iq(q, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]); }
+double ref_reff(int mode, const double *p) { return radius_effective(mode, p[6], p[7], p[8]); }
+#  endif
+/* the form volume the generator uses: 1 for these (no volume parameter, or a
+ * model-level form_volume of 1), fractal_core_shell's own above */
+#  if REF_MODEL != 34
+double ref_form_volume(const double *p) { (void)p; return 1.0; }
+#  endif
 #else
-#  error "REF_MODEL must be 1..33"
+#  error "REF_MODEL must be 1..41"
 #endif
 
 #ifdef NPAR_CALL_FQ

@@ -35,7 +35,10 @@
  *     core_shell_bicelle.c; the Iq-only models guinier.c, gaussian_peak.c,
  *     lorentz.c, dab.c, gel_fit.c, mass_fractal.c, surface_fractal.c,
  *     mass_surface_fractal.c, star_polymer.c, mono_gauss_coil.c, fractal.c
- *     with lib/fractal_sq.c and lib/sas_gamma.c:140-153;
+ *     with lib/fractal_sq.c and lib/sas_gamma.c:140-153, fractal_core_shell.c,
+ *     lamellar_hg.c, lamellar_stack_caille.c, lamellar_hg_stack_caille.c,
+ *     lamellar_stack_paracrystal.c, linear_pearls.c, polymer_micelle.c,
+ *     raspberry.c;
  *     view/jitter rotation with psi qabc_rotation/qabc_apply
  *     kernel_iq.c:251-310
  *   - hard-sphere structure factor hardsphere.c:1-90
@@ -1115,6 +1118,180 @@ static void frc_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
     *f2 = 1.e-4 * p[0] * s * fq * fq;
 }
 
+/* ---- fractal_core_shell.c with lib/core_shell.c: p = radius, thickness,
+ * sld_core, sld_shell, sld_solvent, volfraction, fractal_dim, cor_length ---- */
+static oreal o_fractal_sq(oreal q, oreal radius, oreal D, oreal xi)
+{
+    oreal term;
+    if (q == 0.) term = pow(xi / radius, D) * o_gamma(D + 1.);
+    else if (D == 0.) term = 1.0;
+    else if (D == 1.) term = atan(q * xi) / (q * radius);
+    else {
+        const oreal dm = D - 1.0;
+        const oreal t1 = o_gamma(D + 1.) / dm * sin(dm * atan(q * xi));
+        const oreal t2 = pow(q * radius, -D);
+        const oreal t3 = pow(1.0 + 1.0 / sq(q * xi), -0.5 * dm);
+        term = t1 * t2 * t3;
+    }
+    return 1.0 + term;
+}
+static oreal fcs_vol(const oreal *p) { return O_4PI_3 * cb(p[0] + p[1]); }
+static void fcs_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal s = o_fractal_sq(q, p[0] + p[1], p[6], p[7]);
+    oreal f = O_4PI_3 * cb(p[0]) * o_sph_j1c(q * p[0]) * (p[2] - p[3]);
+    f += O_4PI_3 * cb(p[0] + p[1]) * o_sph_j1c(q * (p[0] + p[1])) * (p[3] - p[4]);
+    *f1 = 0.0;
+    *f2 = 1.0e-4 * p[5] * s * f * f;
+}
+/* ---- lamellar_hg.c: p = length_tail, length_head, sld, sld_head, sld_solvent ---- */
+static void lhg_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal qsq = q * q, drh = p[3] - p[4], drt = p[2] - p[4], qT = q * p[0];
+    oreal Pq = drh * (sin(q * (p[1] + p[0])) - sin(qT)) + drt * sin(qT);
+    Pq *= Pq;
+    Pq *= 4.0 / (qsq);
+    oreal inten = 2.0e-4 * O_PI * Pq / qsq;
+    inten /= 2.0 * (p[1] + p[0]);
+    *f1 = 0.0;
+    *f2 = inten;
+}
+/* ---- Caille S(q) of lamellar_stack_caille.c / lamellar_hg_stack_caille.c ---- */
+static oreal o_caille_sq(oreal q, int n, oreal dd, oreal Cp)
+{
+    const oreal euler = 0.577215664901533;
+    oreal s = 0.0;
+    for (int ii = 1; ii < n; ii++) {
+        const oreal alpha = Cp / 4.0 / O_PI / O_PI * (log(O_PI * ii) + euler);
+        const oreal t2 = 2.0 * q * q * dd * dd * alpha;
+        oreal t = 1.0 - (oreal)ii / (oreal)n;
+        t *= cos(dd * q * ii);
+        t *= exp(-t2 / 2.0);
+        s += t;
+    }
+    s *= 2.0;
+    s += 1.0;
+    return s;
+}
+/* lamellar_stack_caille.c: p = thickness, Nlayers, d_spacing, Caille_parameter, sld, sld_solvent */
+static void lsc_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal contr = p[4] - p[5];
+    const oreal Pq = 2.0 * contr * contr / q / q * (1.0 - cos(q * p[0]));
+    const oreal Sq = o_caille_sq(q, (int)(p[1] + 0.5), p[2], p[3]);
+    oreal inten = 2.0 * O_PI * Pq * Sq / (p[2] * q * q);
+    inten *= 1.0e-04;
+    *f1 = 0.0;
+    *f2 = inten;
+}
+/* lamellar_hg_stack_caille.c: p = length_tail, length_head, Nlayers,
+ * d_spacing, Caille_parameter, sld, sld_head, sld_solvent */
+static void lhc_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    oreal Pq = (p[6] - p[7]) * (sin(q * (p[1] + p[0])) - sin(q * p[0])) + (p[5] - p[7]) * sin(q * p[0]);
+    Pq *= Pq;
+    Pq *= 4.0 / (q * q);
+    const oreal Sq = o_caille_sq(q, (int)(p[2] + 0.5), p[3], p[4]);
+    oreal inten = 2.0 * O_PI * Pq * Sq / (p[3] * q * q);
+    inten *= 1.0e-04;
+    *f1 = 0.0;
+    *f2 = inten;
+}
+/* ---- lamellar_stack_paracrystal.c: p = thickness, Nlayers, d_spacing, sigma_d, sld, sld_solvent ---- */
+static oreal lsp_an(oreal ww, oreal q, oreal davg, int n)
+{
+    oreal an = 4.0 * ww * ww - 2.0 * (ww * ww * ww + ww) * cos(q * davg);
+    an -= 4.0 * pow(ww, (oreal)(n + 2)) * cos((oreal)n * q * davg);
+    an += 2.0 * pow(ww, (oreal)(n + 3)) * cos((oreal)(n - 1) * q * davg);
+    an += 2.0 * pow(ww, (oreal)(n + 1)) * cos((oreal)(n + 1) * q * davg);
+    return an;
+}
+static oreal lsp_sn(oreal ww, oreal q, oreal davg, int n, oreal an)
+{
+    return an / ((oreal)n * sq(1.0 + ww * ww - 2.0 * ww * cos(q * davg)));
+}
+static void lsp_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const int n1 = (int)(p[1]), n2 = n1 + 1;
+    const oreal xn = (oreal)n2 - p[1];
+    const oreal ww = exp(-0.5 * sq(q * p[3] * p[2]));
+    oreal Z = xn * lsp_sn(ww, q, p[2], n1, lsp_an(ww, q, p[2], n1));
+    Z += (1.0 - xn) * lsp_sn(ww, q, p[2], n2, lsp_an(ww, q, p[2], n2));
+    Z += (1.0 - ww * ww) / (1.0 + ww * ww - 2.0 * ww * cos(q * p[2]));
+    const oreal Pbil = sq(o_sinx_x(q * (p[0] / 2.0)));
+    const oreal contr = p[4] - p[5];
+    const oreal inten = 2.0 * O_PI * contr * contr * Pbil * Z / (q * q);
+    *f1 = 0.0;
+    *f2 = 1.0e-4 * inten;
+}
+/* ---- linear_pearls.c: p = radius, edge_sep, num_pearls, sld, sld_solvent ---- */
+static void lpe_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const int n = (int)(p[2] + 0.5);
+    const oreal vol = O_4PI_3 * cb(p[0]);
+    const oreal tot = n * vol;
+    const oreal m = (p[3] - p[4]) * vol;
+    const oreal sep = p[1] + 2.0 * p[0];
+    const oreal psi = o_sph_j1c(q * p[0]);
+    oreal s = (oreal)n;
+    for (int k = 1; k < n; k++) s += 2.0 * (n - k) * o_sinx_x(q * sep * k);
+    *f1 = 0.0;
+    *f2 = 1.0e-4 * s * sq(m * psi) / tot;
+}
+/* ---- polymer_micelle.c: p = ndensity, v_core, v_corona, sld_solvent,
+ * sld_core, sld_corona, radius_core, rg, d_penetration, n_aggreg ---- */
+static void pmi_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal bc = p[1] * (p[4] - p[3]), bk = p[2] * (p[5] - p[3]), n = p[9];
+    const oreal fc = o_sph_j1c(q * p[6]);
+    const oreal t1 = sq(n * bc * fc);
+    const oreal x = sq(q * p[7]);
+    const oreal debye = (x == 0.0) ? 1.0 : 2.0 * (expm1(-x) + x) / (x * x);
+    const oreal t2 = n * bk * bk * debye;
+    const oreal amp = (x == 0.0) ? 1.0 : -expm1(-x) / x;
+    const oreal sk = o_sinx_x(q * (p[6] + p[8] * p[7]));
+    const oreal t3 = 2.0 * n * n * bc * bk * fc * amp * sk;
+    const oreal t4 = n * (n - 1.0) * sq(bk * amp * sk);
+    oreal i = t1 + t2 + t3 + t4;
+    i *= 1.0e-13;
+    i *= p[0];
+    *f1 = 0.0;
+    *f2 = i;
+}
+/* ---- raspberry.c: p = sld_lg, sld_sm, sld_solvent, volfraction_lg,
+ * volfraction_sm, surface_fraction, radius_lg, radius_sm, penetration ---- */
+static oreal ras_reff(int mode, const oreal *p) { return mode == 2 ? p[6] + 2.0 * p[7] - p[8] : p[6]; }
+static void ras_fq(oreal q, const oreal *p, int quad_n, oreal *f1, oreal *f2)
+{
+    (void)quad_n;
+    const oreal vfL = p[3], rL = p[6], vfS = p[4], fSs = p[5], rS = p[7], dS = p[8];
+    const oreal drL = fabs(p[0] - p[2]), drS = fabs(p[1] - p[2]);
+    const oreal VL = O_4PI_3 * rL * rL * rL, VS = O_4PI_3 * rS * rS * rS;
+    const oreal Np = vfS * fSs * VL / vfL / VS;
+    const oreal slT = drL * VL + Np * drS * VS;
+    const oreal psiL = o_sph_j1c(q * rL), psiS = o_sph_j1c(q * rS);
+    const oreal sfLS = psiL * psiS * o_sinx_x(q * (rL + dS * rS));
+    const oreal sfSS = psiS * psiS * o_sinx_x(q * (rL + dS * rS)) * o_sinx_x(q * (rL + dS * rS));
+    oreal f = drL * drL * VL * VL * psiL * psiL;
+    f += Np * drS * drS * VS * VS * psiS * psiS;
+    f += Np * (Np - 1) * drS * drS * VS * VS * sfSS;
+    f += 2 * Np * drL * drS * VL * VS * sfLS;
+    if (f != 0.0) f = f / slT / slT;
+    f = f * (vfL * drL * drL * VL + vfS * fSs * Np * drS * drS * VS);
+    f += vfS * (1.0 - fSs) * drS * drS * VS * psiS * psiS;
+    f *= 1.0e8;
+    f *= 1.0e-12;
+    *f1 = 0.0;
+    *f2 = f;
+}
+
 /* ---- shells of vector parameters: n = (int)(fp_n + 0.5) of a control
  * parameter with limits [0, 10] (core_multi_shell.c:12-18, onion.c:31-38) ---- */
 #define O_NSHELL 10
@@ -1230,10 +1407,12 @@ static int always_valid(const oreal *p) { (void)p; return 1; }
  * 17 core_multi_shell, 18 onion, 19 core_shell_ellipsoid, 20
  * elliptical_cylinder, 21 core_shell_bicelle, 22-32 the Iq-only models guinier,
  * gaussian_peak, lorentz, dab, gel_fit, mass_fractal, surface_fractal,
- * mass_surface_fractal, star_polymer, mono_gauss_coil, fractal (the ids
- * libsaship uses; 5 and 14-16, the
+ * mass_surface_fractal, star_polymer, mono_gauss_coil, fractal, 33-40
+ * fractal_core_shell, lamellar_hg, lamellar_stack_caille,
+ * lamellar_hg_stack_caille, lamellar_stack_paracrystal, linear_pearls,
+ * polymer_micelle, raspberry (the ids libsaship uses; 5 and 14-16, the
  * structure factors, have entries of their own) */
-#define N_MODELS 33
+#define N_MODELS 41
 static const OModel MODELS[N_MODELS] = {
     {3, 2, 1, 0, {0, 1, -1}, always_valid, sph_vol, sph_reff, sph_fq, 0, 0, 0, 0},
     {6, 2, 4, 1, {0, 1, -1}, cyl_valid, cyl_vol, cyl_reff, cyl_fq, cyl_iqac, 0, 0, 0},
@@ -1270,6 +1449,14 @@ static const OModel MODELS[N_MODELS] = {
     {2, 0, 0, 0, {-1}, always_valid, one_vol, zero_reff, star_fq, 0, 0, 1, 0},
     {2, 0, 1, 0, {-1}, always_valid, one_vol, mgc_reff, mgc_fq, 0, 0, 1, 0},
     {6, 2, 1, 0, {4, 5}, always_valid, frc_vol, zero_reff, frc_fq, 0, 0, 1, 0},
+    {8, 3, 2, 0, {2, 3, 4}, always_valid, fcs_vol, zero_reff, fcs_fq, 0, 0, 1, 0},
+    {5, 3, 2, 0, {2, 3, 4}, always_valid, one_vol, zero_reff, lhg_fq, 0, 0, 1, 0},
+    {6, 2, 2, 0, {4, 5}, always_valid, one_vol, zero_reff, lsc_fq, 0, 0, 1, 0},
+    {8, 3, 3, 0, {5, 6, 7}, always_valid, one_vol, zero_reff, lhc_fq, 0, 0, 1, 0},
+    {6, 2, 1, 0, {4, 5}, always_valid, one_vol, zero_reff, lsp_fq, 0, 0, 1, 0},
+    {5, 2, 0, 0, {3, 4}, always_valid, one_vol, zero_reff, lpe_fq, 0, 0, 1, 0},
+    {10, 3, 0, 0, {3, 4, 5}, always_valid, one_vol, zero_reff, pmi_fq, 0, 0, 1, 0},
+    {9, 3, 3, 0, {0, 1, 2}, always_valid, one_vol, ras_reff, ras_fq, 0, 0, 1, 0},
 };
 static int model_ok(int id) { return id >= 0 && id < N_MODELS && MODELS[id].npars > 0; }
 

@@ -567,6 +567,14 @@ int record_width(int id, int kind)
     case 30: return rv_of<sas::StarPolymer>(kind);
     case 31: return rv_of<sas::MonoGaussCoil>(kind);
     case 32: return rv_of<sas::Fractal>(kind);
+    case 33: return rv_of<sas::FractalCoreShell>(kind);
+    case 34: return rv_of<sas::LamellarHg>(kind);
+    case 35: return rv_of<sas::LamellarStackCaille>(kind);
+    case 36: return rv_of<sas::LamellarHgStackCaille>(kind);
+    case 37: return rv_of<sas::LamellarStackParacrystal>(kind);
+    case 38: return rv_of<sas::LinearPearls>(kind);
+    case 39: return rv_of<sas::PolymerMicelle>(kind);
+    case 40: return rv_of<sas::Raspberry>(kind);
     default: return rv_of<sas::FuzzySphere>(kind);
     }
 }
@@ -843,6 +851,14 @@ int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, 
     case 30: return enqueue<sas::StarPolymer>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 31: return enqueue<sas::MonoGaussCoil>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     case 32: return enqueue<sas::Fractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 33: return enqueue<sas::FractalCoreShell>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 34: return enqueue<sas::LamellarHg>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 35: return enqueue<sas::LamellarStackCaille>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 36: return enqueue<sas::LamellarHgStackCaille>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 37: return enqueue<sas::LamellarStackParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 38: return enqueue<sas::LinearPearls>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 39: return enqueue<sas::PolymerMicelle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 40: return enqueue<sas::Raspberry>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     default: return enqueue<sas::FuzzySphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
     }
 }

@@ -85,25 +85,28 @@ inline int cu_count(int dev)
 // 19-21 are core_shell_ellipsoid, elliptical_cylinder, core_shell_bicelle;
 // 22-32 models defined by their intensity alone (no <F>, FP64): guinier,
 // gaussian_peak, lorentz, dab, gel_fit, mass_fractal, surface_fractal,
-// mass_surface_fractal, star_polymer, mono_gauss_coil, fractal.
+// mass_surface_fractal, star_polymer, mono_gauss_coil, fractal, and 33-40
+// fractal_core_shell, lamellar_hg, lamellar_stack_caille,
+// lamellar_hg_stack_caille, lamellar_stack_paracrystal, linear_pearls,
+// polymer_micelle, raspberry.
 constexpr int MODEL_HARDSPHERE = 5;
-constexpr int MODEL_LAST = 32;
+constexpr int MODEL_LAST = 40;
 inline int model_npars(int id)
 {
-    static const int np[33] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6, 24, 44, 9, 8, 10,
-                               1, 2, 1, 1, 5, 3, 3, 4, 2, 2, 6};
+    static const int np[41] = {3, 6, 6, 8, 5, 2, 8, 8, 8, 8, 8, 7, 5, 4, 4, 4, 6, 24, 44, 9, 8, 10,
+                               1, 2, 1, 1, 5, 3, 3, 4, 2, 2, 6, 8, 5, 6, 8, 6, 5, 10, 9};
     return np[id];
 }
 inline int model_maxpd(int id)
 {
-    static const int mp[33] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0, 5, 5, 5, 5, 5,
-                               0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1};
+    static const int mp[41] = {1, 4, 4, 5, 2, 0, 5, 5, 4, 4, 4, 5, 2, 2, 0, 0, 0, 5, 5, 5, 5, 5,
+                               0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 2, 2, 2, 3, 1, 0, 0, 3};
     return mp[id];
 }
 inline int model_nvalues(int id)
 {
-    static const int nv[33] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8, 66, 116, 24, 20, 28,
-                               3, 4, 3, 3, 7, 5, 5, 6, 4, 4, 18};
+    static const int nv[41] = {15, 18, 18, 23, 20, 4, 20, 20, 20, 20, 20, 19, 17, 16, 6, 6, 8, 66, 116, 24, 20, 28,
+                               3, 4, 3, 3, 7, 5, 5, 6, 4, 4, 18, 23, 20, 18, 23, 18, 17, 25, 24};
     return nv[id];
 }
 inline bool model_oriented(int id) { return id == 1 || id == 2 || id == 3 || (id >= 6 && id <= 11) || (id >= 19 && id <= 21); }
@@ -114,7 +117,7 @@ inline bool model_paracrystal(int id) { return id >= 8 && id <= 10; }
 inline bool model_nested(int id) { return id == 6 || id == 7 || id == 20; }
 inline bool model_structure(int id) { return id == MODEL_HARDSPHERE || id == 14 || id == 15 || id == 16; }
 // models defined by I(q) alone (FP64 direct evaluation)
-inline bool model_direct(int id) { return id >= 22 && id <= 32; }
+inline bool model_direct(int id) { return id >= 22 && id <= 40; }
 // the 1D kernel returns <F> beside <F^2> (two outputs per q)
 inline bool model_has_fq(int id) { return !model_structure(id) && !model_paracrystal(id) && !model_direct(id); }
 // models with sld parameters have a magnetic kernel

@@ -2080,6 +2080,189 @@ struct FractalD : DirectBase {
     }
 };
 
+// Teixeira's fractal structure factor (lib/fractal_sq.c), shared by fractal
+// and fractal_core_shell
+__device__ __forceinline__ double fractal_sq_d(double q, double r, double D, double xi)
+{
+    double term;
+    if (q == 0.0) {
+        term = pow(xi / r, D) * sas_gamma_d(D + 1.0);
+    } else if (D == 0.0) {
+        term = 1.0;
+    } else if (D == 1.0) {
+        term = atan(q * xi) / (q * r);
+    } else {
+        const double dm = D - 1.0, x = q * xi;
+        term = sas_gamma_d(D + 1.0) / dm * sin(dm * atan(x)) * pow(q * r, -D) * pow(1.0 + 1.0 / (x * x), -0.5 * dm);
+    }
+    return 1.0 + term;
+}
+// sin(x)/x in FP64 (1 at 0, as kernel_header.c:177)
+SAS_HD double sinc_dd(double x) { return x == 0.0 ? 1.0 : sin(x) / x; }
+
+// fractal_core_shell.c: core-shell spheres (lib/core_shell.c) in a fractal
+// aggregate  (p = radius, thickness, sld_core, sld_shell, sld_solvent,
+// volfraction, fractal_dim, cor_length)
+struct FractalCoreShellD : DirectBase {
+    static constexpr int ID = 33, NPARS = 8, NMAG = 3, MAX_PD = 2;
+    SAS_HD static int sld_param(int k) { return 2 + k; }
+    __device__ static double volume(const double *p) { return FOUR_PI_3_D * cube_d(p[0] + p[1]); }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double rc = p[0], rs = p[0] + p[1];
+        const double f = FOUR_PI_3_D * cube_d(rc) * sph_j1c_dd(q * rc) * (p[2] - p[3])
+                       + FOUR_PI_3_D * cube_d(rs) * sph_j1c_dd(q * rs) * (p[3] - p[4]);
+        return 1.0e-4 * p[5] * fractal_sq_d(q, rs, p[6], p[7]) * f * f;
+    }
+};
+// lamellar_hg.c: a bilayer of tails and heads, normalised by its thickness
+// (p = length_tail, length_head, sld, sld_head, sld_solvent)
+struct LamellarHgD : DirectBase {
+    static constexpr int ID = 34, NPARS = 5, NMAG = 3, MAX_PD = 2;
+    SAS_HD static int sld_param(int k) { return 2 + k; }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double lt = p[0], lh = p[1], q2 = q * q;
+        const double st = sin(q * lt);
+        const double a = (p[3] - p[4]) * (sin(q * (lh + lt)) - st) + (p[2] - p[4]) * st;
+        const double pq = a * a * (4.0 / q2);
+        return 2.0e-4 * PI_D * pq / q2 / (2.0 * (lh + lt));
+    }
+};
+// Caille structure factor of an N-layer stack (lamellar_stack_caille.c,
+// lamellar_hg_stack_caille.c): 1 + 2 sum_k (1 - k/N) cos(q d k)
+// exp(-q^2 d^2 alpha_k), alpha_k = eta/(4 pi^2) (ln(pi k) + Euler's constant)
+__device__ __forceinline__ double caille_sq(double q, int n, double d, double eta)
+{
+    const double euler = 0.577215664901533;
+    double s = 0.0;
+    for (int k = 1; k < n; k++) {
+        const double alpha = eta / 4.0 / PI_D / PI_D * (log(PI_D * k) + euler);
+        s += (1.0 - (double)k / (double)n) * cos(d * q * k) * exp(-(q * q * d * d * alpha));
+    }
+    return 2.0 * s + 1.0;
+}
+// lamellar_stack_caille.c  (p = thickness, Nlayers, d_spacing,
+// Caille_parameter, sld, sld_solvent)
+struct LamellarStackCailleD : DirectBase {
+    static constexpr int ID = 35, NPARS = 6, NMAG = 2, MAX_PD = 2;
+    SAS_HD static int sld_param(int k) { return 4 + k; }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double c = p[4] - p[5];
+        const double pq = 2.0 * c * c / q / q * (1.0 - cos(q * p[0]));
+        const double sq = caille_sq(q, (int)(p[1] + 0.5), p[2], p[3]);
+        return 2.0 * PI_D * pq * sq / (p[2] * q * q) * 1.0e-4;
+    }
+};
+// lamellar_hg_stack_caille.c  (p = length_tail, length_head, Nlayers,
+// d_spacing, Caille_parameter, sld, sld_head, sld_solvent)
+struct LamellarHgStackCailleD : DirectBase {
+    static constexpr int ID = 36, NPARS = 8, NMAG = 3, MAX_PD = 3;
+    SAS_HD static int sld_param(int k) { return 5 + k; }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double lt = p[0], lh = p[1];
+        const double st = sin(q * lt);
+        const double a = (p[6] - p[7]) * (sin(q * (lh + lt)) - st) + (p[5] - p[7]) * st;
+        const double pq = a * a * (4.0 / (q * q));
+        const double sq = caille_sq(q, (int)(p[2] + 0.5), p[3], p[4]);
+        return 2.0 * PI_D * pq * sq / (p[3] * q * q) * 1.0e-4;
+    }
+};
+// lamellar_stack_paracrystal.c (Pedersen): paracrystal S(q) mixed between
+// the two integer layer counts around Nlayers  (p = thickness, Nlayers,
+// d_spacing, sigma_d, sld, sld_solvent)
+struct LamellarStackParacrystalD : DirectBase {
+    static constexpr int ID = 37, NPARS = 6, NMAG = 2, MAX_PD = 1;
+    SAS_HD static int sld_param(int k) { return 4 + k; }
+    __device__ static double sn(double w, double q, double d, int n)
+    {
+        double an = 4.0 * w * w - 2.0 * (w * w * w + w) * cos(q * d);
+        an -= 4.0 * pow(w, (double)(n + 2)) * cos((double)n * q * d);
+        an += 2.0 * pow(w, (double)(n + 3)) * cos((double)(n - 1) * q * d);
+        an += 2.0 * pow(w, (double)(n + 1)) * cos((double)(n + 1) * q * d);
+        const double den = 1.0 + w * w - 2.0 * w * cos(q * d);
+        return an / ((double)n * den * den);
+    }
+    __device__ static double iq(double q, const double *p)
+    {
+        const int n1 = (int)p[1], n2 = n1 + 1;
+        const double xn = (double)n2 - p[1];
+        const double d = p[2], x = q * p[3] * d;
+        const double w = exp(-0.5 * x * x);
+        double z = xn * sn(w, q, d, n1) + (1.0 - xn) * sn(w, q, d, n2);
+        z += (1.0 - w * w) / (1.0 + w * w - 2.0 * w * cos(q * d));
+        const double b = sinc_dd(q * p[0] / 2.0), c = p[4] - p[5];
+        return 1.0e-4 * (2.0 * PI_D * c * c * (b * b) * z / (q * q));
+    }
+};
+// linear_pearls.c: N spheres on a line, centre spacing edge_sep + 2r
+// (p = radius, edge_sep, num_pearls, sld, sld_solvent)
+struct LinearPearlsD : DirectBase {
+    static constexpr int ID = 38, NPARS = 5, NMAG = 2;
+    SAS_HD static int sld_param(int k) { return 3 + k; }
+    __device__ static double iq(double q, const double *p)
+    {
+        const int n = (int)(p[2] + 0.5);
+        const double v = FOUR_PI_3_D * cube_d(p[0]);
+        const double m = (p[3] - p[4]) * v, sep = p[1] + 2.0 * p[0];
+        double s = (double)n;
+        for (int k = 1; k < n; k++) s += 2.0 * (n - k) * sinc_dd(q * sep * k);
+        const double a = m * sph_j1c_dd(q * p[0]);
+        return 1.0e-4 * s * (a * a) / (n * v);
+    }
+};
+// polymer_micelle.c (Pedersen & Gerstenberg): core, Gaussian corona chains
+// and their cross terms  (p = ndensity, v_core, v_corona, sld_solvent,
+// sld_core, sld_corona, radius_core, rg, d_penetration, n_aggreg)
+struct PolymerMicelleD : DirectBase {
+    static constexpr int ID = 39, NPARS = 10, NMAG = 3;
+    SAS_HD static int sld_param(int k) { return 3 + k; }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double bc = p[1] * (p[4] - p[3]), bk = p[2] * (p[5] - p[3]), n = p[9];
+        const double fc = sph_j1c_dd(q * p[6]);
+        const double t1 = (n * bc * fc) * (n * bc * fc);
+        const double x = (q * p[7]) * (q * p[7]);
+        const double debye = x == 0.0 ? 1.0 : 2.0 * (expm1(-x) + x) / (x * x);
+        const double t2 = n * bk * bk * debye;
+        const double amp = x == 0.0 ? 1.0 : -expm1(-x) / x;
+        const double sk = sinc_dd(q * (p[6] + p[8] * p[7]));
+        const double t3 = 2.0 * n * n * bc * bk * fc * amp * sk;
+        const double u = bk * amp * sk;
+        const double t4 = n * (n - 1.0) * u * u;
+        return (t1 + t2 + t3 + t4) * 1.0e-13 * p[0];
+    }
+};
+// raspberry.c (Larson-Smith et al.): large spheres decorated by small ones,
+// the intensity normalised inside (form volume 1)  (p = sld_lg, sld_sm,
+// sld_solvent, volfraction_lg, volfraction_sm, surface_fraction, radius_lg,
+// radius_sm, penetration)
+struct RaspberryD : DirectBase {
+    static constexpr int ID = 40, NPARS = 9, NMAG = 3, MAX_PD = 3;
+    SAS_HD static int sld_param(int k) { return k; }
+    __device__ static double reff(int mode, const double *p)
+    {
+        return mode == 2 ? p[6] + 2.0 * p[7] - p[8] : p[6];
+    }
+    __device__ static double iq(double q, const double *p)
+    {
+        const double rl = p[6], rs = p[7], dl = fabs(p[0] - p[2]), ds = fabs(p[1] - p[2]);
+        const double vl = FOUR_PI_3_D * rl * rl * rl, vs = FOUR_PI_3_D * rs * rs * rs;
+        const double np = p[4] * p[5] * vl / p[3] / vs;
+        const double slt = dl * vl + np * ds * vs;
+        const double pl = sph_j1c_dd(q * rl), ps = sph_j1c_dd(q * rs);
+        const double sc = sinc_dd(q * (rl + p[8] * rs));
+        double f2 = dl * dl * vl * vl * pl * pl + np * ds * ds * vs * vs * ps * ps
+                  + np * (np - 1.0) * ds * ds * vs * vs * (ps * ps * sc * sc) + 2.0 * np * dl * ds * vl * vs * (pl * ps * sc);
+        if (f2 != 0.0) f2 = f2 / slt / slt;
+        f2 = f2 * (p[3] * dl * dl * vl + p[4] * p[5] * np * ds * ds * vs);
+        f2 += p[4] * (1.0 - p[5]) * ds * ds * vs * ps * ps;
+        return f2 * 1.0e8 * 1.0e-12;
+    }
+};
+
 using Guinier = Direct<GuinierD>;
 using GaussianPeak = Direct<GaussianPeakD>;
 using Lorentz = Direct<LorentzD>;
@@ -2091,5 +2274,13 @@ using MassSurfaceFractal = Direct<MassSurfaceFractalD>;
 using StarPolymer = Direct<StarPolymerD>;
 using MonoGaussCoil = Direct<MonoGaussCoilD>;
 using Fractal = Direct<FractalD>;
+using FractalCoreShell = Direct<FractalCoreShellD>;
+using LamellarHg = Direct<LamellarHgD>;
+using LamellarStackCaille = Direct<LamellarStackCailleD>;
+using LamellarHgStackCaille = Direct<LamellarHgStackCailleD>;
+using LamellarStackParacrystal = Direct<LamellarStackParacrystalD>;
+using LinearPearls = Direct<LinearPearlsD>;
+using PolymerMicelle = Direct<PolymerMicelleD>;
+using Raspberry = Direct<RaspberryD>;
 
 }  // namespace sas

@@ -170,6 +170,14 @@ int with_model(int id, F &&f)
     case 30: return f((sas::StarPolymer *)nullptr);
     case 31: return f((sas::MonoGaussCoil *)nullptr);
     case 32: return f((sas::Fractal *)nullptr);
+    case 33: return f((sas::FractalCoreShell *)nullptr);
+    case 34: return f((sas::LamellarHg *)nullptr);
+    case 35: return f((sas::LamellarStackCaille *)nullptr);
+    case 36: return f((sas::LamellarHgStackCaille *)nullptr);
+    case 37: return f((sas::LamellarStackParacrystal *)nullptr);
+    case 38: return f((sas::LinearPearls *)nullptr);
+    case 39: return f((sas::PolymerMicelle *)nullptr);
+    case 40: return f((sas::Raspberry *)nullptr);
     default: return fail(SASHIP_EINVAL, "model " + std::to_string(id) + " has no mesh kernel");
     }
 }

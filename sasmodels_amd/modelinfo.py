@@ -26,8 +26,10 @@ vesicle.py:87-96, fuzzy_sphere.py:113-117, core_multi_shell.py:91-103,
 onion.py:323-333, core_shell_ellipsoid.py:139-149,
 elliptical_cylinder.py:115-122, core_shell_bicelle.py:140-151, guinier.py, gaussian_peak.py,
 lorentz.py, dab.py, gel_fit.py, mass_fractal.py, surface_fractal.py,
-mass_surface_fractal.py, star_polymer.py, mono_gauss_coil.py, fractal.py
-(the ``parameters`` tables), hardsphere.py:84-88,
+mass_surface_fractal.py, star_polymer.py, mono_gauss_coil.py, fractal.py,
+fractal_core_shell.py, lamellar_hg.py, lamellar_stack_caille.py,
+lamellar_hg_stack_caille.py, lamellar_stack_paracrystal.py, linear_pearls.py,
+polymer_micelle.py, raspberry.py (the ``parameters`` tables), hardsphere.py:84-88,
 stickyhardsphere.py:110-121, squarewell.py:83-92, hayter_msa.py:110-117),
 with the attributes the
 kernels use (``have_Fq``, ``valid``, ``radius_effective_modes``, the
@@ -551,6 +553,80 @@ _MODEL_TABLE = {
         ["sld_block", "1e-6/Ang^2", 2.0, [-inf, inf], "sld", "scattering length density of particles"],
         ["sld_solvent", "1e-6/Ang^2", 6.4, [-inf, inf], "sld", "scattering length density of solvent"],
     ], dict(have_Fq=False)),
+    "fractal_core_shell": (33, [
+        ["radius", "Ang", 60.0, [0.0, inf], "volume", "Sphere core radius"],
+        ["thickness", "Ang", 10.0, [0.0, inf], "volume", "Sphere shell thickness"],
+        ["sld_core", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "Sphere core scattering length density"],
+        ["sld_shell", "1e-6/Ang^2", 2.0, [-inf, inf], "sld", "Sphere shell scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 3.0, [-inf, inf], "sld", "Solvent scattering length density"],
+        ["volfraction", "", 0.05, [0.0, inf], "", "Volume fraction of building block spheres"],
+        ["fractal_dim", "", 2.0, [0.0, 6.0], "", "Fractal dimension"],
+        ["cor_length", "Ang", 100.0, [0.0, inf], "", "Correlation length of fractal-like aggregates"],
+    ], dict(have_Fq=False)),
+    # the lamellar models have volume parameters but a form volume of 1
+    "lamellar_hg": (34, [
+        ["length_tail", "Ang", 15, [0, inf], "volume", "Tail thickness ( total = H+T+T+H)"],
+        ["length_head", "Ang", 10, [0, inf], "volume", "Head thickness"],
+        ["sld", "1e-6/Ang^2", 0.4, [-inf, inf], "sld", "Tail scattering length density"],
+        ["sld_head", "1e-6/Ang^2", 3.0, [-inf, inf], "sld", "Head scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 6, [-inf, inf], "sld", "Solvent scattering length density"],
+    ], dict(have_Fq=False)),
+    "lamellar_stack_caille": (35, [
+        ["thickness", "Ang", 30.0, [0, inf], "volume", "sheet thickness"],
+        ["Nlayers", "", 20, [1, inf], "", "Number of layers"],
+        ["d_spacing", "Ang", 400., [0.0, inf], "volume", "lamellar d-spacing of Caille S(Q)"],
+        ["Caille_parameter", "1/Ang^2", 0.1, [0.0, 0.8], "", "Caille parameter"],
+        ["sld", "1e-6/Ang^2", 6.3, [-inf, inf], "sld", "layer scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "Solvent scattering length density"],
+    ], dict(have_Fq=False, single=False)),
+    "lamellar_hg_stack_caille": (36, [
+        ["length_tail", "Ang", 10, [0, inf], "volume", "Tail thickness"],
+        ["length_head", "Ang", 2, [0, inf], "volume", "head thickness"],
+        ["Nlayers", "", 30, [1, inf], "", "Number of layers"],
+        ["d_spacing", "Ang", 40., [0.0, inf], "volume", "lamellar d-spacing of Caille S(Q)"],
+        ["Caille_parameter", "", 0.001, [0.0, 0.8], "", "Caille parameter"],
+        ["sld", "1e-6/Ang^2", 0.4, [-inf, inf], "sld", "Tail scattering length density"],
+        ["sld_head", "1e-6/Ang^2", 2.0, [-inf, inf], "sld", "Head scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 6, [-inf, inf], "sld", "Solvent scattering length density"],
+    ], dict(have_Fq=False, single=False)),
+    "lamellar_stack_paracrystal": (37, [
+        ["thickness", "Ang", 33.0, [0, inf], "volume", "sheet thickness"],
+        ["Nlayers", "", 20, [1, inf], "", "Number of layers"],
+        ["d_spacing", "Ang", 250., [0.0, inf], "", "lamellar spacing of paracrystal stack"],
+        ["sigma_d", "Ang", 0.0, [0.0, inf], "", "Sigma (polydispersity) of the lamellar spacing"],
+        ["sld", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "layer scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 6.34, [-inf, inf], "sld", "Solvent scattering length density"],
+    ], dict(have_Fq=False, single=False)),
+    "linear_pearls": (38, [
+        ["radius", "Ang", 80.0, [0, inf], "", "Radius of the pearls"],
+        ["edge_sep", "Ang", 350.0, [0, inf], "", "Length of the string segment - surface to surface"],
+        ["num_pearls", "", 3.0, [1, inf], "", "Number of the pearls"],
+        ["sld", "1e-6/Ang^2", 1.0, [-inf, inf], "sld", "SLD of the pearl spheres"],
+        ["sld_solvent", "1e-6/Ang^2", 6.3, [-inf, inf], "sld", "SLD of the solvent"],
+    ], dict(have_Fq=False, single=False)),
+    "polymer_micelle": (39, [
+        ["ndensity", "1e15/cm^3", 8.94, [0.0, inf], "", "Number density of micelles"],
+        ["v_core", "Ang^3", 62624.0, [0.0, inf], "", "Core volume "],
+        ["v_corona", "Ang^3", 61940.0, [0.0, inf], "", "Corona volume"],
+        ["sld_solvent", "1e-6/Ang^2", 6.4, [0.0, inf], "sld", "Solvent scattering length density"],
+        ["sld_core", "1e-6/Ang^2", 0.34, [0.0, inf], "sld", "Core scattering length density"],
+        ["sld_corona", "1e-6/Ang^2", 0.8, [0.0, inf], "sld", "Corona scattering length density"],
+        ["radius_core", "Ang", 45.0, [0.0, inf], "", "Radius of core ( must be >> rg )"],
+        ["rg", "Ang", 20.0, [0.0, inf], "", "Radius of gyration of chains in corona"],
+        ["d_penetration", "", 1.0, [-inf, inf], "", "Factor to mimic non-penetration of Gaussian chains"],
+        ["n_aggreg", "", 6.0, [-inf, inf], "", "Aggregation number of the micelle"],
+    ], dict(have_Fq=False, single=False)),
+    "raspberry": (40, [
+        ["sld_lg", "1e-6/Ang^2", -0.4, [-inf, inf], "sld", "large particle scattering length density"],
+        ["sld_sm", "1e-6/Ang^2", 3.5, [-inf, inf], "sld", "small particle scattering length density"],
+        ["sld_solvent", "1e-6/Ang^2", 6.36, [-inf, inf], "sld", "solvent scattering length density"],
+        ["volfraction_lg", "", 0.05, [-inf, inf], "", "volume fraction of large spheres"],
+        ["volfraction_sm", "", 0.005, [-inf, inf], "", "volume fraction of small spheres"],
+        ["surface_fraction", "", 0.4, [-inf, inf], "", "fraction of small spheres at surface"],
+        ["radius_lg", "Ang", 5000, [0, inf], "volume", "radius of large spheres"],
+        ["radius_sm", "Ang", 100, [0, inf], "volume", "radius of small spheres"],
+        ["penetration", "Ang", 0, [-1, 1], "volume", "fractional penetration depth of small spheres into large sphere"],
+    ], dict(have_Fq=False, radius_effective_modes=["radius_large", "radius_outer"])),
     # structure factor: Iq only (no F), no volume, nothing polydisperse;
     # evaluated in FP64 (the reference marks it double precision only)
     "hardsphere": (5, [

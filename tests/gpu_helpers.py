@@ -35,12 +35,18 @@ def buffers(model, q_vectors, pars, quad=0):
 
 
 def masked_relerr(actual, target, mask=MASK):
-    """max |a - t|/|t| over points where |t| > mask * max |t|."""
+    """max |a - t|/|t| over points where |t| > mask * max |t| (max over the
+    finite targets); NaN must sit at the same points in both, else inf."""
     target = np.asarray(target)
     actual = np.asarray(actual)
     if target.size == 0:
         return 0.0
-    sel = np.abs(target) > mask*np.abs(target).max()
+    if (np.isnan(target) != np.isnan(actual)).any():
+        return float("inf")
+    finite = np.isfinite(target)
+    if not finite.any():
+        return 0.0
+    sel = finite & (np.abs(target) > mask*np.abs(target[finite]).max())
     if not sel.any():
         return 0.0
     return float(np.max(np.abs(actual[sel] - target[sel])/np.abs(target[sel])))

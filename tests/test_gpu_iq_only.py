@@ -1,9 +1,9 @@
 """
 GPU parity of the models defined by I(q) alone (no <F>; FP64 evaluation on
 the device): 1D with polydispersity where the model has a volume parameter,
-2D at |q| with a pixel at q = 0, the magnetic kernel of the one model with
-slds (fractal), batched sets, and the magnetic call refused for models
-without slds.
+2D at |q| with a pixel at q = 0 (NaN there for the lamellar models, in both),
+magnetic kernels (one model whose intensity is not linear in its slds),
+batched sets, and the magnetic call refused for models without slds.
 """
 
 import zlib
@@ -38,6 +38,29 @@ PARS = {
     "fractal": lambda r: dict(volfraction=r.uniform(0.01, 0.5), radius=r.uniform(2, 20), fractal_dim=r.uniform(1.2, 3),
                               cor_length=r.uniform(20, 500), sld_block=r.uniform(-1, 7), radius_pd=0.1,
                               radius_pd_n=int(r.integers(1, 12))),
+    "fractal_core_shell": lambda r: dict(radius=r.uniform(5, 60), thickness=r.uniform(1, 20), sld_core=r.uniform(-1, 7),
+                                         sld_shell=r.uniform(-1, 7), volfraction=r.uniform(0.01, 0.5),
+                                         fractal_dim=r.uniform(1.2, 3), cor_length=r.uniform(50, 500),
+                                         radius_pd=0.1, radius_pd_n=5, thickness_pd=0.1, thickness_pd_n=3),
+    "lamellar_hg": lambda r: dict(length_tail=r.uniform(5, 30), length_head=r.uniform(2, 15), sld=r.uniform(-1, 7),
+                                  sld_head=r.uniform(-1, 7), length_tail_pd=0.1, length_tail_pd_n=5),
+    "lamellar_stack_caille": lambda r: dict(thickness=r.uniform(10, 60), Nlayers=int(r.integers(1, 30)),
+                                            d_spacing=r.uniform(50, 500), Caille_parameter=r.uniform(0, 0.8),
+                                            thickness_pd=0.1, thickness_pd_n=5, d_spacing_pd=0.05, d_spacing_pd_n=3),
+    "lamellar_hg_stack_caille": lambda r: dict(length_tail=r.uniform(5, 20), length_head=r.uniform(1, 5),
+                                               Nlayers=int(r.integers(1, 40)), d_spacing=r.uniform(30, 100),
+                                               Caille_parameter=r.uniform(0, 0.8), sld_head=r.uniform(-1, 7),
+                                               d_spacing_pd=0.05, d_spacing_pd_n=3),
+    "lamellar_stack_paracrystal": lambda r: dict(thickness=r.uniform(10, 60), Nlayers=r.uniform(1, 30),
+                                                 d_spacing=r.uniform(50, 500), sigma_d=r.uniform(0, 0.3),
+                                                 thickness_pd=0.1, thickness_pd_n=7),
+    "linear_pearls": lambda r: dict(radius=r.uniform(5, 100), edge_sep=r.uniform(0, 400), num_pearls=r.uniform(1, 10),
+                                    sld=r.uniform(-1, 7)),
+    "polymer_micelle": lambda r: dict(radius_core=r.uniform(20, 80), rg=r.uniform(5, 30), n_aggreg=r.uniform(2, 40),
+                                      d_penetration=r.uniform(0, 2), sld_core=r.uniform(0, 7)),
+    "raspberry": lambda r: dict(radius_lg=r.uniform(500, 3000), radius_sm=r.uniform(20, 200),
+                                penetration=r.uniform(-1, 1), surface_fraction=r.uniform(0, 1), sld_lg=r.uniform(-1, 7),
+                                radius_lg_pd=0.05, radius_lg_pd_n=3, radius_sm_pd=0.1, radius_sm_pd_n=3),
 }
 
 
@@ -71,14 +94,25 @@ def test_iq_only_2d(model):
     hk.release()
 
 
-def test_iq_only_magnetic_fractal():
+MAGNETIC = {
+    "fractal": dict(volfraction=0.1, radius=8, fractal_dim=2.2, cor_length=150, sld_block_M0=3, sld_block_mtheta=40,
+                    sld_block_mphi=20, sld_solvent_M0=1, up_frac_i=0.2, up_frac_f=0.7, up_theta=30, up_phi=10,
+                    radius_pd=0.1, radius_pd_n=5),
+    # |sld - solvent| in the intensity: not linear in the slds
+    "raspberry": dict(radius_lg=800, radius_sm=60, sld_lg_M0=2, sld_lg_mtheta=30, sld_sm_M0=4, sld_sm_mphi=60,
+                      sld_solvent_M0=1, up_frac_i=0.4, up_frac_f=0.9, up_theta=20),
+    "lamellar_hg": dict(length_tail=15, length_head=8, sld_M0=1.5, sld_head_M0=3, sld_head_mtheta=70,
+                        up_frac_i=0.8, up_frac_f=0.1, up_theta=60, up_phi=20),
+}
+
+
+@pytest.mark.parametrize("model", list(MAGNETIC))
+def test_iq_only_magnetic(model):
     x = np.linspace(-0.3, 0.3, 20)
     qv = [a.ravel() for a in np.meshgrid(x, x)]
     qv[0][0] = qv[1][0] = 0.0
-    pars = dict(volfraction=0.1, radius=8, fractal_dim=2.2, cor_length=150, sld_block_M0=3, sld_block_mtheta=40,
-                sld_block_mphi=20, sld_solvent_M0=1, up_frac_i=0.2, up_frac_f=0.7, up_theta=30, up_phi=10,
-                radius_pd=0.1, radius_pd_n=5)
-    hk, ok, det, val, mag = buffers("fractal", qv, pars)
+    print("magnetic", model, flush=True)
+    hk, ok, det, val, mag = buffers(model, qv, MAGNETIC[model])
     assert mag
     hk._call_kernel(det, val, 0.0, mag, 0)
     compare_raw(hk.result, ok.raw(det, val, 0.0, mag, 0), qv[0].size, 1)

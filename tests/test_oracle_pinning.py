@@ -415,6 +415,20 @@ IQ_ONLY = {
     "mono_gauss_coil": lambda r: [r.uniform(0, 100), r.uniform(0, 300)],
     "fractal": lambda r: [r.uniform(0, 1), r.uniform(0.5, 50), r.choice([0.0, 1.0, r.uniform(0, 6)]),
                           r.uniform(1, 500), r.uniform(-2, 8), r.uniform(-2, 8)],
+    "fractal_core_shell": lambda r: [r.uniform(1, 80), r.uniform(0, 30), *r.uniform(-2, 8, 3), r.uniform(0, 1),
+                                     r.choice([0.0, 1.0, r.uniform(0, 6)]), r.uniform(1, 500)],
+    "lamellar_hg": lambda r: [r.uniform(1, 40), r.uniform(0, 20), *r.uniform(-2, 8, 3)],
+    "lamellar_stack_caille": lambda r: [r.uniform(1, 60), r.uniform(1, 40), r.uniform(20, 500), r.uniform(0, 0.8),
+                                        *r.uniform(-2, 8, 2)],
+    "lamellar_hg_stack_caille": lambda r: [r.uniform(1, 30), r.uniform(0, 10), r.uniform(1, 40), r.uniform(20, 200),
+                                           r.uniform(0, 0.8), *r.uniform(-2, 8, 3)],
+    "lamellar_stack_paracrystal": lambda r: [r.uniform(1, 60), r.uniform(1, 40), r.uniform(20, 500),
+                                             r.uniform(0, 50), *r.uniform(-2, 8, 2)],
+    "linear_pearls": lambda r: [r.uniform(1, 100), r.uniform(0, 400), r.uniform(1, 12), *r.uniform(-2, 8, 2)],
+    "polymer_micelle": lambda r: [r.uniform(0, 20), r.uniform(1e3, 1e5), r.uniform(1e3, 1e5), *r.uniform(0, 7, 3),
+                                  r.uniform(10, 100), r.uniform(1, 40), r.uniform(-1, 2), r.uniform(1, 50)],
+    "raspberry": lambda r: [*r.uniform(-2, 8, 3), r.uniform(0.01, 0.5), r.uniform(0.001, 0.1), r.uniform(0, 1),
+                            r.uniform(500, 5000), r.uniform(10, 200), r.uniform(-1, 1)],
 }
 
 
@@ -465,3 +479,26 @@ def test_iq_only_models_against_closed_forms():
         x = q*r
         P = (4/3*math.pi*r**3*(sb - ss)*3*(math.sin(x) - x*math.cos(x))/x**3)**2
         assert math.isclose(_fq_o("fractal", q, [vf, r, D, xi, sb, ss])[1], 1e-4*vf*S*P, rel_tol=1e-9)
+
+
+def test_linear_pearls_printed_values_and_closed_form():
+    """linear_pearls: two of the reference's three printed test values
+    disagree with its own source; the oracle follows the source (compiled in
+    oracle/_ref) at those inputs, and one pearl is the sphere's intensity
+    per volume."""
+    from kat_tables import LINEAR_PEARLS
+    ref = _need_ref("ref_linear_pearls")
+    agree = []
+    for pars, q, printed in LINEAR_PEARLS:
+        p = np.array([pars["radius"], pars["edge_sep"], pars["num_pearls"], pars["sld"], pars["sld_solvent"]])
+        got = _fq_o("linear_pearls", q, p)[1]
+        assert got == ref.ref_iq(q, _ptr(p))
+        agree.append(bool(is_near(printed, got + 1e-3, 5)))
+    assert agree == [False, True, False]      # only the second printed value is the source's
+    rng = np.random.default_rng(71)
+    for _ in range(100):
+        q, r = 10**rng.uniform(-3, 0), rng.uniform(1, 100)
+        sld, solvent = rng.uniform(-2, 8, 2)
+        one = _fq_o("linear_pearls", q, [r, rng.uniform(0, 100), 1.0, sld, solvent])[1]
+        sphere = _fq_o("sphere", q, [sld, solvent, r])[1]
+        assert math.isclose(one, sphere/(4/3*math.pi*r**3), rel_tol=1e-12)
