@@ -81,8 +81,12 @@ inline bool operator==(Counted a, Counted b) { return a.v == b.v; }
 inline bool operator!=(Counted a, Counted b) { return a.v != b.v; }
 inline bool operator!(Counted a) { return !a.v; }
 
+// The wrappers are always inlined so that the compiler treats sin and cos of
+// one argument as it does in the C build of the oracle (fused into one
+// sincos call, whose last bit can differ from separate calls): the counted
+// results then equal liboracle's bit for bit whatever else the file holds.
 #define FLOPCOUNT_TRANS(fn)                                                               \
-    inline Counted fn(Counted a)                                                          \
+    __attribute__((always_inline)) inline Counted fn(Counted a)                          \
     {                                                                                     \
         flopcount::tally().n[flopcount::TRANS]++;                                         \
         return Counted(std::fn(a.v));                                                     \
