@@ -83,6 +83,29 @@ extern "C" {
 #define SASHIP_CORE_SHELL_ELLIPSOID 19
 #define SASHIP_ELLIPTICAL_CYLINDER 20
 #define SASHIP_CORE_SHELL_BICELLE 21
+/* models defined by their intensity alone (no <F>: the 1D result has one
+ * output per q, nq + 4), evaluated in FP64; 2D uses I(|q|) and the magnetic
+ * kernel (models with sld parameters only) re-evaluates I with each spin
+ * cross section's effective slds.  Parameter tables: sasmodels/models/<name>.py */
+#define SASHIP_GUINIER 22
+#define SASHIP_GAUSSIAN_PEAK 23
+#define SASHIP_LORENTZ 24
+#define SASHIP_DAB 25
+#define SASHIP_GEL_FIT 26
+#define SASHIP_MASS_FRACTAL 27
+#define SASHIP_SURFACE_FRACTAL 28
+#define SASHIP_MASS_SURFACE_FRACTAL 29
+#define SASHIP_STAR_POLYMER 30
+#define SASHIP_MONO_GAUSS_COIL 31
+#define SASHIP_FRACTAL 32
+#define SASHIP_FRACTAL_CORE_SHELL 33
+#define SASHIP_LAMELLAR_HG 34
+#define SASHIP_LAMELLAR_STACK_CAILLE 35
+#define SASHIP_LAMELLAR_HG_STACK_CAILLE 36
+#define SASHIP_LAMELLAR_STACK_PARACRYSTAL 37
+#define SASHIP_LINEAR_PEARLS 38
+#define SASHIP_POLYMER_MICELLE 39
+#define SASHIP_RASPBERRY 40
 
 /* kernel kinds: the three kernels generated per model (generate.py:1144-1180) */
 #define SASHIP_KIND_IQ 0        /* <model>_Iq: 1D, <F> and <F^2> */
