@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -46,10 +47,12 @@ struct BatchArgs {
     const int64_t *roff;         // [nsets] record offsets
     const int64_t *poff;         // [nsets] partial offsets (doubles)
     const int *npts;             // [nsets] accepted points (from k_expand)
+    const float *spin;           // magnetic: per-set spin constants (from k_expand)
     const int4 *items;           // (set, tile, chunk, 0)
     const int *n_items;
     int pts_per_item;
     Quad quad;
+    int grab;                    // items per queue atomic
 };
 
 // The queue counter and the partial sums are separate __restrict__ kernel
@@ -61,6 +64,7 @@ struct BatchArgs {
 // row (counters only grow, so every item is handed out exactly once).
 struct QueueCursor {
     int c, dry;
+    int next = 0, end = 0;       // items taken by the last atomic and not yet handed out
 };
 
 __device__ __forceinline__ QueueCursor queue_start()
@@ -68,12 +72,15 @@ __device__ __forceinline__ QueueCursor queue_start()
     // wave-uniform (the compiler cannot tell from threadIdx): keeps the
     // counter, the item and every table address derived from it in scalars
     const int c = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % WORK_QUEUES);
-    return QueueCursor{__builtin_amdgcn_readfirstlane(c), 0};
+    return QueueCursor{__builtin_amdgcn_readfirstlane(c), 0, 0, 0};
 }
 
-// Next item of the wave, or -1 when the queue is empty.
-__device__ __forceinline__ int queue_next(int *queue, int n_items, QueueCursor &cur)
+// Next item of the wave, or -1 when the queue is empty.  One atomic takes
+// `grab` consecutive items of a counter's range: small items (a few mesh
+// points each) would otherwise spend their time on the counters' atomics.
+__device__ __forceinline__ int queue_next(int *queue, int n_items, QueueCursor &cur, int grab = 1)
 {
+    if (cur.next < cur.end) return cur.next++;
     for (;;) {
         const int first = (int)((int64_t)cur.c * n_items / WORK_QUEUES);
         const int last = (int)((int64_t)(cur.c + 1) * n_items / WORK_QUEUES);
@@ -82,11 +89,13 @@ __device__ __forceinline__ int queue_next(int *queue, int n_items, QueueCursor &
             int *ctr = queue + cur.c * WORK_QUEUE_STRIDE;
             // a stale read can only be low: then the atomic tells
             if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < last - first)
-                got = first + atomicAdd(ctr, 1);
+                got = first + atomicAdd(ctr, grab);
         }
         const int item = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
         if (item < last) {
             cur.dry = 0;
+            cur.next = item + 1;
+            cur.end = min(item + grab, last);
             return item;
         }
         if (++cur.dry == WORK_QUEUES) return -1;
@@ -103,7 +112,7 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
     const int n_items = *a.n_items;
     QueueCursor cur = queue_start();
     for (;;) {
-        const int item = queue_next(queue, n_items, cur);
+        const int item = queue_next(queue, n_items, cur, a.grab);
         if (item < 0) break;
         const int4 it = a.items[item];
         const int s = it.x, tile = it.y, chunk = it.z;
@@ -166,7 +175,7 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a, int *__restr
     const int n_items = *a.n_items;
     QueueCursor cur = queue_start();
     for (;;) {
-        const int item = queue_next(queue, n_items, cur);
+        const int item = queue_next(queue, n_items, cur, a.grab);
         if (item < 0) break;
         const int4 it = a.items[item];
         const int s = it.x, tile = it.y, chunk = it.z;
@@ -177,7 +186,7 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a, int *__restr
         Lane2D L;
         L.set(ok, ok ? a.q64[2 * (q0 + iq)] : 0.0, ok ? a.q64[2 * (q0 + iq) + 1] : 0.0);
         Contrast<M, MAG> con;
-        con.setup(a.values + a.voff[s], L.qx, L.qy);
+        con.setup(a.values + a.voff[s], MAG ? a.spin + (size_t)s * spin_floats<M>() : nullptr, L.qx, L.qy);
         const int npts = a.npts[s];
         const int c0 = min(npts, chunk * a.pts_per_item);
         const int c1 = min(npts, c0 + a.pts_per_item);
@@ -445,6 +454,7 @@ struct Plan {
     bool structure = false;           // P@S: per-set structure-factor parameters staged at off_spars
     int nblk = 0;                     // separable path: node blocks per (set, tile)
     int max_nq = 0;                   // largest q count of a set
+    double mean_pts = 0.0;            // generic path: mesh points per work item (upper bound)
 };
 
 }  // namespace batch_host
@@ -466,6 +476,7 @@ struct saship_batch {
     size_t recs_cap = 0;
     int *d_npts = nullptr;
     double *d_sums = nullptr;
+    float *d_spin = nullptr;          // per-set spin constants of the magnetic kernel
     double *d_sprep = nullptr;        // P@S: the q-independent part of S per set (S_PREP doubles)
     int *d_queue = nullptr;
     double *d_partial = nullptr;
@@ -493,6 +504,7 @@ struct saship_batch {
     // launch shape
     int pts_per_item = 16;
     int waves_per_cu = 32;
+    int grab = 0;                     // items per queue atomic (0: from the plan's mean item size)
     // graph replay
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
@@ -594,6 +606,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     std::vector<int64_t> roff(ns), poff(ns), resoff(ns);
     std::vector<int32_t> nchunk(ns);
     int64_t items = 0;
+    double pts_total = 0.0;
     const int P = b->pts_per_item;
     // separable cylinder path when every set's mesh allows it
     if (id == 1 && kind == 0 && b->m->quad_n > 0 && b->separable) {
@@ -625,7 +638,9 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         resoff[s] = p.result_total;
         p.result_total += (int64_t)p.nres * nq + 4;
         items += (int64_t)nchunk[s] * ((nq + sas::QTILE - 1) / sas::QTILE);
+        pts_total += (double)std::max<int64_t>(num_eval, 1) * ((nq + sas::QTILE - 1) / sas::QTILE);
     }
+    p.mean_pts = items > 0 ? pts_total / (double)items : 0.0;
     if (items > 0x7fffffff) return fail(SASHIP_EINVAL, "too many work items in one batch");
     p.n_items = (int)items;
     p.nq_total = b->qoff[ns];
@@ -712,6 +727,18 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
 }
 
 // Separable cylinder path: tables per set, then the node-block evaluation.
+// Items per queue atomic: the batch's setting, else enough items to reach
+// about 8 mesh points per atomic (at most 8 items) while every wave of the
+// grid still gets one grab of its own.
+int queue_grab(const saship_batch *b, const Plan &p, int nwaves)
+{
+    if (b->grab > 0) return b->grab;
+    if (p.mean_pts >= 8.0 || p.mean_pts <= 0.0) return 1;
+    const int want = std::min(8, (int)std::ceil(8.0 / p.mean_pts));
+    const int room = std::max(1, p.n_items / std::max(1, nwaves));
+    return std::max(1, std::min(want, room));
+}
+
 int enqueue_sep(saship_batch *b, const Plan &p, const sas::ExpandArgs &ea, const int64_t *d_poff)
 {
     using namespace sas;
@@ -752,7 +779,7 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     if (copies) HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
     if (b->timing) HIP_TRY(hipEventRecord(b->ev2, b->st));
     ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
-                  b->d_recs, b->d_npts, b->d_sums, b->d_queue};
+                  b->d_recs, b->d_npts, b->d_sums, b->d_queue, b->d_spin};
     if (p.sep_nlp) {
         if (int e = enqueue_sep(b, p, ea, d_poff)) return e;
     } else {
@@ -763,9 +790,9 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     }
     HIP_TRY(hipGetLastError());
 
-    BatchArgs ba{b->q32, b->q64, b->d_qoff, d_values, d_voff, b->d_recs, d_roff, d_poff, b->d_npts,
-                 d_items, d_nitems, b->pts_per_item, b->m->quad};
     const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
+    BatchArgs ba{b->q32, b->q64, b->d_qoff, d_values, d_voff, b->d_recs, d_roff, d_poff, b->d_npts, b->d_spin,
+                 d_items, d_nitems, b->pts_per_item, b->m->quad, queue_grab(b, p, grid * 4)};
     if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
     if (p.kind == 0) {
         if (b->m->quad_n > 0) k_batch_eval_1d<M, true><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
@@ -1003,6 +1030,8 @@ int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_qoff, sizeof(int64_t) * (nsets + 1));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_npts, sizeof(int) * nsets);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_sums, sizeof(double) * 4 * nsets);
+    if (e == hipSuccess && model_magnetic(m->id))
+        e = hipMalloc((void **)&b->d_spin, sizeof(float) * model_spin_floats(m->id) * nsets);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_sprep, sizeof(double) * sas::S_PREP * nsets);
     if (e == hipSuccess)
         e = hipMalloc((void **)&b->d_queue, sizeof(int) * sas::WORK_QUEUES * sas::WORK_QUEUE_STRIDE);
@@ -1274,6 +1303,26 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
     }
     b->pts_per_item = best_p;
     b->waves_per_cu = best_w;
+    // then the items per queue atomic at that shape (0: the plan's choice)
+    int best_g = b->grab;
+    const int grabs[] = {0, 1, 2, 4, 8, 16};
+    for (int G : grabs) {
+        if (rc) break;
+        b->grab = G;
+        double t = 1e300;
+        for (int rep = 0; rep < 3 && rc == 0; rep++) {
+            b->t_ms = 0.0;
+            b->t_all_ms = 0.0;
+            b->ncalls = 0;
+            rc = run(b, kind, details, n_details, values, values_offset, cutoff, 0, nullptr, nullptr);
+            if (rc == 0 && rep > 0) t = std::min(t, b->t_all_ms);
+        }
+        if (rc == 0 && t < best_t) {
+            best_t = t;
+            best_g = G;
+        }
+    }
+    b->grab = best_g;
     b->use_graph = graph;
     b->timing = timing;
     b->t_ms = t_keep;
@@ -1313,7 +1362,7 @@ int saship_batch_release(saship_batch *b)
     (void)hipSetDevice(b->dev);
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
-    void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums, b->d_sprep,
+    void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums, b->d_spin, b->d_sprep,
                         b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw,
                         b->d_srep, b->d_srstep, b->d_sroff, b->d_srw};
     for (void *ptr : dev_ptrs)

@@ -122,6 +122,14 @@ inline bool model_direct(int id) { return id >= 22 && id <= 40; }
 inline bool model_has_fq(int id) { return !model_structure(id) && !model_paracrystal(id) && !model_direct(id); }
 // models with sld parameters have a magnetic kernel
 inline bool model_magnetic(int id) { return model_nvalues(id) > 2 + model_npars(id); }
+// floats of one set's spin constants (sas::spin_floats): 24 + 8 per sld slot
+inline int model_spin_floats(int id)
+{
+    const int nmag = (model_nvalues(id) - 2 - model_npars(id) - 4) / 3;
+    return 24 + 8 * (nmag > 0 ? nmag : 1);
+}
+// bytes of a kernel handle's meta block: npts at 0, the four sums at 64, spin constants at 128
+constexpr int META_SPIN = 128, META_BYTES = 128 + 4 * (24 + 8 * 24);
 
 // Separable cylinder 1D meshes: only the radius (parameter 2) and length
 // (parameter 3) levels have more than one point, each at most SEP_MAX.

@@ -95,7 +95,17 @@ struct ExpandArgs {
     int *npts;                  // [nsets] accepted points
     double *sums;               // [4*nsets] sum w, sum w*V_form, sum w*V_shell, sum w*R_eff
     int *queue;                 // counters reset to 0 by workgroup 0 when not NULL
+    float *spin;                // magnetic kernel: per-set spin constants (spin_floats<M>() each)
 };
+
+// Magnetic set-up that depends only on the set (written once per set by
+// k_expand, read by every lane of the magnetic kernels), in floats:
+//   [0, 6)   cross-section weights (0 when inactive)   [6, 12)  active flags
+//   [12, 15) polarisation axis   [15, 18) spin-flip axis   [18, 21) the third axis
+//   then per sld slot k, at 24 + 8k: the nuclear sld (a double in two words)
+//   and the magnetisation vector (mx, my, mz).
+template <class M> constexpr int spin_floats() { return 24 + 8 * (M::NMAG > 0 ? M::NMAG : 1); }
+template <class M> __device__ void write_spin(const double *values, float *out);
 
 template <class M, int KIND>
 __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
@@ -108,6 +118,9 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_expand(ExpandArgs a)
     const double *values = a.values + (a.voff ? a.voff[set] : 0);
     float4 *recs = a.recs + (a.roff ? a.roff[set] : 0) * RV;
     if (a.queue && set == 0 && threadIdx.x < WORK_QUEUES) a.queue[threadIdx.x * WORK_QUEUE_STRIDE] = 0;
+    if constexpr (KIND == 2) {
+        if (a.spin && threadIdx.x == 0) write_spin<M>(values, a.spin + (size_t)set * spin_floats<M>());
+    }
 
     const int32_t *pd_par = det, *pd_len = det + MP, *pd_off = det + 2 * MP, *pd_stride = det + 3 * MP;
     const int num_eval = det[4 * MP];
@@ -381,29 +394,6 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_fq1d(
 }
 
 // ------------------------------------------------------------------ 2D, plain and magnetic
-// Effective SLDs of the six spin cross sections (kernel_iq.c:127-175) for
-// one magnetised SLD at one q: the magnetisation component perpendicular to
-// q, projected on the polarisation (non-spin-flip: sld -+ P.Mperp) or on
-// the two axes normal to it (spin-flip, real and imaginary).  ux, uy is the
-// unit vector of q and uu its squared norm (which the reference divides by);
-// both depend on q only, the projections on the SLD only, so a lane works
-// them out once instead of once per cross section.
-__device__ __forceinline__ void mag_slds(double ux, double uy, double uu, double cmt, double smt, double cmp,
-                                         double smp, double sld, double mx, double my, double mz, double *eff)
-{
-    const double k = (mx * ux + my * uy) / uu;
-    const double px = mx - k * ux, py = my - k * uy, pz = mz;
-    const double pol = smt * cmp * px + smt * smp * py + cmt * pz;
-    const double flip = -smp * px + cmp * py;
-    const double perp = -cmt * cmp * px - cmt * smp * py + smt * pz;
-    eff[0] = sld - pol;
-    eff[1] = flip;
-    eff[2] = flip;
-    eff[3] = sld + pol;
-    eff[4] = -perp;
-    eff[5] = perp;
-}
-
 // Spin cross-section weights dd, du.re, ud.re, uu, du.im, ud.im (kernel_iq.c:84-124)
 __device__ __forceinline__ void spin_weights(double in_spin, double out_spin, double *w)
 {
@@ -418,7 +408,36 @@ __device__ __forceinline__ void spin_weights(double in_spin, double out_spin, do
     w[5] = w[2];
 }
 
-// Per-lane contrast terms for each spin cross section, from the set's values.
+template <class M>
+__device__ void write_spin(const double *values, float *out)
+{
+    constexpr int NP = M::NPARS;
+    double w[6];
+    spin_weights(values[2 + NP], values[3 + NP], w);
+    double smt, cmt, smp, cmp;
+    sincos(values[4 + NP] * DEG, &smt, &cmt);
+    sincos(values[5 + NP] * DEG, &smp, &cmp);
+    for (int x = 0; x < 6; x++) {
+        const bool act = w[x] > 1.e-8;                    // kernel_iq.c:761
+        out[x] = act ? (float)w[x] : 0.0f;
+        out[6 + x] = act ? 1.0f : 0.0f;
+    }
+    const double axes[9] = {smt * cmp, smt * smp, cmt, -smp, cmp, 0.0, -cmt * cmp, -cmt * smp, smt};
+    for (int i = 0; i < 9; i++) out[12 + i] = (float)axes[i];
+    out[21] = out[22] = out[23] = 0.0f;
+    for (int k = 0; k < M::NMAG; k++) {
+        float *o = out + 24 + 8 * k;
+        put_d(o, 0, values[2 + M::sld_param(k)]);
+        const double *mv = values + 2 + NP + 4 + 3 * k;
+        o[2] = (float)mv[0];
+        o[3] = (float)mv[1];
+        o[4] = (float)mv[2];
+        o[5] = o[6] = o[7] = 0.0f;
+    }
+}
+
+// Per-lane contrast terms for each spin cross section, from the set's values
+// (nuclear) or its spin constants (magnetic).
 template <class M, bool MAG>
 struct Contrast {
     static constexpr int NX = MAG ? 6 : 1;
@@ -431,40 +450,51 @@ struct Contrast {
     // direct-intensity models: the effective slds of each cross section
     double effd[DIRECT && MAG ? NX : 1][DIRECT && MAG ? NM1 : 1];
 
-    __device__ __forceinline__ void setup(const double *values, double qx, double qy)
+    __device__ __forceinline__ void setup(const double *values, const float *spin, double qx, double qy)
     {
-        constexpr int NP = M::NPARS;
-        double sld[NM1];
-#pragma unroll
-        for (int k = 0; k < M::NMAG; k++) sld[k] = values[2 + M::sld_param(k)];
         if constexpr (!MAG) {
+            double sld[NM1];
+#pragma unroll
+            for (int k = 0; k < M::NMAG; k++) sld[k] = values[2 + M::sld_param(k)];
             M::contrasts(values + 2, sld, con[0]);
             xsw[0] = 1.0f;
             active[0] = true;
         } else {
-            double w[6];
-            spin_weights(values[2 + NP], values[3 + NP], w);
-            double smt, cmt, smp, cmp;
-            sincos(values[4 + NP] * DEG, &smt, &cmt);
-            sincos(values[5 + NP] * DEG, &smp, &cmp);
+            // Effective slds of the six spin cross sections (kernel_iq.c:127-175):
+            // the magnetisation component perpendicular to q, projected on the
+            // polarisation (non-spin-flip: sld -+ P.Mperp) or on the two axes
+            // normal to it (spin-flip, real and imaginary).  The axes, weights
+            // and magnetisations are the set's constants (scalar cache); per
+            // lane only the projection perpendicular to q, in FP32, with the
+            // reference's division by the unit vector's squared norm.
+            const SAS_CONST float *sc = as_const(spin);
             const bool q_ok = qx * qx + qy * qy > 1.e-16;   // kernel_iq.c:754
             on = q_ok;
-            const double qn = q_ok ? sqrt(qx * qx + qy * qy) : 1.0;
-            const double ux = qx / qn, uy = qy / qn;
-            const double uu = q_ok ? ux * ux + uy * uy : 1.0;
+            const float fx = (float)qx, fy = (float)qy;
+            const float rn = q_ok ? rsqrt_f(fmaf(fx, fx, fy * fy)) : 1.0f;
+            const float ux = fx * rn, uy = fy * rn;
+            const float uu = q_ok ? fmaf(ux, ux, uy * uy) : 1.0f;
+            const float ax[9] = {sc[12], sc[13], sc[14], sc[15], sc[16], sc[17], sc[18], sc[19], sc[20]};
             double eff[NX][NM1];
 #pragma unroll
             for (int k = 0; k < M::NMAG; k++) {
-                const double *mv = values + 2 + NP + 4 + 3 * k;
-                double e[6];
-                mag_slds(ux, uy, uu, cmt, smt, cmp, smp, sld[k], mv[0], mv[1], mv[2], e);
+                const SAS_CONST float *m = sc + 24 + 8 * k;
+                double sld;
+                __builtin_memcpy(&sld, (const float *)m, sizeof sld);
+                const float mx = m[2], my = m[3], mz = m[4];
+                const float kk = fmaf(mx, ux, my * uy) / uu;
+                const float px = fmaf(-kk, ux, mx), py = fmaf(-kk, uy, my), pz = mz;
+                const float pol = fmaf(ax[0], px, fmaf(ax[1], py, ax[2] * pz));
+                const float flip = fmaf(ax[3], px, ax[4] * py);
+                const float perp = fmaf(ax[6], px, fmaf(ax[7], py, ax[8] * pz));
+                const double e[6] = {sld - pol, flip, flip, sld + pol, -perp, perp};
 #pragma unroll
                 for (int x = 0; x < NX; x++) eff[x][k] = q_ok ? e[x] : 0.0;
             }
 #pragma unroll
             for (int x = 0; x < NX; x++) {
-                active[x] = w[x] > 1.e-8;                    // kernel_iq.c:761
-                xsw[x] = (active[x] && q_ok) ? (float)w[x] : 0.0f;
+                active[x] = sc[6 + x] != 0.0f;
+                xsw[x] = q_ok ? sc[x] : 0.0f;
                 if constexpr (DIRECT) {
 #pragma unroll
                     for (int k = 0; k < NM1; k++) effd[x][k] = eff[x][k];
@@ -600,7 +630,7 @@ constexpr int SUM_BLOCK = 16;
 template <class M, bool MAG>
 __global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
     int nq, const double *__restrict__ q64,
-    const double *__restrict__ values, const float4 *__restrict__ recs,
+    const double *__restrict__ values, const float *__restrict__ spin, const float4 *__restrict__ recs,
     const int *__restrict__ npts_p, int nsplit, double *__restrict__ partial)
 {
     constexpr int RV = rv2<M>();
@@ -612,7 +642,7 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_iq2d(
     Lane2D L;
     L.set(ok, ok ? q64[2 * iq] : 0.0, ok ? q64[2 * iq + 1] : 0.0);
     Contrast<M, MAG> con;
-    con.setup(values, L.qx, L.qy);
+    con.setup(values, spin, L.qx, L.qy);
     const SAS_CONST float4 *rc = as_const(recs);
     KSum acc;
     for (int b0 = s0; b0 < s1; b0 += SUM_BLOCK) {

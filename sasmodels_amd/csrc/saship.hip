@@ -129,7 +129,7 @@ struct Target {
     const float *q32;
     const double *q64;
     float4 *recs;        // compacted mesh records
-    char *meta;          // npts (int) at 0, sums (4 doubles) at 64
+    char *meta;          // npts (int) at 0, sums (4 doubles) at 64, spin constants at META_SPIN
     double *partial;     // per-split partial sums
     double *result;      // nout*nq + 4 doubles, reference layout
     char *sep;           // separable tables, sas::sep::tables_bytes(1, SEP_MAX)
@@ -212,8 +212,10 @@ int launch(const Target &t, int kind, const int32_t *d_det, const double *d_val,
     int *npts = (int *)t.meta;
     double *sums = (double *)(t.meta + 64);
 
+    static_assert(META_SPIN + 4 * sas::spin_floats<M>() <= META_BYTES, "spin constants exceed the meta block");
+    float *spin = (float *)(t.meta + META_SPIN);
     ExpandArgs ea{d_det, d_val, nullptr, nullptr, n_details, pd_start, pd_stop, cutoff, reff_mode,
-                  t.recs, npts, sums, nullptr};
+                  t.recs, npts, sums, nullptr, kind == 2 ? spin : nullptr};
     switch (kind) {
     case 0: k_expand<M, 0><<<1, EXPAND_THREADS, 0, t.st>>>(ea); break;
     case 1: k_expand<M, 1><<<1, EXPAND_THREADS, 0, t.st>>>(ea); break;
@@ -232,9 +234,10 @@ int launch(const Target &t, int kind, const int32_t *d_det, const double *d_val,
             k_fq1d<M, false><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q32, t.q64, t.recs, npts, nsplit, t.m->quad,
                                                               t.partial);
     } else if (kind == 1) {
-        k_iq2d<M, false><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q64, d_val, t.recs, npts, nsplit, t.partial);
+        k_iq2d<M, false><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q64, d_val, nullptr, t.recs, npts, nsplit,
+                                                          t.partial);
     } else {
-        k_iq2d<M, true><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q64, d_val, t.recs, npts, nsplit, t.partial);
+        k_iq2d<M, true><<<grid, EVAL_THREADS, 0, t.st>>>(nq, t.q64, d_val, spin, t.recs, npts, nsplit, t.partial);
     }
     HIP_TRY(hipGetLastError());
     if (t.ev1) HIP_TRY(hipEventRecord(t.ev1, t.st));
@@ -650,7 +653,7 @@ int saship_kernel_create(saship_model *m, int32_t nq, const double *q, int32_t i
     hipError_t e = hipStreamCreateWithFlags(&k->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&k->q32, sizeof(float) * std::max<size_t>(nqv, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&k->q64, sizeof(double) * std::max<size_t>(nqv, 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&k->d_meta, 256);
+    if (e == hipSuccess) e = hipMalloc((void **)&k->d_meta, META_BYTES);
     if (e == hipSuccess) e = hipHostMalloc((void **)&k->h_result, sizeof(double) * (2 * (size_t)nq + 4),
                                              hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&k->hd_result, k->h_result, 0);
@@ -795,7 +798,7 @@ int saship_kernel_call_noalloc(saship_model *m, int32_t kind, int32_t nq, const 
     const size_t o_det = take(sizeof(int32_t) * (size_t)n_details);
     const int64_t nqv = (int64_t)nq * (is2d ? 2 : 1);
     const size_t o_q32 = take(sizeof(float) * (size_t)std::max<int64_t>(1, nqv));
-    const size_t o_meta = take(256);
+    const size_t o_meta = take(META_BYTES);
     int nlp = 0;
     const int path = choose_path(m, kind, details, true, &nlp);
     const int npts_max = std::max(0, pd_stop - pd_start);
