@@ -253,6 +253,17 @@ int saship_batch_call(saship_batch *batch, int32_t kind,
                       double cutoff, int32_t radius_effective_mode,
                       double *result, double *iq);
 
+/* Pinned (page-locked, device-mapped, coherent) host memory of `bytes`
+ * bytes, and its release.  An `iq` array of saship_batch_call in such
+ * memory is written by the call's last kernel directly: no download, no
+ * staging buffer and no host copy (that call is launched eagerly, not from
+ * the batch's graph, since the array's address changes between calls).  No
+ * counterpart in the reference (its OpenCL backend maps its own buffers,
+ * sasmodels/kernelcl.py:514-523); for callers that reuse output arrays
+ * across calls. */
+int saship_host_alloc(int64_t bytes, void **ptr);
+int saship_host_free(void *ptr);
+
 /* Resolution smearing of the batch's I(q) (Pinhole1D / Slit1D weight
  * matrices, sasmodels/resolution.py:172-221; Pinhole2D bin averages,
  * resolution2d.py:91-175): per set s, n_out[s] smeared points; for the

@@ -38,6 +38,8 @@ SIGNATURES = {
     "saship_structure_eval": (ct.c_int, [ct.c_int32, ct.c_void_p, ct.c_int32, ct.c_void_p, ct.c_int32,
                                          ct.c_void_p, ct.c_void_p]),
     "saship_probe_fma": (ct.c_int, [ct.c_int32, ct.POINTER(ct.c_double)]),
+    "saship_host_alloc": (ct.c_int, [ct.c_int64, ct.POINTER(ct.c_void_p)]),
+    "saship_host_free": (ct.c_int, [ct.c_void_p]),
     "saship_batch_replay": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_double)]),
     "saship_batch_set_resolution": (ct.c_int, [ct.c_void_p]*7 + [ct.c_int64]),
     "saship_batch_set_resolution_repeat": (ct.c_int, [ct.c_void_p]*7 + [ct.c_int64] + [ct.c_void_p]*4 + [ct.c_int64]),

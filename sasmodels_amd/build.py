@@ -31,16 +31,35 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def _run(cmds):
+    """Run compiler commands in parallel; (returncode, log) of the first failure or (0, logs)."""
+    procs = [subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True) for c in cmds]
+    logs, rc = [], 0
+    for p in procs:
+        out, _ = p.communicate()
+        logs.append("\n".join(line for line in out.splitlines() if "packed-fp32-ops" not in line))
+        rc = rc or p.returncode
+    return rc, "\n".join(l for l in logs if l.strip())
+
+
 def build_library(verbose=False):
-    """Compile libsaship.so; raises CalledProcessError with the compiler log on failure."""
+    """Compile libsaship.so (one object per source, compiled in parallel, then
+    linked); raises CalledProcessError with the compiler log on failure."""
     tmp = OUTPUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + SOURCES
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    log = "\n".join(line for line in (proc.stdout + proc.stderr).splitlines()
-                    if "packed-fp32-ops" not in line)
-    if proc.returncode != 0:
+    objs = [os.path.join(HERE, "csrc", os.path.basename(src) + ".o") for src in SOURCES]
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    cmds = [[hipcc()] + compile_flags + ["-c", "-o", o, src] for src, o in zip(SOURCES, objs)]
+    rc, log = _run(cmds)
+    if rc == 0:
+        link = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+        rc, link_log = _run([link])
+        log = "\n".join(l for l in (log, link_log) if l.strip())
+    for o in objs:
+        if os.path.exists(o):
+            os.remove(o)
+    if rc != 0:
         sys.stderr.write(log + "\n")
-        raise subprocess.CalledProcessError(proc.returncode, cmd, output=log)
+        raise subprocess.CalledProcessError(rc, cmds[0], output=log)
     if verbose and log.strip():
         print(log)
     os.replace(tmp, OUTPUT)

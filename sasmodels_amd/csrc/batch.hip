@@ -522,6 +522,9 @@ struct saship_batch {
     // large I(q) downloads go in chunks after the launch, each copied out
     // to the caller while the next one is in flight
     bool defer_iq = false;
+    // the caller's I(q) array when it is pinned host memory: the last kernel
+    // of the call writes I(q) there directly (no download, no host copy)
+    double *iq_direct = nullptr;
     std::vector<hipEvent_t> chunk_ev;
     bool timing = false;
     double t_ms = 0.0, t_all_ms = 0.0;
@@ -812,7 +815,9 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
         HIP_TRY(hipGetLastError());
     }
     const dim3 fgrid(b->nsets, std::max(1, (p.max_nq + 255) / 256));
-    double *f_result = want_result ? d_result : nullptr, *f_iq = want_iq ? d_iq : nullptr;
+    double *direct = copies ? b->iq_direct : nullptr;
+    double *f_result = want_result ? d_result : nullptr;
+    double *f_iq = !want_iq ? nullptr : direct && b->n_smear == 0 ? direct : d_iq;
 #define SAS_FINAL(SM)                                                                                        \
     k_batch_final<SM><<<fgrid, 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, d_values, d_voff, d_poff, d_resoff, \
                                                 d_nchunk, b->d_sums, b->d_partial, d_spars, b->d_sprep, b->q64,  \
@@ -832,14 +837,14 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
         const int64_t blocks = (b->n_smear + 255) / 256;
         const SmearRepeat rep{b->d_srep, b->d_srstep, b->d_sroff, b->d_srw};
         k_batch_smear<<<(unsigned)blocks, 256, 0, b->st>>>(b->n_smear, b->d_sfrom, b->d_slen, b->d_sstep, b->d_soff,
-                                                          b->d_sw, rep, d_iq, d_siq);
+                                                          b->d_sw, rep, d_iq, direct ? direct : d_siq);
         HIP_TRY(hipGetLastError());
     }
     if (b->timing) HIP_TRY(hipEventRecord(b->ev3, b->st));
     if (!copies) return 0;
     if (want_result)
         HIP_TRY(hipMemcpyAsync(b->h_out, d_result, sizeof(double) * p.result_total, hipMemcpyDeviceToHost, b->st));
-    if (want_iq && !b->defer_iq)
+    if (want_iq && !b->defer_iq && !direct)
         HIP_TRY(hipMemcpyAsync(b->h_out + sizeof(double) * p.result_total, b->n_smear > 0 ? d_siq : d_iq,
                                sizeof(double) * (b->n_smear > 0 ? b->n_smear : p.nq_total),
                                hipMemcpyDeviceToHost, b->st));
@@ -924,8 +929,11 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     const bool want_result = result != nullptr, want_iq = iq != nullptr;
     const size_t iq_bytes = want_iq ? sizeof(double) * (size_t)(b->n_smear > 0 ? b->n_smear : p.nq_total) : 0;
     constexpr size_t DEFER_IQ_FROM = (size_t)8 << 20, IQ_CHUNK = (size_t)4 << 20;
-    b->defer_iq = iq_bytes >= DEFER_IQ_FROM;
-    if (b->use_graph && !b->timing) {
+    // pinned output: the final (or smearing) kernel writes it, launched
+    // eagerly since its address changes from call to call
+    b->iq_direct = want_iq ? (double *)device_alias(iq) : nullptr;
+    b->defer_iq = !b->iq_direct && iq_bytes >= DEFER_IQ_FROM;
+    if (b->use_graph && !b->timing && !b->iq_direct) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
                                     p.sep_nlp, b->n_smear, b->s_model, b->defer_iq};
@@ -967,11 +975,11 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
         }
         for (size_t c = 0; c < nchunk; c++) {
             const size_t off = c * IQ_CHUNK, len = std::min(IQ_CHUNK, iq_bytes - off);
-            HIP_TRY(hipEventSynchronize(b->chunk_ev[c]));
+            HIP_TRY(event_wait(b->chunk_ev[c]));
             copy_out((char *)iq + off, stage + off, len);
         }
     }
-    HIP_TRY(hipStreamSynchronize(b->st));
+    HIP_TRY(stream_wait(b->st));
     b->last = p;
     b->last_cutoff = cutoff;
     b->last_reff = reff_mode;
@@ -979,7 +987,8 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     b->last_iq = want_iq;
     b->have_last = true;
     if (want_result) copy_out(result, b->h_out, sizeof(double) * p.result_total);
-    if (want_iq && !nchunk) copy_out(iq, b->h_out + sizeof(double) * p.result_total, iq_bytes);
+    if (want_iq && !nchunk && !b->iq_direct) copy_out(iq, b->h_out + sizeof(double) * p.result_total, iq_bytes);
+    b->iq_direct = nullptr;
     if (b->timing) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));

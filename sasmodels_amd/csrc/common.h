@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <condition_variable>
@@ -321,6 +322,48 @@ inline void copy_out(void *dst, const void *src, size_t bytes)
     constexpr size_t PARALLEL_FROM = (size_t)2 << 20;
     if (bytes < PARALLEL_FROM) memcpy(dst, src, bytes);
     else CopyPool::get().copy(dst, src, bytes);
+}
+
+// Wait for the work enqueued on a stream (or up to an event) by polling it.
+// The runtime's blocking waits return 10-20 us after the work has ended,
+// which is most of a small call's overhead; a call waits for its own short
+// device sequence, so it polls for up to SPIN_US and only then blocks.
+constexpr double SPIN_US = 2000.0;
+
+template <class Query, class Block>
+hipError_t spin_wait(Query query, Block block)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = query();
+        if (e != hipErrorNotReady) return e;
+        if (std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() > SPIN_US)
+            return block();
+    }
+}
+
+// The device's address of p when p is device memory or host memory the
+// device can address (pinned), else NULL.
+inline void *device_alias(const void *p)
+{
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return attr.devicePointer;
+}
+
+inline bool device_accessible(const void *p) { return device_alias(p) != nullptr; }
+
+inline hipError_t stream_wait(hipStream_t st)
+{
+    return spin_wait([st] { return hipStreamQuery(st); }, [st] { return hipStreamSynchronize(st); });
+}
+
+inline hipError_t event_wait(hipEvent_t ev)
+{
+    return spin_wait([ev] { return hipEventQuery(ev); }, [ev] { return hipEventSynchronize(ev); });
 }
 
 }  // namespace saship_host

@@ -370,16 +370,6 @@ int scratch_address(int dev, char **out)
     return 0;
 }
 
-// Is p device memory (or host memory the device can address)?
-bool device_accessible(const void *p)
-{
-    hipPointerAttribute_t attr;
-    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-        (void)hipGetLastError();
-        return false;
-    }
-    return attr.devicePointer != nullptr;
-}
 }  // namespace
 
 namespace sas {
@@ -432,7 +422,7 @@ void drop_graph(saship_kernel *k)
 // hand the result over and account its time.
 int finish_call(saship_kernel *k, size_t rbytes, double *result)
 {
-    HIP_TRY(hipStreamSynchronize(k->st));
+    HIP_TRY(stream_wait(k->st));
     memcpy(result, k->h_result, rbytes);
     if (k->timing) {
         float ms = 0.0f;
@@ -467,6 +457,26 @@ int saship_init(int32_t device)
     current_device() = device;
     char *sp = nullptr;
     return scratch_address(device, &sp);
+}
+
+int saship_host_alloc(int64_t bytes, void **ptr)
+{
+    if (!ptr) return fail(SASHIP_EINVAL, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes < 1) return fail(SASHIP_EINVAL, "bytes must be >= 1");
+    HIP_TRY(hipSetDevice(current_device()));
+    void *p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, (size_t)bytes, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e != hipSuccess || !p) return fail(SASHIP_ENOMEM, std::string("pinned host memory: ") + hipGetErrorString(e));
+    *ptr = p;
+    return 0;
+}
+
+int saship_host_free(void *ptr)
+{
+    if (!ptr) return 0;
+    HIP_TRY(hipHostFree(ptr));
+    return 0;
 }
 
 int saship_probe_fma(int32_t iters, double *tflops)

@@ -18,6 +18,8 @@ The computation is FP32 on the device (FP64 for the weight sums), so
 """
 
 import ctypes as ct
+import threading
+import weakref
 
 import numpy as np
 
@@ -52,6 +54,48 @@ def probe_fma(iters=4096):
     out = ct.c_double()
     _lib.check(_lib.lib().saship_probe_fma(int(iters), ct.byref(out)))
     return out.value
+
+
+class PinnedPool:
+    """
+    Output arrays of batched calls in pinned host memory, which the batch's
+    download writes by DMA directly (no staging buffer, no host copy).
+    A buffer goes back to the pool when the last array or view over it is
+    gone, so a caller that keeps every result only grows the pool; past
+    *cap_bytes* of buffers in use, arrays are ordinary host memory again.
+    """
+
+    def __init__(self, cap_bytes=256 << 20):
+        self.cap = int(cap_bytes)
+        self._free = {}           # bytes -> [address]
+        self._held = 0            # bytes of pinned buffers, in use or free
+        self._mu = threading.Lock()
+
+    def empty(self, n):
+        """A float64 array of n values (pinned when the pool has room)."""
+        nbytes = 8*max(int(n), 1)
+        with self._mu:
+            stack = self._free.get(nbytes)
+            addr = stack.pop() if stack else None
+            if addr is None and self._held + nbytes <= self.cap:
+                ptr = ct.c_void_p()
+                if _lib.lib().saship_host_alloc(nbytes, ct.byref(ptr)) == 0 and ptr.value:
+                    addr = ptr.value
+                    self._held += nbytes
+        if addr is None:
+            return np.empty(n, "d")
+        buf = (ct.c_double*(nbytes//8)).from_address(addr)
+        fin = weakref.finalize(buf, self._give_back, nbytes, addr)
+        fin.atexit = False
+        return np.frombuffer(buf, dtype=np.float64, count=n)
+
+    def _give_back(self, nbytes, addr):
+        with self._mu:
+            self._free.setdefault(nbytes, []).append(addr)
+
+
+#: the process's pool of pinned output arrays
+pinned = PinnedPool()
 
 
 class HipModel(KernelModel):
@@ -315,7 +359,9 @@ class HipBatch:
             out=None):
         """(raw accumulators of all sets back to back, I(q) of all sets back to back).
         *out*: a float64 array of the I(q) size to write I(q) into (a fit
-        loop that reuses it avoids faulting in new memory on every call)."""
+        loop that reuses it avoids faulting in new memory on every call);
+        without it I(q) goes into an array from the pinned pool, which the
+        download fills directly."""
         det, vals, voff, ndet = self._pack(calls)
         nout = self._nout()
         result = np.empty(int(np.sum(nout*self.nq + 4)), "d") if want_result else None
@@ -326,7 +372,7 @@ class HipBatch:
                 raise ValueError("out must be a writeable contiguous float64 array of %d values" % n_iq)
             iq = out.reshape(-1)
         else:
-            iq = np.empty(n_iq, "d") if want_iq else None
+            iq = pinned.empty(n_iq) if want_iq else None
         _lib.check(_lib.lib().saship_batch_call(
             self._handle, self._kind(magnetic), det.ctypes.data, ndet, vals.ctypes.data, voff.ctypes.data,
             float(cutoff), int(radius_effective_mode),
