@@ -627,9 +627,8 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     for (int s = 0; s < ns; s++) {
         const int64_t nv = voff_in[s + 1] - voff_in[s];
         const int32_t *det = details + (size_t)s * n_details;
-        const std::string where = "set " + std::to_string(s) + ": ";
-        if (nv < 0) return fail(SASHIP_EINVAL, where + "values_offset must be non-decreasing");
-        if (int e = validate_set(id, kind, det, n_details, nv, 0, -1, where)) return e;
+        if (nv < 0) return fail(SASHIP_EINVAL, set_prefix(s) + "values_offset must be non-decreasing");
+        if (int e = validate_set(id, kind, det, n_details, nv, 0, -1, s)) return e;
         const int64_t num_eval = det[n_details - 4];
         const int nq = (int)(b->qoff[s + 1] - b->qoff[s]);
         p.max_nq = std::max(p.max_nq, nq);
@@ -761,9 +760,12 @@ int enqueue_sep(saship_batch *b, const Plan &p, const sas::ExpandArgs &ea, const
 
 // The device work of one call; with `copies` the staging upload before it
 // and the result downloads after it (without: inputs and outputs stay in HBM).
+// parts: the head (upload, expansion, evaluation) and/or the tail (structure
+// factor, final sums, smearing, downloads).
+constexpr int PART_HEAD = 1, PART_TAIL = 2;
 template <class M>
 int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq,
-            bool copies)
+            bool copies, int parts)
 {
     using namespace sas;
     const char *d = b->d_args;
@@ -779,36 +781,38 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     double *d_result = b->d_out;
     double *d_iq = b->d_out + p.result_total;
 
-    if (copies) HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev2, b->st));
-    ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
-                  b->d_recs, b->d_npts, b->d_sums, b->d_queue, b->d_spin};
-    if (p.sep_nlp) {
-        if (int e = enqueue_sep(b, p, ea, d_poff)) return e;
-    } else {
-    switch (p.kind) {
-    case 0: k_expand<M, 0><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
-    case 1: k_expand<M, 1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
-    default: k_expand<M, 2><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
-    }
-    HIP_TRY(hipGetLastError());
+    if (parts & PART_HEAD) {
+        if (copies) HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
+        if (b->timing) HIP_TRY(hipEventRecord(b->ev2, b->st));
+        ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
+                      b->d_recs, b->d_npts, b->d_sums, b->d_queue, b->d_spin};
+        if (p.sep_nlp) {
+            if (int e = enqueue_sep(b, p, ea, d_poff)) return e;
+        } else {
+        switch (p.kind) {
+        case 0: k_expand<M, 0><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+        case 1: k_expand<M, 1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+        default: k_expand<M, 2><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+        }
+        HIP_TRY(hipGetLastError());
 
-    const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
-    BatchArgs ba{b->q32, b->q64, b->d_qoff, d_values, d_voff, b->d_recs, d_roff, d_poff, b->d_npts, b->d_spin,
-                 d_items, d_nitems, b->pts_per_item, b->m->quad, queue_grab(b, p, grid * 4)};
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
-    if (p.kind == 0) {
-        if (b->m->quad_n > 0) k_batch_eval_1d<M, true><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
-        else k_batch_eval_1d<M, false><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
-    } else if (p.kind == 1) {
-        k_batch_eval_2d<M, false><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
-    } else {
-        k_batch_eval_2d<M, true><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
+        const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
+        BatchArgs ba{b->q32, b->q64, b->d_qoff, d_values, d_voff, b->d_recs, d_roff, d_poff, b->d_npts, b->d_spin,
+                     d_items, d_nitems, b->pts_per_item, b->m->quad, queue_grab(b, p, grid * 4)};
+        if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
+        if (p.kind == 0) {
+            if (b->m->quad_n > 0) k_batch_eval_1d<M, true><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
+            else k_batch_eval_1d<M, false><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
+        } else if (p.kind == 1) {
+            k_batch_eval_2d<M, false><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
+        } else {
+            k_batch_eval_2d<M, true><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
+        }
+        HIP_TRY(hipGetLastError());
+        if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
+        }
     }
-    HIP_TRY(hipGetLastError());
-    if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
-    }
-
+    if (!(parts & PART_TAIL)) return 0;
     const double *d_spars = p.structure ? (const double *)(d + p.off_spars) : nullptr;
     if (p.structure) {
         k_structure_prep<<<(b->nsets + 63) / 64, 64, 0, b->st>>>(b->nsets, b->s_model, d_spars, b->d_sums, b->d_sprep);
@@ -852,46 +856,46 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
 }
 
 int enqueue_model(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool want_result, bool want_iq,
-                  bool copies = true)
+                  bool copies = true, int parts = PART_HEAD | PART_TAIL)
 {
     switch (b->m->id) {
-    case 0: return enqueue<sas::Sphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 1: return enqueue<sas::Cylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 2: return enqueue<sas::Ellipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 3: return enqueue<sas::CoreShellCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 4: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 6: return enqueue<sas::Parallelepiped>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 7: return enqueue<sas::TriaxialEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 8: return enqueue<sas::BccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 9: return enqueue<sas::FccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 10: return enqueue<sas::ScParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 11: return enqueue<sas::HollowCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 12: return enqueue<sas::Vesicle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 17: return enqueue<sas::CoreMultiShell>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 18: return enqueue<sas::Onion>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 19: return enqueue<sas::CoreShellEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 20: return enqueue<sas::EllipticalCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 21: return enqueue<sas::CoreShellBicelle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 22: return enqueue<sas::Guinier>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 23: return enqueue<sas::GaussianPeak>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 24: return enqueue<sas::Lorentz>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 25: return enqueue<sas::Dab>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 26: return enqueue<sas::GelFit>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 27: return enqueue<sas::MassFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 28: return enqueue<sas::SurfaceFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 29: return enqueue<sas::MassSurfaceFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 30: return enqueue<sas::StarPolymer>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 31: return enqueue<sas::MonoGaussCoil>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 32: return enqueue<sas::Fractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 33: return enqueue<sas::FractalCoreShell>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 34: return enqueue<sas::LamellarHg>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 35: return enqueue<sas::LamellarStackCaille>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 36: return enqueue<sas::LamellarHgStackCaille>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 37: return enqueue<sas::LamellarStackParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 38: return enqueue<sas::LinearPearls>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 39: return enqueue<sas::PolymerMicelle>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    case 40: return enqueue<sas::Raspberry>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
-    default: return enqueue<sas::FuzzySphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies);
+    case 0: return enqueue<sas::Sphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 1: return enqueue<sas::Cylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 2: return enqueue<sas::Ellipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 3: return enqueue<sas::CoreShellCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 4: return enqueue<sas::CoreShellSphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 6: return enqueue<sas::Parallelepiped>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 7: return enqueue<sas::TriaxialEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 8: return enqueue<sas::BccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 9: return enqueue<sas::FccParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 10: return enqueue<sas::ScParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 11: return enqueue<sas::HollowCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 12: return enqueue<sas::Vesicle>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 17: return enqueue<sas::CoreMultiShell>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 18: return enqueue<sas::Onion>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 19: return enqueue<sas::CoreShellEllipsoid>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 20: return enqueue<sas::EllipticalCylinder>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 21: return enqueue<sas::CoreShellBicelle>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 22: return enqueue<sas::Guinier>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 23: return enqueue<sas::GaussianPeak>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 24: return enqueue<sas::Lorentz>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 25: return enqueue<sas::Dab>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 26: return enqueue<sas::GelFit>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 27: return enqueue<sas::MassFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 28: return enqueue<sas::SurfaceFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 29: return enqueue<sas::MassSurfaceFractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 30: return enqueue<sas::StarPolymer>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 31: return enqueue<sas::MonoGaussCoil>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 32: return enqueue<sas::Fractal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 33: return enqueue<sas::FractalCoreShell>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 34: return enqueue<sas::LamellarHg>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 35: return enqueue<sas::LamellarStackCaille>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 36: return enqueue<sas::LamellarHgStackCaille>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 37: return enqueue<sas::LamellarStackParacrystal>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 38: return enqueue<sas::LinearPearls>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 39: return enqueue<sas::PolymerMicelle>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    case 40: return enqueue<sas::Raspberry>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
+    default: return enqueue<sas::FuzzySphere>(b, p, cutoff, reff_mode, want_result, want_iq, copies, parts);
     }
 }
 
@@ -929,21 +933,23 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     const bool want_result = result != nullptr, want_iq = iq != nullptr;
     const size_t iq_bytes = want_iq ? sizeof(double) * (size_t)(b->n_smear > 0 ? b->n_smear : p.nq_total) : 0;
     constexpr size_t DEFER_IQ_FROM = (size_t)8 << 20, IQ_CHUNK = (size_t)4 << 20;
-    // pinned output: the final (or smearing) kernel writes it, launched
-    // eagerly since its address changes from call to call
+    // pinned output: the final (or smearing) kernel writes it; that tail is
+    // launched eagerly (after the graph of the head) since the output's
+    // address changes from call to call
     b->iq_direct = want_iq ? (double *)device_alias(iq) : nullptr;
     b->defer_iq = !b->iq_direct && iq_bytes >= DEFER_IQ_FROM;
-    if (b->use_graph && !b->timing && !b->iq_direct) {
+    const int gparts = b->iq_direct ? PART_HEAD : PART_HEAD | PART_TAIL;
+    if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
-                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq};
+                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq, gparts};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
         if (!b->gexec || key != b->gkey) {
             drop_graph(b);
             HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq);
+            const int rc = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq, true, gparts);
             hipGraph_t g = nullptr;
             const hipError_t ce = hipStreamEndCapture(b->st, &g);
             if (rc) {
@@ -956,6 +962,8 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
             b->gkey = key;
         }
         HIP_TRY(hipGraphLaunch(b->gexec, b->st));
+        if (gparts == PART_HEAD)
+            if (int e = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq, true, PART_TAIL)) return e;
     } else {
         if (int e = enqueue_model(b, p, cutoff, reff_mode, want_result, want_iq)) return e;
     }

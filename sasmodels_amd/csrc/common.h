@@ -151,32 +151,36 @@ inline int cylinder_separable(const int32_t *details)
     return (nl + 3) / 4 * 4;
 }
 
+// "set s: " for messages about set s of a batch ("" for s < 0)
+inline std::string set_prefix(int set) { return set < 0 ? std::string() : "set " + std::to_string(set) + ": "; }
+
 // Checks one set's details/values against the model before any device work:
-// every index the kernels derive from them stays inside the buffers.
+// every index the kernels derive from them stays inside the buffers (set:
+// the set's index in a batch, -1 for a single call; only named in messages).
 inline int validate_set(int id, int kind, const int32_t *details, int64_t n_details,
-                        int64_t n_values, int pd_start, int pd_stop, const std::string &where)
+                        int64_t n_values, int pd_start, int pd_stop, int set)
 {
     const int mp = model_maxpd(id);
     if (n_details != 4 * mp + 4)
-        return fail(SASHIP_EINVAL, where + "details must have 4*max_pd+4 = " + std::to_string(4 * mp + 4) + " entries");
+        return fail(SASHIP_EINVAL, set_prefix(set) + "details must have 4*max_pd+4 = " + std::to_string(4 * mp + 4) + " entries");
     const int32_t num_eval = details[4 * mp], num_weights = details[4 * mp + 1];
-    if (num_eval < 0) return fail(SASHIP_EINVAL, where + "num_eval < 0");
+    if (num_eval < 0) return fail(SASHIP_EINVAL, set_prefix(set) + "num_eval < 0");
     if (pd_start < 0 || (pd_stop >= 0 && pd_stop < pd_start))
-        return fail(SASHIP_EINVAL, where + "need 0 <= pd_start <= pd_stop");
-    if (pd_stop > num_eval) return fail(SASHIP_EINVAL, where + "pd_stop exceeds num_eval");
+        return fail(SASHIP_EINVAL, set_prefix(set) + "need 0 <= pd_start <= pd_stop");
+    if (pd_stop > num_eval) return fail(SASHIP_EINVAL, set_prefix(set) + "pd_stop exceeds num_eval");
     if (num_weights < 0 || n_values < model_nvalues(id) + 2 * (int64_t)num_weights)
-        return fail(SASHIP_EINVAL, where + "values shorter than nvalues + 2*num_weights");
+        return fail(SASHIP_EINVAL, set_prefix(set) + "values shorter than nvalues + 2*num_weights");
     for (int i = 0; i < mp; i++) {
         const int par = details[i], len = details[mp + i], off = details[2 * mp + i], stride = details[3 * mp + i];
         if (par < 0 || par >= model_npars(id) || len < 1 || off < 0 || off + len > num_weights || stride < 1)
-            return fail(SASHIP_EINVAL, where + "inconsistent details buffer at pd level " + std::to_string(i));
+            return fail(SASHIP_EINVAL, set_prefix(set) + "inconsistent details buffer at pd level " + std::to_string(i));
     }
-    if (kind == 2 && model_structure(id)) return fail(SASHIP_EINVAL, where + "structure factors have no magnetic kernel");
-    if (kind == 2 && !model_magnetic(id)) return fail(SASHIP_EINVAL, where + "the model has no sld: no magnetic kernel");
+    if (kind == 2 && model_structure(id)) return fail(SASHIP_EINVAL, set_prefix(set) + "structure factors have no magnetic kernel");
+    if (kind == 2 && !model_magnetic(id)) return fail(SASHIP_EINVAL, set_prefix(set) + "the model has no sld: no magnetic kernel");
     if (kind != 0 && model_oriented(id)) {
         const int tp = details[4 * mp + 3];
         const int last = tp + (model_abc(id) ? 2 : 1);
-        if (tp < 0 || last >= model_npars(id)) return fail(SASHIP_EINVAL, where + "theta_par out of range");
+        if (tp < 0 || last >= model_npars(id)) return fail(SASHIP_EINVAL, set_prefix(set) + "theta_par out of range");
     }
     return 0;
 }

@@ -691,7 +691,7 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
         return fail(SASHIP_EINVAL, kind == 0 ? "Iq kernel needs 1D q" : "2D kernels need (qx, qy) input");
     const int id = k->m->id;
     if (pd_stop < pd_start) return fail(SASHIP_EINVAL, "need 0 <= pd_start <= pd_stop");
-    if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, "")) return e;
+    if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, -1)) return e;
     std::lock_guard<std::mutex> lock(k->mu);
     HIP_TRY(hipSetDevice(k->dev));
     const int nq = k->nq, nout = kind == 0 && model_has_fq(id) ? 2 : 1;
@@ -787,7 +787,7 @@ int saship_kernel_call_noalloc(saship_model *m, int32_t kind, int32_t nq, const 
     if (nq > 0 && !q) return fail(SASHIP_EINVAL, "q is NULL");
     const int id = m->id;
     if (pd_stop < pd_start) return fail(SASHIP_EINVAL, "need 0 <= pd_start <= pd_stop");
-    if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, "")) return e;
+    if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, -1)) return e;
     HIP_TRY(hipSetDevice(m->dev));
     if ((nq > 0 && !device_accessible(q)) || !device_accessible(result))
         return fail(SASHIP_EINVAL, "q and result must be device memory");

@@ -18,6 +18,7 @@ The computation is FP32 on the device (FP64 for the weight sums), so
 """
 
 import ctypes as ct
+import itertools
 import threading
 import weakref
 
@@ -341,10 +342,9 @@ class HipBatch:
     def _pack(self, calls):
         if len(calls) != self.nsets:
             raise ValueError("expected %d (call_details, values) pairs, got %d" % (self.nsets, len(calls)))
-        det = np.concatenate([getattr(cd, "buffer", cd) for cd, _ in calls]).astype(np.int32, copy=False)
-        vals = [v for _, v in calls]
-        voff = np.zeros(len(vals) + 1, np.int64)
-        np.cumsum(np.fromiter((len(v) for v in vals), np.int64, len(vals)), out=voff[1:])
+        dets, vals = zip(*calls)
+        det = np.concatenate([getattr(cd, "buffer", cd) for cd in dets]).astype(np.int32, copy=False)
+        voff = np.fromiter(itertools.accumulate(map(len, vals), initial=0), np.int64, len(vals) + 1)
         vals = np.concatenate(vals).astype(np.float64, copy=False)
         return np.ascontiguousarray(det), np.ascontiguousarray(vals), voff, det.size//self.nsets
 
