@@ -824,30 +824,48 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad, nzero, ngroups, mode;
     __shared__ float steps[2];
     if (a.queue && set == 0 && tid < WORK_QUEUES) a.queue[tid * WORK_QUEUE_STRIDE] = 0;
-    for (int i = tid; i < NRMAX * nlp; i += EXPAND_THREADS) {
-        AT[i] = 0.0f;
-        BT[i] = 0.0f;
-    }
-    for (int i = tid; i < NRMAX * NRMAX; i += EXPAND_THREADS) acc[i] = 0;
-    __shared__ int nacc, notap;
+    __shared__ int nacc, notap, nowhole;
     if (tid == 0) {
         bad = 0;
         nacc = 0;
         notap = 0;
+        // the whole mesh, no pruning, no effective radius (checked per point below)
+        nowhole = !(pd_start == 0 && pd_stop == num_eval && a.cutoff <= 0.0 && a.reff_mode == 0);
     }
     __syncthreads();
     // equally spaced grids (every distribution of the reference is a linspace)
     const double dr = NR > 1 ? (rv[NR - 1] - rv[0]) / (NR - 1) : 0.0;
     const double dl = NL > 1 ? (lv[NL - 1] - lv[0]) / (NL - 1) : 0.0;
+    constexpr double WMIN = 1e-150;    // products of two such weights stay positive
     if (tid < NR) {
         if (!(fabs(rv[tid] - (rv[0] + tid * dr)) <= 1e-9 * fabs(rv[NR - 1]))) notap = 1;
+        if (!(rv[tid] >= 0.0 && rw[tid] >= WMIN)) nowhole = 1;
     } else if (tid >= NRMAX && tid - NRMAX < NL) {
         const int j = tid - NRMAX;
         if (!(fabs(lv[j] - (lv[0] + j * dl)) <= 1e-9 * fabs(lv[NL - 1]))) notap = 1;
+        if (!(lv[j] >= 0.0 && lw[j] >= WMIN)) nowhole = 1;
+    } else if (tid >= 2 * NRMAX && tid - 2 * NRMAX < MP) {
+        const int k = tid - 2 * NRMAX;
+        if (k != kr && k != kl && !(pd_weight[pd_off[k]] >= WMIN)) nowhole = 1;
     }
+    __syncthreads();
+    // Every point of an equally spaced product mesh valid and accepted: the
+    // rotation mode needs neither the acceptance mask nor the bilinear
+    // tables, and the weight sums factorise over the two grids (summed in
+    // another order than the walk below, which the other modes take).
+    const bool whole = nowhole == 0 && notap == 0;
+    constexpr int NW = EXPAND_THREADS / 64;
+    __shared__ double red[3][NW];
     double base[NP];
 #pragma unroll
     for (int i = 0; i < NP; i++) base[i] = values[2 + i];
+    if (!whole) {
+    for (int i = tid; i < NRMAX * nlp; i += EXPAND_THREADS) {
+        AT[i] = 0.0f;
+        BT[i] = 0.0f;
+    }
+    for (int i = tid; i < NRMAX * NRMAX; i += EXPAND_THREADS) acc[i] = 0;
+    __syncthreads();
     double acc_w = 0.0, acc_v = 0.0, acc_r = 0.0;
     int nmine = 0;
     // the levels other than radius and length have one point each (a
@@ -891,8 +909,6 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
     }
     if (nmine) atomicAdd(&nacc, nmine);
     // weight sums: within each wave by shuffles, then over the waves
-    constexpr int NW = EXPAND_THREADS / 64;
-    __shared__ double red[3][NW];
     for (int off = 32; off > 0; off >>= 1) {
         acc_w += __shfl_down(acc_w, off);
         acc_v += __shfl_down(acc_v, off);
@@ -910,6 +926,24 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
             red[1][0] += red[1][w];
             red[2][0] += red[2][w];
         }
+    }
+    } else if (tid == 0) {
+        double sr_w = 0.0, sr_v = 0.0, sl_w = 0.0, sl_v = 0.0;
+        for (int i = 0; i < NR; i++) {
+            sr_w += rw[i];
+            sr_v += rw[i] * (rv[i] * rv[i]);
+        }
+        for (int j = 0; j < NL; j++) {
+            sl_w += lw[j];
+            sl_v += lw[j] * lv[j];
+        }
+        double wo = 1.0;
+        for (int k = MP - 1; k >= 0; k--)
+            if (k != kr && k != kl) wo = pd_weight[pd_off[k]] * wo;
+        red[0][0] = sr_w * sl_w * wo;
+        red[1][0] = PI_D * sr_v * sl_v * wo;
+        red[2][0] = 0.0;
+        nacc = NR * NL;
     }
     __syncthreads();
     // every point accepted on equally spaced grids: rotation mode, without
