@@ -362,7 +362,19 @@ class HipBatch:
         loop that reuses it avoids faulting in new memory on every call);
         without it I(q) goes into an array from the pinned pool, which the
         download fills directly."""
-        det, vals, voff, ndet = self._pack(calls)
+        return self.raw_packed(self._pack(calls), cutoff, magnetic, radius_effective_mode, want_result, want_iq,
+                               out)
+
+    def raw_packed(self, packed, cutoff=0.0, magnetic=False, radius_effective_mode=0, want_result=True,
+                   want_iq=True, out=None):
+        """:meth:`raw` on inputs already packed as (details of every set back
+        to back, values of every set back to back, values offsets [nsets+1],
+        details per set)."""
+        det, vals, voff, ndet = packed
+        if det.dtype != np.int32 or vals.dtype != np.float64 or voff.dtype != np.int64 \
+                or det.size != ndet*self.nsets or voff.size != self.nsets + 1 or voff[-1] > vals.size:
+            raise ValueError("packed inputs do not match the batch")
+        det, vals, voff = (np.ascontiguousarray(a) for a in (det, vals, voff))
         nout = self._nout()
         result = np.empty(int(np.sum(nout*self.nq + 4)), "d") if want_result else None
         n_iq = int(self.qoff[-1]) if self._n_out is None else int(self._n_out.sum())
@@ -382,6 +394,16 @@ class HipBatch:
     def __call__(self, calls, cutoff=0.0, magnetic=False, radius_effective_mode=0, out=None):
         """List of I(q), one per set (views of one array: *out* when given, see :meth:`raw`)."""
         _, iq = self.raw(calls, cutoff, magnetic, radius_effective_mode, want_result=False, want_iq=True, out=out)
+        return self._sets(iq)
+
+    def call_packed(self, packed, cutoff=0.0, magnetic=False, radius_effective_mode=0, out=None):
+        """:meth:`__call__` on packed inputs (see :meth:`raw_packed`)."""
+        _, iq = self.raw_packed(packed, cutoff, magnetic, radius_effective_mode, want_result=False, want_iq=True,
+                                out=out)
+        return self._sets(iq)
+
+    def _sets(self, iq):
+        """I(q) of all sets back to back -> one view per set."""
         if self._n_out is not None:
             return np.split(iq, np.cumsum(self._n_out)[:-1])
         if self._uniform:

@@ -345,33 +345,91 @@ class ProductBatch:
             self._idx_cache[nweights] = idx
         return idx
 
-    def _split(self, calls):
+    def _groups(self, calls):
+        """Per run of sets sharing a mesh layout (a fit's population does):
+        (set indices, P's details, P's values as rows, structure parameters as
+        rows, effective-radius modes), from the stacked value vectors."""
         lay = self._layout
-        p_calls, spars, er_modes = [], np.zeros((len(calls), SPARS)), set()
-        for s, (cd, values) in enumerate(calls):
-            scale, background, volfrac, er_mode, beta_mode = lay.modes(values)
-            if beta_mode and self.dim == "2d":
+        cds, vals = zip(*calls)
+        D = np.concatenate([cd.buffer for cd in cds]).reshape(len(cds), -1)
+        lens = np.fromiter(map(len, vals), np.int64, len(vals))
+        if (D == D[0]).all() and (lens == lens[0]).all():
+            groups = [np.arange(len(calls))]
+        else:
+            key = np.concatenate([D, lens[:, None]], axis=1)
+            _, inv = np.unique(key, axis=0, return_inverse=True)
+            inv = inv.ravel()
+            groups = [np.flatnonzero(inv == g) for g in range(inv.max() + 1)]
+        ns = len(range(*lay.s_values.indices(lay.nvalues)))
+        for members in groups:
+            cd = cds[members[0]]
+            V = np.concatenate([vals[s] for s in members]).reshape(len(members), -1)
+            er = V[:, lay.er_mode_index].astype(int) if lay.er_mode_index > 0 else np.zeros(len(members), int)
+            beta = V[:, lay.beta_index] > 0 if lay.beta_index > 0 else np.zeros(len(members), bool)
+            if self.dim == "2d" and beta.any():
                 raise NotImplementedError("beta not yet supported for 2D")
             idx, size = self._p_index(int(cd.num_weights))
-            p_values = np.zeros(size)
-            p_values[:idx.size] = values[idx]
-            p_calls.append((self._p_details(cd), p_values))
-            spars[s, :5] = (values[lay.er_index], volfrac, 1.0 if beta_mode else 0.0, er_mode,
-                            1.0 if lay.volfrac_in_p else 0.0)
-            extra = values[lay.s_values]
-            spars[s, 5:5 + len(extra)] = extra
-            if er_mode > 0:
-                er_modes.add(er_mode)
-        if len(er_modes) > 1:
-            raise ValueError("sets of one call need the same radius_effective_mode (got %s)"
-                             % sorted(er_modes))
-        return p_calls, spars, (er_modes.pop() if er_modes else 0)
+            P = np.zeros((len(members), size))
+            P[:, :idx.size] = V[:, idx]
+            sp = np.zeros((len(members), SPARS))
+            sp[:, 0] = V[:, lay.er_index]
+            sp[:, 1] = V[:, lay.volfrac_index]
+            sp[:, 2] = beta
+            sp[:, 3] = er
+            sp[:, 4] = 1.0 if lay.volfrac_in_p else 0.0
+            sp[:, 5:5 + ns] = V[:, lay.s_values]
+            yield members, self._p_details(cd), P, sp, er
+
+    @staticmethod
+    def _er_mode(modes):
+        modes = set(int(e) for e in np.unique(np.asarray(modes, int)) if e > 0)
+        if len(modes) > 1:
+            raise ValueError("sets of one call need the same radius_effective_mode (got %s)" % sorted(modes))
+        return modes.pop() if modes else 0
+
+    def _split(self, calls):
+        """(P calls, structure parameters per set, the call's effective-radius mode)."""
+        n = len(calls)
+        p_calls, spars, modes = [None]*n, np.zeros((n, SPARS)), []
+        for rows, det, P, sp, er in self._groups(calls):
+            spars[rows] = sp
+            for k, s in enumerate(rows):
+                p_calls[s] = (det, P[k])
+            modes.append(er)
+        return p_calls, spars, self._er_mode(np.concatenate(modes))
+
+    def _split_packed(self, calls):
+        """As :meth:`_split`, with P's calls packed for HipBatch.call_packed
+        (details and values of every set back to back, in set order)."""
+        n = len(calls)
+        groups = list(self._groups(calls))
+        spars, modes = np.zeros((n, SPARS)), []
+        sizes = np.zeros(n, np.int64)
+        for rows, _, P, sp, er in groups:
+            spars[rows] = sp
+            sizes[rows] = P.shape[1]
+            modes.append(er)
+        modes = np.concatenate(modes)
+        ndet = groups[0][1].buffer.size
+        if len(groups) == 1:
+            rows, det, P, _, _ = groups[0]
+            vals = P.ravel()
+            dets = np.tile(det.buffer.astype(np.int32), n)
+        else:
+            vals = np.empty(int(sizes.sum()))
+            dets = np.empty(n*ndet, np.int32)
+            start = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            for rows, det, P, _, _ in groups:
+                vals[start[rows][:, None] + np.arange(P.shape[1])] = P
+                dets.reshape(n, ndet)[rows] = det.buffer
+        voff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        return (dets, vals, voff, ndet), spars, self._er_mode(modes)
 
     def __call__(self, calls, cutoff=0.0, magnetic=False):
         """List of P@S I(q), one per set (smeared when set_resolution is on)."""
-        p_calls, spars, er_mode = self._split(calls)
+        packed, spars, er_mode = self._split_packed(calls)
         self.batch.set_structure(spars, self._s_model)
-        return self.batch(p_calls, cutoff, magnetic, radius_effective_mode=er_mode)
+        return self.batch.call_packed(packed, cutoff, magnetic, radius_effective_mode=er_mode)
 
     def set_resolution(self, resolutions):
         """As :meth:`HipBatch.set_resolution`: smear each set's P@S I(q) on the device."""
