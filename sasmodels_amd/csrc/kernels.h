@@ -786,7 +786,7 @@ __constant__ constexpr SerCoef SER_COEF = make_ser_coef();
 // moment series.  (Templated so that both translation units can
 // instantiate it.)
 template <int VARIANT = 0>
-__global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tables t)
+__device__ __forceinline__ void sep_expand(const ExpandArgs &a, const Tables &t)
 {
     constexpr int MP = Cylinder::MAX_PD, NP = Cylinder::NPARS;
     const int set = blockIdx.x, nlp = t.nlp;
@@ -1172,6 +1172,12 @@ __global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tab
         }
         t.SER[(size_t)set * SER_STRIDE + c] = v;
     }
+}
+
+template <int VARIANT = 0>
+__global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand(ExpandArgs a, Tables t)
+{
+    sep_expand<VARIANT>(a, t);
 }
 
 // Loads of the rotation-mode tables from host or device memory.

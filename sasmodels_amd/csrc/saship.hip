@@ -118,6 +118,42 @@ struct saship_kernel {
     std::mutex mu;
 };
 
+namespace sas {
+// Small uploads (values and details of one call) as a kernel argument: a
+// one-workgroup copy into device memory instead of a DMA transfer, which
+// costs more latency than the few hundred bytes take to move.
+constexpr int STAGE_ARG_BYTES = 2048;
+struct StageIn {
+    int nbytes;
+    alignas(16) unsigned char data[STAGE_ARG_BYTES];
+};
+__global__ __launch_bounds__(128) void k_stage_in(StageIn s, unsigned char *__restrict__ dst)
+{
+    for (int i = threadIdx.x * 16; i < s.nbytes; i += 128 * 16) {
+        const int n = min(16, s.nbytes - i);
+        if (n == 16) *(uint4 *)(dst + i) = *(const uint4 *)(s.data + i);
+        else for (int k = 0; k < n; k++) dst[i + k] = s.data[i + k];
+    }
+}
+
+// Separable expansion of a single call whose values (at byte 0) and details
+// (at byte doff) come as the kernel argument: copied into LDS, read there.
+__global__ __launch_bounds__(EXPAND_THREADS) void k_sep_expand_in(StageIn s, int doff, ExpandArgs a, sep::Tables t)
+{
+    __shared__ __align__(16) unsigned char buf[STAGE_ARG_BYTES];
+    for (int i = threadIdx.x * 16; i < s.nbytes; i += EXPAND_THREADS * 16) {
+        const int n = min(16, s.nbytes - i);
+        if (n == 16) *(uint4 *)(buf + i) = *(const uint4 *)(s.data + i);
+        else for (int k = 0; k < n; k++) buf[i + k] = s.data[i + k];
+    }
+    __syncthreads();
+    a.values = (const double *)buf;
+    a.det = (const int32_t *)(buf + doff);
+    a.voff = nullptr;
+    sep::sep_expand<0>(a, t);
+}
+}  // namespace sas
+
 namespace {
 
 // Where one evaluation reads q and leaves its intermediate and final sums:
@@ -261,8 +297,11 @@ int launch_model(const Target &t, int kind, const int32_t *d_det, const double *
 
 // Separable cylinder path: tables, one wave per (q tile, node), FP64 node sums
 // (t.sep holds the tables, t.partial 2*nq*nodes doubles).
+// With `in` the values and details come as a kernel argument (in->data, the
+// details at byte doff) straight into the expansion, with no staging copy.
 int launch_sep(const Target &t, int nlp, const int32_t *d_det, const double *d_val, int n_details, int pd_start,
-               int pd_stop, double cutoff, int reff_mode, int accumulate)
+               int pd_stop, double cutoff, int reff_mode, int accumulate, const sas::StageIn *in = nullptr,
+               int doff = 0)
 {
     using namespace sas;
     const int nq = t.nq, nnode = t.m->quad.n;
@@ -270,7 +309,8 @@ int launch_sep(const Target &t, int nlp, const int32_t *d_det, const double *d_v
     double *sums = (double *)(t.meta + 64);
     ExpandArgs ea{d_det, d_val, nullptr, nullptr, n_details, pd_start, pd_stop, cutoff, reff_mode,
                   nullptr, (int *)t.meta, sums, nullptr};
-    sep::k_sep_expand<0><<<1, EXPAND_THREADS, 0, t.st>>>(ea, tb);
+    if (in) k_sep_expand_in<<<1, EXPAND_THREADS, 0, t.st>>>(*in, doff, ea, tb);
+    else sep::k_sep_expand<0><<<1, EXPAND_THREADS, 0, t.st>>>(ea, tb);
     HIP_TRY(hipGetLastError());
     const dim3 grid(std::max(1, (nq + 63) / 64), (nnode + 3) / 4);
     if (t.ev0) HIP_TRY(hipEventRecord(t.ev0, t.st));
@@ -372,24 +412,6 @@ int scratch_address(int dev, char **out)
 
 }  // namespace
 
-namespace sas {
-// Small uploads (values and details of one call) as a kernel argument: a
-// one-workgroup copy into device memory instead of a DMA transfer, which
-// costs more latency than the few hundred bytes take to move.
-constexpr int STAGE_ARG_BYTES = 2048;
-struct StageIn {
-    int nbytes;
-    alignas(16) unsigned char data[STAGE_ARG_BYTES];
-};
-__global__ __launch_bounds__(128) void k_stage_in(StageIn s, unsigned char *__restrict__ dst)
-{
-    for (int i = threadIdx.x * 16; i < s.nbytes; i += 128 * 16) {
-        const int n = min(16, s.nbytes - i);
-        if (n == 16) *(uint4 *)(dst + i) = *(const uint4 *)(s.data + i);
-        else for (int k = 0; k < n; k++) dst[i + k] = s.data[i + k];
-    }
-}
-}  // namespace sas
 
 namespace sas {
 // Diagnostic: one structure factor's coefficients (thread 0) and S at every q.
@@ -735,6 +757,11 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
             sas::StageIn in;
             in.nbytes = (int)staged;
             memcpy(in.data, k->h_stage, staged);
+            // the separable expansion reads the argument itself (nothing
+            // later reads the values or details)
+            if (path == 1)
+                return launch_sep(t, nlp, d_det, d_val, n_details, pd_start, pd_stop, cutoff, reff_mode, accumulate,
+                                  &in, (int)doff);
             sas::k_stage_in<<<1, 128, 0, k->st>>>(in, (unsigned char *)k->d_args);
             HIP_TRY(hipGetLastError());
         } else {
