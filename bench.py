@@ -298,10 +298,15 @@ def bench_plain(model, info, steps, warmup, separable=True):
 
 def kernel_ms(batch, calls, reps=10, **kw):
     """Mean device time of the batch's evaluation kernel over *reps* eager
-    calls on *calls* (HIP events on the batch's stream)."""
+    calls on *calls* (HIP events on the batch's stream).  The calls write
+    I(q) to an ordinary host array, so the kernels' outputs stay in HBM (a
+    pinned output array would have the evaluation kernel of single-chunk
+    sets write across PCIe)."""
+    n = int(batch.qoff[-1]) if batch._n_out is None else int(batch._n_out.sum())
+    out = np.empty(n)
     batch.set_timing(True)
     for _ in range(reps):
-        batch(calls, **kw)
+        batch(calls, out=out, **kw)
     ms, n = batch.get_timing()
     batch.set_timing(False)
     return ms/max(n, 1)

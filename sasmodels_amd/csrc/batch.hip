@@ -40,7 +40,8 @@ constexpr int QTILE = 64;   // q points per work item: one per lane
 struct BatchArgs {
     const float *q32;            // 1D: q; 2D: interleaved (qx, qy)
     const double *q64;           // 2D: interleaved (qx, qy) in FP64
-    const int64_t *qoff;         // [nsets+1] q offsets
+    const int64_t *qoff;         // [nsets+1] q offsets (outputs)
+    const int64_t *qdoff;        // [nsets] offsets of each set's q values (0 for all when the sets share q)
     const double *values;
     const int64_t *voff;         // [nsets]
     const float4 *recs;
@@ -53,7 +54,23 @@ struct BatchArgs {
     int pts_per_item;
     Quad quad;
     int grab;                    // items per queue atomic
+    double *iq;                  // every set one chunk, no final kernel: I(q) out (NULL: partial sums)
+    const double *sums;          // [nsets][4] weight sums (from k_expand), for iq
 };
+
+// I(q) of set s from its <F^2> sum when the set is one chunk: the final
+// kernel's normalisation (k_batch_final without a structure factor), in its
+// operation order, so the result is the same to the bit.
+__device__ __forceinline__ double direct_iq(const BatchArgs &a, int s, double f2)
+{
+    const double *sm = a.sums + 4 * s;
+    double total = sm[0];
+    if (total == 0.0) total = 1.0;
+    double shell = sm[2] / total;
+    if (shell == 0.0) shell = 1.0;
+    const double *v = a.values + a.voff[s];
+    return v[0] / shell * (f2 / total) + v[1];
+}
 
 // The queue counter and the partial sums are separate __restrict__ kernel
 // arguments: with every store of a kernel going through them, the compiler
@@ -116,10 +133,10 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
         if (item < 0) break;
         const int4 it = a.items[item];
         const int s = it.x, tile = it.y, chunk = it.z;
-        const int64_t q0 = a.qoff[s];
+        const int64_t q0 = a.qoff[s], qd0 = a.qdoff[s];
         const int nq = (int)(a.qoff[s + 1] - q0);
         const int iq = tile * QTILE + lane;
-        const float q = iq < nq ? a.q32[q0 + iq] : 0.0f;
+        const float q = iq < nq ? a.q32[qd0 + iq] : 0.0f;
         const int npts = a.npts[s];
         const int c0 = min(npts, chunk * a.pts_per_item);
         const int c1 = min(npts, c0 + a.pts_per_item);
@@ -134,7 +151,7 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
             shared = c1 - c0 > 1 && M::shares_lengths(rf, 4 * RV, c0, c1);
             if (shared) M::eval1d_shared(q, rf, 4 * RV, c0, c1, a.quad, a1, a2);
         }
-        const double qd = M::FP64 && iq < nq ? a.q64[q0 + iq] : 0.0;
+        const double qd = M::FP64 && iq < nq ? a.q64[qd0 + iq] : 0.0;
         for (int j = c0; j < c1 && !shared; j++) {
             if constexpr (M::FP64 && RV > BIG_RECORD) {
                 // a record too large for registers is read where it lies
@@ -158,7 +175,9 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
                 a2.add(f2);
             }
         }
-        if (iq < nq) {
+        if (iq < nq && a.iq) {
+            a.iq[q0 + iq] = direct_iq(a, s, M::FP64 ? d2 : a2.value());
+        } else if (iq < nq) {
             double *dst = partial + a.poff[s] + (size_t)chunk * 2 * nq;
             dst[iq] = M::FP64 ? d2 : a2.value();
             dst[nq + iq] = M::FP64 ? d1 : a1.value();
@@ -179,12 +198,12 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a, int *__restr
         if (item < 0) break;
         const int4 it = a.items[item];
         const int s = it.x, tile = it.y, chunk = it.z;
-        const int64_t q0 = a.qoff[s];
+        const int64_t q0 = a.qoff[s], qd0 = a.qdoff[s];
         const int nq = (int)(a.qoff[s + 1] - q0);
         const int iq = tile * QTILE + lane;
         const bool ok = iq < nq;
         Lane2D L;
-        L.set(ok, ok ? a.q64[2 * (q0 + iq)] : 0.0, ok ? a.q64[2 * (q0 + iq) + 1] : 0.0);
+        L.set(ok, ok ? a.q64[2 * (qd0 + iq)] : 0.0, ok ? a.q64[2 * (qd0 + iq) + 1] : 0.0);
         Contrast<M, MAG> con;
         con.setup(a.values + a.voff[s], MAG ? a.spin + (size_t)s * spin_floats<M>() : nullptr, L.qx, L.qy);
         const int npts = a.npts[s];
@@ -198,7 +217,8 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a, int *__restr
             for (int j = b0; j < b1; j++) blk += point2d<M, MAG>(rec + (size_t)j * RV, L, con);
             acc.add(blk);
         }
-        if (ok) partial[a.poff[s] + (size_t)chunk * nq + iq] = acc.value();
+        if (ok && a.iq) a.iq[q0 + iq] = direct_iq(a, s, acc.value());
+        else if (ok) partial[a.poff[s] + (size_t)chunk * nq + iq] = acc.value();
     }
 }
 
@@ -211,6 +231,7 @@ __global__ __launch_bounds__(256) void k_batch_eval_2d(BatchArgs a, int *__restr
 struct SepArgs {
     const float *q32;
     const int64_t *qoff;
+    const int64_t *qdoff;
     sep::Tables t;
     Quad quad;
     const int *order;            // [nsets] set at each position
@@ -265,7 +286,7 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
         const int64_t q0 = a.qoff[s];
         const int nq = (int)(a.qoff[s + 1] - q0);
         const int iq = tile * QTILE + lane;
-        const float q = iq < nq ? a.q32[q0 + iq] : 0.0f;
+        const float q = iq < nq ? a.q32[a.qdoff[s] + iq] : 0.0f;
         const int n0 = blk * a.nodes_per_item, n1 = min(a.quad.n, n0 + a.nodes_per_item);
         KSum a1, a2;
         for (int node = n0 + wave; node < n1; node += 4) {
@@ -323,7 +344,8 @@ __global__ __launch_bounds__(64) void k_structure_prep(int nsets, int s_model, c
 // that a plain batch carries none of their code and registers.
 template <int SM>
 __global__ __launch_bounds__(256) void k_batch_final(
-    int nout, int nres, const int64_t *__restrict__ qoff, const double *__restrict__ values,
+    int nout, int nres, const int64_t *__restrict__ qoff, const int64_t *__restrict__ qdoff,
+    const double *__restrict__ values,
     const int64_t *__restrict__ voff, const int64_t *__restrict__ poff,
     const int64_t *__restrict__ resoff, const int *__restrict__ nchunk,
     const double *__restrict__ sums, const double *__restrict__ partial,
@@ -382,7 +404,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
         if (!iq_out) continue;
         double v = f2 / total;
         if (sp) {
-            const int64_t g = q0 + e;
+            const int64_t g = qdoff[s] + e;
             const double q = is2d ? sqrt(q64[2 * g] * q64[2 * g] + q64[2 * g + 1] * q64[2 * g + 1]) : q64[g];
             const double S = structure_sq_prepared(SM, q, sq_pars, sprep + S_PREP * s);
             const double F1 = f1 / total;
@@ -455,6 +477,7 @@ struct Plan {
     int nblk = 0;                     // separable path: node blocks per (set, tile)
     int max_nq = 0;                   // largest q count of a set
     double mean_pts = 0.0;            // generic path: mesh points per work item (upper bound)
+    bool single_chunk = false;        // generic path: every set's mesh fits one work item
 };
 
 }  // namespace batch_host
@@ -467,6 +490,7 @@ struct saship_batch {
     float *q32 = nullptr;
     double *q64 = nullptr;
     int64_t *d_qoff = nullptr;
+    int64_t *d_qdoff = nullptr;      // q value offset per set (all 0 when the sets share one q vector)
     // per-call buffers
     char *d_args = nullptr;
     size_t args_cap = 0;
@@ -643,6 +667,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         pts_total += (double)std::max<int64_t>(num_eval, 1) * ((nq + sas::QTILE - 1) / sas::QTILE);
     }
     p.mean_pts = items > 0 ? pts_total / (double)items : 0.0;
+    p.single_chunk = !p.sep_nlp && std::all_of(nchunk.begin(), nchunk.end(), [](int32_t c) { return c == 1; });
     if (items > 0x7fffffff) return fail(SASHIP_EINVAL, "too many work items in one batch");
     p.n_items = (int)items;
     p.nq_total = b->qoff[ns];
@@ -748,7 +773,7 @@ int enqueue_sep(saship_batch *b, const Plan &p, const sas::ExpandArgs &ea, const
     sep::k_sep_expand<1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea, t);
     HIP_TRY(hipGetLastError());
     const char *d = b->d_args;
-    const SepArgs sa{b->q32, b->d_qoff, t, b->m->quad, (const int *)(d + p.off_order), (const int *)(d + p.off_ipre),
+    const SepArgs sa{b->q32, b->d_qoff, b->d_qdoff, t, b->m->quad, (const int *)(d + p.off_order), (const int *)(d + p.off_ipre),
                      b->nsets, p.nblk, p.nodes_per_item, d_poff};
     const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
     if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
@@ -781,24 +806,15 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     double *d_result = b->d_out;
     double *d_iq = b->d_out + p.result_total;
 
-    if (parts & PART_HEAD) {
-        if (copies) HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
-        if (b->timing) HIP_TRY(hipEventRecord(b->ev2, b->st));
-        ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
-                      b->d_recs, b->d_npts, b->d_sums, b->d_queue, b->d_spin};
-        if (p.sep_nlp) {
-            if (int e = enqueue_sep(b, p, ea, d_poff)) return e;
-        } else {
-        switch (p.kind) {
-        case 0: k_expand<M, 0><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
-        case 1: k_expand<M, 1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
-        default: k_expand<M, 2><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
-        }
-        HIP_TRY(hipGetLastError());
-
+    // every set one chunk and plain I(q) asked: the evaluation kernel writes
+    // I(q) itself (no partial sums, no final kernel); it then belongs to the
+    // tail, whose output address may change from call to call
+    const bool deval = p.single_chunk && !p.structure && want_iq && !want_result;
+    double *direct = copies ? b->iq_direct : nullptr;
+    auto launch_eval = [&](double *iq_out) -> int {
         const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
-        BatchArgs ba{b->q32, b->q64, b->d_qoff, d_values, d_voff, b->d_recs, d_roff, d_poff, b->d_npts, b->d_spin,
-                     d_items, d_nitems, b->pts_per_item, b->m->quad, queue_grab(b, p, grid * 4)};
+        BatchArgs ba{b->q32, b->q64, b->d_qoff, b->d_qdoff, d_values, d_voff, b->d_recs, d_roff, d_poff, b->d_npts, b->d_spin,
+                     d_items, d_nitems, b->pts_per_item, b->m->quad, queue_grab(b, p, grid * 4), iq_out, b->d_sums};
         if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
         if (p.kind == 0) {
             if (b->m->quad_n > 0) k_batch_eval_1d<M, true><<<grid, 256, 0, b->st>>>(ba, b->d_queue, b->d_partial);
@@ -810,20 +826,42 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
         }
         HIP_TRY(hipGetLastError());
         if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
+        return 0;
+    };
+
+    if (parts & PART_HEAD) {
+        if (copies) HIP_TRY(hipMemcpyAsync(b->d_args, b->h_stage, p.bytes, hipMemcpyHostToDevice, b->st));
+        if (b->timing) HIP_TRY(hipEventRecord(b->ev2, b->st));
+        ExpandArgs ea{d_det, d_values, d_voff, d_roff, p.ndet, 0, -1, cutoff, reff_mode,
+                      b->d_recs, b->d_npts, b->d_sums, b->d_queue, b->d_spin};
+        if (p.sep_nlp) {
+            if (int e = enqueue_sep(b, p, ea, d_poff)) return e;
+        } else {
+            switch (p.kind) {
+            case 0: k_expand<M, 0><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+            case 1: k_expand<M, 1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+            default: k_expand<M, 2><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea); break;
+            }
+            HIP_TRY(hipGetLastError());
+            if (!deval)
+                if (int e = launch_eval(nullptr)) return e;
         }
     }
     if (!(parts & PART_TAIL)) return 0;
+    double *d_siq = d_iq + p.nq_total;
+    if (deval) {
+        if (int e = launch_eval(direct && b->n_smear == 0 ? direct : d_iq)) return e;
+    } else {
     const double *d_spars = p.structure ? (const double *)(d + p.off_spars) : nullptr;
     if (p.structure) {
         k_structure_prep<<<(b->nsets + 63) / 64, 64, 0, b->st>>>(b->nsets, b->s_model, d_spars, b->d_sums, b->d_sprep);
         HIP_TRY(hipGetLastError());
     }
     const dim3 fgrid(b->nsets, std::max(1, (p.max_nq + 255) / 256));
-    double *direct = copies ? b->iq_direct : nullptr;
     double *f_result = want_result ? d_result : nullptr;
     double *f_iq = !want_iq ? nullptr : direct && b->n_smear == 0 ? direct : d_iq;
 #define SAS_FINAL(SM)                                                                                        \
-    k_batch_final<SM><<<fgrid, 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, d_values, d_voff, d_poff, d_resoff, \
+    k_batch_final<SM><<<fgrid, 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, b->d_qdoff, d_values, d_voff, d_poff, d_resoff, \
                                                 d_nchunk, b->d_sums, b->d_partial, d_spars, b->d_sprep, b->q64,  \
                                                 b->is2d, f_result, f_iq)
     switch (p.structure ? b->s_model : 0) {
@@ -836,7 +874,7 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
     }
 #undef SAS_FINAL
     HIP_TRY(hipGetLastError());
-    double *d_siq = d_iq + p.nq_total;
+    }
     if (want_iq && b->n_smear > 0) {
         const int64_t blocks = (b->n_smear + 255) / 256;
         const SmearRepeat rep{b->d_srep, b->d_srstep, b->d_sroff, b->d_srw};
@@ -942,7 +980,7 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
-                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq, gparts};
+                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq, gparts, p.single_chunk};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
@@ -1038,13 +1076,22 @@ int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const
     b->dev = m->dev;
     b->cus = cu_count(m->dev);
     b->qoff = qoff;
-    const size_t nqv = (size_t)total * (b->is2d ? 2 : 1);
+    // sets with the same q vector (a fit population on one data set) share
+    // one copy on the device: the kernels read it through qdoff
+    const size_t dims = b->is2d ? 2 : 1;
+    bool shared = nsets > 1 && nq[0] > 0;
+    for (int s = 1; s < nsets && shared; s++)
+        shared = nq[s] == nq[0] && memcmp(q + (size_t)qoff[s] * dims, q, sizeof(double) * (size_t)nq[0] * dims) == 0;
+    std::vector<int64_t> qdoff(nsets);
+    for (int s = 0; s < nsets; s++) qdoff[s] = shared ? 0 : qoff[s];
+    const size_t nqv = (shared ? (size_t)nq[0] : (size_t)total) * dims;
     std::vector<float> q32(std::max<size_t>(nqv, 1));
     for (size_t i = 0; i < nqv; i++) q32[i] = (float)q[i];
     hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&b->q32, sizeof(float) * std::max<size_t>(nqv, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&b->q64, sizeof(double) * std::max<size_t>(nqv, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_qoff, sizeof(int64_t) * (nsets + 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_qdoff, sizeof(int64_t) * nsets);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_npts, sizeof(int) * nsets);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_sums, sizeof(double) * 4 * nsets);
     if (e == hipSuccess && model_magnetic(m->id))
@@ -1055,6 +1102,7 @@ int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const
     if (e == hipSuccess && nqv) e = hipMemcpy(b->q32, q32.data(), sizeof(float) * nqv, hipMemcpyHostToDevice);
     if (e == hipSuccess && nqv) e = hipMemcpy(b->q64, q, sizeof(double) * nqv, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->d_qoff, qoff.data(), sizeof(int64_t) * (nsets + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_qdoff, qdoff.data(), sizeof(int64_t) * nsets, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipEventCreate(&b->ev2);
@@ -1379,7 +1427,7 @@ int saship_batch_release(saship_batch *b)
     (void)hipSetDevice(b->dev);
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
-    void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_args, b->d_recs, b->d_npts, b->d_sums, b->d_spin, b->d_sprep,
+    void *dev_ptrs[] = {b->q32, b->q64, b->d_qoff, b->d_qdoff, b->d_args, b->d_recs, b->d_npts, b->d_sums, b->d_spin, b->d_sprep,
                         b->d_queue, b->d_partial, b->d_out, b->d_sep, b->d_sfrom, b->d_soff, b->d_sstep, b->d_slen, b->d_sw,
                         b->d_srep, b->d_srstep, b->d_sroff, b->d_srw};
     for (void *ptr : dev_ptrs)

@@ -323,3 +323,59 @@ def test_pinned_output_matches_host_buffer(case):
         del again
     assert pinned._held > 0
     batch.release()
+
+
+@pytest.mark.parametrize("case", ["sphere", "fp64_model", "magnetic", "smeared", "multi_chunk"])
+def test_single_chunk_direct_iq(case):
+    """Sets whose mesh fits one work item: the evaluation kernel writes I(q)
+    itself (no final kernel).  Equal to the bit to the final kernel's I(q)
+    (forced by asking for the raw sums too) and within the GPU tolerance of
+    the oracle, eager, from the graph, and replayed."""
+    rng = np.random.default_rng(43)
+    mag = False
+    if case == "magnetic":
+        model = hip_model("core_shell_sphere")
+        x = np.linspace(-0.3, 0.3, 16)
+        qx, qy = np.meshgrid(x, x)
+        qv = [qx.ravel(), qy.ravel()]
+        kl = model.make_kernel(qv)
+        pars = [dict(radius=rng.uniform(20, 80), thickness=rng.uniform(5, 20), sld_core_M0=rng.uniform(0, 3),
+                     sld_shell_M0=rng.uniform(0, 3), sld_solvent_M0=rng.uniform(0, 3), up_frac_i=rng.uniform(0, 1),
+                     up_frac_f=rng.uniform(0, 1), up_theta=rng.uniform(0, 90)) for _ in range(6)]
+        calls = [_pack(model, kl, p, "2d")[:2] for p in pars]
+        kl.release()
+        q_list = [qv]*len(calls)
+        mag = True
+    else:
+        name = "mono_gauss_coil" if case == "fp64_model" else "sphere"
+        model = hip_model(name)
+        q = np.logspace(-3, np.log10(0.5), 150)
+        kl = type("K", (), {"info": model.info, "dtype": np.dtype("d")})()
+        if name == "mono_gauss_coil":
+            pars = [dict(rg=rng.uniform(10, 80), rg_pd=0.1, rg_pd_n=9) for _ in range(8)]
+        else:
+            pars = [dict(radius=rng.uniform(20, 80), radius_pd=0.1, radius_pd_n=int(rng.integers(1, 30)))
+                    for _ in range(8)]
+        calls = [_pack(model, kl, p, "1d")[:2] for p in pars]
+        q_list = [[q]]*len(calls)
+    batch = HipBatch(model, q_list, use_graph=True)
+    batch.set_options(pts_per_item=4 if case == "multi_chunk" else 64)
+    if case == "smeared":
+        from sasmodels_amd import resolution as R
+        qm = q_list[0][0][5:-5]
+        batch.set_resolution([R.Pinhole1D(qm, 0.05*qm, q_calc=q_list[0][0])]*len(calls))
+    for rep in range(3):
+        print("direct iq", case, "rep", rep, flush=True)
+        got = np.concatenate(batch(calls, magnetic=mag))
+        _, want = batch.raw(calls, magnetic=mag, want_result=True, want_iq=True)
+        np.testing.assert_array_equal(got, want)
+    batch.set_options(use_graph=False)
+    np.testing.assert_array_equal(np.concatenate(batch(calls, magnetic=mag)), want)
+    batch.replay(2)
+    np.testing.assert_array_equal(np.concatenate(batch(calls, magnetic=mag)), want)
+    if case != "smeared":
+        out = batch(calls, magnetic=mag)
+        for (qv, c, o) in zip(q_list, calls, out):
+            ref = OracleKernel(model.info, qv)(c[0], c[1], 0.0, mag)
+            assert masked_relerr(o, ref) <= RTOL
+    batch.release()
