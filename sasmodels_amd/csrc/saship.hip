@@ -488,7 +488,8 @@ int saship_host_alloc(int64_t bytes, void **ptr)
     if (bytes < 1) return fail(SASHIP_EINVAL, "bytes must be >= 1");
     HIP_TRY(hipSetDevice(current_device()));
     void *p = nullptr;
-    const hipError_t e = hipHostMalloc(&p, (size_t)bytes, hipHostMallocMapped | hipHostMallocCoherent);
+    const hipError_t e =
+        hipHostMalloc(&p, (size_t)bytes, hipHostMallocMapped | hipHostMallocCoherent | hipHostMallocPortable);
     if (e != hipSuccess || !p) return fail(SASHIP_ENOMEM, std::string("pinned host memory: ") + hipGetErrorString(e));
     *ptr = p;
     return 0;

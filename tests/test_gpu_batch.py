@@ -379,3 +379,26 @@ def test_single_chunk_direct_iq(case):
             ref = OracleKernel(model.info, qv)(c[0], c[1], 0.0, mag)
             assert masked_relerr(o, ref) <= RTOL
     batch.release()
+
+
+def test_call_packed_checks_its_inputs():
+    """Packed inputs that do not match the batch are refused before any device work."""
+    model = hip_model("sphere")
+    q = np.logspace(-3, 0, 40)
+    kl = type("K", (), {"info": model.info, "dtype": np.dtype("d")})()
+    calls = [_pack(model, kl, dict(radius=20 + 5*i, radius_pd=0.1, radius_pd_n=5), "1d")[:2] for i in range(3)]
+    batch = HipBatch(model, [[q]]*3)
+    packed = batch._pack(calls)
+    want = batch(calls)
+    for a, b in zip(batch.call_packed(packed), want):
+        np.testing.assert_array_equal(a, b)
+    det, vals, voff, ndet = packed
+    with pytest.raises(ValueError):
+        batch.call_packed((det.astype(np.int64), vals, voff, ndet))
+    with pytest.raises(ValueError):
+        batch.call_packed((det, vals[:-1], voff, ndet))
+    with pytest.raises(ValueError):
+        batch.call_packed((det[:-1], vals, voff, ndet))
+    with pytest.raises(ValueError):
+        batch.call_packed((det, vals, voff[:-1], ndet))
+    batch.release()

@@ -44,6 +44,25 @@ def test_separable_matches_oracle_and_generic(nr, nl):
         assert not np.array_equal(sep[:2*Q.size], gen[:2*Q.size])
 
 
+@pytest.mark.parametrize("nr,nl,pd_type", [(35, 35, "lognormal"), (1, 40, "gaussian"), (64, 9, "schulz"),
+                                          (21, 1, "uniform")])
+def test_separable_whole_mesh_sums(nr, nl, pd_type):
+    """A whole unpruned mesh without an effective radius: the expansion sums
+    the weights as products of the two grids' sums instead of walking the
+    mesh.  Raw sums and accumulators against the oracle, and the weight sums
+    equal (to FP64 rounding) to the walk's, which an effective-radius mode
+    forces on the same buffers."""
+    print("whole mesh", nr, nl, pd_type, flush=True)
+    hk, ok, det, val, mag = buffers("cylinder", [Q], _pars(nr, nl, pd_type=pd_type), 76)
+    hk._call_kernel(det, val, 0.0, mag, 0)
+    whole = hk.result.copy()
+    compare_raw(whole, ok.raw(det, val, 0.0, mag, 0), Q.size, 2)
+    hk._call_kernel(det, val, 0.0, mag, 3)
+    walk = hk.result.copy()
+    np.testing.assert_allclose(whole[2*Q.size:2*Q.size + 3], walk[2*Q.size:2*Q.size + 3], rtol=1e-14, atol=0)
+    np.testing.assert_array_equal(whole[:2*Q.size], walk[:2*Q.size])
+
+
 @pytest.mark.parametrize("quad", [20, 500])
 def test_separable_other_rules(quad):
     print("separable quad", quad, flush=True)
