@@ -308,6 +308,118 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
     }
 }
 
+// Separable cylinder sets under the adaptive rule (20/76/500/5000 nodes by
+// q * max(radius, length/2), adaptive.c:11276-11298): items are (set, 64-q
+// tile, rule, block of that rule's nodes), numbered tile by tile; titem[p]
+// = (set, tile, k_lo | k_hi << 8, first item) lists the tiles in item order
+// with the rules their q range can need.  Each lane counts, in FP64 as the
+// reference chooses its rule, the grid prefixes below the rule's threshold
+// and the previous one; a wave whose lanes all see the whole mesh under
+// this rule (or none of it) takes the rotation-mode sums, else the prefix
+// sums.  One partial per (item, output, lane): partial[item*128 + o*64 + lane].
+struct SepAdaptArgs {
+    const float *q32;
+    const double *q64;
+    const int64_t *qoff;
+    const int64_t *qdoff;
+    sep::Tables t;
+    Quad quad;
+    const int4 *titem;
+    int n_titem;
+    int n_items;
+    int nodes_per_item;
+};
+
+__device__ __forceinline__ int adaptive_n(int k) { return k == 0 ? 20 : k == 1 ? 76 : k == 2 ? 500 : 5000; }
+
+__global__ __launch_bounds__(256) void k_batch_sep_adapt(SepAdaptArgs a, int *__restrict__ queue,
+                                                         double *__restrict__ partial)
+{
+    __shared__ double red[2][4][64];
+    __shared__ int next_item;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int4 *titem = a.titem;
+    QueueCursor cur{(int)(blockIdx.x % WORK_QUEUES), 0};
+    int pos = 0;
+    for (;;) {
+        if (wave == 0) {
+            const int it = queue_next(queue, a.n_items, cur);
+            if (lane == 0) next_item = it;
+        }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(next_item);
+        if (item < 0) break;
+        // the tile entry: last p with titem[p].w <= item
+        pos = __builtin_amdgcn_readfirstlane(pos);
+        if (!(titem[pos].w <= item && (pos + 1 == a.n_titem || item < titem[pos + 1].w))) {
+            int lo = 0, hi = a.n_titem;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (titem[mid].w <= item) lo = mid;
+                else hi = mid;
+            }
+            pos = __builtin_amdgcn_readfirstlane(lo);
+        }
+        const int4 te = titem[pos];
+        const int s = te.x, tile = te.y, klo = te.z & 255, khi = te.z >> 8;
+        const int P = a.nodes_per_item;
+        int rem = item - te.w, k = klo;
+        for (; k < khi; k++) {
+            const int nb = (adaptive_n(k) + P - 1) / P;
+            if (rem < nb) break;
+            rem -= nb;
+        }
+        const int64_t q0 = a.qoff[s], qd0 = a.qdoff[s];
+        const int nq = (int)(a.qoff[s + 1] - q0);
+        const int iq = tile * QTILE + lane;
+        const bool ok = iq < nq;
+        const float q = ok ? a.q32[qd0 + iq] : 0.0f;
+        const double qd = ok ? a.q64[qd0 + iq] : 0.0;
+        const SAS_CONST int *dims = as_const(a.t.dims) + 4 * s;
+        const int NR = dims[0], NL = dims[1];
+        const double *G = a.t.G64 + (size_t)s * 2 * sep::NRMAX;
+        // prefixes of the grids below this rule's threshold (ra, rb) and
+        // below the previous rule's (pa, pb)
+        int ra = NR, rb = NL, pa = 0, pb = 0;
+        if (k < 3) {
+            const double T = k == 0 ? 10.0 : k == 1 ? 100.0 : 750.0;
+            ra = sep::rule_count(G, NR, qd, 1.0, T);
+            rb = sep::rule_count(G + sep::NRMAX, NL, qd, 0.5, T);
+        }
+        if (k > 0) {
+            const double T = k == 1 ? 10.0 : k == 2 ? 100.0 : 750.0;
+            pa = sep::rule_count(G, NR, qd, 1.0, T);
+            pb = sep::rule_count(G + sep::NRMAX, NL, qd, 0.5, T);
+        }
+        const bool none = !ok || (ra == pa && rb == pb);
+        const bool whole = ra == NR && rb == NL && pa == 0 && pb == 0;
+        const bool fast = sep::wave_all(whole || none);
+        const int off = a.quad.off[k], n0 = rem * P, n1 = min(adaptive_n(k), n0 + P);
+        KSum a1, a2;
+        for (int node = n0 + wave; node < n1; node += 4) {
+            float f1, f2;
+            const float sn = a.quad.a[off + node], cs = a.quad.b[off + node];
+            if (fast) sep::node_sums(a.t, s, q, sn, cs, f1, f2);
+            else sep::prefix_sums(a.t, s, q, sn, cs, ra, pa, rb, pb, f1, f2);
+            if (none) f1 = f2 = 0.0f;
+            const float w = a.quad.w[off + node];
+            a1.add(w * f1);
+            a2.add(w * f2);
+        }
+        red[0][wave][lane] = a2.value();
+        red[1][wave][lane] = a1.value();
+        __syncthreads();
+        if (wave < 2) {
+            // the tables of a mesh that is not whole on ascending grids cannot
+            // be split by rule: NaN rather than a wrong sum
+            double v = ((red[wave][0][lane] + red[wave][1][lane]) + red[wave][2][lane]) + red[wave][3][lane];
+            if (dims[2] != 2) v = __builtin_nan("");
+            partial[(size_t)item * 2 * QTILE + wave * QTILE + lane] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // P@S (sasmodels/product.py:425-528): S's parameters for set s, at the
 // effective radius of the form (mode > 0) or the given one and at
 // volfraction * form:shell volume ratio, from the staged structure
@@ -348,6 +460,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
     const double *__restrict__ values,
     const int64_t *__restrict__ voff, const int64_t *__restrict__ poff,
     const int64_t *__restrict__ resoff, const int *__restrict__ nchunk,
+    const int2 *__restrict__ tinfo, const int64_t *__restrict__ toff,
     const double *__restrict__ sums, const double *__restrict__ partial,
     const double *__restrict__ spars, const double *__restrict__ sprep,
     const double *__restrict__ q64, int is2d, double *__restrict__ result, double *__restrict__ iq_out)
@@ -378,6 +491,15 @@ __global__ __launch_bounds__(256) void k_batch_final(
         // chunk order is kept; the loads of 8 chunks are issued before their
         // adds so that their latencies overlap
         double f2 = 0.0, f1 = 0.0;
+        if (tinfo) {
+            // adaptive separable items: the tile's items in order, 2 x 64 partials each
+            const int2 ti = tinfo[toff[s] + e / QTILE];
+            const double *pt = partial + (size_t)ti.x * 2 * QTILE + e % QTILE;
+            for (int c = 0; c < ti.y; c++) {
+                f2 += pt[(size_t)c * 2 * QTILE];
+                f1 += pt[(size_t)c * 2 * QTILE + QTILE];
+            }
+        } else {
         const size_t cstride = (size_t)nout * nq;
         int c = 0;
         for (; c + 8 <= nc; c += 8) {
@@ -396,6 +518,7 @@ __global__ __launch_bounds__(256) void k_batch_final(
         for (; c < nc; c++) {
             f2 += part[(size_t)c * cstride + e];
             if (nout == 2) f1 += part[(size_t)c * cstride + nq + e];
+        }
         }
         if (result) {
             result[resoff[s] + (size_t)e * nres] = f2;
@@ -478,6 +601,9 @@ struct Plan {
     int max_nq = 0;                   // largest q count of a set
     double mean_pts = 0.0;            // generic path: mesh points per work item (upper bound)
     bool single_chunk = false;        // generic path: every set's mesh fits one work item
+    bool sep_adapt = false;           // separable path under the adaptive rule (tiled partials)
+    int n_titem = 0;                  // adaptive: tile entries (set, tile, rules, first item)
+    size_t off_tinfo = 0, off_toff = 0;
 };
 
 }  // namespace batch_host
@@ -487,6 +613,10 @@ struct saship_batch {
     int nsets = 0, is2d = 0, dev = 0, cus = 256;
     hipStream_t st = nullptr;
     std::vector<int64_t> qoff;
+    // per set, its first 64-q tile in tile_qmin/tile_qmax: the smallest and
+    // largest q of each tile (the adaptive rule's range over a tile)
+    std::vector<int64_t> tile_off;
+    std::vector<double> tile_qmin, tile_qmax;
     float *q32 = nullptr;
     double *q64 = nullptr;
     int64_t *d_qoff = nullptr;
@@ -620,7 +750,7 @@ int record_width(int id, int kind)
 
 // Validate every set, lay out the staging buffer and fill it (host pinned).
 int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_details,
-                   const double *values, const int64_t *voff_in, Plan &p)
+                   const double *values, const int64_t *voff_in, double cutoff, Plan &p)
 {
     const int id = b->m->id, ns = b->nsets;
     p.kind = kind;
@@ -644,8 +774,33 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         }
         p.sep_nlp = nlp;
     }
-    const int nnode = b->m->quad.n;
-    if (p.sep_nlp) p.nodes_per_item = std::min(P, nnode);
+    // the adaptive rule: separable when every set's mesh is whole on ascending,
+    // equally spaced grids (adaptive_separable); ext: per set the smallest and
+    // largest radius and half length
+    std::vector<double> ext;
+    if (id == 1 && kind == 0 && b->m->quad_n == 0 && b->separable && n_details == model_maxpd(1) * 4 + 4) {
+        int nlp = 0;
+        ext.resize(4 * (size_t)ns);
+        for (int s = 0; s < ns; s++) {
+            const int32_t *det = details + (size_t)s * n_details;
+            const int64_t nv = voff_in[s + 1] - voff_in[s];
+            const int here = cylinder_separable(det);
+            if (!here || nv < 0 || validate_set(id, kind, det, n_details, nv, 0, -1, s) != 0 ||
+                !adaptive_separable(det, values + voff_in[s], cutoff, &ext[4 * (size_t)s])) {
+                nlp = 0;
+                last_error().clear();
+                break;
+            }
+            nlp = std::max(nlp, here);
+        }
+        if (nlp) {
+            p.sep_nlp = nlp;
+            p.sep_adapt = true;
+        }
+    }
+    const int nnode = p.sep_adapt ? 0 : b->m->quad.n;
+    if (p.sep_adapt) p.nodes_per_item = P;
+    else if (p.sep_nlp) p.nodes_per_item = std::min(P, nnode);
     const int nblk = p.sep_nlp ? (nnode + p.nodes_per_item - 1) / p.nodes_per_item : 0;
     p.nblk = nblk;
     for (int s = 0; s < ns; s++) {
@@ -666,6 +821,35 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         items += (int64_t)nchunk[s] * ((nq + sas::QTILE - 1) / sas::QTILE);
         pts_total += (double)std::max<int64_t>(num_eval, 1) * ((nq + sas::QTILE - 1) / sas::QTILE);
     }
+    std::vector<int4> titem;
+    std::vector<int2> tinfo;
+    if (p.sep_adapt) {
+        // tiles in set cost order; per tile the rules its q range can need
+        // and their node blocks, numbered tile by tile
+        std::vector<int> ord(ns);
+        for (int s = 0; s < ns; s++) ord[s] = s;
+        auto cost = [&](int s) { return details[(size_t)s * n_details + n_details - 4]; };
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost(x) > cost(y); });
+        tinfo.assign((size_t)b->tile_off[ns], make_int2(0, 0));
+        int64_t n = 0;
+        for (int s : ord) {
+            const double *e = &ext[4 * (size_t)s];
+            for (int64_t t = b->tile_off[s]; t < b->tile_off[s + 1]; t++) {
+                const int klo = adaptive_rule(b->tile_qmin[t] * std::max(e[0], e[2]));
+                const int khi = adaptive_rule(b->tile_qmax[t] * std::max(e[1], e[3]));
+                int count = 0;
+                for (int k = klo; k <= khi; k++) count += (adaptive_nodes(k) + P - 1) / P;
+                titem.push_back(make_int4(s, (int)(t - b->tile_off[s]), klo | (khi << 8), (int)n));
+                tinfo[t] = make_int2((int)n, count);
+                n += count;
+                if (n > 0x7fffffff / (2 * sas::QTILE)) return fail(SASHIP_EINVAL, "too many work items in one batch");
+            }
+        }
+        items = n;
+        p.partial_total = n * 2 * sas::QTILE;
+        p.n_titem = (int)titem.size();
+        for (int s = 0; s < ns; s++) poff[s] = 0;
+    }
     p.mean_pts = items > 0 ? pts_total / (double)items : 0.0;
     p.single_chunk = !p.sep_nlp && std::all_of(nchunk.begin(), nchunk.end(), [](int32_t c) { return c == 1; });
     if (items > 0x7fffffff) return fail(SASHIP_EINVAL, "too many work items in one batch");
@@ -679,7 +863,12 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     p.off_poff = off; off = align256(off + sizeof(int64_t) * ns);
     p.off_resoff = off; off = align256(off + sizeof(int64_t) * ns);
     p.off_nchunk = off; off = align256(off + sizeof(int32_t) * ns);
-    p.off_items = off; off = align256(off + sizeof(int4) * (size_t)(p.sep_nlp ? 1 : std::max(1, p.n_items)));
+    p.off_items = off;
+    off = align256(off + sizeof(int4) * (size_t)(p.sep_adapt ? std::max(1, p.n_titem) : p.sep_nlp ? 1 : std::max(1, p.n_items)));
+    if (p.sep_adapt) {
+        p.off_tinfo = off; off = align256(off + sizeof(int2) * (size_t)std::max<int64_t>(1, b->tile_off[ns]));
+        p.off_toff = off; off = align256(off + sizeof(int64_t) * ns);
+    }
     p.off_nitems = off; off = align256(off + sizeof(int32_t));
     p.off_order = off; off = align256(off + sizeof(int32_t) * ns);
     p.off_ipre = off; off = align256(off + sizeof(int32_t) * (ns + 1));
@@ -702,6 +891,11 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     memcpy(h + p.off_resoff, resoff.data(), sizeof(int64_t) * ns);
     memcpy(h + p.off_nchunk, nchunk.data(), sizeof(int32_t) * ns);
     if (p.structure) memcpy(h + p.off_spars, b->spars.data(), sizeof(double) * SASHIP_SPARS * ns);
+    if (p.sep_adapt) {
+        memcpy(h + p.off_items, titem.data(), sizeof(int4) * titem.size());
+        memcpy(h + p.off_tinfo, tinfo.data(), sizeof(int2) * tinfo.size());
+        memcpy(h + p.off_toff, b->tile_off.data(), sizeof(int64_t) * ns);
+    }
     // work list, longest items first: items are bucketed by the number of
     // mesh points their chunk can hold (an upper bound before pruning)
     int4 *it = (int4 *)(h + p.off_items);
@@ -773,6 +967,16 @@ int enqueue_sep(saship_batch *b, const Plan &p, const sas::ExpandArgs &ea, const
     sep::k_sep_expand<1><<<b->nsets, EXPAND_THREADS, 0, b->st>>>(ea, t);
     HIP_TRY(hipGetLastError());
     const char *d = b->d_args;
+    if (p.sep_adapt) {
+        const SepAdaptArgs sa{b->q32, b->q64, b->d_qoff, b->d_qdoff, t, b->m->quad, (const int4 *)(d + p.off_items),
+                              p.n_titem, p.n_items, p.nodes_per_item};
+        const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
+        if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
+        k_batch_sep_adapt<<<grid, 256, 0, b->st>>>(sa, b->d_queue, b->d_partial);
+        HIP_TRY(hipGetLastError());
+        if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
+        return 0;
+    }
     const SepArgs sa{b->q32, b->d_qoff, b->d_qdoff, t, b->m->quad, (const int *)(d + p.off_order), (const int *)(d + p.off_ipre),
                      b->nsets, p.nblk, p.nodes_per_item, d_poff};
     const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
@@ -858,12 +1062,14 @@ int enqueue(saship_batch *b, const Plan &p, double cutoff, int reff_mode, bool w
         HIP_TRY(hipGetLastError());
     }
     const dim3 fgrid(b->nsets, std::max(1, (p.max_nq + 255) / 256));
+    const int2 *d_tinfo = p.sep_adapt ? (const int2 *)(d + p.off_tinfo) : nullptr;
+    const int64_t *d_toff = p.sep_adapt ? (const int64_t *)(d + p.off_toff) : nullptr;
     double *f_result = want_result ? d_result : nullptr;
     double *f_iq = !want_iq ? nullptr : direct && b->n_smear == 0 ? direct : d_iq;
 #define SAS_FINAL(SM)                                                                                        \
     k_batch_final<SM><<<fgrid, 256, 0, b->st>>>(p.nout, p.nres, b->d_qoff, b->d_qdoff, d_values, d_voff, d_poff, d_resoff, \
-                                                d_nchunk, b->d_sums, b->d_partial, d_spars, b->d_sprep, b->q64,  \
-                                                b->is2d, f_result, f_iq)
+                                                d_nchunk, d_tinfo, d_toff, b->d_sums, b->d_partial, d_spars,      \
+                                                b->d_sprep, b->q64, b->is2d, f_result, f_iq)
     switch (p.structure ? b->s_model : 0) {
     case 0: SAS_FINAL(0); break;
     case S_HARDSPHERE: SAS_FINAL(S_HARDSPHERE); break;
@@ -966,7 +1172,7 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
 {
     Plan p;
     b->have_last = false;
-    if (int e = plan_and_stage(b, kind, details, n_details, values, voff, p)) return e;
+    if (int e = plan_and_stage(b, kind, details, n_details, values, voff, cutoff, p)) return e;
     if (int e = size_buffers(b, p)) return e;
     const bool want_result = result != nullptr, want_iq = iq != nullptr;
     const size_t iq_bytes = want_iq ? sizeof(double) * (size_t)(b->n_smear > 0 ? b->n_smear : p.nq_total) : 0;
@@ -980,7 +1186,7 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
-                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq, gparts, p.single_chunk};
+                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq, gparts, p.single_chunk, p.sep_adapt, p.n_titem};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
@@ -1076,6 +1282,22 @@ int saship_batch_create(saship_model *m, int32_t nsets, const int32_t *nq, const
     b->dev = m->dev;
     b->cus = cu_count(m->dev);
     b->qoff = qoff;
+    b->tile_off.assign(nsets + 1, 0);
+    for (int s = 0; s < nsets; s++) {
+        const int nt = (nq[s] + sas::QTILE - 1) / sas::QTILE;
+        b->tile_off[s + 1] = b->tile_off[s] + nt;
+        for (int t = 0; t < nt; t++) {
+            double lo = HUGE_VAL, hi = 0.0;
+            for (int i = t * sas::QTILE; i < std::min(nq[s], (t + 1) * sas::QTILE); i++) {
+                const size_t g = (size_t)qoff[s] + i;
+                const double qv = is2d ? std::hypot(q[2 * g], q[2 * g + 1]) : std::fabs(q[g]);
+                lo = std::min(lo, qv);
+                hi = std::max(hi, qv);
+            }
+            b->tile_qmin.push_back(lo);
+            b->tile_qmax.push_back(hi);
+        }
+    }
     // sets with the same q vector (a fit population on one data set) share
     // one copy on the device: the kernels read it through qdoff
     const size_t dims = b->is2d ? 2 : 1;

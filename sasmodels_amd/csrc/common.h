@@ -154,6 +154,56 @@ inline int cylinder_separable(const int32_t *details)
 // "set s: " for messages about set s of a batch ("" for s < 0)
 inline std::string set_prefix(int set) { return set < 0 ? std::string() : "set " + std::to_string(set) + ": "; }
 
+// A separable cylinder set whose mesh the adaptive rule can split by
+// rectangles: every point valid (radius, length >= 0) and accepted at this
+// cutoff (weights >= 1e-150, so products stay positive; for cutoff > 0 the
+// product of the smallest weights clears it by a margin), both grids
+// ascending and equally spaced (the rotation mode of the expansion, which
+// orders its tables by grid).  Out: the smallest and largest radius and
+// half length.  Call after validate_set.
+inline bool adaptive_separable(const int32_t *details, const double *values, double cutoff, double ext[4])
+{
+    const int mp = model_maxpd(1), nv = model_nvalues(1);
+    const int32_t *par = details, *len = details + mp, *off = details + 2 * mp;
+    const int num_weights = details[4 * mp + 1];
+    const double *pv = values + nv, *pw = pv + num_weights;
+    constexpr double WMIN = 1e-150;
+    double wprod = 1.0;
+    int kr = -1, kl = -1;
+    for (int k = 0; k < mp; k++) {
+        if (par[k] == 2) kr = k;
+        else if (par[k] == 3) kl = k;
+        if (par[k] == 2 || par[k] == 3) {
+            double wmin = HUGE_VAL;
+            for (int i = 0; i < len[k]; i++) wmin = std::min(wmin, pw[off[k] + i]);
+            if (!(wmin >= WMIN)) return false;
+            wprod *= wmin;
+        } else {
+            if (!(pw[off[k]] >= WMIN)) return false;
+            wprod *= pw[off[k]];
+        }
+    }
+    if (kr < 0 || kl < 0) return false;
+    if (cutoff > 0.0 && !(wprod > cutoff * (1.0 + 1e-9))) return false;
+    for (int g = 0; g < 2; g++) {
+        const int k = g ? kl : kr, n = len[k];
+        const double *v = pv + off[k];
+        const double d = n > 1 ? (v[n - 1] - v[0]) / (n - 1) : 0.0;
+        for (int i = 0; i < n; i++) {
+            if (!(v[i] >= 0.0)) return false;
+            if (i > 0 && !(v[i] > v[i - 1])) return false;
+            if (!(std::fabs(v[i] - (v[0] + i * d)) <= 1e-9 * std::fabs(v[n - 1]))) return false;
+        }
+        ext[2 * g] = g ? 0.5 * v[0] : v[0];
+        ext[2 * g + 1] = g ? 0.5 * v[n - 1] : v[n - 1];
+    }
+    return true;
+}
+
+// The adaptive rule's size index for q * r (adaptive.c:11290-11297, FP64).
+inline int adaptive_rule(double qr) { return qr < 10.0 ? 0 : qr < 100.0 ? 1 : qr < 750.0 ? 2 : 3; }
+inline int adaptive_nodes(int k) { return k == 0 ? 20 : k == 1 ? 76 : k == 2 ? 500 : 5000; }
+
 // Checks one set's details/values against the model before any device work:
 // every index the kernels derive from them stays inside the buffers (set:
 // the set's index in a batch, -1 for a single call; only named in messages).

@@ -716,14 +716,16 @@ struct Tables {
     float *AT;       // bilinear mode [nlp][NRMAX]: w * 1e-2 * drho * V * pi/4 at (row, column)
     float *BT;       // bilinear mode [nlp][NRMAX]: w * 1e-4 * (drho * V)^2 * pi/4
     int *dims;       // NR, NL, mode (3 interval, 2 rotation, 1 prefix, 0 bilinear), number of row groups
+    double *G64;     // [2][NRMAX] per set: the radius and the length grid in FP64, mesh order
     int nlp;         // columns per set in AT/BT (>= every set's NL)
 };
 
 inline size_t tables_bytes(int nsets, int nlp)
 {
     return (size_t)nsets * (SER_STRIDE * sizeof(double) +
-                            NRMAX * (4 * sizeof(float4) + sizeof(int2) + 2 * sizeof(float) * (size_t)nlp) +
-                            4 * sizeof(int)) + 256;
+                            NRMAX * (4 * sizeof(float4) + sizeof(int2) + 2 * sizeof(float) * (size_t)nlp +
+                                     2 * sizeof(double)) +
+                            4 * sizeof(int)) + 512;
 }
 
 // Carve the tables of nsets sets out of one buffer of tables_bytes(nsets, nlp).
@@ -740,6 +742,8 @@ inline Tables carve(void *base, int nsets, int nlp)
     t.AT = (float *)(t.GRP + n);
     t.BT = t.AT + n * nlp;
     t.dims = (int *)(t.BT + n * nlp);
+    // FP64 grids after the dims, on an 8-byte boundary
+    t.G64 = (double *)(((uintptr_t)(t.dims + 4 * (size_t)nsets) + 7) & ~(uintptr_t)7);
     t.nlp = nlp;
     return t;
 }
@@ -820,6 +824,8 @@ __device__ __forceinline__ void sep_expand(const ExpandArgs &a, const Tables &t)
         grid[3][tid - NRMAX] = pd_weight[pd_off[kl] + tid - NRMAX];
     }
     const double *rv = grid[0], *rw = grid[1], *lv = grid[2], *lw = grid[3];
+    if (tid < NR) t.G64[(size_t)set * 2 * NRMAX + tid] = pd_value[pd_off[kr] + tid];
+    else if (tid >= NRMAX && tid - NRMAX < NL) t.G64[(size_t)set * 2 * NRMAX + tid] = pd_value[pd_off[kl] + tid - NRMAX];
     __shared__ unsigned char acc[NRMAX * NRMAX];
     __shared__ int rowk[NRMAX], roworder[NRMAX], colorder[NRMAX], bad, nzero, ngroups, mode;
     __shared__ float steps[2];
@@ -1674,6 +1680,68 @@ __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, flo
     }
     f1 = F1;
     f2 = F2;
+}
+
+// Adaptive rule (rotation-mode tables, grids ascending): the mesh points
+// evaluated with rule k at q form the rectangle of the first a radius and
+// b length points (q R < T_k and q L/2 < T_k) less the rectangle of the
+// first ap, bp (the smaller rules), so at a node of rule k
+//   F = U1(a) T1(b) - U1(ap) T1(bp),  F^2 = U2(a) T2(b) - U2(ap) T2(bp)
+// with the prefix sums U(a) = sum_{i<a} u_i J_i and T(b) = sum_{j<b} v_j S_j,
+// here point by point (ap <= a, bp <= b per lane).
+__device__ __forceinline__ void prefix_sums(const Tables &t, int set, float q, float sn, float cs, int a, int ap,
+                                            int b, int bp, float &f1, float &f2)
+{
+    const SAS_CONST int *dims = as_const(t.dims) + 4 * set;
+    const int NR = dims[0], NL = dims[1];
+    const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
+    const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
+    const float qs = q * sn, qc = q * cs;
+    float U1 = 0.0f, U2 = 0.0f, U1p = 0.0f, U2p = 0.0f;
+    for (int i = 0; i < NR; i++) {
+        if (i == ap) {
+            U1p = U1;
+            U2p = U2;
+        }
+        const float4 e = ld4(ROW + i);
+        const float J = j1c_f(qs * e.x);
+        if (i < a) {
+            U1 = fmaf(e.y, J, U1);
+            U2 = fmaf(e.z, J * J, U2);
+        }
+    }
+    if (ap >= NR) {
+        U1p = U1;
+        U2p = U2;
+    }
+    float T1 = 0.0f, T2 = 0.0f, T1p = 0.0f, T2p = 0.0f;
+    for (int j = 0; j < NL; j++) {
+        if (j == bp) {
+            T1p = T1;
+            T2p = T2;
+        }
+        const float4 e = ld4(COL + j);
+        const float S = sinc_prod(qc, e.x);
+        if (j < b) {
+            T1 = fmaf(e.y, S, T1);
+            T2 = fmaf(e.z, S * S, T2);
+        }
+    }
+    if (bp >= NL) {
+        T1p = T1;
+        T2p = T2;
+    }
+    f1 = fmaf(U1, T1, -U1p * T1p);
+    f2 = fmaf(U2, T2, -U2p * T2p);
+}
+
+// Number of grid values g[0..n) (ascending) with q * s * g < T, in FP64 as
+// the reference's rule choice (q * fmax(radius, 0.5 * length) < T).
+__device__ __forceinline__ int rule_count(const double *g, int n, double q, double scale, double T)
+{
+    int c = 0;
+    for (int i = 0; i < n; i++) c += q * (scale * g[i]) < T ? 1 : 0;
+    return c;
 }
 
 // Single set: one wave per (64-q tile, Gauss node), four nodes per

@@ -101,6 +101,10 @@ struct saship_kernel {
     double *d_partial = nullptr;
     size_t partial_cap = 0;
     char *d_sep = nullptr;          // separable cylinder tables (R, H, A, B, dims)
+    // cylinder 1D under the adaptive rule: whole separable meshes are
+    // evaluated by a one-set batch (its rule-rectangle separable path)
+    std::vector<double> q_host;
+    saship_batch *adapt = nullptr;
     char *h_stage = nullptr;
     size_t stage_cap = 0;
     double *h_result = nullptr;      // pinned, mapped: the final kernel writes the result here
@@ -683,6 +687,7 @@ int saship_kernel_create(saship_model *m, int32_t nq, const double *q, int32_t i
     const size_t nqv = (size_t)nq * (k->is2d ? 2 : 1);
     std::vector<float> q32(std::max<size_t>(nqv, 1));
     for (size_t i = 0; i < nqv; i++) q32[i] = (float)q[i];
+    if (m->id == 1 && !k->is2d && m->quad_n == 0 && nq > 0) k->q_host.assign(q, q + nq);
     hipError_t e = hipStreamCreateWithFlags(&k->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&k->q32, sizeof(float) * std::max<size_t>(nqv, 1));
     if (e == hipSuccess) e = hipMalloc((void **)&k->q64, sizeof(double) * std::max<size_t>(nqv, 1));
@@ -717,6 +722,19 @@ int saship_kernel_call(saship_kernel *k, int32_t kind, const int32_t *details, i
     if (int e = validate_set(id, kind, details, n_details, n_values, pd_start, pd_stop, -1)) return e;
     std::lock_guard<std::mutex> lock(k->mu);
     HIP_TRY(hipSetDevice(k->dev));
+    double ext[4];
+    if (!k->q_host.empty() && kind == 0 && k->separable && pd_start == 0 &&
+        pd_stop == details[4 * model_maxpd(id)] && cylinder_separable(details) &&
+        adaptive_separable(details, values, cutoff, ext)) {
+        // the whole mesh of a cylinder under the adaptive rule: the one-set
+        // batch's separable path (the same raw sums, in another order)
+        if (!k->adapt) {
+            const int32_t nq1 = k->nq;
+            if (int e = saship_batch_create(k->m, 1, &nq1, k->q_host.data(), 0, &k->adapt)) return e;
+        }
+        const int64_t voff[2] = {0, n_values};
+        return saship_batch_call(k->adapt, 0, details, n_details, values, voff, cutoff, reff_mode, result, nullptr);
+    }
     const int nq = k->nq, nout = kind == 0 && model_has_fq(id) ? 2 : 1;
     const size_t vbytes = sizeof(double) * (size_t)n_values;
     const size_t dbytes = sizeof(int32_t) * (size_t)n_details;
@@ -959,6 +977,7 @@ int saship_kernel_release(saship_kernel *k)
     if (k->d_meta) (void)hipFree(k->d_meta);
     if (k->d_partial) (void)hipFree(k->d_partial);
     if (k->d_sep) (void)hipFree(k->d_sep);
+    if (k->adapt) (void)saship_batch_release(k->adapt);
     if (k->h_stage) (void)hipHostFree(k->h_stage);
     if (k->h_result) (void)hipHostFree(k->h_result);
     if (k->ev0) (void)hipEventDestroy(k->ev0);

@@ -224,3 +224,108 @@ def test_batched_few_items(separable):
             for s in range(nsets):
                 compare_raw(raw[s*n:(s + 1)*n], ref, q.size, 2)
         batch.release()
+
+
+def _adaptive_sets(n, seed):
+    """Random cylinder sets for the adaptive rule (q up to 1, q*max(R, L/2)
+    through every rule threshold; the last set reaches the 5000-point rule)."""
+    rng = np.random.default_rng(seed)
+    sets = []
+    for s in range(n):
+        q = np.logspace(-3, 0, int(rng.integers(60, 250)))
+        nr, nl = [(1, 35), (7, 1), (35, 20), (9, 9)][s % 4]
+        pd_type = ["lognormal", "gaussian"][s % 2]
+        pars = dict(radius=rng.uniform(5, 150), length=rng.uniform(10, 1500), sld=rng.uniform(1, 6),
+                    sld_solvent=rng.uniform(-1, 1),
+                    radius_pd=rng.uniform(0.02, 0.25) if nr > 1 else 0.0, radius_pd_n=nr, radius_pd_type=pd_type,
+                    length_pd=rng.uniform(0.02, 0.25) if nl > 1 else 0.0, length_pd_n=nl, length_pd_type=pd_type)
+        sets.append((q, pars))
+    sets.append((np.logspace(-2, 0, 64), dict(radius=40.0, length=2400.0, length_pd=0.1, length_pd_n=5)))
+    return sets
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batched_separable_adaptive(graph):
+    """The adaptive rule (quad 0) on the separable path: every mesh point
+    keeps the reference's rule (20/76/500/5000 by q*max(R, L/2), FP64), the
+    mesh split into rule rectangles.  Against the FP64 oracle, the generic
+    kernel, and the raw sums; replays equal the eager call."""
+    from sasmodels_amd.kernelhip import HipBatch
+    from gpu_helpers import hip_model, masked_relerr, RTOL
+    from oracle.oracle import OracleKernel
+    from sasmodels_amd import direct_model
+    from sasmodels_amd.details import make_kernel_args
+    model = hip_model("cylinder", 0)
+    sets = _adaptive_sets(12, 51)
+    kl = type("K", (), {"info": model.info, "dtype": np.dtype("d")})()
+    calls = [make_kernel_args(kl, direct_model.get_mesh(model.info, p, dim="1d"))[:2] for _, p in sets]
+    batch = HipBatch(model, [[q] for q, _ in sets], use_graph=graph)
+    batch.set_options(pts_per_item=16)
+    for rep in range(2 if graph else 1):
+        print("adaptive separable graph", graph, "rep", rep, flush=True)
+        raw, iq = batch.raw(calls, 0.0, False, 0)
+        off = 0
+        for (q, pars), (det, val), i0 in zip(sets, calls, np.cumsum([0] + [q.size for q, _ in sets])[:-1]):
+            nq = q.size
+            ora = OracleKernel(model.info, [q]).raw(det, val, 0.0, False, 0)
+            compare_raw(raw[off:off + 2*nq + 4], ora, nq, 2)
+            ref = OracleKernel(model.info, [q])(det, val, 0.0, False)
+            assert masked_relerr(iq[i0:i0 + nq], ref) <= RTOL, pars
+            off += 2*nq + 4
+    batch.set_separable(False)
+    generic = np.concatenate(batch(calls))
+    batch.set_separable(True)
+    sep = np.concatenate(batch(calls))
+    sel = generic > 0
+    assert np.max(np.abs(sep[sel]/generic[sel] - 1)) <= RTOL
+    # the two paths sum in different orders: equal to the last bit would mean
+    # the separable path was not taken
+    assert not np.array_equal(sep, generic)
+    batch.release()
+
+
+def test_batched_separable_adaptive_fallbacks():
+    """Meshes the rule rectangles cannot split (pruned corners at a cutoff)
+    take the generic kernel for the whole batch; results still match the oracle."""
+    from sasmodels_amd.kernelhip import HipBatch
+    from gpu_helpers import hip_model, masked_relerr, RTOL
+    from oracle.oracle import OracleKernel
+    from sasmodels_amd import direct_model
+    from sasmodels_amd.details import make_kernel_args
+    model = hip_model("cylinder", 0)
+    sets = _adaptive_sets(4, 53)
+    kl = type("K", (), {"info": model.info, "dtype": np.dtype("d")})()
+    calls = [make_kernel_args(kl, direct_model.get_mesh(model.info, p, dim="1d"))[:2] for _, p in sets]
+    batch = HipBatch(model, [[q] for q, _ in sets])
+    for cutoff in (1e-3, 1e-12):
+        print("adaptive fallback cutoff", cutoff, flush=True)
+        out = batch(calls, cutoff)
+        for (q, pars), (det, val), o in zip(sets, calls, out):
+            ref = OracleKernel(model.info, [q])(det, val, cutoff, False)
+            assert masked_relerr(o, ref) <= RTOL, (cutoff, pars)
+    batch.release()
+
+
+def test_single_call_separable_adaptive():
+    """Single calls of the cylinder under the adaptive rule take the
+    rule-rectangle separable path for a whole mesh (a one-set batch), the
+    generic kernel for a pd_start/pd_stop piece; both against the oracle."""
+    for q, pars in _adaptive_sets(5, 57):
+        print("single adaptive", pars["radius"], pars["length"], flush=True)
+        hk, ok, det, val, mag = buffers("cylinder", [q], pars, 0)
+        hk._call_kernel(det, val, 0.0, mag, 1)
+        sep = hk.result.copy()
+        compare_raw(sep, ok.raw(det, val, 0.0, mag, 1), q.size, 2)
+        hk.set_separable(False)
+        hk._call_kernel(det, val, 0.0, mag, 1)
+        gen = hk.result.copy()
+        hk.set_separable(True)
+        compare_raw(gen, ok.raw(det, val, 0.0, mag, 1), q.size, 2)
+        if int(det.num_eval) > 1:
+            assert not np.array_equal(sep[:2*q.size], gen[:2*q.size])
+            # a piece of the mesh, then the rest accumulated: the generic path
+            n = int(det.num_eval)
+            hk._call_kernel(det, val, 0.0, mag, 1, 0, n//2)
+            hk._call_kernel(det, val, 0.0, mag, 1, n//2, n)
+            compare_raw(hk.result, ok.raw(det, val, 0.0, mag, 1), q.size, 2)
+        hk.release()
