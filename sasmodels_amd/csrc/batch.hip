@@ -392,14 +392,23 @@ __global__ __launch_bounds__(256) void k_batch_sep_adapt(SepAdaptArgs a, int *__
             pb = sep::rule_count(G + sep::NRMAX, NL, qd, 0.5, T);
         }
         const bool none = !ok || (ra == pa && rb == pb);
-        const bool whole = ra == NR && rb == NL && pa == 0 && pb == 0;
+        // the smaller rules' rectangle is empty where either prefix is
+        const bool whole = ra == NR && rb == NL && (pa == 0 || pb == 0);
         const bool fast = sep::wave_all(whole || none);
+        // one grid whole on every lane: the other grid's band alone is walked
+        // (lanes with nothing to add get an empty band)
+        const bool lband = !fast && sep::wave_all(none || (ra == NR && (pa == 0 || pa == NR || pb == 0)));
+        const bool rband = !fast && !lband && sep::wave_all(none || (rb == NL && (pb == 0 || pb == NL || pa == 0)));
+        const int blo = none ? sep::NRMAX : lband ? (pa == NR ? pb : 0) : (pb == NL ? pa : 0);
+        const int bhi = none ? 0 : lband ? rb : ra;
         const int off = a.quad.off[k], n0 = rem * P, n1 = min(adaptive_n(k), n0 + P);
         KSum a1, a2;
         for (int node = n0 + wave; node < n1; node += 4) {
             float f1, f2;
             const float sn = a.quad.a[off + node], cs = a.quad.b[off + node];
             if (fast) sep::node_sums(a.t, s, q, sn, cs, f1, f2);
+            else if (lband) sep::band_sums<false>(a.t, s, q, sn, cs, blo, bhi, f1, f2);
+            else if (rband) sep::band_sums<true>(a.t, s, q, sn, cs, blo, bhi, f1, f2);
             else sep::prefix_sums(a.t, s, q, sn, cs, ra, pa, rb, pb, f1, f2);
             if (none) f1 = f2 = 0.0f;
             const float w = a.quad.w[off + node];

@@ -1457,13 +1457,10 @@ __device__ __forceinline__ void radius_sums_b(P4 ROWB, float x0, float step, int
 // and the grid ends and steps `ends` = (R first, R last, R step, L/2 first,
 // L/2 last, L/2 step).
 template <class PF4, class PD, class PF>
-__device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const SAS_CONST float4 *COL, PF4 RB,
-                                              PF4 CB, PD ser, PF ends, int NR, int NL, float q, float sn, float cs,
-                                              float &f1, float &f2)
+__device__ __forceinline__ void length_full(const SAS_CONST float4 *COL, PF4 CB, PD ser, PF ends, int NL, float qc,
+                                            float &T1, float &T2)
 {
-    float T1, T2, U1, U2;
-    const float qc = fabsf(q * cs), qs = fabsf(q * sn);
-    const double yl = (double)qc * ser[1], xl = (double)qs * ser[0];
+    const double yl = (double)qc * ser[1];
     if (wave_all(yl <= SER_X)) {
         double P, Q;
         ser_pair<SER_NT1, SER_NT2>(ser + SER_T1, ser + SER_T2, yl * yl, P, Q);
@@ -1474,6 +1471,13 @@ __device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const
     } else {
         length_sums(COL, NL, qc, T1, T2);
     }
+}
+
+template <class PF4, class PD, class PF>
+__device__ __forceinline__ void radius_full(const SAS_CONST float4 *ROW, PF4 RB, PD ser, PF ends, int NR, float qs,
+                                            float &U1, float &U2)
+{
+    const double xl = (double)qs * ser[0];
     if (wave_all(xl <= SER_X)) {
         double P, Q;
         ser_pair<SER_NU1, SER_NU2>(ser + SER_U1, ser + SER_U2, xl * xl, P, Q);
@@ -1484,6 +1488,16 @@ __device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const
     } else {
         radius_sums(ROW, NR, qs, U1, U2);
     }
+}
+
+template <class PF4, class PD, class PF>
+__device__ __forceinline__ void rank_one_sums(const SAS_CONST float4 *ROW, const SAS_CONST float4 *COL, PF4 RB,
+                                              PF4 CB, PD ser, PF ends, int NR, int NL, float q, float sn, float cs,
+                                              float &f1, float &f2)
+{
+    float T1, T2, U1, U2;
+    length_full(COL, CB, ser, ends, NL, fabsf(q * cs), T1, T2);
+    radius_full(ROW, RB, ser, ends, NR, fabsf(q * sn), U1, U2);
     f1 = U1 * T1;
     f2 = U2 * T2;
 }
@@ -1682,13 +1696,58 @@ __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, flo
     f2 = F2;
 }
 
+// Prefix sums along one grid for the adaptive rule: the point terms of
+// the radius (R) or length (L) factors in grid order, added while the point
+// index is below the lane's prefix n and captured at np (np <= n) before the
+// point np is added; the walk ends where every lane's prefix is complete.  Small arguments on every lane take the polynomial,
+// else sin and cos are carried along the grid by the step rotation and
+// recomputed every RESTART points (as the rotation-mode sums).
+template <bool RADIUS, class P4>
+__device__ __forceinline__ void grid_prefix(P4 G, int N, float qa, int n, int np, float &A1, float &A2, float &P1,
+                                            float &P2)
+{
+    A1 = A2 = P1 = P2 = 0.0f;
+    const float xa = qa * ldv(G).x, xb = qa * ldv(G + N - 1).x;
+    const float xmin = fminf(xa, xb), xmax = fmaxf(xa, xb);
+    const bool all_small = wave_all(xmax <= 2.0f), mixed = !wave_all(xmin > 2.0f);
+    float sd = 0.0f, cd = 1.0f, s = 0.0f, c = 1.0f;
+    if (!all_small) sincos_prod(qa, ldv(G).w, sd, cd);
+    int i = 0;
+    for (; i < N; i++) {
+        if (wave_all(i >= n)) break;   // every lane's prefix complete
+        const float4 e = ldv(G + i);
+        const float x = qa * e.x;
+        if (!all_small && (i & (RESTART - 1)) == 0) sincos_prod(qa, e.x, s, c);
+        float f;
+        if (RADIUS) f = all_small || (mixed && x <= 2.0f) ? j1c_small(x) : j1c_big(x, s, c);
+        else f = all_small || (mixed && x < 2.0f) ? sinc_taylor(x) : s * rcp_f(x);
+        if (i == np) {
+            P1 = A1;
+            P2 = A2;
+        }
+        if (i < n) {
+            A1 = fmaf(e.y, f, A1);
+            A2 = fmaf(e.z, f * f, A2);
+        }
+        if (!all_small) {
+            const float s2 = fmaf(s, cd, c * sd);
+            c = fmaf(c, cd, -s * sd);
+            s = s2;
+        }
+    }
+    if (np >= i) {
+        P1 = A1;
+        P2 = A2;
+    }
+}
+
 // Adaptive rule (rotation-mode tables, grids ascending): the mesh points
 // evaluated with rule k at q form the rectangle of the first a radius and
 // b length points (q R < T_k and q L/2 < T_k) less the rectangle of the
 // first ap, bp (the smaller rules), so at a node of rule k
 //   F = U1(a) T1(b) - U1(ap) T1(bp),  F^2 = U2(a) T2(b) - U2(ap) T2(bp)
-// with the prefix sums U(a) = sum_{i<a} u_i J_i and T(b) = sum_{j<b} v_j S_j,
-// here point by point (ap <= a, bp <= b per lane).
+// with the prefix sums U(a) = sum_{i<a} u_i J_i and T(b) = sum_{j<b} v_j S_j
+// (ap <= a, bp <= b per lane).
 __device__ __forceinline__ void prefix_sums(const Tables &t, int set, float q, float sn, float cs, int a, int ap,
                                             int b, int bp, float &f1, float &f2)
 {
@@ -1696,43 +1755,76 @@ __device__ __forceinline__ void prefix_sums(const Tables &t, int set, float q, f
     const int NR = dims[0], NL = dims[1];
     const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
     const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
-    const float qs = q * sn, qc = q * cs;
-    float U1 = 0.0f, U2 = 0.0f, U1p = 0.0f, U2p = 0.0f;
-    for (int i = 0; i < NR; i++) {
-        if (i == ap) {
-            U1p = U1;
-            U2p = U2;
-        }
-        const float4 e = ld4(ROW + i);
-        const float J = j1c_f(qs * e.x);
-        if (i < a) {
-            U1 = fmaf(e.y, J, U1);
-            U2 = fmaf(e.z, J * J, U2);
-        }
-    }
-    if (ap >= NR) {
-        U1p = U1;
-        U2p = U2;
-    }
-    float T1 = 0.0f, T2 = 0.0f, T1p = 0.0f, T2p = 0.0f;
-    for (int j = 0; j < NL; j++) {
-        if (j == bp) {
-            T1p = T1;
-            T2p = T2;
-        }
-        const float4 e = ld4(COL + j);
-        const float S = sinc_prod(qc, e.x);
-        if (j < b) {
-            T1 = fmaf(e.y, S, T1);
-            T2 = fmaf(e.z, S * S, T2);
-        }
-    }
-    if (bp >= NL) {
-        T1p = T1;
-        T2p = T2;
-    }
+    float U1, U2, U1p, U2p, T1, T2, T1p, T2p;
+    grid_prefix<true>(ROW, NR, fabsf(q * sn), a, ap, U1, U2, U1p, U2p);
+    grid_prefix<false>(COL, NL, fabsf(q * cs), b, bp, T1, T2, T1p, T2p);
     f1 = fmaf(U1, T1, -U1p * T1p);
     f2 = fmaf(U2, T2, -U2p * T2p);
+}
+
+// Band sums along one grid: the point terms of the lane's points lo <= i < hi
+// (a lane with lo >= hi adds nothing); the walk covers the wave's smallest
+// lo to its largest hi, with sin and cos carried as in grid_prefix.
+template <bool RADIUS, class P4>
+__device__ __forceinline__ void grid_band(P4 G, int N, float qa, int lo, int hi, float &A1, float &A2)
+{
+    A1 = A2 = 0.0f;
+    int i0 = lo;
+    for (int o = 32; o > 0; o >>= 1) i0 = min(i0, __shfl_xor(i0, o));
+    const float xa = qa * ldv(G).x, xb = qa * ldv(G + N - 1).x;
+    const float xmin = fminf(xa, xb), xmax = fmaxf(xa, xb);
+    const bool all_small = wave_all(xmax <= 2.0f), mixed = !wave_all(xmin > 2.0f);
+    float sd = 0.0f, cd = 1.0f, s = 0.0f, c = 1.0f;
+    if (!all_small) sincos_prod(qa, ldv(G).w, sd, cd);
+    for (int i = i0; i < N; i++) {
+        if (wave_all(i >= hi)) break;
+        const float4 e = ldv(G + i);
+        const float x = qa * e.x;
+        if (!all_small && ((i - i0) & (RESTART - 1)) == 0) sincos_prod(qa, e.x, s, c);
+        float f;
+        if (RADIUS) f = all_small || (mixed && x <= 2.0f) ? j1c_small(x) : j1c_big(x, s, c);
+        else f = all_small || (mixed && x < 2.0f) ? sinc_taylor(x) : s * rcp_f(x);
+        if (i >= lo && i < hi) {
+            A1 = fmaf(e.y, f, A1);
+            A2 = fmaf(e.z, f * f, A2);
+        }
+        if (!all_small) {
+            const float s2 = fmaf(s, cd, c * sd);
+            c = fmaf(c, cd, -s * sd);
+            s = s2;
+        }
+    }
+}
+
+// The adaptive rule where one grid enters whole on every lane: with a = NR
+// and ap in {0, NR} the node's sums are U(NR) times the length band
+// [ap ? bp : 0, b); with b = NL and bp in {0, NL} (RADIUS_BAND) U over the
+// radius band times T(NL).  The whole factor runs as in rank_one_sums
+// (series, scaled tables or the rotation sums).
+template <bool RADIUS_BAND>
+__device__ __forceinline__ void band_sums(const Tables &t, int set, float q, float sn, float cs, int lo, int hi,
+                                          float &f1, float &f2)
+{
+    const SAS_CONST int *dims = as_const(t.dims) + 4 * set;
+    const int NR = dims[0], NL = dims[1];
+    const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
+    const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
+    const SAS_CONST float4 *RB = as_const(t.ROWB) + (size_t)set * NRMAX;
+    const SAS_CONST float4 *CB = as_const(t.COLB) + (size_t)set * NRMAX;
+    const SAS_CONST double *ser = as_const(t.SER) + (size_t)set * SER_STRIDE;
+    const float ends[6] = {ldv(ROW).x, ldv(ROW + NR - 1).x, ldv(ROW).w, ldv(COL).x, ldv(COL + NL - 1).x,
+                           ldv(COL).w};
+    const float qc = fabsf(q * cs), qs = fabsf(q * sn);
+    float U1, U2, T1, T2;
+    if (RADIUS_BAND) {
+        length_full(COL, CB, ser, ends, NL, qc, T1, T2);
+        grid_band<true>(ROW, NR, qs, lo, hi, U1, U2);
+    } else {
+        radius_full(ROW, RB, ser, ends, NR, qs, U1, U2);
+        grid_band<false>(COL, NL, qc, lo, hi, T1, T2);
+    }
+    f1 = U1 * T1;
+    f2 = U2 * T2;
 }
 
 // Number of grid values g[0..n) (ascending) with q * s * g < T, in FP64 as

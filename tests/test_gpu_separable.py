@@ -241,6 +241,13 @@ def _adaptive_sets(n, seed):
                     length_pd=rng.uniform(0.02, 0.25) if nl > 1 else 0.0, length_pd_n=nl, length_pd_type=pd_type)
         sets.append((q, pars))
     sets.append((np.logspace(-2, 0, 64), dict(radius=40.0, length=2400.0, length_pd=0.1, length_pd_n=5)))
+    # R and L/2 alike with wide distributions: both grids cross a threshold
+    # at the same q (neither grid whole under the rule)
+    sets.append((np.logspace(-3, 0, 128), dict(radius=50.0, length=100.0, radius_pd=0.25, radius_pd_n=15,
+                                               length_pd=0.25, length_pd_n=15)))
+    # a disc (R well above L/2): the radius grid alone crosses the thresholds
+    sets.append((np.logspace(-3, 0, 100), dict(radius=300.0, length=20.0, radius_pd=0.2, radius_pd_n=20,
+                                               length_pd=0.2, length_pd_n=9)))
     return sets
 
 
