@@ -114,6 +114,15 @@ def wl_cylinder(n=64, offset=0, cutoff=0.0):
                          "(35x35 mesh), sld 4/1, Gauss-76, cutoff %g; %d curves per batched launch" % (cutoff, n))
 
 
+def wl_cylinder_adaptive(n=64, offset=0):
+    info = _info("cylinder", 0)
+    return dict(name="cylinder_1d_adaptive", info=info, kind=0, quad=0, q_list=[[C2_Q]]*n, cutoff=0.0,
+                calls=c2_calls(info, n, offset),
+                workload="cylinder 1D as the headline curve (1000 log q in [1e-3,1], radius 20 / length 400 "
+                         "lognormal pd 0.1, 35x35 mesh) with the reference's default adaptive orientation rule "
+                         "(20/76/500/5000 points by q*max(R, L/2)); %d curves per batched launch" % n)
+
+
 def wl_sphere(n=4096):
     info = _info("sphere")
     q = np.logspace(-3, np.log10(0.5), 200)
@@ -580,7 +589,7 @@ def bench_ellipsoid_2d(fma_tflops, threads, cpu=True, reps=5):
 
 CONFIG_LEGS = ("sphere_1d", "ellipsoid_2d", "core_shell_cylinder_batched", "magnetic_core_shell_sphere_batched",
                "parallelepiped_batched", "bcc_paracrystal_batched", "cylinder_batched_pinhole",
-               "cylinder_hardsphere_beta_batched")
+               "cylinder_hardsphere_beta_batched", "cylinder_adaptive_batched")
 
 
 def bench_configs(fma_tflops, threads, cpu=True, only=None):
@@ -608,6 +617,9 @@ def bench_configs(fma_tflops, threads, cpu=True, only=None):
     if want("bcc_paracrystal_batched"):
         out["bcc_paracrystal_batched"] = bench_workload(
             wl_bcc(), "k_batch_eval_1d<BccParacrystal> (FP64)", None, fma_tflops, threads, cpu, reps=3)
+    if want("cylinder_adaptive_batched"):
+        out["cylinder_adaptive_batched"] = bench_workload(
+            wl_cylinder_adaptive(), "k_batch_sep_adapt", None, fma_tflops, threads, cpu, reps=10, cpu_sets=2)
 
     if want("cylinder_batched_pinhole"):
         # headline curves measured through a pinhole (dq/q = 5 %): the batch
