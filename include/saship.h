@@ -183,7 +183,9 @@ int saship_kernel_call(saship_kernel *kernel, int32_t kind,
  * Every call on a device shares that device's scratch: calls whose work
  * overlaps in time on one device race, so serialise them (one stream, or
  * synchronise between calls).  Same evaluation path as saship_kernel_call
- * with the separable cylinder path allowed. */
+ * with the separable cylinder path allowed, except that a cylinder under the
+ * adaptive orientation rule runs on the generic mesh kernel (the separable
+ * form of that rule works in a batch's buffers). */
 int saship_kernel_call_noalloc(saship_model *model, int32_t kind, int32_t nq, const double *q,
                                const int32_t *details, int32_t n_details,
                                const double *values, int32_t n_values,
@@ -203,9 +205,11 @@ int saship_structure_eval(int32_t s_model, const double *pars, int32_t nq, const
                           int32_t on_device, double *prep, double *sq);
 
 /* Evaluation strategy: with separable != 0 (the default) a cylinder 1D call
- * with a fixed Gauss rule and a mesh over radius and length only is
- * evaluated as bilinear forms of the radius and length factors (same sums,
- * far fewer special-function evaluations); 0 forces the generic mesh kernel.
+ * with a mesh over radius and length only is evaluated as bilinear forms of
+ * the radius and length factors (same sums, far fewer special-function
+ * evaluations): with a fixed Gauss rule directly, with the adaptive rule
+ * (quad_n = 0) for a whole unpruned mesh on equally spaced grids through a
+ * one-set batch the kernel handle keeps; 0 forces the generic mesh kernel.
  * No reference counterpart. */
 int saship_kernel_set_path(saship_kernel *kernel, int32_t separable);
 

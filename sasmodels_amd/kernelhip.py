@@ -64,6 +64,9 @@ class PinnedPool:
     A buffer goes back to the pool when the last array or view over it is
     gone, so a caller that keeps every result only grows the pool; past
     *cap_bytes* of buffers in use, arrays are ordinary host memory again.
+    The first buffer of a size comes with a spare: a caller that holds the
+    previous result while the next call runs (a fit loop) needs two, and
+    pinned allocation costs far more than a call.
     """
 
     def __init__(self, cap_bytes=256 << 20):
@@ -72,17 +75,27 @@ class PinnedPool:
         self._held = 0            # bytes of pinned buffers, in use or free
         self._mu = threading.Lock()
 
+    def _alloc(self, nbytes):
+        """A new pinned buffer's address, or None (over the cap, or refused); under _mu."""
+        if self._held + nbytes > self.cap:
+            return None
+        ptr = ct.c_void_p()
+        if _lib.lib().saship_host_alloc(nbytes, ct.byref(ptr)) != 0 or not ptr.value:
+            return None
+        self._held += nbytes
+        return ptr.value
+
     def empty(self, n):
         """A float64 array of n values (pinned when the pool has room)."""
         nbytes = 8*max(int(n), 1)
         with self._mu:
-            stack = self._free.get(nbytes)
-            addr = stack.pop() if stack else None
-            if addr is None and self._held + nbytes <= self.cap:
-                ptr = ct.c_void_p()
-                if _lib.lib().saship_host_alloc(nbytes, ct.byref(ptr)) == 0 and ptr.value:
-                    addr = ptr.value
-                    self._held += nbytes
+            first = nbytes not in self._free
+            stack = self._free.setdefault(nbytes, [])
+            addr = stack.pop() if stack else self._alloc(nbytes)
+            if first and addr is not None:
+                spare = self._alloc(nbytes)
+                if spare is not None:
+                    stack.append(spare)
         if addr is None:
             return np.empty(n, "d")
         buf = (ct.c_double*(nbytes//8)).from_address(addr)
@@ -92,7 +105,7 @@ class PinnedPool:
 
     def _give_back(self, nbytes, addr):
         with self._mu:
-            self._free.setdefault(nbytes, []).append(addr)
+            self._free[nbytes].append(addr)
 
 
 #: the process's pool of pinned output arrays

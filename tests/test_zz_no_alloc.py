@@ -56,10 +56,11 @@ def _case(name, quad, q_vectors, pars):
     return model, k, det, val, mag
 
 
-def _compare(name, quad, q_vectors, pars, cutoff=0.0, reff=0):
+def _compare(name, quad, q_vectors, pars, cutoff=0.0, reff=0, separable=True):
     model, k, det, val, mag = _case(name, quad, q_vectors, pars)
     kind = 0 if k.dim == "1d" else (2 if mag else 1)
     q = q_vectors[0] if kind == 0 else np.column_stack([np.ravel(v) for v in q_vectors]).ravel()
+    k.set_separable(separable)
     k._call_kernel(det, val, cutoff, mag, reff)
     want = k.result.copy()
     got = _noalloc(model, kind, q, det, val, cutoff, reff)
@@ -82,9 +83,11 @@ def test_noalloc_matches_kernel_call_twice():
     for sep in (True,):
         _compare("cylinder", 76, [np.logspace(-3, 0, 1000)], big)
         _compare("cylinder", 76, [np.logspace(-3, 0, 200)], small)
-    # generic mesh path (adaptive rule: no separable evaluation)
-    _compare("cylinder", 0, [np.logspace(-3, 0, 700)], dict(big, radius_pd_n=15, length_pd_n=11))
-    _compare("cylinder", 0, [np.logspace(-2, 0, 90)], dict(big, radius_pd_n=5, length_pd_n=3))
+    # the adaptive rule: here the generic mesh kernel (the separable form of
+    # this rule runs through a batch's buffers), compared with the kernel
+    # call forced onto the same path
+    _compare("cylinder", 0, [np.logspace(-3, 0, 700)], dict(big, radius_pd_n=15, length_pd_n=11), separable=False)
+    _compare("cylinder", 0, [np.logspace(-2, 0, 90)], dict(big, radius_pd_n=5, length_pd_n=3), separable=False)
 
 
 def test_noalloc_2d_magnetic_structure():
