@@ -1837,22 +1837,32 @@ __device__ __forceinline__ int rule_count(const double *g, int n, double q, doub
 }
 
 // Single set: one wave per (64-q tile, Gauss node), four nodes per
-// workgroup; partial[(o*nq + iq)*nnode + node] for k_finalize.
+// workgroup, their four sums combined in wave order through LDS:
+// partial[(blk*2 + o)*nq + iq] (blk = the workgroup's node block, o = 0
+// for F^2, 1 for F) for k_finalize_sep.
 template <int VARIANT = 0>
 __global__ __launch_bounds__(256) void k_fq1d_sep(int nq, const float *__restrict__ q32, Tables t, Quad quad,
                                                    double *__restrict__ partial)
 {
+    __shared__ double red[2][4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int iq = blockIdx.x * 64 + lane;
     const int node = blockIdx.y * 4 + wave;
-    if (node >= quad.n) return;
-    const float q = iq < nq ? q32[iq] : 0.0f;
-    float f1, f2;
-    node_sums(t, 0, q, quad.a[node], quad.b[node], f1, f2);
-    if (iq < nq) {
+    double d1 = 0.0, d2 = 0.0;
+    if (node < quad.n) {
+        const float q = iq < nq ? q32[iq] : 0.0f;
+        float f1, f2;
+        node_sums(t, 0, q, quad.a[node], quad.b[node], f1, f2);
         const float w = quad.w[node];
-        partial[((size_t)0 * nq + iq) * quad.n + node] = (double)(w * f2);
-        partial[((size_t)1 * nq + iq) * quad.n + node] = (double)(w * f1);
+        d2 = (double)(w * f2);
+        d1 = (double)(w * f1);
+    }
+    red[0][wave][lane] = d2;
+    red[1][wave][lane] = d1;
+    __syncthreads();
+    if (wave < 2 && iq < nq) {
+        const double v = ((red[wave][0][lane] + red[wave][1][lane]) + red[wave][2][lane]) + red[wave][3][lane];
+        partial[((size_t)blockIdx.y * 2 + wave) * nq + iq] = v;
     }
 }
 

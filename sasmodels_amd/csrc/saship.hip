@@ -61,6 +61,29 @@ __global__ __launch_bounds__(256) void k_finalize(
     }
 }
 
+// The separable single call's sums: one thread per output value (result
+// index iq*2 + o, so that consecutive threads write consecutive doubles of
+// the mapped host result), adding the node blocks' partial[(blk*2 + o)*nq
+// + iq] in block order; the four weight sums after them.  Adds to the
+// result when accumulate is set.
+__global__ __launch_bounds__(256) void k_finalize_sep(int nq, int nblk, const double *__restrict__ partial,
+                                                      const double *__restrict__ sums, int accumulate,
+                                                      double *__restrict__ result)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nv = 2 * (int64_t)nq;
+    if (i < nv) {
+        const int64_t iq = i >> 1, o = i & 1;
+        const double *p = partial + o * nq + iq;
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++) s += p[(size_t)b * 2 * nq];
+        result[i] = accumulate ? result[i] + s : s;
+    } else if (i < nv + 4) {
+        const double v = sums[i - nv];
+        result[i] = accumulate ? result[i] + v : v;
+    }
+}
+
 // Structure-factor evaluation (FP64): nothing is polydisperse, so the mesh
 // is one point of weight 1 whatever pd_start/pd_stop are (kernel_iq.c with
 // MAX_PD 0), counted when 1 > cutoff.  One thread per q, q = |(qx, qy)| for
@@ -321,8 +344,8 @@ int launch_sep(const Target &t, int nlp, const int32_t *d_det, const double *d_v
     sep::k_fq1d_sep<0><<<grid, 256, 0, t.st>>>(nq, t.q32, tb, t.m->quad, t.partial);
     HIP_TRY(hipGetLastError());
     if (t.ev1) HIP_TRY(hipEventRecord(t.ev1, t.st));
-    const int fblocks = std::max(1, (2 * nq + 3) / 4);
-    k_finalize<<<fblocks, 256, 0, t.st>>>(nq, 2, nnode, t.partial, sums, accumulate, t.result);
+    const int fblocks = (2 * nq + 4 + 255) / 256;
+    k_finalize_sep<<<fblocks, 256, 0, t.st>>>(nq, (int)grid.y, t.partial, sums, accumulate, t.result);
     HIP_TRY(hipGetLastError());
     return 0;
 }
