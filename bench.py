@@ -210,6 +210,7 @@ def wl_bcc(n=64):
 WORKLOADS = {
     "cylinder_1d": wl_cylinder,
     "cylinder_1d_cutoff": lambda: wl_cylinder(cutoff=1e-5),
+    "cylinder_1d_adaptive": wl_cylinder_adaptive,
     "sphere_1d": wl_sphere,
     "ellipsoid_2d": wl_ellipsoid_2d,
     "core_shell_cylinder_batched": wl_core_shell_cylinder,
@@ -619,7 +620,8 @@ def bench_configs(fma_tflops, threads, cpu=True, only=None):
             wl_bcc(), "k_batch_eval_1d<BccParacrystal> (FP64)", None, fma_tflops, threads, cpu, reps=3)
     if want("cylinder_adaptive_batched"):
         out["cylinder_adaptive_batched"] = bench_workload(
-            wl_cylinder_adaptive(), "k_batch_sep_adapt", None, fma_tflops, threads, cpu, reps=10, cpu_sets=2)
+            wl_cylinder_adaptive(), "k_batch_sep_adapt", "k_batch_sep_adapt_cylinder", fma_tflops, threads, cpu,
+            reps=10, cpu_sets=2)
 
     if want("cylinder_batched_pinhole"):
         # headline curves measured through a pinhole (dq/q = 5 %): the batch
