@@ -1211,6 +1211,14 @@ SAS_HD bool wave_all(bool p)
 #endif
 }
 
+// Smallest value over the wave, in a scalar register (every lane of the
+// wave active).
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // One length point m: (s, c) = sin and cos of its argument, advanced by the
 // step rotation (sd, cd) for the next.  MIXED: some argument of the wave
 // is below 2 (Taylor series there); the two instantiations keep the series
@@ -1769,8 +1777,7 @@ template <bool RADIUS, class P4>
 __device__ __forceinline__ void grid_band(P4 G, int N, float qa, int lo, int hi, float &A1, float &A2)
 {
     A1 = A2 = 0.0f;
-    int i0 = lo;
-    for (int o = 32; o > 0; o >>= 1) i0 = min(i0, __shfl_xor(i0, o));
+    const int i0 = wave_min(lo);
     const float xa = qa * ldv(G).x, xb = qa * ldv(G + N - 1).x;
     const float xmin = fminf(xa, xb), xmax = fmaxf(xa, xb);
     const bool all_small = wave_all(xmax <= 2.0f), mixed = !wave_all(xmin > 2.0f);
@@ -1796,6 +1803,46 @@ __device__ __forceinline__ void grid_band(P4 G, int N, float qa, int lo, int hi,
     }
 }
 
+// Band sums on the scaled tables (every argument of the wave above 2, as
+// length_sums_b and radius_sums_b): sin and cos start directly at the
+// wave's first band point i0 (grid value g0) and are carried by the step
+// rotation, the run anchors by the rotation through RESTART steps.
+template <bool RADIUS, class P4>
+__device__ __forceinline__ void band_b(P4 GB, float g0, float step, int i0, int N, float qa, int lo, int hi,
+                                       float &A1, float &A2)
+{
+    float sd, cd, sa, ca, s0, c0;
+    sincos_prod(qa, step, sd, cd);
+    sincos_prod(qa, RESTART * step, sa, ca);
+    sincos_prod(qa, g0, s0, c0);
+    const float iq = rcp_f(qa);
+    float A = 0.0f, B = 0.0f;
+    for (int m0 = i0; m0 < N; m0 += RESTART) {
+        if (wave_all(m0 >= hi)) break;
+        float s = s0, c = c0;
+        const int n = N - m0 < RESTART ? N - m0 : RESTART;
+        for (int t = 0; t < n; t++) {
+            const int i = m0 + t;
+            const float4 e = ldv(GB + i);
+            const float f = RADIUS ? j1_mc(iq * e.x, s, c) : s;
+            if (i >= lo && i < hi) {
+                A = fmaf(e.y, f, A);
+                B = fmaf(e.z, f * f, B);
+            }
+            rotate(s, c, sd, cd);
+        }
+        rotate(s0, c0, sa, ca);
+    }
+    if (RADIUS) {
+        const float f = iq * sqrt_f(iq);   // qa^-3/2
+        A1 = A * (1.41421356f * f);
+        A2 = B * (2.0f * f * f);
+    } else {
+        A1 = A * iq;
+        A2 = B * (iq * iq);
+    }
+}
+
 // The adaptive rule where one grid enters whole on every lane: with a = NR
 // and ap in {0, NR} the node's sums are U(NR) times the length band
 // [ap ? bp : 0, b); with b = NL and bp in {0, NL} (RADIUS_BAND) U over the
@@ -1816,12 +1863,19 @@ __device__ __forceinline__ void band_sums(const Tables &t, int set, float q, flo
                            ldv(COL).w};
     const float qc = fabsf(q * cs), qs = fabsf(q * sn);
     float U1, U2, T1, T2;
+    const int i0 = wave_min(lo);
     if (RADIUS_BAND) {
         length_full(COL, CB, ser, ends, NL, qc, T1, T2);
-        grid_band<true>(ROW, NR, qs, lo, hi, U1, U2);
+        if (i0 < NR && ldv(RB).w != 0.0f && wave_all(qs * fminf(ends[0], ends[1]) > 2.0f))
+            band_b<true>(RB, ldv(ROW + i0).x, ends[2], i0, NR, qs, lo, hi, U1, U2);
+        else
+            grid_band<true>(ROW, NR, qs, lo, hi, U1, U2);
     } else {
         radius_full(ROW, RB, ser, ends, NR, qs, U1, U2);
-        grid_band<false>(COL, NL, qc, lo, hi, T1, T2);
+        if (i0 < NL && ldv(CB).w != 0.0f && wave_all(qc * fminf(ends[3], ends[4]) >= 2.0f))
+            band_b<false>(CB, ldv(COL + i0).x, ends[5], i0, NL, qc, lo, hi, T1, T2);
+        else
+            grid_band<false>(COL, NL, qc, lo, hi, T1, T2);
     }
     f1 = U1 * T1;
     f2 = U2 * T2;
