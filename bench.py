@@ -515,12 +515,14 @@ def _oracle_refs(wl, idx, threads):
     return time.perf_counter() - t0, refs
 
 
-def bench_workload(wl, kernel_name, stem, fma_tflops, threads, cpu=True, reps=10, cpu_sets=4):
+def bench_workload(wl, kernel_name, stem, fma_tflops, threads, cpu=True, reps=10, cpu_sets=4, single_sets=64):
     """
     One batched configuration: device-resident launches per second (replays
     of the staged call), host-inclusive call rate, the evaluation kernel's
-    mean time (HIP events), the roofline, the FP64 accuracy of `cpu_sets`
-    sets and the oracle's rate on them (the CPU leg, scaled by mesh work).
+    mean time (HIP events), the roofline, the rate of the first
+    `single_sets` sets called one kernel call each (and the batch's
+    resident rate over it), the FP64 accuracy of `cpu_sets` sets and the
+    oracle's rate on them (the CPU leg, scaled by mesh work).
     """
     from sasmodels_amd import core
     from sasmodels_amd.kernelhip import HipBatch
@@ -536,12 +538,26 @@ def bench_workload(wl, kernel_name, stem, fma_tflops, threads, cpu=True, reps=10
     b(calls, wl["cutoff"], magnetic=mag, out=buf)     # staged for the replays, outputs in HBM
     res = resident_s(b, reps)
     b.release()
+    # the same sets one call each (the loop a caller without batches runs):
+    # a kernel per set made beforehand, each call timed host-inclusive
+    idx1 = range(min(len(calls), single_sets))
+    kernels = [model.make_kernel(wl["q_list"][i]) for i in idx1]
+    for k, i in zip(kernels, idx1):
+        k(*calls[i], wl["cutoff"], mag)
+    t0 = time.perf_counter()
+    for k, i in zip(kernels, idx1):
+        k(*calls[i], wl["cutoff"], mag)
+    dt1 = (time.perf_counter() - t0)/len(kernels)
+    for k in kernels:
+        k.release()
     model.release()
     n = len(calls)
     work = [np.asarray(qv[0]).size*int(d.num_eval) for qv, (d, _) in zip(wl["q_list"], calls)]
     ent = {"workload": wl["workload"], "sets_per_s": n/res, "sets_per_s_host_inclusive": n/dt,
            "sets_per_s_host_inclusive_reused_output": n/dt_out,
            "ms_per_launch": res*1e3, "eval_kernel_ms": ms, "qpd_evals_per_s": sum(work)/(ms*1e-3),
+           "single_call_sets_per_s": 1.0/dt1, "single_call_sets": len(kernels),
+           "batch_over_single_calls": (n/res)*dt1,
            "roofline": roofline(wl["name"], kernel_name, ms, stem=stem, fma_tflops=fma_tflops)}
     if cpu:
         idx = list(range(min(cpu_sets, n)))
