@@ -315,8 +315,10 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
 // with the rules their q range can need.  Each lane counts, in FP64 as the
 // reference chooses its rule, the grid prefixes below the rule's threshold
 // and the previous one; a wave whose lanes all see the whole mesh under
-// this rule (or none of it) takes the rotation-mode sums, else the prefix
-// sums.  One partial per (item, output, lane): partial[item*128 + o*64 + lane].
+// this rule (or none of it) takes the rotation-mode sums, one where a grid
+// enters whole on every lane the band sums of the other grid, else the
+// prefix sums.  One partial per (item, output, lane):
+// partial[item*128 + o*64 + lane].
 struct SepAdaptArgs {
     const float *q32;
     const double *q64;
