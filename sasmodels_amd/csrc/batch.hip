@@ -247,7 +247,11 @@ struct SepArgs {
 // sums its nodes with compensation, and the four sums are combined in a
 // fixed order through LDS, so one partial per output and q leaves the
 // workgroup for the whole block (larger blocks: fewer partials in HBM, no
-// loss of balance inside the workgroup).
+// loss of balance inside the workgroup).  ROT: every set's tables are in
+// rotation mode (the host checked the meshes as for the adaptive rule),
+// so the kernel carries the rank-one sums alone and fewer scalars; a set
+// found otherwise gets NaN rather than a wrong sum.
+template <bool ROT>
 __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ queue, double *__restrict__ partial)
 {
     __shared__ double red[2][4][64];
@@ -291,7 +295,8 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
         KSum a1, a2;
         for (int node = n0 + wave; node < n1; node += 4) {
             float f1, f2;
-            sep::node_sums(a.t, s, q, a.quad.a[node], a.quad.b[node], f1, f2);
+            if constexpr (ROT) sep::node_sums_rot(a.t, s, q, a.quad.a[node], a.quad.b[node], f1, f2);
+            else sep::node_sums(a.t, s, q, a.quad.a[node], a.quad.b[node], f1, f2);
             const float w = a.quad.w[node];
             a1.add(w * f1);
             a2.add(w * f2);
@@ -301,7 +306,8 @@ __global__ __launch_bounds__(256) void k_batch_sep(SepArgs a, int *__restrict__ 
         __syncthreads();
         if (wave < 2 && iq < nq) {
             // wave 0: <F^2>, wave 1: <F>, the four waves' sums in wave order
-            const double v = ((red[wave][0][lane] + red[wave][1][lane]) + red[wave][2][lane]) + red[wave][3][lane];
+            double v = ((red[wave][0][lane] + red[wave][1][lane]) + red[wave][2][lane]) + red[wave][3][lane];
+            if (ROT && as_const(a.t.dims)[4 * s + 2] != 2) v = __builtin_nan("");
             partial[a.poff[s] + (size_t)(blk * 2 + wave) * nq + iq] = v;
         }
         __syncthreads();
@@ -613,6 +619,7 @@ struct Plan {
     double mean_pts = 0.0;            // generic path: mesh points per work item (upper bound)
     bool single_chunk = false;        // generic path: every set's mesh fits one work item
     bool sep_adapt = false;           // separable path under the adaptive rule (tiled partials)
+    bool sep_rot = false;             // fixed-rule separable path, every set's mesh in rotation mode
     int n_titem = 0;                  // adaptive: tile entries (set, tile, rules, first item)
     size_t off_tinfo = 0, off_toff = 0;
 };
@@ -810,6 +817,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         }
     }
     const int nnode = p.sep_adapt ? 0 : b->m->quad.n;
+    bool rot = p.sep_nlp && !p.sep_adapt;
     if (p.sep_adapt) p.nodes_per_item = P;
     else if (p.sep_nlp) p.nodes_per_item = std::min(P, nnode);
     const int nblk = p.sep_nlp ? (nnode + p.nodes_per_item - 1) / p.nodes_per_item : 0;
@@ -819,6 +827,12 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
         const int32_t *det = details + (size_t)s * n_details;
         if (nv < 0) return fail(SASHIP_EINVAL, set_prefix(s) + "values_offset must be non-decreasing");
         if (int e = validate_set(id, kind, det, n_details, nv, 0, -1, s)) return e;
+        // fixed rule: every mesh whole on ascending, equally spaced grids (as
+        // for the adaptive rule) puts every set's tables in rotation mode
+        if (rot) {
+            double ext[4];
+            rot = adaptive_separable(det, values + voff_in[s], cutoff, ext);
+        }
         const int64_t num_eval = det[n_details - 4];
         const int nq = (int)(b->qoff[s + 1] - b->qoff[s]);
         p.max_nq = std::max(p.max_nq, nq);
@@ -863,6 +877,7 @@ int plan_and_stage(saship_batch *b, int kind, const int32_t *details, int32_t n_
     }
     p.mean_pts = items > 0 ? pts_total / (double)items : 0.0;
     p.single_chunk = !p.sep_nlp && std::all_of(nchunk.begin(), nchunk.end(), [](int32_t c) { return c == 1; });
+    p.sep_rot = rot;
     if (items > 0x7fffffff) return fail(SASHIP_EINVAL, "too many work items in one batch");
     p.n_items = (int)items;
     p.nq_total = b->qoff[ns];
@@ -992,7 +1007,8 @@ int enqueue_sep(saship_batch *b, const Plan &p, const sas::ExpandArgs &ea, const
                      b->nsets, p.nblk, p.nodes_per_item, d_poff};
     const int grid = std::max(1, b->cus * b->waves_per_cu / 4);
     if (b->timing) HIP_TRY(hipEventRecord(b->ev0, b->st));
-    k_batch_sep<<<grid, 256, 0, b->st>>>(sa, b->d_queue, b->d_partial);
+    if (p.sep_rot) k_batch_sep<true><<<grid, 256, 0, b->st>>>(sa, b->d_queue, b->d_partial);
+    else k_batch_sep<false><<<grid, 256, 0, b->st>>>(sa, b->d_queue, b->d_partial);
     HIP_TRY(hipGetLastError());
     if (b->timing) HIP_TRY(hipEventRecord(b->ev1, b->st));
     return 0;
@@ -1197,7 +1213,8 @@ int run(saship_batch *b, int kind, const int32_t *details, int32_t n_details, co
     if (b->use_graph && !b->timing) {
         std::vector<int64_t> key = {kind, (int64_t)p.bytes, p.rec_total, p.partial_total, p.result_total,
                                     reff_mode, want_result, want_iq, b->pts_per_item, b->waves_per_cu,
-                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq, gparts, p.single_chunk, p.sep_adapt, p.n_titem};
+                                    p.sep_nlp, b->n_smear, b->s_model, b->defer_iq, gparts, p.single_chunk, p.sep_adapt, p.n_titem,
+                                    p.sep_rot};
         int64_t cbits;
         memcpy(&cbits, &cutoff, sizeof cbits);
         key.push_back(cbits);
@@ -1583,7 +1600,7 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
             b->pts_per_item = P;
             b->waves_per_cu = W;
             double t = 1e300;
-            for (int rep = 0; rep < 3 && rc == 0; rep++) {      // one warm-up, the faster of two timed runs
+            for (int rep = 0; rep < 5 && rc == 0; rep++) {      // one warm-up, the fastest of four timed runs
                 b->t_ms = 0.0;
                 b->t_all_ms = 0.0;
                 b->ncalls = 0;
@@ -1608,7 +1625,7 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
         if (rc) break;
         b->grab = G;
         double t = 1e300;
-        for (int rep = 0; rep < 3 && rc == 0; rep++) {
+        for (int rep = 0; rep < 5 && rc == 0; rep++) {
             b->t_ms = 0.0;
             b->t_all_ms = 0.0;
             b->ncalls = 0;

@@ -1628,6 +1628,21 @@ SAS_HD void interval_sums(P4 ROW, P4 COL, PG GRP, int NR, int NL, int G, float q
 // the node; the node weight is applied by the caller.  Every lane of the
 // wave reads the same table entries (scalar loads: the tables are written
 // by k_sep_expand, an earlier launch, and only read here).
+// node_sums for a set whose tables are in rotation mode (dims[2] == 2):
+// the rank-one sums alone, without the code of the other modes.
+__device__ __forceinline__ void node_sums_rot(const Tables &t, int set, float q, float sn, float cs, float &f1,
+                                              float &f2)
+{
+    const SAS_CONST int *dims = as_const(t.dims) + 4 * set;
+    const int NR = dims[0], NL = dims[1];
+    const SAS_CONST float4 *ROW = as_const(t.ROW) + (size_t)set * NRMAX;
+    const SAS_CONST float4 *COL = as_const(t.COL) + (size_t)set * NRMAX;
+    const float ends[6] = {ldv(ROW).x, ldv(ROW + NR - 1).x, ldv(ROW).w, ldv(COL).x, ldv(COL + NL - 1).x,
+                           ldv(COL).w};
+    rank_one_sums(ROW, COL, as_const(t.ROWB) + (size_t)set * NRMAX, as_const(t.COLB) + (size_t)set * NRMAX,
+                  as_const(t.SER) + (size_t)set * SER_STRIDE, ends, NR, NL, q, sn, cs, f1, f2);
+}
+
 __device__ __forceinline__ void node_sums(const Tables &t, int set, float q, float sn, float cs, float &f1,
                                           float &f2)
 {
