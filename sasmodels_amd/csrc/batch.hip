@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void k_batch_sep_adapt(SepAdaptArgs a, int *__
         for (int node = n0 + wave; node < n1; node += 4) {
             float f1, f2;
             const float sn = a.quad.a[off + node], cs = a.quad.b[off + node];
-            if (fast) sep::node_sums(a.t, s, q, sn, cs, f1, f2);
+            if (fast) sep::node_sums_rot(a.t, s, q, sn, cs, f1, f2);
             else if (lband) sep::band_sums<false>(a.t, s, q, sn, cs, blo, bhi, f1, f2);
             else if (rband) sep::band_sums<true>(a.t, s, q, sn, cs, blo, bhi, f1, f2);
             else sep::prefix_sums(a.t, s, q, sn, cs, ra, pa, rb, pb, f1, f2);
