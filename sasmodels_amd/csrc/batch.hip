@@ -151,6 +151,34 @@ __global__ __launch_bounds__(256) void k_batch_eval_1d(BatchArgs a, int *__restr
             shared = c1 - c0 > 1 && M::shares_lengths(rf, 4 * RV, c0, c1);
             if (shared) M::eval1d_shared(q, rf, 4 * RV, c0, c1, a.quad, a1, a2);
         }
+        if constexpr (!M::FP64 && RV == 1 && !M::SHARED_1D && !M::ORIENTED) {
+            // one-float4 records of unoriented models: four records loaded
+            // together per step, so their scalar loads and address
+            // arithmetic are shared; the same sums in the same order
+            int j = c0;
+            for (; j + 4 <= c1; j += 4) {
+                float4 r4[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) r4[t] = rec[j + t];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const float r[4] = {r4[t].x, r4[t].y, r4[t].z, r4[t].w};
+                    float f1, f2;
+                    M::template eval1d<FIXED>(q, r, a.quad, f1, f2);
+                    a1.add(f1);
+                    a2.add(f2);
+                }
+            }
+            for (; j < c1; j++) {
+                const float4 x = rec[j];
+                const float r[4] = {x.x, x.y, x.z, x.w};
+                float f1, f2;
+                M::template eval1d<FIXED>(q, r, a.quad, f1, f2);
+                a1.add(f1);
+                a2.add(f2);
+            }
+            shared = true;     // summed
+        }
         const double qd = M::FP64 && iq < nq ? a.q64[qd0 + iq] : 0.0;
         for (int j = c0; j < c1 && !shared; j++) {
             if constexpr (M::FP64 && RV > BIG_RECORD) {
