@@ -164,3 +164,38 @@ def test_weights_that_miss_the_threshold_route_to_the_plain_kernel():
     assert s.serves(1e-3)
     I2, bad2, err2 = s.evaluate(curves, lq, threshold=1e-3)
     assert np.isfinite(err2).all()
+
+
+def test_held_out_accuracy_matches_the_validation_record():
+    """Inference I(q) of the shipped weights on fresh curves from the box,
+    judged by the plain kernel: the per-curve maximum relative error (over
+    q where I > 1e-6 of the curve's maximum) keeps to the validation record
+    that decides which thresholds the weights serve (its median and its
+    fraction of curves within 1e-2, with room for a different draw)."""
+    import json
+    import os
+    import torch
+    from sasmodels_amd.surrogate import CylinderSurrogate, sample_box
+    data = os.path.join(os.path.dirname(os.path.abspath(omlp.__file__)), "..", "sasmodels_amd", "data")
+    s = CylinderSurrogate(seed=0)
+    s.load(os.path.join(data, "cylinder_surrogate.npz"))
+    s.repack()
+    with open(os.path.join(data, "cylinder_surrogate.json")) as f:
+        rec = json.load(f)
+    rng = np.random.default_rng(2027)
+    n, nq = 128, 256
+    curves = sample_box(n, rng)
+    lq = np.linspace(-3, 0, nq)
+    plain = np.array(s.plain.iq(curves, [10.0**lq]*n))
+    out = torch.empty(n*nq, dtype=torch.float32, device="cuda")
+    s.infer_into(torch.as_tensor(curves.astype(np.float32), device="cuda"),
+                 torch.as_tensor(lq.astype(np.float32), device="cuda"), out, out_iq=True)
+    I = out.view(n, nq).double().cpu().numpy()
+    mask = plain > 1e-6*plain.max(axis=1, keepdims=True)
+    err = np.where(mask, np.abs(I/plain - 1), 0.0).max(axis=1)
+    median, within = float(np.median(err)), float(np.mean(err <= 1e-2))
+    print("held-out: median per-curve max error %.4f (record %.4f), within 1e-2 %.2f (record %.2f)"
+          % (median, rec["val_median"], within, rec["val_frac_below_1e2"]))
+    assert np.isfinite(err).all()
+    assert median <= 2.5*rec["val_median"]
+    assert within >= 0.5*rec["val_frac_below_1e2"]
