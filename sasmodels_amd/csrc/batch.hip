@@ -1622,28 +1622,47 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
     double best_t = 1e300;
     const int pts[] = {4, 8, 16, 32, 64, 128};
     const int waves[] = {8, 16, 24, 32};
+    constexpr int NP = sizeof(pts) / sizeof(pts[0]), NW = sizeof(waves) / sizeof(waves[0]);
+    double tbest[NP][NW];
     int rc = 0;
-    for (int P : pts) {
-        for (int W : waves) {
-            b->pts_per_item = P;
-            b->waves_per_cu = W;
-            double t = 1e300;
-            for (int rep = 0; rep < 5 && rc == 0; rep++) {      // one warm-up, the fastest of four timed runs
-                b->t_ms = 0.0;
-                b->t_all_ms = 0.0;
-                b->ncalls = 0;
-                rc = run(b, kind, details, n_details, values, values_offset, cutoff, 0, nullptr, nullptr);
-                if (rc == 0 && rep > 0) t = std::min(t, b->t_all_ms);
-            }
-            if (rc) break;
-            if (t < best_t) {
-                best_t = t;
-                best_p = P;
-                best_w = W;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(SASHIP_EHIP, "autotune events");
+    // a shape's time: the call staged once, then the device time of three
+    // back-to-back replays of its work (what the resident rate measures)
+    auto measure = [&](double &t) -> int {
+        b->timing = false;
+        if (int e = run(b, kind, details, n_details, values, values_offset, cutoff, 0, nullptr, nullptr)) return e;
+        HIP_TRY(hipEventRecord(e0, b->st));
+        for (int i = 0; i < 3; i++)
+            if (int e = enqueue_model(b, b->last, cutoff, 0, false, false, false)) return e;
+        HIP_TRY(hipEventRecord(e1, b->st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        t = std::min(t, (double)ms / 3.0);
+        return 0;
+    };
+    // two passes over the shapes, each shape's fastest time over both: a
+    // device still raising its clocks would otherwise favour the shapes
+    // timed last
+    for (int pass = 0; pass < 2 && rc == 0; pass++) {
+        for (int ip = 0; ip < NP && rc == 0; ip++) {
+            for (int iw = 0; iw < NW && rc == 0; iw++) {
+                b->pts_per_item = pts[ip];
+                b->waves_per_cu = waves[iw];
+                double t = pass ? tbest[ip][iw] : 1e300;
+                rc = measure(t);
+                tbest[ip][iw] = t;
             }
         }
-        if (rc) break;
     }
+    for (int ip = 0; ip < NP && rc == 0; ip++)
+        for (int iw = 0; iw < NW; iw++)
+            if (tbest[ip][iw] < best_t) {
+                best_t = tbest[ip][iw];
+                best_p = pts[ip];
+                best_w = waves[iw];
+            }
     b->pts_per_item = best_p;
     b->waves_per_cu = best_w;
     // then the items per queue atomic at that shape (0: the plan's choice)
@@ -1653,13 +1672,7 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
         if (rc) break;
         b->grab = G;
         double t = 1e300;
-        for (int rep = 0; rep < 5 && rc == 0; rep++) {
-            b->t_ms = 0.0;
-            b->t_all_ms = 0.0;
-            b->ncalls = 0;
-            rc = run(b, kind, details, n_details, values, values_offset, cutoff, 0, nullptr, nullptr);
-            if (rc == 0 && rep > 0) t = std::min(t, b->t_all_ms);
-        }
+        for (int rep = 0; rep < 2 && rc == 0; rep++) rc = measure(t);
         if (rc == 0 && t < best_t) {
             best_t = t;
             best_g = G;
@@ -1673,6 +1686,8 @@ int saship_batch_autotune(saship_batch *b, int32_t kind, const int32_t *details,
     b->ncalls = n_keep;
     drop_graph(b);
     b->have_last = false;      // the last staged plan is the sweep's last point
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     if (rc) return rc;
     *best_pts_per_item = best_p;
     *best_waves_per_cu = best_w;
