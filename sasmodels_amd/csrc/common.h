@@ -184,7 +184,9 @@ inline bool adaptive_separable(const int32_t *details, const double *values, dou
         }
     }
     if (kr < 0 || kl < 0) return false;
-    if (cutoff > 0.0 && !(wprod > cutoff * (1.0 + 1e-9))) return false;
+    // every mesh weight above the cutoff (and above 0, which a cutoff of 0
+    // also asks of the weight products), with a margin for their rounding
+    if (!(wprod > std::max(cutoff, 0.0) * (1.0 + 1e-9)) || !(wprod >= 1e-290)) return false;
     for (int g = 0; g < 2; g++) {
         const int k = g ? kl : kr, n = len[k];
         const double *v = pv + off[k];
