@@ -249,7 +249,7 @@ def accuracy(out, refs):
 def host_threads():
     """(threads used by the CPU legs, threads the process may run on, nproc):
     every core of the affinity mask, capped by OMP_NUM_THREADS where the
-    host sets one (the GPU boxes give each GPU a 16-core share)."""
+    host sets one (a shared host may give each GPU a share of its cores)."""
     affinity = len(os.sched_getaffinity(0))
     omp = os.environ.get("OMP_NUM_THREADS", "")
     threads = min(affinity, int(omp)) if omp.isdigit() and int(omp) > 0 else affinity
