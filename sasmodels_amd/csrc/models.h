@@ -439,12 +439,45 @@ struct CoreShellCylinder {
     // Fixed rule, records n0..n1 of a chunk with the same half lengths r[1]
     // and r[3] (only the radius varies): nodes outer, the two length factors
     // once per node.  Adds sum_pts r[6] w f and sum_pts r[7] w f^2 per node.
+    // Nodes in blocks of NB: a record is read once per block instead of once
+    // per node; each node's point sums and their order are unchanged.
     template <class ACC>
     __device__ static void eval1d_shared(float q, const float *rec, int rv, int n0, int n1, const Quad &t, ACC &a1,
                                          ACC &a2)
     {
+        constexpr int NB = 8;
         const float qhc = q * rec[n0 * rv + 1], qhs = q * rec[n0 * rv + 3];
-        for (int i = 0; i < t.n; i++) {
+        int i0 = 0;
+        for (; i0 + NB <= t.n; i0 += NB) {
+            float s[NB], w[NB], S1[NB], S2[NB], t1[NB], t2[NB];
+#pragma unroll
+            for (int k = 0; k < NB; k++) {
+                const float c = t.b[i0 + k];
+                s[k] = t.a[i0 + k];
+                w[k] = t.w[i0 + k];
+                S1[k] = sinc_prod(qhc, c);
+                S2[k] = sinc_prod(qhs, c);
+                t1[k] = 0.0f;
+                t2[k] = 0.0f;
+            }
+            for (int j = n0; j < n1; j++) {
+                const float *r = rec + j * rv;
+                const float qr0 = q * r[0], qr2 = q * r[2];
+#pragma unroll
+                for (int k = 0; k < NB; k++) {
+                    const float f = r[4] * S1[k] * j1c_f(qr0 * s[k]) + r[5] * S2[k] * j1c_f(qr2 * s[k]);
+                    const float wf = w[k] * f;
+                    t1[k] = fmaf(r[6], wf, t1[k]);
+                    t2[k] = fmaf(r[7], wf * f, t2[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NB; k++) {
+                a1.add(t1[k]);
+                a2.add(t2[k]);
+            }
+        }
+        for (int i = i0; i < t.n; i++) {
             const float s = t.a[i], c = t.b[i], w = t.w[i];
             const float S1 = sinc_prod(qhc, c), S2 = sinc_prod(qhs, c);
             float t1 = 0.0f, t2 = 0.0f;
